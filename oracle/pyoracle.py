@@ -1,0 +1,118 @@
+"""ctypes access to the oracle libraries -- TEST INFRASTRUCTURE ONLY.
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg to
+check or time against the product; never imported by cress_amd.
+
+  Oracle(prec)  -- the C restatement (liboracle32.so / liboracle64.so)
+  Ref()         -- the reference's own FP32 routines (libcressref32.so),
+                   present only where the reference was available at build time
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REFDIR = os.path.join(HERE, "_ref")
+
+I = C.c_int
+P = C.c_void_p
+
+
+class Cfg(C.Structure):
+    _fields_ = [("ni", I), ("nj", I), ("nk", I),
+                ("wbc", I), ("ebc", I), ("sbc", I), ("nbc", I), ("bbc", I), ("tbc", I),
+                ("advopt", I), ("impopt", I), ("gwmopt", I), ("buyopt", I), ("divopt", I),
+                ("cphopt", I), ("haiopt", I), ("vspopt", I), ("coropt", I), ("sthopt", I),
+                ("iwest", I), ("ieast", I), ("jsouth", I), ("jnorth", I),
+                ("ksp0", I * 2), ("vspvar", C.c_char * 10),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("weicoe", C.c_double), ("filcoe", C.c_double), ("thresq", C.c_double)]
+
+
+def make_cfg(ni, nj, nk, dx=1000.0, dy=1000.0, dz=500.0, advopt=3, divopt=1, buyopt=1,
+             cphopt=2, vspopt=2, coropt=0, sthopt=1, bbc=2, tbc=2, ksp0=(2, 2),
+             vspvar=b"oooxoooooo", weicoe=0.6, filcoe=0.1, thresq=1e-12):
+    c = Cfg()
+    c.ni, c.nj, c.nk = ni, nj, nk
+    c.wbc = c.ebc = c.sbc = c.nbc = -1
+    c.bbc, c.tbc = bbc, tbc
+    c.advopt, c.impopt, c.gwmopt, c.buyopt, c.divopt = advopt, 1, 0, buyopt, divopt
+    c.cphopt, c.haiopt, c.vspopt, c.coropt, c.sthopt = cphopt, 0, vspopt, coropt, sthopt
+    c.iwest = c.ieast = c.jsouth = c.jnorth = 0
+    c.ksp0[0], c.ksp0[1] = ksp0
+    c.vspvar = vspvar
+    c.dx, c.dy, c.dz = dx, dy, dz
+    c.weicoe, c.filcoe, c.thresq = weicoe, filcoe, thresq
+    return c
+
+
+def ptr(a):
+    if a is None:
+        return None
+    if not a.flags["F_CONTIGUOUS"]:
+        raise ValueError("arrays passed to the oracle must be Fortran-ordered")
+    return a.ctypes.data_as(P)
+
+
+class _Lib:
+    def __init__(self, path):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = C.CDLL(path)
+
+    def __getattr__(self, name):
+        fn = getattr(self.lib, name)
+
+        def call(*args):
+            conv = []
+            for a in args:
+                if isinstance(a, np.ndarray):
+                    conv.append(ptr(a))
+                elif isinstance(a, C.Structure):
+                    conv.append(C.byref(a))
+                elif isinstance(a, float):
+                    conv.append(self._real(a))
+                elif isinstance(a, (bytes,)):
+                    conv.append(C.c_char_p(a))
+                else:
+                    conv.append(a)
+            return fn(*conv)
+        return call
+
+
+class Oracle(_Lib):
+    def __init__(self, prec=32):
+        super().__init__(os.path.join(REFDIR, "liboracle%d.so" % prec))
+        self.prec = prec
+        self.dtype = np.float32 if prec == 32 else np.float64
+        self._real = C.c_float if prec == 32 else C.c_double
+        assert self.lib.or_real_size() == (4 if prec == 32 else 8)
+
+
+class Ref(_Lib):
+    """The reference's own FP32 routines via oracle/ref_harness.f90."""
+
+    def __init__(self):
+        super().__init__(os.path.join(REFDIR, "libcressref32.so"))
+        self.dtype = np.float32
+        self._real = C.c_float
+
+    def configure(self, cfg):
+        f = C.c_float
+        self.lib.ref_configure(I(cfg.wbc), I(cfg.ebc), I(cfg.sbc), I(cfg.nbc), I(cfg.bbc),
+                               I(cfg.tbc), I(cfg.advopt), I(cfg.impopt), I(cfg.gwmopt),
+                               I(cfg.buyopt), I(cfg.divopt), I(cfg.cphopt), I(cfg.haiopt),
+                               I(cfg.vspopt), I(cfg.coropt), f(cfg.dx), f(cfg.dy), f(cfg.dz),
+                               f(cfg.weicoe), f(cfg.filcoe), f(cfg.thresq),
+                               C.c_char_p(bytes(cfg.vspvar).ljust(10, b"x")))
+
+
+def have_ref():
+    return os.path.exists(os.path.join(REFDIR, "libcressref32.so"))
+
+
+def field(cfg, dtype, n=1):
+    """Zeroed Fortran-ordered real(0:ni+1,0:nj+1,1:nk[,n]) block."""
+    shape = (cfg.ni + 2, cfg.nj + 2, cfg.nk) + ((n,) if n > 1 else ())
+    return np.zeros(shape, dtype=dtype, order="F")

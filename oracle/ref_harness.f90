@@ -1,0 +1,402 @@
+! Test-infrastructure harness: bind(C) entry points that call the reference
+! CReSS module procedures unchanged, so that the C restatement in
+! oracle/cress_oracle.c can be checked against the reference's own compiled
+! code on identical inputs.  Linked only into oracle/_ref/libcressref32.so.
+!
+! Every array argument is the reference's explicit-shape
+! real(0:ni+1,0:nj+1,1:nk) block (column-major, x fastest), passed by address.
+! Options are written into the reference's namelist tables (m_comname) with
+! the reference's own ids (m_comindx), exactly where getiname/getrname read
+! them (getiname.f90:84).
+
+      module cress_ref_harness
+
+      use iso_c_binding
+      use m_comindx
+      use m_comname
+      use m_commpi
+      use m_gaussel
+      use m_advs
+      use m_advuvw
+      use m_advp
+      use m_pgrad
+      use m_diverpih
+      use m_diverpiv
+      use m_advbspi
+      use m_pgradiv
+      use m_buoywsi
+      use m_bcycle
+      use m_buoywb
+      use m_coriuv
+      use m_rstuvwc
+      use m_vspuvw
+      use m_forcep
+      use m_forcept
+      use m_forceqv
+      use m_forceq
+      use m_timeflt
+      use m_totalqwi
+      use m_vbcw
+      use m_vbcp
+      use m_vbcs
+      use m_vbcu
+      use m_vbcv
+      use m_vbcwc
+      use m_bbcw
+
+      implicit none
+
+      contains
+
+! Single-rank periodic/closed domain descriptors (the values inimpi/currpe
+! give at numpe=1: currpe.f90:105, inimpi_multi.f90:179).
+      subroutine ref_configure(wbc,ebc,sbc,nbc,bbc,tbc,advopt,impopt,    &
+     &   gwmopt,buyopt,divopt,cphopt,haiopt,vspopt,coropt,               &
+     &   dx,dy,dz,weicoe,filcoe,thresq,vspvar10) bind(C,name='ref_configure')
+      integer(c_int), value :: wbc,ebc,sbc,nbc,bbc,tbc,advopt,impopt
+      integer(c_int), value :: gwmopt,buyopt,divopt,cphopt,haiopt
+      integer(c_int), value :: vspopt,coropt
+      real, value :: dx,dy,dz,weicoe,filcoe,thresq
+      character(kind=c_char), intent(in) :: vspvar10(10)
+      integer i
+      iname(:)=0
+      rname(:)=0.e0
+      do i=1,ncn
+        cname(i)=repeat('x',108)
+      end do
+      iname(idwbc)=wbc
+      iname(idebc)=ebc
+      iname(idsbc)=sbc
+      iname(idnbc)=nbc
+      iname(idbbc)=bbc
+      iname(idtbc)=tbc
+      iname(idadvopt)=advopt
+      iname(idimpopt)=impopt
+      iname(idgwmopt)=gwmopt
+      iname(idbuyopt)=buyopt
+      iname(iddivopt)=divopt
+      iname(idtrnopt)=0
+      iname(idmpopt)=4
+      iname(idmfcopt)=0
+      iname(idsthopt)=1
+      iname(idtubopt)=0
+      iname(idsmtopt)=0
+      iname(idcphopt)=cphopt
+      iname(idhaiopt)=haiopt
+      iname(idqcgopt)=0
+      iname(idvspopt)=vspopt
+      iname(idlspopt)=0
+      iname(idnggopt)=0
+      iname(idexbopt)=0
+      iname(idcoropt)=coropt
+      iname(idcrvopt)=0
+      iname(idsfcopt)=0
+      iname(idaslopt)=0
+      iname(idtrkopt)=0
+      iname(iddiaopt)=0
+      iname(idngropt)=0
+! 4th-order reduction flags: 0 on periodic sides (sethalo.f90:536-580).
+      iname(idiwest)=merge(0,1,wbc.eq.-1)
+      iname(idieast)=merge(0,1,ebc.eq.-1)
+      iname(idjsouth)=merge(0,1,sbc.eq.-1)
+      iname(idjnorth)=merge(0,1,nbc.eq.-1)
+      rname(iddx)=dx
+      rname(iddy)=dy
+      rname(iddz)=dz
+      rname(iddxiv)=1.e0/dx
+      rname(iddyiv)=1.e0/dy
+      rname(iddziv)=1.e0/dz
+      rname(idweicoe)=weicoe
+      rname(idfilcoe)=filcoe
+      rname(idthresq)=thresq
+      do i=1,10
+        cname(idvspvar)(i:i)=vspvar10(i)
+      end do
+      nisub=1
+      njsub=1
+      isub=0
+      jsub=0
+      nigrp=1
+      njgrp=1
+      igrp=0
+      jgrp=0
+      ebw=1
+      ebe=1
+      ebs=1
+      ebn=1
+      ebsw=1
+      ebse=1
+      ebnw=1
+      ebne=1
+      end subroutine ref_configure
+
+      subroutine ref_gaussel(istr,iend,jstr,jend,kstr,kend,ni,nj,nk,     &
+     &   rr,ss,tt,ff,pv) bind(C,name='ref_gaussel')
+      integer(c_int), value :: istr,iend,jstr,jend,kstr,kend,ni,nj,nk
+      real rr(*),ss(*),tt(*),ff(*),pv(*)
+      call s_gaussel(idimpopt,istr,iend,jstr,jend,kstr,kend,ni,nj,nk,    &
+     &               rr,ss,tt,ff,pv)
+      end subroutine ref_gaussel
+
+      subroutine ref_advs(ni,nj,nk,rstxu,rstxv,rstxwc,s,sfrc,vadv,       &
+     &   t1,t2,t3) bind(C,name='ref_advs')
+      integer(c_int), value :: ni,nj,nk
+      real rstxu(*),rstxv(*),rstxwc(*),s(*),sfrc(*),vadv(*)
+      real t1(*),t2(*),t3(*)
+      call s_advs(idadvopt,idiwest,idieast,idjsouth,idjnorth,            &
+     &            iddxiv,iddyiv,iddziv,ni,nj,nk,rstxu,rstxv,rstxwc,s,     &
+     &            sfrc,vadv,t1,t2,t3)
+      end subroutine ref_advs
+
+      subroutine ref_advuvw(ni,nj,nk,rstxu,rstxv,rstxwc,u,v,w,ufrc,vfrc, &
+     &   wfrc,hadv,vadv,t1,t2,t3) bind(C,name='ref_advuvw')
+      integer(c_int), value :: ni,nj,nk
+      real rstxu(*),rstxv(*),rstxwc(*),u(*),v(*),w(*)
+      real ufrc(*),vfrc(*),wfrc(*),hadv(*),vadv(*),t1(*),t2(*),t3(*)
+      call s_advuvw(idadvopt,idiwest,idieast,idjsouth,idjnorth,          &
+     &              iddxiv,iddyiv,iddziv,ni,nj,nk,rstxu,rstxv,rstxwc,     &
+     &              u,v,w,ufrc,vfrc,wfrc,hadv,vadv,t1,t2,t3)
+      end subroutine ref_advuvw
+
+      subroutine ref_rstuvwc(ni,nj,nk,mf8u,mf8v,rst8u,rst8v,rst8w,u,v,wc,&
+     &   rstxu,rstxv,rstxwc) bind(C,name='ref_rstuvwc')
+      integer(c_int), value :: ni,nj,nk
+      real mf8u(*),mf8v(*),rst8u(*),rst8v(*),rst8w(*),u(*),v(*),wc(*)
+      real rstxu(*),rstxv(*),rstxwc(*)
+      call s_rstuvwc(idmpopt,idmfcopt,idiwest,idieast,idjsouth,idjnorth, &
+     &               ni,nj,nk,mf8u,mf8v,rst8u,rst8v,rst8w,u,v,wc,        &
+     &               rstxu,rstxv,rstxwc)
+      end subroutine ref_rstuvwc
+
+      subroutine ref_buoywb(ni,nj,nk,ptbr,qvbr,rst,ptp,qv,qall,wfrc,qvd, &
+     &   wb8s) bind(C,name='ref_buoywb')
+      integer(c_int), value :: ni,nj,nk
+      real ptbr(*),qvbr(*),rst(*),ptp(*),qv(*),qall(*),wfrc(*),qvd(*)
+      real wb8s(*)
+      call s_buoywb(idgwmopt,idcphopt,'moist',ni,nj,nk,ptbr,qvbr,rst,    &
+     &              ptp,qv,qall,wfrc,qvd,wb8s)
+      end subroutine ref_buoywb
+
+      subroutine ref_coriuv(ni,nj,nk,fc,rst,u,v,ufrc,vfrc,t1)            &
+     &   bind(C,name='ref_coriuv')
+      integer(c_int), value :: ni,nj,nk
+      real fc(*),rst(*),u(*),v(*),ufrc(*),vfrc(*),t1(*)
+      call s_coriuv(ni,nj,nk,fc,rst,u,v,ufrc,vfrc,t1)
+      end subroutine ref_coriuv
+
+      subroutine ref_vspuvw(ksp0,ni,nj,nk,ubr,vbr,rst8u,rst8v,rst8w,     &
+     &   up,vp,wp,rbct,dmy,ufrc,vfrc,wfrc,rbct8s) bind(C,name='ref_vspuvw')
+      integer(c_int) :: ksp0(2)
+      integer(c_int), value :: ni,nj,nk
+      real ubr(*),vbr(*),rst8u(*),rst8v(*),rst8w(*),up(*),vp(*),wp(*)
+      real rbct(*),dmy(*),ufrc(*),vfrc(*),wfrc(*),rbct8s(*)
+      integer k2(2)
+      k2(1)=ksp0(1)
+      k2(2)=ksp0(2)
+      call s_vspuvw(idgpvvar,idvspvar,idvspopt,k2,0.e0,ni,nj,nk,         &
+     &              ubr,vbr,rst8u,rst8v,rst8w,up,vp,wp,rbct,             &
+     &              dmy,dmy,dmy,dmy,dmy,dmy,ufrc,vfrc,wfrc,rbct8s)
+      end subroutine ref_vspuvw
+
+      subroutine ref_forcep(ksp0,ni,nj,nk,jcb,jcb8u,jcb8v,jcb8w,u,v,wc,  &
+     &   pp,ppp,rbct,dmy,pfrc,t1,t2,t3,t4,t5,t6,t7,t8)                   &
+     &   bind(C,name='ref_forcep')
+      integer(c_int) :: ksp0(2)
+      integer(c_int), value :: ni,nj,nk
+      real jcb(*),jcb8u(*),jcb8v(*),jcb8w(*),u(*),v(*),wc(*),pp(*),ppp(*)
+      real rbct(*),dmy(*),pfrc(*),t1(*),t2(*),t3(*),t4(*),t5(*),t6(*)
+      real t7(*),t8(*)
+      integer k2(2)
+      k2(1)=ksp0(1)
+      k2(2)=ksp0(2)
+      call s_forcep(idnggvar,idlspvar,idvspvar,idlspopt,idvspopt,        &
+     &              idsmtopt,k2,0.e0,0.e0,ni,nj,nk,jcb,jcb8u,jcb8v,jcb8w, &
+     &              dmy,dmy,u,v,wc,pp,ppp,dmy,rbct,dmy,dmy,pfrc,         &
+     &              t1,t2,t3,t4,t5,t6,t7,t8)
+      end subroutine ref_forcep
+
+      subroutine ref_forcept(ksp0,ni,nj,nk,jcb,ptbr,rbr,rst,rstxu,rstxv, &
+     &   rstxwc,w,ptp,ptpp,rbct,dmy,ptfrc,t1,t2,t3,t4,t5,t6,t7)          &
+     &   bind(C,name='ref_forcept')
+      integer(c_int) :: ksp0(2)
+      integer(c_int), value :: ni,nj,nk
+      real jcb(*),ptbr(*),rbr(*),rst(*),rstxu(*),rstxv(*),rstxwc(*)
+      real w(*),ptp(*),ptpp(*),rbct(*),dmy(*),ptfrc(*)
+      real t1(*),t2(*),t3(*),t4(*),t5(*),t6(*),t7(*)
+      integer k2(2)
+      k2(1)=ksp0(1)
+      k2(2)=ksp0(2)
+      call s_forcept(idnggvar,idlspvar,idvspvar,idlspopt,idvspopt,       &
+     &               idgwmopt,idsmtopt,idadvopt,idtubopt,k2,0.e0,0.e0,   &
+     &               ni,nj,nk,dmy,dmy,jcb,dmy,dmy,dmy,dmy,dmy,dmy,       &
+     &               ptbr,rbr,rst,rstxu,rstxv,rstxwc,w,w,ptp,ptpp,       &
+     &               dmy,dmy,dmy,dmy,rbct,dmy,dmy,ptfrc,                 &
+     &               t1,t2,t3,t4,t5,t6,t7)
+      end subroutine ref_forcept
+
+      subroutine ref_forceqv(ksp0,ni,nj,nk,jcb,qvbr,rbr,rst,rstxu,rstxv, &
+     &   rstxwc,qv,qvp,rbct,dmy,qvfrc,t1,t2,t3,t4,t5,t6)                 &
+     &   bind(C,name='ref_forceqv')
+      integer(c_int) :: ksp0(2)
+      integer(c_int), value :: ni,nj,nk
+      real jcb(*),qvbr(*),rbr(*),rst(*),rstxu(*),rstxv(*),rstxwc(*)
+      real qv(*),qvp(*),rbct(*),dmy(*),qvfrc(*)
+      real t1(*),t2(*),t3(*),t4(*),t5(*),t6(*)
+      integer k2(2)
+      k2(1)=ksp0(1)
+      k2(2)=ksp0(2)
+      call s_forceqv(idnggvar,idlspvar,idvspvar,idlspopt,idvspopt,       &
+     &               idsmtopt,idtubopt,k2,0.e0,0.e0,ni,nj,nk,            &
+     &               dmy,dmy,jcb,dmy,dmy,dmy,dmy,dmy,dmy,qvbr,rbr,       &
+     &               rst,rstxu,rstxv,rstxwc,qv,qvp,dmy,dmy,dmy,          &
+     &               dmy,rbct,dmy,dmy,qvfrc,t1,t2,t3,t4,t5,t6)
+      end subroutine ref_forceqv
+
+! Hydrometeor forcing for cphopt=2: qwtr(:,:,:,1:2)=qc,qr; qice(1:3)=qi,qs,qg;
+! nice(1)=nci.  Arrays are stacked species-last like the reference.
+      subroutine ref_forceq(ksp0,ni,nj,nk,jcb,rbr,rst,rstxu,rstxv,rstxwc,&
+     &   qwtr,qwtrp,qice,qicep,nice,nicep,rbct,dmy,qwfrc,qifrc,nifrc,    &
+     &   t1,t2,t3,t4,t5,t6) bind(C,name='ref_forceq')
+      integer(c_int) :: ksp0(2)
+      integer(c_int), value :: ni,nj,nk
+      real jcb(*),rbr(*),rst(*),rstxu(*),rstxv(*),rstxwc(*)
+      real qwtr(*),qwtrp(*),qice(*),qicep(*),nice(*),nicep(*)
+      real rbct(*),dmy(*),qwfrc(*),qifrc(*),nifrc(*)
+      real t1(*),t2(*),t3(*),t4(*),t5(*),t6(*)
+      integer k2(2)
+      real ngrdmp(2),rtinc(2)
+      k2(1)=ksp0(1)
+      k2(2)=ksp0(2)
+      ngrdmp(:)=0.e0
+      rtinc(:)=0.e0
+      call s_forceq(idnggvar,idlspvar,idvspvar,idngrvar,                 &
+     &              idlspopt,idvspopt,idngropt,idsmtopt,idcphopt,        &
+     &              idhaiopt,idtubopt,k2,0.e0,ngrdmp,0.e0,rtinc,         &
+     &              ni,nj,nk,2,1,3,1,dmy,dmy,jcb,dmy,dmy,                &
+     &              dmy,dmy,dmy,dmy,rbr,rst,rstxu,rstxv,rstxwc,          &
+     &              qwtr,qwtrp,dmy,dmy,qice,qicep,nice,nicep,            &
+     &              dmy,dmy,dmy,dmy,rbct,dmy,dmy,dmy,dmy,                &
+     &              dmy,dmy,dmy,dmy,qwfrc,dmy,qifrc,nifrc,               &
+     &              t1,t2,t3,t4,t5,t6)
+      end subroutine ref_forceq
+
+      subroutine ref_pgrad(dts,ni,nj,nk,jcb,rst8u,rst8v,rst8w,u,v,wc,pp, &
+     &   upg,vpg,wpg,t1,t2,t3,dmy) bind(C,name='ref_pgrad')
+      real, value :: dts
+      integer(c_int), value :: ni,nj,nk
+      real jcb(*),rst8u(*),rst8v(*),rst8w(*),u(*),v(*),wc(*),pp(*)
+      real upg(*),vpg(*),wpg(*),t1(*),t2(*),t3(*),dmy(*)
+      call s_pgrad(idtrnopt,idmpopt,idmfcopt,iddivopt,iddx,iddy,iddz,    &
+     &             iddxiv,iddyiv,iddziv,dts,ni,nj,nk,dmy,dmy,jcb,        &
+     &             dmy,dmy,dmy,dmy,dmy,dmy,rst8u,rst8v,rst8w,u,v,wc,pp,  &
+     &             upg,vpg,wpg,t1,t2,t3)
+      end subroutine ref_pgrad
+
+      subroutine ref_diverpih(ni,nj,nk,jcb8u,jcb8v,rcsq,u,v,pdiv,t1,t2,  &
+     &   t3,dmy) bind(C,name='ref_diverpih')
+      integer(c_int), value :: ni,nj,nk
+      real jcb8u(*),jcb8v(*),rcsq(*),u(*),v(*),pdiv(*),t1(*),t2(*),t3(*)
+      real dmy(*)
+      call s_diverpih(idtrnopt,idmpopt,idmfcopt,iddziv,ni,nj,nk,         &
+     &                dmy,dmy,jcb8u,jcb8v,dmy,dmy,dmy,dmy,rcsq,u,v,pdiv, &
+     &                t1,t2,t3)
+      end subroutine ref_diverpih
+
+      subroutine ref_diverpiv(ni,nj,nk,rcsq,w,pdiv)                      &
+     &   bind(C,name='ref_diverpiv')
+      integer(c_int), value :: ni,nj,nk
+      real rcsq(*),w(*),pdiv(*)
+      call s_diverpiv(iddziv,ni,nj,nk,rcsq,w,pdiv)
+      end subroutine ref_diverpiv
+
+      subroutine ref_advbspi(fore,ni,nj,nk,rst,w,pfrc,phdiv,pvdiv,fp)    &
+     &   bind(C,name='ref_advbspi')
+      integer(c_int), value :: fore,ni,nj,nk
+      real rst(*),w(*),pfrc(*),phdiv(*),pvdiv(*),fp(*)
+      if(fore.ne.0) then
+        call s_advbspi(idweicoe,'fore',ni,nj,nk,rst,w,pfrc,phdiv,pvdiv,fp)
+      else
+        call s_advbspi(idweicoe,'back',ni,nj,nk,rst,w,pfrc,phdiv,pvdiv,fp)
+      end if
+      end subroutine ref_advbspi
+
+      subroutine ref_pgradiv(dts,ni,nj,nk,jcb,wfrc,fp,fw,fpdvj)          &
+     &   bind(C,name='ref_pgradiv')
+      real, value :: dts
+      integer(c_int), value :: ni,nj,nk
+      real jcb(*),wfrc(*),fp(*),fw(*),fpdvj(*)
+      call s_pgradiv(iddziv,idweicoe,dts,ni,nj,nk,jcb,wfrc,fp,fw,fpdvj)
+      end subroutine ref_pgradiv
+
+      subroutine ref_buoywsi(dts,ni,nj,nk,rbr,ptbr,rst,rcsq,pp,ptp,fp,fw,&
+     &   wb8s) bind(C,name='ref_buoywsi')
+      real, value :: dts
+      integer(c_int), value :: ni,nj,nk
+      real rbr(*),ptbr(*),rst(*),rcsq(*),pp(*),ptp(*),fp(*),fw(*),wb8s(*)
+      call s_buoywsi(idgwmopt,idweicoe,dts,ni,nj,nk,rbr,ptbr,rst,rcsq,   &
+     &               pp,ptp,fp,fw,wb8s)
+      end subroutine ref_buoywsi
+
+      subroutine ref_bcycle(iwsnd,iwrcv,iesnd,iercv,jssnd,jsrcv,jnsnd,   &
+     &   jnrcv,ni,nj,kmax,var) bind(C,name='ref_bcycle')
+      integer(c_int), value :: iwsnd,iwrcv,iesnd,iercv
+      integer(c_int), value :: jssnd,jsrcv,jnsnd,jnrcv,ni,nj,kmax
+      real var(*)
+      call s_bcycle(idwbc,idebc,idsbc,idnbc,iwsnd,iwrcv,iesnd,iercv,     &
+     &              jssnd,jsrcv,jnsnd,jnrcv,ni,nj,kmax,var)
+      end subroutine ref_bcycle
+
+! Vertical/bottom boundary fills (vbc*.f90, bbcw.f90); kind selects which.
+      subroutine ref_vbc(kind,ni,nj,nk,a,b,c,d,e,f,g,h)                  &
+     &   bind(C,name='ref_vbc')
+      integer(c_int), value :: kind,ni,nj,nk
+      real a(*),b(*),c(*),d(*),e(*),f(*),g(*),h(*)
+      select case(kind)
+      case(0)
+        call s_vbcs(ni,nj,nk,a)
+      case(1)
+        call s_vbcu(ni,nj,nk,a)
+      case(2)
+        call s_vbcv(ni,nj,nk,a)
+      case(3)
+        call s_vbcp(idbbc,ni,nj,nk,a)
+      case(4)
+        call s_vbcwc(idbbc,idtbc,ni,nj,nk,a)
+      case(5)
+! vbcw(j31,j32,jcb8w,mf,uf,vf,wc,wf,scratch...)
+        call s_vbcw(idbbc,idtbc,idmpopt,idmfcopt,ni,nj,nk,b,b,c,b,d,e,   &
+     &              f,a,g,g,h)
+      case(6)
+! bbcw(j31,j32,mf,aa,uf,vf,wf,scratch)
+        call s_bbcw(idmpopt,idmfcopt,ni,nj,nk,b,b,b,c,d,e,a,g,h)
+      end select
+      end subroutine ref_vbc
+
+      subroutine ref_timeflt(ni,nj,nk,up,uf,vp,vf,wp,wf,ppp,ppf,ptpp,    &
+     &   ptpf,qvp,qvf,qwtrp,qwtrf,qicep,qicef,nicep,nicef,u,v,w,pp,ptp,  &
+     &   qv,qwtr,qice,nice,dmy) bind(C,name='ref_timeflt')
+      integer(c_int), value :: ni,nj,nk
+      real up(*),uf(*),vp(*),vf(*),wp(*),wf(*),ppp(*),ppf(*),ptpp(*)
+      real ptpf(*),qvp(*),qvf(*),qwtrp(*),qwtrf(*),qicep(*),qicef(*)
+      real nicep(*),nicef(*),u(*),v(*),w(*),pp(*),ptp(*),qv(*),qwtr(*)
+      real qice(*),nice(*),dmy(*)
+      integer nqa(0:4),land(1)
+      nqa(:)=1
+      land(1)=0
+      call s_timeflt(idsfcopt,idcphopt,idhaiopt,idqcgopt,idaslopt,       &
+     &               idtrkopt,idtubopt,idfilcoe,'moist',0.e0,ni,nj,nk,   &
+     &               2,1,3,1,nqa,1,land,up,uf,vp,vf,wp,wf,ppp,ppf,       &
+     &               ptpp,ptpf,qvp,qvf,qwtrp,qwtrf,dmy,dmy,qicep,qicef,  &
+     &               nicep,nicef,dmy,dmy,dmy,dmy,dmy,dmy,dmy,dmy,dmy,    &
+     &               dmy,dmy,dmy,u,v,w,pp,ptp,qv,qwtr,dmy,qice,nice,     &
+     &               dmy,dmy,dmy,dmy,dmy,dmy)
+      end subroutine ref_timeflt
+
+      subroutine ref_totalqwi(ni,nj,nk,qwtr,qice,qall)                   &
+     &   bind(C,name='ref_totalqwi')
+      integer(c_int), value :: ni,nj,nk
+      real qwtr(*),qice(*),qall(*)
+      call s_totalqwi(idcphopt,idhaiopt,ni,nj,nk,2,3,qwtr,qice,qall)
+      end subroutine ref_totalqwi
+
+      end module cress_ref_harness
