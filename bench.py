@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Benchmark: CReSS large steps on one MI355X, in cell-steps/s.
+
+A step is one large time step (s_intgdrv) of the whole model on the default
+workload, the 1024x1024x80 warm-core vortex on an f-plane (dtbig=4 s, 8
+acoustic substeps, cold-rain microphysics), with every field resident in HBM
+before the timed region starts.  cells = nx*ny*nz interior cells.
+
+Prints one JSON line (see the repository's benchmark contract) with
+  roofline     -- the dominant kernel: algorithmic bytes per launch over its
+                  average launch time (HIP events on the solver stream over
+                  the timed region), against 8 TB/s;
+  cpu_baseline -- the FP32 oracle (oracle/cress_oracle.c, single thread) on a
+                  128x128x80 sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM = 8.0e12
+
+# Algorithmic bytes per interior cell for one launch of each kernel family:
+# distinct 3-D arrays the launch must read or write once (4 B each), per the
+# counting rule of the step's bandwidth budget (metric/base arrays count as
+# 3-D, 2-D arrays and top-sponge-only reads are not counted).
+KERNEL_ARRAYS = {
+    # prep: u,v,w,rst8u,rst8v,rst8w,jcb8w -> wc,rstxu,rstxv,rstxwc
+    "k_prep": 11,
+    # u,v,up,vp,ubr,vbr,rst8u,rst8v,rst,rstxu,rstxv,rstxwc -> ufrc,vfrc
+    "k_force_uv": 14,
+    # w,wp,ptp,qv..qg(6),qvbr,ptbr,rst,rst8w,rstxu,rstxv,rstxwc -> wfrc
+    "k_force_w": 17,
+    # jcb,jcb8u,jcb8v,jcb8w,u,v,wc,pp,ppp -> pfrc
+    "k_force_p": 10,
+    # per scalar s,sp -> frc (x8) + rstxu,rstxv,rstxwc,rst,qvbr,ptbr,rbr,w
+    "k_force_s": 32,
+    # jcb,rst8u,rst8v,rst8w,uf,vf,wc,ppf -> ptil
+    "k_ac_ptil": 9,
+    # jcb,ptil,ufrc,vfrc,rst8u,rst8v,uf,vf -> uf,vf
+    "k_ac_uv": 10,
+    # uf,vf,ptil,pfrc,wfrc,wf,ppf,jcb,jcb8u,jcb8v,jcb8w,rcsq,rst,rbr,rst8w -> wf,wc,ppf
+    "k_ac_col": 18,
+    # ptpp,ptfrc,qp(6),qvfrc,ncip,rst,qf(5 forcings) -> ptpf,qf(6),ncif
+    "k_steps": 25,
+    # 12 fields x (x, xp, xf) -> 12
+    "k_timeflt": 48,
+}
+
+
+def step_bytes_per_cell(nsstp, cphopt):
+    """Whole-step algorithmic bytes per interior cell: B = 560 + 88*nsstp for
+    the full cold-rain path (forcing 136, acoustic 88/substep, steps 100,
+    microphysics 124, Asselin+satadjst 200)."""
+    if cphopt == 2:
+        return 560 + 88 * nsstp
+    # dry-dynamics variant (qv only): forcing 88, steps 28, Asselin 96
+    return 212 + 88 * nsstp
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="vortex", choices=["vortex", "supercell", "bubble"])
+    ap.add_argument("--nx", type=int, default=None)
+    ap.add_argument("--ny", type=int, default=None)
+    ap.add_argument("--nz", type=int, default=None)
+    ap.add_argument("--cphopt", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-nxy", type=int, default=128)
+    return ap.parse_args()
+
+
+def cpu_baseline(args, nz):
+    from cress_amd import cases
+    from oracle import pyoracle as po
+    nxy = args.cpu_nxy
+    case = cases.build(args.workload, nxy, nxy, nz, cphopt=args.cphopt)
+    m = po.Model(case, 32)
+    m.step(1, first=True)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        m.step(1)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > 10.0 or n >= 50:
+            break
+    return {"value": nxy * nxy * nz * n / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": "FP32 oracle, %s %dx%dx%d, %d large steps after 1 warm-up" % (
+                args.workload, nxy, nxy, nz, n)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    from cress_amd import cases, host
+    t0 = time.time()
+    case = cases.build(args.workload, args.nx, args.ny, args.nz, cphopt=args.cphopt)
+    cfg = case.cfg
+    nx, ny, nz = cfg.ni - 3, cfg.nj - 3, cfg.nk - 3
+    print("bench: %s %dx%dx%d nsstp=%d built in %.1f s" % (args.workload, nx, ny, nz, cfg.nsstp,
+                                                          time.time() - t0), file=sys.stderr, flush=True)
+    s = host.Solver(cfg)
+    case.load(s)
+    del case
+    s.sync()
+    s.step(1, first=True)
+    if args.warmup > 1:
+        s.step(args.warmup - 1)
+    s.sync()
+    print("bench: warm-up done", file=sys.stderr, flush=True)
+    L = host.lib()
+    L.cress_reset_timing(s.ctx)
+    s.set_timing(True)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    s.sync()
+    el = time.perf_counter() - t0
+    s.set_timing(False)
+    cells = nx * ny * nz
+    value = cells * args.steps / el
+    ms_per_step = 1e3 * el / args.steps
+    # per-kernel times over the timed region
+    import ctypes as C
+    nk = L.cress_kernel_count()
+    L.cress_kernel_name.restype = C.c_char_p
+    tot = (C.c_double * nk)()
+    cnt = (C.c_long * nk)()
+    L.cress_kernel_times(s.ctx, tot, cnt, nk)
+    kern = {L.cress_kernel_name(i).decode(): (tot[i], cnt[i]) for i in range(nk) if cnt[i]}
+    stages = s.stage_times()
+    dom = max((k for k in kern if k in KERNEL_ARRAYS), key=lambda k: kern[k][0])
+    avg_ms = kern[dom][0] / kern[dom][1]
+    per_launch_bytes = KERNEL_ARRAYS[dom] * 4.0 * cells
+    achieved = per_launch_bytes / (avg_ms * 1e-3)
+    step_b = step_bytes_per_cell(cfg.nsstp, cfg.cphopt)
+    print("bench: %.2f ms/step, whole-step %.2f TB/s algorithmic; kernels: %s" % (
+        ms_per_step, step_b * cells / (ms_per_step * 1e-3) / 1e12,
+        ", ".join("%s %.3f ms x%d" % (k, v[0] / max(v[1], 1), v[1]) for k, v in
+                  sorted(kern.items(), key=lambda kv: -kv[1][0]))), file=sys.stderr, flush=True)
+    s.close()
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args, nz)
+    out = {
+        "metric": "cell-steps/s (1 GPU), % HBM roofline; ensemble & MLP-microphysics vs plain",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "%s %dx%dx%d" % (args.workload, nx, ny, nz), "nsstp": cfg.nsstp,
+                   "dtbig": cfg.dtbig, "cphopt": cfg.cphopt, "advopt": cfg.advopt,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None,
+                     "step_frac": step_b * cells / (ms_per_step * 1e-3) / PEAK_HBM,
+                     "step_bytes_per_cell": step_b},
+        "stages_ms": {k: v / args.steps for k, v in stages.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

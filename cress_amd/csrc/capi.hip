@@ -1,0 +1,750 @@
+// C ABI (include/cress_hip.h) and the host-side orchestration of one CReSS
+// large step on one MI355X: s_intgdrv (intgdrv.f90:832-1133) with
+// fdmdrv/forcedrv/smalldrv/hevi/steps/cloudphy/timeflt/satadjst/shift2nd
+// and the three-level rotation of swp2nxt done by renaming buffers.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "common.h"
+#include "dyn.h"
+#include "micro.h"
+
+using namespace cress;
+
+#define HIPCHK(x)                                  \
+  do {                                             \
+    hipError_t e_ = (x);                           \
+    if (e_ != hipSuccess) {                        \
+      fprintf(stderr, "cress: %s failed: %s\n", #x, hipGetErrorString(e_)); \
+      return CRESS_EHIP;                           \
+    }                                              \
+  } while (0)
+
+enum { NSTAGE = 8 };
+
+// Kernel families timed individually when timing is on (HIP events on the
+// context stream around every launch).
+enum {
+  KT_PREP, KT_FORCE_UV, KT_FORCE_W, KT_FORCE_P, KT_FORCE_S, KT_PHY2CNT, KT_AC_PTIL, KT_AC_UV,
+  KT_AC_COL, KT_AC_RING, KT_FILL, KT_VBC, KT_STEPS, KT_TIMEFLT, KT_COPY, KT_MICRO, KT_SATADJ,
+  KT_N
+};
+static const char *kKernelNames[KT_N] = {
+    "k_prep", "k_force_uv", "k_force_w", "k_force_p", "k_force_s", "k_phy2cnt", "k_ac_ptil",
+    "k_ac_uv", "k_ac_col", "k_ac_ring", "k_bcycle", "k_vbc", "k_steps", "k_timeflt", "memcpy_d2d",
+    "micro", "satadjst"};
+enum { MAXEV = 1024 };
+
+struct cress_ctx {
+  cress_config cfg;
+  Grid g;
+  Consts c;
+  Opts o;
+  long n;  // floats per 3-D field
+  hipStream_t stream;
+  // base state / metrics
+  float *base[CRESS_NBASE];
+  float *fc, *rbct;
+  // prognostics by physical slot; logical level l lives in slot (rot+l)%3
+  float *u[3], *v[3], *w[3], *pp[3], *ptp[3], *q[3], *nci[3];
+  int rot;
+  float *prwtr, *price;  // precipitation (rate, accumulation)
+  // scratch
+  float *ufrc, *vfrc, *wfrc, *pfrc, *ptfrc, *qvfrc, *wc, *ptil, *rstxu, *rstxv, *rstxwc;
+  MicroScratch ms;
+  std::vector<void *> allocs;
+  int timing;
+  hipEvent_t ev[NSTAGE + 1];
+  double stage_ms[NSTAGE];
+  // per-kernel timing: event pairs of this step, accumulated after it
+  hipEvent_t kev[2 * MAXEV];
+  int kid[MAXEV];
+  int nkev;
+  double kern_ms[KT_N];
+  long kern_n[KT_N];
+};
+
+const char *cress_error_string(int code) {
+  switch (code) {
+    case CRESS_OK: return "ok";
+    case CRESS_EINVAL: return "invalid argument";
+    case CRESS_ENODEV: return "no usable GPU";
+    case CRESS_EHIP: return "HIP runtime error";
+    case CRESS_ENOMEM: return "device out of memory";
+    case CRESS_ERANGE: return "field id or level out of range";
+    case CRESS_EUNSUPPORTED: return "option outside the implemented plain path";
+  }
+  return "unknown error";
+}
+
+int cress_device_count(int *n) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *n = c;
+  return c > 0 ? CRESS_OK : CRESS_ENODEV;
+}
+
+static int dalloc(cress_ctx *x, float **p, size_t nfloat) {
+  void *q = nullptr;
+  if (hipMalloc(&q, nfloat * sizeof(float)) != hipSuccess) return CRESS_ENOMEM;
+  if (hipMemset(q, 0, nfloat * sizeof(float)) != hipSuccess) return CRESS_EHIP;
+  x->allocs.push_back(q);
+  *p = (float *)q;
+  return CRESS_OK;
+}
+
+static int check_config(const cress_config *c) {
+  if (!c || c->ni < 7 || c->nj < 7 || c->nk < 7) return CRESS_EINVAL;
+  if (c->nsstp < 1 || !(c->dx > 0) || !(c->dy > 0) || !(c->dz > 0)) return CRESS_EINVAL;
+  if (!(c->wbc == -1 && c->ebc == -1 && c->sbc == -1 && c->nbc == -1)) return CRESS_EUNSUPPORTED;
+  if ((c->bbc != 2 && c->bbc != 3) || (c->tbc != 2 && c->tbc != 3)) return CRESS_EUNSUPPORTED;
+  if (c->advopt < 1 || c->advopt > 3 || c->impopt != 1 || c->gwmopt != 0) return CRESS_EUNSUPPORTED;
+  if (c->divopt < 0 || c->divopt > 1 || c->buyopt < 0 || c->buyopt > 1) return CRESS_EUNSUPPORTED;
+  if ((c->cphopt != 0 && c->cphopt != 2) || c->haiopt != 0) return CRESS_EUNSUPPORTED;
+  if (c->vspopt != 0 && c->vspopt != 2) return CRESS_EUNSUPPORTED;
+  return CRESS_OK;
+}
+
+// Derived binary32 constants, folded in the same order as the reference.
+static void fold_consts(const cress_config *cf, Consts *c) {
+  const float dx = (float)cf->dx, dy = (float)cf->dy, dz = (float)cf->dz;
+  const float oned24 = 1.0f / 24.0f;
+  c->dxiv = 1.0f / dx;
+  c->dyiv = 1.0f / dy;
+  c->dziv = 1.0f / dz;
+  c->dxv05n = -0.5f * c->dxiv;
+  c->dyv05n = -0.5f * c->dyiv;
+  c->dzv05n = -0.5f * c->dziv;
+  c->dxv25n = -0.25f * c->dxiv;
+  c->dyv25n = -0.25f * c->dyiv;
+  c->dzv25n = -0.25f * c->dziv;
+  c->dxv24 = oned24 * c->dxiv;
+  c->dyv24 = oned24 * c->dyiv;
+  c->dzv24 = oned24 * c->dziv;
+  c->dxv48 = 0.5f * c->dxv24;
+  c->dyv48 = 0.5f * c->dyv24;
+  c->dzv48 = 0.5f * c->dzv24;
+  c->fourd3 = 4.0f / 3.0f;
+  c->g05 = 0.5f * 9.8f;
+  c->g05n = -0.5f * 9.8f;
+  c->weicoe = (float)cf->weicoe;
+  c->weic1m = 1.0f - c->weicoe;
+  c->filcoe = (float)cf->filcoe;
+  c->m1fc2 = 1.0f - 2.0f * c->filcoe;
+  c->dz = dz;
+  c->thresq = (float)cf->thresq;
+}
+
+int cress_ctx_create(const cress_config *cfg, cress_ctx **out) {
+  if (!out) return CRESS_EINVAL;
+  *out = nullptr;
+  int rc = check_config(cfg);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CRESS_ENODEV;
+  cress_ctx *x = new cress_ctx();
+  x->cfg = *cfg;
+  x->g = make_grid(cfg->ni, cfg->nj, cfg->nk);
+  x->n = x->g.n();
+  fold_consts(cfg, &x->c);
+  Opts &o = x->o;
+  o.advopt = cfg->advopt;
+  o.divopt = cfg->divopt;
+  o.buyopt = cfg->buyopt;
+  o.bbc = cfg->bbc;
+  o.tbc = cfg->tbc;
+  o.sthopt = cfg->sthopt;
+  o.coropt = cfg->coropt;
+  o.cphopt = cfg->cphopt;
+  o.iw = cfg->wbc == -1 ? 0 : 1;
+  o.ie = cfg->ebc == -1 ? 0 : 1;
+  o.js = cfg->sbc == -1 ? 0 : 1;
+  o.jn = cfg->nbc == -1 ? 0 : 1;
+  const int vs = cfg->vspopt == 2;
+  o.vsp_u = vs && cfg->vspvar[0] == 'o';
+  o.vsp_v = vs && cfg->vspvar[1] == 'o';
+  o.vsp_w = vs && cfg->vspvar[2] == 'o';
+  o.vsp_p = vs && cfg->vspvar[3] == 'o';
+  o.vsp_pt = vs && cfg->vspvar[4] == 'o';
+  o.vsp_qv = vs && cfg->vspvar[5] == 'o';
+  o.vsp_q = vs && cfg->vspvar[6] == 'o';
+  o.ksp0 = cfg->ksp0[1];
+  if (hipStreamCreate(&x->stream) != hipSuccess) {
+    delete x;
+    return CRESS_EHIP;
+  }
+  const long n = x->n;
+  const long n2 = (long)(cfg->ni + 2) * (cfg->nj + 2);
+#define AL(p, cnt) if ((rc = dalloc(x, &(p), (size_t)(cnt))) != 0) goto fail
+  for (int f = 0; f < CRESS_NBASE; f++) AL(x->base[f], n);
+  AL(x->fc, 2 * n2);
+  AL(x->rbct, 2L * cfg->ni * cfg->nj * cfg->nk);
+  for (int l = 0; l < 3; l++) {
+    AL(x->u[l], n);
+    AL(x->v[l], n);
+    AL(x->w[l], n);
+    AL(x->pp[l], n);
+    AL(x->ptp[l], n);
+    AL(x->q[l], 6 * n);
+    AL(x->nci[l], n);
+  }
+  AL(x->prwtr, 4 * n2);
+  AL(x->price, 6 * n2);
+  AL(x->ufrc, n);
+  AL(x->vfrc, n);
+  AL(x->wfrc, n);
+  AL(x->pfrc, n);
+  AL(x->ptfrc, n);
+  AL(x->qvfrc, n);
+  AL(x->wc, n);
+  AL(x->ptil, n);
+  AL(x->rstxu, n);
+  AL(x->rstxv, n);
+  AL(x->rstxwc, n);
+  if (cfg->cphopt == 2) {
+    for (int f = 0; f < MicroScratch::NF; f++) AL(x->ms.f[f], n);
+    AL(x->ms.col, 64);
+  }
+#undef AL
+  for (int s = 0; s <= NSTAGE; s++) hipEventCreate(&x->ev[s]);
+  for (int s = 0; s < 2 * MAXEV; s++) hipEventCreate(&x->kev[s]);
+  x->rot = 0;
+  *out = x;
+  return CRESS_OK;
+fail:
+  cress_ctx_destroy(x);
+  return rc;
+}
+
+int cress_ctx_destroy(cress_ctx *x) {
+  if (!x) return CRESS_OK;
+  hipStreamSynchronize(x->stream);
+  for (void *p : x->allocs) hipFree(p);
+  for (int s = 0; s <= NSTAGE; s++)
+    if (x->ev[s]) hipEventDestroy(x->ev[s]);
+  for (int s = 0; s < 2 * MAXEV; s++)
+    if (x->kev[s]) hipEventDestroy(x->kev[s]);
+  hipStreamDestroy(x->stream);
+  delete x;
+  return CRESS_OK;
+}
+
+static float *field_ptr(cress_ctx *x, int f, int level, size_t *count) {
+  const long n = x->n;
+  const long n2 = (long)(x->cfg.ni + 2) * (x->cfg.nj + 2);
+  *count = (size_t)n;
+  if (f >= 0 && f < CRESS_NBASE) return level == 0 ? x->base[f] : nullptr;
+  if (f == CRESS_FC) {
+    *count = 2 * n2;
+    return level == 0 ? x->fc : nullptr;
+  }
+  if (f == CRESS_RBCT) {
+    *count = 2L * x->cfg.ni * x->cfg.nj * x->cfg.nk;
+    return level == 0 ? x->rbct : nullptr;
+  }
+  if (f == CRESS_PRWTR) {
+    *count = 4 * n2;
+    return level == 0 ? x->prwtr : nullptr;
+  }
+  if (f == CRESS_PRICE) {
+    *count = 6 * n2;
+    return level == 0 ? x->price : nullptr;
+  }
+  if (f >= CRESS_U && f < CRESS_NPROG_END) {
+    if (level < 0 || level > 2) return nullptr;
+    const int s = (x->rot + level) % 3;
+    switch (f) {
+      case CRESS_U: return x->u[s];
+      case CRESS_V: return x->v[s];
+      case CRESS_W: return x->w[s];
+      case CRESS_PP: return x->pp[s];
+      case CRESS_PTP: return x->ptp[s];
+      case CRESS_NCI: return x->nci[s];
+      default: return x->q[s] + (long)(f - CRESS_QV) * n;
+    }
+  }
+  return nullptr;
+}
+
+int cress_field_size(const cress_ctx *x, int f, size_t *nfloat) {
+  size_t c = 0;
+  if (!x || !nfloat) return CRESS_EINVAL;
+  if (!field_ptr(const_cast<cress_ctx *>(x), f, 0, &c)) return CRESS_ERANGE;
+  *nfloat = c;
+  return CRESS_OK;
+}
+
+int cress_upload(cress_ctx *x, int f, int level, const float *host) {
+  size_t c = 0;
+  if (!x || !host) return CRESS_EINVAL;
+  float *d = field_ptr(x, f, level, &c);
+  if (!d) return CRESS_ERANGE;
+  HIPCHK(hipMemcpyAsync(d, host, c * sizeof(float), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return CRESS_OK;
+}
+
+int cress_download(cress_ctx *x, int f, int level, float *host) {
+  size_t c = 0;
+  if (!x || !host) return CRESS_EINVAL;
+  float *d = field_ptr(x, f, level, &c);
+  if (!d) return CRESS_ERANGE;
+  HIPCHK(hipMemcpyAsync(host, d, c * sizeof(float), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return CRESS_OK;
+}
+
+int cress_field_device_ptr(cress_ctx *x, int f, int level, void **dptr) {
+  size_t c = 0;
+  if (!x || !dptr) return CRESS_EINVAL;
+  float *d = field_ptr(x, f, level, &c);
+  if (!d) return CRESS_ERANGE;
+  *dptr = d;
+  return CRESS_OK;
+}
+
+int cress_sync(cress_ctx *x) {
+  if (!x) return CRESS_EINVAL;
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return CRESS_OK;
+}
+
+int cress_set_timing(cress_ctx *x, int on) {
+  if (!x) return CRESS_EINVAL;
+  x->timing = on;
+  return CRESS_OK;
+}
+
+int cress_stage_times(cress_ctx *x, double *ms, int nstage) {
+  if (!x || !ms) return CRESS_EINVAL;
+  for (int s = 0; s < nstage && s < NSTAGE; s++) ms[s] = x->stage_ms[s];
+  return CRESS_OK;
+}
+
+int cress_kernel_count(void) { return KT_N; }
+
+const char *cress_kernel_name(int id) { return id >= 0 && id < KT_N ? kKernelNames[id] : ""; }
+
+int cress_kernel_times(cress_ctx *x, double *total_ms, long *launches, int n) {
+  if (!x || !total_ms || !launches) return CRESS_EINVAL;
+  for (int k = 0; k < n && k < KT_N; k++) {
+    total_ms[k] = x->kern_ms[k];
+    launches[k] = x->kern_n[k];
+  }
+  return CRESS_OK;
+}
+
+int cress_reset_timing(cress_ctx *x) {
+  if (!x) return CRESS_EINVAL;
+  for (int k = 0; k < KT_N; k++) {
+    x->kern_ms[k] = 0.0;
+    x->kern_n[k] = 0;
+  }
+  for (int s = 0; s < NSTAGE; s++) x->stage_ms[s] = 0.0;
+  return CRESS_OK;
+}
+
+// Launch with optional event bracketing.
+#define LAUNCH(x, id, kern, grid, block, lds, ...)                                     \
+  do {                                                                                 \
+    const bool t_ = (x)->timing && (x)->nkev < MAXEV;                                  \
+    if (t_) hipEventRecord((x)->kev[2 * (x)->nkev], (x)->stream);                      \
+    hipLaunchKernelGGL(kern, grid, block, lds, (x)->stream, __VA_ARGS__);              \
+    if (t_) {                                                                          \
+      hipEventRecord((x)->kev[2 * (x)->nkev + 1], (x)->stream);                        \
+      (x)->kid[(x)->nkev++] = (id);                                                    \
+    }                                                                                  \
+  } while (0)
+
+static void flush_kernel_times(cress_ctx *x) {
+  if (!x->nkev) return;
+  hipEventSynchronize(x->kev[2 * x->nkev - 1]);
+  for (int e = 0; e < x->nkev; e++) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, x->kev[2 * e], x->kev[2 * e + 1]);
+    x->kern_ms[x->kid[e]] += ms;
+    x->kern_n[x->kid[e]] += 1;
+  }
+  x->nkev = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+// ---------------------------------------------------------------------------
+static inline dim3 grid2(int ni, int nj, int bx, int by, int nz) {
+  return dim3((unsigned)((ni + bx - 1) / bx), (unsigned)((nj + by - 1) / by), (unsigned)nz);
+}
+
+static void fill_spec(FillSpec *s, float *p, int iws, int iwr, int ies, int ier, int jss, int jsr,
+                      int jns, int jnr, int kmax) {
+  s->ptr = p;
+  s->iwsnd = iws;
+  s->iwrcv = iwr;
+  s->iesnd = ies;
+  s->iercv = ier;
+  s->jssnd = jss;
+  s->jsrcv = jsr;
+  s->jnsnd = jns;
+  s->jnrcv = jnr;
+  s->kmax = kmax;
+}
+
+static void run_fill(cress_ctx *x, const FillSet &fs) {
+  if (fs.n == 0) return;
+  const Grid &g = x->g;
+  int kmax = 0;
+  for (int f = 0; f < fs.n; f++) kmax = fs.f[f].kmax > kmax ? fs.f[f].kmax : kmax;
+  LAUNCH(x, KT_FILL, k_bcycle_x, dim3((g.nj + 2 + 63) / 64, kmax, fs.n), dim3(64), 0, g, fs);
+  LAUNCH(x, KT_FILL, k_bcycle_y, dim3((g.ni + 2 + 63) / 64, kmax, fs.n), dim3(64), 0, g, fs);
+}
+
+static void run_vbc(cress_ctx *x, const VbcSet &vs) {
+  const Grid &g = x->g;
+  LAUNCH(x, KT_VBC, k_vbc, grid2(g.ni, g.nj, 64, 4, vs.n), dim3(64, 4), 0, g, x->o.bbc, vs);
+}
+
+static inline void stamp(cress_ctx *x, int s) {
+  if (x->timing) hipEventRecord(x->ev[s], x->stream);
+}
+
+// ---------------------------------------------------------------------------
+// One large step.
+// ---------------------------------------------------------------------------
+static int large_step(cress_ctx *x, int first) {
+  const Grid g = x->g;
+  const Opts o = x->o;
+  const Consts c = x->c;
+  const cress_config &cf = x->cfg;
+  hipStream_t st = x->stream;
+  const int ni = g.ni, nj = g.nj, nk = g.nk;
+  const long n = x->n;
+  const int P = (x->rot + 0) % 3, C = (x->rot + 1) % 3, F = (x->rot + 2) % 3;
+  const float dtb = first ? 0.5f * (float)cf.dtbig : (float)cf.dtbig;
+  const float dts = first ? 0.5f * (float)cf.dtsml : (float)cf.dtsml;
+  float **B = x->base;
+  const dim3 b2(64, 4);
+
+  stamp(x, 0);
+  // ---- forcing (forcedrv) on the present level ----
+  LAUNCH(x, KT_PREP, k_prep, grid2(ni + 2, nj + 2, 64, 4, nk), b2, 0, g, o, x->u[C], x->v[C],
+                     x->w[C], B[CRESS_RST8U], B[CRESS_RST8V], B[CRESS_RST8W], B[CRESS_JCB8W], x->wc,
+                     x->rstxu, x->rstxv, x->rstxwc);
+  ForceIn fi;
+  fi.u = x->u[C];
+  fi.v = x->v[C];
+  fi.w = x->w[C];
+  fi.up = x->u[P];
+  fi.vp = x->v[P];
+  fi.wp = x->w[P];
+  fi.ptp = x->ptp[C];
+  fi.q = x->q[C];
+  fi.qvbr = B[CRESS_QVBR];
+  fi.ptbr = B[CRESS_PTBR];
+  fi.ubr = B[CRESS_UBR];
+  fi.vbr = B[CRESS_VBR];
+  fi.rst = B[CRESS_RST];
+  fi.rst8u = B[CRESS_RST8U];
+  fi.rst8v = B[CRESS_RST8V];
+  fi.rst8w = B[CRESS_RST8W];
+  fi.rstxu = x->rstxu;
+  fi.rstxv = x->rstxv;
+  fi.rstxwc = x->rstxwc;
+  fi.fc = x->fc;
+  fi.rbct = x->rbct;
+  fi.qn = n;
+  LAUNCH(x, KT_FORCE_UV, k_force_uv, grid2(ni - 2, nj - 2, 64, 4, nk - 3), b2, 0, g, o, c, fi, x->ufrc,
+                     x->vfrc);
+  LAUNCH(x, KT_FORCE_W, k_force_w, grid2(ni - 3, nj - 3, 64, 4, nk - 2), b2, 0, g, o, c, fi, x->wfrc);
+  LAUNCH(x, KT_FORCE_P, k_force_p, grid2(ni - 3, nj - 3, 64, 4, nk - 3), b2, 0, g, o, c, B[CRESS_JCB],
+                     B[CRESS_JCB8U], B[CRESS_JCB8V], B[CRESS_JCB8W], x->u[C], x->v[C], x->wc, x->pp[C],
+                     x->pp[P], x->rbct, x->pfrc);
+  ScalarIn si;
+  const int nsc = cf.cphopt == 2 ? 8 : 2;
+  {
+    const float *sC[8] = {x->ptp[C], x->q[C], x->q[C] + n, x->q[C] + 2 * n, x->q[C] + 3 * n,
+                          x->q[C] + 4 * n, x->q[C] + 5 * n, x->nci[C]};
+    const float *sP[8] = {x->ptp[P], x->q[P], x->q[P] + n, x->q[P] + 2 * n, x->q[P] + 3 * n,
+                          x->q[P] + 4 * n, x->q[P] + 5 * n, x->nci[P]};
+    float *fr[8] = {x->ptfrc, x->qvfrc, x->q[F] + n, x->q[F] + 2 * n, x->q[F] + 3 * n,
+                    x->q[F] + 4 * n, x->q[F] + 5 * n, x->nci[F]};
+    for (int s = 0; s < 8; s++) {
+      si.s[s] = sC[s];
+      si.sp[s] = sP[s];
+      si.frc[s] = fr[s];
+    }
+  }
+  si.rstxu = x->rstxu;
+  si.rstxv = x->rstxv;
+  si.rstxwc = x->rstxwc;
+  si.rst = B[CRESS_RST];
+  si.qvbr = B[CRESS_QVBR];
+  si.rbct = x->rbct;
+  si.ptbr = B[CRESS_PTBR];
+  si.rbr = B[CRESS_RBR];
+  si.w = x->w[C];
+  LAUNCH(x, KT_FORCE_S, k_force_s, grid2(ni - 3, nj - 3, 64, 4, nsc * (nk - 3)), b2, 0, g, o, c, si, nsc);
+  stamp(x, 1);
+
+  // ---- acoustic substeps (smalldrv / hevi) on the future level ----
+  HIPCHK(hipMemcpyAsync(x->u[F], x->u[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(x->v[F], x->v[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(x->w[F], x->w[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(x->pp[F], x->pp[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  LAUNCH(x, KT_PHY2CNT, k_phy2cnt, grid2(ni - 1, nj - 1, 64, 4, nk), b2, 0, g, o, x->w[F], B[CRESS_JCB8W],
+                     x->wc);
+  const float dz = (float)cf.dz;
+  const float divcv = 0.05f / dts * dz * dz;
+  FillSet fuv;
+  fuv.n = 2;
+  fill_spec(&fuv.f[0], x->u[F], 3, 1, ni - 2, ni, 2, 1, nj - 2, nj - 1, nk);
+  fill_spec(&fuv.f[1], x->v[F], 2, 1, ni - 2, ni - 1, 3, 1, nj - 2, nj, nk);
+  VbcSet vuv;
+  vuv.n = 2;
+  vuv.kind[0] = 1;
+  vuv.ptr[0] = x->u[F];
+  vuv.kind[1] = 2;
+  vuv.ptr[1] = x->v[F];
+  FillSet fwp;
+  fwp.n = 2;
+  fill_spec(&fwp.f[0], x->w[F], 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk);
+  fill_spec(&fwp.f[1], x->pp[F], 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk);
+  AcCol ac;
+  ac.dts = dts;
+  ac.uf = x->u[F];
+  ac.vf = x->v[F];
+  ac.ptil = x->ptil;
+  ac.pfrc = x->pfrc;
+  ac.wfrc = x->wfrc;
+  ac.jcb = B[CRESS_JCB];
+  ac.jcb8u = B[CRESS_JCB8U];
+  ac.jcb8v = B[CRESS_JCB8V];
+  ac.jcb8w = B[CRESS_JCB8W];
+  ac.rcsq = B[CRESS_RCSQ];
+  ac.rst = B[CRESS_RST];
+  ac.rbr = B[CRESS_RBR];
+  ac.rst8w = B[CRESS_RST8W];
+  ac.wf = x->w[F];
+  ac.wc = x->wc;
+  ac.ppf = x->pp[F];
+  const size_t col_lds = (size_t)2 * (nk + 1) * 64 * sizeof(float);
+  const int nring = 2 * (ni - 1) + 2 * (nj - 3);
+  for (int s = 0; s < cf.nsstp; s++) {
+    LAUNCH(x, KT_AC_PTIL, k_ac_ptil, grid2(ni - 1, nj - 1, 64, 4, nk - 1), b2, 0, g, o, c, divcv, B[CRESS_JCB],
+                       B[CRESS_RST8U], B[CRESS_RST8V], B[CRESS_RST8W], x->u[F], x->v[F], x->wc, x->pp[F],
+                       x->ptil);
+    LAUNCH(x, KT_AC_UV, k_ac_uv, grid2(ni - 2, nj - 2, 64, 4, nk - 3), b2, 0, g, c, dts, B[CRESS_JCB],
+                       x->ptil, x->ufrc, x->vfrc, B[CRESS_RST8U], B[CRESS_RST8V], x->u[F], x->v[F]);
+    run_fill(x, fuv);
+    run_vbc(x, vuv);
+    LAUNCH(x, KT_AC_COL, k_ac_col, dim3((ni - 3 + 63) / 64, nj - 3, 1), dim3(64), col_lds, g, o, c, ac);
+    run_fill(x, fwp);
+    LAUNCH(x, KT_AC_RING, k_ac_ring, dim3((nring + 63) / 64), dim3(64), 0, g, o, B[CRESS_JCB8W], x->w[F],
+                       x->wc, x->pp[F]);
+  }
+  stamp(x, 2);
+
+  // ---- scalar leapfrog (steps) ----
+  StepsIn sti;
+  sti.ptpp = x->ptp[P];
+  sti.ptfrc = x->ptfrc;
+  sti.qp = x->q[P];
+  sti.qvfrc = x->qvfrc;
+  sti.ncip = x->nci[P];
+  sti.ptpf = x->ptp[F];
+  sti.qf = x->q[F];
+  sti.ncif = x->nci[F];
+  sti.qn = n;
+  const int withq = cf.cphopt == 2;
+  LAUNCH(x, KT_STEPS, k_steps, grid2(ni - 3, nj - 3, 64, 4, nk - 3), b2, 0, g, 2.0f * dtb, B[CRESS_RST], sti,
+                     withq);
+  {
+    FillSet fs;
+    VbcSet vs;
+    fs.n = withq ? 8 : 2;
+    vs.n = fs.n;
+    float *fl[8] = {x->ptp[F], x->q[F], x->q[F] + n, x->q[F] + 2 * n, x->q[F] + 3 * n,
+                    x->q[F] + 4 * n, x->q[F] + 5 * n, x->nci[F]};
+    for (int s = 0; s < 8; s++) {
+      fill_spec(&fs.f[s], fl[s], 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk);
+      vs.kind[s] = 0;
+      vs.ptr[s] = fl[s];
+    }
+    run_fill(x, fs);
+    run_vbc(x, vs);
+  }
+  stamp(x, 3);
+
+  // ---- microphysics (cloudphy) on the future level ----
+  if (cf.cphopt == 2) {
+    int rc = micro_cloudphy(x->g, x->c, x->o, st, dtb, first, B, x->ptp[P], x->q[P], x->nci[P], x->pp[P],
+                            x->ptp[F], x->q[F], x->nci[F], x->w[F], x->prwtr, x->price, &x->ms);
+    if (rc) return rc;
+  }
+  stamp(x, 4);
+
+  // ---- Asselin filter (timeflt) ----
+  if (!first) {
+    AsselinSet as;
+    as.n = withq ? 12 : 6;
+    float *xc[12] = {x->u[C], x->v[C], x->w[C], x->pp[C], x->ptp[C], x->q[C], x->q[C] + n, x->q[C] + 2 * n,
+                     x->q[C] + 3 * n, x->q[C] + 4 * n, x->q[C] + 5 * n, x->nci[C]};
+    float *xp[12] = {x->u[P], x->v[P], x->w[P], x->pp[P], x->ptp[P], x->q[P], x->q[P] + n, x->q[P] + 2 * n,
+                     x->q[P] + 3 * n, x->q[P] + 4 * n, x->q[P] + 5 * n, x->nci[P]};
+    float *xf[12] = {x->u[F], x->v[F], x->w[F], x->pp[F], x->ptp[F], x->q[F], x->q[F] + n, x->q[F] + 2 * n,
+                     x->q[F] + 3 * n, x->q[F] + 4 * n, x->q[F] + 5 * n, x->nci[F]};
+    const int kinds[12] = {1, 2, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int f = 0; f < 12; f++) {
+      as.kind[f] = kinds[f];
+      as.x[f] = xc[f];
+      as.xp[f] = xp[f];
+      as.xf[f] = xf[f];
+    }
+    LAUNCH(x, KT_TIMEFLT, k_timeflt, grid2(ni, nj, 64, 4, 12 * nk), b2, 0, g, c, as);
+  }
+  stamp(x, 5);
+
+  // ---- saturation adjustment (satadjst) on present and future ----
+  if (cf.cphopt == 2) {
+    int rc = micro_satadjst(x->g, x->c, x->o, st, B, x->pp[C], x->ptp[C], x->q[C], x->nci[C], &x->ms);
+    if (!rc) rc = micro_satadjst(x->g, x->c, x->o, st, B, x->pp[F], x->ptp[F], x->q[F], x->nci[F], &x->ms);
+    if (rc) return rc;
+  }
+  stamp(x, 6);
+
+  // ---- second halo rows for 4th-order advection (shift2nd) ----
+  for (int lv = 0; lv < 2; lv++) {
+    const int S = lv == 0 ? C : F;
+    FillSet fs;
+    fs.n = withq ? 12 : 6;
+    fill_spec(&fs.f[0], x->u[S], 4, 0, ni - 3, ni + 1, 3, 0, nj - 3, nj, nk);
+    fill_spec(&fs.f[1], x->v[S], 3, 0, ni - 3, ni, 4, 0, nj - 3, nj + 1, nk);
+    float *sc[10] = {x->w[S], x->pp[S], x->ptp[S], x->q[S], x->q[S] + n, x->q[S] + 2 * n,
+                     x->q[S] + 3 * n, x->q[S] + 4 * n, x->q[S] + 5 * n, x->nci[S]};
+    for (int f = 0; f < 10; f++) fill_spec(&fs.f[2 + f], sc[f], 3, 0, ni - 3, ni, 3, 0, nj - 3, nj, nk);
+    run_fill(x, fs);
+  }
+  stamp(x, 7);
+  // ---- swp2nxt: past <- present <- future by renaming ----
+  x->rot = (x->rot + 1) % 3;
+  stamp(x, 8);
+  HIPCHK(hipGetLastError());
+  return CRESS_OK;
+}
+
+int cress_large_step(cress_ctx *x, int nsteps, int first) {
+  if (!x || nsteps < 0) return CRESS_EINVAL;
+  for (int s = 0; s < nsteps; s++) {
+    int rc = large_step(x, first && s == 0);
+    if (rc) return rc;
+    if (x->timing) {
+      HIPCHK(hipEventSynchronize(x->ev[NSTAGE]));
+      flush_kernel_times(x);
+      for (int t = 0; t < NSTAGE; t++) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, x->ev[t], x->ev[t + 1]);
+        x->stage_ms[t] += ms;
+      }
+    }
+  }
+  return CRESS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Tier 2 shims
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) hipFree(p);
+  }
+};
+
+static int to_dev(DevBuf &b, const float *h, size_t n) {
+  if (hipMalloc(&b.p, n * sizeof(float)) != hipSuccess) return CRESS_ENOMEM;
+  if (h && hipMemcpy(b.p, h, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return CRESS_EHIP;
+  return CRESS_OK;
+}
+
+int cress_gaussel(int istr, int iend, int jstr, int jend, int kstr, int kend, int ni, int nj, int nk,
+                  float *rr, float *ss, float *tt, float *ff) {
+  if (!rr || !ss || !tt || !ff || ni < 1 || nj < 1 || nk < 1) return CRESS_EINVAL;
+  if (istr < 0 || iend > ni + 1 || jstr < 0 || jend > nj + 1 || kstr < 1 || kend > nk || kstr > kend)
+    return CRESS_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CRESS_ENODEV;
+  const Grid g = make_grid(ni, nj, nk);
+  const size_t n = (size_t)g.n();
+  DevBuf a, b, c, d;
+  int rc;
+  if ((rc = to_dev(a, rr, n)) || (rc = to_dev(b, ss, n)) || (rc = to_dev(c, tt, n)) || (rc = to_dev(d, ff, n)))
+    return rc;
+  const int ncol = (iend - istr + 1) * (jend - jstr + 1);
+  hipLaunchKernelGGL(k_gaussel, dim3((ncol + 63) / 64), dim3(64), 0, 0, g, istr, iend, jstr, jend, kstr, kend,
+                     (const float *)a.p, (float *)b.p, (const float *)c.p, (float *)d.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(ss, b.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(ff, d.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return CRESS_OK;
+}
+
+int cress_advs(const cress_config *cfg, const float *rstxu, const float *rstxv, const float *rstxwc,
+               const float *s, float *sfrc) {
+  if (!cfg || !rstxu || !rstxv || !rstxwc || !s || !sfrc) return CRESS_EINVAL;
+  if (cfg->ni < 7 || cfg->nj < 7 || cfg->nk < 7 || cfg->advopt < 1 || cfg->advopt > 3) return CRESS_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CRESS_ENODEV;
+  const Grid g = make_grid(cfg->ni, cfg->nj, cfg->nk);
+  const size_t n = (size_t)g.n();
+  Consts c;
+  fold_consts(cfg, &c);
+  Opts o = {};
+  o.advopt = cfg->advopt;
+  o.iw = cfg->wbc == -1 ? 0 : 1;
+  o.ie = cfg->ebc == -1 ? 0 : 1;
+  o.js = cfg->sbc == -1 ? 0 : 1;
+  o.jn = cfg->nbc == -1 ? 0 : 1;
+  DevBuf a, b, d, e, f;
+  int rc;
+  if ((rc = to_dev(a, rstxu, n)) || (rc = to_dev(b, rstxv, n)) || (rc = to_dev(d, rstxwc, n)) ||
+      (rc = to_dev(e, s, n)) || (rc = to_dev(f, sfrc, n)))
+    return rc;
+  ScalarIn si = {};
+  si.s[0] = (const float *)e.p;
+  si.frc[0] = (float *)f.p;
+  si.rstxu = (const float *)a.p;
+  si.rstxv = (const float *)b.p;
+  si.rstxwc = (const float *)d.p;
+  hipLaunchKernelGGL(k_advs_only, grid2(g.ni - 3, g.nj - 3, 64, 4, g.nk - 3), dim3(64, 4), 0, 0, g, o, c, si);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(sfrc, f.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return CRESS_OK;
+}
+
+int cress_bcycle(const cress_config *cfg, int iwsnd, int iwrcv, int iesnd, int iercv, int jssnd, int jsrcv,
+                 int jnsnd, int jnrcv, int kmax, float *var) {
+  if (!cfg || !var || kmax < 1 || kmax > cfg->nk) return CRESS_EINVAL;
+  const int ni = cfg->ni, nj = cfg->nj;
+  const int is[4] = {iwsnd, iwrcv, iesnd, iercv}, js[4] = {jssnd, jsrcv, jnsnd, jnrcv};
+  for (int t = 0; t < 4; t++)
+    if (is[t] < 0 || is[t] > ni + 1 || js[t] < 0 || js[t] > nj + 1) return CRESS_EINVAL;
+  if (!(cfg->wbc == -1 && cfg->ebc == -1 && cfg->sbc == -1 && cfg->nbc == -1)) return CRESS_EUNSUPPORTED;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CRESS_ENODEV;
+  const Grid g = make_grid(ni, nj, cfg->nk);
+  const size_t n = (size_t)g.n();
+  DevBuf a;
+  int rc = to_dev(a, var, n);
+  if (rc) return rc;
+  FillSet fs;
+  fs.n = 1;
+  fill_spec(&fs.f[0], (float *)a.p, iwsnd, iwrcv, iesnd, iercv, jssnd, jsrcv, jnsnd, jnrcv, kmax);
+  hipLaunchKernelGGL(k_bcycle_x, dim3((g.nj + 2 + 63) / 64, kmax, 1), dim3(64), 0, 0, g, fs);
+  hipLaunchKernelGGL(k_bcycle_y, dim3((g.ni + 2 + 63) / 64, kmax, 1), dim3(64), 0, 0, g, fs);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(var, a.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return CRESS_OK;
+}

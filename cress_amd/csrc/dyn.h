@@ -1,0 +1,87 @@
+// Kernel argument blocks and launch entry points of the dynamics kernels.
+#pragma once
+#include "common.h"
+
+namespace cress {
+
+struct ForceIn {
+  const float *u, *v, *w, *up, *vp, *wp, *ptp, *q, *qvbr, *ptbr, *ubr, *vbr;
+  const float *rst, *rst8u, *rst8v, *rst8w, *rstxu, *rstxv, *rstxwc, *fc, *rbct;
+  long qn;  // species stride in q (qv,qc,qr,qi,qs,qg)
+};
+
+struct ScalarIn {
+  const float *s[8];    // present level: ptp, qv, qc, qr, qi, qs, qg, nci
+  const float *sp[8];   // past level
+  float *frc[8];        // forcing outputs
+  const float *rstxu, *rstxv, *rstxwc, *rst, *qvbr, *rbct, *ptbr, *rbr, *w;
+};
+
+struct AcCol {
+  float dts;
+  const float *uf, *vf, *ptil, *pfrc, *wfrc;
+  const float *jcb, *jcb8u, *jcb8v, *jcb8w, *rcsq, *rst, *rbr, *rst8w;
+  float *wf, *wc, *ppf;
+};
+
+struct StepsIn {
+  const float *ptpp, *ptfrc, *qp, *qvfrc, *ncip;
+  float *ptpf, *qf, *ncif;
+  long qn;
+};
+
+struct FillSpec {
+  float *ptr;
+  int iwsnd, iwrcv, iesnd, iercv, jssnd, jsrcv, jnsnd, jnrcv, kmax;
+};
+struct FillSet {
+  int n;
+  FillSpec f[16];
+};
+
+struct VbcSet {
+  int n;
+  int kind[16];  // 0 scalar, 1 u, 2 v, 3 p
+  float *ptr[16];
+};
+
+struct AsselinSet {
+  int n;
+  int kind[12];  // 0 scalar, 1 u, 2 v, 3 w
+  float *x[12];
+  const float *xp[12];
+  const float *xf[12];
+};
+
+__global__ void k_phy2cnt(Grid g, Opts o, const float *w, const float *jcb8w, float *wc);
+__global__ void k_prep(Grid g, Opts o, const float *u, const float *v, const float *w,
+                       const float *rst8u, const float *rst8v, const float *rst8w,
+                       const float *jcb8w, float *wc, float *rstxu, float *rstxv, float *rstxwc);
+__global__ void k_force_uv(Grid g, Opts o, Consts c, ForceIn in, float *ufrc, float *vfrc);
+__global__ void k_force_w(Grid g, Opts o, Consts c, ForceIn in, float *wfrc);
+__global__ void k_force_p(Grid g, Opts o, Consts c, const float *jcb, const float *jcb8u,
+                          const float *jcb8v, const float *jcb8w, const float *u, const float *v,
+                          const float *wc, const float *pp, const float *ppp, const float *rbct,
+                          float *pfrc);
+__global__ void k_force_s(Grid g, Opts o, Consts c, ScalarIn in, int nscalar);
+__global__ void k_ac_ptil(Grid g, Opts o, Consts c, float divcv, const float *jcb, const float *rst8u,
+                          const float *rst8v, const float *rst8w, const float *uf, const float *vf,
+                          const float *wc, const float *ppf, float *ptil);
+__global__ void k_ac_uv(Grid g, Consts c, float dts, const float *jcb, const float *ptil,
+                        const float *ufrc, const float *vfrc, const float *rst8u, const float *rst8v,
+                        float *uf, float *vf);
+__global__ void k_ac_col(Grid g, Opts o, Consts c, AcCol a);
+__global__ void k_ac_ring(Grid g, Opts o, const float *jcb8w, float *wf, float *wc, float *ppf);
+__global__ void k_steps(Grid g, float dtb2, const float *rst, StepsIn s, int withq);
+__global__ void k_bcycle_x(Grid g, FillSet fs);
+__global__ void k_bcycle_y(Grid g, FillSet fs);
+__global__ void k_vbc(Grid g, int bbc, VbcSet vs);
+__global__ void k_timeflt(Grid g, Consts c, AsselinSet as);
+
+}  // namespace cress
+
+namespace cress {
+__global__ void k_gaussel(Grid g, int istr, int iend, int jstr, int jend, int kstr, int kend,
+                          const float *rr, float *ss, const float *tt, float *ff);
+__global__ void k_advs_only(Grid g, Opts o, Consts c, ScalarIn in);
+}  // namespace cress

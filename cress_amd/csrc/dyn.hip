@@ -1,0 +1,739 @@
+// Dynamics kernels of one CReSS large step for gfx950: forcing (advection,
+// sponge, Coriolis, buoyancy), the HE-VI acoustic substep, the scalar
+// leapfrog update, the Asselin filter and the on-device halo fills.
+//
+// Every kernel restates a reference routine's arithmetic with the same
+// operation order (the build uses -ffp-contract=off and IEEE division), so
+// the FP32 results equal the reference's FP32 results bitwise.  Ranges are
+// the reference's loop ranges; the routine each kernel replaces is cited.
+#include "common.h"
+#include "dyn.h"
+
+namespace cress {
+
+#define AT(a, i, j, k) a[g.at(i, j, k)]
+
+// rbct(1:ni,1:nj,1:nk,1:2), component 2 (vspdmp.f90:269-285)
+__device__ __forceinline__ float rbct2(const float *rbct, const Grid &g, int i, int j, int k) {
+  return rbct[(long)(i - 1) + (long)g.ni * ((long)(j - 1) + (long)g.nj * ((long)(k - 1) + g.nk))];
+}
+
+// vspuvw.f90:222-238: sponge coefficient averaged to the u/v level
+__device__ __forceinline__ float rbct8s(const float *rbct, const Grid &g, int i, int j, int k) {
+  return 0.25f * (rbct2(rbct, g, i, j, k) + rbct2(rbct, g, i, j, k + 1));
+}
+
+// ---------------------------------------------------------------------------
+// Contravariant w (phy2cnt.f90:193-227 then vbcwc.f90:130-186) and the
+// mass-flux velocities rstx* (rstuvwc.f90:172-210).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float wc_raw(const Grid &g, int sthopt, const float *w,
+                                        const float *jcb8w, int i, int j, int k) {
+  return sthopt == 0 ? AT(w, i, j, k) : AT(w, i, j, k) / AT(jcb8w, i, j, k);
+}
+
+__device__ __forceinline__ float wc_value(const Grid &g, const Opts &o, const float *w,
+                                          const float *jcb8w, int i, int j, int k) {
+  const int nk = g.nk;
+  if (k == 1) {
+    if (o.bbc == 2) return -wc_raw(g, o.sthopt, w, jcb8w, i, j, 3);
+    return wc_raw(g, o.sthopt, w, jcb8w, i, j, 2);
+  }
+  if (k == 2 && o.bbc == 2) return 0.0f;
+  if (k == nk) {
+    if (o.tbc == 2) return -wc_raw(g, o.sthopt, w, jcb8w, i, j, nk - 2);
+    return wc_raw(g, o.sthopt, w, jcb8w, i, j, nk - 1);
+  }
+  if (k == nk - 1 && o.tbc == 2) return 0.0f;
+  return wc_raw(g, o.sthopt, w, jcb8w, i, j, k);
+}
+
+// wc over i=1..ni-1, j=1..nj-1, k=1..nk (the phy2cnt+vbcwc write range)
+__global__ void k_phy2cnt(Grid g, Opts o, const float *w, const float *jcb8w, float *wc) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.ni - 1 || j > g.nj - 1) return;
+  AT(wc, i, j, k) = wc_value(g, o, w, jcb8w, i, j, k);
+}
+
+__global__ void k_prep(Grid g, Opts o, const float *u, const float *v, const float *w,
+                       const float *rst8u, const float *rst8v, const float *rst8w,
+                       const float *jcb8w, float *wc, float *rstxu, float *rstxv, float *rstxwc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.ni + 1 || j > g.nj + 1) return;
+  if (i >= 1 && i <= g.ni - 1 && j >= 1 && j <= g.nj - 1) {
+    const float c = wc_value(g, o, w, jcb8w, i, j, k);
+    AT(wc, i, j, k) = c;
+    AT(rstxwc, i, j, k) = AT(rst8w, i, j, k) * c;
+  }
+  if (k <= g.nk - 1) {
+    if (j >= 1 && j <= g.nj - 1 && i >= o.iw && i <= g.ni + 1 - o.ie)
+      AT(rstxu, i, j, k) = AT(rst8u, i, j, k) * AT(u, i, j, k);
+    if (i >= 1 && i <= g.ni - 1 && j >= o.js && j <= g.nj + 1 - o.jn)
+      AT(rstxv, i, j, k) = AT(rst8v, i, j, k) * AT(v, i, j, k);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Momentum advection, advuvw.f90:209-902.  Each returns f + advection with
+// the reference's accumulation order: advopt=1 f+(...), advopt=2
+// (f+hadv)+(t3+t3), advopt=3 (f+hadv)+vadv.
+// ---------------------------------------------------------------------------
+__device__ float adv_u(float f, const Grid &g, const Opts &o, const Consts &c, const float *u,
+                       const float *rxu, const float *rxv, const float *rxw, int i, int j, int k) {
+  auto t1 = [&](int ii) {
+    return (AT(rxu, ii, j, k) + AT(rxu, ii + 1, j, k)) * (AT(u, ii + 1, j, k) - AT(u, ii, j, k)) * c.dxv25n;
+  };
+  auto t2 = [&](int jj) {
+    return (AT(rxv, i - 1, jj, k) + AT(rxv, i, jj, k)) * (AT(u, i, jj, k) - AT(u, i, jj - 1, k)) * c.dyv25n;
+  };
+  auto t3 = [&](int kk) {
+    return (AT(rxw, i - 1, j, kk) + AT(rxw, i, j, kk)) * (AT(u, i, j, kk) - AT(u, i, j, kk - 1)) * c.dzv25n;
+  };
+  if (o.advopt == 1) return f + ((t3(k) + t3(k + 1)) + ((t1(i - 1) + t1(i)) + (t2(j) + t2(j + 1))));
+  float hadv = (t1(i - 1) + t1(i)) + (t2(j) + t2(j + 1));
+  const bool h4 = j >= 2 + o.js && j <= g.nj - 2 - o.jn && i >= 2 + o.iw && i <= g.ni - 1 - o.ie;
+  if (h4) {
+    auto T1 = [&](int ii) {
+      return (AT(rxu, ii - 1, j, k) + AT(rxu, ii + 1, j, k)) * (AT(u, ii + 1, j, k) - AT(u, ii - 1, j, k)) * c.dxv24;
+    };
+    auto T2 = [&](int jj) {
+      return ((AT(rxv, i - 1, jj, k) + AT(rxv, i, jj, k)) + (AT(rxv, i - 1, jj + 1, k) + AT(rxv, i, jj + 1, k))) *
+             (AT(u, i, jj + 1, k) - AT(u, i, jj - 1, k)) * c.dyv48;
+    };
+    hadv = c.fourd3 * hadv + ((T1(i - 1) + T1(i + 1)) + (T2(j - 1) + T2(j + 1)));
+  }
+  if (o.advopt == 2) return f + hadv + (t3(k) + t3(k + 1));
+  float vadv = t3(k) + t3(k + 1);
+  if (h4 && k >= 3 && k <= g.nk - 3) {
+    auto T3 = [&](int kk) {
+      return ((AT(rxw, i - 1, j, kk) + AT(rxw, i, j, kk)) + (AT(rxw, i - 1, j, kk + 1) + AT(rxw, i, j, kk + 1))) *
+             (AT(u, i, j, kk + 1) - AT(u, i, j, kk - 1)) * c.dzv48;
+    };
+    vadv = c.fourd3 * vadv + (T3(k - 1) + T3(k + 1));
+  }
+  return f + hadv + vadv;
+}
+
+__device__ float adv_v(float f, const Grid &g, const Opts &o, const Consts &c, const float *v,
+                       const float *rxu, const float *rxv, const float *rxw, int i, int j, int k) {
+  auto t1 = [&](int ii) {
+    return (AT(rxu, ii, j - 1, k) + AT(rxu, ii, j, k)) * (AT(v, ii, j, k) - AT(v, ii - 1, j, k)) * c.dxv25n;
+  };
+  auto t2 = [&](int jj) {
+    return (AT(rxv, i, jj, k) + AT(rxv, i, jj + 1, k)) * (AT(v, i, jj + 1, k) - AT(v, i, jj, k)) * c.dyv25n;
+  };
+  auto t3 = [&](int kk) {
+    return (AT(rxw, i, j - 1, kk) + AT(rxw, i, j, kk)) * (AT(v, i, j, kk) - AT(v, i, j, kk - 1)) * c.dzv25n;
+  };
+  if (o.advopt == 1) return f + ((t3(k) + t3(k + 1)) + ((t1(i) + t1(i + 1)) + (t2(j - 1) + t2(j))));
+  float hadv = (t1(i) + t1(i + 1)) + (t2(j - 1) + t2(j));
+  const bool h4 = j >= 2 + o.js && j <= g.nj - 1 - o.jn && i >= 2 + o.iw && i <= g.ni - 2 - o.ie;
+  if (h4) {
+    auto T1 = [&](int ii) {
+      return ((AT(rxu, ii, j - 1, k) + AT(rxu, ii, j, k)) + (AT(rxu, ii + 1, j - 1, k) + AT(rxu, ii + 1, j, k))) *
+             (AT(v, ii + 1, j, k) - AT(v, ii - 1, j, k)) * c.dxv48;
+    };
+    auto T2 = [&](int jj) {
+      return (AT(rxv, i, jj - 1, k) + AT(rxv, i, jj + 1, k)) * (AT(v, i, jj + 1, k) - AT(v, i, jj - 1, k)) * c.dyv24;
+    };
+    hadv = c.fourd3 * hadv + ((T1(i - 1) + T1(i + 1)) + (T2(j - 1) + T2(j + 1)));
+  }
+  if (o.advopt == 2) return f + hadv + (t3(k) + t3(k + 1));
+  float vadv = t3(k) + t3(k + 1);
+  if (h4 && k >= 3 && k <= g.nk - 3) {
+    auto T3 = [&](int kk) {
+      return ((AT(rxw, i, j - 1, kk) + AT(rxw, i, j, kk)) + (AT(rxw, i, j - 1, kk + 1) + AT(rxw, i, j, kk + 1))) *
+             (AT(v, i, j, kk + 1) - AT(v, i, j, kk - 1)) * c.dzv48;
+    };
+    vadv = c.fourd3 * vadv + (T3(k - 1) + T3(k + 1));
+  }
+  return f + hadv + vadv;
+}
+
+__device__ float adv_w(float f, const Grid &g, const Opts &o, const Consts &c, const float *w,
+                       const float *rxu, const float *rxv, const float *rxw, int i, int j, int k) {
+  auto t1 = [&](int ii) {
+    return (AT(rxu, ii, j, k - 1) + AT(rxu, ii, j, k)) * (AT(w, ii, j, k) - AT(w, ii - 1, j, k)) * c.dxv25n;
+  };
+  auto t2 = [&](int jj) {
+    return (AT(rxv, i, jj, k - 1) + AT(rxv, i, jj, k)) * (AT(w, i, jj, k) - AT(w, i, jj - 1, k)) * c.dyv25n;
+  };
+  auto t3 = [&](int kk) {
+    return (AT(rxw, i, j, kk) + AT(rxw, i, j, kk + 1)) * (AT(w, i, j, kk + 1) - AT(w, i, j, kk)) * c.dzv25n;
+  };
+  if (o.advopt == 1) return f + ((t3(k - 1) + t3(k)) + ((t1(i) + t1(i + 1)) + (t2(j) + t2(j + 1))));
+  float hadv = (t1(i) + t1(i + 1)) + (t2(j) + t2(j + 1));
+  const bool h4 = j >= 2 + o.js && j <= g.nj - 2 - o.jn && i >= 2 + o.iw && i <= g.ni - 2 - o.ie;
+  if (h4) {
+    auto T1 = [&](int ii) {
+      return ((AT(rxu, ii, j, k - 1) + AT(rxu, ii, j, k)) + (AT(rxu, ii + 1, j, k - 1) + AT(rxu, ii + 1, j, k))) *
+             (AT(w, ii + 1, j, k) - AT(w, ii - 1, j, k)) * c.dxv48;
+    };
+    auto T2 = [&](int jj) {
+      return ((AT(rxv, i, jj, k - 1) + AT(rxv, i, jj, k)) + (AT(rxv, i, jj + 1, k - 1) + AT(rxv, i, jj + 1, k))) *
+             (AT(w, i, jj + 1, k) - AT(w, i, jj - 1, k)) * c.dyv48;
+    };
+    hadv = c.fourd3 * hadv + ((T1(i - 1) + T1(i + 1)) + (T2(j - 1) + T2(j + 1)));
+  }
+  if (o.advopt == 2) return f + hadv + (t3(k - 1) + t3(k));
+  float vadv = t3(k - 1) + t3(k);
+  if (h4 && k >= 3 && k <= g.nk - 2) {
+    auto T3 = [&](int kk) {
+      return (AT(rxw, i, j, kk - 1) + AT(rxw, i, j, kk + 1)) * (AT(w, i, j, kk + 1) - AT(w, i, j, kk - 1)) * c.dzv24;
+    };
+    vadv = c.fourd3 * vadv + (T3(k - 1) + T3(k + 1));
+  }
+  return f + hadv + vadv;
+}
+
+// ufrc over i=2..ni-1, j=2..nj-2 and vfrc over i=2..ni-2, j=2..nj-1; k=2..nk-2.
+__global__ void k_force_uv(Grid g, Opts o, Consts c, ForceIn in, float *ufrc, float *vfrc) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 2 + blockIdx.z;
+  if (i > g.ni - 1 || j > g.nj - 1) return;
+  const bool sp = k >= o.ksp0 - 1 && k <= g.nk - 2;
+  if (j <= g.nj - 2) {
+    float f = 0.0f;
+    if (o.vsp_u && sp)
+      f = f - AT(in.rst8u, i, j, k) * (rbct8s(in.rbct, g, i - 1, j, k) + rbct8s(in.rbct, g, i, j, k)) *
+                  (AT(in.up, i, j, k) - AT(in.ubr, i, j, k));
+    if (o.coropt) {
+      auto cr = [&](int ii) {
+        return in.fc[(long)ii + g.nx * j] * AT(in.rst, ii, j, k) * (AT(in.v, ii, j, k) + AT(in.v, ii, j + 1, k));
+      };
+      f = f + (cr(i - 1) + cr(i));
+    }
+    AT(ufrc, i, j, k) = adv_u(f, g, o, c, in.u, in.rstxu, in.rstxv, in.rstxwc, i, j, k);
+  }
+  if (i <= g.ni - 2) {
+    float f = 0.0f;
+    if (o.vsp_v && sp)
+      f = f - AT(in.rst8v, i, j, k) * (rbct8s(in.rbct, g, i, j - 1, k) + rbct8s(in.rbct, g, i, j, k)) *
+                  (AT(in.vp, i, j, k) - AT(in.vbr, i, j, k));
+    if (o.coropt) {
+      auto cr = [&](int jj) {
+        return -in.fc[(long)i + g.nx * jj] * AT(in.rst, i, jj, k) * (AT(in.u, i, jj, k) + AT(in.u, i + 1, jj, k));
+      };
+      f = f + (cr(j - 1) + cr(j));
+    }
+    AT(vfrc, i, j, k) = adv_v(f, g, o, c, in.v, in.rstxu, in.rstxv, in.rstxwc, i, j, k);
+  }
+}
+
+// moist buoyancy at a scalar level, buoywb.f90:204-297 (gwmopt=0)
+__device__ __forceinline__ float wb8s_at(const Grid &g, const Opts &o, const Consts &c, const ForceIn &in,
+                                         int i, int j, int k) {
+  const long p = g.at(i, j, k);
+  const float qvbr = in.qvbr[p];
+  const float qvd = in.q[p] - qvbr;
+  if (o.cphopt == 0)
+    return c.g05 * in.rst[p] * (qvd / (0.622f + qvbr) - qvd / (1.0f + qvbr) + in.ptp[p] / in.ptbr[p]);
+  // qall: totalqwi.f90:163-164, qc+qr+qi+qs+qg left to right
+  const float qall = in.q[p + in.qn] + in.q[p + 2 * in.qn] + in.q[p + 3 * in.qn] + in.q[p + 4 * in.qn] +
+                     in.q[p + 5 * in.qn];
+  return c.g05 * in.rst[p] * (qvd / (0.622f + qvbr) - (qvd + qall) / (1.0f + qvbr) + in.ptp[p] / in.ptbr[p]);
+}
+
+// wfrc over i=2..ni-2, j=2..nj-2, k=2..nk-1
+__global__ void k_force_w(Grid g, Opts o, Consts c, ForceIn in, float *wfrc) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 2 + blockIdx.z;
+  if (i > g.ni - 2 || j > g.nj - 2) return;
+  float f = 0.0f;
+  if (o.vsp_w && k >= o.ksp0 && k <= g.nk - 1)
+    f = f - rbct2(in.rbct, g, i, j, k) * AT(in.rst8w, i, j, k) * AT(in.wp, i, j, k);
+  if (o.buyopt == 1 && k >= 3 && k <= g.nk - 2)
+    f = f + (wb8s_at(g, o, c, in, i, j, k - 1) + wb8s_at(g, o, c, in, i, j, k));
+  AT(wfrc, i, j, k) = adv_w(f, g, o, c, in.w, in.rstxu, in.rstxv, in.rstxwc, i, j, k);
+}
+
+// ---------------------------------------------------------------------------
+// Pressure forcing: s_forcep (forcep.f90:254-308) = advp (advp.f90:249-785,
+// diaopt=0) then vspp (vspp.f90:180-194).  pfrc over the interior.
+// ---------------------------------------------------------------------------
+__global__ void k_force_p(Grid g, Opts o, Consts c, const float *jcb, const float *jcb8u,
+                          const float *jcb8v, const float *jcb8w, const float *u, const float *v,
+                          const float *wc, const float *pp, const float *ppp, const float *rbct,
+                          float *pfrc) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 2 + blockIdx.z;
+  if (i > g.ni - 2 || j > g.nj - 2) return;
+  auto jxu = [&](int ii, int jj) { return AT(jcb8u, ii, jj, k) * AT(u, ii, jj, k); };
+  auto jxv = [&](int ii, int jj) { return AT(jcb8v, ii, jj, k) * AT(v, ii, jj, k); };
+  auto jxw = [&](int kk) { return AT(jcb8w, i, j, kk) * AT(wc, i, j, kk); };
+  auto t1 = [&](int ii) { return jxu(ii, j) * (AT(pp, ii, j, k) - AT(pp, ii - 1, j, k)) * c.dxv05n; };
+  auto t2 = [&](int jj) { return jxv(i, jj) * (AT(pp, i, jj, k) - AT(pp, i, jj - 1, k)) * c.dyv05n; };
+  auto t3 = [&](int kk) { return jxw(kk) * (AT(pp, i, j, kk) - AT(pp, i, j, kk - 1)) * c.dzv05n; };
+  float f;
+  if (o.advopt == 1) {
+    f = (t3(k) + t3(k + 1)) + ((t1(i) + t1(i + 1)) + (t2(j) + t2(j + 1)));
+  } else {
+    f = (t1(i) + t1(i + 1)) + (t2(j) + t2(j + 1));
+    const bool h4 = j >= 2 + o.js && j <= g.nj - 2 - o.jn && i >= 2 + o.iw && i <= g.ni - 2 - o.ie;
+    if (h4) {
+      auto T1 = [&](int ii) {
+        return (jxu(ii, j) + jxu(ii + 1, j)) * (AT(pp, ii + 1, j, k) - AT(pp, ii - 1, j, k)) * c.dxv24;
+      };
+      auto T2 = [&](int jj) {
+        return (jxv(i, jj) + jxv(i, jj + 1)) * (AT(pp, i, jj + 1, k) - AT(pp, i, jj - 1, k)) * c.dyv24;
+      };
+      f = c.fourd3 * f + ((T1(i - 1) + T1(i + 1)) + (T2(j - 1) + T2(j + 1)));
+    }
+    if (o.advopt == 2) {
+      f = f + (t3(k) + t3(k + 1));
+    } else {
+      float vadv = t3(k) + t3(k + 1);
+      if (h4 && k >= 3 && k <= g.nk - 3) {
+        auto T3 = [&](int kk) {
+          return (jxw(kk) + jxw(kk + 1)) * (AT(pp, i, j, kk + 1) - AT(pp, i, j, kk - 1)) * c.dzv24;
+        };
+        vadv = c.fourd3 * vadv + (T3(k - 1) + T3(k + 1));
+      }
+      f = f + vadv;
+    }
+  }
+  if (o.vsp_p && k >= o.ksp0 - 1 && k <= g.nk - 2)
+    f = f - 0.5f * (rbct2(rbct, g, i, j, k) + rbct2(rbct, g, i, j, k + 1)) * AT(jcb, i, j, k) * AT(ppp, i, j, k);
+  AT(pfrc, i, j, k) = f;
+}
+
+// ---------------------------------------------------------------------------
+// Scalar forcing: s_forcept/s_forceqv/s_forceq (forcept.f90:344-438,
+// forceqv.f90:290-372, forceq.f90:394-...) = advs (advs.f90:205-445) then the
+// vspopt=2 sponge (vsps.f90:183-197, vspqv.f90:185-200, vsps0.f90:114-127)
+// and, for theta', advbspt (advbspt.f90:143-190).  blockIdx.z = scalar*(nk-3)
+// + (k-2); scalar 0 ptp, 1..6 qv..qg, 7 nci.
+// ---------------------------------------------------------------------------
+
+
+__device__ float adv_s(const Grid &g, const Opts &o, const Consts &c, const float *s, const float *rxu,
+                       const float *rxv, const float *rxw, int i, int j, int k) {
+  auto t1 = [&](int ii) { return AT(rxu, ii, j, k) * (AT(s, ii, j, k) - AT(s, ii - 1, j, k)) * c.dxv05n; };
+  auto t2 = [&](int jj) { return AT(rxv, i, jj, k) * (AT(s, i, jj, k) - AT(s, i, jj - 1, k)) * c.dyv05n; };
+  auto t3 = [&](int kk) { return AT(rxw, i, j, kk) * (AT(s, i, j, kk) - AT(s, i, j, kk - 1)) * c.dzv05n; };
+  if (o.advopt == 1) return (t3(k) + t3(k + 1)) + ((t1(i) + t1(i + 1)) + (t2(j) + t2(j + 1)));
+  float f = (t1(i) + t1(i + 1)) + (t2(j) + t2(j + 1));
+  const bool h4 = j >= 2 + o.js && j <= g.nj - 2 - o.jn && i >= 2 + o.iw && i <= g.ni - 2 - o.ie;
+  if (h4) {
+    auto T1 = [&](int ii) {
+      return (AT(rxu, ii, j, k) + AT(rxu, ii + 1, j, k)) * (AT(s, ii + 1, j, k) - AT(s, ii - 1, j, k)) * c.dxv24;
+    };
+    auto T2 = [&](int jj) {
+      return (AT(rxv, i, jj, k) + AT(rxv, i, jj + 1, k)) * (AT(s, i, jj + 1, k) - AT(s, i, jj - 1, k)) * c.dyv24;
+    };
+    f = c.fourd3 * f + ((T1(i - 1) + T1(i + 1)) + (T2(j - 1) + T2(j + 1)));
+  }
+  if (o.advopt == 2) return f + (t3(k) + t3(k + 1));
+  float vadv = t3(k) + t3(k + 1);
+  if (h4 && k >= 3 && k <= g.nk - 3) {
+    auto T3 = [&](int kk) {
+      return (AT(rxw, i, j, kk) + AT(rxw, i, j, kk + 1)) * (AT(s, i, j, kk + 1) - AT(s, i, j, kk - 1)) * c.dzv24;
+    };
+    vadv = c.fourd3 * vadv + (T3(k - 1) + T3(k + 1));
+  }
+  return f + vadv;
+}
+
+__global__ void k_force_s(Grid g, Opts o, Consts c, ScalarIn in, int nscalar) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int nkk = g.nk - 3;
+  const int sc = blockIdx.z / nkk;
+  const int k = 2 + blockIdx.z % nkk;
+  if (i > g.ni - 2 || j > g.nj - 2 || sc >= nscalar) return;
+  float f = adv_s(g, o, c, in.s[sc], in.rstxu, in.rstxv, in.rstxwc, i, j, k);
+  const int vsp = sc == 0 ? o.vsp_pt : (sc == 1 ? o.vsp_qv : o.vsp_q);
+  if (vsp && k >= o.ksp0 - 1 && k <= g.nk - 2) {
+    const float r = 0.5f * (rbct2(in.rbct, g, i, j, k) + rbct2(in.rbct, g, i, j, k + 1));
+    const float d = sc == 1 ? AT(in.sp[sc], i, j, k) - AT(in.qvbr, i, j, k) : AT(in.sp[sc], i, j, k);
+    f = f - r * AT(in.rst, i, j, k) * d;
+  }
+  if (sc == 0) {
+    const float dziv25 = 0.25f * c.dziv;
+    auto a8 = [&](int kk) {
+      return (AT(in.rbr, i, j, kk - 1) + AT(in.rbr, i, j, kk)) * AT(in.w, i, j, kk) *
+             (AT(in.ptbr, i, j, kk - 1) - AT(in.ptbr, i, j, kk)) * dziv25;
+    };
+    f = f + (a8(k) + a8(k + 1));
+  }
+  AT(in.frc[sc], i, j, k) = f;
+}
+
+// ---------------------------------------------------------------------------
+// Acoustic substep (s_hevi, hevi.f90:339-427).
+// ---------------------------------------------------------------------------
+
+// pgrad.f90:260-306 with diver3d.f90:182-326: the divergence-damped pressure
+// ptil = pp + divcv*jcb*div3d over i=1..ni-1, j=1..nj-1, k=1..nk-1.
+__global__ void k_ac_ptil(Grid g, Opts o, Consts c, float divcv, const float *jcb, const float *rst8u,
+                          const float *rst8v, const float *rst8w, const float *uf, const float *vf,
+                          const float *wc, const float *ppf, float *ptil) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.ni - 1 || j > g.nj - 1) return;
+  const long p = g.at(i, j, k);
+  if (o.divopt == 0) {
+    ptil[p] = ppf[p];
+    return;
+  }
+  const float s1a = rst8u[p] * uf[p], s1b = AT(rst8u, i + 1, j, k) * AT(uf, i + 1, j, k);
+  const float s2a = rst8v[p] * vf[p], s2b = AT(rst8v, i, j + 1, k) * AT(vf, i, j + 1, k);
+  const float s3a = rst8w[p] * wc[p], s3b = AT(rst8w, i, j, k + 1) * AT(wc, i, j, k + 1);
+  const float div = (s3a - s3b) * c.dziv + ((s1a - s1b) * c.dxiv + (s2a - s2b) * c.dyiv);
+  ptil[p] = ppf[p] + divcv * jcb[p] * div;
+}
+
+// s_stepuv, stepuv.f90:288-310: uf += dts*(ufrc+upg)/rst8u with
+// upg = (jcb*ptil)(i-1) - (jcb*ptil)(i) over dx (pgrad.f90:357-395).
+__global__ void k_ac_uv(Grid g, Consts c, float dts, const float *jcb, const float *ptil,
+                        const float *ufrc, const float *vfrc, const float *rst8u, const float *rst8v,
+                        float *uf, float *vf) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 2 + blockIdx.z;
+  if (i > g.ni - 1 || j > g.nj - 1) return;
+  const long p = g.at(i, j, k);
+  const float pc = jcb[p] * ptil[p];
+  if (j <= g.nj - 2) {
+    const float upg = (AT(jcb, i - 1, j, k) * AT(ptil, i - 1, j, k) - pc) * c.dxiv;
+    uf[p] = uf[p] + dts * (ufrc[p] + upg) / rst8u[p];
+  }
+  if (i <= g.ni - 2) {
+    const float vpg = (AT(jcb, i, j - 1, k) * AT(ptil, i, j - 1, k) - pc) * c.dyiv;
+    vf[p] = vf[p] + dts * (vfrc[p] + vpg) / rst8v[p];
+  }
+}
+
+// The vertical part of the substep for one interior column (i,j):
+// diverpih (diverpih.f90:188-214), diverpiv, advbspi 'back', pgradiv
+// (pgradiv.f90:143-170), buoywsi (buoywsi.f90:165-211), stepwi
+// (stepwi.f90:319-427) with the Thomas solve of gaussel (gaussel.f90:169-262),
+// phy2cnt+vbcwc+vbcw for this column, diverpiv + advbspi 'fore', steppi
+// (steppi.f90:211-223) and vbcp (vbcp.f90:126-162).  One lane per column;
+// the forward-sweep coefficients and fp live in LDS ([k][lane]).
+__global__ void __launch_bounds__(64) k_ac_col(Grid g, Opts o, Consts c, AcCol a) {
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x;
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y;
+  if (i > g.ni - 2 || j > g.nj - 2) return;
+  const int nk = g.nk;
+  float *sfp = lds;                       // fp(k), k = 1..nk
+  float *sss = lds + (long)(nk + 1) * 64; // normalised super-diagonal ss'(k)
+#define SFP(k) sfp[(k) * 64 + lane]
+#define SSS(k) sss[(k) * 64 + lane]
+  const float dts = a.dts;
+  const float dtdzw = dts * c.dziv * c.weicoe;
+  const float dtw = dts * c.weicoe;
+  const float sbsqzi = dts * dts * c.weicoe * c.weicoe * c.dziv;
+  const float sbsqg5 = c.g05 * dts * dts * c.weicoe * c.weicoe;
+
+  // pass 1 (upward, k=2..nk-2): fp, fw, explicit w, tridiagonal forward sweep
+  float fp_km1 = 0.0f, fpdvj_km1 = 0.0f, wb_km1 = 0.0f;
+  float fwc_km1 = 0.0f, wcc_km1 = 0.0f;
+  float ssn_km1 = 0.0f, ff_km1 = 0.0f;
+  float wf_k = AT(a.wf, i, j, 2);
+  for (int k = 2; k <= nk - 2; k++) {
+    const long p = g.at(i, j, k);
+    const float wf_kp1 = a.wf[p + g.nxy];
+    const float rcsq = a.rcsq[p], rst = a.rst[p], jcb = a.jcb[p], rbr = a.rbr[p];
+    const float ju0 = a.jcb8u[p] * a.uf[p], ju1 = a.jcb8u[p + 1] * a.uf[p + 1];
+    const float jv0 = a.jcb8v[p] * a.vf[p], jv1 = a.jcb8v[p + g.nx] * a.vf[p + g.nx];
+    const float hdiv = rcsq * ((ju0 - ju1) * c.dxiv + (jv0 - jv1) * c.dyiv);
+    const float vdiv = rcsq * (wf_k - wf_kp1) * c.dziv;
+    const float fp = a.pfrc[p] + hdiv + c.weic1m * (c.g05 * (wf_k + wf_kp1) * rst + vdiv);
+    SFP(k) = fp;
+    const float fpdvj = fp / jcb;
+    const float wb = (a.ppf[p] * rst + fp * rbr * dtw) / rcsq * c.g05n;
+    const float t1 = c.g05 * rbr / rcsq;
+    const float t2 = c.dziv / jcb;
+    const float fwc = t1 + t2, wcc = t1 - t2;
+    if (k >= 3) {
+      const float wpg = (a.ptil[p - g.nxy] - a.ptil[p]) * c.dziv;
+      float fw = a.wfrc[p] + wpg + (fpdvj_km1 - fpdvj) * dtdzw;
+      if (o.buyopt == 1) fw = fw + (wb_km1 + wb);
+      const float rst8w = a.rst8w[p];
+      float wfn = wf_k + dts * fw / rst8w;
+      float rr, ss, tt;
+      if (o.buyopt == 0) {
+        const float aa = sbsqzi / rst8w;
+        const float mm = aa * a.rcsq[p - g.nxy];
+        const float nn = aa * rcsq;
+        rr = -mm * fwc_km1;
+        ss = 1.0f + (nn * fwc - mm * wcc_km1);
+        tt = nn * wcc;
+      } else {
+        const float rstiv = 1.0f / rst8w;
+        const float aa = rstiv * sbsqzi;
+        const float bb = rstiv * sbsqg5;
+        const float mm = bb * a.rst[p - g.nxy] - aa * a.rcsq[p - g.nxy];
+        const float nn = bb * rst + aa * rcsq;
+        rr = mm * fwc_km1;
+        ss = 1.0f + (mm * wcc_km1 + nn * fwc);
+        tt = nn * wcc;
+      }
+      if (k == 3) wfn = wfn + 0.25f * rr * (0.0f + 0.0f);   // bbcw on flat terrain
+      // Thomas forward sweep (gaussel.f90:189-236), kstr=3, kend=nk-2
+      float ssn, ffn;
+      if (k == 3) {
+        const float q = 1.0f / ss;
+        ssn = tt * q;
+        ffn = wfn * q;
+      } else {
+        const float q = 1.0f / (ss - rr * ssn_km1);
+        ssn = tt * q;
+        ffn = (wfn - rr * ff_km1) * q;
+      }
+      SSS(k) = ssn;
+      a.wf[p] = ffn;
+      ssn_km1 = ssn;
+      ff_km1 = ffn;
+    }
+    fp_km1 = fp;
+    fpdvj_km1 = fpdvj;
+    wb_km1 = wb;
+    fwc_km1 = fwc;
+    wcc_km1 = wcc;
+    wf_k = wf_kp1;
+  }
+  (void)fp_km1;
+  // pass 2 (downward): back substitution (gaussel.f90:248-260)
+  float wnext = ff_km1;  // wf(nk-2)
+  for (int k = nk - 3; k >= 3; k--) {
+    const long p = g.at(i, j, k);
+    const float wv = a.wf[p] - SSS(k) * wnext;
+    a.wf[p] = wv;
+    wnext = wv;
+  }
+  // phy2cnt (k=2..nk-1) + vbcwc, then vbcw for this column
+  for (int k = 2; k <= nk - 1; k++) AT(a.wc, i, j, k) = wc_value(g, o, a.wf, a.jcb8w, i, j, k);
+  AT(a.wc, i, j, 1) = wc_value(g, o, a.wf, a.jcb8w, i, j, 1);
+  AT(a.wc, i, j, nk) = wc_value(g, o, a.wf, a.jcb8w, i, j, nk);
+  if (o.bbc == 2) {
+    const float z = 0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
+    AT(a.wf, i, j, 1) = -AT(a.jcb8w, i, j, 3) * AT(a.wc, i, j, 3) - z;
+    AT(a.wf, i, j, 2) = -0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
+  } else {
+    AT(a.wf, i, j, 1) = AT(a.wf, i, j, 2);
+  }
+  if (o.tbc == 2) {
+    const float z = 0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
+    AT(a.wf, i, j, nk) = -AT(a.jcb8w, i, j, nk - 2) * AT(a.wc, i, j, nk - 2) - z;
+    AT(a.wf, i, j, nk - 1) = 0.0f;
+  } else {
+    AT(a.wf, i, j, nk) = AT(a.wf, i, j, nk - 1);
+  }
+  // pass 3 (upward): diverpiv + advbspi 'fore', steppi
+  float wk = AT(a.wf, i, j, 2);
+  for (int k = 2; k <= nk - 2; k++) {
+    const long p = g.at(i, j, k);
+    const float wkp1 = a.wf[p + g.nxy];
+    const float vdiv = a.rcsq[p] * (wk - wkp1) * c.dziv;
+    const float fp = SFP(k) + c.weicoe * (c.g05 * (wk + wkp1) * a.rst[p] + vdiv);
+    a.ppf[p] = a.ppf[p] + dts * fp / a.jcb[p];
+    wk = wkp1;
+  }
+  // vbcp for this column
+  if (o.bbc == 2) AT(a.ppf, i, j, 1) = 2.0f * AT(a.ppf, i, j, 2) - AT(a.ppf, i, j, 3);
+  else if (o.bbc == 3) AT(a.ppf, i, j, 1) = AT(a.ppf, i, j, 2);
+  AT(a.ppf, i, j, nk - 1) = AT(a.ppf, i, j, nk - 2);
+#undef SFP
+#undef SSS
+}
+
+// Ring columns (i or j on the first halo row, within 1..ni-1 x 1..nj-1)
+// after the lateral fill of wf and ppf: phy2cnt+vbcwc, vbcw and vbcp exactly
+// as the reference applies them over i=1..ni-1, j=1..nj-1 (stepwi.f90:540-549,
+// steppi.f90:316).
+__global__ void k_ac_ring(Grid g, Opts o, const float *jcb8w, float *wf, float *wc, float *ppf) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nxr = g.ni - 1, nyr = g.nj - 1;   // ring of the (1..ni-1)x(1..nj-1) box
+  const int nring = 2 * nxr + 2 * (nyr - 2);
+  if (t >= nring) return;
+  int i, j;
+  if (t < nxr) { i = 1 + t; j = 1; }
+  else if (t < 2 * nxr) { i = 1 + (t - nxr); j = g.nj - 1; }
+  else if (t < 2 * nxr + (nyr - 2)) { i = 1; j = 2 + (t - 2 * nxr); }
+  else { i = g.ni - 1; j = 2 + (t - 2 * nxr - (nyr - 2)); }
+  const int nk = g.nk;
+  for (int k = 2; k <= nk - 1; k++) AT(wc, i, j, k) = wc_raw(g, o.sthopt, wf, jcb8w, i, j, k);
+  if (o.bbc == 2) {
+    AT(wc, i, j, 1) = -AT(wc, i, j, 3);
+    AT(wc, i, j, 2) = 0.0f;
+  } else if (o.bbc == 3) {
+    AT(wc, i, j, 1) = AT(wc, i, j, 2);
+  }
+  if (o.tbc == 2) {
+    AT(wc, i, j, nk) = -AT(wc, i, j, nk - 2);
+    AT(wc, i, j, nk - 1) = 0.0f;
+  } else {
+    AT(wc, i, j, nk) = AT(wc, i, j, nk - 1);
+  }
+  if (o.bbc == 2) {
+    const float z = 0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
+    AT(wf, i, j, 1) = -AT(jcb8w, i, j, 3) * AT(wc, i, j, 3) - z;
+    AT(wf, i, j, 2) = -0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
+  } else {
+    AT(wf, i, j, 1) = AT(wf, i, j, 2);
+  }
+  if (o.tbc == 2) {
+    const float z = 0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
+    AT(wf, i, j, nk) = -AT(jcb8w, i, j, nk - 2) * AT(wc, i, j, nk - 2) - z;
+    AT(wf, i, j, nk - 1) = 0.0f;
+  } else {
+    AT(wf, i, j, nk) = AT(wf, i, j, nk - 1);
+  }
+  if (o.bbc == 2) AT(ppf, i, j, 1) = 2.0f * AT(ppf, i, j, 2) - AT(ppf, i, j, 3);
+  else if (o.bbc == 3) AT(ppf, i, j, 1) = AT(ppf, i, j, 2);
+  AT(ppf, i, j, nk - 1) = AT(ppf, i, j, nk - 2);
+}
+
+// ---------------------------------------------------------------------------
+// Scalar leapfrog, s_steps (steps.f90:520-760): s_f = max(0, s_p + frc*dtdrst)
+// for qv..qg, without the clamp for theta' and nci; dtdrst = 2*dtb/rst.
+// The q and nci forcings are read from the future arrays themselves.
+// ---------------------------------------------------------------------------
+__global__ void k_steps(Grid g, float dtb2, const float *rst, StepsIn s, int withq) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 2 + blockIdx.z;
+  if (i > g.ni - 2 || j > g.nj - 2) return;
+  const long p = g.at(i, j, k);
+  const float d = dtb2 / rst[p];
+  s.ptpf[p] = s.ptpp[p] + s.ptfrc[p] * d;
+  const float a = s.qp[p] + s.qvfrc[p] * d;
+  s.qf[p] = a > 0.0f ? a : 0.0f;
+  if (!withq) return;
+  for (int sp = 1; sp < 6; sp++) {
+    const long q = p + sp * s.qn;
+    const float b = s.qp[q] + s.qf[q] * d;
+    s.qf[q] = b > 0.0f ? b : 0.0f;
+  }
+  s.ncif[p] = s.ncip[p] + s.ncif[p] * d;
+}
+
+// ---------------------------------------------------------------------------
+// Halo fills.
+// s_bcycle (bcycle.f90:159-225): x pass over j=0..nj+1 then y pass over
+// i=0..ni+1, for up to 16 fields per launch.
+// ---------------------------------------------------------------------------
+__global__ void k_bcycle_x(Grid g, FillSet fs) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = 1 + blockIdx.y;
+  const int f = blockIdx.z;
+  if (j > g.nj + 1 || f >= fs.n) return;
+  const FillSpec &s = fs.f[f];
+  if (k > s.kmax) return;
+  float *v = s.ptr;
+  AT(v, s.iwrcv, j, k) = AT(v, s.iesnd, j, k);
+  AT(v, s.iercv, j, k) = AT(v, s.iwsnd, j, k);
+}
+
+__global__ void k_bcycle_y(Grid g, FillSet fs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = 1 + blockIdx.y;
+  const int f = blockIdx.z;
+  if (i > g.ni + 1 || f >= fs.n) return;
+  const FillSpec &s = fs.f[f];
+  if (k > s.kmax) return;
+  float *v = s.ptr;
+  AT(v, i, s.jsrcv, k) = AT(v, i, s.jnsnd, k);
+  AT(v, i, s.jnrcv, k) = AT(v, i, s.jssnd, k);
+}
+
+// Vertical fills: vbcs (vbcs.f90:116-134), vbcu, vbcv (same with i to ni or
+// j to nj), vbcp (vbcp.f90:126-162); kind 0 s, 1 u, 2 v, 3 p.
+__global__ void k_vbc(Grid g, int bbc, VbcSet vs) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int f = blockIdx.z;
+  if (f >= vs.n) return;
+  const int kind = vs.kind[f];
+  const int ihi = kind == 1 ? g.ni : g.ni - 1;
+  const int jhi = kind == 2 ? g.nj : g.nj - 1;
+  if (i > ihi || j > jhi) return;
+  float *v = vs.ptr[f];
+  const int nk = g.nk;
+  if (kind == 3) {
+    if (bbc == 2) AT(v, i, j, 1) = 2.0f * AT(v, i, j, 2) - AT(v, i, j, 3);
+    else if (bbc == 3) AT(v, i, j, 1) = AT(v, i, j, 2);
+  } else {
+    AT(v, i, j, 1) = AT(v, i, j, 2);
+  }
+  AT(v, i, j, nk - 1) = AT(v, i, j, nk - 2);
+}
+
+// ---------------------------------------------------------------------------
+// Asselin filter, s_timeflt (timeflt.f90:335-560): x = m1fc2*x +
+// filcoe*(xf + xp) over each field's own box (u: i=1..ni, j=1..nj-1,
+// k=1..nk-1; v: j to nj; w: k to nk; scalars i=1..ni-1, j=1..nj-1).
+// ---------------------------------------------------------------------------
+__global__ void k_timeflt(Grid g, Consts c, AsselinSet as) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int f = blockIdx.z / g.nk;
+  const int k = 1 + blockIdx.z % g.nk;
+  if (f >= as.n) return;
+  const int kind = as.kind[f];
+  const int ihi = kind == 1 ? g.ni : g.ni - 1;
+  const int jhi = kind == 2 ? g.nj : g.nj - 1;
+  const int khi = kind == 3 ? g.nk : g.nk - 1;
+  if (i > ihi || j > jhi || k > khi) return;
+  const long p = g.at(i, j, k);
+  float *x = as.x[f];
+  x[p] = c.m1fc2 * x[p] + c.filcoe * (as.xf[f][p] + as.xp[f][p]);
+}
+
+}  // namespace cress
+
+namespace cress {
+
+// Stand-alone Thomas solve over a box of columns for the s_gaussel shim
+// (gaussel.f90:169-262, impopt=1), one lane per column.
+__global__ void k_gaussel(Grid g, int istr, int iend, int jstr, int jend, int kstr, int kend,
+                          const float *rr, float *ss, const float *tt, float *ff) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nci = iend - istr + 1;
+  const int ncol = nci * (jend - jstr + 1);
+  if (t >= ncol) return;
+  const int i = istr + t % nci, j = jstr + t / nci;
+  const int kem1 = kend - 1;
+  if (kstr == kend) {
+    AT(ff, i, j, kstr) = AT(ff, i, j, kstr) / AT(ss, i, j, kstr);
+  } else {
+    const float aa = 1.0f / AT(ss, i, j, kstr);
+    AT(ss, i, j, kstr) = AT(tt, i, j, kstr) * aa;
+    AT(ff, i, j, kstr) = AT(ff, i, j, kstr) * aa;
+  }
+  if (kstr + 2 <= kend) {
+    for (int k = kstr + 1; k <= kend - 1; k++) {
+      const float aa = 1.0f / (AT(ss, i, j, k) - AT(rr, i, j, k) * AT(ss, i, j, k - 1));
+      AT(ss, i, j, k) = AT(tt, i, j, k) * aa;
+      AT(ff, i, j, k) = (AT(ff, i, j, k) - AT(rr, i, j, k) * AT(ff, i, j, k - 1)) * aa;
+    }
+  }
+  if (kstr + 1 <= kend) {
+    const float aa = 1.0f / (AT(ss, i, j, kend) - AT(rr, i, j, kend) * AT(ss, i, j, kem1));
+    AT(ff, i, j, kend) = (AT(ff, i, j, kend) - AT(rr, i, j, kend) * AT(ff, i, j, kem1)) * aa;
+    for (int k = kend - 1; k >= kstr; k--) AT(ff, i, j, k) = AT(ff, i, j, k) - AT(ss, i, j, k) * AT(ff, i, j, k + 1);
+  }
+}
+
+// advs alone (advs.f90:205-445) for the s_advs shim: sfrc over the interior.
+__global__ void k_advs_only(Grid g, Opts o, Consts c, ScalarIn in) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 2 + blockIdx.z;
+  if (i > g.ni - 2 || j > g.nj - 2) return;
+  AT(in.frc[0], i, j, k) = adv_s(g, o, c, in.s[0], in.rstxu, in.rstxv, in.rstxwc, i, j, k);
+}
+
+}  // namespace cress

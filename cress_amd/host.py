@@ -1,0 +1,171 @@
+"""Host-side mirror of the reference's step interface over the C ABI.
+
+`Solver` plays the role of the reference's time-loop body
+(slvdrv.f90:975-1452 -> s_intgdrv): it owns one device-resident context
+(include/cress_hip.h, tier 1), takes the base state and the three time levels
+once, and advances whole large steps on the GPU.  Arrays are the reference's
+real(0:ni+1,0:nj+1,1:nk) blocks in Fortran order (numpy order="F").
+
+The library is loaded from cress_amd/_lib/libcresship.so; there is no CPU
+fallback.  Importing this module on a machine without that library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libcresship.so")
+
+I = C.c_int
+D = C.c_double
+
+# field ids (include/cress_hip.h)
+BASE_FIELDS = ["jcb", "jcb8u", "jcb8v", "jcb8w", "ubr", "vbr", "pbr", "ptbr", "qvbr", "rbr",
+               "rst", "rst8u", "rst8v", "rst8w", "rcsq"]
+FID = {name: i for i, name in enumerate(BASE_FIELDS)}
+PROG_FIELDS = ["u", "v", "w", "pp", "ptp", "qv", "qc", "qr", "qi", "qs", "qg", "nci"]
+FID.update({name: 32 + i for i, name in enumerate(PROG_FIELDS)})
+FID.update({"fc": 64, "rbct": 65, "prwtr": 66, "price": 67})
+
+ERR = {0: "ok", -1: "invalid argument", -2: "no usable GPU", -3: "HIP runtime error",
+       -4: "device out of memory", -5: "field id or level out of range",
+       -6: "option outside the implemented plain path"}
+
+
+class CressError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__("%s: %s (%d)" % (what, ERR.get(code, "error"), code))
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [("ni", I), ("nj", I), ("nk", I),
+                ("wbc", I), ("ebc", I), ("sbc", I), ("nbc", I), ("bbc", I), ("tbc", I),
+                ("advopt", I), ("impopt", I), ("gwmopt", I), ("buyopt", I), ("divopt", I),
+                ("cphopt", I), ("haiopt", I), ("vspopt", I), ("coropt", I), ("sthopt", I),
+                ("ksp0", I * 2), ("vspvar", C.c_char * 10), ("nsstp", I),
+                ("dx", D), ("dy", D), ("dz", D), ("dtbig", D), ("dtsml", D),
+                ("weicoe", D), ("filcoe", D), ("thresq", D)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded product library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("cress_amd: %s not built (run __graft_entry__.build())" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.cress_ctx_create.argtypes = [C.POINTER(Config), C.POINTER(P)]
+        L.cress_ctx_destroy.argtypes = [P]
+        L.cress_upload.argtypes = [P, I, I, P]
+        L.cress_download.argtypes = [P, I, I, P]
+        L.cress_field_size.argtypes = [P, I, C.POINTER(C.c_size_t)]
+        L.cress_field_device_ptr.argtypes = [P, I, I, C.POINTER(P)]
+        L.cress_large_step.argtypes = [P, I, I]
+        L.cress_sync.argtypes = [P]
+        L.cress_set_timing.argtypes = [P, I]
+        L.cress_stage_times.argtypes = [P, C.POINTER(D), I]
+        L.cress_device_count.argtypes = [C.POINTER(I)]
+        L.cress_error_string.restype = C.c_char_p
+        L.cress_gaussel.argtypes = [I] * 9 + [P] * 4
+        L.cress_advs.argtypes = [C.POINTER(Config)] + [P] * 5
+        L.cress_bcycle.argtypes = [C.POINTER(Config)] + [I] * 9 + [P]
+        _lib = L
+    return _lib
+
+
+def check(rc, what):
+    if rc != 0:
+        raise CressError(rc, what)
+
+
+def make_config(ni, nj, nk, dx, dy, dz, dtbig, dtsml, nsstp=None, advopt=3, divopt=1, buyopt=1,
+                cphopt=2, vspopt=2, coropt=0, sthopt=1, bbc=2, tbc=2, ksp0=(2, 2),
+                vspvar=b"oooxoooooo", weicoe=0.6, filcoe=0.1, thresq=1e-12):
+    c = Config()
+    c.ni, c.nj, c.nk = ni, nj, nk
+    c.wbc = c.ebc = c.sbc = c.nbc = -1
+    c.bbc, c.tbc = bbc, tbc
+    c.advopt, c.impopt, c.gwmopt, c.buyopt, c.divopt = advopt, 1, 0, buyopt, divopt
+    c.cphopt, c.haiopt, c.vspopt, c.coropt, c.sthopt = cphopt, 0, vspopt, coropt, sthopt
+    c.ksp0[0], c.ksp0[1] = ksp0
+    c.vspvar = vspvar
+    if nsstp is None:
+        # slvstep.f90:176: nsstp = 20*int(100*(dtbig+.001))/int(1000*(dtsml+.0001))
+        nsstp = 20 * int(1e2 * (dtbig + 0.001)) // int(1e3 * (dtsml + 0.0001))
+    c.nsstp = nsstp
+    c.dx, c.dy, c.dz, c.dtbig, c.dtsml = dx, dy, dz, dtbig, dtsml
+    c.weicoe, c.filcoe, c.thresq = weicoe, filcoe, thresq
+    return c
+
+
+class Solver:
+    """One device-resident CReSS model on cuda:0 (HIP device 0)."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.L = lib()
+        self.ctx = C.c_void_p()
+        check(self.L.cress_ctx_create(C.byref(cfg), C.byref(self.ctx)), "cress_ctx_create")
+        self.shape = (cfg.ni + 2, cfg.nj + 2, cfg.nk)
+
+    def close(self):
+        if self.ctx:
+            self.L.cress_ctx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def size(self, name):
+        n = C.c_size_t()
+        check(self.L.cress_field_size(self.ctx, FID[name], C.byref(n)), "cress_field_size")
+        return n.value
+
+    def upload(self, name, arr, level=0):
+        a = np.asarray(arr, dtype=np.float32)
+        a = np.asfortranarray(a)
+        if a.size != self.size(name):
+            raise ValueError("%s: expected %d floats, got %d" % (name, self.size(name), a.size))
+        check(self.L.cress_upload(self.ctx, FID[name], level, a.ctypes.data), "upload " + name)
+
+    def download(self, name, level=0):
+        out = np.empty(self.size(name), dtype=np.float32)
+        check(self.L.cress_download(self.ctx, FID[name], level, out.ctypes.data), "download " + name)
+        if name in FID and FID[name] < 64:
+            return out.reshape(self.shape, order="F")
+        return out
+
+    def device_ptr(self, name, level=0):
+        p = C.c_void_p()
+        check(self.L.cress_field_device_ptr(self.ctx, FID[name], level, C.byref(p)), "device_ptr")
+        return p.value
+
+    def step(self, nsteps=1, first=False):
+        check(self.L.cress_large_step(self.ctx, nsteps, 1 if first else 0), "cress_large_step")
+
+    def sync(self):
+        check(self.L.cress_sync(self.ctx), "cress_sync")
+
+    def set_timing(self, on=True):
+        check(self.L.cress_set_timing(self.ctx, 1 if on else 0), "cress_set_timing")
+
+    def stage_times(self):
+        buf = (D * 8)()
+        check(self.L.cress_stage_times(self.ctx, buf, 8), "cress_stage_times")
+        return dict(zip(["force", "acoustic", "steps", "micro", "timeflt", "satadjst", "halo2",
+                         "other"], list(buf)))
+
+
+def device_count():
+    n = I(0)
+    lib().cress_device_count(C.byref(n))
+    return n.value

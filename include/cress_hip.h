@@ -1,0 +1,142 @@
+/* cress_hip.h -- C ABI of the MI355X (gfx950) implementation of one CReSS
+ * large time step.
+ *
+ * Two tiers:
+ *
+ *  1. Device-resident solver context (the performance path).  The host
+ *     uploads the base state and the three time levels once, then calls
+ *     cress_large_step() for as many steps as it likes; nothing crosses PCIe
+ *     between steps.  This replaces the reference's time-loop body
+ *       s_intgdrv   (SrcOrig/intgdrv.f90:111-1133, called from slvdrv.f90:1306)
+ *     with fdmdrv/forcedrv/smalldrv/hevi/steps/cloudphy/timeflt/satadjst/
+ *     shift2nd/swp2nxt inside it, for the plain-path options.
+ *
+ *  2. Per-routine drop-in shims with the reference's argument lists and
+ *     explicit-shape host arrays real(0:ni+1,0:nj+1,1:nk) (column-major,
+ *     x fastest).  Each copies its arrays to the device, runs the same HIP
+ *     kernel the context uses, and copies the outputs back.
+ *
+ * Conventions: every function returns 0 on success or a negative CRESS_E*
+ * code; cress_error_string() names it.  Option values are passed already
+ * resolved (the caller reads the namelist; the library never does).  One
+ * host thread per context; each context owns one HIP stream.  All arrays are
+ * IEEE binary32 unless stated.
+ */
+#ifndef CRESS_HIP_H
+#define CRESS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CRESS_OK = 0,
+  CRESS_EINVAL = -1,      /* bad argument / unsupported option combination */
+  CRESS_ENODEV = -2,      /* no usable GPU */
+  CRESS_EHIP = -3,        /* HIP runtime error */
+  CRESS_ENOMEM = -4,      /* device allocation failed */
+  CRESS_ERANGE = -5,      /* field id / level out of range */
+  CRESS_EUNSUPPORTED = -6 /* option outside the implemented plain path */
+};
+
+/* Resolved run options.  Names and meanings follow the reference namelist
+ * (Doc/0rig/readme_user.conf.txt); dims follow setdim.f90:196-199
+ * (ni = nx+3, nj = ny+3, nk = nz+3). */
+typedef struct cress_config {
+  int ni, nj, nk;
+  int wbc, ebc, sbc, nbc;   /* lateral BCs after the one-rank remap: -1 periodic
+                               (chknlslv.f90:751-790) */
+  int bbc, tbc;             /* bottom/top: 2 rigid, 3 zero gradient */
+  int advopt;               /* 1,2: 2nd/4th order horizontal; 3: 4th order 3-D */
+  int impopt;               /* 1: implicit vertical (Thomas, gaussel) */
+  int gwmopt;               /* 0 */
+  int buyopt;               /* 0/1: buoyancy in the implicit solve */
+  int divopt;               /* 0/1: divergence damping */
+  int cphopt;               /* 0 dry-ish (no microphysics), 2 cold-rain bulk */
+  int haiopt;               /* 0 */
+  int vspopt;               /* 0 off, 2 Rayleigh sponge to the base state */
+  int coropt;               /* 0/1 Coriolis on u,v */
+  int sthopt;               /* 0 uniform dz, >=1 stretched (wc = w/jcb8w) */
+  int ksp0[2];              /* lowest sponge level (vspdmp.f90:167-232) */
+  char vspvar[10];          /* 'o'/'x' for u,v,w,p,pt,qv,q,... */
+  int nsstp;                /* acoustic substeps per large step (slvstep.f90:176) */
+  double dx, dy, dz;        /* m; dz is the computational (zeta) spacing */
+  double dtbig, dtsml;      /* s */
+  double weicoe, filcoe, thresq;
+} cress_config;
+
+/* Field ids for upload/download.  Base state and metrics have one level. */
+enum {
+  /* base state / metrics: level 0 only */
+  CRESS_JCB = 0, CRESS_JCB8U, CRESS_JCB8V, CRESS_JCB8W,
+  CRESS_UBR, CRESS_VBR, CRESS_PBR, CRESS_PTBR, CRESS_QVBR, CRESS_RBR,
+  CRESS_RST, CRESS_RST8U, CRESS_RST8V, CRESS_RST8W, CRESS_RCSQ,
+  CRESS_NBASE,
+  /* prognostics: level 0 past, 1 present, 2 future */
+  CRESS_U = 32, CRESS_V, CRESS_W, CRESS_PP, CRESS_PTP,
+  CRESS_QV, CRESS_QC, CRESS_QR, CRESS_QI, CRESS_QS, CRESS_QG, CRESS_NCI,
+  CRESS_NPROG_END,
+  /* 2-D / special layouts */
+  CRESS_FC = 64,            /* (0:ni+1,0:nj+1,1:2) Coriolis, component 1 used */
+  CRESS_RBCT = 65,          /* (1:ni,1:nj,1:nk,1:2) sponge coefficients */
+  CRESS_PRWTR = 66,         /* (0:ni+1,0:nj+1,1:2,1:2) rain/cloud precip rate, accum */
+  CRESS_PRICE = 67          /* (0:ni+1,0:nj+1,1:2,1:3) ice/snow/graupel precip */
+};
+
+typedef struct cress_ctx cress_ctx;
+
+const char *cress_error_string(int code);
+int cress_device_count(int *n);
+
+/* ---- Tier 1: device-resident context --------------------------------- */
+int cress_ctx_create(const cress_config *cfg, cress_ctx **out);
+int cress_ctx_destroy(cress_ctx *ctx);
+/* Number of floats of one field (its layout as documented above). */
+int cress_field_size(const cress_ctx *ctx, int field, size_t *nfloat);
+int cress_upload(cress_ctx *ctx, int field, int level, const float *host);
+int cress_download(cress_ctx *ctx, int field, int level, float *host);
+/* Device address of a field (for callers that fill device memory
+ * themselves); valid until the next cress_large_step(). */
+int cress_field_device_ptr(cress_ctx *ctx, int field, int level, void **dptr);
+/* Advance nsteps large steps.  first != 0 makes the first of them the
+ * reference's ctime=0 step: forward step, dtb=dtbig/2, dts=dtsml/2, no
+ * Asselin filter (timeint.f90:160-163, intgdrv.f90:968). */
+int cress_large_step(cress_ctx *ctx, int nsteps, int first);
+int cress_sync(cress_ctx *ctx);
+/* Device time of the last cress_large_step call's kernels, per stage, in ms
+ * (measured with HIP events on the context's stream when timing is on). */
+int cress_set_timing(cress_ctx *ctx, int on);
+int cress_stage_times(cress_ctx *ctx, double *ms, int nstage);
+/* Per-kernel-family device time (HIP events around every launch while
+ * timing is on), accumulated since the last cress_reset_timing(). */
+int cress_kernel_count(void);
+const char *cress_kernel_name(int id);
+int cress_kernel_times(cress_ctx *ctx, double *total_ms, long *launches, int n);
+int cress_reset_timing(cress_ctx *ctx);
+
+/* ---- Tier 2: per-routine drop-in shims (host arrays) -------------------
+ * Each mirrors the reference subroutine named in its comment: same array
+ * arguments in the same order, options taken from cfg instead of fp ids. */
+
+/* s_gaussel(fpimpopt,istr,iend,jstr,jend,kstr,kend,ni,nj,kmax,rr,ss,tt,ff,pv)
+ * SrcOrig/gaussel.f90:72-73, impopt=1 path.  ss and ff are overwritten. */
+int cress_gaussel(int istr, int iend, int jstr, int jend, int kstr, int kend,
+                  int ni, int nj, int nk, float *rr, float *ss, float *tt, float *ff);
+
+/* s_advs(fpadvopt,...,ni,nj,nk,rstxu,rstxv,rstxwc,s,sfrc,vadv,tmp1,tmp2,tmp3)
+ * SrcOrig/advs.f90:75-77.  Writes sfrc over the interior. */
+int cress_advs(const cress_config *cfg, const float *rstxu, const float *rstxv,
+               const float *rstxwc, const float *s, float *sfrc);
+
+/* s_bcycle(fpwbc,...,iwsnd,iwrcv,iesnd,iercv,jssnd,jsrcv,jnsnd,jnrcv,ni,nj,kmax,var)
+ * SrcOrig/bcycle.f90:72-74: the one-rank periodic halo fill. */
+int cress_bcycle(const cress_config *cfg, int iwsnd, int iwrcv, int iesnd, int iercv,
+                 int jssnd, int jsrcv, int jnsnd, int jnrcv, int kmax, float *var);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1117,7 +1117,8 @@ static void or_forcedrv(const or_cfg *c, or_state *s, R *qall, R *ufrc, R *vfrc,
   or_advs(c, rstxu, rstxv, rstxwc, s->q[C], qvfrc, t[0], t[1], t[2], t[3]);
   if (c->vspopt >= 1 && c->vspvar[5] == 'o')
     or_vsps(c, s->rst, s->q[P], s->qvbr, s->rbct, qvfrc);
-  /* forceq: qc, qr, qi, qs, qg then nci */
+  /* forceq (abs(cphopt)>=1, forceq.f90:384-388): qc, qr, qi, qs, qg, nci */
+  if (c->cphopt == 0) return;
   for (int sp = QC; sp <= QG; sp++) {
     or_advs(c, rstxu, rstxv, rstxwc, s->q[C] + sp * n, s->q[F] + sp * n, t[0], t[1], t[2], t[3]);
     if (c->vspopt >= 1 && c->vspvar[6] == 'o')
@@ -1177,18 +1178,20 @@ static void or_steps(const or_cfg *c, or_state *s, R dtb, const R *ptfrc, const 
         s->ptp[F][p] = s->ptp[P][p] + ptfrc[p] * d;
         R a = s->q[P][p] + qvfrc[p] * d;
         s->q[F][p] = a > K(0.0) ? a : K(0.0);
+        if (c->cphopt == 0) continue;
         for (int sp = QC; sp <= QG; sp++) {
           R b = s->q[P][p + sp * n] + s->q[F][p + sp * n] * d;
           s->q[F][p + sp * n] = b > K(0.0) ? b : K(0.0);
         }
         s->nci[F][p] = s->nci[P][p] + s->nci[F][p] * d;
       }
+  const int nq = c->cphopt == 0 ? 1 : NQ;
   or_bcycle(c, 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk, s->ptp[F]);
-  for (int sp = 0; sp < NQ; sp++) or_bcycle(c, 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk, s->q[F] + sp * n);
-  or_bcycle(c, 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk, s->nci[F]);
+  for (int sp = 0; sp < nq; sp++) or_bcycle(c, 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk, s->q[F] + sp * n);
+  if (c->cphopt) or_bcycle(c, 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk, s->nci[F]);
   or_vbcs(c, s->ptp[F]);
-  for (int sp = 0; sp < NQ; sp++) or_vbcs(c, s->q[F] + sp * n);
-  or_vbcs(c, s->nci[F]);
+  for (int sp = 0; sp < nq; sp++) or_vbcs(c, s->q[F] + sp * n);
+  if (c->cphopt) or_vbcs(c, s->nci[F]);
 }
 
 /* s_timeflt, timeflt.f90:335-560: Asselin filter on the present level. */
@@ -1213,9 +1216,10 @@ void or_timeflt(const or_cfg *c, or_state *s)
   or_asselin(c, s->w[1], s->w[0], s->w[2], 1, ni - 1, nj - 1, nk);
   or_asselin(c, s->pp[1], s->pp[0], s->pp[2], 1, ni - 1, nj - 1, nk - 1);
   or_asselin(c, s->ptp[1], s->ptp[0], s->ptp[2], 1, ni - 1, nj - 1, nk - 1);
-  for (int sp = 0; sp < NQ; sp++)
+  const int nq = c->cphopt == 0 ? 1 : NQ;
+  for (int sp = 0; sp < nq; sp++)
     or_asselin(c, s->q[1] + sp * n, s->q[0] + sp * n, s->q[2] + sp * n, 1, ni - 1, nj - 1, nk - 1);
-  or_asselin(c, s->nci[1], s->nci[0], s->nci[2], 1, ni - 1, nj - 1, nk - 1);
+  if (c->cphopt) or_asselin(c, s->nci[1], s->nci[0], s->nci[2], 1, ni - 1, nj - 1, nk - 1);
 }
 
 /* s_shift2nd at one rank (shift2nd.f90:4274-4302): second halo row for the
@@ -1229,8 +1233,9 @@ void or_shift2nd(const or_cfg *c, or_state *s, int lv)
   or_bcycle(c, 3, 0, ni - 3, ni, 3, 0, nj - 3, nj, nk, s->w[lv]);
   or_bcycle(c, 3, 0, ni - 3, ni, 3, 0, nj - 3, nj, nk, s->pp[lv]);
   or_bcycle(c, 3, 0, ni - 3, ni, 3, 0, nj - 3, nj, nk, s->ptp[lv]);
-  for (int sp = 0; sp < NQ; sp++) or_bcycle(c, 3, 0, ni - 3, ni, 3, 0, nj - 3, nj, nk, s->q[lv] + sp * n);
-  or_bcycle(c, 3, 0, ni - 3, ni, 3, 0, nj - 3, nj, nk, s->nci[lv]);
+  const int nq = c->cphopt == 0 ? 1 : NQ;
+  for (int sp = 0; sp < nq; sp++) or_bcycle(c, 3, 0, ni - 3, ni, 3, 0, nj - 3, nj, nk, s->q[lv] + sp * n);
+  if (c->cphopt) or_bcycle(c, 3, 0, ni - 3, ni, 3, 0, nj - 3, nj, nk, s->nci[lv]);
 }
 
 /* s_swp2nxt, swp2nxt.f90:354-...: past <- present <- future over the
@@ -1258,9 +1263,10 @@ void or_swp2nxt(const or_cfg *c, or_state *s)
   or_rot(c, s->w[0], s->w[1], s->w[2], ni - ie, nj - jn, nk);
   or_rot(c, s->pp[0], s->pp[1], s->pp[2], ni - ie, nj - jn, nk - 1);
   or_rot(c, s->ptp[0], s->ptp[1], s->ptp[2], ni - ie, nj - jn, nk - 1);
-  for (int sp = 0; sp < NQ; sp++)
+  const int nq = c->cphopt == 0 ? 1 : NQ;
+  for (int sp = 0; sp < nq; sp++)
     or_rot(c, s->q[0] + sp * n, s->q[1] + sp * n, s->q[2] + sp * n, ni - ie, nj - jn, nk - 1);
-  or_rot(c, s->nci[0], s->nci[1], s->nci[2], ni - ie, nj - jn, nk - 1);
+  if (c->cphopt) or_rot(c, s->nci[0], s->nci[1], s->nci[2], ni - ie, nj - jn, nk - 1);
 }
 
 /* Configurations this restatement covers (the plain-path options). */
@@ -1295,8 +1301,10 @@ int or_large_step(const or_cfg *c, or_state *s, int first, double dtbig, double 
   R *qall = s->scr + 7 * n, *qallp = s->scr + 8 * n, *dtdrst = s->scr + 9 * n;
   const R dtb = first ? K(0.5) * (R)dtbig : (R)dtbig;
   const R dts = first ? K(0.5) * (R)dtsml : (R)dtsml;
-  or_totalqwi(c, s->q[1] + n, s->q[1] + 3 * n, qall);
-  or_totalqwi(c, s->q[0] + n, s->q[0] + 3 * n, qallp);
+  if (c->cphopt) {
+    or_totalqwi(c, s->q[1] + n, s->q[1] + 3 * n, qall);
+    or_totalqwi(c, s->q[0] + n, s->q[0] + 3 * n, qallp);
+  }
   or_forcedrv(c, s, qall, ufrc, vfrc, wfrc, pfrc, ptfrc, qvfrc, wc);
   or_smalldrv(c, s, nsstp, dts, ufrc, vfrc, wfrc, pfrc, wc);
   or_steps(c, s, dtb, ptfrc, qvfrc, dtdrst);

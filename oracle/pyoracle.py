@@ -116,3 +116,78 @@ def field(cfg, dtype, n=1):
     """Zeroed Fortran-ordered real(0:ni+1,0:nj+1,1:nk[,n]) block."""
     shape = (cfg.ni + 2, cfg.nj + 2, cfg.nk) + ((n,) if n > 1 else ())
     return np.zeros(shape, dtype=dtype, order="F")
+
+
+BASE = ["jcb", "jcb8u", "jcb8v", "jcb8w", "ubr", "vbr", "pbr", "ptbr", "qvbr", "rbr", "rst",
+        "rst8u", "rst8v", "rst8w", "rcsq"]
+PROG = ["u", "v", "w", "pp", "ptp", "q", "nci"]
+QNAMES = ["qv", "qc", "qr", "qi", "qs", "qg"]
+NSCR = 24
+
+
+class State(C.Structure):
+    _fields_ = ([(b, P) for b in BASE] + [("fc", P), ("rbct", P)] +
+                [(p, P * 3) for p in PROG] + [("prwtr", P), ("price", P), ("scr", P)])
+
+
+def cfg_from_solver_config(sc):
+    """Oracle Cfg equivalent of a cress_amd.host.Config."""
+    c = make_cfg(sc.ni, sc.nj, sc.nk, dx=sc.dx, dy=sc.dy, dz=sc.dz, advopt=sc.advopt,
+                 divopt=sc.divopt, buyopt=sc.buyopt, cphopt=sc.cphopt, vspopt=sc.vspopt,
+                 coropt=sc.coropt, sthopt=sc.sthopt, bbc=sc.bbc, tbc=sc.tbc,
+                 ksp0=(sc.ksp0[0], sc.ksp0[1]), vspvar=bytes(sc.vspvar), weicoe=sc.weicoe,
+                 filcoe=sc.filcoe, thresq=sc.thresq)
+    return c
+
+
+class Model:
+    """The oracle's large-step driver (or_large_step) on numpy arrays."""
+
+    def __init__(self, case, prec=32):
+        self.o = Oracle(prec)
+        self.dt = self.o.dtype
+        sc = case.cfg
+        self.sc = sc
+        self.cfg = cfg_from_solver_config(sc)
+        shp = (sc.ni + 2, sc.nj + 2, sc.nk)
+        self.shape = shp
+        n = shp[0] * shp[1] * shp[2]
+        self.n = n
+        f = lambda *extra: np.zeros(shp + extra, dtype=self.dt, order="F")
+        self.base = {k: np.asfortranarray(case.base[k].astype(self.dt)) for k in BASE}
+        self.fc = np.asfortranarray(case.fc.astype(self.dt))
+        self.rbct = np.asfortranarray(case.rbct.astype(self.dt))
+        self.lv = {p: [f() if p != "q" else f(6) for _ in range(3)] for p in PROG}
+        for lv in (0, 1):
+            for k, a in case.init.items():
+                if k in QNAMES:
+                    self.lv["q"][lv][..., QNAMES.index(k)] = a
+                else:
+                    self.lv[k][lv][...] = a
+        self.prwtr = np.zeros((shp[0], shp[1], 2, 2), self.dt, order="F")
+        self.price = np.zeros((shp[0], shp[1], 2, 3), self.dt, order="F")
+        self.scr = np.zeros(NSCR * n, self.dt)
+        st = State()
+        for k in BASE:
+            setattr(st, k, self.base[k].ctypes.data)
+        st.fc, st.rbct = self.fc.ctypes.data, self.rbct.ctypes.data
+        for p in PROG:
+            arr = getattr(st, p)
+            for l in range(3):
+                arr[l] = self.lv[p][l].ctypes.data
+        st.prwtr, st.price, st.scr = self.prwtr.ctypes.data, self.price.ctypes.data, self.scr.ctypes.data
+        self.state = st
+
+    def step(self, nsteps=1, first=False):
+        for s in range(nsteps):
+            rc = self.o.lib.or_large_step(C.byref(self.cfg), C.byref(self.state),
+                                          1 if (first and s == 0) else 0,
+                                          C.c_double(self.sc.dtbig), C.c_double(self.sc.dtsml),
+                                          self.sc.nsstp)
+            if rc != 0:
+                raise RuntimeError("or_large_step failed: %d" % rc)
+
+    def get(self, name, level=1):
+        if name in QNAMES:
+            return self.lv["q"][level][..., QNAMES.index(name)]
+        return self.lv[name][level]
