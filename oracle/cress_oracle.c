@@ -1284,9 +1284,11 @@ int or_cfg_check(const or_cfg *c)
   return 0;
 }
 
+#include "cress_micro.c"
+
 typedef void (*or_micro_fn)(const or_cfg *, or_state *, R dtb, R *qallp);
-static or_micro_fn g_micro = 0;
-static or_micro_fn g_satadj = 0;
+static or_micro_fn g_micro = micro_hook;
+static or_micro_fn g_satadj = satadj_hook;
 
 /* One large step: s_intgdrv (intgdrv.f90:832-1133) minus I/O, with the
  * time bookkeeping of s_timeint (timeint.f90:160-190): the first step

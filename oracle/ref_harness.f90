@@ -43,6 +43,17 @@
       use m_vbcv
       use m_vbcwc
       use m_bbcw
+      use m_setref
+      use m_getvdens
+      use m_getexner
+      use m_diagnw
+      use m_diagnsg
+      use m_adjstnci
+      use m_termblk
+      use m_srcblk
+      use m_upwqp
+      use m_upwnp
+      use m_satadjst
 
       implicit none
 
@@ -398,5 +409,104 @@
       real qwtr(*),qice(*),qall(*)
       call s_totalqwi(idcphopt,idhaiopt,ni,nj,nk,2,3,qwtr,qice,qall)
       end subroutine ref_totalqwi
+
+
+      subroutine ref_setref() bind(C,name='ref_setref')
+      call s_setref
+      end subroutine ref_setref
+
+      subroutine ref_micro_prologue(ni,nj,nk,rbr,pbr,pp,qw,qi,rbv,pi,p,  &
+     &   nw,nic) bind(C,name='ref_micro_prologue')
+      integer(c_int), value :: ni,nj,nk
+      real rbr(*),pbr(*),pp(*),qw(*),qi(*),rbv(*),pi(*),p(*),nw(*),nic(*)
+      call s_getvdens(ni,nj,nk,rbr,rbv)
+      call s_getexner(ni,nj,nk,pbr,pp,pi,p)
+      call s_diagnw(ni,nj,nk,2,2,rbr,qw,nw)
+      call s_diagnsg(idhaiopt,ni,nj,nk,3,3,rbr,rbv,qi,nic)
+      end subroutine ref_micro_prologue
+
+      subroutine ref_adjstnci(ni,nj,nk,qi,nci) bind(C,name='ref_adjstnci')
+      integer(c_int), value :: ni,nj,nk
+      real qi(*),nci(*)
+      call s_adjstnci(ni,nj,nk,qi,nci)
+      end subroutine ref_adjstnci
+
+! termblk: q(1:6) = qc,qr,qi,qs,qg,(qh=qg); nc(1:6) likewise; u(1:12) out
+      subroutine ref_termblk(ni,nj,nk,rbv,qc,qr,qi,qs,qg,ncc,ncr,nci,    &
+     &   ncs,ncg,ucq,urq,uiq,usq,ugq,ucn,urn,uin,usn,ugn,dmy)             &
+     &   bind(C,name='ref_termblk')
+      integer(c_int), value :: ni,nj,nk
+      real rbv(*),qc(*),qr(*),qi(*),qs(*),qg(*),ncc(*),ncr(*),nci(*)
+      real ncs(*),ncg(*),ucq(*),urq(*),uiq(*),usq(*),ugq(*),ucn(*),urn(*)
+      real uin(*),usn(*),ugn(*),dmy(*)
+      call s_termblk(idcphopt,idhaiopt,idthresq,ni,nj,nk,rbv,qc,qr,qi,   &
+     &               qs,qg,qg,ncc,ncr,nci,ncs,ncg,ncg,ucq,urq,uiq,usq,   &
+     &               ugq,dmy,ucn,urn,uin,usn,ugn,dmy)
+      end subroutine ref_termblk
+
+! srcblk: inp = 22 fields (ptbr,rbr,rbv,pi,p,ptpp,qvp,qcp,qrp,qip,qsp,qgp,
+! nccp,ncrp,ncip,ncsp,ncgp,urq,usq,ugq, dummy, dummy),
+! fut = 8 fields (ptpf,qvf,qcf,qrf,qif,qsf,qgf,ncif), wrk = 80 fields.
+      subroutine ref_srcblk(ni,nj,nk,dtb,thresq,inp,fut,wrk)              &
+     &   bind(C,name='ref_srcblk')
+      integer(c_int), value :: ni,nj,nk
+      real, value :: dtb,thresq
+      real inp(*),fut(*),wrk(*)
+      integer n
+      n=(ni+2)*(nj+2)*nk
+      call s_srcblk(2,0,0,dtb,thresq,ni,nj,nk,                            &
+     &  inp(1),inp(1+n),inp(1+2*n),inp(1+3*n),inp(1+4*n),inp(1+5*n),       &
+     &  inp(1+6*n),inp(1+7*n),inp(1+8*n),inp(1+9*n),inp(1+10*n),           &
+     &  inp(1+11*n),inp(1+11*n),                                           &
+     &  inp(1+12*n),inp(1+13*n),inp(1+14*n),inp(1+15*n),inp(1+16*n),       &
+     &  inp(1+16*n),inp(1+20*n),                                           &
+     &  inp(1+17*n),inp(1+18*n),inp(1+19*n),inp(1+19*n),                   &
+     &  inp(1+21*n),inp(1+21*n),inp(1+21*n),inp(1+21*n),                   &
+     &  fut(1),fut(1+n),fut(1+2*n),fut(1+3*n),fut(1+4*n),fut(1+5*n),       &
+     &  fut(1+6*n),fut(1+6*n),wrk(1+70*n),wrk(1+70*n),fut(1+7*n),          &
+     &  wrk(1+71*n),wrk(1+72*n),wrk(1+73*n),                               &
+     &  wrk(1+74*n),wrk(1+74*n),wrk(1+74*n),wrk(1+74*n),wrk(1+74*n),       &
+     &  wrk(1+74*n),                                                       &
+     &  wrk(1),wrk(1+n),wrk(1+2*n),wrk(1+3*n),wrk(1+4*n),wrk(1+5*n),       &
+     &  wrk(1+6*n),wrk(1+7*n),wrk(1+8*n),wrk(1+9*n),wrk(1+10*n),           &
+     &  wrk(1+11*n),wrk(1+12*n),wrk(1+13*n),wrk(1+14*n),wrk(1+15*n),       &
+     &  wrk(1+16*n),wrk(1+17*n),wrk(1+18*n),wrk(1+19*n),wrk(1+20*n),       &
+     &  wrk(1+21*n),wrk(1+22*n),wrk(1+23*n),wrk(1+24*n),wrk(1+25*n),       &
+     &  wrk(1+26*n),wrk(1+27*n),wrk(1+28*n),wrk(1+29*n),wrk(1+30*n),       &
+     &  wrk(1+31*n),wrk(1+32*n),wrk(1+33*n),wrk(1+34*n),wrk(1+35*n),       &
+     &  wrk(1+36*n),wrk(1+37*n),wrk(1+38*n),wrk(1+39*n),wrk(1+40*n),       &
+     &  wrk(1+41*n),wrk(1+42*n),wrk(1+43*n),wrk(1+44*n),wrk(1+45*n),       &
+     &  wrk(1+46*n),wrk(1+47*n),wrk(1+48*n),wrk(1+49*n),wrk(1+50*n),       &
+     &  wrk(1+51*n),wrk(1+52*n),wrk(1+53*n),wrk(1+54*n),wrk(1+55*n),       &
+     &  wrk(1+56*n),wrk(1+57*n),wrk(1+58*n),wrk(1+59*n),wrk(1+60*n))
+      end subroutine ref_srcblk
+
+      subroutine ref_upwqp(dtp,ni,nj,nk,rbr,rst,uq,qpf,precip,flx)        &
+     &   bind(C,name='ref_upwqp')
+      real, value :: dtp
+      integer(c_int), value :: ni,nj,nk
+      real rbr(*),rst(*),uq(*),qpf(*),precip(*),flx(*)
+      call s_upwqp(idadvopt,iddziv,dtp,ni,nj,nk,rbr,rst,uq,qpf,precip,   &
+     &             flx)
+      end subroutine ref_upwqp
+
+      subroutine ref_upwnp(dtp,ni,nj,nk,rbr,rst,un,ncf,flx)               &
+     &   bind(C,name='ref_upwnp')
+      real, value :: dtp
+      integer(c_int), value :: ni,nj,nk
+      real rbr(*),rst(*),un(*),ncf(*),flx(*)
+      call s_upwnp(iddziv,dtp,ni,nj,nk,rbr,rst,un,ncf,flx)
+      end subroutine ref_upwnp
+
+! satadjst: q(1:6) = qv,qc,qr,qi,qs,qg stacked; nice(1:3); nwtr(1:2)
+      subroutine ref_satadjst(ni,nj,nk,pbr,ptbr,w,pp,ptp,q,nw,nic,pi,p)  &
+     &   bind(C,name='ref_satadjst')
+      integer(c_int), value :: ni,nj,nk
+      real pbr(*),ptbr(*),w(*),pp(*),ptp(*),q(*),nw(*),nic(*),pi(*),p(*)
+      integer n
+      n=(ni+2)*(nj+2)*nk
+      call s_satadjst(idcphopt,ni,nj,nk,2,2,3,3,pbr,ptbr,w,pp,ptp,q(1),  &
+     &                q(1+n),nw,q(1+3*n),nic,pi,p)
+      end subroutine ref_satadjst
 
       end module cress_ref_harness
