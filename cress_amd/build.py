@@ -19,6 +19,9 @@ LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libcresship.so")
 SOURCES = ["dyn.hip", "micro.hip", "capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FSRC = os.path.join(HERE, "fortran")
+FHOST = os.path.join(LIBDIR, "cress_run")
 ARCH = os.environ.get("CRESS_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
@@ -57,7 +60,29 @@ def build(force=True, jobs=4):
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    build_fortran_host()
     return LIB
+
+
+def build_fortran_host():
+    """The Fortran 90 host (fortran/cress_run.f90 over the ISO_C_BINDING
+    module fortran/m_cress_hip.f90), linked against libcresship.so with an
+    rpath to its own directory.  Skipped with a note when no flang exists."""
+    if not os.path.exists(FLANG):
+        sys.stderr.write("cress_amd: %s not found, Fortran host not built\n" % FLANG)
+        return None
+    moddir = os.path.join(LIBDIR, "mod")
+    os.makedirs(moddir, exist_ok=True)
+    objs = []
+    for f in ("m_cress_hip.f90", "cress_run.f90"):
+        obj = os.path.join(moddir, f.replace(".f90", ".o"))
+        _run([FLANG, "-O2", "-ffp-contract=off", "-module-dir", moddir, "-I", moddir, "-c",
+              os.path.join(FSRC, f), "-o", obj])
+        objs.append(obj)
+    tmp = FHOST + ".tmp"
+    _run([FLANG, "-o", tmp] + objs + ["-L" + LIBDIR, "-lcresship", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, FHOST)
+    return FHOST
 
 
 if __name__ == "__main__":

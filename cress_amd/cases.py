@@ -7,13 +7,13 @@ the device in float32.  It is not on the timed path.
 
 Cases (names follow the workload each one is):
   bubble    -- moist warm bubble, Weisman-Klemp sounding, no wind, 64x64x40,
-               dx=dy=1 km, uniform dz=500 m, dtbig=5 s (C1)
+               dx=dy=1 km, uniform dz=500 m, dtbig=5 s
   supercell -- WK sounding + quarter-circle hodograph, 256x256x64, dx=500 m,
-               stretched dz 100 m -> 18 km top, dtbig=3 s, 6 substeps (C2; run
+               stretched dz 100 m -> 18 km top, dtbig=3 s, 6 substeps (run
                here with periodic sides, the only lateral BC implemented)
   vortex    -- 1024x1024x80 tropical sounding with a warm-core vortex on an
                f-plane (f=5e-5), stretched dz 50 m -> 20 km, dtbig=4 s,
-               8 substeps (C3, the bandwidth-stress workload)
+               8 substeps (the bandwidth-stress workload)
 Every case takes nx, ny, nz overrides so that the same physics can be run at
 oracle-sized grids.
 """
@@ -156,6 +156,51 @@ class Case:
         self.cfg, self.base, self.fc, self.rbct, self.init = cfg, base, fc, rbct, init
         self.zph, self.zs, self.name = zph, zs, name
 
+    def write_state(self, path):
+        """The base state, fc, rbct and the past and present levels in the
+        stream format the Fortran host (cress_amd/fortran/cress_run.f90)
+        reads: 8 int32 header ('CRSS', 1, ni, nj, nk, ksp0(2), nrec), then
+        per record int32 field id, level, nfloat and the float32 data."""
+        c = self.cfg
+        recs = [(host.FID[k], 0, v) for k, v in self.base.items()]
+        recs += [(host.FID["fc"], 0, self.fc), (host.FID["rbct"], 0, self.rbct)]
+        for lv in (0, 1):
+            recs += [(host.FID[k], lv, v) for k, v in self.init.items()]
+        with open(path, "wb") as f:
+            hdr = [0x53535243, 1, c.ni, c.nj, c.nk, c.ksp0[0], c.ksp0[1], len(recs)]
+            f.write(np.asarray(hdr, np.int32).tobytes())
+            for fid, lv, v in recs:
+                a = np.asarray(v, np.float32).ravel(order="F")
+                f.write(np.asarray([fid, lv, a.size], np.int32).tobytes())
+                f.write(a.tobytes())
+
+    def write_userconf(self, path, nsteps):
+        """A user.conf with the reference's namelist groups for this case and
+        etime = nsteps * dtbig."""
+        c = self.cfg
+        vsp = bytes(c.vspvar).decode() if not isinstance(c.vspvar, str) else c.vspvar
+        lines = [
+            "&dimset numpe=1, xgroup=1, ygroup=1, xsub=1, ysub=1, xdim=%d, ydim=%d, zdim=%d /"
+            % (c.ni, c.nj, c.nk),
+            "&gridset dx=%r, dy=%r, dz=%r /" % (c.dx, c.dy, c.dz),
+            "&gridsth sthopt=%d /" % c.sthopt,
+            "&terrain trnopt=0 /",
+            "&flength stime=0.0, etime=%r /" % (nsteps * c.dtbig),
+            "&boundry wbc=1, ebc=1, sbc=1, nbc=1, bbc=%d, tbc=%d /" % (c.bbc, c.tbc),
+            "&gpvpram vspopt=%d, vspvar='%s' /" % (c.vspopt, vsp[:10]),
+            "&integrat dtbig=%r, dtsml=%r, gwmopt=%d, impopt=%d, advopt=%d, weicoe=%r, filcoe=%r /"
+            % (c.dtbig, c.dtsml, c.gwmopt, c.impopt, c.advopt, c.weicoe, c.filcoe),
+            "&smoother smtopt=0 /",
+            "&mapfcter mfcopt=0 /",
+            "&coriolis coropt=%d /" % c.coropt,
+            "&buoyancy buyopt=%d /" % c.buyopt,
+            "&ddamping divopt=%d /" % c.divopt,
+            "&cloudphy cphopt=%d, haiopt=%d, thresq=%r /" % (c.cphopt, c.haiopt, c.thresq),
+            "&turbulen tubopt=0 /",
+        ]
+        with open(path, "w") as f:
+            f.write("\n".join(lines) + "\n")
+
     def load(self, solver):
         for k, v in self.base.items():
             solver.upload(k, v)
@@ -171,9 +216,56 @@ def _periodic_index(n_int, nfull, offset=2):
     return offset + np.mod(np.arange(nfull) - offset, n_int)
 
 
+def _hydrometeors(X, Y, xc, yc, zs, tk, p, qv, ip, jp, seed, ring):
+    """Seed every cold-rain species so that all of the microphysics' branches
+    are visited: a cos^2 blob (or, for the vortex, a Gaussian eyewall ring)
+    with per-cell noise; cloud water above -40 C, rain above -10 C,
+    ice below -5 C, snow below +2 C, graupel below +7 C, and 1% water
+    supersaturation where the weight exceeds one half.  The ring radius is
+    100 km or 0.6 of the half-width of smaller domains, its e-folding width
+    0.3 of the radius."""
+    if ring:
+        # the ring radius and width shrink with small test domains
+        r = np.sqrt((X - xc) ** 2 + (Y - yc) ** 2)
+        r0 = min(100000.0, 0.6 * min(xc, yc))
+        h = np.exp(-((r - r0) / (0.3 * r0)) ** 2)
+    else:
+        rr = np.sqrt(((X - xc) / (0.35 * max(2 * xc, 1.0))) ** 2 + ((Y - yc) / (0.35 * max(2 * yc, 1.0))) ** 2)
+        h = np.where(rr < 1.0, np.cos(0.5 * CC * np.minimum(rr, 1.0)) ** 2, 0.0)
+    nk = len(zs)
+    I, J = np.meshgrid(ip - 2, jp - 2, indexing="ij")
+    out = {s: np.zeros(X.shape + (nk,)) for s in ("qc", "qr", "qi", "qs", "qg", "nci")}
+    qvn = np.broadcast_to(qv[None, None, :], X.shape + (nk,)).copy()
+    for k in range(nk):
+        if zs[k] > 14000.0:
+            continue
+        lin = (I + 4096 * (J + 4096 * k)).astype(np.uint64)
+        un = splitmix_uniform(seed + 17, lin.ravel()).reshape(I.shape)
+        b = h * (0.25 + 0.75 * un)
+        t = tk[k]
+        if t > 233.16:
+            out["qc"][:, :, k] = 1.0e-3 * b
+        if t > 263.16:
+            out["qr"][:, :, k] = 2.0e-3 * b
+        if t < 268.16:
+            out["qi"][:, :, k] = 1.0e-4 * b
+            out["nci"][:, :, k] = 1.0e-4 * b / 5.0e-12
+        if t < 275.16:
+            out["qs"][:, :, k] = 5.0e-4 * b
+        if t < 280.16:
+            out["qg"][:, :, k] = 8.0e-4 * b
+        qvs = tetens_qvs(t, p[k])
+        qvn[:, :, k] = np.where(b > 0.5, 1.01 * qvs, qvn[:, :, k])
+    out["qv"] = qvn
+    return out
+
+
 def build(name="bubble", nx=None, ny=None, nz=None, seed=None, cphopt=2, dtbig=None,
-          dtsml=None, nsstp=None, advopt=3):
-    """Return a Case with config, base state and the initial state."""
+          dtsml=None, nsstp=None, advopt=3, moist=None):
+    """Return a Case with config, base state and the initial state.
+
+    moist (default: cphopt == 2) seeds the hydrometeor species and a
+    supersaturated core so that the microphysics is active from step one."""
     if name == "bubble":
         nx, ny, nz = nx or 64, ny or 64, nz or 40
         dx = dy = 1000.0
@@ -341,6 +433,13 @@ def build(name="bubble", nx=None, ny=None, nz=None, seed=None, cphopt=2, dtbig=N
     if cphopt == 2:
         for s in ("qc", "qr", "qi", "qs", "qg", "nci"):
             init[s] = np.zeros(shape)
+        if moist is None or moist:
+            tk = th * (p / P0) ** (RD / CP)
+            init.update(_hydrometeors(X, Y, xc, yc, zs, tk, p, qv, ip, jp, seed, name == "vortex"))
+            for s in ("qv", "qc", "qr", "qi", "qs", "qg", "nci"):
+                a = init[s]
+                a[:, :, 0] = a[:, :, 1]
+                a[:, :, nk - 2] = a[:, :, nk - 3]
     init = {k: np.asfortranarray(a.astype(np.float32)) for k, a in init.items()}
     cfg = host.make_config(ni, nj, nk, dx, dy, dz, dtbig, dtsml, nsstp=nsstp, advopt=advopt,
                            cphopt=cphopt, vspopt=2, coropt=coropt, sthopt=sthopt,

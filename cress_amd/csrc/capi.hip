@@ -13,6 +13,8 @@
 #include "dyn.h"
 #include "micro.h"
 
+#include <cstring>
+
 using namespace cress;
 
 #define HIPCHK(x)                                  \
@@ -30,13 +32,13 @@ enum { NSTAGE = 8 };
 // context stream around every launch).
 enum {
   KT_PREP, KT_FORCE_UV, KT_FORCE_W, KT_FORCE_P, KT_FORCE_S, KT_PHY2CNT, KT_AC_PTIL, KT_AC_UV,
-  KT_AC_COL, KT_AC_RING, KT_FILL, KT_VBC, KT_STEPS, KT_TIMEFLT, KT_COPY, KT_MICRO, KT_SATADJ,
-  KT_N
+  KT_AC_COL, KT_AC_RING, KT_FILL, KT_VBC, KT_STEPS, KT_TIMEFLT, KT_COPY, KT_MICRO_SEED, KT_MICRO_SRC,
+  KT_MICRO_FALL, KT_SATADJ, KT_N
 };
 static const char *kKernelNames[KT_N] = {
     "k_prep", "k_force_uv", "k_force_w", "k_force_p", "k_force_s", "k_phy2cnt", "k_ac_ptil",
     "k_ac_uv", "k_ac_col", "k_ac_ring", "k_bcycle", "k_vbc", "k_steps", "k_timeflt", "memcpy_d2d",
-    "micro", "satadjst"};
+    "k_micro_seed", "k_micro_src", "k_micro_fall", "k_satadjst"};
 enum { MAXEV = 1024 };
 
 struct cress_ctx {
@@ -208,6 +210,8 @@ int cress_ctx_create(const cress_config *cfg, cress_ctx **out) {
   if (cfg->cphopt == 2) {
     for (int f = 0; f < MicroScratch::NF; f++) AL(x->ms.f[f], n);
     AL(x->ms.col, 64);
+    rc = micro_init_tables();
+    if (rc) goto fail;
   }
 #undef AL
   for (int s = 0; s <= NSTAGE; s++) hipEventCreate(&x->ev[s]);
@@ -580,9 +584,47 @@ static int large_step(cress_ctx *x, int first) {
 
   // ---- microphysics (cloudphy) on the future level ----
   if (cf.cphopt == 2) {
-    int rc = micro_cloudphy(x->g, x->c, x->o, st, dtb, first, B, x->ptp[P], x->q[P], x->nci[P], x->pp[P],
-                            x->ptp[F], x->q[F], x->nci[F], x->w[F], x->prwtr, x->price, &x->ms);
-    if (rc) return rc;
+    // leapfrog: the microphysics step is 2*dtb (cloudphy.f90:281-285)
+    const float dtbs = 2.0f * dtb;
+    const MicroK mk = micro_fold(dtbs, cf.thresq, (float)cf.dz);
+    MicroArgs ma;
+    ma.jcb = B[CRESS_JCB];
+    ma.pbr = B[CRESS_PBR];
+    ma.ptbr = B[CRESS_PTBR];
+    ma.rbr = B[CRESS_RBR];
+    ma.ppp = x->pp[P];
+    ma.ptpp = x->ptp[P];
+    ma.qp = x->q[P];
+    ma.ncip = x->nci[P];
+    ma.ptpf = x->ptp[F];
+    ma.qf = x->q[F];
+    ma.ncif = x->nci[F];
+    ma.urq = x->ms.f[0];
+    ma.usq = x->ms.f[1];
+    ma.ugq = x->ms.f[2];
+    ma.dtpmin = (unsigned *)x->ms.col;
+    ma.qn = n;
+    unsigned seed;
+    memcpy(&seed, &dtbs, sizeof seed);
+    LAUNCH(x, KT_MICRO_SEED, k_micro_seed, dim3(1), dim3(64), 0, ma.dtpmin, seed);
+    LAUNCH(x, KT_MICRO_SRC, k_micro_src, grid2(ni - 1, nj - 1, 64, 4, nk - 1), b2, 0, g, mk, ma);
+    FallArgs fa;
+    fa.rbr = B[CRESS_RBR];
+    fa.rst = B[CRESS_RST];
+    fa.qp = x->q[P];
+    fa.urq = ma.urq;
+    fa.usq = ma.usq;
+    fa.ugq = ma.ugq;
+    fa.qf = x->q[F];
+    fa.ncif = x->nci[F];
+    fa.prwtr = x->prwtr;
+    fa.price = x->price;
+    fa.dtpmin = ma.dtpmin;
+    fa.qn = n;
+    fa.dtb = dtbs;
+    fa.dziv = c.dziv;
+    fa.thresq = c.thresq;
+    LAUNCH(x, KT_MICRO_FALL, k_micro_fall, grid2(ni - 1, nj - 1, 64, 4, 1), b2, 0, g, fa);
   }
   stamp(x, 4);
 
@@ -609,9 +651,11 @@ static int large_step(cress_ctx *x, int first) {
 
   // ---- saturation adjustment (satadjst) on present and future ----
   if (cf.cphopt == 2) {
-    int rc = micro_satadjst(x->g, x->c, x->o, st, B, x->pp[C], x->ptp[C], x->q[C], x->nci[C], &x->ms);
-    if (!rc) rc = micro_satadjst(x->g, x->c, x->o, st, B, x->pp[F], x->ptp[F], x->q[F], x->nci[F], &x->ms);
-    if (rc) return rc;
+    for (int lv = 0; lv < 2; lv++) {
+      const int S = lv == 0 ? C : F;
+      LAUNCH(x, KT_SATADJ, k_satadjst, grid2(ni - 1, nj - 1, 64, 4, nk - 1), b2, 0, g, c.thresq, B[CRESS_PBR],
+             B[CRESS_PTBR], x->pp[S], x->ptp[S], x->q[S], x->nci[S], n);
+    }
   }
   stamp(x, 6);
 

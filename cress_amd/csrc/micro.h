@@ -1,21 +1,55 @@
-// Bulk cold-rain microphysics (cphopt=2) entry points: s_cloudphy
-// (cloudphy.f90:277-470) and s_satadjst (satadjst.f90:148-189).
+// Bulk cold-rain microphysics (cphopt=2, haiopt=0): the kernels of
+// s_cloudphy (cloudphy.f90:277-470 -> coldblk.f90:321-481) and s_satadjst
+// (satadjst.f90:148-189).  The context's step driver launches them.
 #pragma once
 #include "common.h"
 
 namespace cress {
 
 struct MicroScratch {
-  enum { NF = 16 };
-  float *f[NF];   // 3-D work fields
-  float *col;     // small reduction buffer (fall-speed CFL minimum)
+  enum { NF = 3 };
+  float *f[NF];   // fall speeds of rain, snow, graupel (termblk) for the sedimentation kernel
+  float *col;     // one word: binary32 bits of the sedimentation CFL minimum
 };
 
-int micro_cloudphy(const Grid &g, const Consts &c, const Opts &o, hipStream_t st, float dtb, int first,
-                   float *const *base, const float *ptpp, const float *qp, const float *ncip, const float *ppp,
-                   float *ptpf, float *qf, float *ncif, const float *wf, float *prwtr, float *price,
-                   MicroScratch *ms);
-int micro_satadjst(const Grid &g, const Consts &c, const Opts &o, hipStream_t st, float *const *base,
-                   const float *pp, float *ptp, float *q, float *nci, MicroScratch *ms);
+// Per-call constants, folded on the host in binary32 in the reference's
+// operation order (each routine's local parameters, dtb = microphysics step).
+struct MicroK {
+  float dtb, thresq, dz;
+  float cnu, cdv, pdiaqr, pdiaqs, pdiaqg, cvntr, cvnts, cvntg;    // setblk.f90:280-330
+  float curq, cusq, cugq;                                         // termblk.f90:280-300
+  float rwdt2, cc45, cknd, cdar;                                  // nuc1stc.f90:172-200
+  float cclcr, cclcs, cclcg, cclri, cclrs, cclrg, cclis, cclig,   // collect.f90:297-321
+      cclsr, cclsg, cclsg3;
+  float cc2dtn, cc2dt, cagin_ag, ccdtb4, qccm, diaqs0, cagin_cv,  // prodctwg, melting, aggregat,
+      ccncr, ccnsg, cfrrg, mi0352, mi0353;                        // depsit, convers, freezing, nuc2nd
+};
+
+struct MicroArgs {
+  const float *jcb, *pbr, *ptbr, *rbr;
+  const float *ppp, *ptpp, *qp, *ncip;  // past level (qp: six species stacked)
+  float *ptpf, *qf, *ncif;              // future level
+  float *urq, *usq, *ugq;               // fall speeds handed to k_micro_fall
+  unsigned *dtpmin;                     // binary32 bits of the CFL minimum
+  long qn;                              // floats per species block
+};
+
+struct FallArgs {
+  const float *rbr, *rst, *qp, *urq, *usq, *ugq;
+  float *qf, *ncif, *prwtr, *price;
+  const unsigned *dtpmin;
+  long qn;
+  float dtb, dziv, thresq;
+};
+
+// Koenig table into constant memory (once per process and device set).
+int micro_init_tables();
+MicroK micro_fold(float dtb_micro, float thresq, float dz);
+
+__global__ void k_micro_seed(unsigned *p, unsigned v);
+__global__ void k_micro_src(Grid g, MicroK K, MicroArgs a);
+__global__ void k_micro_fall(Grid g, FallArgs a);
+__global__ void k_satadjst(Grid g, float thresq, const float *pbr, const float *ptbr, const float *pp, float *ptp,
+                           float *q, float *nci, long qn);
 
 }  // namespace cress

@@ -1,17 +1,709 @@
-// Bulk cold-rain microphysics for gfx950 (cphopt=2): see micro.h.
+// Cold-rain bulk microphysics (cphopt=2, haiopt=0) for gfx950.
+//
+// s_cloudphy -> s_coldblk (cloudphy.f90:277-338, coldblk.f90:321-481) as
+//   k_micro_src : one lane per cell, all of getvdens, getexner, diagnw,
+//                 diagnsg, adjstnci, termblk and the srcblk chain (setblk,
+//                 nuc1stv, nuc1stc, collect, prodctwg, distrpg, aggregat,
+//                 melting, depsit, convers, nuc2nd, freezing, shedding,
+//                 more0q, newblk) in registers, then adjstnci; it writes
+//                 the three precipitation fall speeds and folds the
+//                 sedimentation CFL minimum (fallblk.f90:311-437) into one
+//                 device word;
+//   k_micro_fall: one lane per column, fallblk's npstp upwind sub-steps
+//                 (upwqp.f90, upwnp.f90) with the surface precipitation;
+// and s_satadjst (satadjst.f90:154-187) as k_satadjst, one lane per cell.
+// Rates follow the reference's formulas and operation order; options fixed:
+// flqcqi_opt=1, evapor_opt=1, nuc2nd_opt=1, advopt<=3.
 #include "micro.h"
 
 namespace cress {
 
-int micro_cloudphy(const Grid &, const Consts &, const Opts &, hipStream_t, float, int, float *const *,
-                   const float *, const float *, const float *, const float *, float *, float *, float *,
-                   const float *, float *, float *, MicroScratch *) {
-  return CRESS_EUNSUPPORTED;
+namespace mc {
+// comphy.f90, commath.f90, temparam.f90 (UserModAdd overlay)
+constexpr float g = 9.8f, cp = 1004.0f, cw = 4.218e3f, ci = 2.106e3f, rd = 287.0f, rv = 461.0f;
+constexpr float epsva = 0.622f, es0 = 610.78f, r0 = 1.225f, p0 = 1.e5f, t0 = 273.16f;
+constexpr float tlow = 233.16f, t0cel = 0.0f, t20 = 293.16f, t23 = 296.16f, p20 = 101325.0f;
+constexpr float lmb20 = 6.6e-8f, diasl = 6.e-7f, boltz = 1.38066e-23f, sc = 0.6f;
+constexpr float lv0 = 2.50078e6f, lf0 = 3.34e5f, kp0 = 10.79f, kpa = 2.4e-1f, nu0 = 1.328e-5f;
+constexpr float dv0 = 2.23e-5f, xl = 0.25f, nr0 = 8.e6f, ns0 = 1.8e6f, ng0 = 1.1e6f;
+constexpr float aur = 842.0f, aui = 770.0f, aus = 27.0f, aug = 132.0f;
+constexpr float bur = 0.8f, bus = 0.5f, bug = 0.64f, gur = 0.5f, gus = 0.5f, gug = 0.5f;
+constexpr float rhow = 1.e3f, rhoi = 2.e2f, rhos = 1.e2f, rhog = 4.e2f;
+constexpr float ers = 1.0f, erg = 1.0f, eir = 1.0f, eis = 0.1f, eig = 0.1f, esr = 1.0f, esg = 0.001f;
+constexpr float ecc = 0.55f, eii = 0.1f, ess = 0.1f;
+constexpr float mr0 = 2.7e-10f, mi0 = 1.e-12f, ms0 = 1.76e-10f, mg0 = 7.e-10f;
+constexpr float mrmax = 2.68e-4f, mimax = 1.4e-10f, msmax = 6.e-4f, mgmax = 8.e-3f;
+constexpr float nclcst = 1.e8f, diaqcm = 2.e-5f, eps = 1.e-20f, cc = 3.141592f, ln10 = 2.30259f;
+constexpr float gf1bur = 0.931384f, gf1bus = 0.886227f, gf1bug = 0.898642f;
+constexpr float gf3bur = 4.69417f, gf3bus = 3.32335f, gf3bug = 3.89076f;
+constexpr float gf4bur = 17.8379f, gf4bus = 11.6314f, gf4bug = 14.1624f;
+constexpr float gf5bur = 1.82736f, gf5bus = 1.60836f, gf5bug = 1.70506f;
+constexpr float ucqcst = 0.1f, uiqcst = 0.0001f, uincst = 0.0001f;
+}  // namespace mc
+
+// Koenig (1971) ice-crystal growth parameters (setref.f90:473-600), turned
+// into ckoe*10^(3 pkoe) on the host in binary32 and kept in constant memory.
+__constant__ float c_ckoe[41];
+__constant__ float c_pkoe[41];
+
+// exp and log: evaluated in binary64 and rounded once to binary32, i.e. the
+// correctly rounded binary32 result but for rare double roundings.  glibc's
+// expf/logf (the reference's FP32 build) agree with this in all but ~0.06%
+// of arguments.
+__device__ __forceinline__ float ex(float x) { return (float)exp((double)x); }
+__device__ __forceinline__ float lg(float x) { return (float)log((double)x); }
+
+__device__ __forceinline__ float fminr(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float fmaxr(float a, float b) { return a > b ? a : b; }
+
+// Upper bound on sedimentation sub-steps: reached only by fall speeds above
+// dz*jcb*4096/dtb (kilometres per second), where the clamp keeps every lane's
+// loop finite instead of following a corrupted state.
+constexpr int kMaxFallSteps = 4096;
+
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_micro_src(Grid g, MicroK K, MicroArgs a) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  float cfl = K.dtb;
+  if (i <= g.ni - 1 && j <= g.nj - 1) {
+    using namespace mc;
+    const long p = g.at(i, j, k);
+    const long qn = a.qn;
+    const float thresq = K.thresq, dtb = K.dtb;
+    const float rbr = a.rbr[p];
+    // getvdens, getexner (past p')
+    const float rbv = 1.0f / rbr;
+    const float pr = a.pbr[p] + a.ppp[p];
+    const float pi = ex((rd / cp) * lg((1.0f / p0) * pr));
+    // past level
+    const float qvp = a.qp[p], qcp = a.qp[p + qn], qrp = a.qp[p + 2 * qn], qip = a.qp[p + 3 * qn];
+    const float qsp = a.qp[p + 4 * qn], qgp = a.qp[p + 5 * qn], ncip = a.ncip[p];
+    // diagnw / diagnsg
+    const float nccp = nclcst * (1.0f / rbr);
+    const float ncrp = fminr(fmaxr(sqrtf(sqrtf((nr0 * nr0 * nr0 / (cc * rhow)) * rbr * qrp)) * (1.0f / rbr),
+                                   (1.e-2f / mrmax) * qrp), (1.e2f / mr0) * qrp);
+    const float ncsp = fminr(fmaxr(sqrtf(sqrtf((ns0 * ns0 * ns0 / (cc * rhos)) * rbr * qsp)) * rbv,
+                                   (1.e-2f / msmax) * qsp), (1.e2f / ms0) * qsp);
+    const float ncgp = fminr(fmaxr(sqrtf(sqrtf((ng0 * ng0 * ng0 / (cc * rhog)) * rbr * qgp)) * rbv,
+                                   (1.e-2f / mgmax) * qgp), (1.e2f / mg0) * qgp);
+    // future level; adjstnci before the sources
+    float ptpf = a.ptpf[p], qvf = a.qf[p], qcf = a.qf[p + qn], qrf = a.qf[p + 2 * qn];
+    float qif = a.qf[p + 3 * qn], qsf = a.qf[p + 4 * qn], qgf = a.qf[p + 5 * qn];
+    float ncif = fminr(fmaxr(a.ncif[p], (0.5f / mimax) * qif), (1.e2f / mi0) * qif);
+    // termblk
+    const float ucq = qcp > thresq ? ucqcst : 0.0f;
+    const float uiq = qip > thresq ? uiqcst : 0.0f;
+    const float lnr0r = lg(r0 * rbv);
+    const float urq = qrp > thresq
+        ? K.curq * ex(gur * lnr0r) * ex((bur * (1.0f / 3.0f)) * lg((1.0f / (cc * rhow)) * qrp / ncrp)) : 0.0f;
+    const float usq = qsp > thresq
+        ? K.cusq * ex(gus * lnr0r) * ex((bus * (1.0f / 3.0f)) * lg((1.0f / (cc * rhos)) * qsp / ncsp)) : 0.0f;
+    const float ugq = qgp > thresq
+        ? K.cugq * ex(gug * lnr0r) * ex((bug * (1.0f / 3.0f)) * lg((1.0f / (cc * rhog)) * qgp / ncgp)) : 0.0f;
+    // ---- setblk ----
+    const float oned3 = 1.0f / 3.0f;
+    const float t = (a.ptbr[p] + a.ptpp[p]) * pi;
+    const float tcel = t - t0;
+    const float esw = es0 * ex(17.269f * tcel / (t - 35.86f));
+    const float esi = es0 * ex(21.875f * tcel / (t - 7.66f));
+    const float qvsst0 = qvp - (epsva * es0) / (pr - es0);
+    const float qvsw = epsva * esw / (pr - esw);
+    const float qvsi = epsva * esi / (pr - esi);
+    const float lv = lv0 * ex((0.167f + 3.67e-4f * t) * lg(t0 / t));
+    const float lf = lf0 + (cw - ci) * tcel;
+    const float ls = lv + lf;
+    const float p20dvp = p20 / pr;
+    const float nu = p20dvp * ex(1.754f * lg(K.cnu * t));
+    const float kp = (kp0 / (120.0f + t)) * ex(1.5f * lg((1.0f / t23) * t));
+    const float mu = rbr * nu;
+    const float dv = p20dvp * ex(1.81f * lg(K.cdv * t));
+    const float cvnt = sqrtf(sqrtf(r0 * rbv) / nu);
+    const float diaqc = qcp > thresq ? ex(oned3 * lg((6.0f / (cc * rhow)) * qcp / nccp)) : 0.0f;
+    float diaqr = 0.0f, vntr = 0.0f, mi = 0.0f, diaqi = 0.0f, diaqs = 0.0f, vnts = 0.0f, diaqg = 0.0f, vntg = 0.0f;
+    if (qrp > thresq) {
+      diaqr = ex(oned3 * lg((1.0f / (cc * rhow)) * qrp / ncrp));
+      vntr = rbr * ncrp * (0.78f * diaqr + cvnt * K.cvntr * ex(K.pdiaqr * lg(diaqr)));
+    }
+    if (qip > thresq) {
+      mi = qip / ncip;
+      diaqi = ex(oned3 * lg((6.0f / (cc * rhoi)) * mi));
+    }
+    if (qsp > thresq) {
+      diaqs = ex(oned3 * lg((1.0f / (cc * rhos)) * qsp / ncsp));
+      vnts = rbr * ncsp * (0.78f * diaqs + cvnt * K.cvnts * ex(K.pdiaqs * lg(diaqs)));
+    }
+    if (qgp > thresq) {
+      diaqg = ex(oned3 * lg((1.0f / (cc * rhog)) * qgp / ncgp));
+      vntg = rbr * ncgp * (0.78f * diaqg + cvnt * K.cvntg * ex(K.pdiaqg * lg(diaqg)));
+    }
+    // ---- nuc1stv ----
+    float nuvi = 0.0f;
+    if (qvp > thresq && qvp > qvsi && t > tlow && t < t0) {
+      const float qvssi = qvp - qvsi;
+      const float aa = 15.25f * qvp / qvsi - 10.08f;
+      nuvi = aa < 40.0f ? fmaxr(0.0f, fminr(mi0 * ex(aa) * rbv - qip, qvssi)) : qvssi;
+    }
+    // ---- nuc1stc ----
+    float nuci = 0.0f;
+    if (qcp > thresq) {
+      const float nufci = (t > tlow && t < t0)
+          ? fminr(K.rwdt2 * (ex(-0.66f * tcel) - 1.0f) * qcp * qcp / nccp, qcp) : 0.0f;
+      float nucci = 0.0f;
+      if (t > tlow && t < 270.16f) {
+        const float piv = 1.0f / pr;
+        const float knd = K.cknd * piv * t;
+        const float dar = K.cdar * t * (1.0f + knd) / mu;
+        const float f1 = K.cc45 * ex(1.3f * lg(270.16f - t)) * diaqc;
+        // the reference's contact branch compares tc with t where tc=t, so
+        // its thermophoretic term is always zero (nuc1stc.f90:334-360)
+        nucci = (0.0f + f1 * dar) * qcp * dtb;
+      }
+      const float nuhci = t <= tlow ? qcp : 0.0f;
+      nuci = nufci + nucci + nuhci;
+    }
+    // ---- collect ----
+    const float r0rsq = sqrtf(r0 * rbr);
+    float diaqr2 = 0.0f, diaqr3 = 0.0f, qr2b = 0.0f, diaqs2 = 0.0f, diaqs3 = 0.0f, qs2b = 0.0f;
+    float diaqg2 = 0.0f, qg2b = 0.0f;
+    if (qrp > thresq) { diaqr2 = diaqr * diaqr; diaqr3 = diaqr * diaqr2; qr2b = r0rsq * ex((2.0f + bur) * lg(diaqr)); }
+    if (qsp > thresq) { diaqs2 = diaqs * diaqs; diaqs3 = diaqs * diaqs2; qs2b = r0rsq * ex((2.0f + bus) * lg(diaqs)); }
+    if (qgp > thresq) { diaqg2 = diaqg * diaqg; qg2b = r0rsq * ex((2.0f + bug) * lg(diaqg)); }
+    float clcr = 0.0f, clcs = 0.0f, clcg = 0.0f, clri = 0.0f, clrs = 0.0f, clrg = 0.0f, clir = 0.0f, clsr = 0.0f;
+    float clis = 0.0f, clig = 0.0f, clsg = 0.0f, aa, bb, sink;
+    if (t > tlow) {
+      if (qcp > thresq) {
+        const float cstk = ((1.0f / 9.0f) * rhow) * diaqc * diaqc;
+        if (qrp > thresq) {
+          aa = cstk * urq / (diaqr * mu);
+          bb = aa + 0.5f;
+          aa = aa * aa / (bb * bb);
+          clcr = K.cclcr * aa * qr2b * ncrp * qcp;
+        }
+        if (qsp > thresq) {
+          aa = cstk * usq / (diaqs * mu);
+          bb = aa + 0.5f;
+          const float ecs = aa * aa / (bb * bb);
+          clcs = K.cclcs * qs2b * ecs * ncsp * qcp;
+        }
+        if (qgp > thresq) {
+          aa = cstk * ugq / (diaqg * mu);
+          bb = aa + 0.5f;
+          aa = aa * aa / (bb * bb);
+          clcg = K.cclcg * aa * qg2b * ncgp * qcp;
+        }
+        sink = clcr + clcs + clcg;
+        if (qcp < sink) { aa = qcp / sink; clcr = clcr * aa; clcs = clcs * aa; clcg = clcg * aa; }
+      }
+      if (qrp > thresq) {
+        if (qip > thresq && t < t0) { clri = K.cclri * qr2b * ncrp * qip; clir = clri; }
+        if (qsp > thresq) {
+          aa = 2.0f * diaqr * diaqs;
+          bb = urq - usq;
+          bb = ncrp * ncsp * rbr * sqrtf(bb * bb + 0.04f * urq * usq);
+          clrs = bb * K.cclrs * (5.0f * diaqr2 + aa + 0.5f * diaqs2) * diaqr3;
+          clsr = bb * K.cclsr * (5.0f * diaqs2 + aa + 0.5f * diaqr2) * diaqs3;
+        }
+        if (qgp > thresq) {
+          aa = 5.0f * diaqr2 + 2.0f * diaqr * diaqg + 0.5f * diaqg2;
+          bb = urq - ugq;
+          clrg = aa * K.cclrg * diaqr3 * ncrp * ncgp * rbr * sqrtf(bb * bb + 0.04f * urq * ugq);
+        }
+        sink = clri + clrs + clrg;
+        if (qrp < sink) { aa = qrp / sink; clri = clri * aa; clrs = clrs * aa; clrg = clrg * aa; }
+      }
+    }
+    if (qip > thresq && t < t0) {
+      if (qsp > thresq) clis = K.cclis * qs2b * ncsp * qip;
+      if (qgp > thresq) clig = K.cclig * qg2b * ncgp * qip;
+      sink = clir + clis + clig;
+      if (qip < sink) { aa = qip / sink; clir = clir * aa; clis = clis * aa; clig = clig * aa; }
+    }
+    if (qsp > thresq) {
+      if (qgp > thresq) {
+        aa = 5.0f * diaqs2 + 2.0f * diaqs * diaqg + 0.5f * diaqg2;
+        bb = usq - ugq;
+        clsg = aa * (t < t0 ? K.cclsg : K.cclsg3) * diaqs3 * ncsp * ncgp * rbr * sqrtf(bb * bb + 0.04f * usq * ugq);
+      }
+      sink = clsr + clsg;
+      if (qsp < sink) { aa = qsp / sink; clsr = clsr * aa; clsg = clsg * aa; }
+    }
+    // ---- prodctwg ----
+    float pgwet = -1.0f;
+    if (qgp > thresq && tcel < t0cel) {
+      const float pgdry = clcg + clrg + clig + clsg;
+      const float lfice = lf + cw * tcel;
+      const float cligw = (1.0f / eig) * clig, clsgw = (1.0f / esg) * clsg;
+      const float pg = K.cc2dtn * vntg * (lv * dv * rbr * qvsst0 + kp * tcel) / (lfice * rbr) +
+                       (cligw + clsgw) * (1.0f - ci * tcel / lfice);
+      if (pg > 0.0f && pg < pgdry) {
+        pgwet = pg;
+        clig = cligw;
+        clsg = clsgw;
+        sink = clir + clis + clig;
+        if (qip < sink) { aa = qip / sink; clir = clir * aa; clis = clis * aa; clig = clig * aa; }
+        sink = clsr + clsg;
+        if (qsp < sink) { aa = qsp / sink; clsr = clsr * aa; clsg = clsg * aa; }
+      }
+    }
+    // ---- distrpg ----
+    float clrsg = 0.0f;
+    if (qsp > thresq) {
+      if (t < t0) {
+        aa = diaqs * diaqs;
+        bb = diaqr * diaqr;
+        aa = (rhos * rhos) * aa * aa * aa;
+        bb = (rhow * rhow) * bb * bb * bb;
+        const float alpha = aa / (aa + bb), alpha1 = 1.0f - alpha;
+        clrsg = alpha1 * clrs;
+        clrs = alpha * clrs;
+        clsr = alpha1 * clsr;
+      } else {
+        clrsg = clrs;
+        clrs = 0.0f;
+      }
+    }
+    // ---- aggregat ----
+    const float agin = qip > thresq ? rbr * ncip * qip * ex(oned3 * lg(K.cagin_ag * rbv)) : 0.0f;
+    // ---- melting ----
+    const float cmlxr1 = K.cc2dt * rbv * (kp * tcel + lv * dv * rbr * qvsst0);
+    const float cmlxr2 = cw * tcel;
+    float mlic = (qip > thresq && tcel >= t0cel) ? qip : 0.0f;
+    float mlsr = (qsp > thresq && tcel >= t0cel) ? fmaxr((cmlxr1 * vnts + cmlxr2 * (clcs + clrs)) / lf, 0.0f) : 0.0f;
+    float mlgr = (qgp > thresq && tcel >= t0cel) ? fmaxr((cmlxr1 * vntg + cmlxr2 * (clcg + clrg)) / lf, 0.0f) : 0.0f;
+    // ---- depsit ----
+    float vdvr = 0.0f, vdvi = 0.0f, vdvs = 0.0f, vdvg = 0.0f;
+    if (t > tlow) {
+      const float qvssw = qvp - qvsw, qvssi = qvp - qvsi;
+      const float ad = 1.0f / (rv * kp * t * t);
+      const float bd = dv * rbr;
+      const float cd = K.ccdtb4 * rbv;
+      const float gi = 1.0f / (ls * ls * ad + 1.0f / (bd * qvsi));
+      const float cvdvx1 = cd * (qvp / qvsw - 1.0f) / (lv * lv * ad + 1.0f / (bd * qvsw));
+      const float cvdvx2 = K.ccdtb4 * dv * qvsst0;
+      const float cvdvx3 = cd * (qvp / qvsi - 1.0f);
+      const float cvdvx4 = ad * ls * lf;
+      if (qrp > thresq && qvp < qvsw) vdvr = fmaxr(cvdvx1 * vntr, qvssw);
+      if (qip > thresq && t < t0 - 0.01f) {
+        const int itsc = (int)(-tcel);
+        vdvi = c_ckoe[itsc] * (qvp - qvsi) / (qvsw - qvsi) * ncip * ex(c_pkoe[itsc] * lg(mi)) * dtb;
+      }
+      if (qsp > thresq) {
+        if (t > t0) vdvs = mlsr > 0.0f ? cvdvx2 * vnts : cvdvx1 * vnts;
+        else vdvs = gi * (cvdvx3 * vnts - cvdvx4 * clcs);
+      }
+      if (qgp > thresq) {
+        if (t > t0) vdvg = mlgr > 0.0f ? cvdvx2 * vntg : cvdvx1 * vntg;
+        else vdvg = gi * (cvdvx3 * vntg - cvdvx4 * clcg);
+      }
+      sink = vdvi + vdvs + vdvg;
+      if ((qvssi < sink && qvssi > 0.0f) || (qvssi > sink && qvssi < 0.0f)) {
+        const float h = qvssi / sink;
+        vdvi = vdvi * h; vdvs = vdvs * h; vdvg = vdvg * h;
+      }
+    }
+    // ---- convers ----
+    float cncr = 0.0f, cnis = 0.0f, cnsg = 0.0f;
+    if (qcp > thresq && t > tlow && qcp >= K.qccm * nccp)
+      cncr = K.ccncr * rbr * ex(-oned3 * lg(nccp)) * ex((7.0f / 3.0f) * lg(qcp)) / mu;
+    if (t < t0) {
+      if (qip > thresq) {
+        float cn = rbr * ex(oned3 * lg(K.cagin_cv * rbv)) * qip * qip / lg(K.diaqs0 / diaqi);
+        if (vdvi >= 0.0f) {
+          const float ms05 = 0.5f * ms0;
+          if (mi < ms05) cn = cn + mi / (ms0 - mi) * vdvi;
+          else cn = cn + (vdvi + (1.0f - ms05 / mi) * qip);
+        }
+        cnis = fminr(cn, qip);
+      }
+      if (qsp > thresq && vdvs > 0.0f && vdvs < clcs) cnsg = fminr(K.ccnsg * clcs, qsp);
+    }
+    // ---- nuc2nd ----
+    float spsi = 0.0f, spgi = 0.0f;
+    if (t <= 270.16f && t > 265.16f) {
+      const float fr = t > 268.16f ? (270.16f - t) * K.mi0352 * rbv : (t - 265.16f) * K.mi0353 * rbv;
+      spsi = fr * clcs;
+      spgi = pgwet > 0.0f ? 0.0f : fr * clcg;
+    }
+    // ---- freezing ----
+    float frrg = 0.0f;
+    if (qrp > thresq && tcel < t0cel) {
+      if (tcel <= tlow - t0) {
+        frrg = qrp;
+      } else {
+        const float d3 = diaqr * diaqr * diaqr;
+        frrg = fminr(qrp, K.cfrrg * d3 * d3 * (ex(-0.66f * tcel) - 1.0f) * ncrp);
+      }
+    }
+    // ---- shedding ----
+    float shsr = (qsp > thresq && t >= t0) ? clcs + clrs : 0.0f;
+    float shgr = 0.0f;
+    if (qgp > thresq) {
+      if (t >= t0) shgr = clcg + clrg;
+      else if (pgwet > 0.0f) shgr = (clcg + clrg + clig + clsg) - pgwet;
+    }
+    // ---- more0q ----
+    float h;
+    if (qcp > thresq) {
+      sink = nuci + clcr + clcs + clcg + cncr - mlic;
+      if (qcf < sink) {
+        h = qcf / sink;
+        nuci *= h; clcr *= h; clcs *= h; clcg *= h; cncr *= h; mlic *= h;
+      }
+    }
+    if (qip > thresq) {
+      sink = clir + clis + clig + cnis + mlic - vdvi - nuvi - nuci - spsi - spgi;
+      if (qif < sink) {
+        h = qif / sink;
+        clir *= h; clis *= h; clig *= h; cnis *= h; mlic *= h; vdvi *= h; nuvi *= h; nuci *= h; spsi *= h; spgi *= h;
+      }
+    }
+    if (qrp > thresq) {
+      sink = clri + clrs + clrg + clrsg + frrg - vdvr - clcr - cncr - mlsr - mlgr - shsr - shgr;
+      if (qrf < sink) {
+        h = qrf / sink;
+        clri *= h; clrs *= h; clrg *= h; clrsg *= h; frrg *= h; vdvr *= h; clcr *= h; cncr *= h;
+        mlsr *= h; mlgr *= h; shsr *= h; shgr *= h;
+      }
+    }
+    if (qsp > thresq) {
+      sink = clsr + clsg + cnsg + spsi + mlsr + shsr - vdvs - clcs - clrs - clis - cnis;
+      if (qsf < sink) {
+        h = qsf / sink;
+        clsr *= h; clsg *= h; cnsg *= h; spsi *= h; mlsr *= h; shsr *= h; vdvs *= h; clcs *= h; clrs *= h;
+        clis *= h; cnis *= h;
+      }
+    }
+    if (qgp > thresq) {
+      sink = spgi + mlgr + shgr - vdvg - clri - clir - clsr - clcg - clrg - clig - clsg - clrsg - cnsg - frrg;
+      if (qgf < sink) {
+        h = qgf / sink;
+        spgi *= h; mlgr *= h; shgr *= h; vdvg *= h; clri *= h; clir *= h; clsr *= h; clcg *= h; clrg *= h;
+        clig *= h; clsg *= h; clrsg *= h; cnsg *= h; frrg *= h;
+      }
+    }
+    // ---- newblk ----
+    const float cppiv = 1.0f / (cp * pi);
+    const float clix = clir + clis + clig;
+    const float spxi = spsi + spgi;
+    const float cxcr = clcr + cncr;
+    const float cxsg = clsr + clsg + cnsg;
+    const float nmci = nuci - mlic;
+    const float mssr = mlsr + shsr;
+    const float msgr = mlgr + shgr;
+    const float nuvdvx = vdvi + vdvs + vdvg + nuvi;
+    const float clnmci = clcs + clcg + nmci;
+    const float cfmsxr = clri + clrs + clrg + clrsg + frrg - mssr - msgr;
+    ptpf = ptpf + (vdvr * lv + (clnmci + cfmsxr) * lf + nuvdvx * ls) * cppiv;
+    float qvsink = nuvdvx + vdvr;
+    float qxsink = cfmsxr - cxcr - vdvr;
+    if (qrf < qxsink) {
+      qxsink = cxcr + clnmci + (qxsink - qrf);
+      qrf = 0.0f;
+    } else {
+      qrf = qrf - qxsink;
+      qxsink = cxcr + clnmci;
+    }
+    if (qcf < qxsink) {
+      const float dqv = qxsink - qcf;
+      ptpf = ptpf + cppiv * dqv * lv;
+      qvsink = qvsink + dqv;
+      qcf = 0.0f;
+    } else {
+      qcf = qcf - qxsink;
+    }
+    qxsink = clix - spxi - nmci + cnis - vdvi - nuvi;
+    if (qif < qxsink) {
+      const float dqv = qxsink - qif;
+      ptpf = ptpf + cppiv * dqv * ls;
+      qvsink = qvsink + dqv;
+      qif = 0.0f;
+    } else {
+      qif = qif - qxsink;
+    }
+    qvf = fmaxr(qvf - qvsink, 0.0f);
+    qsf = fmaxr(qsf - (cxsg + mssr + spsi - vdvs - clcs - clrs - clis - cnis), 0.0f);
+    qgf = fmaxr(qgf + (cxsg - msgr - spgi + vdvg + clri + clir + clcg + clrg + clig + clrsg + frrg), 0.0f);
+    if (qcp > thresq) ncif = ncif + nuci * nccp / qcp;
+    if (qip > thresq) {
+      if (vdvi < 0.0f)
+        ncif = ncif - (agin + (clix + mlic - vdvi) / mi - (spxi + nuvi) * (1.0f / mi0) + cnis * (1.0f / ms0));
+      else
+        ncif = ncif - (agin + (clix + mlic) / mi - (spxi + nuvi) * (1.0f / mi0) + cnis * (1.0f / ms0));
+    } else {
+      ncif = ncif + (spxi + nuvi) * (1.0f / mi0);
+    }
+    // adjstnci after the sources
+    ncif = fminr(fmaxr(ncif, (0.5f / mimax) * qif), (1.e2f / mi0) * qif);
+    a.ptpf[p] = ptpf;
+    a.qf[p] = qvf;
+    a.qf[p + qn] = qcf;
+    a.qf[p + 2 * qn] = qrf;
+    a.qf[p + 3 * qn] = qif;
+    a.qf[p + 4 * qn] = qsf;
+    a.qf[p + 5 * qn] = qgf;
+    a.ncif[p] = ncif;
+    a.urq[p] = urq;
+    a.usq[p] = usq;
+    a.ugq[p] = ugq;
+    // sedimentation CFL (fallblk.f90:364-377)
+    const float dzjcb = K.dz * a.jcb[p];
+    cfl = fminr(dzjcb / (ucq + eps), cfl);
+    cfl = fminr(dzjcb / (urq + eps), cfl);
+    cfl = fminr(dzjcb / (uiq + eps), cfl);
+    cfl = fminr(dzjcb / (usq + eps), cfl);
+    cfl = fminr(dzjcb / (ugq + eps), cfl);
+  }
+  // block minimum, then one atomic per wavefront-reduced block
+  __shared__ float red[256];
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  red[tid] = cfl;
+  __syncthreads();
+  for (int s = (blockDim.x * blockDim.y) / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] = fminr(red[tid], red[tid + s]);
+    __syncthreads();
+  }
+  if (tid == 0) atomicMin(a.dtpmin, __float_as_uint(red[0]));
 }
 
-int micro_satadjst(const Grid &, const Consts &, const Opts &, hipStream_t, float *const *, const float *,
-                   float *, float *, float *, MicroScratch *) {
-  return CRESS_EUNSUPPORTED;
+// One upwind sedimentation column sweep of fallblk for a mixing ratio (with
+// precipitation, upwqp.f90:164-202) or a number (upwnp.f90:143-176).
+__device__ __forceinline__ void upw_column(const Grid &g, int i, int j, float dtp, float dziv, const float *rbr,
+                                           const float *rst, const float *u, float *q, float *precip,
+                                           long n2) {
+  const int nk = g.nk;
+  const float dzvdt = dziv * dtp;
+  long p = g.at(i, j, 1);
+  float flx_k = rbr[p] * u[p] * q[p];
+  const float flx1 = flx_k;
+  float flx2 = 0.0f;
+  for (int k = 1; k <= nk - 2; k++) {
+    const float flx_kp1 = rbr[p + g.nxy] * u[p + g.nxy] * q[p + g.nxy];
+    if (k == 1) flx2 = flx_kp1;
+    q[p] = fmaxr(q[p] + (flx_kp1 - flx_k) / rst[p] * dzvdt, 0.0f);
+    flx_k = flx_kp1;
+    p += g.nxy;
+  }
+  q[g.at(i, j, nk - 1)] = q[g.at(i, j, nk - 2)];
+  if (precip) {
+    const long q2 = (long)i + g.nx * j;
+    const float rate = (flx1 + flx2) * (0.5f / mc::rhow);
+    precip[q2] = rate;
+    precip[q2 + n2] = precip[q2 + n2] + rate * (0.5f * dtp);
+  }
+}
+
+// fallblk.f90:439-541 for one column (i,j) in 1..ni-1 x 1..nj-1, then the
+// trailing adjstnci (coldblk.f90:463-465).
+__global__ void k_micro_fall(Grid g, FallArgs a) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.ni - 1 || j > g.nj - 1) return;
+  const float dtb = a.dtb;
+  float dtp = fminr(__uint_as_float(*a.dtpmin), dtb);
+  int npstp;
+  if (dtp < dtb) {
+    const float r = (dtb + 0.001f) / dtp;
+    npstp = r < (float)(kMaxFallSteps - 1) ? (int)r + 1 : kMaxFallSteps;
+    dtp = dtb / (float)npstp;
+  } else {
+    npstp = 1;
+    dtp = dtb;
+  }
+  const long n2 = g.nx * (long)(g.nj + 2);
+  const long qn = a.qn;
+  // cloud water and ice fall at constant speeds where present on the past
+  // level (termblk.f90:303-330), so their speeds are recomputed here
+  float *qc = a.qf + qn, *qr = a.qf + 2 * qn, *qi = a.qf + 3 * qn, *qs = a.qf + 4 * qn, *qg = a.qf + 5 * qn;
+  for (int s = 0; s < npstp; s++) {
+    // qc, qi (upwqp) and nci (upwnp) with u from the past level
+    for (int sp = 0; sp < 3; sp++) {
+      float *q = sp == 0 ? qc : (sp == 1 ? qi : a.ncif);
+      const float *qpres = sp == 0 ? a.qp + qn : a.qp + 3 * qn;
+      const float ucst = sp == 0 ? mc::ucqcst : (sp == 1 ? mc::uiqcst : mc::uincst);
+      float *prec = sp == 0 ? a.prwtr : (sp == 1 ? a.price : nullptr);
+      const float dzvdt = a.dziv * dtp;
+      long p = g.at(i, j, 1);
+      float u_k = qpres[p] > a.thresq ? ucst : 0.0f;
+      float flx_k = a.rbr[p] * u_k * q[p];
+      const float flx1 = flx_k;
+      float flx2 = 0.0f;
+      for (int k = 1; k <= g.nk - 2; k++) {
+        const long pu = p + g.nxy;
+        const float u_kp1 = qpres[pu] > a.thresq ? ucst : 0.0f;
+        const float flx_kp1 = a.rbr[pu] * u_kp1 * q[pu];
+        if (k == 1) flx2 = flx_kp1;
+        q[p] = fmaxr(q[p] + (flx_kp1 - flx_k) / a.rst[p] * dzvdt, 0.0f);
+        flx_k = flx_kp1;
+        p = pu;
+      }
+      q[g.at(i, j, g.nk - 1)] = q[g.at(i, j, g.nk - 2)];
+      if (prec) {
+        const long q2 = (long)i + g.nx * j;
+        const float rate = (flx1 + flx2) * (0.5f / mc::rhow);
+        prec[q2] = rate;
+        prec[q2 + n2] = prec[q2 + n2] + rate * (0.5f * dtp);
+      }
+    }
+  }
+  for (int s = 0; s < npstp; s++) {
+    upw_column(g, i, j, dtp, a.dziv, a.rbr, a.rst, a.urq, qr, a.prwtr + 2 * n2, n2);
+    upw_column(g, i, j, dtp, a.dziv, a.rbr, a.rst, a.usq, qs, a.price + 2 * n2, n2);
+    upw_column(g, i, j, dtp, a.dziv, a.rbr, a.rst, a.ugq, qg, a.price + 4 * n2, n2);
+  }
+  for (int k = 1; k <= g.nk - 1; k++) {
+    const long p = g.at(i, j, k);
+    a.ncif[p] = fminr(fmaxr(a.ncif[p], (0.5f / mc::mimax) * qi[p]), (1.e2f / mc::mi0) * qi[p]);
+  }
+}
+
+// s_satadjst for one cell: getexner (getexner.f90:123-131), siadjst
+// (siadjst.f90:159-269) and swadjst (swadjst.f90:237-310, cphopt<=3).
+__global__ void k_satadjst(Grid g, float thresq, const float *pbr, const float *ptbr, const float *pp,
+                           float *ptp, float *q, float *nci, long qn) {
+  using namespace mc;
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.ni - 1 || j > g.nj - 1) return;
+  const long p = g.at(i, j, k);
+  const float pr = pbr[p] + pp[p];
+  const float pi = ex((rd / cp) * lg((1.0f / p0) * pr));
+  const float tb = ptbr[p];
+  float th = ptp[p], qv = q[p], qc = q[p + qn], qi = q[p + 3 * qn], ni = nci[p];
+  // ice
+  float t = (tb + th) * pi;
+  if (t <= tlow) {
+    for (int pass = 0; pass < 2; pass++) {
+      if (pass == 1) {
+        t = (tb + th) * pi;
+        if (!(t <= tlow)) break;
+      }
+      const float tcel = t - t0;
+      const float aa = 1.0f / (t - 7.66f);
+      const float bb = aa * tcel;
+      const float esi = es0 * ex(21.875f * bb);
+      const float qvsi = epsva * esi / (pr - esi);
+      if (qi > thresq || qv > qvsi) {
+        const float lscpi = (lv0 * ex((0.167f + 3.67e-4f * t) * lg(t0 / t)) + (lf0 + (cw - ci) * tcel)) / (cp * pi);
+        const float dqi = (qvsi - qv) / (1.0f + 21.875f * aa * (1.0f - bb) * qvsi * lscpi * pi);
+        if (qi > dqi) {
+          ni = qi > thresq ? ni - dqi * ni / qi : ni - dqi * (1.0f / mi0);
+          th = th - dqi * lscpi;
+          qv = qv + dqi;
+          qi = qi - dqi;
+        } else {
+          ni = 0.0f;
+          th = th - qi * lscpi;
+          qv = qv + qi;
+          qi = 0.0f;
+        }
+      }
+    }
+  }
+  // water
+  for (int pass = 0; pass < 2; pass++) {
+    t = (tb + th) * pi;
+    const float aa = 1.0f / (t - 35.86f);
+    const float bb = aa * (t - t0);
+    const float esw = es0 * ex(17.269f * bb);
+    const float qvsw = epsva * esw / (pr - esw);
+    if (!(qc > thresq || qv > qvsw)) break;
+    const float lvcpi = lv0 * ex((0.167f + 3.67e-4f * t) * lg(t0 / t)) / (cp * pi);
+    const float dqc = (qvsw - qv) / (1.0f + 17.269f * aa * (1.0f - bb) * qvsw * lvcpi * pi);
+    if (qc > dqc) {
+      th = th - dqc * lvcpi;
+      qv = qv + dqc;
+      qc = qc - dqc;
+    } else {
+      th = th - qc * lvcpi;
+      qv = qv + qc;
+      qc = 0.0f;
+    }
+  }
+  ptp[p] = th;
+  q[p] = qv;
+  q[p + qn] = qc;
+  q[p + 3 * qn] = qi;
+  nci[p] = ni;
+}
+
+__global__ void k_micro_seed(unsigned *p, unsigned v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// ---------------------------------------------------------------------------
+int micro_init_tables() {
+  static int done = 0;
+  if (done) return 0;
+  const float c[41] = {
+      0.0f, .7939e-10f, .7841e-9f, .3369e-8f, .4336e-8f, .5285e-8f, .3728e-8f, .1852e-8f, .2991e-9f,
+      .4248e-9f, .7434e-9f, .1812e-8f, .4394e-8f, .9145e-8f, .1725e-7f, .3348e-7f, .1725e-7f, .9175e-8f,
+      .4412e-8f, .2252e-8f, .9115e-9f, .4876e-9f, .3473e-9f, .4758e-9f, .6306e-9f, .8573e-9f, .7868e-9f,
+      .7192e-9f, .6513e-9f, .5956e-9f, .5333e-9f, .4834e-9f, .4335e-9f, .3836e-9f, .3337e-9f, .2838e-9f,
+      .2339e-9f, .1840e-9f, .1341e-9f, .8420e-10f, .3430e-10f};
+  const float pk[41] = {
+      0.0f, .4006f, .4831f, .5320f, .5307f, .5319f, .5249f, .4888f, .3894f, .4047f, .4318f, .4771f, .5183f,
+      .5463f, .5651f, .5813f, .5655f, .5478f, .5203f, .4906f, .4447f, .4126f, .3960f, .4149f, .4320f, .4506f,
+      .4483f, .4460f, .4433f, .4413f, .4382f, .4361f, .4340f, .4319f, .4298f, .4277f, .4256f, .4235f, .4214f,
+      .4193f, .4172f};
+  float ck[41];
+  for (int t = 0; t < 41; t++) ck[t] = c[t] * expf(3.0f * pk[t] * mc::ln10);
+  if (hipMemcpyToSymbol(HIP_SYMBOL(c_ckoe), ck, sizeof(ck)) != hipSuccess) return CRESS_EHIP;
+  if (hipMemcpyToSymbol(HIP_SYMBOL(c_pkoe), pk, sizeof(pk)) != hipSuccess) return CRESS_EHIP;
+  done = 1;
+  return 0;
+}
+
+MicroK micro_fold(float dtb, float thresq, float dz) {
+  using namespace mc;
+  MicroK K;
+  const float oned3 = 1.0f / 3.0f, oned6 = 1.0f / 6.0f;
+  K.dtb = dtb;
+  K.thresq = thresq;
+  K.dz = dz;
+  float cdv = 1.0f / t0;
+  K.cnu = expf(.57013f * logf(nu0)) * cdv;
+  K.cdv = expf(.55249f * logf(dv0)) * cdv;
+  K.pdiaqr = .5f * (5.0f + bur) - 1.0f;
+  K.pdiaqs = .5f * (5.0f + bus) - 1.0f;
+  K.pdiaqg = .5f * (5.0f + bug) - 1.0f;
+  K.cvntr = .31f * sqrtf(aur) * expf(oned3 * logf(sc)) * gf5bur;
+  K.cvnts = .31f * sqrtf(aus) * expf(oned3 * logf(sc)) * gf5bus;
+  K.cvntg = .31f * sqrtf(aug) * expf(oned3 * logf(sc)) * gf5bug;
+  K.curq = oned6 * aur * gf4bur;
+  K.cusq = oned6 * aus * gf4bus;
+  K.cugq = oned6 * aug * gf4bug;
+  K.rwdt2 = (100.0f / rhow) * dtb;
+  K.cc45 = 4.e5f * cc;
+  K.cknd = 2.0f * lmb20 * p20 / (t20 * diasl);
+  K.cdar = boltz / (3.0f * cc * diasl);
+  K.cclcr = .25f * aur * gf3bur * cc * dtb;
+  K.cclcs = .25f * aus * gf3bus * cc * dtb;
+  K.cclcg = .25f * aug * gf3bug * cc * dtb;
+  K.cclri = .25f * aur * gf3bur * eir * cc * dtb;
+  K.cclrs = ers * cc * cc * rhow * dtb;
+  K.cclrg = erg * cc * cc * rhow * dtb;
+  K.cclis = .25f * aus * gf3bus * eis * cc * dtb;
+  K.cclig = .25f * aug * gf3bug * eig * cc * dtb;
+  K.cclsr = esr * cc * cc * rhos * dtb;
+  K.cclsg = esg * cc * cc * rhos * dtb;
+  K.cclsg3 = cc * cc * rhos * dtb;
+  K.cc2dtn = -2.0f * cc * dtb;
+  K.cc2dt = 2.0f * cc * dtb;
+  K.cagin_ag = r0 * expf(3.0f * logf(.5f * aui * eii * xl / rhoi * dtb));
+  K.ccdtb4 = 2.0f * cc * dtb;
+  K.qccm = oned6 * rhow * cc * diaqcm * diaqcm * diaqcm;
+  K.diaqs0 = expf(oned3 * logf(6.0f * ms0 / (cc * rhos)));
+  K.cagin_cv = r0 * expf(3.0f * logf(.5f * oned3 * aui * eii * xl / rhoi * dtb));
+  K.ccncr = .104f * g * ecc * expf(-oned3 * logf(rhow)) * dtb;
+  K.ccnsg = rhog * (1.0f / (rhog - rhos));
+  K.cfrrg = 20.0f * 100.0f * cc * cc * rhow * dtb;
+  K.mi0352 = .5f * 3.5e8f * mi0;
+  K.mi0353 = oned3 * 3.5e8f * mi0;
+  (void)ess;
+  return K;
 }
 
 }  // namespace cress

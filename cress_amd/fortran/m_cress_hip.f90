@@ -1,0 +1,198 @@
+!-----------------------------------------------------------------------
+! m_cress_hip -- ISO_C_BINDING interface to the C ABI of the MI355X large
+! step (include/cress_hip.h).  A Fortran 90 host uses this module to drive
+! the device-resident context (tier 1) or to call the per-routine shims
+! (tier 2) with the reference's explicit-shape arrays.
+!
+! The derived type cress_config is interoperable with the C struct of the
+! same name: same member order, C int / double / char kinds.
+!-----------------------------------------------------------------------
+module m_cress_hip
+
+  use iso_c_binding
+
+  implicit none
+
+  private
+
+  integer(c_int), parameter, public :: cress_ok = 0
+  integer(c_int), parameter, public :: cress_einval = -1
+  integer(c_int), parameter, public :: cress_enodev = -2
+  integer(c_int), parameter, public :: cress_ehip = -3
+  integer(c_int), parameter, public :: cress_enomem = -4
+  integer(c_int), parameter, public :: cress_erange = -5
+  integer(c_int), parameter, public :: cress_eunsupported = -6
+
+! field ids (base state: level 0 only; prognostics: 0 past, 1 present,
+! 2 future)
+  integer(c_int), parameter, public :: fid_jcb = 0, fid_jcb8u = 1,      &
+     &  fid_jcb8v = 2, fid_jcb8w = 3, fid_ubr = 4, fid_vbr = 5,         &
+     &  fid_pbr = 6, fid_ptbr = 7, fid_qvbr = 8, fid_rbr = 9,           &
+     &  fid_rst = 10, fid_rst8u = 11, fid_rst8v = 12, fid_rst8w = 13,   &
+     &  fid_rcsq = 14, fid_nbase = 15
+  integer(c_int), parameter, public :: fid_u = 32, fid_v = 33,          &
+     &  fid_w = 34, fid_pp = 35, fid_ptp = 36, fid_qv = 37, fid_qc = 38,&
+     &  fid_qr = 39, fid_qi = 40, fid_qs = 41, fid_qg = 42, fid_nci = 43
+  integer(c_int), parameter, public :: fid_fc = 64, fid_rbct = 65,      &
+     &  fid_prwtr = 66, fid_price = 67
+
+  type, bind(c), public :: cress_config
+    integer(c_int) :: ni, nj, nk
+    integer(c_int) :: wbc, ebc, sbc, nbc
+    integer(c_int) :: bbc, tbc
+    integer(c_int) :: advopt, impopt, gwmopt, buyopt, divopt
+    integer(c_int) :: cphopt, haiopt, vspopt, coropt, sthopt
+    integer(c_int) :: ksp0(2)
+    character(kind=c_char) :: vspvar(10)
+    integer(c_int) :: nsstp
+    real(c_double) :: dx, dy, dz
+    real(c_double) :: dtbig, dtsml
+    real(c_double) :: weicoe, filcoe, thresq
+  end type cress_config
+
+  public :: cress_error_string, cress_device_count
+  public :: cress_ctx_create, cress_ctx_destroy, cress_field_size
+  public :: cress_upload, cress_download, cress_large_step, cress_sync
+  public :: cress_set_timing, cress_stage_times
+  public :: cress_gaussel, cress_advs, cress_bcycle
+  public :: cress_errmsg
+
+  interface
+
+    function cress_error_string(code) bind(c, name='cress_error_string')
+      import :: c_int, c_ptr
+      integer(c_int), value :: code
+      type(c_ptr) :: cress_error_string
+    end function cress_error_string
+
+    function cress_device_count(n) bind(c, name='cress_device_count')
+      import :: c_int
+      integer(c_int), intent(out) :: n
+      integer(c_int) :: cress_device_count
+    end function cress_device_count
+
+    function cress_ctx_create(cfg, ctx) bind(c, name='cress_ctx_create')
+      import :: c_int, c_ptr, cress_config
+      type(cress_config), intent(in) :: cfg
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int) :: cress_ctx_create
+    end function cress_ctx_create
+
+    function cress_ctx_destroy(ctx) bind(c, name='cress_ctx_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: cress_ctx_destroy
+    end function cress_ctx_destroy
+
+    function cress_field_size(ctx, field, nfloat)                       &
+     &  bind(c, name='cress_field_size')
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field
+      integer(c_size_t), intent(out) :: nfloat
+      integer(c_int) :: cress_field_size
+    end function cress_field_size
+
+    function cress_upload(ctx, field, level, host)                      &
+     &  bind(c, name='cress_upload')
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field, level
+      real(c_float), intent(in) :: host(*)
+      integer(c_int) :: cress_upload
+    end function cress_upload
+
+    function cress_download(ctx, field, level, host)                    &
+     &  bind(c, name='cress_download')
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field, level
+      real(c_float), intent(inout) :: host(*)
+      integer(c_int) :: cress_download
+    end function cress_download
+
+    function cress_large_step(ctx, nsteps, first)                       &
+     &  bind(c, name='cress_large_step')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsteps, first
+      integer(c_int) :: cress_large_step
+    end function cress_large_step
+
+    function cress_sync(ctx) bind(c, name='cress_sync')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: cress_sync
+    end function cress_sync
+
+    function cress_set_timing(ctx, on) bind(c, name='cress_set_timing')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+      integer(c_int) :: cress_set_timing
+    end function cress_set_timing
+
+    function cress_stage_times(ctx, ms, nstage)                         &
+     &  bind(c, name='cress_stage_times')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: ms(*)
+      integer(c_int), value :: nstage
+      integer(c_int) :: cress_stage_times
+    end function cress_stage_times
+
+! tier 2: s_gaussel (gaussel.f90:72-73), impopt=1
+    function cress_gaussel(istr, iend, jstr, jend, kstr, kend,          &
+     &  ni, nj, nk, rr, ss, tt, ff) bind(c, name='cress_gaussel')
+      import :: c_int, c_float
+      integer(c_int), value :: istr, iend, jstr, jend, kstr, kend
+      integer(c_int), value :: ni, nj, nk
+      real(c_float), intent(in) :: rr(*), tt(*)
+      real(c_float), intent(inout) :: ss(*), ff(*)
+      integer(c_int) :: cress_gaussel
+    end function cress_gaussel
+
+! tier 2: s_advs (advs.f90:75-77)
+    function cress_advs(cfg, rstxu, rstxv, rstxwc, s, sfrc)             &
+     &  bind(c, name='cress_advs')
+      import :: c_int, c_float, cress_config
+      type(cress_config), intent(in) :: cfg
+      real(c_float), intent(in) :: rstxu(*), rstxv(*), rstxwc(*), s(*)
+      real(c_float), intent(inout) :: sfrc(*)
+      integer(c_int) :: cress_advs
+    end function cress_advs
+
+! tier 2: s_bcycle (bcycle.f90:72-74)
+    function cress_bcycle(cfg, iwsnd, iwrcv, iesnd, iercv,              &
+     &  jssnd, jsrcv, jnsnd, jnrcv, kmax, var) bind(c, name='cress_bcycle')
+      import :: c_int, c_float, cress_config
+      type(cress_config), intent(in) :: cfg
+      integer(c_int), value :: iwsnd, iwrcv, iesnd, iercv
+      integer(c_int), value :: jssnd, jsrcv, jnsnd, jnrcv, kmax
+      real(c_float), intent(inout) :: var(*)
+      integer(c_int) :: cress_bcycle
+    end function cress_bcycle
+
+  end interface
+
+contains
+
+! The message of a return code as a Fortran string.
+  function cress_errmsg(code) result(msg)
+    integer(c_int), intent(in) :: code
+    character(len=64) :: msg
+    character(kind=c_char), pointer :: cs(:)
+    type(c_ptr) :: p
+    integer :: n
+
+    msg = ' '
+    p = cress_error_string(code)
+    if(.not.c_associated(p)) return
+    call c_f_pointer(p, cs, [64])
+    do n = 1, 64
+      if(cs(n) == c_null_char) exit
+      msg(n:n) = cs(n)
+    end do
+  end function cress_errmsg
+
+end module m_cress_hip

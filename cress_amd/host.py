@@ -142,6 +142,8 @@ class Solver:
         check(self.L.cress_download(self.ctx, FID[name], level, out.ctypes.data), "download " + name)
         if name in FID and FID[name] < 64:
             return out.reshape(self.shape, order="F")
+        if name in ("prwtr", "price"):
+            return out.reshape(self.shape[:2] + (2, 2 if name == "prwtr" else 3), order="F")
         return out
 
     def device_ptr(self, name, level=0):

@@ -188,6 +188,11 @@ class Model:
                 raise RuntimeError("or_large_step failed: %d" % rc)
 
     def get(self, name, level=1):
+        """A prognostic at a time level (0 past, 1 present, 2 future), or the
+        precipitation arrays 'prwtr' (rate/accumulation x cloud/rain) and
+        'price' (rate/accumulation x ice/snow/graupel)."""
+        if name in ("prwtr", "price"):
+            return getattr(self, name)
         if name in QNAMES:
             return self.lv["q"][level][..., QNAMES.index(name)]
         return self.lv[name][level]

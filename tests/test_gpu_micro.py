@@ -1,0 +1,97 @@
+"""GPU parity of the large step with the cold-rain microphysics (cphopt=2)
+against the oracle, through the C ABI, on states seeded with every
+hydrometeor species and a supersaturated core.
+
+The GPU evaluates exp/log in binary64 and rounds once; the reference's FP32
+build calls glibc's expf/logf, which differ from that in ~0.06% of
+arguments, so the microphysics cannot be bitwise and the bars are relative:
+err(x) = max|x - x_ref| / max|x_ref| over the interior of each field.
+  * one large step: err vs the FP32 oracle <= 1e-5 for every field;
+  * ten large steps vs the FP64 oracle: err <= max(1e-5, 2 * err of the FP32
+    oracle itself), i.e. within the envelope of the reference's own FP32
+    arithmetic (that envelope exceeds 1e-5 for theta' and qc on this state:
+    theta' is a perturbation of ~1.5 K on a ~300 K total whose binary32
+    spacing is 3e-5 K).
+"""
+import numpy as np
+import pytest
+
+from cress_amd import cases, host
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["u", "v", "w", "pp", "ptp", "qv", "qc", "qr", "qi", "qs", "qg", "nci"]
+
+
+def interior(a, cfg):
+    return a[2:cfg.ni - 1, 2:cfg.nj - 1, 1:cfg.nk - 2]
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+
+
+def gpu_run(case, nsteps):
+    s = host.Solver(case.cfg)
+    case.load(s)
+    s.step(nsteps, first=True)
+    return s
+
+
+@pytest.mark.parametrize("name,dims", [("bubble", (12, 10, 24)), ("vortex", (16, 14, 20))])
+def test_micro_one_step_vs_fp32_oracle(name, dims):
+    case = cases.build(name, *dims, cphopt=2)
+    s = gpu_run(case, 1)
+    m = po.Model(case, 32)
+    m.step(1, first=True)
+    errs = {}
+    for f in FIELDS:
+        errs[f] = rel(interior(s.download(f, 1), case.cfg), interior(m.get(f, 1), case.cfg))
+    for f in ("prwtr", "price"):
+        errs[f] = rel(s.download(f)[1:-1, 1:-1], m.get(f)[1:-1, 1:-1])
+    s.close()
+    print(name, " ".join("%s %.1e" % kv for kv in errs.items()))
+    # the state really exercises the microphysics
+    assert np.abs(m.get("qc", 1)).max() > 1e-5 and np.abs(m.get("qi", 1)).max() > 1e-6
+    assert np.abs(m.get("prwtr")[:, :, 0, 1]).max() > 0.0
+    for f, e in errs.items():
+        assert e <= 1e-5, (f, e)
+
+
+def test_micro_ten_steps_vs_fp64_oracle():
+    case = cases.build("bubble", 12, 10, 24, cphopt=2)
+    s = gpu_run(case, 10)
+    m32 = po.Model(case, 32)
+    m32.step(10, first=True)
+    m64 = po.Model(case, 64)
+    m64.step(10, first=True)
+    out = []
+    for f in FIELDS:
+        x64 = interior(m64.get(f, 1), case.cfg)
+        eg = rel(interior(s.download(f, 1), case.cfg), x64)
+        eo = rel(interior(m32.get(f, 1), case.cfg), x64)
+        out.append((f, eg, eo))
+    s.close()
+    print(" ".join("%s gpu %.1e ref32 %.1e" % t for t in out))
+    for f, eg, eo in out:
+        assert eg <= max(1e-5, 2.0 * eo), (f, eg, eo)
+
+
+def test_micro_water_budget():
+    """Total water (rst-weighted sum of the six species over the interior)
+    after three steps agrees with the oracle's to 1e-5 relative: no species
+    loses or gains mass in the GPU's source, sedimentation or adjustment
+    kernels that the oracle's do not."""
+    case = cases.build("bubble", 12, 10, 24, cphopt=2)
+    s = gpu_run(case, 3)
+    m = po.Model(case, 32)
+    m.step(3, first=True)
+    cfg = case.cfg
+    rst = interior(case.base["rst"], cfg).astype(np.float64)
+    tot_g = sum((rst * interior(s.download(q, 1), cfg)).sum() for q in ("qv", "qc", "qr", "qi", "qs", "qg"))
+    tot_o = sum((rst * interior(m.get(q, 1), cfg)).sum() for q in ("qv", "qc", "qr", "qi", "qs", "qg"))
+    s.close()
+    assert abs(tot_g - tot_o) <= 1e-5 * abs(tot_o)
