@@ -534,7 +534,6 @@ static int large_step(cress_ctx *x, int first) {
   ac.wf = x->w[F];
   ac.wc = x->wc;
   ac.ppf = x->pp[F];
-  const size_t col_lds = (size_t)2 * (nk + 1) * 64 * sizeof(float);
   const int nring = 2 * (ni - 1) + 2 * (nj - 3);
   for (int s = 0; s < cf.nsstp; s++) {
     LAUNCH(x, KT_AC_PTIL, k_ac_ptil, grid2(ni - 1, nj - 1, 64, 4, nk - 1), b2, 0, g, o, c, divcv, B[CRESS_JCB],
@@ -544,7 +543,7 @@ static int large_step(cress_ctx *x, int first) {
                        x->ptil, x->ufrc, x->vfrc, B[CRESS_RST8U], B[CRESS_RST8V], x->u[F], x->v[F]);
     run_fill(x, fuv);
     run_vbc(x, vuv);
-    LAUNCH(x, KT_AC_COL, k_ac_col, dim3((ni - 3 + 63) / 64, nj - 3, 1), dim3(64), col_lds, g, o, c, ac);
+    LAUNCH(x, KT_AC_COL, k_ac_col, grid2(ni - 3, nj - 3, 64, 2, 1), dim3(64, 2), 0, g, o, c, ac);
     run_fill(x, fwp);
     LAUNCH(x, KT_AC_RING, k_ac_ring, dim3((nring + 63) / 64), dim3(64), 0, g, o, B[CRESS_JCB8W], x->w[F],
                        x->wc, x->pp[F]);

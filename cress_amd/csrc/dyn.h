@@ -19,7 +19,8 @@ struct ScalarIn {
 
 struct AcCol {
   float dts;
-  const float *uf, *vf, *ptil, *pfrc, *wfrc;
+  const float *uf, *vf, *pfrc, *wfrc;
+  float *ptil;  // read; its consumed slots hold the Thomas super-diagonal
   const float *jcb, *jcb8u, *jcb8v, *jcb8w, *rcsq, *rst, *rbr, *rst8w;
   float *wf, *wc, *ppf;
 };

@@ -417,19 +417,22 @@ __global__ void k_ac_uv(Grid g, Consts c, float dts, const float *jcb, const flo
 // (pgradiv.f90:143-170), buoywsi (buoywsi.f90:165-211), stepwi
 // (stepwi.f90:319-427) with the Thomas solve of gaussel (gaussel.f90:169-262),
 // phy2cnt+vbcwc+vbcw for this column, diverpiv + advbspi 'fore', steppi
-// (steppi.f90:211-223) and vbcp (vbcp.f90:126-162).  One lane per column;
-// the forward-sweep coefficients and fp live in LDS ([k][lane]).
-__global__ void __launch_bounds__(64) k_ac_col(Grid g, Opts o, Consts c, AcCol a) {
-  extern __shared__ float lds[];
-  const int lane = threadIdx.x;
+// (steppi.f90:211-223) and vbcp (vbcp.f90:126-162).
+//
+// One lane per column and no LDS, so occupancy is set by registers alone.
+// Two per-column intermediates go to global slots that nothing else reads
+// while the kernel runs: fp(k) into wc(k) (wc is written with its final
+// value in the last pass), and the Thomas super-diagonal ss'(k) into
+// ptil(k-1), whose value the column has consumed by then (ptil is read by
+// this column only, and recomputed before its next use).
+__global__ void __launch_bounds__(128) k_ac_col(Grid g, Opts o, Consts c, AcCol a) {
   const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 2 + blockIdx.y;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.ni - 2 || j > g.nj - 2) return;
   const int nk = g.nk;
-  float *sfp = lds;                       // fp(k), k = 1..nk
-  float *sss = lds + (long)(nk + 1) * 64; // normalised super-diagonal ss'(k)
-#define SFP(k) sfp[(k) * 64 + lane]
-#define SSS(k) sss[(k) * 64 + lane]
+  const long nxy = g.nxy;
+  float *ptil = a.ptil;
+  float *wf = a.wf, *wc = a.wc, *ppf = a.ppf;
   const float dts = a.dts;
   const float dtdzw = dts * c.dziv * c.weicoe;
   const float dtw = dts * c.weicoe;
@@ -437,27 +440,29 @@ __global__ void __launch_bounds__(64) k_ac_col(Grid g, Opts o, Consts c, AcCol a
   const float sbsqg5 = c.g05 * dts * dts * c.weicoe * c.weicoe;
 
   // pass 1 (upward, k=2..nk-2): fp, fw, explicit w, tridiagonal forward sweep
-  float fp_km1 = 0.0f, fpdvj_km1 = 0.0f, wb_km1 = 0.0f;
-  float fwc_km1 = 0.0f, wcc_km1 = 0.0f;
+  float fpdvj_km1 = 0.0f, wb_km1 = 0.0f, pt_km1 = 0.0f;
+  float fwc_km1 = 0.0f, wcc_km1 = 0.0f, rcsq_km1 = 0.0f, rst_km1 = 0.0f;
   float ssn_km1 = 0.0f, ff_km1 = 0.0f;
-  float wf_k = AT(a.wf, i, j, 2);
-  for (int k = 2; k <= nk - 2; k++) {
-    const long p = g.at(i, j, k);
-    const float wf_kp1 = a.wf[p + g.nxy];
+  const long p0 = g.at(i, j, 2);
+  float wf_k = wf[p0];
+  long p = p0;
+  for (int k = 2; k <= nk - 2; k++, p += nxy) {
+    const float wf_kp1 = wf[p + nxy];
     const float rcsq = a.rcsq[p], rst = a.rst[p], jcb = a.jcb[p], rbr = a.rbr[p];
     const float ju0 = a.jcb8u[p] * a.uf[p], ju1 = a.jcb8u[p + 1] * a.uf[p + 1];
     const float jv0 = a.jcb8v[p] * a.vf[p], jv1 = a.jcb8v[p + g.nx] * a.vf[p + g.nx];
     const float hdiv = rcsq * ((ju0 - ju1) * c.dxiv + (jv0 - jv1) * c.dyiv);
     const float vdiv = rcsq * (wf_k - wf_kp1) * c.dziv;
     const float fp = a.pfrc[p] + hdiv + c.weic1m * (c.g05 * (wf_k + wf_kp1) * rst + vdiv);
-    SFP(k) = fp;
+    wc[p] = fp;
+    const float pt_k = ptil[p];
     const float fpdvj = fp / jcb;
-    const float wb = (a.ppf[p] * rst + fp * rbr * dtw) / rcsq * c.g05n;
+    const float wb = (ppf[p] * rst + fp * rbr * dtw) / rcsq * c.g05n;
     const float t1 = c.g05 * rbr / rcsq;
     const float t2 = c.dziv / jcb;
     const float fwc = t1 + t2, wcc = t1 - t2;
     if (k >= 3) {
-      const float wpg = (a.ptil[p - g.nxy] - a.ptil[p]) * c.dziv;
+      const float wpg = (pt_km1 - pt_k) * c.dziv;
       float fw = a.wfrc[p] + wpg + (fpdvj_km1 - fpdvj) * dtdzw;
       if (o.buyopt == 1) fw = fw + (wb_km1 + wb);
       const float rst8w = a.rst8w[p];
@@ -465,7 +470,7 @@ __global__ void __launch_bounds__(64) k_ac_col(Grid g, Opts o, Consts c, AcCol a
       float rr, ss, tt;
       if (o.buyopt == 0) {
         const float aa = sbsqzi / rst8w;
-        const float mm = aa * a.rcsq[p - g.nxy];
+        const float mm = aa * rcsq_km1;
         const float nn = aa * rcsq;
         rr = -mm * fwc_km1;
         ss = 1.0f + (nn * fwc - mm * wcc_km1);
@@ -474,7 +479,7 @@ __global__ void __launch_bounds__(64) k_ac_col(Grid g, Opts o, Consts c, AcCol a
         const float rstiv = 1.0f / rst8w;
         const float aa = rstiv * sbsqzi;
         const float bb = rstiv * sbsqg5;
-        const float mm = bb * a.rst[p - g.nxy] - aa * a.rcsq[p - g.nxy];
+        const float mm = bb * rst_km1 - aa * rcsq_km1;
         const float nn = bb * rst + aa * rcsq;
         rr = mm * fwc_km1;
         ss = 1.0f + (mm * wcc_km1 + nn * fwc);
@@ -492,61 +497,66 @@ __global__ void __launch_bounds__(64) k_ac_col(Grid g, Opts o, Consts c, AcCol a
         ssn = tt * q;
         ffn = (wfn - rr * ff_km1) * q;
       }
-      SSS(k) = ssn;
-      a.wf[p] = ffn;
+      ptil[p - nxy] = ssn;
+      wf[p] = ffn;
       ssn_km1 = ssn;
       ff_km1 = ffn;
     }
-    fp_km1 = fp;
     fpdvj_km1 = fpdvj;
     wb_km1 = wb;
     fwc_km1 = fwc;
     wcc_km1 = wcc;
+    rcsq_km1 = rcsq;
+    rst_km1 = rst;
+    pt_km1 = pt_k;
     wf_k = wf_kp1;
   }
-  (void)fp_km1;
   // pass 2 (downward): back substitution (gaussel.f90:248-260)
   float wnext = ff_km1;  // wf(nk-2)
-  for (int k = nk - 3; k >= 3; k--) {
-    const long p = g.at(i, j, k);
-    const float wv = a.wf[p] - SSS(k) * wnext;
-    a.wf[p] = wv;
+  p = g.at(i, j, nk - 3);
+  for (int k = nk - 3; k >= 3; k--, p -= nxy) {
+    const float wv = wf[p] - ptil[p - nxy] * wnext;
+    wf[p] = wv;
     wnext = wv;
   }
-  // phy2cnt (k=2..nk-1) + vbcwc, then vbcw for this column
-  for (int k = 2; k <= nk - 1; k++) AT(a.wc, i, j, k) = wc_value(g, o, a.wf, a.jcb8w, i, j, k);
-  AT(a.wc, i, j, 1) = wc_value(g, o, a.wf, a.jcb8w, i, j, 1);
-  AT(a.wc, i, j, nk) = wc_value(g, o, a.wf, a.jcb8w, i, j, nk);
+  // phy2cnt+vbcwc at the boundary rows, from wf before vbcw
+  const float wc1 = wc_value(g, o, wf, a.jcb8w, i, j, 1);
+  const float wc2 = wc_value(g, o, wf, a.jcb8w, i, j, 2);
+  const float wcm = wc_value(g, o, wf, a.jcb8w, i, j, nk - 1);
+  const float wcn = wc_value(g, o, wf, a.jcb8w, i, j, nk);
+  // vbcw for this column (flat terrain)
   if (o.bbc == 2) {
     const float z = 0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
-    AT(a.wf, i, j, 1) = -AT(a.jcb8w, i, j, 3) * AT(a.wc, i, j, 3) - z;
-    AT(a.wf, i, j, 2) = -0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
+    AT(wf, i, j, 1) = -AT(a.jcb8w, i, j, 3) * wc_raw(g, o.sthopt, wf, a.jcb8w, i, j, 3) - z;
+    AT(wf, i, j, 2) = -0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
   } else {
-    AT(a.wf, i, j, 1) = AT(a.wf, i, j, 2);
+    AT(wf, i, j, 1) = AT(wf, i, j, 2);
   }
   if (o.tbc == 2) {
     const float z = 0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
-    AT(a.wf, i, j, nk) = -AT(a.jcb8w, i, j, nk - 2) * AT(a.wc, i, j, nk - 2) - z;
-    AT(a.wf, i, j, nk - 1) = 0.0f;
+    AT(wf, i, j, nk) = -AT(a.jcb8w, i, j, nk - 2) * wc_raw(g, o.sthopt, wf, a.jcb8w, i, j, nk - 2) - z;
+    AT(wf, i, j, nk - 1) = 0.0f;
   } else {
-    AT(a.wf, i, j, nk) = AT(a.wf, i, j, nk - 1);
+    AT(wf, i, j, nk) = AT(wf, i, j, nk - 1);
   }
-  // pass 3 (upward): diverpiv + advbspi 'fore', steppi
-  float wk = AT(a.wf, i, j, 2);
-  for (int k = 2; k <= nk - 2; k++) {
-    const long p = g.at(i, j, k);
-    const float wkp1 = a.wf[p + g.nxy];
+  // pass 3 (upward): diverpiv + advbspi 'fore', steppi; wc's final values
+  p = p0;
+  float wk = wf[p];
+  for (int k = 2; k <= nk - 2; k++, p += nxy) {
+    const float wkp1 = wf[p + nxy];
     const float vdiv = a.rcsq[p] * (wk - wkp1) * c.dziv;
-    const float fp = SFP(k) + c.weicoe * (c.g05 * (wk + wkp1) * a.rst[p] + vdiv);
-    a.ppf[p] = a.ppf[p] + dts * fp / a.jcb[p];
+    const float fp = wc[p] + c.weicoe * (c.g05 * (wk + wkp1) * a.rst[p] + vdiv);
+    ppf[p] = ppf[p] + dts * fp / a.jcb[p];
+    wc[p] = k == 2 ? wc2 : (o.sthopt == 0 ? wk : wk / a.jcb8w[p]);
     wk = wkp1;
   }
+  AT(wc, i, j, 1) = wc1;
+  AT(wc, i, j, nk - 1) = wcm;
+  AT(wc, i, j, nk) = wcn;
   // vbcp for this column
-  if (o.bbc == 2) AT(a.ppf, i, j, 1) = 2.0f * AT(a.ppf, i, j, 2) - AT(a.ppf, i, j, 3);
-  else if (o.bbc == 3) AT(a.ppf, i, j, 1) = AT(a.ppf, i, j, 2);
-  AT(a.ppf, i, j, nk - 1) = AT(a.ppf, i, j, nk - 2);
-#undef SFP
-#undef SSS
+  if (o.bbc == 2) AT(ppf, i, j, 1) = 2.0f * AT(ppf, i, j, 2) - AT(ppf, i, j, 3);
+  else if (o.bbc == 3) AT(ppf, i, j, 1) = AT(ppf, i, j, 2);
+  AT(ppf, i, j, nk - 1) = AT(ppf, i, j, nk - 2);
 }
 
 // Ring columns (i or j on the first halo row, within 1..ni-1 x 1..nj-1)
