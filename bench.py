@@ -25,11 +25,11 @@ sys.path.insert(0, ROOT)
 PEAK_HBM = 8.0e12
 
 # Algorithmic bytes per interior cell for one launch of each kernel family:
-# distinct 3-D arrays the launch must read or write once (4 B each), per the
-# counting rule of the step's bandwidth budget (metric/base arrays count as
-# 3-D, 2-D arrays and top-sponge-only reads are not counted).
+# distinct 3-D arrays the launch must read plus those it must write (4 B
+# each), with metric/base arrays counted as 3-D and 2-D arrays and
+# top-sponge-only reads left out.
 KERNEL_ARRAYS = {
-    # prep: u,v,w,rst8u,rst8v,rst8w,jcb8w -> wc,rstxu,rstxv,rstxwc
+    # u,v,w,rst8u,rst8v,rst8w,jcb8w -> wc,rstxu,rstxv,rstxwc
     "k_prep": 11,
     # u,v,up,vp,ubr,vbr,rst8u,rst8v,rst,rstxu,rstxv,rstxwc -> ufrc,vfrc
     "k_force_uv": 14,
@@ -37,19 +37,43 @@ KERNEL_ARRAYS = {
     "k_force_w": 17,
     # jcb,jcb8u,jcb8v,jcb8w,u,v,wc,pp,ppp -> pfrc
     "k_force_p": 10,
-    # per scalar s,sp -> frc (x8) + rstxu,rstxv,rstxwc,rst,qvbr,ptbr,rbr,w
-    "k_force_s": 32,
+    # 8 scalars, rstxu,rstxv,rstxwc,ptbr,rbr,w -> 8 forcings
+    "k_force_s": 22,
     # jcb,rst8u,rst8v,rst8w,uf,vf,wc,ppf -> ptil
     "k_ac_ptil": 9,
     # jcb,ptil,ufrc,vfrc,rst8u,rst8v,uf,vf -> uf,vf
     "k_ac_uv": 10,
-    # uf,vf,ptil,pfrc,wfrc,wf,ppf,jcb,jcb8u,jcb8v,jcb8w,rcsq,rst,rbr,rst8w -> wf,wc,ppf
+    # uf,vf,ptil,pfrc,wfrc,jcb,jcb8u,jcb8v,jcb8w,rcsq,rst,rbr,rst8w,wf,ppf -> wf,wc,ppf
     "k_ac_col": 18,
-    # ptpp,ptfrc,qp(6),qvfrc,ncip,rst,qf(5 forcings) -> ptpf,qf(6),ncif
+    # ptpp,ptfrc,qp(6),qvfrc,ncip,rst,qf(5 forcings),ncif -> ptpf,qf(6),ncif
     "k_steps": 25,
-    # 12 fields x (x, xp, xf) -> 12
-    "k_timeflt": 48,
+    # 12 fields x 3 levels, pbr, ptbr -> 12 present, 5 future (satadjst)
+    "k_asselin_sat": 55,
+    # jcb,pbr,ptbr,rbr, past pp,ptp,q(6),nci, future ptp,q(6),nci -> future ptp,q(6),nci, urq,usq,ugq
+    "k_micro_src": 32,
+    # rbr,rst,urq,usq,ugq, past qc,qi, future qc,qr,qi,qs,qg,nci -> future qc,qr,qi,qs,qg,nci
+    "k_micro_fall": 19,
 }
+
+# Calibrated HBM bytes per launch (FETCH_SIZE/WRITE_SIZE passes of
+# rocprofv3), one file per workload; see profiles/.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "vortex_1024x1024x80_traffic.txt")
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of a kernel from the committed counter summary
+    of the same workload, or None."""
+    try:
+        lines = open(TRAFFIC_FILE).read().splitlines()
+    except OSError:
+        return None
+    if len(lines) < 2 or lines[1].strip() != workload:
+        return None
+    for ln in lines[2:]:
+        f = ln.split()
+        if len(f) == 3 and f[0] == kernel:
+            return float(f[1]) + float(f[2])
+    return None
 
 
 def step_bytes_per_cell(nsstp, cphopt):
@@ -144,6 +168,8 @@ def main():
     per_launch_bytes = KERNEL_ARRAYS[dom] * 4.0 * cells
     achieved = per_launch_bytes / (avg_ms * 1e-3)
     step_b = step_bytes_per_cell(cfg.nsstp, cfg.cphopt)
+    traffic = measured_traffic("workload %s %dx%dx%d nsstp=%d cphopt=%d" % (
+        args.workload, nx, ny, nz, cfg.nsstp, cfg.cphopt), dom)
     print("bench: %.2f ms/step, whole-step %.2f TB/s algorithmic; kernels: %s" % (
         ms_per_step, step_b * cells / (ms_per_step * 1e-3) / 1e12,
         ", ".join("%s %.3f ms x%d" % (k, v[0] / max(v[1], 1), v[1]) for k, v in
@@ -169,7 +195,7 @@ def main():
                    "dtbig": cfg.dtbig, "cphopt": cfg.cphopt, "advopt": cfg.advopt,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
                      "step_frac": step_b * cells / (ms_per_step * 1e-3) / PEAK_HBM,
                      "step_bytes_per_cell": step_b},
         "stages_ms": {k: v / args.steps for k, v in stages.items()},

@@ -28,6 +28,19 @@ using namespace cress;
 
 enum { NSTAGE = 8 };
 
+// Launch shape of the column kernel: CRESS_COL_BY rows of 64 columns per
+// workgroup; CRESS_COL_LDS bytes of (unused) LDS per workgroup cap how many
+// column wavefronts share a CU.  Measured at 1024x1024x80 (ms per launch):
+// 8 waves/SIMD 2.84, 5 waves/SIMD (16 KiB) 2.27, 3 waves/SIMD (24 KiB)
+// 2.33, 2 waves/SIMD (40 KiB) 2.64: past ~5 waves per SIMD the sweeps
+// evict each other's neighbour rows from L2.
+#ifndef CRESS_COL_BY
+#define CRESS_COL_BY 2
+#endif
+#ifndef CRESS_COL_LDS
+#define CRESS_COL_LDS 16384
+#endif
+
 // Kernel families timed individually when timing is on (HIP events on the
 // context stream around every launch).
 enum {
@@ -102,6 +115,8 @@ static int dalloc(cress_ctx *x, float **p, size_t nfloat) {
 static int check_config(const cress_config *c) {
   if (!c || c->ni < 7 || c->nj < 7 || c->nk < 7) return CRESS_EINVAL;
   if (c->nsstp < 1 || !(c->dx > 0) || !(c->dy > 0) || !(c->dz > 0)) return CRESS_EINVAL;
+  // kernels address within a field by 32-bit byte offsets
+  if ((long)(c->ni + 2) * (long)(c->nj + 2) * (long)c->nk * 4L >= (1L << 32)) return CRESS_EUNSUPPORTED;
   if (!(c->wbc == -1 && c->ebc == -1 && c->sbc == -1 && c->nbc == -1)) return CRESS_EUNSUPPORTED;
   if ((c->bbc != 2 && c->bbc != 3) || (c->tbc != 2 && c->tbc != 3)) return CRESS_EUNSUPPORTED;
   if (c->advopt < 1 || c->advopt > 3 || c->impopt != 1 || c->gwmopt != 0) return CRESS_EUNSUPPORTED;
@@ -490,7 +505,7 @@ static int large_step(cress_ctx *x, int first) {
   si.ptbr = B[CRESS_PTBR];
   si.rbr = B[CRESS_RBR];
   si.w = x->w[C];
-  LAUNCH(x, KT_FORCE_S, k_force_s, grid2(ni - 3, nj - 3, 64, 4, nsc * (nk - 3)), b2, 0, g, o, c, si, nsc);
+  LAUNCH(x, KT_FORCE_S, k_force_s, grid2(ni - 3, nj - 3, 64, 4, nk - 3), b2, 0, g, o, c, si, nsc);
   stamp(x, 1);
 
   // ---- acoustic substeps (smalldrv / hevi) on the future level ----
@@ -543,7 +558,8 @@ static int large_step(cress_ctx *x, int first) {
                        x->ptil, x->ufrc, x->vfrc, B[CRESS_RST8U], B[CRESS_RST8V], x->u[F], x->v[F]);
     run_fill(x, fuv);
     run_vbc(x, vuv);
-    LAUNCH(x, KT_AC_COL, k_ac_col, grid2(ni - 3, nj - 3, 64, 2, 1), dim3(64, 2), 0, g, o, c, ac);
+    LAUNCH(x, KT_AC_COL, k_ac_col, grid2(ni - 3, nj - 3, 64, CRESS_COL_BY, 1), dim3(64, CRESS_COL_BY),
+           CRESS_COL_LDS, g, o, c, ac);
     run_fill(x, fwp);
     LAUNCH(x, KT_AC_RING, k_ac_ring, dim3((nring + 63) / 64), dim3(64), 0, g, o, B[CRESS_JCB8W], x->w[F],
                        x->wc, x->pp[F]);
@@ -620,40 +636,55 @@ static int large_step(cress_ctx *x, int first) {
     fa.price = x->price;
     fa.dtpmin = ma.dtpmin;
     fa.qn = n;
+    fa.n2 = (long)(ni + 2) * (nj + 2);
     fa.dtb = dtbs;
     fa.dziv = c.dziv;
     fa.thresq = c.thresq;
-    LAUNCH(x, KT_MICRO_FALL, k_micro_fall, grid2(ni - 1, nj - 1, 64, 4, 1), b2, 0, g, fa);
+    LAUNCH(x, KT_MICRO_FALL, k_micro_fall, dim3((ni - 1 + 63) / 64, nj - 1), dim3(64, 6), 0, g, fa);
   }
   stamp(x, 4);
 
-  // ---- Asselin filter (timeflt) ----
-  if (!first) {
-    AsselinSet as;
-    as.n = withq ? 12 : 6;
+  // ---- Asselin filter (timeflt), then satadjst on present and future ----
+  {
     float *xc[12] = {x->u[C], x->v[C], x->w[C], x->pp[C], x->ptp[C], x->q[C], x->q[C] + n, x->q[C] + 2 * n,
                      x->q[C] + 3 * n, x->q[C] + 4 * n, x->q[C] + 5 * n, x->nci[C]};
     float *xp[12] = {x->u[P], x->v[P], x->w[P], x->pp[P], x->ptp[P], x->q[P], x->q[P] + n, x->q[P] + 2 * n,
                      x->q[P] + 3 * n, x->q[P] + 4 * n, x->q[P] + 5 * n, x->nci[P]};
     float *xf[12] = {x->u[F], x->v[F], x->w[F], x->pp[F], x->ptp[F], x->q[F], x->q[F] + n, x->q[F] + 2 * n,
                      x->q[F] + 3 * n, x->q[F] + 4 * n, x->q[F] + 5 * n, x->nci[F]};
-    const int kinds[12] = {1, 2, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int f = 0; f < 12; f++) {
-      as.kind[f] = kinds[f];
-      as.x[f] = xc[f];
-      as.xp[f] = xp[f];
-      as.xf[f] = xf[f];
-    }
-    LAUNCH(x, KT_TIMEFLT, k_timeflt, grid2(ni, nj, 64, 4, 12 * nk), b2, 0, g, c, as);
-  }
-  stamp(x, 5);
-
-  // ---- saturation adjustment (satadjst) on present and future ----
-  if (cf.cphopt == 2) {
-    for (int lv = 0; lv < 2; lv++) {
-      const int S = lv == 0 ? C : F;
-      LAUNCH(x, KT_SATADJ, k_satadjst, grid2(ni - 1, nj - 1, 64, 4, nk - 1), b2, 0, g, c.thresq, B[CRESS_PBR],
-             B[CRESS_PTBR], x->pp[S], x->ptp[S], x->q[S], x->nci[S], n);
+    if (!first && cf.cphopt == 2) {
+      // pointwise filter and both adjustments fused per cell
+      AsselinSat as;
+      for (int f = 0; f < 12; f++) {
+        as.x[f] = xc[f];
+        as.xp[f] = xp[f];
+        as.xf[f] = xf[f];
+      }
+      as.pbr = B[CRESS_PBR];
+      as.ptbr = B[CRESS_PTBR];
+      LAUNCH(x, KT_TIMEFLT, k_asselin_sat, grid2(ni, nj, 64, 4, nk), b2, 0, g, c.m1fc2, c.filcoe, c.thresq, as);
+      stamp(x, 5);
+    } else {
+      if (!first) {
+        AsselinSet as;
+        as.n = withq ? 12 : 6;
+        const int kinds[12] = {1, 2, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int f = 0; f < 12; f++) {
+          as.kind[f] = kinds[f];
+          as.x[f] = xc[f];
+          as.xp[f] = xp[f];
+          as.xf[f] = xf[f];
+        }
+        LAUNCH(x, KT_TIMEFLT, k_timeflt, grid2(ni, nj, 64, 4, 12 * nk), b2, 0, g, c, as);
+      }
+      stamp(x, 5);
+      if (cf.cphopt == 2) {
+        for (int lv = 0; lv < 2; lv++) {
+          const int S = lv == 0 ? C : F;
+          LAUNCH(x, KT_SATADJ, k_satadjst, grid2(ni - 1, nj - 1, 64, 4, nk - 1), b2, 0, g, c.thresq,
+                 B[CRESS_PBR], B[CRESS_PTBR], x->pp[S], x->ptp[S], x->q[S], x->nci[S], n);
+        }
+      }
     }
   }
   stamp(x, 6);

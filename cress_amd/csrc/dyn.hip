@@ -12,6 +12,10 @@
 namespace cress {
 
 #define AT(a, i, j, k) a[g.at(i, j, k)]
+// Load / store at a 32-bit byte offset from a kernel-argument base: the
+// scalar-base + vector-offset addressing form, one VGPR per address.
+#define LD(arr, off) (*(const float *)((const char *)(arr) + (off)))
+#define ST(arr, off, v) (*(float *)((char *)(arr) + (off)) = (v))
 
 // rbct(1:ni,1:nj,1:nk,1:2), component 2 (vspdmp.f90:269-285)
 __device__ __forceinline__ float rbct2(const float *rbct, const Grid &g, int i, int j, int k) {
@@ -341,29 +345,87 @@ __device__ float adv_s(const Grid &g, const Opts &o, const Consts &c, const floa
   return f + vadv;
 }
 
-__global__ void k_force_s(Grid g, Opts o, Consts c, ScalarIn in, int nscalar) {
+// advs for one scalar at (i,j,k) from flux weights already in registers:
+// fu[m] = rstxu(i-1+m), fv[m] = rstxv(j-1+m), fw[m] = rstxwc(k-1+m), m=0..3.
+// Same terms and operation order as adv_s.
+__device__ __forceinline__ float adv_s_pre(const Grid &g, const Opts &o, const Consts &c, bool h4,
+                                           bool v4, const float *fu, const float *fv, const float *fw,
+                                           const float *s, uint32_t q) {
+  const uint32_t sx = 4u * (uint32_t)g.nx, sk = 4u * (uint32_t)g.nxy;
+  const float s0 = LD(s, q);
+  const float sxm1 = LD(s, q - 4), sxp1 = LD(s, q + 4);
+  const float sym1 = LD(s, q - sx), syp1 = LD(s, q + sx);
+  const float szm1 = LD(s, q - sk), szp1 = LD(s, q + sk);
+  const float t1a = fu[1] * (s0 - sxm1) * c.dxv05n, t1b = fu[2] * (sxp1 - s0) * c.dxv05n;
+  const float t2a = fv[1] * (s0 - sym1) * c.dyv05n, t2b = fv[2] * (syp1 - s0) * c.dyv05n;
+  const float t3a = fw[1] * (s0 - szm1) * c.dzv05n, t3b = fw[2] * (szp1 - s0) * c.dzv05n;
+  if (o.advopt == 1) return (t3a + t3b) + ((t1a + t1b) + (t2a + t2b));
+  float f = (t1a + t1b) + (t2a + t2b);
+  if (h4) {
+    const float sxm2 = LD(s, q - 8), sxp2 = LD(s, q + 8);
+    const float sym2 = LD(s, q - 2 * sx), syp2 = LD(s, q + 2 * sx);
+    const float T1a = (fu[0] + fu[1]) * (s0 - sxm2) * c.dxv24;
+    const float T1b = (fu[2] + fu[3]) * (sxp2 - s0) * c.dxv24;
+    const float T2a = (fv[0] + fv[1]) * (s0 - sym2) * c.dyv24;
+    const float T2b = (fv[2] + fv[3]) * (syp2 - s0) * c.dyv24;
+    f = c.fourd3 * f + ((T1a + T1b) + (T2a + T2b));
+  }
+  if (o.advopt == 2) return f + (t3a + t3b);
+  float vadv = t3a + t3b;
+  if (h4 && v4) {
+    const float szm2 = LD(s, q - 2 * sk), szp2 = LD(s, q + 2 * sk);
+    const float T3a = (fw[0] + fw[1]) * (s0 - szm2) * c.dzv24;
+    const float T3b = (fw[2] + fw[3]) * (szp2 - s0) * c.dzv24;
+    vadv = c.fourd3 * vadv + (T3a + T3b);
+  }
+  return f + vadv;
+}
+
+// forcept/forceqv/forceq for all scalars of one interior cell: advs
+// (advs.f90:205-420), the vspopt=2 sponge (vsps.f90:160-198, vspqv) and, for
+// theta', advbspt (advbspt.f90:143-190).  The flux weights are loaded once
+// and shared by the scalars.
+__global__ void __launch_bounds__(256) k_force_s(Grid g, Opts o, Consts c, ScalarIn in, int nscalar) {
   const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int nkk = g.nk - 3;
-  const int sc = blockIdx.z / nkk;
-  const int k = 2 + blockIdx.z % nkk;
-  if (i > g.ni - 2 || j > g.nj - 2 || sc >= nscalar) return;
-  float f = adv_s(g, o, c, in.s[sc], in.rstxu, in.rstxv, in.rstxwc, i, j, k);
-  const int vsp = sc == 0 ? o.vsp_pt : (sc == 1 ? o.vsp_qv : o.vsp_q);
-  if (vsp && k >= o.ksp0 - 1 && k <= g.nk - 2) {
-    const float r = 0.5f * (rbct2(in.rbct, g, i, j, k) + rbct2(in.rbct, g, i, j, k + 1));
-    const float d = sc == 1 ? AT(in.sp[sc], i, j, k) - AT(in.qvbr, i, j, k) : AT(in.sp[sc], i, j, k);
-    f = f - r * AT(in.rst, i, j, k) * d;
+  const int k = 2 + blockIdx.z;
+  if (i > g.ni - 2 || j > g.nj - 2) return;
+  const uint32_t sx = 4u * (uint32_t)g.nx, sk = 4u * (uint32_t)g.nxy;
+  const uint32_t q = 4u * (uint32_t)g.at(i, j, k);
+  float fu[4], fv[4], fw[4];
+  for (int m = 0; m < 4; m++) {
+    fu[m] = LD(in.rstxu, q + 4u * m - 4u);
+    fv[m] = LD(in.rstxv, q + sx * m - sx);
   }
-  if (sc == 0) {
-    const float dziv25 = 0.25f * c.dziv;
-    auto a8 = [&](int kk) {
-      return (AT(in.rbr, i, j, kk - 1) + AT(in.rbr, i, j, kk)) * AT(in.w, i, j, kk) *
-             (AT(in.ptbr, i, j, kk - 1) - AT(in.ptbr, i, j, kk)) * dziv25;
-    };
-    f = f + (a8(k) + a8(k + 1));
+  fw[0] = LD(in.rstxwc, q - sk);
+  fw[1] = LD(in.rstxwc, q);
+  fw[2] = LD(in.rstxwc, q + sk);
+  fw[3] = LD(in.rstxwc, q + 2 * sk);
+  const bool h4 = j >= 2 + o.js && j <= g.nj - 2 - o.jn && i >= 2 + o.iw && i <= g.ni - 2 - o.ie;
+  const bool v4 = k >= 3 && k <= g.nk - 3;
+  const bool sponge = k >= o.ksp0 - 1 && k <= g.nk - 2;
+  float r = 0.0f, rst = 0.0f;
+  if (sponge) {
+    r = 0.5f * (rbct2(in.rbct, g, i, j, k) + rbct2(in.rbct, g, i, j, k + 1));
+    rst = LD(in.rst, q);
   }
-  AT(in.frc[sc], i, j, k) = f;
+  for (int sc = 0; sc < nscalar; sc++) {
+    float f = adv_s_pre(g, o, c, h4, v4, fu, fv, fw, in.s[sc], q);
+    const int vsp = sc == 0 ? o.vsp_pt : (sc == 1 ? o.vsp_qv : o.vsp_q);
+    if (vsp && sponge) {
+      const float d = sc == 1 ? LD(in.sp[sc], q) - LD(in.qvbr, q) : LD(in.sp[sc], q);
+      f = f - r * rst * d;
+    }
+    if (sc == 0) {
+      const float dziv25 = 0.25f * c.dziv;
+      auto a8 = [&](uint32_t qq) {
+        return (LD(in.rbr, qq - sk) + LD(in.rbr, qq)) * LD(in.w, qq) * (LD(in.ptbr, qq - sk) - LD(in.ptbr, qq)) *
+               dziv25;
+      };
+      f = f + (a8(q) + a8(q + sk));
+    }
+    ST(in.frc[sc], q, f);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -430,7 +492,8 @@ __global__ void __launch_bounds__(128) k_ac_col(Grid g, Opts o, Consts c, AcCol 
   const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.ni - 2 || j > g.nj - 2) return;
   const int nk = g.nk;
-  const long nxy = g.nxy;
+  // byte offsets (a field is below 4 GiB; checked at context creation)
+  const uint32_t sk = 4u * (uint32_t)g.nxy, sx = 4u * (uint32_t)g.nx;
   float *ptil = a.ptil;
   float *wf = a.wf, *wc = a.wc, *ppf = a.ppf;
   const float dts = a.dts;
@@ -443,29 +506,29 @@ __global__ void __launch_bounds__(128) k_ac_col(Grid g, Opts o, Consts c, AcCol 
   float fpdvj_km1 = 0.0f, wb_km1 = 0.0f, pt_km1 = 0.0f;
   float fwc_km1 = 0.0f, wcc_km1 = 0.0f, rcsq_km1 = 0.0f, rst_km1 = 0.0f;
   float ssn_km1 = 0.0f, ff_km1 = 0.0f;
-  const long p0 = g.at(i, j, 2);
-  float wf_k = wf[p0];
-  long p = p0;
-  for (int k = 2; k <= nk - 2; k++, p += nxy) {
-    const float wf_kp1 = wf[p + nxy];
-    const float rcsq = a.rcsq[p], rst = a.rst[p], jcb = a.jcb[p], rbr = a.rbr[p];
-    const float ju0 = a.jcb8u[p] * a.uf[p], ju1 = a.jcb8u[p + 1] * a.uf[p + 1];
-    const float jv0 = a.jcb8v[p] * a.vf[p], jv1 = a.jcb8v[p + g.nx] * a.vf[p + g.nx];
+  const uint32_t q0 = 4u * (uint32_t)g.at(i, j, 2);
+  float wf_k = LD(wf, q0);
+  uint32_t q = q0;
+  for (int k = 2; k <= nk - 2; k++, q += sk) {
+    const float wf_kp1 = LD(wf, q + sk);
+    const float rcsq = LD(a.rcsq, q), rst = LD(a.rst, q), jcb = LD(a.jcb, q), rbr = LD(a.rbr, q);
+    const float ju0 = LD(a.jcb8u, q) * LD(a.uf, q), ju1 = LD(a.jcb8u, q + 4) * LD(a.uf, q + 4);
+    const float jv0 = LD(a.jcb8v, q) * LD(a.vf, q), jv1 = LD(a.jcb8v, q + sx) * LD(a.vf, q + sx);
     const float hdiv = rcsq * ((ju0 - ju1) * c.dxiv + (jv0 - jv1) * c.dyiv);
     const float vdiv = rcsq * (wf_k - wf_kp1) * c.dziv;
-    const float fp = a.pfrc[p] + hdiv + c.weic1m * (c.g05 * (wf_k + wf_kp1) * rst + vdiv);
-    wc[p] = fp;
-    const float pt_k = ptil[p];
+    const float fp = LD(a.pfrc, q) + hdiv + c.weic1m * (c.g05 * (wf_k + wf_kp1) * rst + vdiv);
+    ST(wc, q, fp);
+    const float pt_k = LD(ptil, q);
     const float fpdvj = fp / jcb;
-    const float wb = (ppf[p] * rst + fp * rbr * dtw) / rcsq * c.g05n;
+    const float wb = (LD(ppf, q) * rst + fp * rbr * dtw) / rcsq * c.g05n;
     const float t1 = c.g05 * rbr / rcsq;
     const float t2 = c.dziv / jcb;
     const float fwc = t1 + t2, wcc = t1 - t2;
     if (k >= 3) {
       const float wpg = (pt_km1 - pt_k) * c.dziv;
-      float fw = a.wfrc[p] + wpg + (fpdvj_km1 - fpdvj) * dtdzw;
+      float fw = LD(a.wfrc, q) + wpg + (fpdvj_km1 - fpdvj) * dtdzw;
       if (o.buyopt == 1) fw = fw + (wb_km1 + wb);
-      const float rst8w = a.rst8w[p];
+      const float rst8w = LD(a.rst8w, q);
       float wfn = wf_k + dts * fw / rst8w;
       float rr, ss, tt;
       if (o.buyopt == 0) {
@@ -489,16 +552,16 @@ __global__ void __launch_bounds__(128) k_ac_col(Grid g, Opts o, Consts c, AcCol 
       // Thomas forward sweep (gaussel.f90:189-236), kstr=3, kend=nk-2
       float ssn, ffn;
       if (k == 3) {
-        const float q = 1.0f / ss;
-        ssn = tt * q;
-        ffn = wfn * q;
+        const float r = 1.0f / ss;
+        ssn = tt * r;
+        ffn = wfn * r;
       } else {
-        const float q = 1.0f / (ss - rr * ssn_km1);
-        ssn = tt * q;
-        ffn = (wfn - rr * ff_km1) * q;
+        const float r = 1.0f / (ss - rr * ssn_km1);
+        ssn = tt * r;
+        ffn = (wfn - rr * ff_km1) * r;
       }
-      ptil[p - nxy] = ssn;
-      wf[p] = ffn;
+      ST(ptil, q - sk, ssn);
+      ST(wf, q, ffn);
       ssn_km1 = ssn;
       ff_km1 = ffn;
     }
@@ -511,24 +574,38 @@ __global__ void __launch_bounds__(128) k_ac_col(Grid g, Opts o, Consts c, AcCol 
     pt_km1 = pt_k;
     wf_k = wf_kp1;
   }
-  // pass 2 (downward): back substitution (gaussel.f90:248-260)
-  float wnext = ff_km1;  // wf(nk-2)
-  p = g.at(i, j, nk - 3);
-  for (int k = nk - 3; k >= 3; k--, p -= nxy) {
-    const float wv = wf[p] - ptil[p - nxy] * wnext;
-    wf[p] = wv;
-    wnext = wv;
-  }
-  // phy2cnt+vbcwc at the boundary rows, from wf before vbcw
-  const float wc1 = wc_value(g, o, wf, a.jcb8w, i, j, 1);
+  // pass 2 (downward, k=nk-2..2): back substitution (gaussel.f90:248-260)
+  // fused with diverpiv + advbspi 'fore' and steppi, whose per-level update
+  // of ppf needs only w(k), w(k+1) and fp(k), in any order of k.  The
+  // levels 2 and nk-1 take w after vbcw; wc gets its final value.
   const float wc2 = wc_value(g, o, wf, a.jcb8w, i, j, 2);
   const float wcm = wc_value(g, o, wf, a.jcb8w, i, j, nk - 1);
-  const float wcn = wc_value(g, o, wf, a.jcb8w, i, j, nk);
-  // vbcw for this column (flat terrain)
+  const float w2 = o.bbc == 2 ? -0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f)) : AT(wf, i, j, 2);
+  const float wm = o.tbc == 2 ? 0.0f : AT(wf, i, j, nk - 1);
+  float wkp1 = wm;
+  float wk = ff_km1;   // wf(nk-2): the last forward-sweep value is final
+  q = q0 + (uint32_t)(nk - 4) * sk;
+  for (int k = nk - 2; k >= 2; k--, q -= sk) {
+    if (k <= nk - 3 && k >= 3) {
+      wk = LD(wf, q) - LD(ptil, q - sk) * wkp1;
+      ST(wf, q, wk);
+    } else if (k == 2) {
+      wk = w2;
+    }
+    const float vdiv = LD(a.rcsq, q) * (wk - wkp1) * c.dziv;
+    const float fp = LD(wc, q) + c.weicoe * (c.g05 * (wk + wkp1) * LD(a.rst, q) + vdiv);
+    ST(ppf, q, LD(ppf, q) + dts * fp / LD(a.jcb, q));
+    ST(wc, q, k == 2 ? wc2 : (o.sthopt == 0 ? wk : wk / LD(a.jcb8w, q)));
+    wkp1 = wk;
+  }
+  // phy2cnt+vbcwc at the outer rows and vbcw, from the solved column
+  AT(wc, i, j, nk - 1) = wcm;
+  AT(wc, i, j, 1) = wc_value(g, o, wf, a.jcb8w, i, j, 1);
+  AT(wc, i, j, nk) = wc_value(g, o, wf, a.jcb8w, i, j, nk);
   if (o.bbc == 2) {
     const float z = 0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
     AT(wf, i, j, 1) = -AT(a.jcb8w, i, j, 3) * wc_raw(g, o.sthopt, wf, a.jcb8w, i, j, 3) - z;
-    AT(wf, i, j, 2) = -0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
+    AT(wf, i, j, 2) = w2;
   } else {
     AT(wf, i, j, 1) = AT(wf, i, j, 2);
   }
@@ -539,20 +616,6 @@ __global__ void __launch_bounds__(128) k_ac_col(Grid g, Opts o, Consts c, AcCol 
   } else {
     AT(wf, i, j, nk) = AT(wf, i, j, nk - 1);
   }
-  // pass 3 (upward): diverpiv + advbspi 'fore', steppi; wc's final values
-  p = p0;
-  float wk = wf[p];
-  for (int k = 2; k <= nk - 2; k++, p += nxy) {
-    const float wkp1 = wf[p + nxy];
-    const float vdiv = a.rcsq[p] * (wk - wkp1) * c.dziv;
-    const float fp = wc[p] + c.weicoe * (c.g05 * (wk + wkp1) * a.rst[p] + vdiv);
-    ppf[p] = ppf[p] + dts * fp / a.jcb[p];
-    wc[p] = k == 2 ? wc2 : (o.sthopt == 0 ? wk : wk / a.jcb8w[p]);
-    wk = wkp1;
-  }
-  AT(wc, i, j, 1) = wc1;
-  AT(wc, i, j, nk - 1) = wcm;
-  AT(wc, i, j, nk) = wcn;
   // vbcp for this column
   if (o.bbc == 2) AT(ppf, i, j, 1) = 2.0f * AT(ppf, i, j, 2) - AT(ppf, i, j, 3);
   else if (o.bbc == 3) AT(ppf, i, j, 1) = AT(ppf, i, j, 2);

@@ -38,8 +38,18 @@ struct FallArgs {
   const float *rbr, *rst, *qp, *urq, *usq, *ugq;
   float *qf, *ncif, *prwtr, *price;
   const unsigned *dtpmin;
-  long qn;
+  long qn;   // floats per species block
+  long n2;   // floats per 2-D level of the precipitation arrays
   float dtb, dziv, thresq;
+};
+
+// The twelve prognostics at three levels (u, v, w, pp, ptp, qv, qc, qr, qi,
+// qs, qg, nci) and the base state the saturation adjustment needs.
+struct AsselinSat {
+  float *x[12];
+  const float *xp[12];
+  float *xf[12];
+  const float *pbr, *ptbr;
 };
 
 // Koenig table into constant memory (once per process and device set).
@@ -49,6 +59,7 @@ MicroK micro_fold(float dtb_micro, float thresq, float dz);
 __global__ void k_micro_seed(unsigned *p, unsigned v);
 __global__ void k_micro_src(Grid g, MicroK K, MicroArgs a);
 __global__ void k_micro_fall(Grid g, FallArgs a);
+__global__ void k_asselin_sat(Grid g, float m1fc2, float filcoe, float thresq, AsselinSat a);
 __global__ void k_satadjst(Grid g, float thresq, const float *pbr, const float *ptbr, const float *pp, float *ptp,
                            float *q, float *nci, long qn);
 

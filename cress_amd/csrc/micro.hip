@@ -53,6 +53,9 @@ __constant__ float c_pkoe[41];
 __device__ __forceinline__ float ex(float x) { return (float)exp((double)x); }
 __device__ __forceinline__ float lg(float x) { return (float)log((double)x); }
 
+#define LD(arr, off) (*(const float *)((const char *)(arr) + (off)))
+#define ST(arr, off, v) (*(float *)((char *)(arr) + (off)) = (v))
+
 __device__ __forceinline__ float fminr(float a, float b) { return a < b ? a : b; }
 __device__ __forceinline__ float fmaxr(float a, float b) { return a > b ? a : b; }
 
@@ -467,39 +470,17 @@ __global__ void __launch_bounds__(256) k_micro_src(Grid g, MicroK K, MicroArgs a
   if (tid == 0) atomicMin(a.dtpmin, __float_as_uint(red[0]));
 }
 
-// One upwind sedimentation column sweep of fallblk for a mixing ratio (with
-// precipitation, upwqp.f90:164-202) or a number (upwnp.f90:143-176).
-__device__ __forceinline__ void upw_column(const Grid &g, int i, int j, float dtp, float dziv, const float *rbr,
-                                           const float *rst, const float *u, float *q, float *precip,
-                                           long n2) {
-  const int nk = g.nk;
-  const float dzvdt = dziv * dtp;
-  long p = g.at(i, j, 1);
-  float flx_k = rbr[p] * u[p] * q[p];
-  const float flx1 = flx_k;
-  float flx2 = 0.0f;
-  for (int k = 1; k <= nk - 2; k++) {
-    const float flx_kp1 = rbr[p + g.nxy] * u[p + g.nxy] * q[p + g.nxy];
-    if (k == 1) flx2 = flx_kp1;
-    q[p] = fmaxr(q[p] + (flx_kp1 - flx_k) / rst[p] * dzvdt, 0.0f);
-    flx_k = flx_kp1;
-    p += g.nxy;
-  }
-  q[g.at(i, j, nk - 1)] = q[g.at(i, j, nk - 2)];
-  if (precip) {
-    const long q2 = (long)i + g.nx * j;
-    const float rate = (flx1 + flx2) * (0.5f / mc::rhow);
-    precip[q2] = rate;
-    precip[q2 + n2] = precip[q2 + n2] + rate * (0.5f * dtp);
-  }
-}
-
-// fallblk.f90:439-541 for one column (i,j) in 1..ni-1 x 1..nj-1, then the
-// trailing adjstnci (coldblk.f90:463-465).
-__global__ void k_micro_fall(Grid g, FallArgs a) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  if (i > g.ni - 1 || j > g.nj - 1) return;
+// fallblk.f90:439-541 on the columns (i,j), i in 1..ni-1, j = 1+blockIdx.y,
+// one wavefront per species (threadIdx.y: qc, qi, nci, qr, qs, qg), each
+// doing its npstp upwind sub-steps (upwqp.f90:164-202 with the surface
+// precipitation rate and accumulation; upwnp.f90:143-176 for nci); then the
+// trailing adjstnci (coldblk.f90:463-465) once qi is final.  Species are
+// independent in fallblk, so the reference's species order does not matter.
+__global__ void __launch_bounds__(384) k_micro_fall(Grid g, FallArgs a) {
+  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+  const int j = 1 + blockIdx.y;
+  const int sp = threadIdx.y;
+  const bool in = i <= g.ni - 1;
   const float dtb = a.dtb;
   float dtp = fminr(__uint_as_float(*a.dtpmin), dtb);
   int npstp;
@@ -511,67 +492,73 @@ __global__ void k_micro_fall(Grid g, FallArgs a) {
     npstp = 1;
     dtp = dtb;
   }
-  const long n2 = g.nx * (long)(g.nj + 2);
+  const uint32_t sk = 4u * (uint32_t)g.nxy;
+  const uint32_t q1 = 4u * (uint32_t)g.at(i, j, 1);
   const long qn = a.qn;
-  // cloud water and ice fall at constant speeds where present on the past
-  // level (termblk.f90:303-330), so their speeds are recomputed here
-  float *qc = a.qf + qn, *qr = a.qf + 2 * qn, *qi = a.qf + 3 * qn, *qs = a.qf + 4 * qn, *qg = a.qf + 5 * qn;
-  for (int s = 0; s < npstp; s++) {
-    // qc, qi (upwqp) and nci (upwnp) with u from the past level
-    for (int sp = 0; sp < 3; sp++) {
-      float *q = sp == 0 ? qc : (sp == 1 ? qi : a.ncif);
-      const float *qpres = sp == 0 ? a.qp + qn : a.qp + 3 * qn;
-      const float ucst = sp == 0 ? mc::ucqcst : (sp == 1 ? mc::uiqcst : mc::uincst);
-      float *prec = sp == 0 ? a.prwtr : (sp == 1 ? a.price : nullptr);
-      const float dzvdt = a.dziv * dtp;
-      long p = g.at(i, j, 1);
-      float u_k = qpres[p] > a.thresq ? ucst : 0.0f;
-      float flx_k = a.rbr[p] * u_k * q[p];
+  if (in) {
+    // species table: the future array, its fall speed and precipitation;
+    // cloud water and ice fall at constant speeds where present on the past
+    // level (termblk.f90:303-330)
+    float *q;
+    const float *ufield = nullptr, *qpres = nullptr;
+    float ucst = 0.0f, *prec = nullptr;
+    switch (sp) {
+      case 0: q = a.qf + qn; qpres = a.qp + qn; ucst = mc::ucqcst; prec = a.prwtr; break;
+      case 1: q = a.qf + 3 * qn; qpres = a.qp + 3 * qn; ucst = mc::uiqcst; prec = a.price; break;
+      case 2: q = a.ncif; qpres = a.qp + 3 * qn; ucst = mc::uincst; break;
+      case 3: q = a.qf + 2 * qn; ufield = a.urq; prec = a.prwtr + 2 * a.n2; break;
+      case 4: q = a.qf + 4 * qn; ufield = a.usq; prec = a.price + 2 * a.n2; break;
+      default: q = a.qf + 5 * qn; ufield = a.ugq; prec = a.price + 4 * a.n2; break;
+    }
+    const float thresq = a.thresq;
+    auto speed = [&](uint32_t o) {
+      return ufield ? LD(ufield, o) : (LD(qpres, o) > thresq ? ucst : 0.0f);
+    };
+    const float dzvdt = a.dziv * dtp;
+    for (int s = 0; s < npstp; s++) {
+      uint32_t o = q1;
+      float q_k = LD(q, o);
+      float flx_k = LD(a.rbr, o) * speed(o) * q_k;
       const float flx1 = flx_k;
       float flx2 = 0.0f;
-      for (int k = 1; k <= g.nk - 2; k++) {
-        const long pu = p + g.nxy;
-        const float u_kp1 = qpres[pu] > a.thresq ? ucst : 0.0f;
-        const float flx_kp1 = a.rbr[pu] * u_kp1 * q[pu];
+      for (int k = 1; k <= g.nk - 2; k++, o += sk) {
+        const float q_kp1 = LD(q, o + sk);
+        const float flx_kp1 = LD(a.rbr, o + sk) * speed(o + sk) * q_kp1;
         if (k == 1) flx2 = flx_kp1;
-        q[p] = fmaxr(q[p] + (flx_kp1 - flx_k) / a.rst[p] * dzvdt, 0.0f);
+        q_k = fmaxr(q_k + (flx_kp1 - flx_k) / LD(a.rst, o) * dzvdt, 0.0f);
+        ST(q, o, q_k);
         flx_k = flx_kp1;
-        p = pu;
+        q_k = q_kp1;
       }
-      q[g.at(i, j, g.nk - 1)] = q[g.at(i, j, g.nk - 2)];
+      // top copy: q(nk-1) = q(nk-2) after the update
+      ST(q, q1 + (uint32_t)(g.nk - 2) * sk, LD(q, q1 + (uint32_t)(g.nk - 3) * sk));
       if (prec) {
-        const long q2 = (long)i + g.nx * j;
+        const long c2 = (long)i + g.nx * j;
         const float rate = (flx1 + flx2) * (0.5f / mc::rhow);
-        prec[q2] = rate;
-        prec[q2 + n2] = prec[q2 + n2] + rate * (0.5f * dtp);
+        prec[c2] = rate;
+        prec[c2 + a.n2] = prec[c2 + a.n2] + rate * (0.5f * dtp);
       }
     }
   }
-  for (int s = 0; s < npstp; s++) {
-    upw_column(g, i, j, dtp, a.dziv, a.rbr, a.rst, a.urq, qr, a.prwtr + 2 * n2, n2);
-    upw_column(g, i, j, dtp, a.dziv, a.rbr, a.rst, a.usq, qs, a.price + 2 * n2, n2);
-    upw_column(g, i, j, dtp, a.dziv, a.rbr, a.rst, a.ugq, qg, a.price + 4 * n2, n2);
-  }
-  for (int k = 1; k <= g.nk - 1; k++) {
-    const long p = g.at(i, j, k);
-    a.ncif[p] = fminr(fmaxr(a.ncif[p], (0.5f / mc::mimax) * qi[p]), (1.e2f / mc::mi0) * qi[p]);
+  __syncthreads();
+  if (in && sp == 2) {
+    const float *qi = a.qf + 3 * qn;
+    uint32_t o = q1;
+    for (int k = 1; k <= g.nk - 1; k++, o += sk) {
+      const float qv = LD(qi, o);
+      ST(a.ncif, o, fminr(fmaxr(LD(a.ncif, o), (0.5f / mc::mimax) * qv), (1.e2f / mc::mi0) * qv));
+    }
   }
 }
 
 // s_satadjst for one cell: getexner (getexner.f90:123-131), siadjst
-// (siadjst.f90:159-269) and swadjst (swadjst.f90:237-310, cphopt<=3).
-__global__ void k_satadjst(Grid g, float thresq, const float *pbr, const float *ptbr, const float *pp,
-                           float *ptp, float *q, float *nci, long qn) {
+// (siadjst.f90:159-269) and swadjst (swadjst.f90:237-310, cphopt<=3), on
+// the cell's theta', qv, qc, qi, nci in registers.
+__device__ __forceinline__ void satadj_cell(float thresq, float pbr, float pp, float tb, float &th, float &qv,
+                                            float &qc, float &qi, float &ni) {
   using namespace mc;
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 1 + blockIdx.z;
-  if (i > g.ni - 1 || j > g.nj - 1) return;
-  const long p = g.at(i, j, k);
-  const float pr = pbr[p] + pp[p];
+  const float pr = pbr + pp;
   const float pi = ex((rd / cp) * lg((1.0f / p0) * pr));
-  const float tb = ptbr[p];
-  float th = ptp[p], qv = q[p], qc = q[p + qn], qi = q[p + 3 * qn], ni = nci[p];
   // ice
   float t = (tb + th) * pi;
   if (t <= tlow) {
@@ -622,11 +609,64 @@ __global__ void k_satadjst(Grid g, float thresq, const float *pbr, const float *
       qc = 0.0f;
     }
   }
+}
+
+// s_satadjst on one level over i=1..ni-1, j=1..nj-1, k=1..nk-1.
+__global__ void k_satadjst(Grid g, float thresq, const float *pbr, const float *ptbr, const float *pp,
+                           float *ptp, float *q, float *nci, long qn) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.ni - 1 || j > g.nj - 1) return;
+  const long p = g.at(i, j, k);
+  float th = ptp[p], qv = q[p], qc = q[p + qn], qi = q[p + 3 * qn], ni = nci[p];
+  satadj_cell(thresq, pbr[p], pp[p], ptbr[p], th, qv, qc, qi, ni);
   ptp[p] = th;
   q[p] = qv;
   q[p + qn] = qc;
   q[p + 3 * qn] = qi;
   nci[p] = ni;
+}
+
+// s_timeflt (timeflt.f90:348-484) for the twelve prognostics of one cell,
+// x = (1-2c) x + c (x_f + x_p) over each field's range (u to i=ni, v to
+// j=nj, w to k=nk, the rest to ni-1, nj-1, nk-1), followed, inside
+// i<=ni-1, j<=nj-1, k<=nk-1, by s_satadjst on the filtered present level
+// and on the future level (intgdrv.f90:968-998).  Every step is pointwise,
+// so doing them per cell in this order is the reference's result.
+// Fields: u, v, w, pp, ptp, qv, qc, qr, qi, qs, qg, nci.
+__global__ void __launch_bounds__(256) k_asselin_sat(Grid g, float m1fc2, float filcoe, float thresq,
+                                                     AsselinSat a) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.ni || j > g.nj) return;
+  const uint32_t q = 4u * (uint32_t)g.at(i, j, k);
+  const bool in = i <= g.ni - 1 && j <= g.nj - 1 && k <= g.nk - 1;
+  float x[12], xf[12];
+  for (int f = 0; f < 12; f++) {
+    const bool ok = in || (f == 0 && i == g.ni && j <= g.nj - 1 && k <= g.nk - 1) ||
+                    (f == 1 && j == g.nj && i <= g.ni - 1 && k <= g.nk - 1) ||
+                    (f == 2 && k == g.nk && i <= g.ni - 1 && j <= g.nj - 1);
+    if (!ok) continue;
+    xf[f] = LD(a.xf[f], q);
+    x[f] = m1fc2 * LD(a.x[f], q) + filcoe * (xf[f] + LD(a.xp[f], q));
+    if (!in || f <= 3 || f == 7 || f == 9 || f == 10) ST(a.x[f], q, x[f]);
+  }
+  if (!in) return;
+  const float pbr = LD(a.pbr, q), tb = LD(a.ptbr, q);
+  satadj_cell(thresq, pbr, x[3], tb, x[4], x[5], x[6], x[8], x[11]);
+  ST(a.x[4], q, x[4]);
+  ST(a.x[5], q, x[5]);
+  ST(a.x[6], q, x[6]);
+  ST(a.x[8], q, x[8]);
+  ST(a.x[11], q, x[11]);
+  satadj_cell(thresq, pbr, xf[3], tb, xf[4], xf[5], xf[6], xf[8], xf[11]);
+  ST(a.xf[4], q, xf[4]);
+  ST(a.xf[5], q, xf[5]);
+  ST(a.xf[6], q, xf[6]);
+  ST(a.xf[8], q, xf[8]);
+  ST(a.xf[11], q, xf[11]);
 }
 
 __global__ void k_micro_seed(unsigned *p, unsigned v) {
