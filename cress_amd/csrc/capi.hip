@@ -449,7 +449,7 @@ static int large_step(cress_ctx *x, int first) {
 
   stamp(x, 0);
   // ---- forcing (forcedrv) on the present level ----
-  LAUNCH(x, KT_PREP, k_prep, grid2(ni + 2, nj + 2, 64, 4, nk), b2, 0, g, o, x->u[C], x->v[C],
+  LAUNCH(x, KT_PREP, k_prep, flat(tiles(ni + 2, nj + 2, 64, 4, nk)), b2, 0, tiles(ni + 2, nj + 2, 64, 4, nk), g, o, x->u[C], x->v[C],
                      x->w[C], B[CRESS_RST8U], B[CRESS_RST8V], B[CRESS_RST8W], B[CRESS_JCB8W], x->wc,
                      x->rstxu, x->rstxv, x->rstxwc);
   ForceIn fi;
@@ -475,10 +475,10 @@ static int large_step(cress_ctx *x, int first) {
   fi.fc = x->fc;
   fi.rbct = x->rbct;
   fi.qn = n;
-  LAUNCH(x, KT_FORCE_UV, k_force_uv, grid2(ni - 2, nj - 2, 64, 4, nk - 3), b2, 0, g, o, c, fi, x->ufrc,
+  LAUNCH(x, KT_FORCE_UV, k_force_uv, flat(tiles(ni - 2, nj - 2, 64, 4, nk - 3, 2)), b2, 0, tiles(ni - 2, nj - 2, 64, 4, nk - 3, 2), g, o, c, fi, x->ufrc,
                      x->vfrc);
-  LAUNCH(x, KT_FORCE_W, k_force_w, grid2(ni - 3, nj - 3, 64, 4, nk - 2), b2, 0, g, o, c, fi, x->wfrc);
-  LAUNCH(x, KT_FORCE_P, k_force_p, grid2(ni - 3, nj - 3, 64, 4, nk - 3), b2, 0, g, o, c, B[CRESS_JCB],
+  LAUNCH(x, KT_FORCE_W, k_force_w, flat(tiles(ni - 3, nj - 3, 64, 4, nk - 2, 2)), b2, 0, tiles(ni - 3, nj - 3, 64, 4, nk - 2, 2), g, o, c, fi, x->wfrc);
+  LAUNCH(x, KT_FORCE_P, k_force_p, flat(tiles(ni - 3, nj - 3, 64, 4, nk - 3, 2)), b2, 0, tiles(ni - 3, nj - 3, 64, 4, nk - 3, 2), g, o, c, B[CRESS_JCB],
                      B[CRESS_JCB8U], B[CRESS_JCB8V], B[CRESS_JCB8W], x->u[C], x->v[C], x->wc, x->pp[C],
                      x->pp[P], x->rbct, x->pfrc);
   ScalarIn si;
@@ -505,7 +505,7 @@ static int large_step(cress_ctx *x, int first) {
   si.ptbr = B[CRESS_PTBR];
   si.rbr = B[CRESS_RBR];
   si.w = x->w[C];
-  LAUNCH(x, KT_FORCE_S, k_force_s, grid2(ni - 3, nj - 3, 64, 4, nk - 3), b2, 0, g, o, c, si, nsc);
+  LAUNCH(x, KT_FORCE_S, k_force_s, flat(tiles(ni - 3, nj - 3, 64, 4, nk - 3, 2)), b2, 0, tiles(ni - 3, nj - 3, 64, 4, nk - 3, 2), g, o, c, si, nsc);
   stamp(x, 1);
 
   // ---- acoustic substeps (smalldrv / hevi) on the future level ----
@@ -513,7 +513,7 @@ static int large_step(cress_ctx *x, int first) {
   HIPCHK(hipMemcpyAsync(x->v[F], x->v[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(x->w[F], x->w[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(x->pp[F], x->pp[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
-  LAUNCH(x, KT_PHY2CNT, k_phy2cnt, grid2(ni - 1, nj - 1, 64, 4, nk), b2, 0, g, o, x->w[F], B[CRESS_JCB8W],
+  LAUNCH(x, KT_PHY2CNT, k_phy2cnt, flat(tiles(ni - 1, nj - 1, 64, 4, nk)), b2, 0, tiles(ni - 1, nj - 1, 64, 4, nk), g, o, x->w[F], B[CRESS_JCB8W],
                      x->wc);
   const float dz = (float)cf.dz;
   const float divcv = 0.05f / dts * dz * dz;
@@ -551,15 +551,15 @@ static int large_step(cress_ctx *x, int first) {
   ac.ppf = x->pp[F];
   const int nring = 2 * (ni - 1) + 2 * (nj - 3);
   for (int s = 0; s < cf.nsstp; s++) {
-    LAUNCH(x, KT_AC_PTIL, k_ac_ptil, grid2(ni - 1, nj - 1, 64, 4, nk - 1), b2, 0, g, o, c, divcv, B[CRESS_JCB],
+    LAUNCH(x, KT_AC_PTIL, k_ac_ptil, flat(tiles(ni - 1, nj - 1, 64, 4, nk - 1, 2)), b2, 0, tiles(ni - 1, nj - 1, 64, 4, nk - 1, 2), g, o, c, divcv, B[CRESS_JCB],
                        B[CRESS_RST8U], B[CRESS_RST8V], B[CRESS_RST8W], x->u[F], x->v[F], x->wc, x->pp[F],
                        x->ptil);
-    LAUNCH(x, KT_AC_UV, k_ac_uv, grid2(ni - 2, nj - 2, 64, 4, nk - 3), b2, 0, g, c, dts, B[CRESS_JCB],
+    LAUNCH(x, KT_AC_UV, k_ac_uv, flat(tiles(ni - 2, nj - 2, 64, 4, nk - 3)), b2, 0, tiles(ni - 2, nj - 2, 64, 4, nk - 3), g, c, dts, B[CRESS_JCB],
                        x->ptil, x->ufrc, x->vfrc, B[CRESS_RST8U], B[CRESS_RST8V], x->u[F], x->v[F]);
     run_fill(x, fuv);
     run_vbc(x, vuv);
-    LAUNCH(x, KT_AC_COL, k_ac_col, grid2(ni - 3, nj - 3, 64, CRESS_COL_BY, 1), dim3(64, CRESS_COL_BY),
-           CRESS_COL_LDS, g, o, c, ac);
+    LAUNCH(x, KT_AC_COL, k_ac_col, flat(tiles(ni - 3, nj - 3, 64, CRESS_COL_BY, 1, 1)), dim3(64, CRESS_COL_BY),
+           CRESS_COL_LDS, tiles(ni - 3, nj - 3, 64, CRESS_COL_BY, 1, 1), g, o, c, ac);
     run_fill(x, fwp);
     LAUNCH(x, KT_AC_RING, k_ac_ring, dim3((nring + 63) / 64), dim3(64), 0, g, o, B[CRESS_JCB8W], x->w[F],
                        x->wc, x->pp[F]);
@@ -578,7 +578,7 @@ static int large_step(cress_ctx *x, int first) {
   sti.ncif = x->nci[F];
   sti.qn = n;
   const int withq = cf.cphopt == 2;
-  LAUNCH(x, KT_STEPS, k_steps, grid2(ni - 3, nj - 3, 64, 4, nk - 3), b2, 0, g, 2.0f * dtb, B[CRESS_RST], sti,
+  LAUNCH(x, KT_STEPS, k_steps, flat(tiles(ni - 3, nj - 3, 64, 4, nk - 3)), b2, 0, tiles(ni - 3, nj - 3, 64, 4, nk - 3), g, 2.0f * dtb, B[CRESS_RST], sti,
                      withq);
   {
     FillSet fs;
@@ -622,7 +622,7 @@ static int large_step(cress_ctx *x, int first) {
     unsigned seed;
     memcpy(&seed, &dtbs, sizeof seed);
     LAUNCH(x, KT_MICRO_SEED, k_micro_seed, dim3(1), dim3(64), 0, ma.dtpmin, seed);
-    LAUNCH(x, KT_MICRO_SRC, k_micro_src, grid2(ni - 1, nj - 1, 64, 4, nk - 1), b2, 0, g, mk, ma);
+    LAUNCH(x, KT_MICRO_SRC, k_micro_src, flat(tiles(ni - 1, nj - 1, 64, 4, nk - 1)), b2, 0, tiles(ni - 1, nj - 1, 64, 4, nk - 1), g, mk, ma);
     FallArgs fa;
     fa.rbr = B[CRESS_RBR];
     fa.rst = B[CRESS_RST];
@@ -640,7 +640,8 @@ static int large_step(cress_ctx *x, int first) {
     fa.dtb = dtbs;
     fa.dziv = c.dziv;
     fa.thresq = c.thresq;
-    LAUNCH(x, KT_MICRO_FALL, k_micro_fall, dim3((ni - 1 + 63) / 64, nj - 1), dim3(64, 6), 0, g, fa);
+    LAUNCH(x, KT_MICRO_FALL, k_micro_fall, flat(tiles(ni - 1, nj - 1, 64, 1, 1, 1)), dim3(64, 6), 0,
+           tiles(ni - 1, nj - 1, 64, 1, 1, 1), g, fa);
   }
   stamp(x, 4);
 
@@ -662,7 +663,7 @@ static int large_step(cress_ctx *x, int first) {
       }
       as.pbr = B[CRESS_PBR];
       as.ptbr = B[CRESS_PTBR];
-      LAUNCH(x, KT_TIMEFLT, k_asselin_sat, grid2(ni, nj, 64, 4, nk), b2, 0, g, c.m1fc2, c.filcoe, c.thresq, as);
+      LAUNCH(x, KT_TIMEFLT, k_asselin_sat, flat(tiles(ni, nj, 64, 4, nk)), b2, 0, tiles(ni, nj, 64, 4, nk), g, c.m1fc2, c.filcoe, c.thresq, as);
       stamp(x, 5);
     } else {
       if (!first) {
@@ -675,13 +676,13 @@ static int large_step(cress_ctx *x, int first) {
           as.xp[f] = xp[f];
           as.xf[f] = xf[f];
         }
-        LAUNCH(x, KT_TIMEFLT, k_timeflt, grid2(ni, nj, 64, 4, 12 * nk), b2, 0, g, c, as);
+        LAUNCH(x, KT_TIMEFLT, k_timeflt, flat(tiles(ni, nj, 64, 4, 12 * nk)), b2, 0, tiles(ni, nj, 64, 4, 12 * nk), g, c, as);
       }
       stamp(x, 5);
       if (cf.cphopt == 2) {
         for (int lv = 0; lv < 2; lv++) {
           const int S = lv == 0 ? C : F;
-          LAUNCH(x, KT_SATADJ, k_satadjst, grid2(ni - 1, nj - 1, 64, 4, nk - 1), b2, 0, g, c.thresq,
+          LAUNCH(x, KT_SATADJ, k_satadjst, flat(tiles(ni - 1, nj - 1, 64, 4, nk - 1)), b2, 0, tiles(ni - 1, nj - 1, 64, 4, nk - 1), g, c.thresq,
                  B[CRESS_PBR], B[CRESS_PTBR], x->pp[S], x->ptp[S], x->q[S], x->nci[S], n);
         }
       }

@@ -54,30 +54,30 @@ struct AsselinSet {
   const float *xf[12];
 };
 
-__global__ void k_phy2cnt(Grid g, Opts o, const float *w, const float *jcb8w, float *wc);
-__global__ void k_prep(Grid g, Opts o, const float *u, const float *v, const float *w,
+__global__ void k_phy2cnt(Tiles T, Grid g, Opts o, const float *w, const float *jcb8w, float *wc);
+__global__ void k_prep(Tiles T, Grid g, Opts o, const float *u, const float *v, const float *w,
                        const float *rst8u, const float *rst8v, const float *rst8w,
                        const float *jcb8w, float *wc, float *rstxu, float *rstxv, float *rstxwc);
-__global__ void k_force_uv(Grid g, Opts o, Consts c, ForceIn in, float *ufrc, float *vfrc);
-__global__ void k_force_w(Grid g, Opts o, Consts c, ForceIn in, float *wfrc);
-__global__ void k_force_p(Grid g, Opts o, Consts c, const float *jcb, const float *jcb8u,
+__global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *ufrc, float *vfrc);
+__global__ void k_force_w(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *wfrc);
+__global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, const float *jcb8u,
                           const float *jcb8v, const float *jcb8w, const float *u, const float *v,
                           const float *wc, const float *pp, const float *ppp, const float *rbct,
                           float *pfrc);
-__global__ void k_force_s(Grid g, Opts o, Consts c, ScalarIn in, int nscalar);
-__global__ void k_ac_ptil(Grid g, Opts o, Consts c, float divcv, const float *jcb, const float *rst8u,
+__global__ void k_force_s(Tiles T, Grid g, Opts o, Consts c, ScalarIn in, int nscalar);
+__global__ void k_ac_ptil(Tiles T, Grid g, Opts o, Consts c, float divcv, const float *jcb, const float *rst8u,
                           const float *rst8v, const float *rst8w, const float *uf, const float *vf,
                           const float *wc, const float *ppf, float *ptil);
-__global__ void k_ac_uv(Grid g, Consts c, float dts, const float *jcb, const float *ptil,
+__global__ void k_ac_uv(Tiles T, Grid g, Consts c, float dts, const float *jcb, const float *ptil,
                         const float *ufrc, const float *vfrc, const float *rst8u, const float *rst8v,
                         float *uf, float *vf);
-__global__ void k_ac_col(Grid g, Opts o, Consts c, AcCol a);
+__global__ void k_ac_col(Tiles T, Grid g, Opts o, Consts c, AcCol a);
 __global__ void k_ac_ring(Grid g, Opts o, const float *jcb8w, float *wf, float *wc, float *ppf);
-__global__ void k_steps(Grid g, float dtb2, const float *rst, StepsIn s, int withq);
+__global__ void k_steps(Tiles T, Grid g, float dtb2, const float *rst, StepsIn s, int withq);
 __global__ void k_bcycle_x(Grid g, FillSet fs);
 __global__ void k_bcycle_y(Grid g, FillSet fs);
 __global__ void k_vbc(Grid g, int bbc, VbcSet vs);
-__global__ void k_timeflt(Grid g, Consts c, AsselinSet as);
+__global__ void k_timeflt(Tiles T, Grid g, Consts c, AsselinSet as);
 
 }  // namespace cress
 

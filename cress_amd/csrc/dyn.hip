@@ -53,20 +53,22 @@ __device__ __forceinline__ float wc_value(const Grid &g, const Opts &o, const fl
 }
 
 // wc over i=1..ni-1, j=1..nj-1, k=1..nk (the phy2cnt+vbcwc write range)
-__global__ void k_phy2cnt(Grid g, Opts o, const float *w, const float *jcb8w, float *wc) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+__global__ void k_phy2cnt(Tiles T, Grid g, Opts o, const float *w, const float *jcb8w, float *wc) {
+  const uint3 xt = xcd_tile(T);
+  const int i = 1 + xt.x * blockDim.x + threadIdx.x;
+  const int j = 1 + xt.y * blockDim.y + threadIdx.y;
+  const int k = 1 + xt.z;
   if (i > g.ni - 1 || j > g.nj - 1) return;
   AT(wc, i, j, k) = wc_value(g, o, w, jcb8w, i, j, k);
 }
 
-__global__ void k_prep(Grid g, Opts o, const float *u, const float *v, const float *w,
+__global__ void k_prep(Tiles T, Grid g, Opts o, const float *u, const float *v, const float *w,
                        const float *rst8u, const float *rst8v, const float *rst8w,
                        const float *jcb8w, float *wc, float *rstxu, float *rstxv, float *rstxwc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const uint3 xt = xcd_tile(T);
+  const int i = xt.x * blockDim.x + threadIdx.x;
+  const int j = xt.y * blockDim.y + threadIdx.y;
+  const int k = 1 + xt.z;
   if (i > g.ni + 1 || j > g.nj + 1) return;
   if (i >= 1 && i <= g.ni - 1 && j >= 1 && j <= g.nj - 1) {
     const float c = wc_value(g, o, w, jcb8w, i, j, k);
@@ -195,10 +197,11 @@ __device__ float adv_w(float f, const Grid &g, const Opts &o, const Consts &c, c
 }
 
 // ufrc over i=2..ni-1, j=2..nj-2 and vfrc over i=2..ni-2, j=2..nj-1; k=2..nk-2.
-__global__ void k_force_uv(Grid g, Opts o, Consts c, ForceIn in, float *ufrc, float *vfrc) {
-  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 2 + blockIdx.z;
+__global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *ufrc, float *vfrc) {
+  const uint3 xt = xcd_tile(T);
+  const int i = 2 + xt.x * blockDim.x + threadIdx.x;
+  const int j = 2 + xt.y * blockDim.y + threadIdx.y;
+  const int k = 2 + xt.z;
   if (i > g.ni - 1 || j > g.nj - 1) return;
   const bool sp = k >= o.ksp0 - 1 && k <= g.nk - 2;
   if (j <= g.nj - 2) {
@@ -244,10 +247,11 @@ __device__ __forceinline__ float wb8s_at(const Grid &g, const Opts &o, const Con
 }
 
 // wfrc over i=2..ni-2, j=2..nj-2, k=2..nk-1
-__global__ void k_force_w(Grid g, Opts o, Consts c, ForceIn in, float *wfrc) {
-  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 2 + blockIdx.z;
+__global__ void k_force_w(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *wfrc) {
+  const uint3 xt = xcd_tile(T);
+  const int i = 2 + xt.x * blockDim.x + threadIdx.x;
+  const int j = 2 + xt.y * blockDim.y + threadIdx.y;
+  const int k = 2 + xt.z;
   if (i > g.ni - 2 || j > g.nj - 2) return;
   float f = 0.0f;
   if (o.vsp_w && k >= o.ksp0 && k <= g.nk - 1)
@@ -261,13 +265,14 @@ __global__ void k_force_w(Grid g, Opts o, Consts c, ForceIn in, float *wfrc) {
 // Pressure forcing: s_forcep (forcep.f90:254-308) = advp (advp.f90:249-785,
 // diaopt=0) then vspp (vspp.f90:180-194).  pfrc over the interior.
 // ---------------------------------------------------------------------------
-__global__ void k_force_p(Grid g, Opts o, Consts c, const float *jcb, const float *jcb8u,
+__global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, const float *jcb8u,
                           const float *jcb8v, const float *jcb8w, const float *u, const float *v,
                           const float *wc, const float *pp, const float *ppp, const float *rbct,
                           float *pfrc) {
-  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 2 + blockIdx.z;
+  const uint3 xt = xcd_tile(T);
+  const int i = 2 + xt.x * blockDim.x + threadIdx.x;
+  const int j = 2 + xt.y * blockDim.y + threadIdx.y;
+  const int k = 2 + xt.z;
   if (i > g.ni - 2 || j > g.nj - 2) return;
   auto jxu = [&](int ii, int jj) { return AT(jcb8u, ii, jj, k) * AT(u, ii, jj, k); };
   auto jxv = [&](int ii, int jj) { return AT(jcb8v, ii, jj, k) * AT(v, ii, jj, k); };
@@ -312,8 +317,8 @@ __global__ void k_force_p(Grid g, Opts o, Consts c, const float *jcb, const floa
 // Scalar forcing: s_forcept/s_forceqv/s_forceq (forcept.f90:344-438,
 // forceqv.f90:290-372, forceq.f90:394-...) = advs (advs.f90:205-445) then the
 // vspopt=2 sponge (vsps.f90:183-197, vspqv.f90:185-200, vsps0.f90:114-127)
-// and, for theta', advbspt (advbspt.f90:143-190).  blockIdx.z = scalar*(nk-3)
-// + (k-2); scalar 0 ptp, 1..6 qv..qg, 7 nci.
+// and, for theta', advbspt (advbspt.f90:143-190).  Scalars: 0 ptp, 1..6
+// qv..qg, 7 nci.
 // ---------------------------------------------------------------------------
 
 
@@ -385,10 +390,11 @@ __device__ __forceinline__ float adv_s_pre(const Grid &g, const Opts &o, const C
 // (advs.f90:205-420), the vspopt=2 sponge (vsps.f90:160-198, vspqv) and, for
 // theta', advbspt (advbspt.f90:143-190).  The flux weights are loaded once
 // and shared by the scalars.
-__global__ void __launch_bounds__(256) k_force_s(Grid g, Opts o, Consts c, ScalarIn in, int nscalar) {
-  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 2 + blockIdx.z;
+__global__ void __launch_bounds__(256) k_force_s(Tiles T, Grid g, Opts o, Consts c, ScalarIn in, int nscalar) {
+  const uint3 xt = xcd_tile(T);
+  const int i = 2 + xt.x * blockDim.x + threadIdx.x;
+  const int j = 2 + xt.y * blockDim.y + threadIdx.y;
+  const int k = 2 + xt.z;
   if (i > g.ni - 2 || j > g.nj - 2) return;
   const uint32_t sx = 4u * (uint32_t)g.nx, sk = 4u * (uint32_t)g.nxy;
   const uint32_t q = 4u * (uint32_t)g.at(i, j, k);
@@ -434,12 +440,13 @@ __global__ void __launch_bounds__(256) k_force_s(Grid g, Opts o, Consts c, Scala
 
 // pgrad.f90:260-306 with diver3d.f90:182-326: the divergence-damped pressure
 // ptil = pp + divcv*jcb*div3d over i=1..ni-1, j=1..nj-1, k=1..nk-1.
-__global__ void k_ac_ptil(Grid g, Opts o, Consts c, float divcv, const float *jcb, const float *rst8u,
+__global__ void k_ac_ptil(Tiles T, Grid g, Opts o, Consts c, float divcv, const float *jcb, const float *rst8u,
                           const float *rst8v, const float *rst8w, const float *uf, const float *vf,
                           const float *wc, const float *ppf, float *ptil) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const uint3 xt = xcd_tile(T);
+  const int i = 1 + xt.x * blockDim.x + threadIdx.x;
+  const int j = 1 + xt.y * blockDim.y + threadIdx.y;
+  const int k = 1 + xt.z;
   if (i > g.ni - 1 || j > g.nj - 1) return;
   const long p = g.at(i, j, k);
   if (o.divopt == 0) {
@@ -455,12 +462,13 @@ __global__ void k_ac_ptil(Grid g, Opts o, Consts c, float divcv, const float *jc
 
 // s_stepuv, stepuv.f90:288-310: uf += dts*(ufrc+upg)/rst8u with
 // upg = (jcb*ptil)(i-1) - (jcb*ptil)(i) over dx (pgrad.f90:357-395).
-__global__ void k_ac_uv(Grid g, Consts c, float dts, const float *jcb, const float *ptil,
+__global__ void k_ac_uv(Tiles T, Grid g, Consts c, float dts, const float *jcb, const float *ptil,
                         const float *ufrc, const float *vfrc, const float *rst8u, const float *rst8v,
                         float *uf, float *vf) {
-  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 2 + blockIdx.z;
+  const uint3 xt = xcd_tile(T);
+  const int i = 2 + xt.x * blockDim.x + threadIdx.x;
+  const int j = 2 + xt.y * blockDim.y + threadIdx.y;
+  const int k = 2 + xt.z;
   if (i > g.ni - 1 || j > g.nj - 1) return;
   const long p = g.at(i, j, k);
   const float pc = jcb[p] * ptil[p];
@@ -487,9 +495,10 @@ __global__ void k_ac_uv(Grid g, Consts c, float dts, const float *jcb, const flo
 // value in the last pass), and the Thomas super-diagonal ss'(k) into
 // ptil(k-1), whose value the column has consumed by then (ptil is read by
 // this column only, and recomputed before its next use).
-__global__ void __launch_bounds__(128) k_ac_col(Grid g, Opts o, Consts c, AcCol a) {
-  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+__global__ void __launch_bounds__(128) k_ac_col(Tiles T, Grid g, Opts o, Consts c, AcCol a) {
+  const uint3 xt = xcd_tile(T);
+  const int i = 2 + xt.x * blockDim.x + threadIdx.x;
+  const int j = 2 + xt.y * blockDim.y + threadIdx.y;
   if (i > g.ni - 2 || j > g.nj - 2) return;
   const int nk = g.nk;
   // byte offsets (a field is below 4 GiB; checked at context creation)
@@ -674,10 +683,11 @@ __global__ void k_ac_ring(Grid g, Opts o, const float *jcb8w, float *wf, float *
 // for qv..qg, without the clamp for theta' and nci; dtdrst = 2*dtb/rst.
 // The q and nci forcings are read from the future arrays themselves.
 // ---------------------------------------------------------------------------
-__global__ void k_steps(Grid g, float dtb2, const float *rst, StepsIn s, int withq) {
-  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 2 + blockIdx.z;
+__global__ void k_steps(Tiles T, Grid g, float dtb2, const float *rst, StepsIn s, int withq) {
+  const uint3 xt = xcd_tile(T);
+  const int i = 2 + xt.x * blockDim.x + threadIdx.x;
+  const int j = 2 + xt.y * blockDim.y + threadIdx.y;
+  const int k = 2 + xt.z;
   if (i > g.ni - 2 || j > g.nj - 2) return;
   const long p = g.at(i, j, k);
   const float d = dtb2 / rst[p];
@@ -749,11 +759,12 @@ __global__ void k_vbc(Grid g, int bbc, VbcSet vs) {
 // filcoe*(xf + xp) over each field's own box (u: i=1..ni, j=1..nj-1,
 // k=1..nk-1; v: j to nj; w: k to nk; scalars i=1..ni-1, j=1..nj-1).
 // ---------------------------------------------------------------------------
-__global__ void k_timeflt(Grid g, Consts c, AsselinSet as) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int f = blockIdx.z / g.nk;
-  const int k = 1 + blockIdx.z % g.nk;
+__global__ void k_timeflt(Tiles T, Grid g, Consts c, AsselinSet as) {
+  const uint3 xt = xcd_tile(T);
+  const int i = 1 + xt.x * blockDim.x + threadIdx.x;
+  const int j = 1 + xt.y * blockDim.y + threadIdx.y;
+  const int f = xt.z / g.nk;
+  const int k = 1 + xt.z % g.nk;
   if (f >= as.n) return;
   const int kind = as.kind[f];
   const int ihi = kind == 1 ? g.ni : g.ni - 1;

@@ -57,10 +57,10 @@ int micro_init_tables();
 MicroK micro_fold(float dtb_micro, float thresq, float dz);
 
 __global__ void k_micro_seed(unsigned *p, unsigned v);
-__global__ void k_micro_src(Grid g, MicroK K, MicroArgs a);
-__global__ void k_micro_fall(Grid g, FallArgs a);
-__global__ void k_asselin_sat(Grid g, float m1fc2, float filcoe, float thresq, AsselinSat a);
-__global__ void k_satadjst(Grid g, float thresq, const float *pbr, const float *ptbr, const float *pp, float *ptp,
+__global__ void k_micro_src(Tiles T, Grid g, MicroK K, MicroArgs a);
+__global__ void k_micro_fall(Tiles T, Grid g, FallArgs a);
+__global__ void k_asselin_sat(Tiles T, Grid g, float m1fc2, float filcoe, float thresq, AsselinSat a);
+__global__ void k_satadjst(Tiles T, Grid g, float thresq, const float *pbr, const float *ptbr, const float *pp, float *ptp,
                            float *q, float *nci, long qn);
 
 }  // namespace cress

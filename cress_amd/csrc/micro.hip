@@ -65,10 +65,11 @@ __device__ __forceinline__ float fmaxr(float a, float b) { return a > b ? a : b;
 constexpr int kMaxFallSteps = 4096;
 
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_micro_src(Grid g, MicroK K, MicroArgs a) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+__global__ void __launch_bounds__(256) k_micro_src(Tiles T, Grid g, MicroK K, MicroArgs a) {
+  const uint3 xt = xcd_tile(T);
+  const int i = 1 + xt.x * blockDim.x + threadIdx.x;
+  const int j = 1 + xt.y * blockDim.y + threadIdx.y;
+  const int k = 1 + xt.z;
   float cfl = K.dtb;
   if (i <= g.ni - 1 && j <= g.nj - 1) {
     using namespace mc;
@@ -470,15 +471,16 @@ __global__ void __launch_bounds__(256) k_micro_src(Grid g, MicroK K, MicroArgs a
   if (tid == 0) atomicMin(a.dtpmin, __float_as_uint(red[0]));
 }
 
-// fallblk.f90:439-541 on the columns (i,j), i in 1..ni-1, j = 1+blockIdx.y,
+// fallblk.f90:439-541 on the columns (i,j), i in 1..ni-1, j = 1+tile y,
 // one wavefront per species (threadIdx.y: qc, qi, nci, qr, qs, qg), each
 // doing its npstp upwind sub-steps (upwqp.f90:164-202 with the surface
 // precipitation rate and accumulation; upwnp.f90:143-176 for nci); then the
 // trailing adjstnci (coldblk.f90:463-465) once qi is final.  Species are
 // independent in fallblk, so the reference's species order does not matter.
-__global__ void __launch_bounds__(384) k_micro_fall(Grid g, FallArgs a) {
-  const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-  const int j = 1 + blockIdx.y;
+__global__ void __launch_bounds__(384) k_micro_fall(Tiles T, Grid g, FallArgs a) {
+  const uint3 xt = xcd_tile(T);
+  const int i = 1 + xt.x * 64 + threadIdx.x;
+  const int j = 1 + xt.y;
   const int sp = threadIdx.y;
   const bool in = i <= g.ni - 1;
   const float dtb = a.dtb;
@@ -612,11 +614,12 @@ __device__ __forceinline__ void satadj_cell(float thresq, float pbr, float pp, f
 }
 
 // s_satadjst on one level over i=1..ni-1, j=1..nj-1, k=1..nk-1.
-__global__ void k_satadjst(Grid g, float thresq, const float *pbr, const float *ptbr, const float *pp,
+__global__ void k_satadjst(Tiles T, Grid g, float thresq, const float *pbr, const float *ptbr, const float *pp,
                            float *ptp, float *q, float *nci, long qn) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const uint3 xt = xcd_tile(T);
+  const int i = 1 + xt.x * blockDim.x + threadIdx.x;
+  const int j = 1 + xt.y * blockDim.y + threadIdx.y;
+  const int k = 1 + xt.z;
   if (i > g.ni - 1 || j > g.nj - 1) return;
   const long p = g.at(i, j, k);
   float th = ptp[p], qv = q[p], qc = q[p + qn], qi = q[p + 3 * qn], ni = nci[p];
@@ -635,11 +638,12 @@ __global__ void k_satadjst(Grid g, float thresq, const float *pbr, const float *
 // and on the future level (intgdrv.f90:968-998).  Every step is pointwise,
 // so doing them per cell in this order is the reference's result.
 // Fields: u, v, w, pp, ptp, qv, qc, qr, qi, qs, qg, nci.
-__global__ void __launch_bounds__(256) k_asselin_sat(Grid g, float m1fc2, float filcoe, float thresq,
+__global__ void __launch_bounds__(256) k_asselin_sat(Tiles T, Grid g, float m1fc2, float filcoe, float thresq,
                                                      AsselinSat a) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 1 + blockIdx.z;
+  const uint3 xt = xcd_tile(T);
+  const int i = 1 + xt.x * blockDim.x + threadIdx.x;
+  const int j = 1 + xt.y * blockDim.y + threadIdx.y;
+  const int k = 1 + xt.z;
   if (i > g.ni || j > g.nj) return;
   const uint32_t q = 4u * (uint32_t)g.at(i, j, k);
   const bool in = i <= g.ni - 1 && j <= g.nj - 1 && k <= g.nk - 1;
