@@ -216,7 +216,7 @@ def _periodic_index(n_int, nfull, offset=2):
     return offset + np.mod(np.arange(nfull) - offset, n_int)
 
 
-def _hydrometeors(X, Y, xc, yc, zs, tk, p, qv, ip, jp, seed, ring):
+def _hydrometeors(X, Y, xc, yc, zs, tk, p, qv, ip, jp, seed, ring, cellnoise=True):
     """Seed every cold-rain species so that all of the microphysics' branches
     are visited: a cos^2 blob (or, for the vortex, a Gaussian eyewall ring)
     with per-cell noise; cloud water above -40 C, rain above -10 C,
@@ -241,7 +241,7 @@ def _hydrometeors(X, Y, xc, yc, zs, tk, p, qv, ip, jp, seed, ring):
             continue
         lin = (I + 4096 * (J + 4096 * k)).astype(np.uint64)
         un = splitmix_uniform(seed + 17, lin.ravel()).reshape(I.shape)
-        b = h * (0.25 + 0.75 * un)
+        b = h * (0.25 + 0.75 * un) if cellnoise else h
         t = tk[k]
         if t > 233.16:
             out["qc"][:, :, k] = 1.0e-3 * b
@@ -261,11 +261,14 @@ def _hydrometeors(X, Y, xc, yc, zs, tk, p, qv, ip, jp, seed, ring):
 
 
 def build(name="bubble", nx=None, ny=None, nz=None, seed=None, cphopt=2, dtbig=None,
-          dtsml=None, nsstp=None, advopt=3, moist=None):
+          dtsml=None, nsstp=None, advopt=3, moist=None, noise=None, divopt=1):
     """Return a Case with config, base state and the initial state.
 
     moist (default: cphopt == 2) seeds the hydrometeor species and a
-    supersaturated core so that the microphysics is active from step one."""
+    supersaturated core so that the microphysics is active from step one.
+    noise overrides the case's theta' noise amplitude; noise=0 also drops the
+    per-cell factor of the hydrometeor seeding, leaving a state with the
+    case's mirror symmetries."""
     if name == "bubble":
         nx, ny, nz = nx or 64, ny or 64, nz or 40
         dx = dy = 1000.0
@@ -273,7 +276,7 @@ def build(name="bubble", nx=None, ny=None, nz=None, seed=None, cphopt=2, dtbig=N
         dtbig = dtbig or 5.0
         dtsml = dtsml or 2.0          # 6 substeps is not representable (chknlslv.f90:2384-2392)
         sounding, amp, rx, rz, zc = wk_sounding, 2.0, 10000.0, 1500.0, 1500.0
-        noise, seed = 0.1, 1 if seed is None else seed
+        noise0, seed = 0.1, 1 if seed is None else seed
         coropt, fcor, shear = 0, 0.0, None
         botbar = 15000.0
     elif name == "supercell":
@@ -283,7 +286,7 @@ def build(name="bubble", nx=None, ny=None, nz=None, seed=None, cphopt=2, dtbig=N
         dtbig = dtbig or 3.0
         dtsml = dtsml or 1.0
         sounding, amp, rx, rz, zc = wk_sounding, 3.0, 10000.0, 1500.0, 1500.0
-        noise, seed = 0.1, 2 if seed is None else seed
+        noise0, seed = 0.1, 2 if seed is None else seed
         coropt, fcor, shear = 0, 0.0, 30.0
         botbar = 15000.0
     elif name == "vortex":
@@ -293,11 +296,12 @@ def build(name="bubble", nx=None, ny=None, nz=None, seed=None, cphopt=2, dtbig=N
         dtbig = dtbig or 4.0
         dtsml = dtsml or 1.0
         sounding, amp, rx, rz, zc = tropical_sounding, 0.0, 0.0, 1.0, 0.0
-        noise, seed = 0.2, 3 if seed is None else seed
+        noise0, seed = 0.2, 3 if seed is None else seed
         coropt, fcor, shear = 1, 5.0e-5, None
         botbar = 15000.0
     else:
         raise ValueError(name)
+    noise = noise0 if noise is None else float(noise)
     ni, nj, nk = nx + 3, ny + 3, nz + 3
     if sthopt == 1:
         # keep the physical top of the configuration: ztop = nz*dz_comp
@@ -435,13 +439,14 @@ def build(name="bubble", nx=None, ny=None, nz=None, seed=None, cphopt=2, dtbig=N
             init[s] = np.zeros(shape)
         if moist is None or moist:
             tk = th * (p / P0) ** (RD / CP)
-            init.update(_hydrometeors(X, Y, xc, yc, zs, tk, p, qv, ip, jp, seed, name == "vortex"))
+            init.update(_hydrometeors(X, Y, xc, yc, zs, tk, p, qv, ip, jp, seed, name == "vortex",
+                                      cellnoise=noise > 0.0))
             for s in ("qv", "qc", "qr", "qi", "qs", "qg", "nci"):
                 a = init[s]
                 a[:, :, 0] = a[:, :, 1]
                 a[:, :, nk - 2] = a[:, :, nk - 3]
     init = {k: np.asfortranarray(a.astype(np.float32)) for k, a in init.items()}
     cfg = host.make_config(ni, nj, nk, dx, dy, dz, dtbig, dtsml, nsstp=nsstp, advopt=advopt,
-                           cphopt=cphopt, vspopt=2, coropt=coropt, sthopt=sthopt,
+                           divopt=divopt, cphopt=cphopt, vspopt=2, coropt=coropt, sthopt=sthopt,
                            ksp0=(ksp0, ksp0), vspvar=b"oooxoooooo")
     return Case(cfg, base, fc, rbct, init, zw, zs, name)

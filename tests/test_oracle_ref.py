@@ -367,3 +367,32 @@ def test_totalqwi(libs):
     o.or_totalqwi(c, qw, qi, a)
     r.ref_totalqwi(NI, NJ, NK, qw, qi, b)
     same(a, b)
+
+
+def test_timeflt(libs):
+    """or_timeflt (Asselin filter of all twelve prognostics over each field's
+    own range) against the reference's s_timeflt, whole arrays."""
+    rng = np.random.default_rng(140)
+    c = cfg_for(cphopt=2)
+    o, r = libs
+    r.configure(c)
+    names = ["u", "v", "w", "pp", "ptp", "q", "nci"]
+    nsp = {"q": 6}
+    lv = {m: [rnd(c, rng, n=nsp.get(m, 1)) for _ in range(3)] for m in names}
+    ref = {m: [a.copy(order="F") for a in lv[m]] for m in names}
+    st = po.State()
+    for m in names:
+        arr = getattr(st, m)
+        for level in range(3):
+            arr[level] = lv[m][level].ctypes.data
+    o.lib.or_timeflt(po.C.byref(c), po.C.byref(st))
+    n = lv["u"][0].size
+    q = [a.reshape(-1, order="F") for a in ref["q"]]
+    dmy = po.field(c, np.float32, 8)
+    r.ref_timeflt(NI, NJ, NK, ref["u"][0], ref["u"][2], ref["v"][0], ref["v"][2], ref["w"][0], ref["w"][2],
+                  ref["pp"][0], ref["pp"][2], ref["ptp"][0], ref["ptp"][2], q[0][:n], q[2][:n],
+                  q[0][n:3 * n], q[2][n:3 * n], q[0][3 * n:], q[2][3 * n:], ref["nci"][0], ref["nci"][2],
+                  ref["u"][1], ref["v"][1], ref["w"][1], ref["pp"][1], ref["ptp"][1], q[1][:n],
+                  q[1][n:3 * n], q[1][3 * n:], ref["nci"][1], dmy)
+    for m in names:
+        same(lv[m][1], ref[m][1])

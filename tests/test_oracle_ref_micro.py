@@ -198,3 +198,21 @@ def test_satadjst(libs, seed):
     # invariant: total water qv+qc+qi unchanged up to rounding
     tw = lambda x: x[..., 0].astype(np.float64) + x[..., 1] + x[..., 3]
     np.testing.assert_allclose(tw(qa), tw(q), rtol=2e-6, atol=1e-9)
+
+
+def test_adjstnci(libs):
+    """or_adjstnci (cloud-ice number clipped to the mean-mass bounds of qi)
+    against the reference's s_adjstnci, whole arrays."""
+    rng = np.random.default_rng(77)
+    c = cfg()
+    o, r = libs
+    f = lambda a: np.asfortranarray(a.astype(np.float32))
+    shp = po.field(c, np.float32).shape
+    qi = f(np.where(rng.uniform(0, 1, shp) < 0.7, np.exp(rng.uniform(np.log(1e-9), np.log(2e-3), shp)), 0.0))
+    nci = f(qi * np.exp(rng.uniform(np.log(1e8), np.log(1e15), shp)))
+    a, b = nci.copy(order="F"), nci.copy(order="F")
+    o.or_adjstnci(c, qi, a)
+    r.ref_adjstnci(NI, NJ, NK, qi, b)
+    np.testing.assert_array_equal(a, b)
+    # both bounds are active somewhere in this draw
+    assert (a < nci).any() and (a > nci).any()
