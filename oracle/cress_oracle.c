@@ -11,6 +11,18 @@
  *   - option branches not taken by the supported configuration (flat terrain
  *     trnopt=0, mfcopt=0, tubopt=0, smtopt=0, exbopt=0, single rank) are not
  *     restated; cfg_check() rejects configurations outside that set.
+ *
+ * Pinning (tests/): bitwise against the reference's own compiled routines
+ * for advs, advuvw, rstuvwc, buoywb, coriuv, vspuvw, forcep/advp,
+ * forcept/forceqv/forceq, pgrad/diver3d, diverpih, diverpiv, advbspi,
+ * pgradiv, buoywsi, gaussel, bcycle, vbc*, totalqwi, timeflt and the whole
+ * microphysics of cress_micro.c.  stepuv, stepwi, steppi, steps, the
+ * drivers and shift2nd cannot be built from the reference (their modules
+ * reach MPI); they are pinned by whole-step invariants (mirror, transpose,
+ * translation, boundedness).  A mistake in them that is symmetric and
+ * stable (e.g. a wrong but symmetric coefficient of stepwi's tridiagonal
+ * system) is not caught: for those routines parity is unpinned against
+ * the reference.
  */
 #include <math.h>
 #include <stdlib.h>
