@@ -63,6 +63,9 @@ __device__ __forceinline__ float fmaxr(float a, float b) { return a > b ? a : b;
 // dz*jcb*4096/dtb (kilometres per second), where the clamp keeps every lane's
 // loop finite instead of following a corrupted state.
 constexpr int kMaxFallSteps = 4096;
+// Sub-step counts up to this run as one pipelined sweep (registers scale
+// with it); larger counts fall back to one sweep per sub-step.
+constexpr int kFallPipe = 8;
 
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_micro_src(Tiles T, Grid g, MicroK K, MicroArgs a) {
@@ -517,28 +520,102 @@ __global__ void __launch_bounds__(384) k_micro_fall(Tiles T, Grid g, FallArgs a)
       return ufield ? LD(ufield, o) : (LD(qpres, o) > thresq ? ucst : 0.0f);
     };
     const float dzvdt = a.dziv * dtp;
-    for (int s = 0; s < npstp; s++) {
-      uint32_t o = q1;
-      float q_k = LD(q, o);
-      float flx_k = LD(a.rbr, o) * speed(o) * q_k;
-      const float flx1 = flx_k;
-      float flx2 = 0.0f;
-      for (int k = 1; k <= g.nk - 2; k++, o += sk) {
-        const float q_kp1 = LD(q, o + sk);
-        const float flx_kp1 = LD(a.rbr, o + sk) * speed(o + sk) * q_kp1;
-        if (k == 1) flx2 = flx_kp1;
-        q_k = fmaxr(q_k + (flx_kp1 - flx_k) / LD(a.rst, o) * dzvdt, 0.0f);
-        ST(q, o, q_k);
-        flx_k = flx_kp1;
-        q_k = q_kp1;
+    const int nk = g.nk;
+    if (npstp <= kFallPipe) {
+      // All npstp sub-steps in one upward sweep, skewed by one level per
+      // sub-step: sub-step s updates level m-s+1 at iteration m, from
+      // sub-step s-1's value one level up, which the same iteration has
+      // just produced.  Only the last sub-step's values reach memory.
+      // lo_q/lo_f: q and flux of sub-step s's input at its current level;
+      // out: sub-step s's newest output (kept when s passes level nk-2,
+      // which makes it the top copy q(nk-1) = q(nk-2)); cw/rw: c = rbr*u
+      // one level above, and rst at, sub-step s's current level.
+      float lo_q[kFallPipe], lo_f[kFallPipe], out[kFallPipe], cw[kFallPipe], rw[kFallPipe];
+      const float c1 = LD(a.rbr, q1) * speed(q1);
+      const float q01 = LD(q, q1);
+      lo_q[0] = q01;
+      lo_f[0] = c1 * q01;
+#pragma unroll
+      for (int s = 0; s < kFallPipe; s++) {
+        if (s > 0) {
+          lo_q[s] = 0.0f;
+          lo_f[s] = 0.0f;
+        }
+        out[s] = 0.0f;
+        cw[s] = 0.0f;
+        rw[s] = 0.0f;
       }
-      // top copy: q(nk-1) = q(nk-2) after the update
-      ST(q, q1 + (uint32_t)(g.nk - 2) * sk, LD(q, q1 + (uint32_t)(g.nk - 3) * sk));
+      const long c2 = (long)i + g.nx * j;
+      float acc = prec ? prec[c2 + a.n2] : 0.0f;
+      float rate = 0.0f;
+      const int mlast = nk - 2 + npstp - 1;
+      for (int m = 1; m <= mlast; m++) {
+        // shift the coefficient windows one level and take level m+1 / m
+#pragma unroll
+        for (int s = kFallPipe - 1; s > 0; s--) {
+          cw[s] = cw[s - 1];
+          rw[s] = rw[s - 1];
+        }
+        float hi = 0.0f;
+        if (m <= nk - 2) {
+          const uint32_t om = q1 + (uint32_t)(m - 1) * sk;
+          cw[0] = LD(a.rbr, om + sk) * speed(om + sk);
+          rw[0] = LD(a.rst, om);
+          hi = LD(q, om + sk);   // q_0(m+1)
+        }
+#pragma unroll
+        for (int s = 0; s < kFallPipe; s++) {
+          if (s >= npstp) break;
+          const int k = m - s;   // level of sub-step s+1 at this iteration
+          if (k >= 1 && k <= nk - 2) {
+            const float qin = s == 0 ? hi : out[s - 1];
+            const float fhi = cw[s] * qin;
+            const float qn = fmaxr(lo_q[s] + (fhi - lo_f[s]) / rw[s] * dzvdt, 0.0f);
+            if (k == 1) {
+              // first update of this sub-step: its surface rate
+              rate = (lo_f[s] + fhi) * (0.5f / mc::rhow);
+              acc = acc + rate * (0.5f * dtp);
+            }
+            lo_q[s] = qin;
+            lo_f[s] = fhi;
+            out[s] = qn;
+            if (s + 1 < kFallPipe && k == 1) {
+              lo_q[s + 1] = qn;
+              lo_f[s + 1] = c1 * qn;
+            }
+            if (s == npstp - 1) ST(q, q1 + (uint32_t)(k - 1) * sk, qn);
+          }
+        }
+      }
+      ST(q, q1 + (uint32_t)(nk - 2) * sk, out[npstp - 1]);
       if (prec) {
-        const long c2 = (long)i + g.nx * j;
-        const float rate = (flx1 + flx2) * (0.5f / mc::rhow);
         prec[c2] = rate;
-        prec[c2 + a.n2] = prec[c2 + a.n2] + rate * (0.5f * dtp);
+        prec[c2 + a.n2] = acc;
+      }
+    } else {
+      for (int s = 0; s < npstp; s++) {
+        uint32_t o = q1;
+        float q_k = LD(q, o);
+        float flx_k = LD(a.rbr, o) * speed(o) * q_k;
+        const float flx1 = flx_k;
+        float flx2 = 0.0f;
+        for (int k = 1; k <= nk - 2; k++, o += sk) {
+          const float q_kp1 = LD(q, o + sk);
+          const float flx_kp1 = LD(a.rbr, o + sk) * speed(o + sk) * q_kp1;
+          if (k == 1) flx2 = flx_kp1;
+          q_k = fmaxr(q_k + (flx_kp1 - flx_k) / LD(a.rst, o) * dzvdt, 0.0f);
+          ST(q, o, q_k);
+          flx_k = flx_kp1;
+          q_k = q_kp1;
+        }
+        // top copy: q(nk-1) = q(nk-2) after the update
+        ST(q, q1 + (uint32_t)(nk - 2) * sk, LD(q, q1 + (uint32_t)(nk - 3) * sk));
+        if (prec) {
+          const long c2 = (long)i + g.nx * j;
+          const float rate = (flx1 + flx2) * (0.5f / mc::rhow);
+          prec[c2] = rate;
+          prec[c2 + a.n2] = prec[c2 + a.n2] + rate * (0.5f * dtp);
+        }
       }
     }
   }
