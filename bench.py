@@ -10,8 +10,8 @@ Prints one JSON line (see the repository's benchmark contract) with
   roofline     -- the dominant kernel: algorithmic bytes per launch over its
                   average launch time (HIP events on the solver stream over
                   the timed region), against 8 TB/s;
-  cpu_baseline -- the FP32 oracle (oracle/cress_oracle.c, single thread) on a
-                  128x128x80 sample of the same workload.
+  cpu_baseline -- the FP32 oracle (oracle/cress_oracle.c, OpenMP over the
+                  host's threads) on a 128x128x80 sample of the same workload.
 """
 import argparse
 import json
@@ -102,11 +102,15 @@ def parse():
 
 
 def cpu_baseline(args, nz):
+    """The FP32 oracle (C restatement, OpenMP over levels) on the host cores,
+    on a cpu_nxy x cpu_nxy x nz sample of the same workload: about 10 s of
+    steps after one warm-up step."""
     from cress_amd import cases
     from oracle import pyoracle as po
     nxy = args.cpu_nxy
     case = cases.build(args.workload, nxy, nxy, nz, cphopt=args.cphopt)
     m = po.Model(case, 32)
+    threads = int(m.o.lib.or_threads())
     m.step(1, first=True)
     n, t0 = 0, time.perf_counter()
     while True:
@@ -115,9 +119,9 @@ def cpu_baseline(args, nz):
         el = time.perf_counter() - t0
         if el > 10.0 or n >= 50:
             break
-    return {"value": nxy * nxy * nz * n / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": "FP32 oracle, %s %dx%dx%d, %d large steps after 1 warm-up" % (
-                args.workload, nxy, nxy, nz, n)}
+    return {"value": nxy * nxy * nz * n / el, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": "FP32 oracle (C restatement, %d OpenMP threads), %s %dx%dx%d, %d large steps after 1 warm-up"
+                      % (threads, args.workload, nxy, nxy, nz, n)}
 
 
 def main():

@@ -154,7 +154,13 @@ enum {
   MW_NW
 };
 
+#define MLOOP_SERIAL                            \
+  for (int k = 1; k <= nk - 1; k++)             \
+    for (int j = 1; j <= nj - 1; j++)           \
+      for (int i = 1; i <= ni - 1; i++)
+/* pointwise sweeps: levels across OpenMP threads (same per-cell result) */
 #define MLOOP                                   \
+  OMP_FOR                                       \
   for (int k = 1; k <= nk - 1; k++)             \
     for (int j = 1; j <= nj - 1; j++)           \
       for (int i = 1; i <= ni - 1; i++)
@@ -290,6 +296,7 @@ static void or_setblk(const or_cfg *c, or_src *s)
   const R cvntr = K(.31) * SQRT(C_AUR) * EXP(ONED3 * LOG(C_SC)) * C_GF5BUR;
   const R cvnts = K(.31) * SQRT(C_AUS) * EXP(ONED3 * LOG(C_SC)) * C_GF5BUS;
   const R cvntg = K(.31) * SQRT(C_AUG) * EXP(ONED3 * LOG(C_SC)) * C_GF5BUG;
+  OMP_FOR
   for (int k = 1; k <= nk - 1; k++) {
     R *nu = W(FLX) + NX * NY * (size_t)(k - 1);   /* nu(i,j): the level's 2-D scratch */
     for (int j = 1; j <= nj - 1; j++)
@@ -951,6 +958,7 @@ void or_upwqp(const or_cfg *c, R dtp, const R *rbr, const R *rst, const R *uq, R
   const R dtp05 = K(.5) * dtp, dzvdt = dziv * dtp, rwiv05 = K(.5) / C_RHOW;
   const size_t n2 = NX * NY;
   MLOOP { size_t p = PIX; qpflx[p] = rbr[p] * uq[p] * qpf[p]; }
+  OMP_FOR
   for (int k = 1; k <= nk - 2; k++)
     for (int j = 1; j <= nj - 1; j++)
       for (int i = 1; i <= ni - 1; i++) {
@@ -975,6 +983,7 @@ void or_upwnp(const or_cfg *c, R dtp, const R *rbr, const R *rst, const R *un, R
   const R dzvdt = dziv * dtp;
   const size_t n2 = NX * NY;
   MLOOP { size_t p = PIX; ncflx[p] = rbr[p] * un[p] * ncf[p]; }
+  OMP_FOR
   for (int k = 1; k <= nk - 2; k++)
     for (int j = 1; j <= nj - 1; j++)
       for (int i = 1; i <= ni - 1; i++) {
@@ -1000,7 +1009,7 @@ int or_fallblk(const or_cfg *c, R dtb, const R *jcb, const R *rbr, const R *rst,
   DECL_DIMS(c);
   const R dz = (R)c->dz;
   R dtpc = dtb, dtpr = dtb, dtpi = dtb, dtps = dtb, dtpg = dtb;
-  MLOOP {
+  MLOOP_SERIAL {
     size_t p = PIX;
     R dzjcb = dz * jcb[p];
     dtpc = rmin(dzjcb / (ucq[p] + C_EPS), dtpc);

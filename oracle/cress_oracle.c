@@ -25,6 +25,7 @@
  * the reference.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 #include "cress_oracle.h"
@@ -46,6 +47,14 @@
 #endif
 
 typedef REAL R;
+
+/* OpenMP threads the parallel sweeps below use */
+int or_threads(void) { return omp_get_max_threads(); }
+
+/* k-sweeps whose iterations write only level k run across OpenMP threads
+ * (the per-cell arithmetic, hence the result, is the same for any thread
+ * count); recurrences in k and reductions stay serial. */
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
 
 #define DECL_DIMS(c) \
   const int ni = (c)->ni, nj = (c)->nj, nk = (c)->nk; \
@@ -197,6 +206,7 @@ void or_vbcw(const or_cfg *c, const R *jcb8w, const R *wc, R *wf)
 void or_phy2cnt(const or_cfg *c, int sthopt, const R *jcb8w, const R *w, R *wc)
 {
   DECL_DIMS(c);
+  OMP_FOR
   for (int k = 2; k <= nk - 1; k++)
     for (int j = 1; j <= nj - 1; j++)
       for (int i = 1; i <= ni - 1; i++)
@@ -213,6 +223,7 @@ void or_rstuvwc(const or_cfg *c, const R *rst8u, const R *rst8v, const R *rst8w,
                 const R *u, const R *v, const R *wc, R *rstxu, R *rstxv, R *rstxwc)
 {
   DECL_DIMS(c);
+  OMP_FOR
   for (int k = 1; k <= nk - 1; k++) {
     for (int j = 1; j <= nj - 1; j++)
       for (int i = c->iwest; i <= ni + 1 - c->ieast; i++)
@@ -221,6 +232,7 @@ void or_rstuvwc(const or_cfg *c, const R *rst8u, const R *rst8v, const R *rst8w,
       for (int i = 1; i <= ni - 1; i++)
         A3(rstxv, i, j, k) = A3(rst8v, i, j, k) * A3(v, i, j, k);
   }
+  OMP_FOR
   for (int k = 1; k <= nk; k++)
     for (int j = 1; j <= nj - 1; j++)
       for (int i = 1; i <= ni - 1; i++)
@@ -238,11 +250,13 @@ void or_advs(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc,
   const R dxv05n = K(-0.5) * dxiv, dyv05n = K(-0.5) * dyiv, dzv05n = K(-0.5) * dziv;
   const R dxv24 = ONED24 * dxiv, dyv24 = ONED24 * dyiv, dzv24 = ONED24 * dziv;
   if (advopt > 3) {
+    OMP_FOR
     for (int k = 2; k <= nk - 2; k++)
       for (int j = 2; j <= nj - 2; j++)
         for (int i = 2; i <= ni - 2; i++) A3(sfrc, i, j, k) = K(0.0);
     return;
   }
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++) {
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 1; i++)
@@ -251,10 +265,12 @@ void or_advs(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc,
       for (int i = 2; i <= ni - 2; i++)
         A3(tmp2, i, j, k) = A3(rstxv, i, j, k) * (A3(s, i, j, k) - A3(s, i, j - 1, k)) * dyv05n;
   }
+  OMP_FOR
   for (int k = 2; k <= nk - 1; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++)
         A3(tmp3, i, j, k) = A3(rstxwc, i, j, k) * (A3(s, i, j, k) - A3(s, i, j, k - 1)) * dzv05n;
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++) {
@@ -269,6 +285,7 @@ void or_advs(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc,
         }
       }
   if (advopt == 1) return;
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++) {
     for (int j = 2 + js; j <= nj - 2 - jn; j++)
       for (int i = 1 + iw; i <= ni - 1 - ie; i++)
@@ -279,6 +296,7 @@ void or_advs(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc,
         A3(tmp2, i, j, k) = (A3(rstxv, i, j, k) + A3(rstxv, i, j + 1, k)) *
                             (A3(s, i, j + 1, k) - A3(s, i, j - 1, k)) * dyv24;
   }
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2 + js; j <= nj - 2 - jn; j++)
       for (int i = 2 + iw; i <= ni - 2 - ie; i++)
@@ -286,21 +304,25 @@ void or_advs(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc,
                             ((A3(tmp1, i - 1, j, k) + A3(tmp1, i + 1, j, k)) +
                              (A3(tmp2, i, j - 1, k) + A3(tmp2, i, j + 1, k)));
   if (advopt == 2) {
+    OMP_FOR
     for (int k = 2; k <= nk - 2; k++)
       for (int j = 2; j <= nj - 2; j++)
         for (int i = 2; i <= ni - 2; i++)
           A3(sfrc, i, j, k) = A3(sfrc, i, j, k) + (A3(tmp3, i, j, k) + A3(tmp3, i, j, k + 1));
     return;
   }
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2 + js; j <= nj - 2 - jn; j++)
       for (int i = 2 + iw; i <= ni - 2 - ie; i++)
         A3(tmp3, i, j, k) = (A3(rstxwc, i, j, k) + A3(rstxwc, i, j, k + 1)) *
                             (A3(s, i, j, k + 1) - A3(s, i, j, k - 1)) * dzv24;
+  OMP_FOR
   for (int k = 3; k <= nk - 3; k++)
     for (int j = 2 + js; j <= nj - 2 - jn; j++)
       for (int i = 2 + iw; i <= ni - 2 - ie; i++)
         A3(vadv, i, j, k) = FOURD3 * A3(vadv, i, j, k) + (A3(tmp3, i, j, k - 1) + A3(tmp3, i, j, k + 1));
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++) A3(sfrc, i, j, k) = A3(sfrc, i, j, k) + A3(vadv, i, j, k);
@@ -321,6 +343,7 @@ void or_advuvw(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc,
   (void)dxv48;
 
   /* ---- u (advuvw.f90:231-448) ---- */
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++) {
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 1; i <= ni - 1; i++)
@@ -331,11 +354,13 @@ void or_advuvw(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc,
         A3(tmp2, i, j, k) = (A3(rstxv, i - 1, j, k) + A3(rstxv, i, j, k)) *
                             (A3(u, i, j, k) - A3(u, i, j - 1, k)) * dyv25n;
   }
+  OMP_FOR
   for (int k = 2; k <= nk - 1; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 1; i++)
         A3(tmp3, i, j, k) = (A3(rstxwc, i - 1, j, k) + A3(rstxwc, i, j, k)) *
                             (A3(u, i, j, k) - A3(u, i, j, k - 1)) * dzv25n;
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 1; i++) {
@@ -350,6 +375,7 @@ void or_advuvw(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc,
         }
       }
   if (advopt >= 2) {
+    OMP_FOR
     for (int k = 2; k <= nk - 2; k++) {
       for (int j = 2 + js; j <= nj - 2 - jn; j++)
         for (int i = 1 + iw; i <= ni - ie; i++)
@@ -361,6 +387,7 @@ void or_advuvw(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc,
                                (A3(rstxv, i - 1, j + 1, k) + A3(rstxv, i, j + 1, k))) *
                               (A3(u, i, j + 1, k) - A3(u, i, j - 1, k)) * dyv48;
     }
+    OMP_FOR
     for (int k = 2; k <= nk - 2; k++)
       for (int j = 2 + js; j <= nj - 2 - jn; j++)
         for (int i = 2 + iw; i <= ni - 1 - ie; i++)
@@ -392,6 +419,7 @@ void or_advuvw(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc,
   }
 
   /* ---- v (advuvw.f90:458-675) ---- */
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++) {
     for (int j = 2; j <= nj - 1; j++)
       for (int i = 2; i <= ni - 1; i++)
@@ -402,11 +430,13 @@ void or_advuvw(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc,
         A3(tmp2, i, j, k) = (A3(rstxv, i, j, k) + A3(rstxv, i, j + 1, k)) *
                             (A3(v, i, j + 1, k) - A3(v, i, j, k)) * dyv25n;
   }
+  OMP_FOR
   for (int k = 2; k <= nk - 1; k++)
     for (int j = 2; j <= nj - 1; j++)
       for (int i = 2; i <= ni - 2; i++)
         A3(tmp3, i, j, k) = (A3(rstxwc, i, j - 1, k) + A3(rstxwc, i, j, k)) *
                             (A3(v, i, j, k) - A3(v, i, j, k - 1)) * dzv25n;
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 1; j++)
       for (int i = 2; i <= ni - 2; i++) {
@@ -421,6 +451,7 @@ void or_advuvw(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc,
         }
       }
   if (advopt >= 2) {
+    OMP_FOR
     for (int k = 2; k <= nk - 2; k++) {
       for (int j = 2 + js; j <= nj - 1 - jn; j++)
         for (int i = 1 + iw; i <= ni - 1 - ie; i++)
@@ -432,6 +463,7 @@ void or_advuvw(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc,
           A3(tmp2, i, j, k) = (A3(rstxv, i, j - 1, k) + A3(rstxv, i, j + 1, k)) *
                               (A3(v, i, j + 1, k) - A3(v, i, j - 1, k)) * dyv24;
     }
+    OMP_FOR
     for (int k = 2; k <= nk - 2; k++)
       for (int j = 2 + js; j <= nj - 1 - jn; j++)
         for (int i = 2 + iw; i <= ni - 2 - ie; i++)
@@ -463,6 +495,7 @@ void or_advuvw(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc,
   }
 
   /* ---- w (advuvw.f90:685-902) ---- */
+  OMP_FOR
   for (int k = 2; k <= nk - 1; k++) {
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 1; i++)
@@ -473,11 +506,13 @@ void or_advuvw(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc,
         A3(tmp2, i, j, k) = (A3(rstxv, i, j, k - 1) + A3(rstxv, i, j, k)) *
                             (A3(w, i, j, k) - A3(w, i, j - 1, k)) * dyv25n;
   }
+  OMP_FOR
   for (int k = 1; k <= nk - 1; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++)
         A3(tmp3, i, j, k) = (A3(rstxwc, i, j, k) + A3(rstxwc, i, j, k + 1)) *
                             (A3(w, i, j, k + 1) - A3(w, i, j, k)) * dzv25n;
+  OMP_FOR
   for (int k = 2; k <= nk - 1; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++) {
@@ -492,6 +527,7 @@ void or_advuvw(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc,
         }
       }
   if (advopt >= 2) {
+    OMP_FOR
     for (int k = 2; k <= nk - 1; k++) {
       for (int j = 2 + js; j <= nj - 2 - jn; j++)
         for (int i = 1 + iw; i <= ni - 1 - ie; i++)
@@ -504,6 +540,7 @@ void or_advuvw(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc,
                                (A3(rstxv, i, j + 1, k - 1) + A3(rstxv, i, j + 1, k))) *
                               (A3(w, i, j + 1, k) - A3(w, i, j - 1, k)) * dyv48;
     }
+    OMP_FOR
     for (int k = 2; k <= nk - 1; k++)
       for (int j = 2 + js; j <= nj - 2 - jn; j++)
         for (int i = 2 + iw; i <= ni - 2 - ie; i++)
@@ -546,20 +583,24 @@ void or_advp(const or_cfg *c, const R *jcb8u, const R *jcb8v, const R *jcb8w,
   const R dxv05n = K(-0.5) * dxiv, dyv05n = K(-0.5) * dyiv, dzv05n = K(-0.5) * dziv;
   const R dxv24 = ONED24 * dxiv, dyv24 = ONED24 * dyiv, dzv24 = ONED24 * dziv;
   if (advopt > 3) {
+    OMP_FOR
     for (int k = 2; k <= nk - 2; k++)
       for (int j = 2; j <= nj - 2; j++)
         for (int i = 2; i <= ni - 2; i++) A3(pfrc, i, j, k) = K(0.0);
     return;
   }
+  OMP_FOR
   for (int k = 1; k <= nk - 1; k++) {
     for (int j = 1; j <= nj - 1; j++)
       for (int i = 1; i <= ni; i++) A3(jcbxu, i, j, k) = A3(jcb8u, i, j, k) * A3(u, i, j, k);
     for (int j = 1; j <= nj; j++)
       for (int i = 1; i <= ni - 1; i++) A3(jcbxv, i, j, k) = A3(jcb8v, i, j, k) * A3(v, i, j, k);
   }
+  OMP_FOR
   for (int k = 1; k <= nk; k++)
     for (int j = 1; j <= nj - 1; j++)
       for (int i = 1; i <= ni - 1; i++) A3(jcbxwc, i, j, k) = A3(jcb8w, i, j, k) * A3(wc, i, j, k);
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++) {
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 1; i++)
@@ -568,10 +609,12 @@ void or_advp(const or_cfg *c, const R *jcb8u, const R *jcb8v, const R *jcb8w,
       for (int i = 2; i <= ni - 2; i++)
         A3(tmp2, i, j, k) = A3(jcbxv, i, j, k) * (A3(pp, i, j, k) - A3(pp, i, j - 1, k)) * dyv05n;
   }
+  OMP_FOR
   for (int k = 2; k <= nk - 1; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++)
         A3(tmp3, i, j, k) = A3(jcbxwc, i, j, k) * (A3(pp, i, j, k) - A3(pp, i, j, k - 1)) * dzv05n;
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++) {
@@ -586,6 +629,7 @@ void or_advp(const or_cfg *c, const R *jcb8u, const R *jcb8v, const R *jcb8w,
         }
       }
   if (advopt == 1) return;
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++) {
     for (int j = 2 + js; j <= nj - 2 - jn; j++)
       for (int i = 1 + iw; i <= ni - 1 - ie; i++)
@@ -596,6 +640,7 @@ void or_advp(const or_cfg *c, const R *jcb8u, const R *jcb8v, const R *jcb8w,
         A3(tmp2, i, j, k) = (A3(jcbxv, i, j, k) + A3(jcbxv, i, j + 1, k)) *
                             (A3(pp, i, j + 1, k) - A3(pp, i, j - 1, k)) * dyv24;
   }
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2 + js; j <= nj - 2 - jn; j++)
       for (int i = 2 + iw; i <= ni - 2 - ie; i++)
@@ -603,21 +648,25 @@ void or_advp(const or_cfg *c, const R *jcb8u, const R *jcb8v, const R *jcb8w,
                             ((A3(tmp1, i - 1, j, k) + A3(tmp1, i + 1, j, k)) +
                              (A3(tmp2, i, j - 1, k) + A3(tmp2, i, j + 1, k)));
   if (advopt == 2) {
+    OMP_FOR
     for (int k = 2; k <= nk - 2; k++)
       for (int j = 2; j <= nj - 2; j++)
         for (int i = 2; i <= ni - 2; i++)
           A3(pfrc, i, j, k) = A3(pfrc, i, j, k) + (A3(tmp3, i, j, k) + A3(tmp3, i, j, k + 1));
     return;
   }
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2 + js; j <= nj - 2 - jn; j++)
       for (int i = 2 + iw; i <= ni - 2 - ie; i++)
         A3(tmp3, i, j, k) = (A3(jcbxwc, i, j, k) + A3(jcbxwc, i, j, k + 1)) *
                             (A3(pp, i, j, k + 1) - A3(pp, i, j, k - 1)) * dzv24;
+  OMP_FOR
   for (int k = 3; k <= nk - 3; k++)
     for (int j = 2 + js; j <= nj - 2 - jn; j++)
       for (int i = 2 + iw; i <= ni - 2 - ie; i++)
         A3(vadv, i, j, k) = FOURD3 * A3(vadv, i, j, k) + (A3(tmp3, i, j, k - 1) + A3(tmp3, i, j, k + 1));
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++) A3(pfrc, i, j, k) = A3(pfrc, i, j, k) + A3(vadv, i, j, k);
@@ -630,18 +679,22 @@ void or_coriuv(const or_cfg *c, const R *fc, const R *rst, const R *u, const R *
   DECL_DIMS(c);
   /* fc(0:ni+1,0:nj+1,1:2): component 1 is read */
 #define FC1(i, j) fc[(size_t)(i) + NX * (size_t)(j)]
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 1; i <= ni - 1; i++)
         A3(tmp1, i, j, k) = FC1(i, j) * A3(rst, i, j, k) * (A3(v, i, j, k) + A3(v, i, j + 1, k));
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 1; i++)
         A3(ufrc, i, j, k) = A3(ufrc, i, j, k) + (A3(tmp1, i - 1, j, k) + A3(tmp1, i, j, k));
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 1; j <= nj - 1; j++)
       for (int i = 2; i <= ni - 2; i++)
         A3(tmp1, i, j, k) = -FC1(i, j) * A3(rst, i, j, k) * (A3(u, i, j, k) + A3(u, i + 1, j, k));
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 1; j++)
       for (int i = 2; i <= ni - 2; i++)
@@ -655,9 +708,11 @@ void or_buoywb(const or_cfg *c, const R *ptbr, const R *qvbr, const R *rst,
 {
   DECL_DIMS(c);
   const R g05 = K(0.5) * GRAV, epsva = K(0.622);
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++) A3(qvd, i, j, k) = A3(qv, i, j, k) - A3(qvbr, i, j, k);
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++) {
@@ -671,6 +726,7 @@ void or_buoywb(const or_cfg *c, const R *ptbr, const R *qvbr, const R *rst,
                (A3(qvd, i, j, k) + A3(qall, i, j, k)) / (K(1.0) + A3(qvbr, i, j, k)) +
                A3(ptp, i, j, k) / A3(ptbr, i, j, k));
       }
+  OMP_FOR
   for (int k = 3; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++)
@@ -691,12 +747,14 @@ void or_vspuvw(const or_cfg *c, const R *ubr, const R *vbr, const R *rst8u,
   const char *vv = c->vspvar;
   if (c->vspopt != 2) return;
   if (vv[0] == 'o' || vv[1] == 'o') {
+    OMP_FOR
     for (int k = k0 - 1; k <= nk - 2; k++)
       for (int j = 1; j <= nj - 1; j++)
         for (int i = 1; i <= ni - 1; i++)
           A3(rbct8s, i, j, k) = K(0.25) * (RBCT2(i, j, k) + RBCT2(i, j, k + 1));
   }
   if (vv[0] == 'o') {
+    OMP_FOR
     for (int k = k0 - 1; k <= nk - 2; k++)
       for (int j = 2; j <= nj - 2; j++)
         for (int i = 2; i <= ni - 1; i++)
@@ -704,6 +762,7 @@ void or_vspuvw(const or_cfg *c, const R *ubr, const R *vbr, const R *rst8u,
               (A3(rbct8s, i - 1, j, k) + A3(rbct8s, i, j, k)) * (A3(up, i, j, k) - A3(ubr, i, j, k));
   }
   if (vv[1] == 'o') {
+    OMP_FOR
     for (int k = k0 - 1; k <= nk - 2; k++)
       for (int j = 2; j <= nj - 1; j++)
         for (int i = 2; i <= ni - 2; i++)
@@ -711,6 +770,7 @@ void or_vspuvw(const or_cfg *c, const R *ubr, const R *vbr, const R *rst8u,
               (A3(rbct8s, i, j - 1, k) + A3(rbct8s, i, j, k)) * (A3(vp, i, j, k) - A3(vbr, i, j, k));
   }
   if (vv[2] == 'o') {
+    OMP_FOR
     for (int k = k0; k <= nk - 1; k++)
       for (int j = 2; j <= nj - 2; j++)
         for (int i = 2; i <= ni - 2; i++)
@@ -724,6 +784,7 @@ void or_vspuvw(const or_cfg *c, const R *ubr, const R *vbr, const R *rst8u,
 void or_vsps(const or_cfg *c, const R *rst, const R *sp, const R *ref, const R *rbct, R *sfrc)
 {
   DECL_DIMS(c);
+  OMP_FOR
   for (int k = c->ksp0[1] - 1; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++) {
@@ -743,11 +804,13 @@ void or_advbspt(const or_cfg *c, const R *ptbr, const R *rbr, const R *w, R *pta
   DECL_DIMS(c);
   const R dziv = (R)1 / (R)c->dz;
   const R dziv25 = K(0.25) * dziv;
+  OMP_FOR
   for (int k = 2; k <= nk - 1; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++)
         A3(pta8w, i, j, k) = (A3(rbr, i, j, k - 1) + A3(rbr, i, j, k)) * A3(w, i, j, k) *
                              (A3(ptbr, i, j, k - 1) - A3(ptbr, i, j, k)) * dziv25;
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++)
@@ -760,6 +823,7 @@ void or_totalqwi(const or_cfg *c, const R *qw, const R *qi, R *qall)
 {
   DECL_DIMS(c);
   const size_t n = (size_t)nfield(c);
+  OMP_FOR
   for (int k = 1; k <= nk - 1; k++)
     for (int j = 1; j <= nj - 1; j++)
       for (int i = 1; i <= ni - 1; i++) {
@@ -779,15 +843,18 @@ void or_diver3d(const or_cfg *c, const R *rst8u, const R *rst8v, const R *rst8w,
 {
   DECL_DIMS(c);
   const R dxiv = (R)1 / (R)c->dx, dyiv = (R)1 / (R)c->dy, dziv = (R)1 / (R)c->dz;
+  OMP_FOR
   for (int k = 1; k <= nk - 1; k++) {
     for (int j = 1; j <= nj - 1; j++)
       for (int i = 1; i <= ni; i++) A3(s1, i, j, k) = A3(rst8u, i, j, k) * A3(u, i, j, k);
     for (int j = 1; j <= nj; j++)
       for (int i = 1; i <= ni - 1; i++) A3(s2, i, j, k) = A3(rst8v, i, j, k) * A3(v, i, j, k);
   }
+  OMP_FOR
   for (int k = 1; k <= nk; k++)
     for (int j = 1; j <= nj - 1; j++)
       for (int i = 1; i <= ni - 1; i++) A3(s3, i, j, k) = A3(rst8w, i, j, k) * A3(wc, i, j, k);
+  OMP_FOR
   for (int k = 1; k <= nk - 1; k++)
     for (int j = 1; j <= nj - 1; j++)
       for (int i = 1; i <= ni - 1; i++)
@@ -810,18 +877,22 @@ void or_pgrad(const or_cfg *c, R dts, const R *jcb, const R *rst8u, const R *rst
   const R divcv = divndc / dts * dz * dz;
   if (c->divopt >= 1)
     or_diver3d(c, rst8u, rst8v, rst8w, u, v, wc, tmp1, tmp2, tmp3, wpg);
+  OMP_FOR
   for (int k = 1; k <= nk - 1; k++)
     for (int j = 1; j <= nj - 1; j++)
       for (int i = 1; i <= ni - 1; i++)
         A3(tmp3, i, j, k) = c->divopt == 0 ? A3(pp, i, j, k)
                             : A3(pp, i, j, k) + divcv * A3(jcb, i, j, k) * A3(tmp1, i, j, k);
+  OMP_FOR
   for (int k = 2; k <= nk - 1; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++)
         A3(wpg, i, j, k) = (A3(tmp3, i, j, k - 1) - A3(tmp3, i, j, k)) * dziv;
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 1; j <= nj - 1; j++)
       for (int i = 1; i <= ni - 1; i++) A3(tmp3, i, j, k) = A3(jcb, i, j, k) * A3(tmp3, i, j, k);
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++) {
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 1; i++)
@@ -839,17 +910,20 @@ void or_diverpih(const or_cfg *c, const R *jcb8u, const R *jcb8v, const R *rcsq,
 {
   DECL_DIMS(c);
   const R dxiv = (R)1 / (R)c->dx, dyiv = (R)1 / (R)c->dy;
+  OMP_FOR
   for (int k = 1; k <= nk - 1; k++) {
     for (int j = 1; j <= nj - 1; j++)
       for (int i = 1; i <= ni; i++) A3(tmp1, i, j, k) = A3(jcb8u, i, j, k) * A3(u, i, j, k);
     for (int j = 1; j <= nj; j++)
       for (int i = 1; i <= ni - 1; i++) A3(tmp2, i, j, k) = A3(jcb8v, i, j, k) * A3(v, i, j, k);
   }
+  OMP_FOR
   for (int k = 1; k <= nk - 1; k++)
     for (int j = 1; j <= nj - 1; j++)
       for (int i = 1; i <= ni - 1; i++)
         A3(pdiv, i, j, k) = (A3(tmp1, i, j, k) - A3(tmp1, i + 1, j, k)) * dxiv +
                             (A3(tmp2, i, j, k) - A3(tmp2, i, j + 1, k)) * dyiv;
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++) A3(pdiv, i, j, k) = A3(rcsq, i, j, k) * A3(pdiv, i, j, k);
@@ -860,6 +934,7 @@ void or_diverpiv(const or_cfg *c, const R *rcsq, const R *w, R *pdiv)
 {
   DECL_DIMS(c);
   const R dziv = (R)1 / (R)c->dz;
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++)
@@ -874,6 +949,7 @@ void or_advbspi(const or_cfg *c, int fore, const R *rst, const R *w, const R *pf
   const R weicoe = (R)c->weicoe;
   const R weic1m = K(1.0) - weicoe;
   const R g05 = K(0.5) * GRAV;
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++) {
@@ -893,9 +969,11 @@ void or_pgradiv(const or_cfg *c, R dts, const R *jcb, const R *wfrc, const R *fp
   DECL_DIMS(c);
   const R dziv = (R)1 / (R)c->dz;
   const R dtdzw = dts * dziv * (R)c->weicoe;
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++) A3(fpdvj, i, j, k) = A3(fp, i, j, k) / A3(jcb, i, j, k);
+  OMP_FOR
   for (int k = 3; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++)
@@ -910,11 +988,13 @@ void or_buoywsi(const or_cfg *c, R dts, const R *rbr, const R *rst, const R *rcs
   DECL_DIMS(c);
   const R dtw = dts * (R)c->weicoe;
   const R g05n = K(-0.5) * GRAV;
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++)
         A3(wb8s, i, j, k) = (A3(pp, i, j, k) * A3(rst, i, j, k) + A3(fp, i, j, k) * A3(rbr, i, j, k) * dtw) /
                             A3(rcsq, i, j, k) * g05n;
+  OMP_FOR
   for (int k = 3; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++)
@@ -940,9 +1020,11 @@ void or_gaussel(const or_cfg *c, int istr, int iend, int jstr, int jend, int kst
         A3(ff, i, j, kstr) = A3(ff, i, j, kstr) * aa;
       }
   }
+  /* the recurrences run in k; columns (j) are independent */
   if (kstr + 2 <= kend) {
-    for (int k = kstr + 1; k <= kend - 1; k++)
-      for (int j = jstr; j <= jend; j++)
+    OMP_FOR
+    for (int j = jstr; j <= jend; j++)
+      for (int k = kstr + 1; k <= kend - 1; k++)
         for (int i = istr; i <= iend; i++) {
           R aa = K(1.0) / (A3(ss, i, j, k) - A3(rr, i, j, k) * A3(ss, i, j, k - 1));
           A3(ss, i, j, k) = A3(tt, i, j, k) * aa;
@@ -955,8 +1037,9 @@ void or_gaussel(const or_cfg *c, int istr, int iend, int jstr, int jend, int kst
         R aa = K(1.0) / (A3(ss, i, j, kend) - A3(rr, i, j, kend) * A3(ss, i, j, kem1));
         A3(ff, i, j, kend) = (A3(ff, i, j, kend) - A3(rr, i, j, kend) * A3(ff, i, j, kem1)) * aa;
       }
-    for (int k = kend - 1; k >= kstr; k--)
-      for (int j = jstr; j <= jend; j++)
+    OMP_FOR
+    for (int j = jstr; j <= jend; j++)
+      for (int k = kend - 1; k >= kstr; k--)
         for (int i = istr; i <= iend; i++)
           A3(ff, i, j, k) = A3(ff, i, j, k) - A3(ss, i, j, k) * A3(ff, i, j, k + 1);
   }
@@ -973,6 +1056,7 @@ void or_stepuv(const or_cfg *c, R dts, const R *rst8u, const R *rst8v, const R *
                const R *vfrc, const R *usml, const R *vsml, R *uf, R *vf)
 {
   DECL_DIMS(c);
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++) {
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 1; i++)
@@ -1001,22 +1085,26 @@ void or_stepwi(const or_cfg *c, R dts, const R *jcb, const R *jcb8w, const R *rb
   const R g05 = K(0.5) * GRAV;
   const R sbsqzi = dts * dts * weicoe * weicoe * dziv;
   const R sbsqg5 = K(0.5) * GRAV * dts * dts * weicoe * weicoe;
+  OMP_FOR
   for (int k = 3; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++)
         A3(wf, i, j, k) = A3(wf, i, j, k) + dts * A3(fw, i, j, k) / A3(rst8w, i, j, k);
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++) {
         A3(tmp1, i, j, k) = g05 * A3(rbr, i, j, k) / A3(rcsq, i, j, k);
         A3(tmp2, i, j, k) = dziv / A3(jcb, i, j, k);
       }
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++) {
         A3(fw, i, j, k) = A3(tmp1, i, j, k) + A3(tmp2, i, j, k);
         A3(wc, i, j, k) = A3(tmp1, i, j, k) - A3(tmp2, i, j, k);
       }
+  OMP_FOR
   for (int k = 3; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++) {
@@ -1053,6 +1141,7 @@ void or_stepwi(const or_cfg *c, R dts, const R *jcb, const R *jcb8w, const R *rb
 void or_steppi(const or_cfg *c, R dts, const R *jcb, const R *fp, R *ppf)
 {
   DECL_DIMS(c);
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++)
@@ -1179,9 +1268,11 @@ static void or_steps(const or_cfg *c, or_state *s, R dtb, const R *ptfrc, const 
   const size_t n = (size_t)nfield(c);
   const int P = 0, F = 2;
   const R dtb2 = K(2.0) * dtb;
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++) A3(dtdrst, i, j, k) = dtb2 / A3(s->rst, i, j, k);
+  OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++) {
@@ -1213,6 +1304,7 @@ static void or_asselin(const or_cfg *c, R *x, const R *xp, const R *xf, int ilo,
   DECL_DIMS(c);
   const R filcoe = (R)c->filcoe;
   const R m1fc2 = K(1.0) - K(2.0) * filcoe;
+  OMP_FOR
   for (int k = 1; k <= khi; k++)
     for (int j = 1; j <= jhi; j++)
       for (int i = ilo; i <= ihi; i++)
