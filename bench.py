@@ -39,10 +39,8 @@ KERNEL_ARRAYS = {
     "k_force_p": 10,
     # 8 scalars, rstxu,rstxv,rstxwc,ptbr,rbr,w -> 8 forcings
     "k_force_s": 22,
-    # jcb,rst8u,rst8v,rst8w,uf,vf,wc,ppf -> ptil
-    "k_ac_ptil": 9,
-    # jcb,ptil,ufrc,vfrc,rst8u,rst8v,uf,vf -> uf,vf
-    "k_ac_uv": 10,
+    # jcb,rst8u,rst8v,rst8w,uf,vf,wc,ppf,ufrc,vfrc -> ptil,uf,vf
+    "k_ac_ptuv": 13,
     # uf,vf,ptil,pfrc,wfrc,jcb,jcb8u,jcb8v,jcb8w,rcsq,rst,rbr,rst8w,wf,ppf -> wf,wc,ppf
     "k_ac_col": 18,
     # ptpp,ptfrc,qp(6),qvfrc,ncip,rst,qf(5 forcings),ncif -> ptpf,qf(6),ncif

@@ -14,6 +14,7 @@
 #include "micro.h"
 
 #include <cstring>
+#include <utility>
 
 using namespace cress;
 
@@ -44,13 +45,13 @@ enum { NSTAGE = 8 };
 // Kernel families timed individually when timing is on (HIP events on the
 // context stream around every launch).
 enum {
-  KT_PREP, KT_FORCE_UV, KT_FORCE_W, KT_FORCE_P, KT_FORCE_S, KT_PHY2CNT, KT_AC_PTIL, KT_AC_UV,
+  KT_PREP, KT_FORCE_UV, KT_FORCE_W, KT_FORCE_P, KT_FORCE_S, KT_PHY2CNT, KT_AC_PTUV,
   KT_AC_COL, KT_AC_RING, KT_FILL, KT_VBC, KT_STEPS, KT_TIMEFLT, KT_COPY, KT_MICRO_SEED, KT_MICRO_SRC,
   KT_MICRO_FALL, KT_SATADJ, KT_N
 };
 static const char *kKernelNames[KT_N] = {
-    "k_prep", "k_force_uv", "k_force_w", "k_force_p", "k_force_s", "k_phy2cnt", "k_ac_ptil",
-    "k_ac_uv", "k_ac_col", "k_ac_ring", "k_bcycle", "k_vbc", "k_steps", "k_timeflt", "memcpy_d2d",
+    "k_prep", "k_force_uv", "k_force_w", "k_force_p", "k_force_s", "k_phy2cnt", "k_ac_ptuv",
+    "k_ac_col", "k_ac_ring", "k_bcycle", "k_vbc", "k_steps", "k_timeflt", "memcpy_d2d",
     "k_micro_seed", "k_micro_src", "k_micro_fall", "k_satadjst"};
 enum { MAXEV = 1024 };
 
@@ -70,6 +71,7 @@ struct cress_ctx {
   float *prwtr, *price;  // precipitation (rate, accumulation)
   // scratch
   float *ufrc, *vfrc, *wfrc, *pfrc, *ptfrc, *qvfrc, *wc, *ptil, *rstxu, *rstxv, *rstxwc;
+  float *ualt, *valt;  // second buffers of the future uf, vf during the acoustic loop
   MicroScratch ms;
   std::vector<void *> allocs;
   int timing;
@@ -219,6 +221,8 @@ int cress_ctx_create(const cress_config *cfg, cress_ctx **out) {
   AL(x->qvfrc, n);
   AL(x->wc, n);
   AL(x->ptil, n);
+  AL(x->ualt, n);
+  AL(x->valt, n);
   AL(x->rstxu, n);
   AL(x->rstxv, n);
   AL(x->rstxwc, n);
@@ -509,32 +513,37 @@ static int large_step(cress_ctx *x, int first) {
   stamp(x, 1);
 
   // ---- acoustic substeps (smalldrv / hevi) on the future level ----
+  // uf and vf alternate between two buffers each substep (the fused
+  // ptil/stepuv kernel reads one, writes the other); both start as copies
+  // of the past level so that cells no kernel writes agree.
   HIPCHK(hipMemcpyAsync(x->u[F], x->u[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(x->v[F], x->v[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(x->ualt, x->u[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(x->valt, x->v[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(x->w[F], x->w[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(x->pp[F], x->pp[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
-  LAUNCH(x, KT_PHY2CNT, k_phy2cnt, flat(tiles(ni - 1, nj - 1, 64, 4, nk)), b2, 0, tiles(ni - 1, nj - 1, 64, 4, nk), g, o, x->w[F], B[CRESS_JCB8W],
-                     x->wc);
+  LAUNCH(x, KT_PHY2CNT, k_phy2cnt, flat(tiles(ni - 1, nj - 1, 64, 4, nk)), b2, 0, tiles(ni - 1, nj - 1, 64, 4, nk), g,
+         o, x->w[F], B[CRESS_JCB8W], x->wc);
   const float dz = (float)cf.dz;
   const float divcv = 0.05f / dts * dz * dz;
-  FillSet fuv;
-  fuv.n = 2;
-  fill_spec(&fuv.f[0], x->u[F], 3, 1, ni - 2, ni, 2, 1, nj - 2, nj - 1, nk);
-  fill_spec(&fuv.f[1], x->v[F], 2, 1, ni - 2, ni - 1, 3, 1, nj - 2, nj, nk);
-  VbcSet vuv;
-  vuv.n = 2;
-  vuv.kind[0] = 1;
-  vuv.ptr[0] = x->u[F];
-  vuv.kind[1] = 2;
-  vuv.ptr[1] = x->v[F];
   FillSet fwp;
   fwp.n = 2;
   fill_spec(&fwp.f[0], x->w[F], 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk);
   fill_spec(&fwp.f[1], x->pp[F], 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk);
+  AcPtUv pu;
+  pu.dts = dts;
+  pu.divcv = divcv;
+  pu.jcb = B[CRESS_JCB];
+  pu.rst8u = B[CRESS_RST8U];
+  pu.rst8v = B[CRESS_RST8V];
+  pu.rst8w = B[CRESS_RST8W];
+  pu.wc = x->wc;
+  pu.ppf = x->pp[F];
+  pu.ufrc = x->ufrc;
+  pu.vfrc = x->vfrc;
+  pu.ptil = x->ptil;
   AcCol ac;
   ac.dts = dts;
-  ac.uf = x->u[F];
-  ac.vf = x->v[F];
   ac.ptil = x->ptil;
   ac.pfrc = x->pfrc;
   ac.wfrc = x->wfrc;
@@ -550,19 +559,34 @@ static int large_step(cress_ctx *x, int first) {
   ac.wc = x->wc;
   ac.ppf = x->pp[F];
   const int nring = 2 * (ni - 1) + 2 * (nj - 3);
+  const Tiles tpu = tiles(ni - 1, nj - 1, 64, 4, nk - 1, 2);
+  const Tiles tcol = tiles(ni - 3, nj - 3, 64, CRESS_COL_BY, 1, 1);
   for (int s = 0; s < cf.nsstp; s++) {
-    LAUNCH(x, KT_AC_PTIL, k_ac_ptil, flat(tiles(ni - 1, nj - 1, 64, 4, nk - 1, 2)), b2, 0, tiles(ni - 1, nj - 1, 64, 4, nk - 1, 2), g, o, c, divcv, B[CRESS_JCB],
-                       B[CRESS_RST8U], B[CRESS_RST8V], B[CRESS_RST8W], x->u[F], x->v[F], x->wc, x->pp[F],
-                       x->ptil);
-    LAUNCH(x, KT_AC_UV, k_ac_uv, flat(tiles(ni - 2, nj - 2, 64, 4, nk - 3)), b2, 0, tiles(ni - 2, nj - 2, 64, 4, nk - 3), g, c, dts, B[CRESS_JCB],
-                       x->ptil, x->ufrc, x->vfrc, B[CRESS_RST8U], B[CRESS_RST8V], x->u[F], x->v[F]);
+    pu.uf = x->u[F];
+    pu.vf = x->v[F];
+    pu.ufn = x->ualt;
+    pu.vfn = x->valt;
+    LAUNCH(x, KT_AC_PTUV, k_ac_ptuv, flat(tpu), b2, 0, tpu, g, o, c, pu);
+    std::swap(x->u[F], x->ualt);
+    std::swap(x->v[F], x->valt);
+    FillSet fuv;
+    fuv.n = 2;
+    fill_spec(&fuv.f[0], x->u[F], 3, 1, ni - 2, ni, 2, 1, nj - 2, nj - 1, nk);
+    fill_spec(&fuv.f[1], x->v[F], 2, 1, ni - 2, ni - 1, 3, 1, nj - 2, nj, nk);
+    VbcSet vuv;
+    vuv.n = 2;
+    vuv.kind[0] = 1;
+    vuv.ptr[0] = x->u[F];
+    vuv.kind[1] = 2;
+    vuv.ptr[1] = x->v[F];
     run_fill(x, fuv);
     run_vbc(x, vuv);
-    LAUNCH(x, KT_AC_COL, k_ac_col, flat(tiles(ni - 3, nj - 3, 64, CRESS_COL_BY, 1, 1)), dim3(64, CRESS_COL_BY),
-           CRESS_COL_LDS, tiles(ni - 3, nj - 3, 64, CRESS_COL_BY, 1, 1), g, o, c, ac);
+    ac.uf = x->u[F];
+    ac.vf = x->v[F];
+    LAUNCH(x, KT_AC_COL, k_ac_col, flat(tcol), dim3(64, CRESS_COL_BY), CRESS_COL_LDS, tcol, g, o, c, ac);
     run_fill(x, fwp);
-    LAUNCH(x, KT_AC_RING, k_ac_ring, dim3((nring + 63) / 64), dim3(64), 0, g, o, B[CRESS_JCB8W], x->w[F],
-                       x->wc, x->pp[F]);
+    LAUNCH(x, KT_AC_RING, k_ac_ring, dim3((nring + 63) / 64), dim3(64), 0, g, o, B[CRESS_JCB8W], x->w[F], x->wc,
+           x->pp[F]);
   }
   stamp(x, 2);
 

@@ -17,6 +17,12 @@ struct ScalarIn {
   const float *rstxu, *rstxv, *rstxwc, *rst, *qvbr, *rbct, *ptbr, *rbr, *w;
 };
 
+struct AcPtUv {
+  float dts, divcv;
+  const float *jcb, *rst8u, *rst8v, *rst8w, *uf, *vf, *wc, *ppf, *ufrc, *vfrc;
+  float *ptil, *ufn, *vfn;   // ufn/vfn: the other buffer of the uf/vf pair
+};
+
 struct AcCol {
   float dts;
   const float *uf, *vf, *pfrc, *wfrc;
@@ -65,12 +71,7 @@ __global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, c
                           const float *wc, const float *pp, const float *ppp, const float *rbct,
                           float *pfrc);
 __global__ void k_force_s(Tiles T, Grid g, Opts o, Consts c, ScalarIn in, int nscalar);
-__global__ void k_ac_ptil(Tiles T, Grid g, Opts o, Consts c, float divcv, const float *jcb, const float *rst8u,
-                          const float *rst8v, const float *rst8w, const float *uf, const float *vf,
-                          const float *wc, const float *ppf, float *ptil);
-__global__ void k_ac_uv(Tiles T, Grid g, Consts c, float dts, const float *jcb, const float *ptil,
-                        const float *ufrc, const float *vfrc, const float *rst8u, const float *rst8v,
-                        float *uf, float *vf);
+__global__ void k_ac_ptuv(Tiles T, Grid g, Opts o, Consts c, AcPtUv a);
 __global__ void k_ac_col(Tiles T, Grid g, Opts o, Consts c, AcCol a);
 __global__ void k_ac_ring(Grid g, Opts o, const float *jcb8w, float *wf, float *wc, float *ppf);
 __global__ void k_steps(Tiles T, Grid g, float dtb2, const float *rst, StepsIn s, int withq);

@@ -438,47 +438,50 @@ __global__ void __launch_bounds__(256) k_force_s(Tiles T, Grid g, Opts o, Consts
 // Acoustic substep (s_hevi, hevi.f90:339-427).
 // ---------------------------------------------------------------------------
 
-// pgrad.f90:260-306 with diver3d.f90:182-326: the divergence-damped pressure
-// ptil = pp + divcv*jcb*div3d over i=1..ni-1, j=1..nj-1, k=1..nk-1.
-__global__ void k_ac_ptil(Tiles T, Grid g, Opts o, Consts c, float divcv, const float *jcb, const float *rst8u,
-                          const float *rst8v, const float *rst8w, const float *uf, const float *vf,
-                          const float *wc, const float *ppf, float *ptil) {
-  const uint3 xt = xcd_tile(T);
-  const int i = 1 + xt.x * blockDim.x + threadIdx.x;
-  const int j = 1 + xt.y * blockDim.y + threadIdx.y;
-  const int k = 1 + xt.z;
-  if (i > g.ni - 1 || j > g.nj - 1) return;
+// pgrad.f90:260-306 with diver3d.f90:182-326 (the divergence-damped pressure
+// ptil = pp + divcv*jcb*div3d over i=1..ni-1, j=1..nj-1, k=1..nk-1) fused
+// with stepuv.f90:288-310's interior update uf += dts*(ufrc+upg)/rst8u,
+// upg = ((jcb*ptil)(i-1) - (jcb*ptil)(i))/dx (pgrad.f90:357-395), and the
+// same for vf.  A 64x4 tile computes ptil on itself and on its i-1 column
+// and j-1 row into LDS, then updates uf and vf from it.  uf and vf are read
+// from one buffer and written to another, because neighbouring tiles read
+// the old values for their halo ptil; ptil goes to memory for the column
+// kernel's vertical pressure gradient.
+__device__ __forceinline__ float ptil_at(const Grid &g, const Opts &o, const Consts &c, const AcPtUv &a, int i,
+                                         int j, int k) {
   const long p = g.at(i, j, k);
-  if (o.divopt == 0) {
-    ptil[p] = ppf[p];
-    return;
-  }
-  const float s1a = rst8u[p] * uf[p], s1b = AT(rst8u, i + 1, j, k) * AT(uf, i + 1, j, k);
-  const float s2a = rst8v[p] * vf[p], s2b = AT(rst8v, i, j + 1, k) * AT(vf, i, j + 1, k);
-  const float s3a = rst8w[p] * wc[p], s3b = AT(rst8w, i, j, k + 1) * AT(wc, i, j, k + 1);
+  if (o.divopt == 0) return a.ppf[p];
+  const float s1a = a.rst8u[p] * a.uf[p], s1b = a.rst8u[p + 1] * a.uf[p + 1];
+  const float s2a = a.rst8v[p] * a.vf[p], s2b = a.rst8v[p + g.nx] * a.vf[p + g.nx];
+  const float s3a = a.rst8w[p] * a.wc[p], s3b = a.rst8w[p + g.nxy] * a.wc[p + g.nxy];
   const float div = (s3a - s3b) * c.dziv + ((s1a - s1b) * c.dxiv + (s2a - s2b) * c.dyiv);
-  ptil[p] = ppf[p] + divcv * jcb[p] * div;
+  return a.ppf[p] + a.divcv * a.jcb[p] * div;
 }
 
-// s_stepuv, stepuv.f90:288-310: uf += dts*(ufrc+upg)/rst8u with
-// upg = (jcb*ptil)(i-1) - (jcb*ptil)(i) over dx (pgrad.f90:357-395).
-__global__ void k_ac_uv(Tiles T, Grid g, Consts c, float dts, const float *jcb, const float *ptil,
-                        const float *ufrc, const float *vfrc, const float *rst8u, const float *rst8v,
-                        float *uf, float *vf) {
+__global__ void __launch_bounds__(256) k_ac_ptuv(Tiles T, Grid g, Opts o, Consts c, AcPtUv a) {
   const uint3 xt = xcd_tile(T);
-  const int i = 2 + xt.x * blockDim.x + threadIdx.x;
-  const int j = 2 + xt.y * blockDim.y + threadIdx.y;
-  const int k = 2 + xt.z;
-  if (i > g.ni - 1 || j > g.nj - 1) return;
+  __shared__ float P[5][66];   // P[y][x]: j = j0-1+y, i = i0-1+x
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i0 = 1 + xt.x * 64, j0 = 1 + xt.y * 4, k = 1 + xt.z;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool own = i <= g.ni - 1 && j <= g.nj - 1;
+  const float pown = own ? ptil_at(g, o, c, a, i, j, k) : 0.0f;
+  P[ty + 1][tx + 1] = pown;
+  if (ty == 0) P[0][tx + 1] = (j0 - 1 >= 1 && i <= g.ni - 1) ? ptil_at(g, o, c, a, i, j0 - 1, k) : 0.0f;
+  if (tx == 0) P[ty + 1][0] = (i0 - 1 >= 1 && j <= g.nj - 1) ? ptil_at(g, o, c, a, i0 - 1, j, k) : 0.0f;
+  __syncthreads();
+  if (!own) return;
   const long p = g.at(i, j, k);
-  const float pc = jcb[p] * ptil[p];
+  a.ptil[p] = pown;
+  if (k < 2 || k > g.nk - 2 || i < 2 || j < 2) return;
+  const float pc = a.jcb[p] * pown;
   if (j <= g.nj - 2) {
-    const float upg = (AT(jcb, i - 1, j, k) * AT(ptil, i - 1, j, k) - pc) * c.dxiv;
-    uf[p] = uf[p] + dts * (ufrc[p] + upg) / rst8u[p];
+    const float upg = (a.jcb[p - 1] * P[ty + 1][tx] - pc) * c.dxiv;
+    a.ufn[p] = a.uf[p] + a.dts * (a.ufrc[p] + upg) / a.rst8u[p];
   }
   if (i <= g.ni - 2) {
-    const float vpg = (AT(jcb, i, j - 1, k) * AT(ptil, i, j - 1, k) - pc) * c.dyiv;
-    vf[p] = vf[p] + dts * (vfrc[p] + vpg) / rst8v[p];
+    const float vpg = (a.jcb[p - g.nx] * P[ty][tx + 1] - pc) * c.dyiv;
+    a.vfn[p] = a.vf[p] + a.dts * (a.vfrc[p] + vpg) / a.rst8v[p];
   }
 }
 
