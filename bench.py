@@ -47,8 +47,9 @@ KERNEL_ARRAYS = {
     "k_steps": 25,
     # 12 fields x 3 levels, pbr, ptbr -> 12 present, 5 future (satadjst)
     "k_asselin_sat": 55,
-    # jcb,pbr,ptbr,rbr, past pp,ptp,q(6),nci, future ptp,q(6),nci -> future ptp,q(6),nci, urq,usq,ugq
-    "k_micro_src": 32,
+    # jcb,pbr,ptbr,rbr,rst, past pp,ptp,q(6),nci, forcings ptfrc,qvfrc,5 q,nci
+    #   -> future ptp,q(6),nci (steps fused in), urq,usq,ugq
+    "k_micro_src": 33,
     # rbr,rst,urq,usq,ugq, past qc,qi, future qc,qr,qi,qs,qg,nci -> future qc,qr,qi,qs,qg,nci
     "k_micro_fall": 19,
 }

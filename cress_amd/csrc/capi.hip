@@ -72,6 +72,7 @@ struct cress_ctx {
   // scratch
   float *ufrc, *vfrc, *wfrc, *pfrc, *ptfrc, *qvfrc, *wc, *ptil, *rstxu, *rstxv, *rstxwc;
   float *ualt, *valt;  // second buffers of the future uf, vf during the acoustic loop
+  float *sfrc;         // forcings of qc, qr, qi, qs, qg, nci (cphopt=2)
   MicroScratch ms;
   std::vector<void *> allocs;
   int timing;
@@ -228,6 +229,7 @@ int cress_ctx_create(const cress_config *cfg, cress_ctx **out) {
   AL(x->rstxwc, n);
   if (cfg->cphopt == 2) {
     for (int f = 0; f < MicroScratch::NF; f++) AL(x->ms.f[f], n);
+    AL(x->sfrc, 6 * n);
     AL(x->ms.col, 64);
     rc = micro_init_tables();
     if (rc) goto fail;
@@ -492,8 +494,11 @@ static int large_step(cress_ctx *x, int first) {
                           x->q[C] + 4 * n, x->q[C] + 5 * n, x->nci[C]};
     const float *sP[8] = {x->ptp[P], x->q[P], x->q[P] + n, x->q[P] + 2 * n, x->q[P] + 3 * n,
                           x->q[P] + 4 * n, x->q[P] + 5 * n, x->nci[P]};
-    float *fr[8] = {x->ptfrc, x->qvfrc, x->q[F] + n, x->q[F] + 2 * n, x->q[F] + 3 * n,
-                    x->q[F] + 4 * n, x->q[F] + 5 * n, x->nci[F]};
+    // with microphysics the species forcings get their own arrays (the
+    // fused steps+microphysics kernel reads them at a cell's periodic image
+    // while writing the future level); without, only theta' and qv step
+    float *fr[8] = {x->ptfrc, x->qvfrc, x->sfrc, x->sfrc + n, x->sfrc + 2 * n,
+                    x->sfrc + 3 * n, x->sfrc + 4 * n, x->sfrc + 5 * n};
     for (int s = 0; s < 8; s++) {
       si.s[s] = sC[s];
       si.sp[s] = sP[s];
@@ -559,14 +564,14 @@ static int large_step(cress_ctx *x, int first) {
   ac.wc = x->wc;
   ac.ppf = x->pp[F];
   const int nring = 2 * (ni - 1) + 2 * (nj - 3);
-  const Tiles tpu = tiles(ni - 1, nj - 1, 64, 4, nk - 1, 2);
+  const Tiles tpu = tiles(ni - 1, nj - 1, 64, CRESS_PTUV_BY, nk - 1, CRESS_PTUV_ORDER);
   const Tiles tcol = tiles(ni - 3, nj - 3, 64, CRESS_COL_BY, 1, 1);
   for (int s = 0; s < cf.nsstp; s++) {
     pu.uf = x->u[F];
     pu.vf = x->v[F];
     pu.ufn = x->ualt;
     pu.vfn = x->valt;
-    LAUNCH(x, KT_AC_PTUV, k_ac_ptuv, flat(tpu), b2, 0, tpu, g, o, c, pu);
+    LAUNCH(x, KT_AC_PTUV, k_ac_ptuv, flat(tpu), dim3(64, CRESS_PTUV_BY), 0, tpu, g, o, c, pu);
     std::swap(x->u[F], x->ualt);
     std::swap(x->v[F], x->valt);
     FillSet fuv;
@@ -591,27 +596,28 @@ static int large_step(cress_ctx *x, int first) {
   stamp(x, 2);
 
   // ---- scalar leapfrog (steps) ----
-  StepsIn sti;
-  sti.ptpp = x->ptp[P];
-  sti.ptfrc = x->ptfrc;
-  sti.qp = x->q[P];
-  sti.qvfrc = x->qvfrc;
-  sti.ncip = x->nci[P];
-  sti.ptpf = x->ptp[F];
-  sti.qf = x->q[F];
-  sti.ncif = x->nci[F];
-  sti.qn = n;
+  // With microphysics the leapfrog and its halo fills are done inside the
+  // microphysics source kernel, cell by cell (below).
   const int withq = cf.cphopt == 2;
-  LAUNCH(x, KT_STEPS, k_steps, flat(tiles(ni - 3, nj - 3, 64, 4, nk - 3)), b2, 0, tiles(ni - 3, nj - 3, 64, 4, nk - 3), g, 2.0f * dtb, B[CRESS_RST], sti,
-                     withq);
-  {
+  if (!withq) {
+    StepsIn sti;
+    sti.ptpp = x->ptp[P];
+    sti.ptfrc = x->ptfrc;
+    sti.qp = x->q[P];
+    sti.qvfrc = x->qvfrc;
+    sti.ncip = x->nci[P];
+    sti.ptpf = x->ptp[F];
+    sti.qf = x->q[F];
+    sti.ncif = x->nci[F];
+    sti.qn = n;
+    LAUNCH(x, KT_STEPS, k_steps, flat(tiles(ni - 3, nj - 3, 64, 4, nk - 3)), b2, 0,
+           tiles(ni - 3, nj - 3, 64, 4, nk - 3), g, 2.0f * dtb, B[CRESS_RST], sti, 0);
     FillSet fs;
     VbcSet vs;
-    fs.n = withq ? 8 : 2;
+    fs.n = 2;
     vs.n = fs.n;
-    float *fl[8] = {x->ptp[F], x->q[F], x->q[F] + n, x->q[F] + 2 * n, x->q[F] + 3 * n,
-                    x->q[F] + 4 * n, x->q[F] + 5 * n, x->nci[F]};
-    for (int s = 0; s < 8; s++) {
+    float *fl[2] = {x->ptp[F], x->q[F]};
+    for (int s = 0; s < 2; s++) {
       fill_spec(&fs.f[s], fl[s], 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk);
       vs.kind[s] = 0;
       vs.ptr[s] = fl[s];
@@ -636,6 +642,10 @@ static int large_step(cress_ctx *x, int first) {
     ma.qp = x->q[P];
     ma.ncip = x->nci[P];
     ma.ptpf = x->ptp[F];
+    ma.rst = B[CRESS_RST];
+    ma.ptfrc = x->ptfrc;
+    ma.qvfrc = x->qvfrc;
+    ma.sfrc = x->sfrc;
     ma.qf = x->q[F];
     ma.ncif = x->nci[F];
     ma.urq = x->ms.f[0];

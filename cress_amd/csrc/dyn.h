@@ -17,6 +17,14 @@ struct ScalarIn {
   const float *rstxu, *rstxv, *rstxwc, *rst, *qvbr, *rbct, *ptbr, *rbr, *w;
 };
 
+// Rows per workgroup of the fused ptil/stepuv kernel and its tile order.
+#ifndef CRESS_PTUV_BY
+#define CRESS_PTUV_BY 4
+#endif
+#ifndef CRESS_PTUV_ORDER
+#define CRESS_PTUV_ORDER 2
+#endif
+
 struct AcPtUv {
   float dts, divcv;
   const float *jcb, *rst8u, *rst8v, *rst8w, *uf, *vf, *wc, *ppf, *ufrc, *vfrc;

@@ -442,7 +442,7 @@ __global__ void __launch_bounds__(256) k_force_s(Tiles T, Grid g, Opts o, Consts
 // ptil = pp + divcv*jcb*div3d over i=1..ni-1, j=1..nj-1, k=1..nk-1) fused
 // with stepuv.f90:288-310's interior update uf += dts*(ufrc+upg)/rst8u,
 // upg = ((jcb*ptil)(i-1) - (jcb*ptil)(i))/dx (pgrad.f90:357-395), and the
-// same for vf.  A 64x4 tile computes ptil on itself and on its i-1 column
+// same for vf.  A 64 x CRESS_PTUV_BY tile computes ptil on itself and on its i-1 column
 // and j-1 row into LDS, then updates uf and vf from it.  uf and vf are read
 // from one buffer and written to another, because neighbouring tiles read
 // the old values for their halo ptil; ptil goes to memory for the column
@@ -458,11 +458,11 @@ __device__ __forceinline__ float ptil_at(const Grid &g, const Opts &o, const Con
   return a.ppf[p] + a.divcv * a.jcb[p] * div;
 }
 
-__global__ void __launch_bounds__(256) k_ac_ptuv(Tiles T, Grid g, Opts o, Consts c, AcPtUv a) {
+__global__ void __launch_bounds__(64 * CRESS_PTUV_BY) k_ac_ptuv(Tiles T, Grid g, Opts o, Consts c, AcPtUv a) {
   const uint3 xt = xcd_tile(T);
-  __shared__ float P[5][66];   // P[y][x]: j = j0-1+y, i = i0-1+x
+  __shared__ float P[CRESS_PTUV_BY + 1][66];   // P[y][x]: j = j0-1+y, i = i0-1+x
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i0 = 1 + xt.x * 64, j0 = 1 + xt.y * 4, k = 1 + xt.z;
+  const int i0 = 1 + xt.x * 64, j0 = 1 + xt.y * CRESS_PTUV_BY, k = 1 + xt.z;
   const int i = i0 + tx, j = j0 + ty;
   const bool own = i <= g.ni - 1 && j <= g.nj - 1;
   const float pown = own ? ptil_at(g, o, c, a, i, j, k) : 0.0f;

@@ -26,7 +26,8 @@ struct MicroK {
 };
 
 struct MicroArgs {
-  const float *jcb, *pbr, *ptbr, *rbr;
+  const float *jcb, *pbr, *ptbr, *rbr, *rst;
+  const float *ptfrc, *qvfrc, *sfrc;    // steps forcings (sfrc: qc, qr, qi, qs, qg, nci)
   const float *ppp, *ptpp, *qp, *ncip;  // past level (qp: six species stacked)
   float *ptpf, *qf, *ncif;              // future level
   float *urq, *usq, *ugq;               // fall speeds handed to k_micro_fall

@@ -95,10 +95,25 @@ __global__ void __launch_bounds__(256) k_micro_src(Tiles T, Grid g, MicroK K, Mi
                                    (1.e-2f / msmax) * qsp), (1.e2f / ms0) * qsp);
     const float ncgp = fminr(fmaxr(sqrtf(sqrtf((ng0 * ng0 * ng0 / (cc * rhog)) * rbr * qgp)) * rbv,
                                    (1.e-2f / mgmax) * qgp), (1.e2f / mg0) * qgp);
-    // future level; adjstnci before the sources
-    float ptpf = a.ptpf[p], qvf = a.qf[p], qcf = a.qf[p + qn], qrf = a.qf[p + 2 * qn];
-    float qif = a.qf[p + 3 * qn], qsf = a.qf[p + 4 * qn], qgf = a.qf[p + 5 * qn];
-    float ncif = fminr(fmaxr(a.ncif[p], (0.5f / mimax) * qif), (1.e2f / mi0) * qif);
+    // future level: the steps leapfrog (steps.f90:520-760) of this cell's
+    // source -- its periodic image from bcycle (i=1 <- ni-2, i=ni-1 <- 2,
+    // the same in j) and vbcs's level copy (k=1 <- 2, k=nk-1 <- nk-2) --
+    // which is the value the reference's fills leave here
+    const int is = i == 1 ? g.ni - 2 : (i == g.ni - 1 ? 2 : i);
+    const int js = j == 1 ? g.nj - 2 : (j == g.nj - 1 ? 2 : j);
+    const int ks = k == 1 ? 2 : (k == g.nk - 1 ? g.nk - 2 : k);
+    const long ps = g.at(is, js, ks);
+    const float d = dtb / a.rst[ps];   // dtb is the leapfrog's 2*dtb here
+    float ptpf = a.ptpp[ps] + a.ptfrc[ps] * d;
+    float qvf = fmaxr(a.qp[ps] + a.qvfrc[ps] * d, 0.0f);
+    float qcf = fmaxr(a.qp[ps + qn] + a.sfrc[ps] * d, 0.0f);
+    float qrf = fmaxr(a.qp[ps + 2 * qn] + a.sfrc[ps + qn] * d, 0.0f);
+    float qif = fmaxr(a.qp[ps + 3 * qn] + a.sfrc[ps + 2 * qn] * d, 0.0f);
+    float qsf = fmaxr(a.qp[ps + 4 * qn] + a.sfrc[ps + 3 * qn] * d, 0.0f);
+    float qgf = fmaxr(a.qp[ps + 5 * qn] + a.sfrc[ps + 4 * qn] * d, 0.0f);
+    float ncif = a.ncip[ps] + a.sfrc[ps + 5 * qn] * d;
+    // adjstnci before the sources
+    ncif = fminr(fmaxr(ncif, (0.5f / mimax) * qif), (1.e2f / mi0) * qif);
     // termblk
     const float ucq = qcp > thresq ? ucqcst : 0.0f;
     const float uiq = qip > thresq ? uiqcst : 0.0f;
