@@ -564,6 +564,18 @@ __global__ void __launch_bounds__(384) k_micro_fall(Tiles T, Grid g, FallArgs a)
       float acc = prec ? prec[c2 + a.n2] : 0.0f;
       float rate = 0.0f;
       const int mlast = nk - 2 + npstp - 1;
+      // the inputs of level m are loaded one iteration ahead (rbr, fall
+      // speed and q at m+1, rst at m), so each lane keeps two levels of
+      // loads in flight; writes trail at level <= m, never ahead of them
+      float nrb = 0.0f, nsp = 0.0f, nrs = 0.0f, nq = 0.0f;
+      auto fetch = [&](int m) {
+        const uint32_t om = q1 + (uint32_t)(m - 1) * sk;
+        nrb = LD(a.rbr, om + sk);
+        nsp = speed(om + sk);
+        nrs = LD(a.rst, om);
+        nq = LD(q, om + sk);
+      };
+      fetch(1);
       for (int m = 1; m <= mlast; m++) {
         // shift the coefficient windows one level and take level m+1 / m
 #pragma unroll
@@ -573,10 +585,10 @@ __global__ void __launch_bounds__(384) k_micro_fall(Tiles T, Grid g, FallArgs a)
         }
         float hi = 0.0f;
         if (m <= nk - 2) {
-          const uint32_t om = q1 + (uint32_t)(m - 1) * sk;
-          cw[0] = LD(a.rbr, om + sk) * speed(om + sk);
-          rw[0] = LD(a.rst, om);
-          hi = LD(q, om + sk);   // q_0(m+1)
+          cw[0] = nrb * nsp;
+          rw[0] = nrs;
+          hi = nq;   // q_0(m+1)
+          if (m + 1 <= nk - 2) fetch(m + 1);
         }
 #pragma unroll
         for (int s = 0; s < kFallPipe; s++) {
