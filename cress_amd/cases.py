@@ -186,7 +186,8 @@ class Case:
             "&gridsth sthopt=%d /" % c.sthopt,
             "&terrain trnopt=0 /",
             "&flength stime=0.0, etime=%r /" % (nsteps * c.dtbig),
-            "&boundry wbc=1, ebc=1, sbc=1, nbc=1, bbc=%d, tbc=%d /" % (c.bbc, c.tbc),
+            "&boundry wbc=%d, ebc=%d, sbc=%d, nbc=%d, bbc=%d, tbc=%d, gwave=%r /"
+            % tuple([1 if b == -1 else b for b in (c.wbc, c.ebc, c.sbc, c.nbc)] + [c.bbc, c.tbc, c.gwave]),
             "&gpvpram vspopt=%d, vspvar='%s' /" % (c.vspopt, vsp[:10]),
             "&integrat dtbig=%r, dtsml=%r, gwmopt=%d, impopt=%d, advopt=%d, weicoe=%r, filcoe=%r /"
             % (c.dtbig, c.dtsml, c.gwmopt, c.impopt, c.advopt, c.weicoe, c.filcoe),
@@ -261,14 +262,20 @@ def _hydrometeors(X, Y, xc, yc, zs, tk, p, qv, ip, jp, seed, ring, cellnoise=Tru
 
 
 def build(name="bubble", nx=None, ny=None, nz=None, seed=None, cphopt=2, dtbig=None,
-          dtsml=None, nsstp=None, advopt=3, moist=None, noise=None, divopt=1):
+          dtsml=None, nsstp=None, advopt=3, moist=None, noise=None, divopt=1, wbc=None,
+          gwave=None):
     """Return a Case with config, base state and the initial state.
+
+    wbc: -1 periodic or 7 open (radiation at the constant phase speed gwave,
+    m/s) on all four sides; the supercell defaults to open sides with
+    gwave=30, the other cases to periodic.
 
     moist (default: cphopt == 2) seeds the hydrometeor species and a
     supersaturated core so that the microphysics is active from step one.
     noise overrides the case's theta' noise amplitude; noise=0 also drops the
     per-cell factor of the hydrometeor seeding, leaving a state with the
     case's mirror symmetries."""
+    open_default, gwave_default = name == "supercell", 30.0
     if name == "bubble":
         nx, ny, nz = nx or 64, ny or 64, nz or 40
         dx = dy = 1000.0
@@ -448,5 +455,7 @@ def build(name="bubble", nx=None, ny=None, nz=None, seed=None, cphopt=2, dtbig=N
     init = {k: np.asfortranarray(a.astype(np.float32)) for k, a in init.items()}
     cfg = host.make_config(ni, nj, nk, dx, dy, dz, dtbig, dtsml, nsstp=nsstp, advopt=advopt,
                            divopt=divopt, cphopt=cphopt, vspopt=2, coropt=coropt, sthopt=sthopt,
-                           ksp0=(ksp0, ksp0), vspvar=b"oooxoooooo")
+                           ksp0=(ksp0, ksp0), vspvar=b"oooxoooooo",
+                           wbc=(7 if open_default else -1) if wbc is None else wbc,
+                           gwave=(gwave_default if gwave is None else gwave))
     return Case(cfg, base, fc, rbct, init, zw, zs, name)

@@ -110,6 +110,7 @@ program cress_run
   smtopt = 0; tubopt = 0; buyopt = 1; divopt = 0; haiopt = 0
   gwmopt = 0; impopt = 1; weicoe = 0.6; filcoe = 0.1
   vspopt = 0; vspvar = 'xxxxxxxxxx'; thresq = 1.e-8
+  lbcvar = 'xxxxxxxxxx'; lbnews = 0.0; lbnorm = 0.0; gwave = 0.0
 
   open(10, file=trim(confn), status='old', action='read', iostat=ios)
   if(ios /= 0) call fatal('cannot open '//trim(confn))
@@ -137,6 +138,12 @@ program cress_run
   end if
   if(trnopt /= 0 .or. mfcopt /= 0 .or. smtopt /= 0 .or. tubopt /= 0) then
     call fatal('trnopt, mfcopt, smtopt and tubopt must be 0')
+  end if
+
+! open sides: the constant-phase-speed radiation (option 7) without the
+! lateral damping toward the base state (lbcvar 'o' with lbnews/lbnorm)
+  if(index(lbcvar(1:10), 'o') > 0 .and. (lbnews /= 0.0 .or. lbnorm /= 0.0)) then
+    call fatal('lateral boundary damping (lbcvar/lbnews/lbnorm) is not supported')
   end if
 
 ! one rank: periodic flags become -1 (chknlslv.f90:751-790)
@@ -192,6 +199,7 @@ program cress_run
   cfg%weicoe = weicoe
   cfg%filcoe = filcoe
   cfg%thresq = thresq
+  cfg%gwave = gwave
 
   if(trim(arg4) == '--check') then
     write(*,'(a,3i8)') 'dims', cfg%ni, cfg%nj, cfg%nk
@@ -202,8 +210,8 @@ program cress_run
      &  cfg%coropt, cfg%sthopt
     write(*,'(a,3i6)') 'ksp0_nsstp', cfg%ksp0(1), cfg%ksp0(2), cfg%nsstp
     write(*,'(2a)') 'vspvar ', vspvar(1:10)
-    write(*,'(a,8es24.16)') 'reals', cfg%dx, cfg%dy, cfg%dz, cfg%dtbig, &
-     &  cfg%dtsml, cfg%weicoe, cfg%filcoe, cfg%thresq
+    write(*,'(a,9es24.16)') 'reals', cfg%dx, cfg%dy, cfg%dz, cfg%dtbig, &
+     &  cfg%dtsml, cfg%weicoe, cfg%filcoe, cfg%thresq, cfg%gwave
     write(*,'(a,i8)') 'nsteps', nint((etime - stime) / dtbig)
     stop
   end if

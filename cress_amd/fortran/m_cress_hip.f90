@@ -48,6 +48,7 @@ module m_cress_hip
     real(c_double) :: dx, dy, dz
     real(c_double) :: dtbig, dtsml
     real(c_double) :: weicoe, filcoe, thresq
+    real(c_double) :: gwave
   end type cress_config
 
   public :: cress_error_string, cress_device_count

@@ -48,7 +48,8 @@ enum {
 typedef struct cress_config {
   int ni, nj, nk;
   int wbc, ebc, sbc, nbc;   /* lateral BCs after the one-rank remap: -1 periodic
-                               (chknlslv.f90:751-790) */
+                               (chknlslv.f90:751-790) on all four sides, or 7
+                               (open, constant phase speed) on all four */
   int bbc, tbc;             /* bottom/top: 2 rigid, 3 zero gradient */
   int advopt;               /* 1,2: 2nd/4th order horizontal; 3: 4th order 3-D */
   int impopt;               /* 1: implicit vertical (Thomas, gaussel) */
@@ -66,6 +67,9 @@ typedef struct cress_config {
   double dx, dy, dz;        /* m; dz is the computational (zeta) spacing */
   double dtbig, dtsml;      /* s */
   double weicoe, filcoe, thresq;
+  double gwave;             /* open sides (wbc..nbc = 7): the constant phase
+                               speed of the radiative fills, m/s
+                               (&boundry gwave, phvuvw.f90:334-338) */
 } cress_config;
 
 /* Field ids for upload/download.  Base state and metrics have one level. */

@@ -16,7 +16,10 @@
  * for advs, advuvw, rstuvwc, buoywb, coriuv, vspuvw, forcep/advp,
  * forcept/forceqv/forceq, pgrad/diver3d, diverpih, diverpiv, advbspi,
  * pgradiv, buoywsi, gaussel, bcycle, vbc*, totalqwi, timeflt and the whole
- * microphysics of cress_micro.c.  stepuv, stepwi, steppi, steps, the
+ * microphysics of cress_micro.c; on open sides (option 7, cress_obc.c)
+ * also phvuvw, phvs, rbcu, rbcv, rbcw, rbcss, rbcpt, rbcqv, rbcq, rbcs,
+ * bc4news, the advection routines with the wall reduction, and the lbc*
+ * routines' being no-ops.  stepuv, stepwi, steppi, steps, the
  * drivers and shift2nd cannot be built from the reference (their modules
  * reach MPI); they are pinned by whole-step invariants (mirror, transpose,
  * translation, boundedness).  A mistake in them that is symmetric and
@@ -201,6 +204,8 @@ void or_vbcw(const or_cfg *c, const R *jcb8w, const R *wc, R *wf)
   }
 }
 
+#include "cress_obc.c"
+
 /* s_phy2cnt, phy2cnt.f90:193-227 (trnopt=0) then vbcwc (:411).  The
  * advopt>=4 halo branch (:350-405) is not taken for advopt<=3. */
 void or_phy2cnt(const or_cfg *c, int sthopt, const R *jcb8w, const R *w, R *wc)
@@ -211,6 +216,7 @@ void or_phy2cnt(const or_cfg *c, int sthopt, const R *jcb8w, const R *w, R *wc)
     for (int j = 1; j <= nj - 1; j++)
       for (int i = 1; i <= ni - 1; i++)
         A3(wc, i, j, k) = sthopt == 0 ? A3(w, i, j, k) : A3(w, i, j, k) / A3(jcb8w, i, j, k);
+  /* the lateral fills of wc (:396-405) run only for advopt>=4 */
   or_vbcwc(c, wc);
 }
 
@@ -1045,17 +1051,25 @@ void or_gaussel(const or_cfg *c, int istr, int iend, int jstr, int jend, int kst
   }
 }
 
-/* Lateral fills after a velocity/scalar update at one rank (bcycle then the
- * corner average bc4news.f90:153-204 and lbc*, which act only on walls;
- * only periodic sides are supported, see or_cfg_check). */
+/* Lateral fills after a velocity/scalar update at one rank: bcycle on
+ * periodic sides; on open sides (option 7) the radiative fills of
+ * cress_obc.c and the corner average bc4news. */
 
-/* s_stepuv, stepuv.f90:288-310 update, then bcycle (:390-394), bc4news and
- * lbcu/lbcv (no-ops on periodic sides), vbcu/vbcv (:423-424).  The rbcu/rbcv
- * radiative fills (:266-280) act only on open sides (wbc>=4). */
+/* s_stepuv, stepuv.f90:266-280 radiative fills rbcu/rbcv (open sides),
+ * :288-310 update, then bcycle (:390-394, periodic sides), bc4news
+ * (:400-401, open sides), lbcu/lbcv (no-ops for option 7 with advopt<=3,
+ * lbcu.f90:159-200), vbcu/vbcv (:423-424).  cp: the phase speeds (NULL on a
+ * periodic domain). */
 void or_stepuv(const or_cfg *c, R dts, const R *rst8u, const R *rst8v, const R *ufrc,
-               const R *vfrc, const R *usml, const R *vsml, R *uf, R *vf)
+               const R *vfrc, const R *usml, const R *vsml, R *uf, R *vf, R *cp)
 {
   DECL_DIMS(c);
+  R *cx[5], *cy[5];
+  if (cp) {
+    or_cp_split(c, cp, cx, cy);
+    or_rbcu(c, cx[0], cy[0], uf);
+    or_rbcv(c, cx[1], cy[1], vf);
+  }
   OMP_FOR
   for (int k = 2; k <= nk - 2; k++) {
     for (int j = 2; j <= nj - 2; j++)
@@ -1067,6 +1081,8 @@ void or_stepuv(const or_cfg *c, R dts, const R *rst8u, const R *rst8v, const R *
   }
   or_bcycle(c, 3, 1, ni - 2, ni, 2, 1, nj - 2, nj - 1, nk, uf);
   or_bcycle(c, 2, 1, ni - 2, ni - 1, 3, 1, nj - 2, nj, nk, vf);
+  or_bc4news(c, 1, ni, 1, nj - 1, nk, uf);
+  or_bc4news(c, 1, ni - 1, 1, nj, nk, vf);
   or_vbcu(c, uf);
   or_vbcv(c, vf);
 }
@@ -1077,9 +1093,14 @@ void or_stepuv(const or_cfg *c, R dts, const R *rst8u, const R *rst8v, const R *
  * tmp1..tmp3: sub/diag/super diagonals. */
 void or_stepwi(const or_cfg *c, R dts, const R *jcb, const R *jcb8w, const R *rbr,
                const R *rst, const R *rst8w, const R *rcsq, R *fw, R *wf, R *wc,
-               R *tmp1, R *tmp2, R *tmp3)
+               R *tmp1, R *tmp2, R *tmp3, R *cp)
 {
   DECL_DIMS(c);
+  if (cp) {   /* rbcw on open sides (stepwi.f90:286-296) */
+    R *cx[5], *cy[5];
+    or_cp_split(c, cp, cx, cy);
+    or_rbcw(c, cx[2], cy[2], wf);
+  }
   const R dziv = (R)1 / (R)c->dz;
   const R weicoe = (R)c->weicoe;
   const R g05 = K(0.5) * GRAV;
@@ -1133,20 +1154,28 @@ void or_stepwi(const or_cfg *c, R dts, const R *jcb, const R *jcb8w, const R *rb
       A3(wf, i, j, 3) = A3(wf, i, j, 3) + K(0.25) * A3(tmp1, i, j, 3) * (K(0.0) + K(0.0));
   or_gaussel(c, 2, ni - 2, 2, nj - 2, 3, nk - 2, tmp1, tmp2, tmp3, wf);
   or_bcycle(c, 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk, wf);
+  or_bc4news(c, 1, ni - 1, 1, nj - 1, nk, wf);
   or_phy2cnt(c, 1, jcb8w, wf, wc);
   or_vbcw(c, jcb8w, wc, wf);
 }
 
-/* s_steppi, steppi.f90:211-223 update, bcycle (:293), vbcp (:316). */
-void or_steppi(const or_cfg *c, R dts, const R *jcb, const R *fp, R *ppf)
+/* s_steppi, steppi.f90: rbcss on open sides, :211-223 update, bcycle
+ * (:293), bc4news, lbcs (a no-op for option 7 with advopt<=3), vbcp (:316). */
+void or_steppi(const or_cfg *c, R dts, const R *jcb, const R *fp, R *ppf, R *cp)
 {
   DECL_DIMS(c);
+  if (cp) {
+    R *cx[5], *cy[5];
+    or_cp_split(c, cp, cx, cy);
+    or_rbcss(c, cx[3], cy[3], ppf);
+  }
   OMP_FOR
   for (int k = 2; k <= nk - 2; k++)
     for (int j = 2; j <= nj - 2; j++)
       for (int i = 2; i <= ni - 2; i++)
         A3(ppf, i, j, k) = A3(ppf, i, j, k) + dts * A3(fp, i, j, k) / A3(jcb, i, j, k);
   or_bcycle(c, 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk, ppf);
+  or_bc4news(c, 1, ni - 1, 1, nj - 1, nk, ppf);
   or_vbcp(c, ppf);
 }
 
@@ -1174,6 +1203,8 @@ typedef struct {
   R *prwtr, *price;
   /* scratch: 3-D fields; at least NSCR of them */
   R *scr;
+  /* open-boundary phase speeds (or_cp_size floats), NULL when periodic */
+  R *cp;
 } or_state;
 
 #define NSCR 24
@@ -1246,7 +1277,7 @@ static void or_smalldrv(const or_cfg *c, or_state *s, int nsstp, R dts,
   for (int isstp = 1; isstp <= nsstp; isstp++) {
     or_pgrad(c, dts, s->jcb, s->rst8u, s->rst8v, s->rst8w, s->u[F], s->v[F], wc, s->pp[F],
              tmp[1], tmp[2], tmp[3], tmp[4], tmp[5], tmp[6]);
-    or_stepuv(c, dts, s->rst8u, s->rst8v, ufrc, vfrc, tmp[1], tmp[2], s->u[F], s->v[F]);
+    or_stepuv(c, dts, s->rst8u, s->rst8v, ufrc, vfrc, tmp[1], tmp[2], s->u[F], s->v[F], s->cp);
     or_diverpih(c, s->jcb8u, s->jcb8v, s->rcsq, s->u[F], s->v[F], tmp[2], tmp[4], tmp[5]);
     or_diverpiv(c, s->rcsq, s->w[F], tmp[4]);
     or_advbspi(c, 0, s->rst, s->w[F], pfrc, tmp[2], tmp[4], tmp[1]);
@@ -1254,14 +1285,17 @@ static void or_smalldrv(const or_cfg *c, or_state *s, int nsstp, R dts,
     if (c->buyopt == 1)
       or_buoywsi(c, dts, s->rbr, s->rst, s->rcsq, s->pp[F], tmp[1], tmp[3], tmp[4]);
     or_stepwi(c, dts, s->jcb, s->jcb8w, s->rbr, s->rst, s->rst8w, s->rcsq, tmp[3], s->w[F], wc,
-              tmp[4], tmp[5], tmp[6]);
+              tmp[4], tmp[5], tmp[6], s->cp);
     or_diverpiv(c, s->rcsq, s->w[F], tmp[4]);
     or_advbspi(c, 1, s->rst, s->w[F], pfrc, tmp[3], tmp[4], tmp[1]);
-    or_steppi(c, dts, s->jcb, tmp[1], s->pp[F]);
+    or_steppi(c, dts, s->jcb, tmp[1], s->pp[F], s->cp);
   }
 }
 
-/* s_steps, steps.f90:520-760 update, bcycle (:3979-4171), vbcs (tail). */
+/* s_steps, steps.f90:520-760 update, the radiative fills on open sides
+ * (rbcpt/rbcqv/rbcq/rbcs, :1024-1230; lbcs is a no-op for option 7 with
+ * advopt<=3), bcycle (:3979-4171, periodic sides), bc4news (:4182-4351,
+ * open sides), vbcs (tail). */
 static void or_steps(const or_cfg *c, or_state *s, R dtb, const R *ptfrc, const R *qvfrc, R *dtdrst)
 {
   DECL_DIMS(c);
@@ -1289,9 +1323,21 @@ static void or_steps(const or_cfg *c, or_state *s, R dtb, const R *ptfrc, const 
         s->nci[F][p] = s->nci[P][p] + s->nci[F][p] * d;
       }
   const int nq = c->cphopt == 0 ? 1 : NQ;
+  if (s->cp) {
+    R *cx[5], *cy[5];
+    or_cp_split(c, s->cp, cx, cy);
+    const int C = 1;
+    or_rbclf(c, 0, cx[4], cy[4], s->ptp[C], s->ptp[P], s->ptp[F]);
+    for (int sp = 0; sp < nq; sp++)
+      or_rbclf(c, 1, cx[4], cy[4], s->q[C] + sp * n, s->q[P] + sp * n, s->q[F] + sp * n);
+    if (c->cphopt) or_rbclf(c, 0, cx[4], cy[4], s->nci[C], s->nci[P], s->nci[F]);
+  }
   or_bcycle(c, 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk, s->ptp[F]);
   for (int sp = 0; sp < nq; sp++) or_bcycle(c, 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk, s->q[F] + sp * n);
   if (c->cphopt) or_bcycle(c, 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk, s->nci[F]);
+  or_bc4news(c, 1, ni - 1, 1, nj - 1, nk, s->ptp[F]);
+  for (int sp = 0; sp < nq; sp++) or_bc4news(c, 1, ni - 1, 1, nj - 1, nk, s->q[F] + sp * n);
+  if (c->cphopt) or_bc4news(c, 1, ni - 1, 1, nj - 1, nk, s->nci[F]);
   or_vbcs(c, s->ptp[F]);
   for (int sp = 0; sp < nq; sp++) or_vbcs(c, s->q[F] + sp * n);
   if (c->cphopt) or_vbcs(c, s->nci[F]);
@@ -1327,7 +1373,8 @@ void or_timeflt(const or_cfg *c, or_state *s)
 }
 
 /* s_shift2nd at one rank (shift2nd.f90:4274-4302): second halo row for the
- * 4th-order stencils. */
+ * 4th-order stencils (periodic sides); on open sides only the corners of
+ * the outer ring, bc4news (:4590-4700). */
 void or_shift2nd(const or_cfg *c, or_state *s, int lv)
 {
   const int ni = c->ni, nj = c->nj, nk = c->nk;
@@ -1340,6 +1387,13 @@ void or_shift2nd(const or_cfg *c, or_state *s, int lv)
   const int nq = c->cphopt == 0 ? 1 : NQ;
   for (int sp = 0; sp < nq; sp++) or_bcycle(c, 3, 0, ni - 3, ni, 3, 0, nj - 3, nj, nk, s->q[lv] + sp * n);
   if (c->cphopt) or_bcycle(c, 3, 0, ni - 3, ni, 3, 0, nj - 3, nj, nk, s->nci[lv]);
+  or_bc4news(c, 0, ni + 1, 0, nj, nk, s->u[lv]);
+  or_bc4news(c, 0, ni, 0, nj + 1, nk, s->v[lv]);
+  or_bc4news(c, 0, ni, 0, nj, nk, s->w[lv]);
+  or_bc4news(c, 0, ni, 0, nj, nk, s->pp[lv]);
+  or_bc4news(c, 0, ni, 0, nj, nk, s->ptp[lv]);
+  for (int sp = 0; sp < nq; sp++) or_bc4news(c, 0, ni, 0, nj, nk, s->q[lv] + sp * n);
+  if (c->cphopt) or_bc4news(c, 0, ni, 0, nj, nk, s->nci[lv]);
 }
 
 /* s_swp2nxt, swp2nxt.f90:354-...: past <- present <- future over the
@@ -1377,7 +1431,9 @@ void or_swp2nxt(const or_cfg *c, or_state *s)
 int or_cfg_check(const or_cfg *c)
 {
   if (c->ni < 7 || c->nj < 7 || c->nk < 7) return -1;
-  if (!(c->wbc == -1 && c->ebc == -1 && c->sbc == -1 && c->nbc == -1)) return -2;
+  const int per = c->wbc == -1 && c->ebc == -1 && c->sbc == -1 && c->nbc == -1;
+  const int opn = c->wbc == 7 && c->ebc == 7 && c->sbc == 7 && c->nbc == 7;
+  if (!per && !opn) return -2;
   if (c->bbc != 2 && c->bbc != 3) return -3;
   if (c->tbc != 2 && c->tbc != 3) return -3;
   if (c->advopt < 1 || c->advopt > 3) return -4;
@@ -1401,6 +1457,7 @@ static or_micro_fn g_satadj = satadj_hook;
 int or_large_step(const or_cfg *c, or_state *s, int first, double dtbig, double dtsml, int nsstp)
 {
   if (or_cfg_check(c)) return or_cfg_check(c);
+  if (or_open(c) && !s->cp) return -8;
   const size_t n = (size_t)nfield(c);
   R *ufrc = s->scr, *vfrc = s->scr + n, *wfrc = s->scr + 2 * n, *pfrc = s->scr + 3 * n;
   R *ptfrc = s->scr + 4 * n, *qvfrc = s->scr + 5 * n, *wc = s->scr + 6 * n;
@@ -1419,6 +1476,7 @@ int or_large_step(const or_cfg *c, or_state *s, int first, double dtbig, double 
   if (c->cphopt == 2 && g_satadj) g_satadj(c, s, dtb, NULL);
   or_shift2nd(c, s, 1);
   or_shift2nd(c, s, 2);
+  if (s->cp) or_phasev(c, dtb, dts, s->cp);
   or_swp2nxt(c, s);
   return 0;
 }

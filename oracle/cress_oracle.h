@@ -30,6 +30,7 @@ typedef struct {
   int ksp0[2];
   char vspvar[10];                    /* 'o'/'x' per u,v,w,p,pt,qv,q,... */
   double dx, dy, dz, weicoe, filcoe, thresq;
+  double gwave;                       /* open-boundary phase speed (m/s) */
 } or_cfg;
 
 #endif

@@ -27,19 +27,24 @@ class Cfg(C.Structure):
                 ("iwest", I), ("ieast", I), ("jsouth", I), ("jnorth", I),
                 ("ksp0", I * 2), ("vspvar", C.c_char * 10),
                 ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
-                ("weicoe", C.c_double), ("filcoe", C.c_double), ("thresq", C.c_double)]
+                ("weicoe", C.c_double), ("filcoe", C.c_double), ("thresq", C.c_double),
+                ("gwave", C.c_double)]
 
 
 def make_cfg(ni, nj, nk, dx=1000.0, dy=1000.0, dz=500.0, advopt=3, divopt=1, buyopt=1,
              cphopt=2, vspopt=2, coropt=0, sthopt=1, bbc=2, tbc=2, ksp0=(2, 2),
-             vspvar=b"oooxoooooo", weicoe=0.6, filcoe=0.1, thresq=1e-12):
+             vspvar=b"oooxoooooo", weicoe=0.6, filcoe=0.1, thresq=1e-12, wbc=-1, gwave=0.0):
+    """wbc: -1 periodic or 7 open (radiation with the constant phase speed
+    gwave) on all four sides."""
     c = Cfg()
     c.ni, c.nj, c.nk = ni, nj, nk
-    c.wbc = c.ebc = c.sbc = c.nbc = -1
+    c.wbc = c.ebc = c.sbc = c.nbc = wbc
     c.bbc, c.tbc = bbc, tbc
     c.advopt, c.impopt, c.gwmopt, c.buyopt, c.divopt = advopt, 1, 0, buyopt, divopt
     c.cphopt, c.haiopt, c.vspopt, c.coropt, c.sthopt = cphopt, 0, vspopt, coropt, sthopt
-    c.iwest = c.ieast = c.jsouth = c.jnorth = 0
+    # 4th-order reduction next to non-periodic sides (sethalo.f90:536-580)
+    c.iwest = c.ieast = c.jsouth = c.jnorth = 0 if wbc == -1 else 1
+    c.gwave = gwave
     c.ksp0[0], c.ksp0[1] = ksp0
     c.vspvar = vspvar
     c.dx, c.dy, c.dz = dx, dy, dz
@@ -127,7 +132,7 @@ NSCR = 24
 
 class State(C.Structure):
     _fields_ = ([(b, P) for b in BASE] + [("fc", P), ("rbct", P)] +
-                [(p, P * 3) for p in PROG] + [("prwtr", P), ("price", P), ("scr", P)])
+                [(p, P * 3) for p in PROG] + [("prwtr", P), ("price", P), ("scr", P), ("cp", P)])
 
 
 def cfg_from_solver_config(sc):
@@ -136,7 +141,7 @@ def cfg_from_solver_config(sc):
                  divopt=sc.divopt, buyopt=sc.buyopt, cphopt=sc.cphopt, vspopt=sc.vspopt,
                  coropt=sc.coropt, sthopt=sc.sthopt, bbc=sc.bbc, tbc=sc.tbc,
                  ksp0=(sc.ksp0[0], sc.ksp0[1]), vspvar=bytes(sc.vspvar), weicoe=sc.weicoe,
-                 filcoe=sc.filcoe, thresq=sc.thresq)
+                 filcoe=sc.filcoe, thresq=sc.thresq, wbc=sc.wbc, gwave=sc.gwave)
     return c
 
 
@@ -176,6 +181,13 @@ class Model:
             for l in range(3):
                 arr[l] = self.lv[p][l].ctypes.data
         st.prwtr, st.price, st.scr = self.prwtr.ctypes.data, self.price.ctypes.data, self.scr.ctypes.data
+        # open-boundary phase speeds: zero until the first step's phasev
+        # (allocslv.f90:2015 zeroes them; a cold start reads no restart)
+        self.cp = None
+        if sc.wbc == 7:
+            self.o.lib.or_cp_size.restype = C.c_size_t
+            self.cp = np.zeros(self.o.lib.or_cp_size(C.byref(self.cfg)), self.dt)
+            st.cp = self.cp.ctypes.data
         self.state = st
 
     def step(self, nsteps=1, first=False):

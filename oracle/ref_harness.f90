@@ -54,6 +54,22 @@
       use m_upwqp
       use m_upwnp
       use m_satadjst
+      use m_phvuvw
+      use m_phvs
+      use m_rbcu
+      use m_rbcv
+      use m_rbcw
+      use m_rbcss
+      use m_rbcpt
+      use m_rbcqv
+      use m_rbcq
+      use m_rbcs
+      use m_bc4news
+      use m_lbcu
+      use m_lbcv
+      use m_lbcw
+      use m_lbcs
+      use m_lbcwc
 
       implicit none
 
@@ -508,5 +524,113 @@
       call s_satadjst(idcphopt,ni,nj,nk,2,2,3,3,pbr,ptbr,w,pp,ptp,q(1),  &
      &                q(1+n),nw,q(1+3*n),nic,pi,p)
       end subroutine ref_satadjst
+
+! Open lateral boundaries (wbc..nbc >= 4): phase speeds and radiation
+! fills with the namelist values ref_configure left (exbopt=0, nggopt=0,
+! lspopt=0, lbcvar/gpvvar all 'x', lbnews=lbnorm=0).  gwave is set here.
+      subroutine ref_phvuvw(gwave,dtb,dts,ni,nj,nk,rmf,rmf8u,rmf8v,      &
+     &   u,ucpx,ucpy,vcpx,vcpy,wcpx,wcpy,u8v,v8u,cpavex,cpavey)          &
+     &   bind(C,name='ref_phvuvw')
+      real, value :: gwave,dtb,dts
+      integer(c_int), value :: ni,nj,nk
+      real rmf(*),rmf8u(*),rmf8v(*),u(*)
+      real ucpx(*),ucpy(*),vcpx(*),vcpy(*),wcpx(*),wcpy(*)
+      real u8v(*),v8u(*),cpavex(*),cpavey(*)
+      rname(idgwave)=gwave
+      call s_phvuvw(idexbvar,idexbopt,idwbc,idebc,idsbc,idnbc,idmpopt,   &
+     &              idmfcopt,iddxiv,iddyiv,idgwave,dtb,dts,ni,nj,nk,     &
+     &              rmf,rmf8u,rmf8v,u,u,u,u,u,u,u,u,u,                   &
+     &              ucpx,ucpy,vcpx,vcpy,wcpx,wcpy,u8v,v8u,cpavex,cpavey)
+      end subroutine ref_phvuvw
+
+      subroutine ref_phvs(gwave,sml,dtb,dts,ni,nj,nk,rmf,u,scpx,scpy,    &
+     &   cpavex,cpavey) bind(C,name='ref_phvs')
+      real, value :: gwave,dtb,dts
+      integer(c_int), value :: sml,ni,nj,nk
+      real rmf(*),u(*),scpx(*),scpy(*),cpavex(*),cpavey(*)
+      character(len=3) fproc
+      rname(idgwave)=gwave
+      fproc='big'
+      if(sml.ne.0) fproc='sml'
+      call s_phvs(idexbvar,idexbopt,idwbc,idebc,idsbc,idnbc,idadvopt,    &
+     &            idmpopt,idmfcopt,iddxiv,iddyiv,idgwave,fproc,4,dtb,dts, &
+     &            dts,ni,nj,nk,rmf,u,u,u,u,u,scpx,scpy,cpavex,cpavey)
+      end subroutine ref_phvs
+
+! kind: 1 rbcu, 2 rbcv, 3 rbcw, 4 rbcss (one field updated in place, its
+! base/forcing companions dummies)
+      subroutine ref_rbcuvw(kind,dts,ni,nj,nk,cpx,cpy,dum,var)           &
+     &   bind(C,name='ref_rbcuvw')
+      integer(c_int), value :: kind,ni,nj,nk
+      real, value :: dts
+      real cpx(*),cpy(*),dum(*),var(*)
+      if(kind.eq.1) then
+        call s_rbcu(idlbcvar,idwbc,idebc,idsbc,idnbc,idnggopt,idlspopt,  &
+     &              idvspopt,idlbnews,idlbnorm,1,dts,0.e0,ni,nj,nk,dum,  &
+     &              cpx,cpy,dum,dum,var)
+      else if(kind.eq.2) then
+        call s_rbcv(idlbcvar,idwbc,idebc,idsbc,idnbc,idnggopt,idlspopt,  &
+     &              idvspopt,idlbnews,idlbnorm,1,dts,0.e0,ni,nj,nk,dum,  &
+     &              cpx,cpy,dum,dum,var)
+      else if(kind.eq.3) then
+        call s_rbcw(idgpvvar,idlbcvar,idwbc,idebc,idsbc,idnbc,idnggopt,  &
+     &              idlspopt,idvspopt,idlbnews,1,dts,0.e0,ni,nj,nk,      &
+     &              cpx,cpy,dum,dum,var)
+      else
+        call s_rbcss(idlbcvar,idwbc,idebc,idsbc,idnbc,idnggopt,idlspopt, &
+     &               idvspopt,idlbnews,4,1,dts,0.e0,ni,nj,nk,cpx,cpy,    &
+     &               dum,dum,var)
+      end if
+      end subroutine ref_rbcuvw
+
+! kind: 1 rbcpt, 2 rbcqv, 3 rbcq (qc slot), 4 rbcs (nci slot)
+      subroutine ref_rbclf(kind,dt,ni,nj,nk,s,sp,cpx,cpy,dum,sf)         &
+     &   bind(C,name='ref_rbclf')
+      integer(c_int), value :: kind,ni,nj,nk
+      real, value :: dt
+      real s(*),sp(*),cpx(*),cpy(*),dum(*),sf(*)
+      if(kind.eq.1) then
+        call s_rbcpt(idlbcvar,idwbc,idebc,idsbc,idnbc,idnggopt,idlspopt, &
+     &               idvspopt,idadvopt,idlbnews,1,dt,0.e0,ni,nj,nk,s,sp, &
+     &               cpx,cpy,dum,dum,sf)
+      else if(kind.eq.2) then
+        call s_rbcqv(idgpvvar,idlbcvar,idwbc,idebc,idsbc,idnbc,idnggopt, &
+     &               idlspopt,idvspopt,idadvopt,idlbnews,1,dt,0.e0,      &
+     &               ni,nj,nk,dum,s,sp,cpx,cpy,dum,dum,sf)
+      else if(kind.eq.3) then
+        call s_rbcq(idgpvvar,idlbcvar,idwbc,idebc,idsbc,idnbc,idnggopt,  &
+     &              idlspopt,idvspopt,idadvopt,idlbnews,3,7,1,dt,0.e0,   &
+     &              ni,nj,nk,s,sp,cpx,cpy,dum,dum,sf)
+      else
+        call s_rbcs(idlbcvar,idwbc,idebc,idsbc,idnbc,idadvopt,idlbnews,  &
+     &              7,dt,ni,nj,nk,s,sp,cpx,cpy,sf)
+      end if
+      end subroutine ref_rbclf
+
+      subroutine ref_bc4news(isw,ise,jss,jsn,ni,nj,kmax,var)             &
+     &   bind(C,name='ref_bc4news')
+      integer(c_int), value :: isw,ise,jss,jsn,ni,nj,kmax
+      real var(*)
+      call s_bc4news(idwbc,idebc,idsbc,idnbc,isw,ise,jss,jsn,ni,nj,kmax, &
+     &               var)
+      end subroutine ref_bc4news
+
+! kind: 1 lbcu, 2 lbcv, 3 lbcw, 4 lbcs, 5 lbcwc
+      subroutine ref_lbc(kind,ni,nj,nk,var) bind(C,name='ref_lbc')
+      integer(c_int), value :: kind,ni,nj,nk
+      real var(*)
+      select case(kind)
+      case(1)
+        call s_lbcu(idwbc,idebc,idsbc,idnbc,idadvopt,ni,nj,nk,var)
+      case(2)
+        call s_lbcv(idwbc,idebc,idsbc,idnbc,idadvopt,ni,nj,nk,var)
+      case(3)
+        call s_lbcw(idwbc,idebc,idsbc,idnbc,idadvopt,ni,nj,nk,var)
+      case(4)
+        call s_lbcs(idwbc,idebc,idsbc,idnbc,idadvopt,ni,nj,nk,var)
+      case default
+        call s_lbcwc(idwbc,idebc,idsbc,idnbc,ni,nj,nk,var)
+      end select
+      end subroutine ref_lbc
 
       end module cress_ref_harness

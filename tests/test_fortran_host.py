@@ -51,7 +51,7 @@ def test_fortran_host_resolves_options(tmp_path):
     assert [int(x) for x in kv["ksp0_nsstp"]] == [c.ksp0[0], c.ksp0[1], c.nsstp]
     assert kv["vspvar"][0] == bytes(c.vspvar).decode()
     reals = [float(x) for x in kv["reals"]]
-    want = [c.dx, c.dy, c.dz, c.dtbig, c.dtsml, c.weicoe, c.filcoe, c.thresq]
+    want = [c.dx, c.dy, c.dz, c.dtbig, c.dtsml, c.weicoe, c.filcoe, c.thresq, c.gwave]
     # user.conf values pass through default REAL, as in the reference
     np.testing.assert_array_equal(np.float32(reals), np.float32(want))
     assert int(kv["nsteps"][0]) == 4

@@ -46,10 +46,13 @@ def run_both(libs, cfg, name_or, name_ref, or_args, ref_args):
     getattr(r, name_ref)(*ref_args)
 
 
+# wbc=7: open sides, where the 4th-order terms drop to 2nd order next to
+# the boundary (iwest..jnorth = 1, sethalo.f90:536-580)
+@pytest.mark.parametrize("wbc", [-1, 7])
 @pytest.mark.parametrize("advopt", [1, 2, 3])
-def test_advs(libs, advopt):
+def test_advs(libs, advopt, wbc):
     rng = np.random.default_rng(10 + advopt)
-    c = cfg_for(advopt)
+    c = cfg_for(advopt, wbc=wbc)
     ux, vx, wx, s = (rnd(c, rng) for _ in range(4))
     outs = [[rnd(c, rng) for _ in range(5)]]
     outs.append([a.copy(order="F") for a in outs[0]])
@@ -60,10 +63,11 @@ def test_advs(libs, advopt):
     same(outs[0][0], outs[1][0])   # sfrc
 
 
+@pytest.mark.parametrize("wbc", [-1, 7])
 @pytest.mark.parametrize("advopt", [1, 2, 3])
-def test_advuvw(libs, advopt):
+def test_advuvw(libs, advopt, wbc):
     rng = np.random.default_rng(20 + advopt)
-    c = cfg_for(advopt)
+    c = cfg_for(advopt, wbc=wbc)
     ins = [rnd(c, rng) for _ in range(6)]
     a = [rnd(c, rng) for _ in range(8)]
     b = [x.copy(order="F") for x in a]
@@ -75,9 +79,10 @@ def test_advuvw(libs, advopt):
         same(x, y)
 
 
-def test_rstuvwc(libs):
+@pytest.mark.parametrize("wbc", [-1, 7])
+def test_rstuvwc(libs, wbc):
     rng = np.random.default_rng(40)
-    c = cfg_for()
+    c = cfg_for(wbc=wbc)
     r8u, r8v, r8w, u, v, wc = (rnd(c, rng) for _ in range(6))
     mf = rnd(c, rng)
     a = [rnd(c, rng) for _ in range(3)]
@@ -161,9 +166,10 @@ def forcing_inputs(c, rng):
     return d
 
 
-def test_forcep(libs):
+@pytest.mark.parametrize("wbc", [-1, 7])
+def test_forcep(libs, wbc):
     rng = np.random.default_rng(80)
-    c = cfg_for(ksp0=(4, 5), vspvar=b"oooooooooo")
+    c = cfg_for(ksp0=(4, 5), vspvar=b"oooooooooo", wbc=wbc)
     d = forcing_inputs(c, rng)
     dmy = rnd(c, rng, n=2)
     a = [rnd(c, rng) for _ in range(9)]
