@@ -12,6 +12,7 @@
 #include "common.h"
 #include "dyn.h"
 #include "micro.h"
+#include "obc.h"
 
 #include <cstring>
 #include <utility>
@@ -47,12 +48,12 @@ enum { NSTAGE = 8 };
 enum {
   KT_PREP, KT_FORCE_UV, KT_FORCE_W, KT_FORCE_P, KT_FORCE_S, KT_PHY2CNT, KT_AC_PTUV,
   KT_AC_COL, KT_AC_RING, KT_FILL, KT_VBC, KT_STEPS, KT_TIMEFLT, KT_COPY, KT_MICRO_SEED, KT_MICRO_SRC,
-  KT_MICRO_FALL, KT_SATADJ, KT_N
+  KT_MICRO_FALL, KT_SATADJ, KT_OBC, KT_N
 };
 static const char *kKernelNames[KT_N] = {
     "k_prep", "k_force_uv", "k_force_w", "k_force_p", "k_force_s", "k_phy2cnt", "k_ac_ptuv",
     "k_ac_col", "k_ac_ring", "k_bcycle", "k_vbc", "k_steps", "k_timeflt", "memcpy_d2d",
-    "k_micro_seed", "k_micro_src", "k_micro_fall", "k_satadjst"};
+    "k_micro_seed", "k_micro_src", "k_micro_fall", "k_satadjst", "k_obc"};
 enum { MAXEV = 1024 };
 
 struct cress_ctx {
@@ -74,6 +75,10 @@ struct cress_ctx {
   float *ualt, *valt;  // second buffers of the future uf, vf during the acoustic loop
   float *sfrc;         // forcings of qc, qr, qi, qs, qg, nci (cphopt=2)
   MicroScratch ms;
+  // open sides: dtb, dts of the last large step (the phase speeds phasev
+  // leaves for the next one); obc_have=0 before any step
+  float obc_dtb, obc_dts;
+  int obc_have;
   std::vector<void *> allocs;
   int timing;
   hipEvent_t ev[NSTAGE + 1];
@@ -120,7 +125,10 @@ static int check_config(const cress_config *c) {
   if (c->nsstp < 1 || !(c->dx > 0) || !(c->dy > 0) || !(c->dz > 0)) return CRESS_EINVAL;
   // kernels address within a field by 32-bit byte offsets
   if ((long)(c->ni + 2) * (long)(c->nj + 2) * (long)c->nk * 4L >= (1L << 32)) return CRESS_EUNSUPPORTED;
-  if (!(c->wbc == -1 && c->ebc == -1 && c->sbc == -1 && c->nbc == -1)) return CRESS_EUNSUPPORTED;
+  const int periodic = c->wbc == -1 && c->ebc == -1 && c->sbc == -1 && c->nbc == -1;
+  const int open = c->wbc == 7 && c->ebc == 7 && c->sbc == 7 && c->nbc == 7;
+  if (!periodic && !open) return CRESS_EUNSUPPORTED;
+  if (open && !(c->gwave >= 0.0 && c->gwave < 1.0e6)) return CRESS_EINVAL;
   if ((c->bbc != 2 && c->bbc != 3) || (c->tbc != 2 && c->tbc != 3)) return CRESS_EUNSUPPORTED;
   if (c->advopt < 1 || c->advopt > 3 || c->impopt != 1 || c->gwmopt != 0) return CRESS_EUNSUPPORTED;
   if (c->divopt < 0 || c->divopt > 1 || c->buyopt < 0 || c->buyopt > 1) return CRESS_EUNSUPPORTED;
@@ -436,6 +444,53 @@ static inline void stamp(cress_ctx *x, int s) {
   if (x->timing) hipEventRecord(x->ev[s], x->stream);
 }
 
+// Phase speeds of the open-side fills for the coming step (phvuvw.f90:334-346
+// and the option-7 branches; phvs.f90:270-290), from the previous step's dtb
+// and dts, in binary32 in the reference's order: gdxdt = gwave*dtb*dxiv,
+// clipped to [-1,1]; u, v, w and p' scale it by dts/dtb, the large-step
+// scalars use gamma = 2c/(1-+c).  Zero before the first step of a cold start
+// (allocslv.f90:2015); a context whose first step is not the model's first
+// takes the previous step to have been a full one (dtbig, dtsml), which is
+// what a restart file written after it holds.
+static void obc_speeds(cress_ctx *x, int first, ObcAc *ac, ObcLf *lf) {
+  *ac = ObcAc{0.0f, 0.0f, 0.0f, 0.0f};
+  *lf = ObcLf{0.0f, 0.0f, 0.0f, 0.0f};
+  if (!x->obc_have) {
+    if (first) return;
+    x->obc_dtb = (float)x->cfg.dtbig;
+    x->obc_dts = (float)x->cfg.dtsml;
+    x->obc_have = 1;
+  }
+  const float dtb = x->obc_dtb, dts = x->obc_dts;
+  const float gwave = (float)x->cfg.gwave;
+  const float gx = gwave * dtb * x->c.dxiv, gy = gwave * dtb * x->c.dyiv;
+  const float bw = fmaxf(-gx, -1.0f), be = fminf(gx, 1.0f);
+  const float bs = fmaxf(-gy, -1.0f), bn = fminf(gy, 1.0f);
+  const float r = dts / dtb;
+  ac->cw = bw * r;
+  ac->ce = be * r;
+  ac->cs = bs * r;
+  ac->cn = bn * r;
+  lf->gw = 2.0f * bw / (1.0f - bw);
+  lf->ge = 2.0f * be / (1.0f + be);
+  lf->gs = 2.0f * bs / (1.0f - bs);
+  lf->gn = 2.0f * bn / (1.0f + bn);
+}
+
+static void corner_spec(CornerSet *cs, float *p, int isw, int ise, int jss, int jsn) {
+  const int f = cs->n++;
+  cs->ptr[f] = p;
+  cs->isw[f] = isw;
+  cs->ise[f] = ise;
+  cs->jss[f] = jss;
+  cs->jsn[f] = jsn;
+}
+
+static void run_corners(cress_ctx *x, const CornerSet &cs) {
+  if (cs.n == 0) return;
+  LAUNCH(x, KT_OBC, k_obc_corners, dim3((x->g.nk + 63) / 64, cs.n), dim3(64), 0, x->g, cs);
+}
+
 // ---------------------------------------------------------------------------
 // One large step.
 // ---------------------------------------------------------------------------
@@ -452,6 +507,10 @@ static int large_step(cress_ctx *x, int first) {
   const float dts = first ? 0.5f * (float)cf.dtsml : (float)cf.dtsml;
   float **B = x->base;
   const dim3 b2(64, 4);
+  const int open = cf.wbc == 7;
+  ObcAc oac;
+  ObcLf olf;
+  obc_speeds(x, first, &oac, &olf);
 
   stamp(x, 0);
   // ---- forcing (forcedrv) on the present level ----
@@ -574,8 +633,30 @@ static int large_step(cress_ctx *x, int first) {
     LAUNCH(x, KT_AC_PTUV, k_ac_ptuv, flat(tpu), dim3(64, CRESS_PTUV_BY), 0, tpu, g, o, c, pu);
     std::swap(x->u[F], x->ualt);
     std::swap(x->v[F], x->valt);
+    if (open) {
+      // rbcu/rbcv from the previous substep's u, v (stepuv.f90:266-280),
+      // rbcw and rbcss on w, p' before the column kernel updates the
+      // interior next to them (stepwi.f90:286-296, steppi), then the corner
+      // means of stepuv, stepwi and steppi (bc4news)
+      ObcAcArgs oa;
+      oa.uold = x->ualt;
+      oa.vold = x->valt;
+      oa.unew = x->u[F];
+      oa.vnew = x->v[F];
+      oa.wf = x->w[F];
+      oa.ppf = x->pp[F];
+      LAUNCH(x, KT_OBC, k_obc_ac_edges, dim3((2 * (nj + 2) + 2 * (ni + 2) + 63) / 64, nk), dim3(64), 0, g,
+             oac, oa);
+      CornerSet cs;
+      cs.n = 0;
+      corner_spec(&cs, x->u[F], 1, ni, 1, nj - 1);
+      corner_spec(&cs, x->v[F], 1, ni - 1, 1, nj);
+      corner_spec(&cs, x->w[F], 1, ni - 1, 1, nj - 1);
+      corner_spec(&cs, x->pp[F], 1, ni - 1, 1, nj - 1);
+      run_corners(x, cs);
+    }
     FillSet fuv;
-    fuv.n = 2;
+    fuv.n = open ? 0 : 2;
     fill_spec(&fuv.f[0], x->u[F], 3, 1, ni - 2, ni, 2, 1, nj - 2, nj - 1, nk);
     fill_spec(&fuv.f[1], x->v[F], 2, 1, ni - 2, ni - 1, 3, 1, nj - 2, nj, nk);
     VbcSet vuv;
@@ -589,7 +670,7 @@ static int large_step(cress_ctx *x, int first) {
     ac.uf = x->u[F];
     ac.vf = x->v[F];
     LAUNCH(x, KT_AC_COL, k_ac_col, flat(tcol), dim3(64, CRESS_COL_BY), CRESS_COL_LDS, tcol, g, o, c, ac);
-    run_fill(x, fwp);
+    if (!open) run_fill(x, fwp);
     LAUNCH(x, KT_AC_RING, k_ac_ring, dim3((nring + 63) / 64), dim3(64), 0, g, o, B[CRESS_JCB8W], x->w[F], x->wc,
            x->pp[F]);
   }
@@ -612,6 +693,17 @@ static int large_step(cress_ctx *x, int first) {
     sti.qn = n;
     LAUNCH(x, KT_STEPS, k_steps, flat(tiles(ni - 3, nj - 3, 64, 4, nk - 3)), b2, 0,
            tiles(ni - 3, nj - 3, 64, 4, nk - 3), g, 2.0f * dtb, B[CRESS_RST], sti, 0);
+    if (open) {
+      ObcStepsArgs osa;
+      osa.s[0] = x->ptp[C];
+      osa.sp[0] = x->ptp[P];
+      osa.sf[0] = x->ptp[F];
+      osa.s[1] = x->q[C];
+      osa.sp[1] = x->q[P];
+      osa.sf[1] = x->q[F];
+      const int nring = 2 * (ni - 1) + 2 * (nj - 3);
+      LAUNCH(x, KT_OBC, k_obc_steps, dim3((nring + 63) / 64, nk - 3, 2), dim3(64), 0, g, olf, osa);
+    }
     FillSet fs;
     VbcSet vs;
     fs.n = 2;
@@ -622,6 +714,7 @@ static int large_step(cress_ctx *x, int first) {
       vs.kind[s] = 0;
       vs.ptr[s] = fl[s];
     }
+    if (open) fs.n = 0;
     run_fill(x, fs);
     run_vbc(x, vs);
   }
@@ -653,6 +746,11 @@ static int large_step(cress_ctx *x, int first) {
     ma.ugq = x->ms.f[2];
     ma.dtpmin = (unsigned *)x->ms.col;
     ma.qn = n;
+    ma.ptpc = x->ptp[C];
+    ma.qc = x->q[C];
+    ma.ncic = x->nci[C];
+    ma.lf = olf;
+    ma.open = open;
     unsigned seed;
     memcpy(&seed, &dtbs, sizeof seed);
     LAUNCH(x, KT_MICRO_SEED, k_micro_seed, dim3(1), dim3(64), 0, ma.dtpmin, seed);
@@ -725,7 +823,19 @@ static int large_step(cress_ctx *x, int first) {
   stamp(x, 6);
 
   // ---- second halo rows for 4th-order advection (shift2nd) ----
-  for (int lv = 0; lv < 2; lv++) {
+  for (int lv = 0; lv < 2 && open; lv++) {
+    // open sides: only the corners of the outer ring (shift2nd.f90:4590-4700)
+    const int S = lv == 0 ? C : F;
+    CornerSet cs;
+    cs.n = 0;
+    corner_spec(&cs, x->u[S], 0, ni + 1, 0, nj);
+    corner_spec(&cs, x->v[S], 0, ni, 0, nj + 1);
+    float *sc[10] = {x->w[S], x->pp[S], x->ptp[S], x->q[S], x->q[S] + n, x->q[S] + 2 * n,
+                     x->q[S] + 3 * n, x->q[S] + 4 * n, x->q[S] + 5 * n, x->nci[S]};
+    for (int f = 0; f < (withq ? 10 : 4); f++) corner_spec(&cs, sc[f], 0, ni, 0, nj);
+    run_corners(x, cs);
+  }
+  for (int lv = 0; lv < 2 && !open; lv++) {
     const int S = lv == 0 ? C : F;
     FillSet fs;
     fs.n = withq ? 12 : 6;
@@ -737,6 +847,10 @@ static int large_step(cress_ctx *x, int first) {
     run_fill(x, fs);
   }
   stamp(x, 7);
+  // ---- phasev: this step's dtb, dts set the next step's phase speeds ----
+  x->obc_dtb = dtb;
+  x->obc_dts = dts;
+  x->obc_have = 1;
   // ---- swp2nxt: past <- present <- future by renaming ----
   x->rot = (x->rot + 1) % 3;
   stamp(x, 8);

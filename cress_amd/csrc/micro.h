@@ -3,6 +3,7 @@
 // (satadjst.f90:148-189).  The context's step driver launches them.
 #pragma once
 #include "common.h"
+#include "obc.h"
 
 namespace cress {
 
@@ -33,6 +34,11 @@ struct MicroArgs {
   float *urq, *usq, *ugq;               // fall speeds handed to k_micro_fall
   unsigned *dtpmin;                     // binary32 bits of the CFL minimum
   long qn;                              // floats per species block
+  // open lateral boundaries: the present level the radiative fills read and
+  // their gammas; open=0 on a periodic domain
+  const float *ptpc, *qc, *ncic;
+  ObcLf lf;
+  int open;
 };
 
 struct FallArgs {

@@ -95,23 +95,37 @@ __global__ void __launch_bounds__(256) k_micro_src(Tiles T, Grid g, MicroK K, Mi
                                    (1.e-2f / msmax) * qsp), (1.e2f / ms0) * qsp);
     const float ncgp = fminr(fmaxr(sqrtf(sqrtf((ng0 * ng0 * ng0 / (cc * rhog)) * rbr * qgp)) * rbv,
                                    (1.e-2f / mgmax) * qgp), (1.e2f / mg0) * qgp);
-    // future level: the steps leapfrog (steps.f90:520-760) of this cell's
-    // source -- its periodic image from bcycle (i=1 <- ni-2, i=ni-1 <- 2,
-    // the same in j) and vbcs's level copy (k=1 <- 2, k=nk-1 <- nk-2) --
-    // which is the value the reference's fills leave here
-    const int is = i == 1 ? g.ni - 2 : (i == g.ni - 1 ? 2 : i);
-    const int js = j == 1 ? g.nj - 2 : (j == g.nj - 1 ? 2 : j);
+    // future level: the value the reference's steps (steps.f90:520-760)
+    // and its fills leave at this cell.  vbcs copies levels (k=1 <- 2,
+    // k=nk-1 <- nk-2).  Periodic sides: the leapfrog of the cell's periodic
+    // image from bcycle (i=1 <- ni-2, i=ni-1 <- 2, the same in j).  Open
+    // sides: the leapfrog inside; on the boundary ring the radiative fills
+    // (rbcpt, rbcqv, rbcq, rbcs) and bc4news's corner means.
     const int ks = k == 1 ? 2 : (k == g.nk - 1 ? g.nk - 2 : k);
-    const long ps = g.at(is, js, ks);
-    const float d = dtb / a.rst[ps];   // dtb is the leapfrog's 2*dtb here
-    float ptpf = a.ptpp[ps] + a.ptfrc[ps] * d;
-    float qvf = fmaxr(a.qp[ps] + a.qvfrc[ps] * d, 0.0f);
-    float qcf = fmaxr(a.qp[ps + qn] + a.sfrc[ps] * d, 0.0f);
-    float qrf = fmaxr(a.qp[ps + 2 * qn] + a.sfrc[ps + qn] * d, 0.0f);
-    float qif = fmaxr(a.qp[ps + 3 * qn] + a.sfrc[ps + 2 * qn] * d, 0.0f);
-    float qsf = fmaxr(a.qp[ps + 4 * qn] + a.sfrc[ps + 3 * qn] * d, 0.0f);
-    float qgf = fmaxr(a.qp[ps + 5 * qn] + a.sfrc[ps + 4 * qn] * d, 0.0f);
-    float ncif = a.ncip[ps] + a.sfrc[ps + 5 * qn] * d;
+    float ptpf, qvf, qcf, qrf, qif, qsf, qgf, ncif;
+    if (a.open && (i == 1 || i == g.ni - 1 || j == 1 || j == g.nj - 1)) {
+      ptpf = obc_lf_ring(g, a.lf, false, a.ptpc, a.ptpp, i, j, ks);
+      qvf = obc_lf_ring(g, a.lf, true, a.qc, a.qp, i, j, ks);
+      qcf = obc_lf_ring(g, a.lf, true, a.qc + qn, a.qp + qn, i, j, ks);
+      qrf = obc_lf_ring(g, a.lf, true, a.qc + 2 * qn, a.qp + 2 * qn, i, j, ks);
+      qif = obc_lf_ring(g, a.lf, true, a.qc + 3 * qn, a.qp + 3 * qn, i, j, ks);
+      qsf = obc_lf_ring(g, a.lf, true, a.qc + 4 * qn, a.qp + 4 * qn, i, j, ks);
+      qgf = obc_lf_ring(g, a.lf, true, a.qc + 5 * qn, a.qp + 5 * qn, i, j, ks);
+      ncif = obc_lf_ring(g, a.lf, false, a.ncic, a.ncip, i, j, ks);
+    } else {
+      const int is = a.open ? i : (i == 1 ? g.ni - 2 : (i == g.ni - 1 ? 2 : i));
+      const int js = a.open ? j : (j == 1 ? g.nj - 2 : (j == g.nj - 1 ? 2 : j));
+      const long ps = g.at(is, js, ks);
+      const float d = dtb / a.rst[ps];   // dtb is the leapfrog's 2*dtb here
+      ptpf = a.ptpp[ps] + a.ptfrc[ps] * d;
+      qvf = fmaxr(a.qp[ps] + a.qvfrc[ps] * d, 0.0f);
+      qcf = fmaxr(a.qp[ps + qn] + a.sfrc[ps] * d, 0.0f);
+      qrf = fmaxr(a.qp[ps + 2 * qn] + a.sfrc[ps + qn] * d, 0.0f);
+      qif = fmaxr(a.qp[ps + 3 * qn] + a.sfrc[ps + 2 * qn] * d, 0.0f);
+      qsf = fmaxr(a.qp[ps + 4 * qn] + a.sfrc[ps + 3 * qn] * d, 0.0f);
+      qgf = fmaxr(a.qp[ps + 5 * qn] + a.sfrc[ps + 4 * qn] * d, 0.0f);
+      ncif = a.ncip[ps] + a.sfrc[ps + 5 * qn] * d;
+    }
     // adjstnci before the sources
     ncif = fminr(fmaxr(ncif, (0.5f / mimax) * qif), (1.e2f / mi0) * qif);
     // termblk
