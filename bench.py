@@ -196,6 +196,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "%s %dx%dx%d" % (args.workload, nx, ny, nz), "nsstp": cfg.nsstp,
                    "dtbig": cfg.dtbig, "cphopt": cfg.cphopt, "advopt": cfg.advopt,
+                   "lateral": ("open (option 7, gwave %g m/s)" % cfg.gwave) if cfg.wbc == 7 else "periodic",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,

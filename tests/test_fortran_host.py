@@ -32,9 +32,10 @@ def _files(tmp_path, case, nsteps):
     return conf, st, out
 
 
-def test_fortran_host_resolves_options(tmp_path):
+@pytest.mark.parametrize("name", ["vortex", "supercell"])
+def test_fortran_host_resolves_options(tmp_path, name):
     _need_exe()
-    case = cases.build("vortex", 16, 14, 20, cphopt=2)
+    case = cases.build(name, 16, 14, 20, cphopt=2)
     conf, st, out = _files(tmp_path, case, 4)
     r = subprocess.run([EXE, conf, st, out, "--check"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -99,9 +100,10 @@ def _read_out(path):
 
 
 @pytest.mark.gpu
-def test_fortran_host_matches_python_host(tmp_path):
+@pytest.mark.parametrize("wbc", [-1, 7])
+def test_fortran_host_matches_python_host(tmp_path, wbc):
     _need_exe()
-    case = cases.build("bubble", 12, 10, 24, cphopt=2)
+    case = cases.build("bubble", 12, 10, 24, cphopt=2, wbc=wbc, gwave=30.0)
     conf, st, out = _files(tmp_path, case, 3)
     r = subprocess.run([EXE, conf, st, out], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
