@@ -191,7 +191,7 @@ class Case:
             "&gpvpram vspopt=%d, vspvar='%s' /" % (c.vspopt, vsp[:10]),
             "&integrat dtbig=%r, dtsml=%r, gwmopt=%d, impopt=%d, advopt=%d, weicoe=%r, filcoe=%r /"
             % (c.dtbig, c.dtsml, c.gwmopt, c.impopt, c.advopt, c.weicoe, c.filcoe),
-            "&smoother smtopt=0 /",
+            "&smoother smtopt=%d, smhcoe=%r, smvcoe=%r /" % (c.smtopt, c.smhcoe, c.smvcoe),
             "&mapfcter mfcopt=0 /",
             "&coriolis coropt=%d /" % c.coropt,
             "&buoyancy buyopt=%d /" % c.buyopt,
@@ -263,12 +263,14 @@ def _hydrometeors(X, Y, xc, yc, zs, tk, p, qv, ip, jp, seed, ring, cellnoise=Tru
 
 def build(name="bubble", nx=None, ny=None, nz=None, seed=None, cphopt=2, dtbig=None,
           dtsml=None, nsstp=None, advopt=3, moist=None, noise=None, divopt=1, wbc=None,
-          gwave=None):
+          gwave=None, smtopt=0, smhcoe=0.0008333, smvcoe=0.0008333):
     """Return a Case with config, base state and the initial state.
 
     wbc: -1 periodic or 7 open (radiation at the constant phase speed gwave,
     m/s) on all four sides; the supercell defaults to open sides with
     gwave=30, the other cases to periodic.
+    smtopt: 0 none, 2/3 the 4th-order smoothing with smhcoe, smvcoe (1/s;
+    defaults are the reference's documented values).
 
     moist (default: cphopt == 2) seeds the hydrometeor species and a
     supersaturated core so that the microphysics is active from step one.
@@ -457,5 +459,7 @@ def build(name="bubble", nx=None, ny=None, nz=None, seed=None, cphopt=2, dtbig=N
                            divopt=divopt, cphopt=cphopt, vspopt=2, coropt=coropt, sthopt=sthopt,
                            ksp0=(ksp0, ksp0), vspvar=b"oooxoooooo",
                            wbc=(7 if open_default else -1) if wbc is None else wbc,
-                           gwave=(gwave_default if gwave is None else gwave))
+                           gwave=(gwave_default if gwave is None else gwave),
+                           smtopt=smtopt, smhcoe=smhcoe if smtopt else 0.0,
+                           smvcoe=smvcoe if smtopt else 0.0)
     return Case(cfg, base, fc, rbct, init, zw, zs, name)

@@ -671,8 +671,8 @@ static int large_step(cress_ctx *x, int first) {
     ac.vf = x->v[F];
     LAUNCH(x, KT_AC_COL, k_ac_col, flat(tcol), dim3(64, CRESS_COL_BY), CRESS_COL_LDS, tcol, g, o, c, ac);
     if (!open) run_fill(x, fwp);
-    LAUNCH(x, KT_AC_RING, k_ac_ring, dim3((nring + 63) / 64), dim3(64), 0, g, o, B[CRESS_JCB8W], x->w[F], x->wc,
-           x->pp[F]);
+    LAUNCH(x, KT_AC_RING, k_ac_ring, dim3((nring + 63) / 64, nk), dim3(64), 0, g, o, B[CRESS_JCB8W], x->w[F],
+           x->wc, x->pp[F]);
   }
   stamp(x, 2);
 

@@ -637,48 +637,54 @@ __global__ void __launch_bounds__(128) k_ac_col(Tiles T, Grid g, Opts o, Consts 
 // Ring columns (i or j on the first halo row, within 1..ni-1 x 1..nj-1)
 // after the lateral fill of wf and ppf: phy2cnt+vbcwc, vbcw and vbcp exactly
 // as the reference applies them over i=1..ni-1, j=1..nj-1 (stepwi.f90:540-549,
-// steppi.f90:316).
+// steppi.f90:316).  One thread per (ring column, level): every value a level
+// needs is recomputed from wf/ppf levels that no thread of the launch writes
+// (wc at k from w(k); the bottom and top rows from levels 2, 3, nk-2, nk-1).
 __global__ void k_ac_ring(Grid g, Opts o, const float *jcb8w, float *wf, float *wc, float *ppf) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = 1 + blockIdx.y;
   const int nxr = g.ni - 1, nyr = g.nj - 1;   // ring of the (1..ni-1)x(1..nj-1) box
   const int nring = 2 * nxr + 2 * (nyr - 2);
-  if (t >= nring) return;
+  if (t >= nring || k > g.nk) return;
   int i, j;
   if (t < nxr) { i = 1 + t; j = 1; }
   else if (t < 2 * nxr) { i = 1 + (t - nxr); j = g.nj - 1; }
   else if (t < 2 * nxr + (nyr - 2)) { i = 1; j = 2 + (t - 2 * nxr); }
   else { i = g.ni - 1; j = 2 + (t - 2 * nxr - (nyr - 2)); }
   const int nk = g.nk;
-  for (int k = 2; k <= nk - 1; k++) AT(wc, i, j, k) = wc_raw(g, o.sthopt, wf, jcb8w, i, j, k);
-  if (o.bbc == 2) {
-    AT(wc, i, j, 1) = -AT(wc, i, j, 3);
+  auto raw = [&](int kk) { return wc_raw(g, o.sthopt, wf, jcb8w, i, j, kk); };
+  const float z = 0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
+  if (k == 1) {
+    if (o.bbc == 2) {
+      const float w3 = raw(3);
+      AT(wc, i, j, 1) = -w3;
+      AT(wf, i, j, 1) = -AT(jcb8w, i, j, 3) * w3 - z;
+      AT(ppf, i, j, 1) = 2.0f * AT(ppf, i, j, 2) - AT(ppf, i, j, 3);
+    } else {
+      AT(wc, i, j, 1) = raw(2);
+      AT(wf, i, j, 1) = AT(wf, i, j, 2);
+      if (o.bbc == 3) AT(ppf, i, j, 1) = AT(ppf, i, j, 2);
+    }
+  } else if (k == nk) {
+    if (o.tbc == 2) {
+      const float wm = raw(nk - 2);
+      AT(wc, i, j, nk) = -wm;
+      AT(wf, i, j, nk) = -AT(jcb8w, i, j, nk - 2) * wm - z;
+    } else {
+      AT(wc, i, j, nk) = raw(nk - 1);
+      AT(wf, i, j, nk) = AT(wf, i, j, nk - 1);
+    }
+  } else if (k == 2 && o.bbc == 2) {
     AT(wc, i, j, 2) = 0.0f;
-  } else if (o.bbc == 3) {
-    AT(wc, i, j, 1) = AT(wc, i, j, 2);
-  }
-  if (o.tbc == 2) {
-    AT(wc, i, j, nk) = -AT(wc, i, j, nk - 2);
-    AT(wc, i, j, nk - 1) = 0.0f;
-  } else {
-    AT(wc, i, j, nk) = AT(wc, i, j, nk - 1);
-  }
-  if (o.bbc == 2) {
-    const float z = 0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
-    AT(wf, i, j, 1) = -AT(jcb8w, i, j, 3) * AT(wc, i, j, 3) - z;
     AT(wf, i, j, 2) = -0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
-  } else {
-    AT(wf, i, j, 1) = AT(wf, i, j, 2);
-  }
-  if (o.tbc == 2) {
-    const float z = 0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
-    AT(wf, i, j, nk) = -AT(jcb8w, i, j, nk - 2) * AT(wc, i, j, nk - 2) - z;
+  } else if (k == nk - 1 && o.tbc == 2) {
+    AT(wc, i, j, nk - 1) = 0.0f;
     AT(wf, i, j, nk - 1) = 0.0f;
+    AT(ppf, i, j, nk - 1) = AT(ppf, i, j, nk - 2);
   } else {
-    AT(wf, i, j, nk) = AT(wf, i, j, nk - 1);
+    AT(wc, i, j, k) = raw(k);
+    if (k == nk - 1) AT(ppf, i, j, nk - 1) = AT(ppf, i, j, nk - 2);
   }
-  if (o.bbc == 2) AT(ppf, i, j, 1) = 2.0f * AT(ppf, i, j, 2) - AT(ppf, i, j, 3);
-  else if (o.bbc == 3) AT(ppf, i, j, 1) = AT(ppf, i, j, 2);
-  AT(ppf, i, j, nk - 1) = AT(ppf, i, j, nk - 2);
 }
 
 // ---------------------------------------------------------------------------

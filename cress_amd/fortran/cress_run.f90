@@ -136,8 +136,11 @@ program cress_run
     call fatal('one GPU runs the whole domain: numpe, xsub, ysub, '//   &
      &         'xgroup, ygroup must be 1')
   end if
-  if(trnopt /= 0 .or. mfcopt /= 0 .or. smtopt /= 0 .or. tubopt /= 0) then
-    call fatal('trnopt, mfcopt, smtopt and tubopt must be 0')
+  if(trnopt /= 0 .or. mfcopt /= 0 .or. tubopt /= 0) then
+    call fatal('trnopt, mfcopt and tubopt must be 0')
+  end if
+  if(smtopt /= 0 .and. smtopt /= 2 .and. smtopt /= 3) then
+    call fatal('smtopt must be 0, 2 or 3')
   end if
 
 ! open sides: the constant-phase-speed radiation (option 7) without the
@@ -200,6 +203,9 @@ program cress_run
   cfg%filcoe = filcoe
   cfg%thresq = thresq
   cfg%gwave = gwave
+  cfg%smtopt = smtopt
+  cfg%smhcoe = smhcoe
+  cfg%smvcoe = smvcoe
 
   if(trim(arg4) == '--check') then
     write(*,'(a,3i8)') 'dims', cfg%ni, cfg%nj, cfg%nk
@@ -210,8 +216,10 @@ program cress_run
      &  cfg%coropt, cfg%sthopt
     write(*,'(a,3i6)') 'ksp0_nsstp', cfg%ksp0(1), cfg%ksp0(2), cfg%nsstp
     write(*,'(2a)') 'vspvar ', vspvar(1:10)
-    write(*,'(a,9es24.16)') 'reals', cfg%dx, cfg%dy, cfg%dz, cfg%dtbig, &
-     &  cfg%dtsml, cfg%weicoe, cfg%filcoe, cfg%thresq, cfg%gwave
+    write(*,'(a,11es24.16)') 'reals', cfg%dx, cfg%dy, cfg%dz, cfg%dtbig,&
+     &  cfg%dtsml, cfg%weicoe, cfg%filcoe, cfg%thresq, cfg%gwave,         &
+     &  cfg%smhcoe, cfg%smvcoe
+    write(*,'(a,i4)') 'smtopt', cfg%smtopt
     write(*,'(a,i8)') 'nsteps', nint((etime - stime) / dtbig)
     stop
   end if

@@ -49,6 +49,8 @@ module m_cress_hip
     real(c_double) :: dtbig, dtsml
     real(c_double) :: weicoe, filcoe, thresq
     real(c_double) :: gwave
+    integer(c_int) :: smtopt
+    real(c_double) :: smhcoe, smvcoe
   end type cress_config
 
   public :: cress_error_string, cress_device_count

@@ -46,7 +46,8 @@ class Config(C.Structure):
                 ("cphopt", I), ("haiopt", I), ("vspopt", I), ("coropt", I), ("sthopt", I),
                 ("ksp0", I * 2), ("vspvar", C.c_char * 10), ("nsstp", I),
                 ("dx", D), ("dy", D), ("dz", D), ("dtbig", D), ("dtsml", D),
-                ("weicoe", D), ("filcoe", D), ("thresq", D), ("gwave", D)]
+                ("weicoe", D), ("filcoe", D), ("thresq", D), ("gwave", D),
+                ("smtopt", I), ("smhcoe", D), ("smvcoe", D)]
 
 
 _lib = None
@@ -87,7 +88,7 @@ def check(rc, what):
 def make_config(ni, nj, nk, dx, dy, dz, dtbig, dtsml, nsstp=None, advopt=3, divopt=1, buyopt=1,
                 cphopt=2, vspopt=2, coropt=0, sthopt=1, bbc=2, tbc=2, ksp0=(2, 2),
                 vspvar=b"oooxoooooo", weicoe=0.6, filcoe=0.1, thresq=1e-12, wbc=-1,
-                gwave=0.0):
+                gwave=0.0, smtopt=0, smhcoe=0.0, smvcoe=0.0):
     """wbc: -1 periodic or 7 open (radiation at the constant phase speed
     gwave, m/s) on all four sides."""
     c = Config()
@@ -105,6 +106,7 @@ def make_config(ni, nj, nk, dx, dy, dz, dtbig, dtsml, nsstp=None, advopt=3, divo
     c.dx, c.dy, c.dz, c.dtbig, c.dtsml = dx, dy, dz, dtbig, dtsml
     c.weicoe, c.filcoe, c.thresq = weicoe, filcoe, thresq
     c.gwave = gwave
+    c.smtopt, c.smhcoe, c.smvcoe = smtopt, smhcoe, smvcoe
     return c
 
 

@@ -70,6 +70,9 @@ typedef struct cress_config {
   double gwave;             /* open sides (wbc..nbc = 7): the constant phase
                                speed of the radiative fills, m/s
                                (&boundry gwave, phvuvw.f90:334-338) */
+  int smtopt;               /* 0 none; 2, 3: 4th-order smoothing of the past
+                               level, horizontal / also vertical (&smoother) */
+  double smhcoe, smvcoe;    /* smoothing coefficients, 1/s */
 } cress_config;
 
 /* Field ids for upload/download.  Base state and metrics have one level. */

@@ -23,7 +23,8 @@ def build_oracle():
     os.makedirs(OUT, exist_ok=True)
     src = os.path.join(HERE, "cress_oracle.c")
     deps = [src, os.path.join(HERE, "cress_oracle.h"), os.path.join(HERE, "cress_micro.c"),
-            os.path.join(HERE, "cress_obc.c")]
+            os.path.join(HERE, "cress_obc.c"),
+            os.path.join(HERE, "cress_smooth.c")]
     libs = []
     for prec, defs in ((32, ["-DOR_FP32", "-DREAL=float"]), (64, ["-DREAL=double"])):
         lib = os.path.join(OUT, "liboracle%d.so" % prec)

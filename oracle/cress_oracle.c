@@ -205,6 +205,7 @@ void or_vbcw(const or_cfg *c, const R *jcb8w, const R *wc, R *wf)
 }
 
 #include "cress_obc.c"
+#include "cress_smooth.c"
 
 /* s_phy2cnt, phy2cnt.f90:193-227 (trnopt=0) then vbcwc (:411).  The
  * advopt>=4 halo branch (:350-405) is not taken for advopt<=3. */
@@ -1228,6 +1229,12 @@ static void or_forcedrv(const or_cfg *c, or_state *s, R *qall, R *ufrc, R *vfrc,
   memset(vfrc, 0, n * sizeof(R));
   memset(wfrc, 0, n * sizeof(R));
   or_phy2cnt(c, c->sthopt, s->jcb8w, s->w[C], wc);
+  const int sm = c->smtopt % 10 == 2 || c->smtopt % 10 == 3;
+  /* smoothing of the past level (forceuvw.f90:467-476, forcep.f90:273-280,
+   * forcept.f90:362-370, forceqv.f90:308-316, forceq.f90:413-420 ...) */
+  if (sm)
+    or_smoo4uvw(c, s->jcb8u, s->jcb8v, s->jcb8w, s->ubr, s->vbr, s->rst8u, s->rst8v, s->rst8w,
+                s->u[P], s->v[P], s->w[P], ufrc, vfrc, wfrc, t[0], t[1], t[2], t[3]);
   if (c->vspopt >= 1)
     or_vspuvw(c, s->ubr, s->vbr, s->rst8u, s->rst8v, s->rst8w, s->u[P], s->v[P], s->w[P],
               s->rbct, ufrc, vfrc, wfrc, t[0]);
@@ -1240,23 +1247,28 @@ static void or_forcedrv(const or_cfg *c, or_state *s, R *qall, R *ufrc, R *vfrc,
   /* forcep */
   or_advp(c, s->jcb8u, s->jcb8v, s->jcb8w, s->u[C], s->v[C], wc, s->pp[C], pfrc,
           t[0], t[1], t[2], t[3], t[4], t[5], t[6]);
+  if (sm) or_smoo4s(c, 2, NULL, NULL, s->pp[P], pfrc, t[0], t[1], t[2], t[3]);
   if (c->vspopt >= 1 && c->vspvar[3] == 'o') or_vsps(c, s->jcb, s->pp[P], NULL, s->rbct, pfrc);
   /* forcept */
   or_advs(c, rstxu, rstxv, rstxwc, s->ptp[C], ptfrc, t[0], t[1], t[2], t[3]);
+  if (sm) or_smoo4s(c, 0, s->rbr, NULL, s->ptp[P], ptfrc, t[0], t[1], t[2], t[3]);
   if (c->vspopt >= 1 && c->vspvar[4] == 'o') or_vsps(c, s->rst, s->ptp[P], NULL, s->rbct, ptfrc);
   or_advbspt(c, s->ptbr, s->rbr, s->w[C], ptfrc, t[0]);
   /* forceqv */
   or_advs(c, rstxu, rstxv, rstxwc, s->q[C], qvfrc, t[0], t[1], t[2], t[3]);
+  if (sm) or_smoo4s(c, 1, s->rbr, s->qvbr, s->q[P], qvfrc, t[0], t[1], t[2], t[3]);
   if (c->vspopt >= 1 && c->vspvar[5] == 'o')
     or_vsps(c, s->rst, s->q[P], s->qvbr, s->rbct, qvfrc);
   /* forceq (abs(cphopt)>=1, forceq.f90:384-388): qc, qr, qi, qs, qg, nci */
   if (c->cphopt == 0) return;
   for (int sp = QC; sp <= QG; sp++) {
     or_advs(c, rstxu, rstxv, rstxwc, s->q[C] + sp * n, s->q[F] + sp * n, t[0], t[1], t[2], t[3]);
+    if (sm) or_smoo4s(c, 0, s->rbr, NULL, s->q[P] + sp * n, s->q[F] + sp * n, t[0], t[1], t[2], t[3]);
     if (c->vspopt >= 1 && c->vspvar[6] == 'o')
       or_vsps(c, s->rst, s->q[P] + sp * n, NULL, s->rbct, s->q[F] + sp * n);
   }
   or_advs(c, rstxu, rstxv, rstxwc, s->nci[C], s->nci[F], t[0], t[1], t[2], t[3]);
+  if (sm) or_smoo4s(c, 0, s->rbr, NULL, s->nci[P], s->nci[F], t[0], t[1], t[2], t[3]);
   if (c->vspopt >= 1 && c->vspvar[6] == 'o') or_vsps(c, s->rst, s->nci[P], NULL, s->rbct, s->nci[F]);
 }
 
@@ -1441,6 +1453,7 @@ int or_cfg_check(const or_cfg *c)
   if (c->cphopt != 0 && c->cphopt != 2) return -6;
   if (c->haiopt != 0) return -6;
   if (c->vspopt != 0 && c->vspopt != 2) return -7;
+  if (c->smtopt != 0 && c->smtopt != 2 && c->smtopt != 3) return -9;
   return 0;
 }
 

@@ -31,6 +31,8 @@ typedef struct {
   char vspvar[10];                    /* 'o'/'x' per u,v,w,p,pt,qv,q,... */
   double dx, dy, dz, weicoe, filcoe, thresq;
   double gwave;                       /* open-boundary phase speed (m/s) */
+  int smtopt;                         /* 0 none, 2/3 4th-order smoothing */
+  double smhcoe, smvcoe;              /* smoothing coefficients (1/s) */
 } or_cfg;
 
 #endif

@@ -28,12 +28,14 @@ class Cfg(C.Structure):
                 ("ksp0", I * 2), ("vspvar", C.c_char * 10),
                 ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
                 ("weicoe", C.c_double), ("filcoe", C.c_double), ("thresq", C.c_double),
-                ("gwave", C.c_double)]
+                ("gwave", C.c_double), ("smtopt", I), ("smhcoe", C.c_double),
+                ("smvcoe", C.c_double)]
 
 
 def make_cfg(ni, nj, nk, dx=1000.0, dy=1000.0, dz=500.0, advopt=3, divopt=1, buyopt=1,
              cphopt=2, vspopt=2, coropt=0, sthopt=1, bbc=2, tbc=2, ksp0=(2, 2),
-             vspvar=b"oooxoooooo", weicoe=0.6, filcoe=0.1, thresq=1e-12, wbc=-1, gwave=0.0):
+             vspvar=b"oooxoooooo", weicoe=0.6, filcoe=0.1, thresq=1e-12, wbc=-1, gwave=0.0,
+             smtopt=0, smhcoe=0.0, smvcoe=0.0):
     """wbc: -1 periodic or 7 open (radiation with the constant phase speed
     gwave) on all four sides."""
     c = Cfg()
@@ -49,6 +51,7 @@ def make_cfg(ni, nj, nk, dx=1000.0, dy=1000.0, dz=500.0, advopt=3, divopt=1, buy
     c.vspvar = vspvar
     c.dx, c.dy, c.dz = dx, dy, dz
     c.weicoe, c.filcoe, c.thresq = weicoe, filcoe, thresq
+    c.smtopt, c.smhcoe, c.smvcoe = smtopt, smhcoe, smvcoe
     return c
 
 
@@ -141,7 +144,8 @@ def cfg_from_solver_config(sc):
                  divopt=sc.divopt, buyopt=sc.buyopt, cphopt=sc.cphopt, vspopt=sc.vspopt,
                  coropt=sc.coropt, sthopt=sc.sthopt, bbc=sc.bbc, tbc=sc.tbc,
                  ksp0=(sc.ksp0[0], sc.ksp0[1]), vspvar=bytes(sc.vspvar), weicoe=sc.weicoe,
-                 filcoe=sc.filcoe, thresq=sc.thresq, wbc=sc.wbc, gwave=sc.gwave)
+                 filcoe=sc.filcoe, thresq=sc.thresq, wbc=sc.wbc, gwave=sc.gwave,
+                 smtopt=sc.smtopt, smhcoe=sc.smhcoe, smvcoe=sc.smvcoe)
     return c
 
 

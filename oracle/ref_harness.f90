@@ -70,6 +70,10 @@
       use m_lbcw
       use m_lbcs
       use m_lbcwc
+      use m_smoo4s
+      use m_smoo4qv
+      use m_smoo4p
+      use m_smoo4uvw
 
       implicit none
 
@@ -632,5 +636,36 @@
         call s_lbcwc(idwbc,idebc,idsbc,idnbc,ni,nj,nk,var)
       end select
       end subroutine ref_lbc
+
+! 4th-order smoothing (smtopt 2/3) with the coefficients set here.
+! kind: 0 smoo4s, 1 smoo4qv, 2 smoo4p, 3 smoo4uvw (u,v,w in a..c, their
+! forcings in f1..f3, base/metrics in m1..m8)
+      subroutine ref_smoo4(kind,smtopt,smh,smv,ni,nj,nk,a,b,c,f1,f2,f3,   &
+     &   m1,m2,m3,m4,m5,m6,m7,m8,t1,t2,t3,t4,t5) bind(C,name='ref_smoo4')
+      integer(c_int), value :: kind,smtopt,ni,nj,nk
+      real, value :: smh,smv
+      real a(*),b(*),c(*),f1(*),f2(*),f3(*)
+      real m1(*),m2(*),m3(*),m4(*),m5(*),m6(*),m7(*),m8(*)
+      real t1(*),t2(*),t3(*),t4(*),t5(*)
+      iname(idsmtopt)=smtopt
+      rname(idsmhcoe)=smh
+      rname(idsmvcoe)=smv
+      select case(kind)
+      case(0)
+        call s_smoo4s(idsmtopt,idiwest,idieast,idjsouth,idjnorth,        &
+     &                idsmhcoe,idsmvcoe,ni,nj,nk,m1,a,f1,t4,t5,t1,t2,t3)
+      case(1)
+        call s_smoo4qv(idsmtopt,idiwest,idieast,idjsouth,idjnorth,       &
+     &                 idsmhcoe,idsmvcoe,ni,nj,nk,m2,m1,a,f1,t4,t5,      &
+     &                 t1,t2,t3)
+      case(2)
+        call s_smoo4p(idsmtopt,idiwest,idieast,idjsouth,idjnorth,        &
+     &                idsmhcoe,idsmvcoe,ni,nj,nk,a,f1,t5,t1,t2,t3)
+      case default
+        call s_smoo4uvw(idsmtopt,idiwest,idieast,idjsouth,idjnorth,      &
+     &                  idsmhcoe,idsmvcoe,ni,nj,nk,m1,m2,m3,m4,m5,m6,m7,  &
+     &                  m8,a,b,c,f1,f2,f3,t1,t2,t3,t4,t5)
+      end select
+      end subroutine ref_smoo4
 
       end module cress_ref_harness

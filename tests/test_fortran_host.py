@@ -52,10 +52,12 @@ def test_fortran_host_resolves_options(tmp_path, name):
     assert [int(x) for x in kv["ksp0_nsstp"]] == [c.ksp0[0], c.ksp0[1], c.nsstp]
     assert kv["vspvar"][0] == bytes(c.vspvar).decode()
     reals = [float(x) for x in kv["reals"]]
-    want = [c.dx, c.dy, c.dz, c.dtbig, c.dtsml, c.weicoe, c.filcoe, c.thresq, c.gwave]
+    want = [c.dx, c.dy, c.dz, c.dtbig, c.dtsml, c.weicoe, c.filcoe, c.thresq, c.gwave, c.smhcoe,
+            c.smvcoe]
     # user.conf values pass through default REAL, as in the reference
     np.testing.assert_array_equal(np.float32(reals), np.float32(want))
     assert int(kv["nsteps"][0]) == 4
+    assert int(kv["smtopt"][0]) == c.smtopt
 
 
 def test_fortran_host_rejects_multirank_and_bad_state(tmp_path):

@@ -28,9 +28,10 @@ def run(case, prec, nsteps):
     return m
 
 
-def test_mirror_symmetry_x_and_y():
+@pytest.mark.parametrize("smtopt", [0, 3])
+def test_mirror_symmetry_x_and_y(smtopt):
     nx, ny = 12, 10
-    case = cases.build("bubble", nx, ny, 16, cphopt=2, noise=0.0)
+    case = cases.build("bubble", nx, ny, 16, cphopt=2, noise=0.0, smtopt=smtopt)
     m = run(case, 64, 6)
     ix = np.arange(2, nx + 2)          # interior i (Fortran index = array index)
     jy = np.arange(2, ny + 2)
