@@ -134,6 +134,9 @@ static int check_config(const cress_config *c) {
   if (c->divopt < 0 || c->divopt > 1 || c->buyopt < 0 || c->buyopt > 1) return CRESS_EUNSUPPORTED;
   if ((c->cphopt != 0 && c->cphopt != 2) || c->haiopt != 0) return CRESS_EUNSUPPORTED;
   if (c->vspopt != 0 && c->vspopt != 2) return CRESS_EUNSUPPORTED;
+  if (c->smtopt != 0 && c->smtopt != 2 && c->smtopt != 3) return CRESS_EUNSUPPORTED;
+  if (c->smtopt && !(c->smhcoe >= 0.0 && c->smvcoe >= 0.0 && c->smhcoe < 1.0e6 && c->smvcoe < 1.0e6))
+    return CRESS_EINVAL;
   return CRESS_OK;
 }
 
@@ -165,6 +168,8 @@ static void fold_consts(const cress_config *cf, Consts *c) {
   c->m1fc2 = 1.0f - 2.0f * c->filcoe;
   c->dz = dz;
   c->thresq = (float)cf->thresq;
+  c->smh = (float)cf->smhcoe;
+  c->smv = (float)cf->smvcoe;
 }
 
 int cress_ctx_create(const cress_config *cfg, cress_ctx **out) {
@@ -201,6 +206,7 @@ int cress_ctx_create(const cress_config *cfg, cress_ctx **out) {
   o.vsp_qv = vs && cfg->vspvar[5] == 'o';
   o.vsp_q = vs && cfg->vspvar[6] == 'o';
   o.ksp0 = cfg->ksp0[1];
+  o.smtopt = cfg->smtopt;
   if (hipStreamCreate(&x->stream) != hipSuccess) {
     delete x;
     return CRESS_EHIP;
@@ -539,13 +545,28 @@ static int large_step(cress_ctx *x, int first) {
   fi.rstxwc = x->rstxwc;
   fi.fc = x->fc;
   fi.rbct = x->rbct;
+  fi.jcb8u = B[CRESS_JCB8U];
+  fi.jcb8v = B[CRESS_JCB8V];
+  fi.jcb8w = B[CRESS_JCB8W];
   fi.qn = n;
-  LAUNCH(x, KT_FORCE_UV, k_force_uv, flat(tiles(ni - 2, nj - 2, 64, 4, nk - 3, 2)), b2, 0, tiles(ni - 2, nj - 2, 64, 4, nk - 3, 2), g, o, c, fi, x->ufrc,
-                     x->vfrc);
-  LAUNCH(x, KT_FORCE_W, k_force_w, flat(tiles(ni - 3, nj - 3, 64, 4, nk - 2, 2)), b2, 0, tiles(ni - 3, nj - 3, 64, 4, nk - 2, 2), g, o, c, fi, x->wfrc);
-  LAUNCH(x, KT_FORCE_P, k_force_p, flat(tiles(ni - 3, nj - 3, 64, 4, nk - 3, 2)), b2, 0, tiles(ni - 3, nj - 3, 64, 4, nk - 3, 2), g, o, c, B[CRESS_JCB],
-                     B[CRESS_JCB8U], B[CRESS_JCB8V], B[CRESS_JCB8W], x->u[C], x->v[C], x->wc, x->pp[C],
-                     x->pp[P], x->rbct, x->pfrc);
+  // the forcing kernels come in two builds: with the 4th-order smoothing
+  // (smtopt 2/3) and without, so that the plain path carries none of its
+  // registers
+  const bool sm = o.smtopt != 0;
+  const Tiles tuv = tiles(ni - 2, nj - 2, 64, 4, nk - 3, 2);
+  const Tiles tw = tiles(ni - 3, nj - 3, 64, 4, nk - 2, 2);
+  const Tiles tp = tiles(ni - 3, nj - 3, 64, 4, nk - 3, 2);
+  if (sm) {
+    LAUNCH(x, KT_FORCE_UV, k_force_uv<true>, flat(tuv), b2, 0, tuv, g, o, c, fi, x->ufrc, x->vfrc);
+    LAUNCH(x, KT_FORCE_W, k_force_w<true>, flat(tw), b2, 0, tw, g, o, c, fi, x->wfrc);
+    LAUNCH(x, KT_FORCE_P, k_force_p<true>, flat(tp), b2, 0, tp, g, o, c, B[CRESS_JCB], B[CRESS_JCB8U],
+           B[CRESS_JCB8V], B[CRESS_JCB8W], x->u[C], x->v[C], x->wc, x->pp[C], x->pp[P], x->rbct, x->pfrc);
+  } else {
+    LAUNCH(x, KT_FORCE_UV, k_force_uv<false>, flat(tuv), b2, 0, tuv, g, o, c, fi, x->ufrc, x->vfrc);
+    LAUNCH(x, KT_FORCE_W, k_force_w<false>, flat(tw), b2, 0, tw, g, o, c, fi, x->wfrc);
+    LAUNCH(x, KT_FORCE_P, k_force_p<false>, flat(tp), b2, 0, tp, g, o, c, B[CRESS_JCB], B[CRESS_JCB8U],
+           B[CRESS_JCB8V], B[CRESS_JCB8W], x->u[C], x->v[C], x->wc, x->pp[C], x->pp[P], x->rbct, x->pfrc);
+  }
   ScalarIn si;
   const int nsc = cf.cphopt == 2 ? 8 : 2;
   {
@@ -573,7 +594,10 @@ static int large_step(cress_ctx *x, int first) {
   si.ptbr = B[CRESS_PTBR];
   si.rbr = B[CRESS_RBR];
   si.w = x->w[C];
-  LAUNCH(x, KT_FORCE_S, k_force_s, flat(tiles(ni - 3, nj - 3, 64, 4, nk - 3, 2)), b2, 0, tiles(ni - 3, nj - 3, 64, 4, nk - 3, 2), g, o, c, si, nsc);
+  if (sm)
+    LAUNCH(x, KT_FORCE_S, k_force_s<true>, flat(tp), b2, 0, tp, g, o, c, si, nsc);
+  else
+    LAUNCH(x, KT_FORCE_S, k_force_s<false>, flat(tp), b2, 0, tp, g, o, c, si, nsc);
   stamp(x, 1);
 
   // ---- acoustic substeps (smalldrv / hevi) on the future level ----

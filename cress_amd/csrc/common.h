@@ -96,6 +96,7 @@ struct Consts {
   float weicoe, weic1m, filcoe, m1fc2;
   float dz;
   float thresq;
+  float smh, smv;                     // smoothing coefficients smhcoe, smvcoe
 };
 
 // Options the kernels branch on, in one POD so launches stay small.
@@ -104,6 +105,7 @@ struct Opts {
   int iw, ie, js, jn;  // 4th-order reduction flags (0 on periodic sides)
   int vsp_u, vsp_v, vsp_w, vsp_p, vsp_pt, vsp_qv, vsp_q;
   int ksp0;  // ksp0(2): the vspopt=2 sponge's lowest level
+  int smtopt;  // 0, or 2/3: 4th-order smoothing horizontal / also vertical
 };
 
 }  // namespace cress

@@ -7,6 +7,7 @@ namespace cress {
 struct ForceIn {
   const float *u, *v, *w, *up, *vp, *wp, *ptp, *q, *qvbr, *ptbr, *ubr, *vbr;
   const float *rst, *rst8u, *rst8v, *rst8w, *rstxu, *rstxv, *rstxwc, *fc, *rbct;
+  const float *jcb8u, *jcb8v, *jcb8w;  // for the smoothing of u, v, w
   long qn;  // species stride in q (qv,qc,qr,qi,qs,qg)
 };
 
@@ -74,12 +75,17 @@ __global__ void k_phy2cnt(Tiles T, Grid g, Opts o, const float *w, const float *
 __global__ void k_prep(Tiles T, Grid g, Opts o, const float *u, const float *v, const float *w,
                        const float *rst8u, const float *rst8v, const float *rst8w,
                        const float *jcb8w, float *wc, float *rstxu, float *rstxv, float *rstxwc);
+// SM: with the 4th-order smoothing (smtopt 2/3) compiled in
+template <bool SM>
 __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *ufrc, float *vfrc);
+template <bool SM>
 __global__ void k_force_w(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *wfrc);
+template <bool SM>
 __global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, const float *jcb8u,
                           const float *jcb8v, const float *jcb8w, const float *u, const float *v,
                           const float *wc, const float *pp, const float *ppp, const float *rbct,
                           float *pfrc);
+template <bool SM>
 __global__ void k_force_s(Tiles T, Grid g, Opts o, Consts c, ScalarIn in, int nscalar);
 __global__ void k_ac_ptuv(Tiles T, Grid g, Opts o, Consts c, AcPtUv a);
 __global__ void k_ac_col(Tiles T, Grid g, Opts o, Consts c, AcCol a);

@@ -27,6 +27,32 @@ __device__ __forceinline__ float rbct8s(const float *rbct, const Grid &g, int i,
   return 0.25f * (rbct2(rbct, g, i, j, k) + rbct2(rbct, g, i, j, k + 1));
 }
 
+// 4th-order numerical smoothing of one output point (smoo4s.f90:187-298 and
+// its u/v/w, qv and p' twins in smoo4uvw.f90, smoo4qv.f90, smoo4p.f90):
+// a(di, dj, dk) is the prepared field at an offset (rbr*s, rst8u*(u-ubr)/
+// jcb8u, ...).  The 2nd-order part f + smhcoe*(t1+t2) + smvcoe*t3 applies
+// everywhere; the 4th-order part, built from the neighbours' second
+// differences, inside the band where the 4th-order advection applies.
+// kb / kt: the point is the lowest / highest level of its range, where the
+// reference continues t3 by copy (smoo4s.f90:270-275).
+template <class A>
+__device__ __forceinline__ float smooth4(float f, const Consts &c, int smtopt, bool band, bool kb, bool kt,
+                                         A a) {
+  const float a0 = a(0, 0, 0);
+  const float t1 = (a(1, 0, 0) + a(-1, 0, 0)) - 2.0f * a0;
+  const float t2 = (a(0, 1, 0) + a(0, -1, 0)) - 2.0f * a0;
+  const float t3 = (a(0, 0, 1) + a(0, 0, -1)) - 2.0f * a0;
+  f = f + c.smh * (t1 + t2) + c.smv * t3;
+  if (!band) return f;
+  auto T1 = [&](int d) { return (a(d + 1, 0, 0) + a(d - 1, 0, 0)) - 2.0f * a(d, 0, 0); };
+  auto T2 = [&](int d) { return (a(0, d + 1, 0) + a(0, d - 1, 0)) - 2.0f * a(0, d, 0); };
+  auto T3 = [&](int d) { return (a(0, 0, d + 1) + a(0, 0, d - 1)) - 2.0f * a(0, 0, d); };
+  const float h = (t1 - (T1(1) + T1(-1))) + (t2 - (T2(1) + T2(-1)));
+  if (smtopt == 2) return f + c.smh * h;
+  const float t3p = kt ? t3 : T3(1), t3m = kb ? t3 : T3(-1);
+  return f + (c.smv * (t3 - (t3p + t3m)) + c.smh * h);
+}
+
 // ---------------------------------------------------------------------------
 // Contravariant w (phy2cnt.f90:193-227 then vbcwc.f90:130-186) and the
 // mass-flux velocities rstx* (rstuvwc.f90:172-210).
@@ -197,6 +223,9 @@ __device__ float adv_w(float f, const Grid &g, const Opts &o, const Consts &c, c
 }
 
 // ufrc over i=2..ni-1, j=2..nj-2 and vfrc over i=2..ni-2, j=2..nj-1; k=2..nk-2.
+// Accumulation order of forceuvw.f90:434-626: 0, smoothing, sponge,
+// Coriolis, advection.
+template <bool SM>
 __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *ufrc, float *vfrc) {
   const uint3 xt = xcd_tile(T);
   const int i = 2 + xt.x * blockDim.x + threadIdx.x;
@@ -206,6 +235,13 @@ __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float 
   const bool sp = k >= o.ksp0 - 1 && k <= g.nk - 2;
   if (j <= g.nj - 2) {
     float f = 0.0f;
+    if (SM) {
+      const bool band = j >= 2 + o.js && j <= g.nj - 2 - o.jn && i >= 2 + o.iw && i <= g.ni - 1 - o.ie;
+      f = smooth4(f, c, o.smtopt, band, k == 2, k == g.nk - 2, [&](int di, int dj, int dk) {
+        const long p = g.at(i + di, j + dj, k + dk);
+        return in.rst8u[p] * (in.up[p] - in.ubr[p]) / in.jcb8u[p];
+      });
+    }
     if (o.vsp_u && sp)
       f = f - AT(in.rst8u, i, j, k) * (rbct8s(in.rbct, g, i - 1, j, k) + rbct8s(in.rbct, g, i, j, k)) *
                   (AT(in.up, i, j, k) - AT(in.ubr, i, j, k));
@@ -219,6 +255,13 @@ __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float 
   }
   if (i <= g.ni - 2) {
     float f = 0.0f;
+    if (SM) {
+      const bool band = j >= 2 + o.js && j <= g.nj - 1 - o.jn && i >= 2 + o.iw && i <= g.ni - 2 - o.ie;
+      f = smooth4(f, c, o.smtopt, band, k == 2, k == g.nk - 2, [&](int di, int dj, int dk) {
+        const long p = g.at(i + di, j + dj, k + dk);
+        return in.rst8v[p] * (in.vp[p] - in.vbr[p]) / in.jcb8v[p];
+      });
+    }
     if (o.vsp_v && sp)
       f = f - AT(in.rst8v, i, j, k) * (rbct8s(in.rbct, g, i, j - 1, k) + rbct8s(in.rbct, g, i, j, k)) *
                   (AT(in.vp, i, j, k) - AT(in.vbr, i, j, k));
@@ -247,6 +290,7 @@ __device__ __forceinline__ float wb8s_at(const Grid &g, const Opts &o, const Con
 }
 
 // wfrc over i=2..ni-2, j=2..nj-2, k=2..nk-1
+template <bool SM>
 __global__ void k_force_w(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *wfrc) {
   const uint3 xt = xcd_tile(T);
   const int i = 2 + xt.x * blockDim.x + threadIdx.x;
@@ -254,6 +298,13 @@ __global__ void k_force_w(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *
   const int k = 2 + xt.z;
   if (i > g.ni - 2 || j > g.nj - 2) return;
   float f = 0.0f;
+  if (SM) {
+    const bool band = j >= 2 + o.js && j <= g.nj - 2 - o.jn && i >= 2 + o.iw && i <= g.ni - 2 - o.ie;
+    f = smooth4(f, c, o.smtopt, band, k == 2, k == g.nk - 1, [&](int di, int dj, int dk) {
+      const long p = g.at(i + di, j + dj, k + dk);
+      return in.rst8w[p] * in.wp[p] / in.jcb8w[p];
+    });
+  }
   if (o.vsp_w && k >= o.ksp0 && k <= g.nk - 1)
     f = f - rbct2(in.rbct, g, i, j, k) * AT(in.rst8w, i, j, k) * AT(in.wp, i, j, k);
   if (o.buyopt == 1 && k >= 3 && k <= g.nk - 2)
@@ -265,6 +316,7 @@ __global__ void k_force_w(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *
 // Pressure forcing: s_forcep (forcep.f90:254-308) = advp (advp.f90:249-785,
 // diaopt=0) then vspp (vspp.f90:180-194).  pfrc over the interior.
 // ---------------------------------------------------------------------------
+template <bool SM>
 __global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, const float *jcb8u,
                           const float *jcb8v, const float *jcb8w, const float *u, const float *v,
                           const float *wc, const float *pp, const float *ppp, const float *rbct,
@@ -307,6 +359,11 @@ __global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, c
       }
       f = f + vadv;
     }
+  }
+  if (SM) {
+    const bool band = j >= 2 + o.js && j <= g.nj - 2 - o.jn && i >= 2 + o.iw && i <= g.ni - 2 - o.ie;
+    f = smooth4(f, c, o.smtopt, band, k == 2, k == g.nk - 2,
+                [&](int di, int dj, int dk) { return AT(ppp, i + di, j + dj, k + dk); });
   }
   if (o.vsp_p && k >= o.ksp0 - 1 && k <= g.nk - 2)
     f = f - 0.5f * (rbct2(rbct, g, i, j, k) + rbct2(rbct, g, i, j, k + 1)) * AT(jcb, i, j, k) * AT(ppp, i, j, k);
@@ -390,6 +447,7 @@ __device__ __forceinline__ float adv_s_pre(const Grid &g, const Opts &o, const C
 // (advs.f90:205-420), the vspopt=2 sponge (vsps.f90:160-198, vspqv) and, for
 // theta', advbspt (advbspt.f90:143-190).  The flux weights are loaded once
 // and shared by the scalars.
+template <bool SM>
 __global__ void __launch_bounds__(256) k_force_s(Tiles T, Grid g, Opts o, Consts c, ScalarIn in, int nscalar) {
   const uint3 xt = xcd_tile(T);
   const int i = 2 + xt.x * blockDim.x + threadIdx.x;
@@ -417,6 +475,13 @@ __global__ void __launch_bounds__(256) k_force_s(Tiles T, Grid g, Opts o, Consts
   }
   for (int sc = 0; sc < nscalar; sc++) {
     float f = adv_s_pre(g, o, c, h4, v4, fu, fv, fw, in.s[sc], q);
+    if (SM) {
+      const float *sp = in.sp[sc];
+      f = smooth4(f, c, o.smtopt, h4, k == 2, k == g.nk - 2, [&](int di, int dj, int dk) {
+        const long p = g.at(i + di, j + dj, k + dk);
+        return sc == 1 ? in.rbr[p] * (sp[p] - in.qvbr[p]) : in.rbr[p] * sp[p];
+      });
+    }
     const int vsp = sc == 0 ? o.vsp_pt : (sc == 1 ? o.vsp_qv : o.vsp_q);
     if (vsp && sponge) {
       const float d = sc == 1 ? LD(in.sp[sc], q) - LD(in.qvbr, q) : LD(in.sp[sc], q);
@@ -828,5 +893,19 @@ __global__ void k_advs_only(Grid g, Opts o, Consts c, ScalarIn in) {
   if (i > g.ni - 2 || j > g.nj - 2) return;
   AT(in.frc[0], i, j, k) = adv_s(g, o, c, in.s[0], in.rstxu, in.rstxv, in.rstxwc, i, j, k);
 }
+
+// The forcing kernels with and without the smoothing compiled in.
+template __global__ void k_force_uv<false>(Tiles, Grid, Opts, Consts, ForceIn, float *, float *);
+template __global__ void k_force_uv<true>(Tiles, Grid, Opts, Consts, ForceIn, float *, float *);
+template __global__ void k_force_w<false>(Tiles, Grid, Opts, Consts, ForceIn, float *);
+template __global__ void k_force_w<true>(Tiles, Grid, Opts, Consts, ForceIn, float *);
+template __global__ void k_force_p<false>(Tiles, Grid, Opts, Consts, const float *, const float *, const float *,
+                                          const float *, const float *, const float *, const float *,
+                                          const float *, const float *, const float *, float *);
+template __global__ void k_force_p<true>(Tiles, Grid, Opts, Consts, const float *, const float *, const float *,
+                                         const float *, const float *, const float *, const float *,
+                                         const float *, const float *, const float *, float *);
+template __global__ void k_force_s<false>(Tiles, Grid, Opts, Consts, ScalarIn, int);
+template __global__ void k_force_s<true>(Tiles, Grid, Opts, Consts, ScalarIn, int);
 
 }  // namespace cress
