@@ -24,35 +24,44 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM = 8.0e12
 
-# Algorithmic bytes per interior cell for one launch of each kernel family:
-# distinct 3-D arrays the launch must read plus those it must write (4 B
-# each), with metric/base arrays counted as 3-D and 2-D arrays and
-# top-sponge-only reads left out.
+# Algorithmic bytes per interior cell for one launch of each kernel family,
+# as (fields, metrics): the distinct 3-D arrays the launch must read plus
+# those it must write (4 B each; 2-D arrays and top-sponge-only reads left
+# out), split into prognostic/work fields and base-state/metric fields.  A
+# metric field uploaded with all columns equal (flat terrain, horizontally
+# uniform base state -- every workload here) is read from a per-level table
+# instead of HBM, so it adds no bytes; otherwise it counts as a field.
 KERNEL_ARRAYS = {
-    # u,v,w,rst8u,rst8v,rst8w,jcb8w -> wc,rstxu,rstxv,rstxwc
-    "k_prep": 11,
-    # u,v,up,vp,ubr,vbr,rst8u,rst8v,rst,rstxu,rstxv,rstxwc -> ufrc,vfrc
-    "k_force_uv": 14,
-    # w,wp,ptp,qv..qg(6),qvbr,ptbr,rst,rst8w,rstxu,rstxv,rstxwc -> wfrc
-    "k_force_w": 17,
-    # jcb,jcb8u,jcb8v,jcb8w,u,v,wc,pp,ppp -> pfrc
-    "k_force_p": 10,
-    # 8 scalars, rstxu,rstxv,rstxwc,ptbr,rbr,w -> 8 forcings
-    "k_force_s": 22,
-    # jcb,rst8u,rst8v,rst8w,uf,vf,wc,ppf,ufrc,vfrc -> ptil,uf,vf
-    "k_ac_ptuv": 13,
-    # uf,vf,ptil,pfrc,wfrc,jcb,jcb8u,jcb8v,jcb8w,rcsq,rst,rbr,rst8w,wf,ppf -> wf,wc,ppf
-    "k_ac_col": 18,
-    # ptpp,ptfrc,qp(6),qvfrc,ncip,rst,qf(5 forcings),ncif -> ptpf,qf(6),ncif
-    "k_steps": 25,
-    # 12 fields x 3 levels, pbr, ptbr -> 12 present, 5 future (satadjst)
-    "k_asselin_sat": 55,
-    # jcb,pbr,ptbr,rbr,rst, past pp,ptp,q(6),nci, forcings ptfrc,qvfrc,5 q,nci
-    #   -> future ptp,q(6),nci (steps fused in), urq,usq,ugq
-    "k_micro_src": 33,
-    # rbr,rst,urq,usq,ugq, past qc,qi, future qc,qr,qi,qs,qg,nci -> future qc,qr,qi,qs,qg,nci
-    "k_micro_fall": 19,
+    # u,v,w -> wc,rstxu,rstxv,rstxwc | rst8u,rst8v,rst8w,jcb8w
+    "k_prep": (7, 4),
+    # u,v,up,vp,rstxu,rstxv,rstxwc -> ufrc,vfrc | ubr,vbr,rst8u,rst8v,rst
+    "k_force_uv": (9, 5),
+    # w,wp,ptp,qv..qg(6),rstxu,rstxv,rstxwc -> wfrc | qvbr,ptbr,rst,rst8w
+    "k_force_w": (13, 4),
+    # u,v,wc,pp,ppp -> pfrc | jcb,jcb8u,jcb8v,jcb8w
+    "k_force_p": (6, 4),
+    # 8 scalars, rstxu,rstxv,rstxwc,w -> 8 forcings | ptbr,rbr
+    "k_force_s": (20, 2),
+    # uf,vf,wc,ppf,ufrc,vfrc -> ptil,uf,vf | jcb,rst8u,rst8v,rst8w
+    "k_ac_ptuv": (9, 4),
+    # uf,vf,ptil,pfrc,wfrc,wf,ppf -> wf,wc,ppf | jcb,jcb8u,jcb8v,jcb8w,rcsq,rst,rbr,rst8w
+    "k_ac_col": (10, 8),
+    # ptpp,ptfrc,qp(6),qvfrc,ncip,qf(5 forcings),ncif -> ptpf,qf(6),ncif | rst
+    "k_steps": (24, 1),
+    # 12 fields x 3 levels -> 12 present, 5 future (satadjst) | pbr,ptbr
+    "k_asselin_sat": (53, 2),
+    # past pp,ptp,q(6),nci, forcings ptfrc,qvfrc,5 q,nci -> future ptp,q(6),nci,
+    #   urq,usq,ugq | jcb,pbr,ptbr,rbr,rst
+    "k_micro_src": (28, 5),
+    # urq,usq,ugq, past qc,qi, future qc,qr,qi,qs,qg,nci -> future qc,qr,qi,qs,qg,nci | rbr,rst
+    "k_micro_fall": (17, 2),
 }
+
+
+def kernel_arrays(name, uniform=True):
+    f, m = KERNEL_ARRAYS[name]
+    return f if uniform else f + m
+
 
 # Calibrated HBM bytes per launch (FETCH_SIZE/WRITE_SIZE passes of
 # rocprofv3), one file per workload; see profiles/.
@@ -75,14 +84,16 @@ def measured_traffic(workload, kernel):
     return None
 
 
-def step_bytes_per_cell(nsstp, cphopt):
-    """Whole-step algorithmic bytes per interior cell: B = 560 + 88*nsstp for
-    the full cold-rain path (forcing 136, acoustic 88/substep, steps 100,
-    microphysics 124, Asselin+satadjst 200)."""
+def step_bytes_per_cell(nsstp, cphopt, uniform=True):
+    """Whole-step algorithmic bytes per interior cell: the sum over one
+    step's launches of their algorithmic bytes (KERNEL_ARRAYS)."""
+    k = lambda n: 4 * kernel_arrays(n, uniform)
+    forcing = k("k_prep") + k("k_force_uv") + k("k_force_w") + k("k_force_p") + k("k_force_s")
+    acoustic = k("k_ac_ptuv") + k("k_ac_col")
     if cphopt == 2:
-        return 560 + 88 * nsstp
-    # dry-dynamics variant (qv only): forcing 88, steps 28, Asselin 96
-    return 212 + 88 * nsstp
+        return forcing + nsstp * acoustic + k("k_micro_src") + k("k_micro_fall") + k("k_asselin_sat")
+    # dry dynamics: two scalars in k_force_s and k_steps, six fields in timeflt
+    return (forcing - 4 * 12) + nsstp * acoustic + 4 * 6 + 4 * 24
 
 
 def parse():
@@ -168,7 +179,7 @@ def main():
     stages = s.stage_times()
     dom = max((k for k in kern if k in KERNEL_ARRAYS), key=lambda k: kern[k][0])
     avg_ms = kern[dom][0] / kern[dom][1]
-    per_launch_bytes = KERNEL_ARRAYS[dom] * 4.0 * cells
+    per_launch_bytes = kernel_arrays(dom) * 4.0 * cells
     achieved = per_launch_bytes / (avg_ms * 1e-3)
     step_b = step_bytes_per_cell(cfg.nsstp, cfg.cphopt)
     traffic = measured_traffic("workload %s %dx%dx%d nsstp=%d cphopt=%d" % (

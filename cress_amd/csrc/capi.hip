@@ -30,18 +30,6 @@ using namespace cress;
 
 enum { NSTAGE = 8 };
 
-// Launch shape of the column kernel: CRESS_COL_BY rows of 64 columns per
-// workgroup; CRESS_COL_LDS bytes of (unused) LDS per workgroup cap how many
-// column wavefronts share a CU.  Measured at 1024x1024x80 (ms per launch):
-// 8 waves/SIMD 2.84, 5 waves/SIMD (16 KiB) 2.27, 3 waves/SIMD (24 KiB)
-// 2.33, 2 waves/SIMD (40 KiB) 2.64: past ~5 waves per SIMD the sweeps
-// evict each other's neighbour rows from L2.
-#ifndef CRESS_COL_BY
-#define CRESS_COL_BY 2
-#endif
-#ifndef CRESS_COL_LDS
-#define CRESS_COL_LDS 16384
-#endif
 
 // Kernel families timed individually when timing is on (HIP events on the
 // context stream around every launch).
@@ -66,6 +54,7 @@ struct cress_ctx {
   // base state / metrics
   float *base[CRESS_NBASE];
   float *fc, *rbct;
+  float *tab;  // per-level copies of the horizontally uniform base fields (g.tab)
   // prognostics by physical slot; logical level l lives in slot (rot+l)%3
   float *u[3], *v[3], *w[3], *pp[3], *ptp[3], *q[3], *nci[3];
   int rot;
@@ -215,6 +204,9 @@ int cress_ctx_create(const cress_config *cfg, cress_ctx **out) {
   const long n2 = (long)(cfg->ni + 2) * (cfg->nj + 2);
 #define AL(p, cnt) if ((rc = dalloc(x, &(p), (size_t)(cnt))) != 0) goto fail
   for (int f = 0; f < CRESS_NBASE; f++) AL(x->base[f], n);
+  AL(x->tab, (long)CRESS_NBASE * cfg->nk);
+  x->g.tab = x->tab;
+  x->g.umask = 0u;
   AL(x->fc, 2 * n2);
   AL(x->rbct, 2L * cfg->ni * cfg->nj * cfg->nk);
   for (int l = 0; l < 3; l++) {
@@ -317,6 +309,33 @@ int cress_field_size(const cress_ctx *x, int f, size_t *nfloat) {
   return CRESS_OK;
 }
 
+// A base field whose columns all hold the same bits, level by level: its
+// per-level values go to the table and the kernels read them from there.
+static int note_uniform(cress_ctx *x, int f, const float *host) {
+  const long nxy = x->g.nxy;
+  const int nk = x->g.nk;
+  bool uni = true;
+  for (int k = 0; k < nk && uni; k++) {
+    const float *pl = host + (long)k * nxy;
+    for (long p = 1; p < nxy; p++)
+      if (memcmp(pl + p, pl, sizeof(float)) != 0) {
+        uni = false;
+        break;
+      }
+  }
+  if (uni) {
+    std::vector<float> col(nk);
+    for (int k = 0; k < nk; k++) col[k] = host[(long)k * nxy];
+    HIPCHK(hipMemcpyAsync(x->tab + (long)f * nk, col.data(), nk * sizeof(float), hipMemcpyHostToDevice,
+                          x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->g.umask |= 1u << f;
+  } else {
+    x->g.umask &= ~(1u << f);
+  }
+  return CRESS_OK;
+}
+
 int cress_upload(cress_ctx *x, int f, int level, const float *host) {
   size_t c = 0;
   if (!x || !host) return CRESS_EINVAL;
@@ -324,6 +343,7 @@ int cress_upload(cress_ctx *x, int f, int level, const float *host) {
   if (!d) return CRESS_ERANGE;
   HIPCHK(hipMemcpyAsync(d, host, c * sizeof(float), hipMemcpyHostToDevice, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
+  if (f >= 0 && f < CRESS_NBASE) return note_uniform(x, f, host);
   return CRESS_OK;
 }
 
@@ -342,6 +362,8 @@ int cress_field_device_ptr(cress_ctx *x, int f, int level, void **dptr) {
   if (!x || !dptr) return CRESS_EINVAL;
   float *d = field_ptr(x, f, level, &c);
   if (!d) return CRESS_ERANGE;
+  // the caller may write the field: read it as a 3-D field from now on
+  if (f >= 0 && f < CRESS_NBASE) x->g.umask &= ~(1u << f);
   *dptr = d;
   return CRESS_OK;
 }

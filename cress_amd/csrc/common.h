@@ -16,6 +16,14 @@ struct Grid {
   int ni, nj, nk;
   long nx;   // ni+2
   long nxy;  // (ni+2)*(nj+2)
+  // Base-state and metric fields (ids CRESS_JCB..CRESS_RCSQ) whose uploaded
+  // columns are all equal -- flat terrain and a horizontally uniform base
+  // state -- are also held as one value per level: tab[id*nk + k-1], bit id
+  // of umask set.  Kernels read those through MP/MT/MQ, which take the
+  // table (a wave-uniform address, served by the scalar cache) instead of
+  // streaming the 3-D field; the values are the same bits either way.
+  const float *tab;
+  unsigned umask;
   __host__ __device__ long at(int i, int j, int k) const {
     return (long)i + nx * (long)j + nxy * (long)(k - 1);
   }
@@ -29,8 +37,19 @@ inline Grid make_grid(int ni, int nj, int nk) {
   g.nk = nk;
   g.nx = ni + 2;
   g.nxy = (long)(ni + 2) * (nj + 2);
+  g.tab = nullptr;
+  g.umask = 0u;
   return g;
 }
+
+// A base-state / metric field at linear index p (MP) or (i,j) (MT) on level
+// k; MQ: at byte offset q, for the kernels that address by offsets.  Need a
+// Grid named g in scope.
+#define MUNI(id) ((g.umask >> (id)) & 1u)
+#define MTAB(id, k) g.tab[(id) * g.nk + (k) - 1]
+#define MP(id, arr, p, k) (MUNI(id) ? MTAB(id, k) : (arr)[p])
+#define MT(id, arr, i, j, k) (MUNI(id) ? MTAB(id, k) : (arr)[g.at(i, j, k)])
+#define MQ(id, arr, q, k) (MUNI(id) ? MTAB(id, k) : *(const float *)((const char *)(arr) + (q)))
 
 // Flattened launches with an XCD-aware tile order.  The workgroups of a
 // launch go to the 8 XCDs round-robin (workgroup b to XCD b % 8), and each

@@ -18,6 +18,20 @@ struct ScalarIn {
   const float *rstxu, *rstxv, *rstxwc, *rst, *qvbr, *rbct, *ptbr, *rbr, *w;
 };
 
+// Launch shape of the column kernel: CRESS_COL_BY rows of 64 columns per
+// workgroup; CRESS_COL_LDS bytes of LDS per workgroup cap how many column
+// wavefronts share a CU (and hold the per-level Thomas table when the
+// metrics are uniform).  Measured at 1024x1024x80 (ms per launch): 8
+// waves/SIMD 2.84, 5 waves/SIMD (16 KiB) 2.27, 3 waves/SIMD (24 KiB) 2.33,
+// 2 waves/SIMD (40 KiB) 2.64: past ~5 waves per SIMD the sweeps evict each
+// other's neighbour rows from L2.
+#ifndef CRESS_COL_BY
+#define CRESS_COL_BY 2
+#endif
+#ifndef CRESS_COL_LDS
+#define CRESS_COL_LDS 16384
+#endif
+
 // Rows per workgroup of the fused ptil/stepuv kernel and its tile order
 // (measured at 1024x1024x80, ms per launch: x-fastest 64x8 1.405, 64x4
 // 1.412; z-fastest 64x4 1.571, 64x8 1.580).

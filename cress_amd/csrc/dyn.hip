@@ -59,7 +59,7 @@ __device__ __forceinline__ float smooth4(float f, const Consts &c, int smtopt, b
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float wc_raw(const Grid &g, int sthopt, const float *w,
                                         const float *jcb8w, int i, int j, int k) {
-  return sthopt == 0 ? AT(w, i, j, k) : AT(w, i, j, k) / AT(jcb8w, i, j, k);
+  return sthopt == 0 ? AT(w, i, j, k) : AT(w, i, j, k) / MT(CRESS_JCB8W, jcb8w, i, j, k);
 }
 
 __device__ __forceinline__ float wc_value(const Grid &g, const Opts &o, const float *w,
@@ -99,13 +99,13 @@ __global__ void k_prep(Tiles T, Grid g, Opts o, const float *u, const float *v, 
   if (i >= 1 && i <= g.ni - 1 && j >= 1 && j <= g.nj - 1) {
     const float c = wc_value(g, o, w, jcb8w, i, j, k);
     AT(wc, i, j, k) = c;
-    AT(rstxwc, i, j, k) = AT(rst8w, i, j, k) * c;
+    AT(rstxwc, i, j, k) = MT(CRESS_RST8W, rst8w, i, j, k) * c;
   }
   if (k <= g.nk - 1) {
     if (j >= 1 && j <= g.nj - 1 && i >= o.iw && i <= g.ni + 1 - o.ie)
-      AT(rstxu, i, j, k) = AT(rst8u, i, j, k) * AT(u, i, j, k);
+      AT(rstxu, i, j, k) = MT(CRESS_RST8U, rst8u, i, j, k) * AT(u, i, j, k);
     if (i >= 1 && i <= g.ni - 1 && j >= o.js && j <= g.nj + 1 - o.jn)
-      AT(rstxv, i, j, k) = AT(rst8v, i, j, k) * AT(v, i, j, k);
+      AT(rstxv, i, j, k) = MT(CRESS_RST8V, rst8v, i, j, k) * AT(v, i, j, k);
   }
 }
 
@@ -239,15 +239,17 @@ __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float 
       const bool band = j >= 2 + o.js && j <= g.nj - 2 - o.jn && i >= 2 + o.iw && i <= g.ni - 1 - o.ie;
       f = smooth4(f, c, o.smtopt, band, k == 2, k == g.nk - 2, [&](int di, int dj, int dk) {
         const long p = g.at(i + di, j + dj, k + dk);
-        return in.rst8u[p] * (in.up[p] - in.ubr[p]) / in.jcb8u[p];
+        const int kk = k + dk;
+        return MP(CRESS_RST8U, in.rst8u, p, kk) * (in.up[p] - MP(CRESS_UBR, in.ubr, p, kk)) /
+               MP(CRESS_JCB8U, in.jcb8u, p, kk);
       });
     }
     if (o.vsp_u && sp)
-      f = f - AT(in.rst8u, i, j, k) * (rbct8s(in.rbct, g, i - 1, j, k) + rbct8s(in.rbct, g, i, j, k)) *
-                  (AT(in.up, i, j, k) - AT(in.ubr, i, j, k));
+      f = f - MT(CRESS_RST8U, in.rst8u, i, j, k) * (rbct8s(in.rbct, g, i - 1, j, k) + rbct8s(in.rbct, g, i, j, k)) *
+                  (AT(in.up, i, j, k) - MT(CRESS_UBR, in.ubr, i, j, k));
     if (o.coropt) {
       auto cr = [&](int ii) {
-        return in.fc[(long)ii + g.nx * j] * AT(in.rst, ii, j, k) * (AT(in.v, ii, j, k) + AT(in.v, ii, j + 1, k));
+        return in.fc[(long)ii + g.nx * j] * MT(CRESS_RST, in.rst, ii, j, k) * (AT(in.v, ii, j, k) + AT(in.v, ii, j + 1, k));
       };
       f = f + (cr(i - 1) + cr(i));
     }
@@ -259,15 +261,17 @@ __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float 
       const bool band = j >= 2 + o.js && j <= g.nj - 1 - o.jn && i >= 2 + o.iw && i <= g.ni - 2 - o.ie;
       f = smooth4(f, c, o.smtopt, band, k == 2, k == g.nk - 2, [&](int di, int dj, int dk) {
         const long p = g.at(i + di, j + dj, k + dk);
-        return in.rst8v[p] * (in.vp[p] - in.vbr[p]) / in.jcb8v[p];
+        const int kk = k + dk;
+        return MP(CRESS_RST8V, in.rst8v, p, kk) * (in.vp[p] - MP(CRESS_VBR, in.vbr, p, kk)) /
+               MP(CRESS_JCB8V, in.jcb8v, p, kk);
       });
     }
     if (o.vsp_v && sp)
-      f = f - AT(in.rst8v, i, j, k) * (rbct8s(in.rbct, g, i, j - 1, k) + rbct8s(in.rbct, g, i, j, k)) *
-                  (AT(in.vp, i, j, k) - AT(in.vbr, i, j, k));
+      f = f - MT(CRESS_RST8V, in.rst8v, i, j, k) * (rbct8s(in.rbct, g, i, j - 1, k) + rbct8s(in.rbct, g, i, j, k)) *
+                  (AT(in.vp, i, j, k) - MT(CRESS_VBR, in.vbr, i, j, k));
     if (o.coropt) {
       auto cr = [&](int jj) {
-        return -in.fc[(long)i + g.nx * jj] * AT(in.rst, i, jj, k) * (AT(in.u, i, jj, k) + AT(in.u, i + 1, jj, k));
+        return -in.fc[(long)i + g.nx * jj] * MT(CRESS_RST, in.rst, i, jj, k) * (AT(in.u, i, jj, k) + AT(in.u, i + 1, jj, k));
       };
       f = f + (cr(j - 1) + cr(j));
     }
@@ -279,14 +283,15 @@ __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float 
 __device__ __forceinline__ float wb8s_at(const Grid &g, const Opts &o, const Consts &c, const ForceIn &in,
                                          int i, int j, int k) {
   const long p = g.at(i, j, k);
-  const float qvbr = in.qvbr[p];
+  const float qvbr = MP(CRESS_QVBR, in.qvbr, p, k);
   const float qvd = in.q[p] - qvbr;
+  const float rst = MP(CRESS_RST, in.rst, p, k), ptbr = MP(CRESS_PTBR, in.ptbr, p, k);
   if (o.cphopt == 0)
-    return c.g05 * in.rst[p] * (qvd / (0.622f + qvbr) - qvd / (1.0f + qvbr) + in.ptp[p] / in.ptbr[p]);
+    return c.g05 * rst * (qvd / (0.622f + qvbr) - qvd / (1.0f + qvbr) + in.ptp[p] / ptbr);
   // qall: totalqwi.f90:163-164, qc+qr+qi+qs+qg left to right
   const float qall = in.q[p + in.qn] + in.q[p + 2 * in.qn] + in.q[p + 3 * in.qn] + in.q[p + 4 * in.qn] +
                      in.q[p + 5 * in.qn];
-  return c.g05 * in.rst[p] * (qvd / (0.622f + qvbr) - (qvd + qall) / (1.0f + qvbr) + in.ptp[p] / in.ptbr[p]);
+  return c.g05 * rst * (qvd / (0.622f + qvbr) - (qvd + qall) / (1.0f + qvbr) + in.ptp[p] / ptbr);
 }
 
 // wfrc over i=2..ni-2, j=2..nj-2, k=2..nk-1
@@ -302,11 +307,12 @@ __global__ void k_force_w(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *
     const bool band = j >= 2 + o.js && j <= g.nj - 2 - o.jn && i >= 2 + o.iw && i <= g.ni - 2 - o.ie;
     f = smooth4(f, c, o.smtopt, band, k == 2, k == g.nk - 1, [&](int di, int dj, int dk) {
       const long p = g.at(i + di, j + dj, k + dk);
-      return in.rst8w[p] * in.wp[p] / in.jcb8w[p];
+      const int kk = k + dk;
+      return MP(CRESS_RST8W, in.rst8w, p, kk) * in.wp[p] / MP(CRESS_JCB8W, in.jcb8w, p, kk);
     });
   }
   if (o.vsp_w && k >= o.ksp0 && k <= g.nk - 1)
-    f = f - rbct2(in.rbct, g, i, j, k) * AT(in.rst8w, i, j, k) * AT(in.wp, i, j, k);
+    f = f - rbct2(in.rbct, g, i, j, k) * MT(CRESS_RST8W, in.rst8w, i, j, k) * AT(in.wp, i, j, k);
   if (o.buyopt == 1 && k >= 3 && k <= g.nk - 2)
     f = f + (wb8s_at(g, o, c, in, i, j, k - 1) + wb8s_at(g, o, c, in, i, j, k));
   AT(wfrc, i, j, k) = adv_w(f, g, o, c, in.w, in.rstxu, in.rstxv, in.rstxwc, i, j, k);
@@ -326,9 +332,9 @@ __global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, c
   const int j = 2 + xt.y * blockDim.y + threadIdx.y;
   const int k = 2 + xt.z;
   if (i > g.ni - 2 || j > g.nj - 2) return;
-  auto jxu = [&](int ii, int jj) { return AT(jcb8u, ii, jj, k) * AT(u, ii, jj, k); };
-  auto jxv = [&](int ii, int jj) { return AT(jcb8v, ii, jj, k) * AT(v, ii, jj, k); };
-  auto jxw = [&](int kk) { return AT(jcb8w, i, j, kk) * AT(wc, i, j, kk); };
+  auto jxu = [&](int ii, int jj) { return MT(CRESS_JCB8U, jcb8u, ii, jj, k) * AT(u, ii, jj, k); };
+  auto jxv = [&](int ii, int jj) { return MT(CRESS_JCB8V, jcb8v, ii, jj, k) * AT(v, ii, jj, k); };
+  auto jxw = [&](int kk) { return MT(CRESS_JCB8W, jcb8w, i, j, kk) * AT(wc, i, j, kk); };
   auto t1 = [&](int ii) { return jxu(ii, j) * (AT(pp, ii, j, k) - AT(pp, ii - 1, j, k)) * c.dxv05n; };
   auto t2 = [&](int jj) { return jxv(i, jj) * (AT(pp, i, jj, k) - AT(pp, i, jj - 1, k)) * c.dyv05n; };
   auto t3 = [&](int kk) { return jxw(kk) * (AT(pp, i, j, kk) - AT(pp, i, j, kk - 1)) * c.dzv05n; };
@@ -366,7 +372,8 @@ __global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, c
                 [&](int di, int dj, int dk) { return AT(ppp, i + di, j + dj, k + dk); });
   }
   if (o.vsp_p && k >= o.ksp0 - 1 && k <= g.nk - 2)
-    f = f - 0.5f * (rbct2(rbct, g, i, j, k) + rbct2(rbct, g, i, j, k + 1)) * AT(jcb, i, j, k) * AT(ppp, i, j, k);
+    f = f - 0.5f * (rbct2(rbct, g, i, j, k) + rbct2(rbct, g, i, j, k + 1)) * MT(CRESS_JCB, jcb, i, j, k) *
+                AT(ppp, i, j, k);
   AT(pfrc, i, j, k) = f;
 }
 
@@ -471,7 +478,7 @@ __global__ void __launch_bounds__(256) k_force_s(Tiles T, Grid g, Opts o, Consts
   float r = 0.0f, rst = 0.0f;
   if (sponge) {
     r = 0.5f * (rbct2(in.rbct, g, i, j, k) + rbct2(in.rbct, g, i, j, k + 1));
-    rst = LD(in.rst, q);
+    rst = MQ(CRESS_RST, in.rst, q, k);
   }
   for (int sc = 0; sc < nscalar; sc++) {
     float f = adv_s_pre(g, o, c, h4, v4, fu, fv, fw, in.s[sc], q);
@@ -479,21 +486,24 @@ __global__ void __launch_bounds__(256) k_force_s(Tiles T, Grid g, Opts o, Consts
       const float *sp = in.sp[sc];
       f = smooth4(f, c, o.smtopt, h4, k == 2, k == g.nk - 2, [&](int di, int dj, int dk) {
         const long p = g.at(i + di, j + dj, k + dk);
-        return sc == 1 ? in.rbr[p] * (sp[p] - in.qvbr[p]) : in.rbr[p] * sp[p];
+        const int kk = k + dk;
+        return sc == 1 ? MP(CRESS_RBR, in.rbr, p, kk) * (sp[p] - MP(CRESS_QVBR, in.qvbr, p, kk))
+                       : MP(CRESS_RBR, in.rbr, p, kk) * sp[p];
       });
     }
     const int vsp = sc == 0 ? o.vsp_pt : (sc == 1 ? o.vsp_qv : o.vsp_q);
     if (vsp && sponge) {
-      const float d = sc == 1 ? LD(in.sp[sc], q) - LD(in.qvbr, q) : LD(in.sp[sc], q);
+      const float d = sc == 1 ? LD(in.sp[sc], q) - MQ(CRESS_QVBR, in.qvbr, q, k) : LD(in.sp[sc], q);
       f = f - r * rst * d;
     }
     if (sc == 0) {
       const float dziv25 = 0.25f * c.dziv;
-      auto a8 = [&](uint32_t qq) {
-        return (LD(in.rbr, qq - sk) + LD(in.rbr, qq)) * LD(in.w, qq) * (LD(in.ptbr, qq - sk) - LD(in.ptbr, qq)) *
-               dziv25;
+      // level kk of offset qq
+      auto a8 = [&](uint32_t qq, int kk) {
+        return (MQ(CRESS_RBR, in.rbr, qq - sk, kk - 1) + MQ(CRESS_RBR, in.rbr, qq, kk)) * LD(in.w, qq) *
+               (MQ(CRESS_PTBR, in.ptbr, qq - sk, kk - 1) - MQ(CRESS_PTBR, in.ptbr, qq, kk)) * dziv25;
       };
-      f = f + (a8(q) + a8(q + sk));
+      f = f + (a8(q, k) + a8(q + sk, k + 1));
     }
     ST(in.frc[sc], q, f);
   }
@@ -516,11 +526,13 @@ __device__ __forceinline__ float ptil_at(const Grid &g, const Opts &o, const Con
                                          int j, int k) {
   const long p = g.at(i, j, k);
   if (o.divopt == 0) return a.ppf[p];
-  const float s1a = a.rst8u[p] * a.uf[p], s1b = a.rst8u[p + 1] * a.uf[p + 1];
-  const float s2a = a.rst8v[p] * a.vf[p], s2b = a.rst8v[p + g.nx] * a.vf[p + g.nx];
-  const float s3a = a.rst8w[p] * a.wc[p], s3b = a.rst8w[p + g.nxy] * a.wc[p + g.nxy];
+  const float s1a = MP(CRESS_RST8U, a.rst8u, p, k) * a.uf[p], s1b = MP(CRESS_RST8U, a.rst8u, p + 1, k) * a.uf[p + 1];
+  const float s2a = MP(CRESS_RST8V, a.rst8v, p, k) * a.vf[p];
+  const float s2b = MP(CRESS_RST8V, a.rst8v, p + g.nx, k) * a.vf[p + g.nx];
+  const float s3a = MP(CRESS_RST8W, a.rst8w, p, k) * a.wc[p];
+  const float s3b = MP(CRESS_RST8W, a.rst8w, p + g.nxy, k + 1) * a.wc[p + g.nxy];
   const float div = (s3a - s3b) * c.dziv + ((s1a - s1b) * c.dxiv + (s2a - s2b) * c.dyiv);
-  return a.ppf[p] + a.divcv * a.jcb[p] * div;
+  return a.ppf[p] + a.divcv * MP(CRESS_JCB, a.jcb, p, k) * div;
 }
 
 __global__ void __launch_bounds__(64 * CRESS_PTUV_BY) k_ac_ptuv(Tiles T, Grid g, Opts o, Consts c, AcPtUv a) {
@@ -539,14 +551,14 @@ __global__ void __launch_bounds__(64 * CRESS_PTUV_BY) k_ac_ptuv(Tiles T, Grid g,
   const long p = g.at(i, j, k);
   a.ptil[p] = pown;
   if (k < 2 || k > g.nk - 2 || i < 2 || j < 2) return;
-  const float pc = a.jcb[p] * pown;
+  const float pc = MP(CRESS_JCB, a.jcb, p, k) * pown;
   if (j <= g.nj - 2) {
-    const float upg = (a.jcb[p - 1] * P[ty + 1][tx] - pc) * c.dxiv;
-    a.ufn[p] = a.uf[p] + a.dts * (a.ufrc[p] + upg) / a.rst8u[p];
+    const float upg = (MP(CRESS_JCB, a.jcb, p - 1, k) * P[ty + 1][tx] - pc) * c.dxiv;
+    a.ufn[p] = a.uf[p] + a.dts * (a.ufrc[p] + upg) / MP(CRESS_RST8U, a.rst8u, p, k);
   }
   if (i <= g.ni - 2) {
-    const float vpg = (a.jcb[p - g.nx] * P[ty][tx + 1] - pc) * c.dyiv;
-    a.vfn[p] = a.vf[p] + a.dts * (a.vfrc[p] + vpg) / a.rst8v[p];
+    const float vpg = (MP(CRESS_JCB, a.jcb, p - g.nx, k) * P[ty][tx + 1] - pc) * c.dyiv;
+    a.vfn[p] = a.vf[p] + a.dts * (a.vfrc[p] + vpg) / MP(CRESS_RST8V, a.rst8v, p, k);
   }
 }
 
@@ -557,17 +569,23 @@ __global__ void __launch_bounds__(64 * CRESS_PTUV_BY) k_ac_ptuv(Tiles T, Grid g,
 // phy2cnt+vbcwc+vbcw for this column, diverpiv + advbspi 'fore', steppi
 // (steppi.f90:211-223) and vbcp (vbcp.f90:126-162).
 //
-// One lane per column and no LDS, so occupancy is set by registers alone.
-// Two per-column intermediates go to global slots that nothing else reads
-// while the kernel runs: fp(k) into wc(k) (wc is written with its final
-// value in the last pass), and the Thomas super-diagonal ss'(k) into
-// ptil(k-1), whose value the column has consumed by then (ptil is read by
-// this column only, and recomputed before its next use).
+// One lane per column.  Two per-column intermediates go to global slots
+// that nothing else reads while the kernel runs: fp(k) into wc(k) (wc is
+// written with its final value in the last pass), and the Thomas
+// super-diagonal ss'(k) into ptil(k-1), whose value the column has consumed
+// by then (ptil is read by this column only, and recomputed before its
+// next use).  ss' depends on the metrics and dts alone, so when those are
+// horizontally uniform it is one number per level: the workgroup's first
+// lane keeps it in an LDS table instead, and the ptil round trip goes away.
 __global__ void __launch_bounds__(128) k_ac_col(Tiles T, Grid g, Opts o, Consts c, AcCol a) {
+  extern __shared__ float sst[];
   const uint3 xt = xcd_tile(T);
   const int i = 2 + xt.x * blockDim.x + threadIdx.x;
   const int j = 2 + xt.y * blockDim.y + threadIdx.y;
-  if (i > g.ni - 2 || j > g.nj - 2) return;
+  const bool act = i <= g.ni - 2 && j <= g.nj - 2;   // lane (0,0) always is
+  const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
+  const bool uni = MUNI(CRESS_RST8W) && MUNI(CRESS_RCSQ) && MUNI(CRESS_RST) && MUNI(CRESS_RBR) &&
+                   MUNI(CRESS_JCB) && g.nk * 4 <= CRESS_COL_LDS;
   const int nk = g.nk;
   // byte offsets (a field is below 4 GiB; checked at context creation)
   const uint32_t sk = 4u * (uint32_t)g.nxy, sx = 4u * (uint32_t)g.nx;
@@ -584,13 +602,16 @@ __global__ void __launch_bounds__(128) k_ac_col(Tiles T, Grid g, Opts o, Consts 
   float fwc_km1 = 0.0f, wcc_km1 = 0.0f, rcsq_km1 = 0.0f, rst_km1 = 0.0f;
   float ssn_km1 = 0.0f, ff_km1 = 0.0f;
   const uint32_t q0 = 4u * (uint32_t)g.at(i, j, 2);
-  float wf_k = LD(wf, q0);
+  float wf_k = act ? LD(wf, q0) : 0.0f;
   uint32_t q = q0;
-  for (int k = 2; k <= nk - 2; k++, q += sk) {
+  for (int k = 2; k <= nk - 2 && act; k++, q += sk) {
     const float wf_kp1 = LD(wf, q + sk);
-    const float rcsq = LD(a.rcsq, q), rst = LD(a.rst, q), jcb = LD(a.jcb, q), rbr = LD(a.rbr, q);
-    const float ju0 = LD(a.jcb8u, q) * LD(a.uf, q), ju1 = LD(a.jcb8u, q + 4) * LD(a.uf, q + 4);
-    const float jv0 = LD(a.jcb8v, q) * LD(a.vf, q), jv1 = LD(a.jcb8v, q + sx) * LD(a.vf, q + sx);
+    const float rcsq = MQ(CRESS_RCSQ, a.rcsq, q, k), rst = MQ(CRESS_RST, a.rst, q, k);
+    const float jcb = MQ(CRESS_JCB, a.jcb, q, k), rbr = MQ(CRESS_RBR, a.rbr, q, k);
+    const float ju0 = MQ(CRESS_JCB8U, a.jcb8u, q, k) * LD(a.uf, q);
+    const float ju1 = MQ(CRESS_JCB8U, a.jcb8u, q + 4, k) * LD(a.uf, q + 4);
+    const float jv0 = MQ(CRESS_JCB8V, a.jcb8v, q, k) * LD(a.vf, q);
+    const float jv1 = MQ(CRESS_JCB8V, a.jcb8v, q + sx, k) * LD(a.vf, q + sx);
     const float hdiv = rcsq * ((ju0 - ju1) * c.dxiv + (jv0 - jv1) * c.dyiv);
     const float vdiv = rcsq * (wf_k - wf_kp1) * c.dziv;
     const float fp = LD(a.pfrc, q) + hdiv + c.weic1m * (c.g05 * (wf_k + wf_kp1) * rst + vdiv);
@@ -605,7 +626,7 @@ __global__ void __launch_bounds__(128) k_ac_col(Tiles T, Grid g, Opts o, Consts 
       const float wpg = (pt_km1 - pt_k) * c.dziv;
       float fw = LD(a.wfrc, q) + wpg + (fpdvj_km1 - fpdvj) * dtdzw;
       if (o.buyopt == 1) fw = fw + (wb_km1 + wb);
-      const float rst8w = LD(a.rst8w, q);
+      const float rst8w = MQ(CRESS_RST8W, a.rst8w, q, k);
       float wfn = wf_k + dts * fw / rst8w;
       float rr, ss, tt;
       if (o.buyopt == 0) {
@@ -637,7 +658,8 @@ __global__ void __launch_bounds__(128) k_ac_col(Tiles T, Grid g, Opts o, Consts 
         ssn = tt * r;
         ffn = (wfn - rr * ff_km1) * r;
       }
-      ST(ptil, q - sk, ssn);
+      if (!uni) ST(ptil, q - sk, ssn);
+      else if (lead) sst[k] = ssn;
       ST(wf, q, ffn);
       ssn_km1 = ssn;
       ff_km1 = ffn;
@@ -651,6 +673,8 @@ __global__ void __launch_bounds__(128) k_ac_col(Tiles T, Grid g, Opts o, Consts 
     pt_km1 = pt_k;
     wf_k = wf_kp1;
   }
+  if (uni) __syncthreads();   // the ss' table is complete
+  if (!act) return;
   // pass 2 (downward, k=nk-2..2): back substitution (gaussel.f90:248-260)
   // fused with diverpiv + advbspi 'fore' and steppi, whose per-level update
   // of ppf needs only w(k), w(k+1) and fp(k), in any order of k.  The
@@ -664,15 +688,15 @@ __global__ void __launch_bounds__(128) k_ac_col(Tiles T, Grid g, Opts o, Consts 
   q = q0 + (uint32_t)(nk - 4) * sk;
   for (int k = nk - 2; k >= 2; k--, q -= sk) {
     if (k <= nk - 3 && k >= 3) {
-      wk = LD(wf, q) - LD(ptil, q - sk) * wkp1;
+      wk = LD(wf, q) - (uni ? sst[k] : LD(ptil, q - sk)) * wkp1;
       ST(wf, q, wk);
     } else if (k == 2) {
       wk = w2;
     }
-    const float vdiv = LD(a.rcsq, q) * (wk - wkp1) * c.dziv;
-    const float fp = LD(wc, q) + c.weicoe * (c.g05 * (wk + wkp1) * LD(a.rst, q) + vdiv);
-    ST(ppf, q, LD(ppf, q) + dts * fp / LD(a.jcb, q));
-    ST(wc, q, k == 2 ? wc2 : (o.sthopt == 0 ? wk : wk / LD(a.jcb8w, q)));
+    const float vdiv = MQ(CRESS_RCSQ, a.rcsq, q, k) * (wk - wkp1) * c.dziv;
+    const float fp = LD(wc, q) + c.weicoe * (c.g05 * (wk + wkp1) * MQ(CRESS_RST, a.rst, q, k) + vdiv);
+    ST(ppf, q, LD(ppf, q) + dts * fp / MQ(CRESS_JCB, a.jcb, q, k));
+    ST(wc, q, k == 2 ? wc2 : (o.sthopt == 0 ? wk : wk / MQ(CRESS_JCB8W, a.jcb8w, q, k)));
     wkp1 = wk;
   }
   // phy2cnt+vbcwc at the outer rows and vbcw, from the solved column
@@ -681,14 +705,14 @@ __global__ void __launch_bounds__(128) k_ac_col(Tiles T, Grid g, Opts o, Consts 
   AT(wc, i, j, nk) = wc_value(g, o, wf, a.jcb8w, i, j, nk);
   if (o.bbc == 2) {
     const float z = 0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
-    AT(wf, i, j, 1) = -AT(a.jcb8w, i, j, 3) * wc_raw(g, o.sthopt, wf, a.jcb8w, i, j, 3) - z;
+    AT(wf, i, j, 1) = -MT(CRESS_JCB8W, a.jcb8w, i, j, 3) * wc_raw(g, o.sthopt, wf, a.jcb8w, i, j, 3) - z;
     AT(wf, i, j, 2) = w2;
   } else {
     AT(wf, i, j, 1) = AT(wf, i, j, 2);
   }
   if (o.tbc == 2) {
     const float z = 0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
-    AT(wf, i, j, nk) = -AT(a.jcb8w, i, j, nk - 2) * wc_raw(g, o.sthopt, wf, a.jcb8w, i, j, nk - 2) - z;
+    AT(wf, i, j, nk) = -MT(CRESS_JCB8W, a.jcb8w, i, j, nk - 2) * wc_raw(g, o.sthopt, wf, a.jcb8w, i, j, nk - 2) - z;
     AT(wf, i, j, nk - 1) = 0.0f;
   } else {
     AT(wf, i, j, nk) = AT(wf, i, j, nk - 1);
@@ -723,7 +747,7 @@ __global__ void k_ac_ring(Grid g, Opts o, const float *jcb8w, float *wf, float *
     if (o.bbc == 2) {
       const float w3 = raw(3);
       AT(wc, i, j, 1) = -w3;
-      AT(wf, i, j, 1) = -AT(jcb8w, i, j, 3) * w3 - z;
+      AT(wf, i, j, 1) = -MT(CRESS_JCB8W, jcb8w, i, j, 3) * w3 - z;
       AT(ppf, i, j, 1) = 2.0f * AT(ppf, i, j, 2) - AT(ppf, i, j, 3);
     } else {
       AT(wc, i, j, 1) = raw(2);
@@ -734,7 +758,7 @@ __global__ void k_ac_ring(Grid g, Opts o, const float *jcb8w, float *wf, float *
     if (o.tbc == 2) {
       const float wm = raw(nk - 2);
       AT(wc, i, j, nk) = -wm;
-      AT(wf, i, j, nk) = -AT(jcb8w, i, j, nk - 2) * wm - z;
+      AT(wf, i, j, nk) = -MT(CRESS_JCB8W, jcb8w, i, j, nk - 2) * wm - z;
     } else {
       AT(wc, i, j, nk) = raw(nk - 1);
       AT(wf, i, j, nk) = AT(wf, i, j, nk - 1);
@@ -764,7 +788,7 @@ __global__ void k_steps(Tiles T, Grid g, float dtb2, const float *rst, StepsIn s
   const int k = 2 + xt.z;
   if (i > g.ni - 2 || j > g.nj - 2) return;
   const long p = g.at(i, j, k);
-  const float d = dtb2 / rst[p];
+  const float d = dtb2 / MP(CRESS_RST, rst, p, k);
   s.ptpf[p] = s.ptpp[p] + s.ptfrc[p] * d;
   const float a = s.qp[p] + s.qvfrc[p] * d;
   s.qf[p] = a > 0.0f ? a : 0.0f;

@@ -79,10 +79,10 @@ __global__ void __launch_bounds__(256) k_micro_src(Tiles T, Grid g, MicroK K, Mi
     const long p = g.at(i, j, k);
     const long qn = a.qn;
     const float thresq = K.thresq, dtb = K.dtb;
-    const float rbr = a.rbr[p];
+    const float rbr = MP(CRESS_RBR, a.rbr, p, k);
     // getvdens, getexner (past p')
     const float rbv = 1.0f / rbr;
-    const float pr = a.pbr[p] + a.ppp[p];
+    const float pr = MP(CRESS_PBR, a.pbr, p, k) + a.ppp[p];
     const float pi = ex((rd / cp) * lg((1.0f / p0) * pr));
     // past level
     const float qvp = a.qp[p], qcp = a.qp[p + qn], qrp = a.qp[p + 2 * qn], qip = a.qp[p + 3 * qn];
@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(256) k_micro_src(Tiles T, Grid g, MicroK K, Mi
       const int is = a.open ? i : (i == 1 ? g.ni - 2 : (i == g.ni - 1 ? 2 : i));
       const int js = a.open ? j : (j == 1 ? g.nj - 2 : (j == g.nj - 1 ? 2 : j));
       const long ps = g.at(is, js, ks);
-      const float d = dtb / a.rst[ps];   // dtb is the leapfrog's 2*dtb here
+      const float d = dtb / MP(CRESS_RST, a.rst, ps, ks);   // dtb is the leapfrog's 2*dtb here
       ptpf = a.ptpp[ps] + a.ptfrc[ps] * d;
       qvf = fmaxr(a.qp[ps] + a.qvfrc[ps] * d, 0.0f);
       qcf = fmaxr(a.qp[ps + qn] + a.sfrc[ps] * d, 0.0f);
@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(256) k_micro_src(Tiles T, Grid g, MicroK K, Mi
         ? K.cugq * ex(gug * lnr0r) * ex((bug * (1.0f / 3.0f)) * lg((1.0f / (cc * rhog)) * qgp / ncgp)) : 0.0f;
     // ---- setblk ----
     const float oned3 = 1.0f / 3.0f;
-    const float t = (a.ptbr[p] + a.ptpp[p]) * pi;
+    const float t = (MP(CRESS_PTBR, a.ptbr, p, k) + a.ptpp[p]) * pi;
     const float tcel = t - t0;
     const float esw = es0 * ex(17.269f * tcel / (t - 35.86f));
     const float esi = es0 * ex(21.875f * tcel / (t - 7.66f));
@@ -484,7 +484,7 @@ __global__ void __launch_bounds__(256) k_micro_src(Tiles T, Grid g, MicroK K, Mi
     a.usq[p] = usq;
     a.ugq[p] = ugq;
     // sedimentation CFL (fallblk.f90:364-377)
-    const float dzjcb = K.dz * a.jcb[p];
+    const float dzjcb = K.dz * MP(CRESS_JCB, a.jcb, p, k);
     cfl = fminr(dzjcb / (ucq + eps), cfl);
     cfl = fminr(dzjcb / (urq + eps), cfl);
     cfl = fminr(dzjcb / (uiq + eps), cfl);
@@ -560,7 +560,7 @@ __global__ void __launch_bounds__(384) k_micro_fall(Tiles T, Grid g, FallArgs a)
       // which makes it the top copy q(nk-1) = q(nk-2)); cw/rw: c = rbr*u
       // one level above, and rst at, sub-step s's current level.
       float lo_q[kFallPipe], lo_f[kFallPipe], out[kFallPipe], cw[kFallPipe], rw[kFallPipe];
-      const float c1 = LD(a.rbr, q1) * speed(q1);
+      const float c1 = MQ(CRESS_RBR, a.rbr, q1, 1) * speed(q1);
       const float q01 = LD(q, q1);
       lo_q[0] = q01;
       lo_f[0] = c1 * q01;
@@ -584,9 +584,9 @@ __global__ void __launch_bounds__(384) k_micro_fall(Tiles T, Grid g, FallArgs a)
       float nrb = 0.0f, nsp = 0.0f, nrs = 0.0f, nq = 0.0f;
       auto fetch = [&](int m) {
         const uint32_t om = q1 + (uint32_t)(m - 1) * sk;
-        nrb = LD(a.rbr, om + sk);
+        nrb = MQ(CRESS_RBR, a.rbr, om + sk, m + 1);
         nsp = speed(om + sk);
-        nrs = LD(a.rst, om);
+        nrs = MQ(CRESS_RST, a.rst, om, m);
         nq = LD(q, om + sk);
       };
       fetch(1);
@@ -637,14 +637,14 @@ __global__ void __launch_bounds__(384) k_micro_fall(Tiles T, Grid g, FallArgs a)
       for (int s = 0; s < npstp; s++) {
         uint32_t o = q1;
         float q_k = LD(q, o);
-        float flx_k = LD(a.rbr, o) * speed(o) * q_k;
+        float flx_k = MQ(CRESS_RBR, a.rbr, o, 1) * speed(o) * q_k;
         const float flx1 = flx_k;
         float flx2 = 0.0f;
         for (int k = 1; k <= nk - 2; k++, o += sk) {
           const float q_kp1 = LD(q, o + sk);
-          const float flx_kp1 = LD(a.rbr, o + sk) * speed(o + sk) * q_kp1;
+          const float flx_kp1 = MQ(CRESS_RBR, a.rbr, o + sk, k + 1) * speed(o + sk) * q_kp1;
           if (k == 1) flx2 = flx_kp1;
-          q_k = fmaxr(q_k + (flx_kp1 - flx_k) / LD(a.rst, o) * dzvdt, 0.0f);
+          q_k = fmaxr(q_k + (flx_kp1 - flx_k) / MQ(CRESS_RST, a.rst, o, k) * dzvdt, 0.0f);
           ST(q, o, q_k);
           flx_k = flx_kp1;
           q_k = q_kp1;
@@ -741,7 +741,7 @@ __global__ void k_satadjst(Tiles T, Grid g, float thresq, const float *pbr, cons
   if (i > g.ni - 1 || j > g.nj - 1) return;
   const long p = g.at(i, j, k);
   float th = ptp[p], qv = q[p], qc = q[p + qn], qi = q[p + 3 * qn], ni = nci[p];
-  satadj_cell(thresq, pbr[p], pp[p], ptbr[p], th, qv, qc, qi, ni);
+  satadj_cell(thresq, MP(CRESS_PBR, pbr, p, k), pp[p], MP(CRESS_PTBR, ptbr, p, k), th, qv, qc, qi, ni);
   ptp[p] = th;
   q[p] = qv;
   q[p + qn] = qc;
@@ -776,7 +776,7 @@ __global__ void __launch_bounds__(256) k_asselin_sat(Tiles T, Grid g, float m1fc
     if (!in || f <= 3 || f == 7 || f == 9 || f == 10) ST(a.x[f], q, x[f]);
   }
   if (!in) return;
-  const float pbr = LD(a.pbr, q), tb = LD(a.ptbr, q);
+  const float pbr = MQ(CRESS_PBR, a.pbr, q, k), tb = MQ(CRESS_PTBR, a.ptbr, q, k);
   satadj_cell(thresq, pbr, x[3], tb, x[4], x[5], x[6], x[8], x[11]);
   ST(a.x[4], q, x[4]);
   ST(a.x[5], q, x[5]);

@@ -106,7 +106,11 @@ int cress_field_size(const cress_ctx *ctx, int field, size_t *nfloat);
 int cress_upload(cress_ctx *ctx, int field, int level, const float *host);
 int cress_download(cress_ctx *ctx, int field, int level, float *host);
 /* Device address of a field (for callers that fill device memory
- * themselves); valid until the next cress_large_step(). */
+ * themselves); valid until the next cress_large_step().  A base-state or
+ * metric field uploaded with every column equal (flat terrain, a
+ * horizontally uniform base state) is also kept as one value per level and
+ * read from there; asking for its device address drops that copy, since the
+ * caller may write the field. */
 int cress_field_device_ptr(cress_ctx *ctx, int field, int level, void **dptr);
 /* Advance nsteps large steps.  first != 0 makes the first of them the
  * reference's ctime=0 step: forward step, dtb=dtbig/2, dts=dtsml/2, no

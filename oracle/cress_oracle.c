@@ -1156,7 +1156,7 @@ void or_stepwi(const or_cfg *c, R dts, const R *jcb, const R *jcb8w, const R *rb
   or_gaussel(c, 2, ni - 2, 2, nj - 2, 3, nk - 2, tmp1, tmp2, tmp3, wf);
   or_bcycle(c, 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk, wf);
   or_bc4news(c, 1, ni - 1, 1, nj - 1, nk, wf);
-  or_phy2cnt(c, 1, jcb8w, wf, wc);
+  or_phy2cnt(c, c->sthopt, jcb8w, wf, wc);   /* stepwi.f90:527-529, idsthopt */
   or_vbcw(c, jcb8w, wc, wf);
 }
 
