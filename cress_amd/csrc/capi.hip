@@ -112,8 +112,8 @@ static int dalloc(cress_ctx *x, float **p, size_t nfloat) {
 static int check_config(const cress_config *c) {
   if (!c || c->ni < 7 || c->nj < 7 || c->nk < 7) return CRESS_EINVAL;
   if (c->nsstp < 1 || !(c->dx > 0) || !(c->dy > 0) || !(c->dz > 0)) return CRESS_EINVAL;
-  // kernels address within a field by 32-bit byte offsets
-  if ((long)(c->ni + 2) * (long)(c->nj + 2) * (long)c->nk * 4L >= (1L << 32)) return CRESS_EUNSUPPORTED;
+  // kernels address within a (row-padded) field by 32-bit byte offsets
+  if (make_grid_padded(c->ni, c->nj, c->nk).n() * 4L >= (1L << 32)) return CRESS_EUNSUPPORTED;
   const int periodic = c->wbc == -1 && c->ebc == -1 && c->sbc == -1 && c->nbc == -1;
   const int open = c->wbc == 7 && c->ebc == 7 && c->sbc == 7 && c->nbc == 7;
   if (!periodic && !open) return CRESS_EUNSUPPORTED;
@@ -170,7 +170,7 @@ int cress_ctx_create(const cress_config *cfg, cress_ctx **out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CRESS_ENODEV;
   cress_ctx *x = new cress_ctx();
   x->cfg = *cfg;
-  x->g = make_grid(cfg->ni, cfg->nj, cfg->nk);
+  x->g = make_grid_padded(cfg->ni, cfg->nj, cfg->nk);
   x->n = x->g.n();
   fold_consts(cfg, &x->c);
   Opts &o = x->o;
@@ -264,10 +264,13 @@ int cress_ctx_destroy(cress_ctx *x) {
   return CRESS_OK;
 }
 
-static float *field_ptr(cress_ctx *x, int f, int level, size_t *count) {
+// Device address of a field; *count: its float count in the host layout;
+// *pad3d: a 3-D field stored with padded rows on the device.
+static float *field_ptr(cress_ctx *x, int f, int level, size_t *count, bool *pad3d = nullptr) {
   const long n = x->n;
+  if (pad3d) *pad3d = (f >= 0 && f < CRESS_NBASE) || (f >= CRESS_U && f < CRESS_NPROG_END);
   const long n2 = (long)(x->cfg.ni + 2) * (x->cfg.nj + 2);
-  *count = (size_t)n;
+  *count = (size_t)(x->cfg.ni + 2) * (size_t)(x->cfg.nj + 2) * (size_t)x->cfg.nk;
   if (f >= 0 && f < CRESS_NBASE) return level == 0 ? x->base[f] : nullptr;
   if (f == CRESS_FC) {
     *count = 2 * n2;
@@ -312,7 +315,7 @@ int cress_field_size(const cress_ctx *x, int f, size_t *nfloat) {
 // A base field whose columns all hold the same bits, level by level: its
 // per-level values go to the table and the kernels read them from there.
 static int note_uniform(cress_ctx *x, int f, const float *host) {
-  const long nxy = x->g.nxy;
+  const long nxy = (long)(x->cfg.ni + 2) * (x->cfg.nj + 2);   // host layout
   const int nk = x->g.nk;
   bool uni = true;
   for (int k = 0; k < nk && uni; k++) {
@@ -336,12 +339,29 @@ static int note_uniform(cress_ctx *x, int f, const float *host) {
   return CRESS_OK;
 }
 
+// Host rows of ni+2 floats <-> device rows of g.nx floats.
+static int copy_field(cress_ctx *x, float *d, const float *h, bool up) {
+  const size_t hrow = (size_t)(x->cfg.ni + 2) * sizeof(float), drow = (size_t)x->g.nx * sizeof(float);
+  const size_t rows = (size_t)(x->cfg.nj + 2) * (size_t)x->cfg.nk;
+  if (up)
+    HIPCHK(hipMemcpy2DAsync(d, drow, h, hrow, hrow, rows, hipMemcpyHostToDevice, x->stream));
+  else
+    HIPCHK(hipMemcpy2DAsync((void *)h, hrow, d, drow, hrow, rows, hipMemcpyDeviceToHost, x->stream));
+  return CRESS_OK;
+}
+
 int cress_upload(cress_ctx *x, int f, int level, const float *host) {
   size_t c = 0;
+  bool p3 = false;
   if (!x || !host) return CRESS_EINVAL;
-  float *d = field_ptr(x, f, level, &c);
+  float *d = field_ptr(x, f, level, &c, &p3);
   if (!d) return CRESS_ERANGE;
-  HIPCHK(hipMemcpyAsync(d, host, c * sizeof(float), hipMemcpyHostToDevice, x->stream));
+  if (p3) {
+    int rc = copy_field(x, d, host, true);
+    if (rc) return rc;
+  } else {
+    HIPCHK(hipMemcpyAsync(d, host, c * sizeof(float), hipMemcpyHostToDevice, x->stream));
+  }
   HIPCHK(hipStreamSynchronize(x->stream));
   if (f >= 0 && f < CRESS_NBASE) return note_uniform(x, f, host);
   return CRESS_OK;
@@ -349,10 +369,16 @@ int cress_upload(cress_ctx *x, int f, int level, const float *host) {
 
 int cress_download(cress_ctx *x, int f, int level, float *host) {
   size_t c = 0;
+  bool p3 = false;
   if (!x || !host) return CRESS_EINVAL;
-  float *d = field_ptr(x, f, level, &c);
+  float *d = field_ptr(x, f, level, &c, &p3);
   if (!d) return CRESS_ERANGE;
-  HIPCHK(hipMemcpyAsync(host, d, c * sizeof(float), hipMemcpyDeviceToHost, x->stream));
+  if (p3) {
+    int rc = copy_field(x, d, host, false);
+    if (rc) return rc;
+  } else {
+    HIPCHK(hipMemcpyAsync(host, d, c * sizeof(float), hipMemcpyDeviceToHost, x->stream));
+  }
   HIPCHK(hipStreamSynchronize(x->stream));
   return CRESS_OK;
 }
@@ -365,6 +391,13 @@ int cress_field_device_ptr(cress_ctx *x, int f, int level, void **dptr) {
   // the caller may write the field: read it as a 3-D field from now on
   if (f >= 0 && f < CRESS_NBASE) x->g.umask &= ~(1u << f);
   *dptr = d;
+  return CRESS_OK;
+}
+
+int cress_field_pitch(const cress_ctx *x, size_t *row, size_t *plane) {
+  if (!x || !row || !plane) return CRESS_EINVAL;
+  *row = (size_t)x->g.nx;
+  *plane = (size_t)x->g.nxy;
   return CRESS_OK;
 }
 
@@ -575,9 +608,9 @@ static int large_step(cress_ctx *x, int first) {
   // (smtopt 2/3) and without, so that the plain path carries none of its
   // registers
   const bool sm = o.smtopt != 0;
-  const Tiles tuv = tiles(ni - 2, nj - 2, 64, 4, nk - 3, 2);
-  const Tiles tw = tiles(ni - 3, nj - 3, 64, 4, nk - 2, 2);
-  const Tiles tp = tiles(ni - 3, nj - 3, 64, 4, nk - 3, 2);
+  const Tiles tuv = tiles(ni, nj - 2, 64, 4, nk - 3, 2);
+  const Tiles tw = tiles(ni - 1, nj - 3, 64, 4, nk - 2, 2);
+  const Tiles tp = tiles(ni - 1, nj - 3, 64, 4, nk - 3, 2);
   if (sm) {
     LAUNCH(x, KT_FORCE_UV, k_force_uv<true>, flat(tuv), b2, 0, tuv, g, o, c, fi, x->ufrc, x->vfrc);
     LAUNCH(x, KT_FORCE_W, k_force_w<true>, flat(tw), b2, 0, tw, g, o, c, fi, x->wfrc);
@@ -632,7 +665,7 @@ static int large_step(cress_ctx *x, int first) {
   HIPCHK(hipMemcpyAsync(x->valt, x->v[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(x->w[F], x->w[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(x->pp[F], x->pp[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
-  LAUNCH(x, KT_PHY2CNT, k_phy2cnt, flat(tiles(ni - 1, nj - 1, 64, 4, nk)), b2, 0, tiles(ni - 1, nj - 1, 64, 4, nk), g,
+  LAUNCH(x, KT_PHY2CNT, k_phy2cnt, flat(tiles(ni, nj - 1, 64, 4, nk)), b2, 0, tiles(ni, nj - 1, 64, 4, nk), g,
          o, x->w[F], B[CRESS_JCB8W], x->wc);
   const float dz = (float)cf.dz;
   const float divcv = 0.05f / dts * dz * dz;
@@ -669,8 +702,8 @@ static int large_step(cress_ctx *x, int first) {
   ac.wc = x->wc;
   ac.ppf = x->pp[F];
   const int nring = 2 * (ni - 1) + 2 * (nj - 3);
-  const Tiles tpu = tiles(ni - 1, nj - 1, 64, CRESS_PTUV_BY, nk - 1, CRESS_PTUV_ORDER);
-  const Tiles tcol = tiles(ni - 3, nj - 3, 64, CRESS_COL_BY, 1, 1);
+  const Tiles tpu = tiles(ni, nj - 1, 64, CRESS_PTUV_BY, nk - 1, CRESS_PTUV_ORDER);
+  const Tiles tcol = tiles(ni - 1, nj - 3, 64, CRESS_COL_BY, 1, 1);
   for (int s = 0; s < cf.nsstp; s++) {
     pu.uf = x->u[F];
     pu.vf = x->v[F];
@@ -737,8 +770,8 @@ static int large_step(cress_ctx *x, int first) {
     sti.qf = x->q[F];
     sti.ncif = x->nci[F];
     sti.qn = n;
-    LAUNCH(x, KT_STEPS, k_steps, flat(tiles(ni - 3, nj - 3, 64, 4, nk - 3)), b2, 0,
-           tiles(ni - 3, nj - 3, 64, 4, nk - 3), g, 2.0f * dtb, B[CRESS_RST], sti, 0);
+    LAUNCH(x, KT_STEPS, k_steps, flat(tiles(ni - 1, nj - 3, 64, 4, nk - 3)), b2, 0,
+           tiles(ni - 1, nj - 3, 64, 4, nk - 3), g, 2.0f * dtb, B[CRESS_RST], sti, 0);
     if (open) {
       ObcStepsArgs osa;
       osa.s[0] = x->ptp[C];
@@ -800,7 +833,7 @@ static int large_step(cress_ctx *x, int first) {
     unsigned seed;
     memcpy(&seed, &dtbs, sizeof seed);
     LAUNCH(x, KT_MICRO_SEED, k_micro_seed, dim3(1), dim3(64), 0, ma.dtpmin, seed);
-    LAUNCH(x, KT_MICRO_SRC, k_micro_src, flat(tiles(ni - 1, nj - 1, 64, 4, nk - 1)), b2, 0, tiles(ni - 1, nj - 1, 64, 4, nk - 1), g, mk, ma);
+    LAUNCH(x, KT_MICRO_SRC, k_micro_src, flat(tiles(ni, nj - 1, 64, 4, nk - 1)), b2, 0, tiles(ni, nj - 1, 64, 4, nk - 1), g, mk, ma);
     FallArgs fa;
     fa.rbr = B[CRESS_RBR];
     fa.rst = B[CRESS_RST];
@@ -818,8 +851,8 @@ static int large_step(cress_ctx *x, int first) {
     fa.dtb = dtbs;
     fa.dziv = c.dziv;
     fa.thresq = c.thresq;
-    LAUNCH(x, KT_MICRO_FALL, k_micro_fall, flat(tiles(ni - 1, nj - 1, 64, 1, 1, 1)), dim3(64, 6), 0,
-           tiles(ni - 1, nj - 1, 64, 1, 1, 1), g, fa);
+    const Tiles tfall = tiles(ni - CRESS_FALL_ORG, nj - 1, 64, 1, 1, 1);
+    LAUNCH(x, KT_MICRO_FALL, k_micro_fall, flat(tfall), dim3(64, 6), 0, tfall, g, fa);
   }
   stamp(x, 4);
 
@@ -841,7 +874,7 @@ static int large_step(cress_ctx *x, int first) {
       }
       as.pbr = B[CRESS_PBR];
       as.ptbr = B[CRESS_PTBR];
-      LAUNCH(x, KT_TIMEFLT, k_asselin_sat, flat(tiles(ni, nj, 64, 4, nk)), b2, 0, tiles(ni, nj, 64, 4, nk), g, c.m1fc2, c.filcoe, c.thresq, as);
+      LAUNCH(x, KT_TIMEFLT, k_asselin_sat, flat(tiles(ni + 1, nj, 64, 4, nk)), b2, 0, tiles(ni + 1, nj, 64, 4, nk), g, c.m1fc2, c.filcoe, c.thresq, as);
       stamp(x, 5);
     } else {
       if (!first) {
@@ -854,13 +887,13 @@ static int large_step(cress_ctx *x, int first) {
           as.xp[f] = xp[f];
           as.xf[f] = xf[f];
         }
-        LAUNCH(x, KT_TIMEFLT, k_timeflt, flat(tiles(ni, nj, 64, 4, 12 * nk)), b2, 0, tiles(ni, nj, 64, 4, 12 * nk), g, c, as);
+        LAUNCH(x, KT_TIMEFLT, k_timeflt, flat(tiles(ni + 1, nj, 64, 4, 12 * nk)), b2, 0, tiles(ni + 1, nj, 64, 4, 12 * nk), g, c, as);
       }
       stamp(x, 5);
       if (cf.cphopt == 2) {
         for (int lv = 0; lv < 2; lv++) {
           const int S = lv == 0 ? C : F;
-          LAUNCH(x, KT_SATADJ, k_satadjst, flat(tiles(ni - 1, nj - 1, 64, 4, nk - 1)), b2, 0, tiles(ni - 1, nj - 1, 64, 4, nk - 1), g, c.thresq,
+          LAUNCH(x, KT_SATADJ, k_satadjst, flat(tiles(ni, nj - 1, 64, 4, nk - 1)), b2, 0, tiles(ni, nj - 1, 64, 4, nk - 1), g, c.thresq,
                  B[CRESS_PBR], B[CRESS_PTBR], x->pp[S], x->ptp[S], x->q[S], x->nci[S], n);
         }
       }

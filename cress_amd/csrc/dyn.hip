@@ -81,10 +81,10 @@ __device__ __forceinline__ float wc_value(const Grid &g, const Opts &o, const fl
 // wc over i=1..ni-1, j=1..nj-1, k=1..nk (the phy2cnt+vbcwc write range)
 __global__ void k_phy2cnt(Tiles T, Grid g, Opts o, const float *w, const float *jcb8w, float *wc) {
   const uint3 xt = xcd_tile(T);
-  const int i = 1 + xt.x * blockDim.x + threadIdx.x;
+  const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 1 + xt.y * blockDim.y + threadIdx.y;
   const int k = 1 + xt.z;
-  if (i > g.ni - 1 || j > g.nj - 1) return;
+  if (i < 1 || i > g.ni - 1 || j > g.nj - 1) return;
   AT(wc, i, j, k) = wc_value(g, o, w, jcb8w, i, j, k);
 }
 
@@ -228,10 +228,10 @@ __device__ float adv_w(float f, const Grid &g, const Opts &o, const Consts &c, c
 template <bool SM>
 __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *ufrc, float *vfrc) {
   const uint3 xt = xcd_tile(T);
-  const int i = 2 + xt.x * blockDim.x + threadIdx.x;
+  const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 2 + xt.y * blockDim.y + threadIdx.y;
   const int k = 2 + xt.z;
-  if (i > g.ni - 1 || j > g.nj - 1) return;
+  if (i < 2 || i > g.ni - 1 || j > g.nj - 1) return;
   const bool sp = k >= o.ksp0 - 1 && k <= g.nk - 2;
   if (j <= g.nj - 2) {
     float f = 0.0f;
@@ -249,7 +249,7 @@ __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float 
                   (AT(in.up, i, j, k) - MT(CRESS_UBR, in.ubr, i, j, k));
     if (o.coropt) {
       auto cr = [&](int ii) {
-        return in.fc[(long)ii + g.nx * j] * MT(CRESS_RST, in.rst, ii, j, k) * (AT(in.v, ii, j, k) + AT(in.v, ii, j + 1, k));
+        return in.fc[(long)ii + (long)(g.ni + 2) * j] * MT(CRESS_RST, in.rst, ii, j, k) * (AT(in.v, ii, j, k) + AT(in.v, ii, j + 1, k));
       };
       f = f + (cr(i - 1) + cr(i));
     }
@@ -271,7 +271,7 @@ __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float 
                   (AT(in.vp, i, j, k) - MT(CRESS_VBR, in.vbr, i, j, k));
     if (o.coropt) {
       auto cr = [&](int jj) {
-        return -in.fc[(long)i + g.nx * jj] * MT(CRESS_RST, in.rst, i, jj, k) * (AT(in.u, i, jj, k) + AT(in.u, i + 1, jj, k));
+        return -in.fc[(long)i + (long)(g.ni + 2) * jj] * MT(CRESS_RST, in.rst, i, jj, k) * (AT(in.u, i, jj, k) + AT(in.u, i + 1, jj, k));
       };
       f = f + (cr(j - 1) + cr(j));
     }
@@ -298,10 +298,10 @@ __device__ __forceinline__ float wb8s_at(const Grid &g, const Opts &o, const Con
 template <bool SM>
 __global__ void k_force_w(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *wfrc) {
   const uint3 xt = xcd_tile(T);
-  const int i = 2 + xt.x * blockDim.x + threadIdx.x;
+  const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 2 + xt.y * blockDim.y + threadIdx.y;
   const int k = 2 + xt.z;
-  if (i > g.ni - 2 || j > g.nj - 2) return;
+  if (i < 2 || i > g.ni - 2 || j > g.nj - 2) return;
   float f = 0.0f;
   if (SM) {
     const bool band = j >= 2 + o.js && j <= g.nj - 2 - o.jn && i >= 2 + o.iw && i <= g.ni - 2 - o.ie;
@@ -328,10 +328,10 @@ __global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, c
                           const float *wc, const float *pp, const float *ppp, const float *rbct,
                           float *pfrc) {
   const uint3 xt = xcd_tile(T);
-  const int i = 2 + xt.x * blockDim.x + threadIdx.x;
+  const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 2 + xt.y * blockDim.y + threadIdx.y;
   const int k = 2 + xt.z;
-  if (i > g.ni - 2 || j > g.nj - 2) return;
+  if (i < 2 || i > g.ni - 2 || j > g.nj - 2) return;
   auto jxu = [&](int ii, int jj) { return MT(CRESS_JCB8U, jcb8u, ii, jj, k) * AT(u, ii, jj, k); };
   auto jxv = [&](int ii, int jj) { return MT(CRESS_JCB8V, jcb8v, ii, jj, k) * AT(v, ii, jj, k); };
   auto jxw = [&](int kk) { return MT(CRESS_JCB8W, jcb8w, i, j, kk) * AT(wc, i, j, kk); };
@@ -457,10 +457,10 @@ __device__ __forceinline__ float adv_s_pre(const Grid &g, const Opts &o, const C
 template <bool SM>
 __global__ void __launch_bounds__(256) k_force_s(Tiles T, Grid g, Opts o, Consts c, ScalarIn in, int nscalar) {
   const uint3 xt = xcd_tile(T);
-  const int i = 2 + xt.x * blockDim.x + threadIdx.x;
+  const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 2 + xt.y * blockDim.y + threadIdx.y;
   const int k = 2 + xt.z;
-  if (i > g.ni - 2 || j > g.nj - 2) return;
+  if (i < 2 || i > g.ni - 2 || j > g.nj - 2) return;
   const uint32_t sx = 4u * (uint32_t)g.nx, sk = 4u * (uint32_t)g.nxy;
   const uint32_t q = 4u * (uint32_t)g.at(i, j, k);
   float fu[4], fv[4], fw[4];
@@ -539,12 +539,12 @@ __global__ void __launch_bounds__(64 * CRESS_PTUV_BY) k_ac_ptuv(Tiles T, Grid g,
   const uint3 xt = xcd_tile(T);
   __shared__ float P[CRESS_PTUV_BY + 1][66];   // P[y][x]: j = j0-1+y, i = i0-1+x
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i0 = 1 + xt.x * 64, j0 = 1 + xt.y * CRESS_PTUV_BY, k = 1 + xt.z;
+  const int i0 = xt.x * 64, j0 = 1 + xt.y * CRESS_PTUV_BY, k = 1 + xt.z;
   const int i = i0 + tx, j = j0 + ty;
-  const bool own = i <= g.ni - 1 && j <= g.nj - 1;
+  const bool own = i >= 1 && i <= g.ni - 1 && j <= g.nj - 1;
   const float pown = own ? ptil_at(g, o, c, a, i, j, k) : 0.0f;
   P[ty + 1][tx + 1] = pown;
-  if (ty == 0) P[0][tx + 1] = (j0 - 1 >= 1 && i <= g.ni - 1) ? ptil_at(g, o, c, a, i, j0 - 1, k) : 0.0f;
+  if (ty == 0) P[0][tx + 1] = (j0 - 1 >= 1 && i >= 1 && i <= g.ni - 1) ? ptil_at(g, o, c, a, i, j0 - 1, k) : 0.0f;
   if (tx == 0) P[ty + 1][0] = (i0 - 1 >= 1 && j <= g.nj - 1) ? ptil_at(g, o, c, a, i0 - 1, j, k) : 0.0f;
   __syncthreads();
   if (!own) return;
@@ -580,10 +580,11 @@ __global__ void __launch_bounds__(64 * CRESS_PTUV_BY) k_ac_ptuv(Tiles T, Grid g,
 __global__ void __launch_bounds__(128) k_ac_col(Tiles T, Grid g, Opts o, Consts c, AcCol a) {
   extern __shared__ float sst[];
   const uint3 xt = xcd_tile(T);
-  const int i = 2 + xt.x * blockDim.x + threadIdx.x;
+  const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 2 + xt.y * blockDim.y + threadIdx.y;
-  const bool act = i <= g.ni - 2 && j <= g.nj - 2;   // lane (0,0) always is
-  const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
+  const bool act = i >= 2 && i <= g.ni - 2 && j <= g.nj - 2;
+  // the first column of the workgroup's first row, always active
+  const bool lead = threadIdx.y == 0 && i == (xt.x == 0 ? 2 : (int)(xt.x * blockDim.x));
   const bool uni = MUNI(CRESS_RST8W) && MUNI(CRESS_RCSQ) && MUNI(CRESS_RST) && MUNI(CRESS_RBR) &&
                    MUNI(CRESS_JCB) && g.nk * 4 <= CRESS_COL_LDS;
   const int nk = g.nk;
@@ -783,10 +784,10 @@ __global__ void k_ac_ring(Grid g, Opts o, const float *jcb8w, float *wf, float *
 // ---------------------------------------------------------------------------
 __global__ void k_steps(Tiles T, Grid g, float dtb2, const float *rst, StepsIn s, int withq) {
   const uint3 xt = xcd_tile(T);
-  const int i = 2 + xt.x * blockDim.x + threadIdx.x;
+  const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 2 + xt.y * blockDim.y + threadIdx.y;
   const int k = 2 + xt.z;
-  if (i > g.ni - 2 || j > g.nj - 2) return;
+  if (i < 2 || i > g.ni - 2 || j > g.nj - 2) return;
   const long p = g.at(i, j, k);
   const float d = dtb2 / MP(CRESS_RST, rst, p, k);
   s.ptpf[p] = s.ptpp[p] + s.ptfrc[p] * d;
@@ -859,7 +860,7 @@ __global__ void k_vbc(Grid g, int bbc, VbcSet vs) {
 // ---------------------------------------------------------------------------
 __global__ void k_timeflt(Tiles T, Grid g, Consts c, AsselinSet as) {
   const uint3 xt = xcd_tile(T);
-  const int i = 1 + xt.x * blockDim.x + threadIdx.x;
+  const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 1 + xt.y * blockDim.y + threadIdx.y;
   const int f = xt.z / g.nk;
   const int k = 1 + xt.z % g.nk;
@@ -868,7 +869,7 @@ __global__ void k_timeflt(Tiles T, Grid g, Consts c, AsselinSet as) {
   const int ihi = kind == 1 ? g.ni : g.ni - 1;
   const int jhi = kind == 2 ? g.nj : g.nj - 1;
   const int khi = kind == 3 ? g.nk : g.nk - 1;
-  if (i > ihi || j > jhi || k > khi) return;
+  if (i < 1 || i > ihi || j > jhi || k > khi) return;
   const long p = g.at(i, j, k);
   float *x = as.x[f];
   x[p] = c.m1fc2 * x[p] + c.filcoe * (as.xf[f][p] + as.xp[f][p]);

@@ -70,11 +70,11 @@ constexpr int kFallPipe = 8;
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_micro_src(Tiles T, Grid g, MicroK K, MicroArgs a) {
   const uint3 xt = xcd_tile(T);
-  const int i = 1 + xt.x * blockDim.x + threadIdx.x;
+  const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 1 + xt.y * blockDim.y + threadIdx.y;
   const int k = 1 + xt.z;
   float cfl = K.dtb;
-  if (i <= g.ni - 1 && j <= g.nj - 1) {
+  if (i >= 1 && i <= g.ni - 1 && j <= g.nj - 1) {
     using namespace mc;
     const long p = g.at(i, j, k);
     const long qn = a.qn;
@@ -511,10 +511,10 @@ __global__ void __launch_bounds__(256) k_micro_src(Tiles T, Grid g, MicroK K, Mi
 // independent in fallblk, so the reference's species order does not matter.
 __global__ void __launch_bounds__(384) k_micro_fall(Tiles T, Grid g, FallArgs a) {
   const uint3 xt = xcd_tile(T);
-  const int i = 1 + xt.x * 64 + threadIdx.x;
+  const int i = CRESS_FALL_ORG + xt.x * 64 + threadIdx.x;
   const int j = 1 + xt.y;
   const int sp = threadIdx.y;
-  const bool in = i <= g.ni - 1;
+  const bool in = i >= 1 && i <= g.ni - 1;
   const float dtb = a.dtb;
   float dtp = fminr(__uint_as_float(*a.dtpmin), dtb);
   int npstp;
@@ -574,7 +574,7 @@ __global__ void __launch_bounds__(384) k_micro_fall(Tiles T, Grid g, FallArgs a)
         cw[s] = 0.0f;
         rw[s] = 0.0f;
       }
-      const long c2 = (long)i + g.nx * j;
+      const long c2 = (long)i + (long)(g.ni + 2) * j;   // 2-D precipitation arrays
       float acc = prec ? prec[c2 + a.n2] : 0.0f;
       float rate = 0.0f;
       const int mlast = nk - 2 + npstp - 1;
@@ -652,7 +652,7 @@ __global__ void __launch_bounds__(384) k_micro_fall(Tiles T, Grid g, FallArgs a)
         // top copy: q(nk-1) = q(nk-2) after the update
         ST(q, q1 + (uint32_t)(nk - 2) * sk, LD(q, q1 + (uint32_t)(nk - 3) * sk));
         if (prec) {
-          const long c2 = (long)i + g.nx * j;
+          const long c2 = (long)i + (long)(g.ni + 2) * j;
           const float rate = (flx1 + flx2) * (0.5f / mc::rhow);
           prec[c2] = rate;
           prec[c2 + a.n2] = prec[c2 + a.n2] + rate * (0.5f * dtp);
@@ -735,10 +735,10 @@ __device__ __forceinline__ void satadj_cell(float thresq, float pbr, float pp, f
 __global__ void k_satadjst(Tiles T, Grid g, float thresq, const float *pbr, const float *ptbr, const float *pp,
                            float *ptp, float *q, float *nci, long qn) {
   const uint3 xt = xcd_tile(T);
-  const int i = 1 + xt.x * blockDim.x + threadIdx.x;
+  const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 1 + xt.y * blockDim.y + threadIdx.y;
   const int k = 1 + xt.z;
-  if (i > g.ni - 1 || j > g.nj - 1) return;
+  if (i < 1 || i > g.ni - 1 || j > g.nj - 1) return;
   const long p = g.at(i, j, k);
   float th = ptp[p], qv = q[p], qc = q[p + qn], qi = q[p + 3 * qn], ni = nci[p];
   satadj_cell(thresq, MP(CRESS_PBR, pbr, p, k), pp[p], MP(CRESS_PTBR, ptbr, p, k), th, qv, qc, qi, ni);
@@ -759,10 +759,10 @@ __global__ void k_satadjst(Tiles T, Grid g, float thresq, const float *pbr, cons
 __global__ void __launch_bounds__(256) k_asselin_sat(Tiles T, Grid g, float m1fc2, float filcoe, float thresq,
                                                      AsselinSat a) {
   const uint3 xt = xcd_tile(T);
-  const int i = 1 + xt.x * blockDim.x + threadIdx.x;
+  const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 1 + xt.y * blockDim.y + threadIdx.y;
   const int k = 1 + xt.z;
-  if (i > g.ni || j > g.nj) return;
+  if (i < 1 || i > g.ni || j > g.nj) return;
   const uint32_t q = 4u * (uint32_t)g.at(i, j, k);
   const bool in = i <= g.ni - 1 && j <= g.nj - 1 && k <= g.nk - 1;
   float x[12], xf[12];

@@ -54,7 +54,7 @@ module m_cress_hip
   end type cress_config
 
   public :: cress_error_string, cress_device_count
-  public :: cress_ctx_create, cress_ctx_destroy, cress_field_size
+  public :: cress_ctx_create, cress_ctx_destroy, cress_field_size, cress_field_pitch
   public :: cress_upload, cress_download, cress_large_step, cress_sync
   public :: cress_set_timing, cress_stage_times
   public :: cress_gaussel, cress_advs, cress_bcycle
@@ -95,6 +95,14 @@ module m_cress_hip
       integer(c_size_t), intent(out) :: nfloat
       integer(c_int) :: cress_field_size
     end function cress_field_size
+
+    function cress_field_pitch(ctx, row, plane)                         &
+     &  bind(c, name='cress_field_pitch')
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: ctx
+      integer(c_size_t), intent(out) :: row, plane
+      integer(c_int) :: cress_field_pitch
+    end function cress_field_pitch
 
     function cress_upload(ctx, field, level, host)                      &
      &  bind(c, name='cress_upload')

@@ -67,6 +67,7 @@ def lib():
         L.cress_download.argtypes = [P, I, I, P]
         L.cress_field_size.argtypes = [P, I, C.POINTER(C.c_size_t)]
         L.cress_field_device_ptr.argtypes = [P, I, I, C.POINTER(P)]
+        L.cress_field_pitch.argtypes = [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
         L.cress_large_step.argtypes = [P, I, I]
         L.cress_sync.argtypes = [P]
         L.cress_set_timing.argtypes = [P, I]
@@ -151,6 +152,12 @@ class Solver:
         if name in ("prwtr", "price"):
             return out.reshape(self.shape[:2] + (2, 2 if name == "prwtr" else 3), order="F")
         return out
+
+    def pitch(self):
+        """(row, plane) float strides of the 3-D fields on the device."""
+        r, p = C.c_size_t(), C.c_size_t()
+        check(self.L.cress_field_pitch(self.ctx, C.byref(r), C.byref(p)), "cress_field_pitch")
+        return r.value, p.value
 
     def device_ptr(self, name, level=0):
         p = C.c_void_p()

@@ -105,8 +105,15 @@ int cress_ctx_destroy(cress_ctx *ctx);
 int cress_field_size(const cress_ctx *ctx, int field, size_t *nfloat);
 int cress_upload(cress_ctx *ctx, int field, int level, const float *host);
 int cress_download(cress_ctx *ctx, int field, int level, float *host);
+/* Device layout of the 3-D fields: element (i,j,k) of a field lies
+ * i + row*j + plane*(k-1) floats from its device address, row >= ni+2
+ * (rows are padded to 128-byte multiples); upload/download convert from and
+ * to the host layout row = ni+2.  2-D fields (fc, rbct, prwtr, price) are
+ * stored as on the host. */
+int cress_field_pitch(const cress_ctx *ctx, size_t *row, size_t *plane);
 /* Device address of a field (for callers that fill device memory
- * themselves); valid until the next cress_large_step().  A base-state or
+ * themselves, in the layout of cress_field_pitch); valid until the next
+ * cress_large_step().  A base-state or
  * metric field uploaded with every column equal (flat terrain, a
  * horizontally uniform base state) is also kept as one value per level and
  * read from there; asking for its device address drops that copy, since the
