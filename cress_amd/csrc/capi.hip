@@ -70,6 +70,7 @@ struct cress_ctx {
   int obc_have;
   std::vector<void *> allocs;
   int timing;
+  int tkern;  // timing of one kernel family only (-1: every family and the stages)
   hipEvent_t ev[NSTAGE + 1];
   double stage_ms[NSTAGE];
   // per-kernel timing: event pairs of this step, accumulated after it
@@ -408,8 +409,9 @@ int cress_sync(cress_ctx *x) {
 }
 
 int cress_set_timing(cress_ctx *x, int on) {
-  if (!x) return CRESS_EINVAL;
-  x->timing = on;
+  if (!x || on < 0 || on >= 2 + KT_N) return CRESS_EINVAL;
+  x->timing = on != 0;
+  x->tkern = on >= 2 ? on - 2 : -1;
   return CRESS_OK;
 }
 
@@ -445,7 +447,7 @@ int cress_reset_timing(cress_ctx *x) {
 // Launch with optional event bracketing.
 #define LAUNCH(x, id, kern, grid, block, lds, ...)                                     \
   do {                                                                                 \
-    const bool t_ = (x)->timing && (x)->nkev < MAXEV;                                  \
+    const bool t_ = (x)->timing && (x)->nkev < MAXEV && ((x)->tkern < 0 || (x)->tkern == (id)); \
     if (t_) hipEventRecord((x)->kev[2 * (x)->nkev], (x)->stream);                      \
     hipLaunchKernelGGL(kern, grid, block, lds, (x)->stream, __VA_ARGS__);              \
     if (t_) {                                                                          \
@@ -502,7 +504,7 @@ static void run_vbc(cress_ctx *x, const VbcSet &vs) {
 }
 
 static inline void stamp(cress_ctx *x, int s) {
-  if (x->timing) hipEventRecord(x->ev[s], x->stream);
+  if (x->timing && x->tkern < 0) hipEventRecord(x->ev[s], x->stream);
 }
 
 // Phase speeds of the open-side fills for the coming step (phvuvw.f90:334-346
@@ -942,7 +944,7 @@ int cress_large_step(cress_ctx *x, int nsteps, int first) {
   for (int s = 0; s < nsteps; s++) {
     int rc = large_step(x, first && s == 0);
     if (rc) return rc;
-    if (x->timing) {
+    if (x->timing && x->tkern < 0) {
       HIPCHK(hipEventSynchronize(x->ev[NSTAGE]));
       flush_kernel_times(x);
       for (int t = 0; t < NSTAGE; t++) {
@@ -950,8 +952,12 @@ int cress_large_step(cress_ctx *x, int nsteps, int first) {
         hipEventElapsedTime(&ms, x->ev[t], x->ev[t + 1]);
         x->stage_ms[t] += ms;
       }
+    } else if (x->timing && x->nkev > MAXEV / 2) {
+      flush_kernel_times(x);
     }
   }
+  // one family timed: its events are read once, after the last step
+  if (x->timing && x->tkern >= 0) flush_kernel_times(x);
   return CRESS_OK;
 }
 

@@ -125,7 +125,10 @@ int cress_field_device_ptr(cress_ctx *ctx, int field, int level, void **dptr);
 int cress_large_step(cress_ctx *ctx, int nsteps, int first);
 int cress_sync(cress_ctx *ctx);
 /* Device time of the last cress_large_step call's kernels, per stage, in ms
- * (measured with HIP events on the context's stream when timing is on). */
+ * (measured with HIP events on the context's stream when timing is on).
+ * on: 0 off; 1 every kernel family and the stages (the host waits for each
+ * step's events); 2 + id: only the launches of kernel family id are
+ * bracketed, nothing else is recorded and the steps run back to back. */
 int cress_set_timing(cress_ctx *ctx, int on);
 int cress_stage_times(cress_ctx *ctx, double *ms, int nstage);
 /* Per-kernel-family device time (HIP events around every launch while
