@@ -19,17 +19,18 @@ struct ScalarIn {
 };
 
 // Launch shape of the column kernel: CRESS_COL_BY rows of 64 columns per
-// workgroup; CRESS_COL_LDS bytes of LDS per workgroup cap how many column
-// wavefronts share a CU (and hold the per-level Thomas table when the
-// metrics are uniform).  Measured at 1024x1024x80 (ms per launch): 8
-// waves/SIMD 2.84, 5 waves/SIMD (16 KiB) 2.27, 3 waves/SIMD (24 KiB) 2.33,
-// 2 waves/SIMD (40 KiB) 2.64: past ~5 waves per SIMD the sweeps evict each
-// other's neighbour rows from L2.
+// workgroup; CRESS_COL_LDS bytes of LDS per workgroup hold the per-level
+// Thomas table when the metrics are uniform and cap how many column
+// wavefronts share a CU.  Measured at 1024x1024x80, ms per launch: with
+// the metrics streamed as 3-D fields, 8 waves/SIMD 2.84, 5 (16 KiB) 2.27,
+// 3 (24 KiB) 2.33 -- more sweeps evicted each other's neighbour rows from
+// L2; with the metrics from per-level tables, 16 KiB 1.41, 12 KiB 1.27,
+// 8 KiB 1.26, 4 KiB (occupancy set by registers) 1.26.
 #ifndef CRESS_COL_BY
 #define CRESS_COL_BY 2
 #endif
 #ifndef CRESS_COL_LDS
-#define CRESS_COL_LDS 16384
+#define CRESS_COL_LDS 4096
 #endif
 
 // Rows per workgroup of the fused ptil/stepuv kernel and its tile order
