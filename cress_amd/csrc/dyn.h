@@ -34,10 +34,13 @@ struct ScalarIn {
 #endif
 
 // Rows per workgroup of the fused ptil/stepuv kernel and its tile order
-// (measured at 1024x1024x80, ms per launch: x-fastest 64x8 1.405, 64x4
-// 1.412; z-fastest 64x4 1.571, 64x8 1.580).
+// (measured at 1024x1024x80, ms per launch, metrics streamed as 3-D fields:
+// x-fastest 64x8 1.405, 64x4 1.412; z-fastest 64x4 1.571, 64x8 1.580; from
+// per-level tables: 64x4 1.11, 64x8 1.14, 64x16 1.08, z-fastest 64x8 1.16.
+// A variant with 64x4 threads each marching 16 rows in y, ptil(i,j-1)
+// carried in a register and ptil(i-1,j) from a wave shuffle, ran 1.17-1.21.)
 #ifndef CRESS_PTUV_BY
-#define CRESS_PTUV_BY 8
+#define CRESS_PTUV_BY 16
 #endif
 #ifndef CRESS_PTUV_ORDER
 #define CRESS_PTUV_ORDER 1

@@ -605,27 +605,46 @@ __global__ void __launch_bounds__(128) k_ac_col(Tiles T, Grid g, Opts o, Consts 
   const uint32_t q0 = 4u * (uint32_t)g.at(i, j, 2);
   float wf_k = act ? LD(wf, q0) : 0.0f;
   uint32_t q = q0;
+  // The field values of level k+1 are loaded before level k's stores, so
+  // that their latency overlaps level k's arithmetic (the loads of one
+  // level do not wait for the previous level's writes).  Level k+1 reads
+  // wf(k+2), ptil(k+1) and ppf(k+1), which pass 1 has not written yet.
+  float n_wf1 = 0.0f, n_u0 = 0.0f, n_u1 = 0.0f, n_v0 = 0.0f, n_v1 = 0.0f;
+  float n_pf = 0.0f, n_pt = 0.0f, n_pp = 0.0f, n_fw = 0.0f;
+  auto pre1 = [&](uint32_t qq) {
+    n_wf1 = LD(wf, qq + sk);
+    n_u0 = LD(a.uf, qq);
+    n_u1 = LD(a.uf, qq + 4);
+    n_v0 = LD(a.vf, qq);
+    n_v1 = LD(a.vf, qq + sx);
+    n_pf = LD(a.pfrc, qq);
+    n_pt = LD(ptil, qq);
+    n_pp = LD(ppf, qq);
+    n_fw = LD(a.wfrc, qq);
+  };
+  if (act) pre1(q0);
   for (int k = 2; k <= nk - 2 && act; k++, q += sk) {
-    const float wf_kp1 = LD(wf, q + sk);
+    const float wf_kp1 = n_wf1, u0 = n_u0, u1 = n_u1, v0 = n_v0, v1 = n_v1;
+    const float pfrc = n_pf, pt_k = n_pt, pp_k = n_pp, wfrc = n_fw;
+    if (k + 1 <= nk - 2) pre1(q + sk);
     const float rcsq = MQ(CRESS_RCSQ, a.rcsq, q, k), rst = MQ(CRESS_RST, a.rst, q, k);
     const float jcb = MQ(CRESS_JCB, a.jcb, q, k), rbr = MQ(CRESS_RBR, a.rbr, q, k);
-    const float ju0 = MQ(CRESS_JCB8U, a.jcb8u, q, k) * LD(a.uf, q);
-    const float ju1 = MQ(CRESS_JCB8U, a.jcb8u, q + 4, k) * LD(a.uf, q + 4);
-    const float jv0 = MQ(CRESS_JCB8V, a.jcb8v, q, k) * LD(a.vf, q);
-    const float jv1 = MQ(CRESS_JCB8V, a.jcb8v, q + sx, k) * LD(a.vf, q + sx);
+    const float ju0 = MQ(CRESS_JCB8U, a.jcb8u, q, k) * u0;
+    const float ju1 = MQ(CRESS_JCB8U, a.jcb8u, q + 4, k) * u1;
+    const float jv0 = MQ(CRESS_JCB8V, a.jcb8v, q, k) * v0;
+    const float jv1 = MQ(CRESS_JCB8V, a.jcb8v, q + sx, k) * v1;
     const float hdiv = rcsq * ((ju0 - ju1) * c.dxiv + (jv0 - jv1) * c.dyiv);
     const float vdiv = rcsq * (wf_k - wf_kp1) * c.dziv;
-    const float fp = LD(a.pfrc, q) + hdiv + c.weic1m * (c.g05 * (wf_k + wf_kp1) * rst + vdiv);
+    const float fp = pfrc + hdiv + c.weic1m * (c.g05 * (wf_k + wf_kp1) * rst + vdiv);
     ST(wc, q, fp);
-    const float pt_k = LD(ptil, q);
     const float fpdvj = fp / jcb;
-    const float wb = (LD(ppf, q) * rst + fp * rbr * dtw) / rcsq * c.g05n;
+    const float wb = (pp_k * rst + fp * rbr * dtw) / rcsq * c.g05n;
     const float t1 = c.g05 * rbr / rcsq;
     const float t2 = c.dziv / jcb;
     const float fwc = t1 + t2, wcc = t1 - t2;
     if (k >= 3) {
       const float wpg = (pt_km1 - pt_k) * c.dziv;
-      float fw = LD(a.wfrc, q) + wpg + (fpdvj_km1 - fpdvj) * dtdzw;
+      float fw = wfrc + wpg + (fpdvj_km1 - fpdvj) * dtdzw;
       if (o.buyopt == 1) fw = fw + (wb_km1 + wb);
       const float rst8w = MQ(CRESS_RST8W, a.rst8w, q, k);
       float wfn = wf_k + dts * fw / rst8w;
@@ -687,16 +706,30 @@ __global__ void __launch_bounds__(128) k_ac_col(Tiles T, Grid g, Opts o, Consts 
   float wkp1 = wm;
   float wk = ff_km1;   // wf(nk-2): the last forward-sweep value is final
   q = q0 + (uint32_t)(nk - 4) * sk;
+  // as in pass 1, level k-1's values are loaded before level k's stores
+  // (pass 2 writes each level once, going down)
+  float m_wf = 0.0f, m_ss = 0.0f, m_fp = 0.0f, m_pp = 0.0f;
+  auto pre2 = [&](int kk, uint32_t qq) {
+    if (kk <= nk - 3 && kk >= 3) {
+      m_wf = LD(wf, qq);
+      m_ss = uni ? sst[kk] : LD(ptil, qq - sk);
+    }
+    m_fp = LD(wc, qq);
+    m_pp = LD(ppf, qq);
+  };
+  pre2(nk - 2, q);
   for (int k = nk - 2; k >= 2; k--, q -= sk) {
+    const float ff_k = m_wf, ss_k = m_ss, fp_k = m_fp, pp_k = m_pp;
+    if (k - 1 >= 2) pre2(k - 1, q - sk);
     if (k <= nk - 3 && k >= 3) {
-      wk = LD(wf, q) - (uni ? sst[k] : LD(ptil, q - sk)) * wkp1;
+      wk = ff_k - ss_k * wkp1;
       ST(wf, q, wk);
     } else if (k == 2) {
       wk = w2;
     }
     const float vdiv = MQ(CRESS_RCSQ, a.rcsq, q, k) * (wk - wkp1) * c.dziv;
-    const float fp = LD(wc, q) + c.weicoe * (c.g05 * (wk + wkp1) * MQ(CRESS_RST, a.rst, q, k) + vdiv);
-    ST(ppf, q, LD(ppf, q) + dts * fp / MQ(CRESS_JCB, a.jcb, q, k));
+    const float fp = fp_k + c.weicoe * (c.g05 * (wk + wkp1) * MQ(CRESS_RST, a.rst, q, k) + vdiv);
+    ST(ppf, q, pp_k + dts * fp / MQ(CRESS_JCB, a.jcb, q, k));
     ST(wc, q, k == 2 ? wc2 : (o.sthopt == 0 ? wk : wk / MQ(CRESS_JCB8W, a.jcb8w, q, k)));
     wkp1 = wk;
   }
