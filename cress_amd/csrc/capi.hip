@@ -610,18 +610,20 @@ static int large_step(cress_ctx *x, int first) {
   // (smtopt 2/3) and without, so that the plain path carries none of its
   // registers
   const bool sm = o.smtopt != 0;
-  const Tiles tuv = tiles(ni, nj - 2, 64, 4, nk - 3, 2);
-  const Tiles tw = tiles(ni - 1, nj - 3, 64, 4, nk - 2, 2);
-  const Tiles tp = tiles(ni - 1, nj - 3, 64, 4, nk - 3, 2);
+  const dim3 bf(64, CRESS_FF_BY), bs(64, CRESS_FS_BY);
+  const Tiles tuv = tiles(ni, nj - 2, 64, CRESS_FF_BY, nk - 3, CRESS_FF_ORDER);
+  const Tiles tw = tiles(ni - 1, nj - 3, 64, CRESS_FF_BY, nk - 2, CRESS_FF_ORDER);
+  const Tiles tp = tiles(ni - 1, nj - 3, 64, CRESS_FF_BY, nk - 3, CRESS_FF_ORDER);
+  const Tiles ts = tiles(ni - 1, nj - 3, 64, CRESS_FS_BY, nk - 3, CRESS_FS_ORDER);
   if (sm) {
-    LAUNCH(x, KT_FORCE_UV, k_force_uv<true>, flat(tuv), b2, 0, tuv, g, o, c, fi, x->ufrc, x->vfrc);
-    LAUNCH(x, KT_FORCE_W, k_force_w<true>, flat(tw), b2, 0, tw, g, o, c, fi, x->wfrc);
-    LAUNCH(x, KT_FORCE_P, k_force_p<true>, flat(tp), b2, 0, tp, g, o, c, B[CRESS_JCB], B[CRESS_JCB8U],
+    LAUNCH(x, KT_FORCE_UV, k_force_uv<true>, flat(tuv), bf, 0, tuv, g, o, c, fi, x->ufrc, x->vfrc);
+    LAUNCH(x, KT_FORCE_W, k_force_w<true>, flat(tw), bf, 0, tw, g, o, c, fi, x->wfrc);
+    LAUNCH(x, KT_FORCE_P, k_force_p<true>, flat(tp), bf, 0, tp, g, o, c, B[CRESS_JCB], B[CRESS_JCB8U],
            B[CRESS_JCB8V], B[CRESS_JCB8W], x->u[C], x->v[C], x->wc, x->pp[C], x->pp[P], x->rbct, x->pfrc);
   } else {
-    LAUNCH(x, KT_FORCE_UV, k_force_uv<false>, flat(tuv), b2, 0, tuv, g, o, c, fi, x->ufrc, x->vfrc);
-    LAUNCH(x, KT_FORCE_W, k_force_w<false>, flat(tw), b2, 0, tw, g, o, c, fi, x->wfrc);
-    LAUNCH(x, KT_FORCE_P, k_force_p<false>, flat(tp), b2, 0, tp, g, o, c, B[CRESS_JCB], B[CRESS_JCB8U],
+    LAUNCH(x, KT_FORCE_UV, k_force_uv<false>, flat(tuv), bf, 0, tuv, g, o, c, fi, x->ufrc, x->vfrc);
+    LAUNCH(x, KT_FORCE_W, k_force_w<false>, flat(tw), bf, 0, tw, g, o, c, fi, x->wfrc);
+    LAUNCH(x, KT_FORCE_P, k_force_p<false>, flat(tp), bf, 0, tp, g, o, c, B[CRESS_JCB], B[CRESS_JCB8U],
            B[CRESS_JCB8V], B[CRESS_JCB8W], x->u[C], x->v[C], x->wc, x->pp[C], x->pp[P], x->rbct, x->pfrc);
   }
   ScalarIn si;
@@ -652,9 +654,9 @@ static int large_step(cress_ctx *x, int first) {
   si.rbr = B[CRESS_RBR];
   si.w = x->w[C];
   if (sm)
-    LAUNCH(x, KT_FORCE_S, k_force_s<true>, flat(tp), b2, 0, tp, g, o, c, si, nsc);
+    LAUNCH(x, KT_FORCE_S, k_force_s<true>, flat(ts), bs, 0, ts, g, o, c, si, nsc);
   else
-    LAUNCH(x, KT_FORCE_S, k_force_s<false>, flat(tp), b2, 0, tp, g, o, c, si, nsc);
+    LAUNCH(x, KT_FORCE_S, k_force_s<false>, flat(ts), bs, 0, ts, g, o, c, si, nsc);
   stamp(x, 1);
 
   // ---- acoustic substeps (smalldrv / hevi) on the future level ----

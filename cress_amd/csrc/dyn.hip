@@ -455,7 +455,7 @@ __device__ __forceinline__ float adv_s_pre(const Grid &g, const Opts &o, const C
 // theta', advbspt (advbspt.f90:143-190).  The flux weights are loaded once
 // and shared by the scalars.
 template <bool SM>
-__global__ void __launch_bounds__(256) k_force_s(Tiles T, Grid g, Opts o, Consts c, ScalarIn in, int nscalar) {
+__global__ void __launch_bounds__(64 * CRESS_FS_BY) k_force_s(Tiles T, Grid g, Opts o, Consts c, ScalarIn in, int nscalar) {
   const uint3 xt = xcd_tile(T);
   const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 2 + xt.y * blockDim.y + threadIdx.y;
