@@ -74,7 +74,7 @@ def build_fortran_host():
     moddir = os.path.join(LIBDIR, "mod")
     os.makedirs(moddir, exist_ok=True)
     objs = []
-    for f in ("m_cress_hip.f90", "cress_run.f90"):
+    for f in ("m_cress_hip.f90", "m_cress_files.f90", "cress_run.f90"):
         obj = os.path.join(moddir, f.replace(".f90", ".o"))
         _run([FLANG, "-O2", "-ffp-contract=off", "-module-dir", moddir, "-I", moddir, "-c",
               os.path.join(FSRC, f), "-o", obj])

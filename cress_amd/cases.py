@@ -174,9 +174,10 @@ class Case:
                 f.write(np.asarray([fid, lv, a.size], np.int32).tobytes())
                 f.write(a.tobytes())
 
-    def write_userconf(self, path, nsteps):
-        """A user.conf with the reference's namelist groups for this case and
-        etime = nsteps * dtbig."""
+    def write_userconf(self, path, nsteps, stime=0.0, extra=()):
+        """A user.conf with the reference's namelist groups for this case,
+        the forecast from stime to etime = stime + nsteps * dtbig, and the
+        extra namelist lines given (e.g. "&outfomat ... /")."""
         c = self.cfg
         vsp = bytes(c.vspvar).decode() if not isinstance(c.vspvar, str) else c.vspvar
         lines = [
@@ -185,7 +186,7 @@ class Case:
             "&gridset dx=%r, dy=%r, dz=%r /" % (c.dx, c.dy, c.dz),
             "&gridsth sthopt=%d /" % c.sthopt,
             "&terrain trnopt=0 /",
-            "&flength stime=0.0, etime=%r /" % (nsteps * c.dtbig),
+            "&flength stime=%r, etime=%r /" % (stime, stime + nsteps * c.dtbig),
             "&boundry wbc=%d, ebc=%d, sbc=%d, nbc=%d, bbc=%d, tbc=%d, gwave=%r /"
             % tuple([1 if b == -1 else b for b in (c.wbc, c.ebc, c.sbc, c.nbc)] + [c.bbc, c.tbc, c.gwave]),
             "&gpvpram vspopt=%d, vspvar='%s' /" % (c.vspopt, vsp[:10]),
@@ -198,7 +199,7 @@ class Case:
             "&ddamping divopt=%d /" % c.divopt,
             "&cloudphy cphopt=%d, haiopt=%d, thresq=%r /" % (c.cphopt, c.haiopt, c.thresq),
             "&turbulen tubopt=0 /",
-        ]
+        ] + list(extra)
         with open(path, "w") as f:
             f.write("\n".join(lines) + "\n")
 

@@ -6,10 +6,18 @@
 ! With --check it prints the resolved options and stops before touching
 ! the GPU.
 !
+! With an &outfomat group in user.conf it also writes the reference's dump
+! files (dmpvar, every dmpitv seconds and at etime) and, with resopt=1, its
+! restart files (every resitv seconds and at etime), as m_cress_files
+! describes; iniopt=2 or 12 in &initype starts from the restart file of
+! time stime after the state file is loaded (the state file still supplies
+! the metrics, which a restart file does not hold).  The step count follows
+! slvstep.f90:168-171 and gettime.f90:122-123 (times in ms).
+!
 ! Reads the run options from the reference's user.conf namelist groups
 ! (group and member names as declared in SrcOrig/defname.f90:806-862), the
-! base state and initial levels from state.bin, advances the model
-! nint((etime-stime)/dtbig) large steps on the GPU through the C ABI, and
+! base state and initial levels from state.bin, advances the model from
+! stime to etime on the GPU through the C ABI, and
 ! writes the present level of every prognostic and the precipitation
 ! arrays to out.bin.
 !
@@ -29,6 +37,7 @@ program cress_run
 
   use iso_c_binding
   use m_cress_hip
+  use m_cress_files
 
   implicit none
 
@@ -58,6 +67,13 @@ program cress_run
   integer cphopt, haiopt, ncbinw, qcgopt
   real thresq, dtcmph(1:6), bbinw, sbinw, eledlt
   integer tubopt, isoopt
+  character(len=108) exprim, crsdir, datdir
+  integer iniopt, snddim, masopt
+  character(len=108) sndtyp
+  real zsnd0, psnd0, maseps, alpha1, alpha2
+  integer dmpfmt, dmplev, dmpmon, resopt, mxnopt
+  character(len=108) dmpvar, mxnvar
+  real dmpitv, monitv, resitv, mxnitv
 
   namelist /dimset/ numpe,xgroup,ygroup,xsub,ysub,xdim,ydim,zdim
   namelist /gridset/ dx,dy,dz,ulat,ulon,riu,rju
@@ -80,6 +96,12 @@ program cress_run
   namelist /cloudphy/ cphopt,haiopt,thresq,dtcmph,ncbinw,bbinw,sbinw,   &
      &                    qcgopt,eledlt
   namelist /turbulen/ tubopt,isoopt
+  namelist /runame/ exprim
+  namelist /drname/ crsdir,datdir
+  namelist /initype/ iniopt,snddim,sndtyp,zsnd0,psnd0,masopt,maseps,     &
+     &                   alpha1,alpha2
+  namelist /outfomat/ dmpfmt,dmplev,dmpmon,dmpvar,dmpitv,monitv,         &
+     &                    resopt,resitv,mxnopt,mxnvar,mxnitv
 
   type(cress_config) :: cfg
   type(c_ptr) :: ctx
@@ -93,6 +115,9 @@ program cress_run
      &  fid_pp, fid_ptp, fid_qv, fid_qc, fid_qr, fid_qi, fid_qs, fid_qg, &
      &  fid_nci /)
   integer(c_size_t) :: nf
+  logical :: outf, moist, opsides, lastfirst
+  integer(8) :: dtms, ctime, ftime, nb0, nb1, ib, etms
+  real(c_float), allocatable :: qvb(:,:,:)
 
   if(command_argument_count() < 3 .or. command_argument_count() > 4) then
     write(*,'(a)') 'usage: cress_run <user.conf> <state.bin> <out.bin> [--check]'
@@ -111,6 +136,9 @@ program cress_run
   gwmopt = 0; impopt = 1; weicoe = 0.6; filcoe = 0.1
   vspopt = 0; vspvar = 'xxxxxxxxxx'; thresq = 1.e-8
   lbcvar = 'xxxxxxxxxx'; lbnews = 0.0; lbnorm = 0.0; gwave = 0.0
+  exprim = 'cress'; crsdir = './'; iniopt = 1
+  dmpfmt = 2; dmplev = 1; dmpmon = 0; dmpvar = repeat('x', 108)
+  dmpitv = 0.0; monitv = 0.0; resopt = 0; resitv = 0.0
 
   open(10, file=trim(confn), status='old', action='read', iostat=ios)
   if(ios /= 0) call fatal('cannot open '//trim(confn))
@@ -130,6 +158,11 @@ program cress_run
   call rd_opt('ddamping')
   call rd_opt('cloudphy')
   call rd_opt('turbulen')
+  call rd_opt('runame')
+  call rd_opt('drname')
+  call rd_opt('initype')
+  outf = .false.
+  call rd_opt('outfomat')
   close(10)
 
   if(numpe /= 1 .or. xsub*xgroup /= 1 .or. ysub*ygroup /= 1) then
@@ -141,6 +174,19 @@ program cress_run
   end if
   if(smtopt /= 0 .and. smtopt /= 2 .and. smtopt /= 3) then
     call fatal('smtopt must be 0, 2 or 3')
+  end if
+
+  if(outf) then
+    if(dmpfmt /= 2 .or. dmplev /= 1 .or. dmpmon /= 0) then
+      call fatal('dumps: only dmpfmt=2, dmplev=1, dmpmon=0 are written')
+    end if
+    if(dmpvar(15:15) == 'o') call fatal('dumps: dmpvar(15:15) (zph) is not held')
+    if(dmpitv < 1.0 .or. (resopt == 1 .and. resitv < 1.0)) then
+      call fatal('dmpitv and resitv must be at least 1 s')
+    end if
+  end if
+  if((iniopt == 2 .or. iniopt == 12) .and. stime <= 0.0) then
+    call fatal('iniopt=2 needs the restart time stime > 0')
   end if
 
 ! open sides: the constant-phase-speed radiation (option 7) without the
@@ -220,7 +266,14 @@ program cress_run
      &  cfg%dtsml, cfg%weicoe, cfg%filcoe, cfg%thresq, cfg%gwave,         &
      &  cfg%smhcoe, cfg%smvcoe
     write(*,'(a,i4)') 'smtopt', cfg%smtopt
-    write(*,'(a,i8)') 'nsteps', nint((etime - stime) / dtbig)
+    dtms = 10_8*int(1.e2*(dtbig+.001e0), 8)
+    write(*,'(a,i8)') 'nsteps', 1000_8*int(etime+.1e0, 8)/dtms -        &
+     &  1000_8*int(stime+.1e0, 8)/dtms
+    write(*,'(a,l2,4i4,2a)') 'outfomat', outf, dmpfmt, dmplev, dmpmon,  &
+     &  resopt, ' ', trim(dmpvar(1:15))
+    write(*,'(a,3es24.16)') 'intervals', dmpitv, resitv, stime
+    write(*,'(a,i4,4a)') 'files', iniopt, ' ', trim(crsdir), ' ',       &
+     &  trim(exprim)
     stop
   end if
 
@@ -243,17 +296,66 @@ program cress_run
   end do
   close(11)
 
-  nsteps = nint((etime - stime) / dtbig)
-  first = 0
-  if(stime == 0.0) first = 1
-  if(nsteps > 0) then
-    rc = cress_large_step(ctx, 1, first)
-    call check(rc, 'cress_large_step')
+  opsides = wbc == 7
+! fmois (chkmoist.f90:113-155): qv > 0 anywhere on i,j < n-1, k < nk
+  allocate(qvb(0:xdim+1,0:ydim+1,1:zdim))
+  rc = cress_download(ctx, fid_qv, 1, qvb)
+  call check(rc, 'cress_download')
+  moist = maxval(qvb(1:xdim-1,1:ydim-1,1:zdim-1)) > 0.0
+  deallocate(qvb)
+  if(iniopt == 2 .or. iniopt == 12) then
+    call read_restart(ctx, trim(out_name(crsdir, exprim, 'res',         &
+     &  1000_8*int(stime+.1e0, 8))), xdim, ydim, zdim, cphopt, opsides, dx, &
+     &  dy, gwave, dtbig, dtsml, moist)
   end if
-  if(nsteps > 1) then
-    rc = cress_large_step(ctx, nsteps - 1, 0)
-    call check(rc, 'cress_large_step')
+
+! steps ib = nb0..nb1 from ctime = dtms*(ib-1) to ftime = dtms*ib
+  dtms = 10_8*int(1.e2*(dtbig+.001e0), 8)
+  nb0 = 1000_8*int(stime+.1e0, 8)/dtms + 1
+  nb1 = 1000_8*int(etime+.1e0, 8)/dtms
+  etms = 1000_8*int(etime+.1e0, 8)
+  nsteps = int(max(nb1 - nb0 + 1, 0_8))
+  if(outf .and. dtms*(nb0 - 1) == 0) then
+    call dump_at(0_8, 0)
   end if
+  lastfirst = .false.
+  ib = nb0
+  do while(ib <= nb1)
+    ctime = dtms*(ib - 1)
+    first = 0
+    if(ctime == 0) first = 1
+    if(outf) then
+      n = 1
+    else
+      n = int(nb1 - ib + 1)
+      if(first == 1) n = 1
+    end if
+    rc = cress_large_step(ctx, n, first)
+    call check(rc, 'cress_large_step')
+    lastfirst = first == 1
+    ib = ib + n
+    ftime = dtms*(ib - 1)
+    if(outf) then
+      if(mod(ftime, 1000_8*int(dmpitv+.1e0, 8)) == 0 .or. ftime == etms) then
+        call dump_at(ftime, 1)
+      end if
+      if(resopt == 1) then
+        if(mod(ftime, 1000_8*int(resitv+.1e0, 8)) == 0 .or. ftime == etms) then
+          rc = cress_sync(ctx)
+          call check(rc, 'cress_sync')
+          if(lastfirst) then
+            call write_restart(ctx, trim(out_name(crsdir, exprim, 'res', ftime)), &
+     &        xdim, ydim, zdim, moist, cphopt, opsides, dx, dy, gwave,       &
+     &        .5e0*dtbig, .5e0*dtsml)
+          else
+            call write_restart(ctx, trim(out_name(crsdir, exprim, 'res', ftime)), &
+     &        xdim, ydim, zdim, moist, cphopt, opsides, dx, dy, gwave,       &
+     &        dtbig, dtsml)
+          end if
+        end if
+      end if
+    end if
+  end do
   rc = cress_sync(ctx)
   call check(rc, 'cress_sync')
 
@@ -281,6 +383,21 @@ program cress_run
      &  ' x', zdim
 
 contains
+
+! The dump of time t (ms) from level lev, then the reset of the dumped
+! precipitation accumulations.
+  subroutine dump_at(t, lev)
+    integer(8), intent(in) :: t
+    integer, intent(in) :: lev
+    integer(c_int) :: rc2
+    rc2 = cress_sync(ctx)
+    call check(rc2, 'cress_sync')
+    call write_dump(ctx, trim(out_name(crsdir, exprim, 'dmp', t)), xdim, &
+     &  ydim, zdim, moist, cphopt, dmpvar, lev)
+    if(moist .and. dmpvar(14:14) == 'o') then
+      call reset_accum(ctx, xdim, ydim, cphopt)
+    end if
+  end subroutine dump_at
 
 ! Reads one optional namelist group; a missing group keeps the defaults.
   subroutine rd_opt(grp)
@@ -316,6 +433,15 @@ contains
       read(10, nml=cloudphy, iostat=ios2)
     case('turbulen')
       read(10, nml=turbulen, iostat=ios2)
+    case('runame')
+      read(10, nml=runame, iostat=ios2)
+    case('drname')
+      read(10, nml=drname, iostat=ios2)
+    case('initype')
+      read(10, nml=initype, iostat=ios2)
+    case('outfomat')
+      read(10, nml=outfomat, iostat=ios2)
+      outf = ios2 == 0
     end select
     if(ios2 > 0) call fatal('namelist '//grp)
   end subroutine rd_opt

@@ -121,3 +121,30 @@ def test_fortran_host_matches_python_host(tmp_path, wbc):
         np.testing.assert_array_equal(recs[host.FID[name]], s.download(name).ravel(order="F"),
                                       err_msg=name)
     s.close()
+
+
+def test_fortran_host_reads_output_groups(tmp_path):
+    """&runame, &drname, &initype and &outfomat (defname.f90:802-871) are
+    read; dumps other than dmpfmt=2/dmplev=1/dmpmon=0 are refused before
+    anything runs."""
+    _need_exe()
+    case = cases.build("bubble", 12, 10, 12, cphopt=2)
+    conf, st, out = (str(tmp_path / n) for n in ("user.conf", "state.bin", "out.bin"))
+    case.write_state(st)
+    groups = ["&runame exprim='run1' /", "&drname crsdir='%s/' /" % tmp_path,
+              "&initype iniopt=2 /",
+              "&outfomat dmpfmt=2, dmplev=1, dmpmon=0, dmpvar='ooooooo-xxxxxox', dmpitv=60.0, "
+              "resopt=1, resitv=120.0 /"]
+    case.write_userconf(conf, 4, stime=30.0, extra=groups)
+    r = subprocess.run([EXE, conf, st, out, "--check"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    kv = {ln.split()[0]: ln.split()[1:] for ln in r.stdout.splitlines() if ln.split()}
+    assert kv["outfomat"] == ["T", "2", "1", "0", "1", "ooooooo-xxxxxox"]
+    assert [float(x) for x in kv["intervals"]] == [60.0, 120.0, 30.0]
+    assert kv["files"] == ["2", str(tmp_path) + "/", "run1"]
+    assert int(kv["nsteps"][0]) == 4
+    for bad, msg in (("dmplev=2", "dmplev"), ("dmpfmt=1", "dmpfmt"), ("dmpmon=1", "dmpmon")):
+        g = groups[:3] + ["&outfomat %s, dmpitv=60.0 /" % bad]
+        case.write_userconf(conf, 4, extra=g)
+        r = subprocess.run([EXE, conf, st, out, "--check"], capture_output=True, text=True, timeout=60)
+        assert r.returncode != 0 and msg in r.stdout, r.stdout
