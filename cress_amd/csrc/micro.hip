@@ -67,8 +67,21 @@ constexpr int kMaxFallSteps = 4096;
 // with it); larger counts fall back to one sweep per sub-step.
 constexpr int kFallPipe = 8;
 
+// Minimum wavefronts per SIMD asked of the compiler for the source kernel
+// (0: its own choice, 121 VGPRs and 4 waves).  Measured at 1024x1024x80,
+// ms per launch: 4 waves 4.67; 5 (96 VGPRs, 32 B/lane spilled) 4.81;
+// 6 (80 VGPRs, 76 B/lane spilled) 4.76.
+#ifndef CRESS_MICRO_WPE
+#define CRESS_MICRO_WPE 0
+#endif
+#if CRESS_MICRO_WPE > 0
+#define CRESS_MICRO_ATTR __attribute__((amdgpu_waves_per_eu(CRESS_MICRO_WPE)))
+#else
+#define CRESS_MICRO_ATTR
+#endif
+
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_micro_src(Tiles T, Grid g, MicroK K, MicroArgs a) {
+__global__ void __launch_bounds__(256) CRESS_MICRO_ATTR k_micro_src(Tiles T, Grid g, MicroK K, MicroArgs a) {
   const uint3 xt = xcd_tile(T);
   const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 1 + xt.y * blockDim.y + threadIdx.y;
@@ -660,11 +673,12 @@ __global__ void __launch_bounds__(384) k_micro_fall(Tiles T, Grid g, FallArgs a)
       }
     }
   }
+  // adjstnci is pointwise: the six wavefronts share the column's levels
   __syncthreads();
-  if (in && sp == 2) {
+  if (in) {
     const float *qi = a.qf + 3 * qn;
-    uint32_t o = q1;
-    for (int k = 1; k <= g.nk - 1; k++, o += sk) {
+    for (int k = 1 + sp; k <= g.nk - 1; k += (int)blockDim.y) {
+      const uint32_t o = q1 + (uint32_t)(k - 1) * sk;
       const float qv = LD(qi, o);
       ST(a.ncif, o, fminr(fmaxr(LD(a.ncif, o), (0.5f / mc::mimax) * qv), (1.e2f / mc::mi0) * qv));
     }
