@@ -964,6 +964,201 @@ int cress_large_step(cress_ctx *x, int nsteps, int first) {
 }
 
 // ---------------------------------------------------------------------------
+// Ensemble: several contexts of any sizes stepped together.  Their large
+// steps are captured once into one hipGraph whose branches are the members'
+// own streams (fork and join by events on a master stream), so one graph
+// launch advances every member by a cycle of steps and the members' kernels
+// run side by side.  A captured step bakes in the buffers of the level
+// rotation (and of the uf/vf alternation when nsstp is odd), so a cycle is
+// 3 steps, or 6 when a member has odd nsstp: the rotation state after a
+// cycle is the one it started from and the graph can be replayed as is.
+// Steps that do not fill a cycle, and the cold-start first step, run
+// eagerly; a graph is recaptured when the members' rotation state is not
+// the one it was captured in.
+// ---------------------------------------------------------------------------
+// The host-side state of a member that its step launches depend on.
+struct EnsState {
+  int rot;
+  float *u[3], *v[3], *ualt, *valt;
+  float obc_dtb, obc_dts;
+  int obc_have;
+  bool operator==(const EnsState &o) const {
+    if (rot != o.rot || ualt != o.ualt || valt != o.valt || obc_have != o.obc_have) return false;
+    if (obc_have && (obc_dtb != o.obc_dtb || obc_dts != o.obc_dts)) return false;
+    for (int l = 0; l < 3; l++)
+      if (u[l] != o.u[l] || v[l] != o.v[l]) return false;
+    return true;
+  }
+};
+
+static EnsState ens_get(const cress_ctx *x) {
+  EnsState t;
+  t.rot = x->rot;
+  for (int l = 0; l < 3; l++) {
+    t.u[l] = x->u[l];
+    t.v[l] = x->v[l];
+  }
+  t.ualt = x->ualt;
+  t.valt = x->valt;
+  t.obc_dtb = x->obc_dtb;
+  t.obc_dts = x->obc_dts;
+  t.obc_have = x->obc_have;
+  return t;
+}
+
+static void ens_set(cress_ctx *x, const EnsState &t) {
+  x->rot = t.rot;
+  for (int l = 0; l < 3; l++) {
+    x->u[l] = t.u[l];
+    x->v[l] = t.v[l];
+  }
+  x->ualt = t.ualt;
+  x->valt = t.valt;
+  x->obc_dtb = t.obc_dtb;
+  x->obc_dts = t.obc_dts;
+  x->obc_have = t.obc_have;
+}
+
+struct cress_ens {
+  std::vector<cress_ctx *> m;
+  hipStream_t master;
+  hipEvent_t fork;
+  std::vector<hipEvent_t> join;
+  hipGraphExec_t exec;
+  bool have;
+  int cycle;
+  std::vector<EnsState> in, out;   // the members' state the graph starts from and leaves
+};
+
+int cress_ens_create(cress_ctx *const *members, int n, cress_ens **out) {
+  if (!members || n < 1 || !out) return CRESS_EINVAL;
+  cress_ens *e = new cress_ens();
+  e->cycle = 3;
+  e->have = false;
+  for (int i = 0; i < n; i++) {
+    bool dup = !members[i];
+    for (int k = 0; k < i && !dup; k++) dup = members[k] == members[i];
+    if (dup) {
+      delete e;
+      return CRESS_EINVAL;
+    }
+    e->m.push_back(members[i]);
+    if (members[i]->cfg.nsstp % 2) e->cycle = 6;
+  }
+  if (hipStreamCreate(&e->master) != hipSuccess) {
+    delete e;
+    return CRESS_EHIP;
+  }
+  e->join.resize(n);
+  bool ok = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) == hipSuccess;
+  for (int i = 0; i < n && ok; i++) ok = hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming) == hipSuccess;
+  if (!ok) return CRESS_EHIP;
+  *out = e;
+  return CRESS_OK;
+}
+
+int cress_ens_destroy(cress_ens *e) {
+  if (!e) return CRESS_EINVAL;
+  hipStreamSynchronize(e->master);
+  if (e->have) hipGraphExecDestroy(e->exec);
+  for (hipEvent_t ev : e->join) hipEventDestroy(ev);
+  hipEventDestroy(e->fork);
+  hipStreamDestroy(e->master);
+  delete e;
+  return CRESS_OK;
+}
+
+// One cycle of steps of every member into a graph; the members' host-side
+// state is put back to where it was, and the state the cycle leaves is kept
+// for after each launch of the graph.
+static int ens_capture(cress_ens *e) {
+  if (e->have) {
+    hipGraphExecDestroy(e->exec);
+    e->have = false;
+  }
+  const size_t n = e->m.size();
+  e->in.resize(n);
+  e->out.resize(n);
+  for (size_t i = 0; i < n; i++) {
+    if (e->m[i]->timing) return CRESS_EINVAL;   // events are not captured
+    e->in[i] = ens_get(e->m[i]);
+  }
+  HIPCHK(hipStreamBeginCapture(e->master, hipStreamCaptureModeThreadLocal));
+  int rc = hipEventRecord(e->fork, e->master) == hipSuccess ? CRESS_OK : CRESS_EHIP;
+  for (size_t i = 0; i < n && rc == CRESS_OK; i++) {
+    cress_ctx *x = e->m[i];
+    if (hipStreamWaitEvent(x->stream, e->fork, 0) != hipSuccess) rc = CRESS_EHIP;
+    for (int s = 0; s < e->cycle && rc == CRESS_OK; s++) rc = large_step(x, 0);
+    if (rc == CRESS_OK && hipEventRecord(e->join[i], x->stream) != hipSuccess) rc = CRESS_EHIP;
+    if (rc == CRESS_OK && hipStreamWaitEvent(e->master, e->join[i], 0) != hipSuccess) rc = CRESS_EHIP;
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t end = hipStreamEndCapture(e->master, &graph);
+  for (size_t i = 0; i < n; i++) {
+    e->out[i] = ens_get(e->m[i]);
+    ens_set(e->m[i], e->in[i]);
+  }
+  if (rc != CRESS_OK || end != hipSuccess) {
+    if (graph) hipGraphDestroy(graph);
+    return rc != CRESS_OK ? rc : CRESS_EHIP;
+  }
+  const hipError_t ie = hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (ie != hipSuccess) return CRESS_EHIP;
+  e->have = true;
+  return CRESS_OK;
+}
+
+static bool ens_ready(const cress_ens *e) {
+  if (!e->have) return false;
+  for (size_t i = 0; i < e->m.size(); i++)
+    if (!(ens_get(e->m[i]) == e->in[i])) return false;
+  return true;
+}
+
+int cress_ens_step(cress_ens *e, int nsteps, int first) {
+  if (!e || nsteps < 0) return CRESS_EINVAL;
+  int done = 0;
+  if (first && nsteps > 0) {
+    for (cress_ctx *x : e->m) {
+      const int rc = large_step(x, 1);
+      if (rc) return rc;
+    }
+    done = 1;
+  }
+  while (nsteps - done >= e->cycle) {
+    // every member's earlier work is ordered before the graph
+    for (size_t i = 0; i < e->m.size(); i++) {
+      HIPCHK(hipEventRecord(e->join[i], e->m[i]->stream));
+      HIPCHK(hipStreamWaitEvent(e->master, e->join[i], 0));
+    }
+    if (!ens_ready(e)) {
+      const int rc = ens_capture(e);
+      if (rc) return rc;
+    }
+    HIPCHK(hipGraphLaunch(e->exec, e->master));
+    for (size_t i = 0; i < e->m.size(); i++) ens_set(e->m[i], e->out[i]);
+    // and the members' later work after it
+    HIPCHK(hipEventRecord(e->fork, e->master));
+    for (cress_ctx *x : e->m) HIPCHK(hipStreamWaitEvent(x->stream, e->fork, 0));
+    done += e->cycle;
+  }
+  for (; done < nsteps; done++)
+    for (cress_ctx *x : e->m) {
+      const int rc = large_step(x, 0);
+      if (rc) return rc;
+    }
+  return CRESS_OK;
+}
+
+int cress_ens_sync(cress_ens *e) {
+  if (!e) return CRESS_EINVAL;
+  HIPCHK(hipStreamSynchronize(e->master));
+  for (cress_ctx *x : e->m) HIPCHK(hipStreamSynchronize(x->stream));
+  return CRESS_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Tier 2 shims
 // ---------------------------------------------------------------------------
 struct DevBuf {

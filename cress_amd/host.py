@@ -77,6 +77,10 @@ def lib():
         L.cress_gaussel.argtypes = [I] * 9 + [P] * 4
         L.cress_advs.argtypes = [C.POINTER(Config)] + [P] * 5
         L.cress_bcycle.argtypes = [C.POINTER(Config)] + [I] * 9 + [P]
+        L.cress_ens_create.argtypes = [C.POINTER(P), I, C.POINTER(P)]
+        L.cress_ens_step.argtypes = [P, I, I]
+        L.cress_ens_sync.argtypes = [P]
+        L.cress_ens_destroy.argtypes = [P]
         _lib = L
     return _lib
 
@@ -178,6 +182,35 @@ class Solver:
         check(self.L.cress_stage_times(self.ctx, buf, 8), "cress_stage_times")
         return dict(zip(["force", "acoustic", "steps", "micro", "timeflt", "satadjst", "halo2",
                          "other"], list(buf)))
+
+
+class Ensemble:
+    """Solvers of any sizes stepped together: each cycle of large steps of
+    every member is one launch of a captured hipGraph (cress_ens_*)."""
+
+    def __init__(self, solvers):
+        self.members = list(solvers)
+        self.L = lib()
+        arr = (C.c_void_p * len(self.members))(*[m.ctx for m in self.members])
+        self.e = C.c_void_p()
+        check(self.L.cress_ens_create(arr, len(self.members), C.byref(self.e)), "cress_ens_create")
+
+    def step(self, nsteps=1, first=False):
+        check(self.L.cress_ens_step(self.e, nsteps, 1 if first else 0), "cress_ens_step")
+
+    def sync(self):
+        check(self.L.cress_ens_sync(self.e), "cress_ens_sync")
+
+    def close(self):
+        if self.e:
+            self.L.cress_ens_destroy(self.e)
+            self.e = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def device_count():

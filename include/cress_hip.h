@@ -138,6 +138,22 @@ const char *cress_kernel_name(int id);
 int cress_kernel_times(cress_ctx *ctx, double *total_ms, long *launches, int n);
 int cress_reset_timing(cress_ctx *ctx);
 
+/* ---- Ensemble: contexts of any sizes stepped together ------------------
+ * The reference runs one model per job; these have no counterpart there.
+ * They batch independent contexts (members) on one GPU: a cycle of large
+ * steps of every member (3 steps, 6 when a member's nsstp is odd) is
+ * captured once into one hipGraph whose parallel branches are the members'
+ * own streams, and each cycle is then one graph launch.  Results are
+ * bitwise those of stepping each member with cress_large_step.  Members
+ * keep their own state and are uploaded/downloaded as usual; timing must be
+ * off while they are stepped here.  cress_ens_step's first != 0 makes the
+ * first step every member's cold-start step (run without the graph). */
+typedef struct cress_ens cress_ens;
+int cress_ens_create(cress_ctx *const *members, int n, cress_ens **out);
+int cress_ens_step(cress_ens *e, int nsteps, int first);
+int cress_ens_sync(cress_ens *e);
+int cress_ens_destroy(cress_ens *e);
+
 /* ---- Tier 2: per-routine drop-in shims (host arrays) -------------------
  * Each mirrors the reference subroutine named in its comment: same array
  * arguments in the same order, options taken from cfg instead of fp ids. */
