@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -108,7 +110,61 @@ def parse():
     ap.add_argument("--cphopt", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-nxy", type=int, default=128)
+    ap.add_argument("--ensemble", type=int, default=12,
+                    help="members of the batched-ensemble measurement (0: none)")
+    ap.add_argument("--ens-steps", type=int, default=6)
     return ap.parse_args()
+
+
+def ensemble_bench(args):
+    """The batched ensemble against the same members stepped one after
+    another: members drawn like the reference-style supercell ensemble (open
+    sides, cold rain, dtbig 3 s, 6 substeps; nx=ny from 128..384, nz 48 or
+    64, bubble 1-4 K, shear 15-35 m/s, member m seeded 1000+m), a cold-start
+    step and one graph cycle of warm-up, then ens_steps large steps each
+    way.  Steps are a multiple of the 6-substep members' 3-step cycle."""
+    from cress_amd import cases, host
+    rng = np.random.default_rng(7)
+    solvers, cells = [], 0
+    t0 = time.time()
+    for m in range(args.ensemble):
+        n = int(rng.choice([128, 192, 256, 320, 384]))
+        nz = int(rng.choice([48, 64]))
+        c = cases.build("supercell", n, n, nz, seed=1000 + m, amp=1.0 + 3.0 * rng.random(),
+                        shear=15.0 + 20.0 * rng.random())
+        s = host.Solver(c.cfg)
+        c.load(s)
+        del c
+        solvers.append(s)
+        cells += n * n * nz
+        print("bench: ensemble member %d %dx%dx%d" % (m, n, n, nz), file=sys.stderr, flush=True)
+    print("bench: ensemble of %d members (%.1f M cells) built in %.1f s"
+          % (args.ensemble, cells / 1e6, time.time() - t0), file=sys.stderr, flush=True)
+    ens = host.Ensemble(solvers)
+    ens.step(1, first=True)
+    ens.step(6)
+    ens.sync()
+    k = args.ens_steps
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.step(k)
+        s.sync()
+    seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ens.step(k)
+    ens.sync()
+    bat = time.perf_counter() - t0
+    ens.close()
+    for s in solvers:
+        s.close()
+    out = {"members": args.ensemble, "cells": cells, "steps": k,
+           "sequential_cell_steps_per_s": cells * k / seq, "batched_cell_steps_per_s": cells * k / bat,
+           "speedup": seq / bat,
+           "how": "members one after another with cress_large_step vs one hipGraph launch per "
+                  "3-step cycle with one branch per member (cress_ens_step)"}
+    print("bench: ensemble sequential %.1f ms/step, batched %.1f ms/step, speedup %.2f"
+          % (1e3 * seq / k, 1e3 * bat / k, seq / bat), file=sys.stderr, flush=True)
+    return out
 
 
 def cpu_baseline(args, nz):
@@ -201,6 +257,7 @@ def main():
               ", ".join("%s %.3f ms x%d" % (k, v[0] / max(v[1], 1), v[1]) for k, v in
                         sorted(kern.items(), key=lambda kv: -kv[1][0]))), file=sys.stderr, flush=True)
     s.close()
+    ensemble = ensemble_bench(args) if args.ensemble > 0 else None
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args, nz)
@@ -226,6 +283,7 @@ def main():
                      "step_frac": step_b * cells / (ms_per_step * 1e-3) / PEAK_HBM,
                      "step_bytes_per_cell": step_b},
         "stages_ms": {k: v / nwarm for k, v in stages.items()},
+        "ensemble": ensemble,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

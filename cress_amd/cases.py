@@ -264,7 +264,7 @@ def _hydrometeors(X, Y, xc, yc, zs, tk, p, qv, ip, jp, seed, ring, cellnoise=Tru
 
 def build(name="bubble", nx=None, ny=None, nz=None, seed=None, cphopt=2, dtbig=None,
           dtsml=None, nsstp=None, advopt=3, moist=None, noise=None, divopt=1, wbc=None,
-          gwave=None, smtopt=0, smhcoe=0.0008333, smvcoe=0.0008333):
+          gwave=None, smtopt=0, smhcoe=0.0008333, smvcoe=0.0008333, amp=None, shear=None):
     """Return a Case with config, base state and the initial state.
 
     wbc: -1 periodic or 7 open (radiation at the constant phase speed gwave,
@@ -272,6 +272,8 @@ def build(name="bubble", nx=None, ny=None, nz=None, seed=None, cphopt=2, dtbig=N
     gwave=30, the other cases to periodic.
     smtopt: 0 none, 2/3 the 4th-order smoothing with smhcoe, smvcoe (1/s;
     defaults are the reference's documented values).
+    amp, shear override the case's bubble amplitude (K) and hodograph
+    shear U_s (m/s; cases with a hodograph only).
 
     moist (default: cphopt == 2) seeds the hydrometeor species and a
     supersaturated core so that the microphysics is active from step one.
@@ -312,6 +314,10 @@ def build(name="bubble", nx=None, ny=None, nz=None, seed=None, cphopt=2, dtbig=N
     else:
         raise ValueError(name)
     noise = noise0 if noise is None else float(noise)
+    if amp is not None:
+        amp = float(amp)
+    if shear is not None and name == "supercell":
+        shear = float(shear)
     ni, nj, nk = nx + 3, ny + 3, nz + 3
     if sthopt == 1:
         # keep the physical top of the configuration: ztop = nz*dz_comp
