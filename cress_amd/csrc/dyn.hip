@@ -414,25 +414,52 @@ __device__ float adv_s(const Grid &g, const Opts &o, const Consts &c, const floa
   return f + vadv;
 }
 
-// advs for one scalar at (i,j,k) from flux weights already in registers:
-// fu[m] = rstxu(i-1+m), fv[m] = rstxv(j-1+m), fw[m] = rstxwc(k-1+m), m=0..3.
-// Same terms and operation order as adv_s.
-__device__ __forceinline__ float adv_s_pre(const Grid &g, const Opts &o, const Consts &c, bool h4,
-                                           bool v4, const float *fu, const float *fv, const float *fw,
-                                           const float *s, uint32_t q) {
+// The stencil values of one scalar that advs reads at (i,j,k): the cell,
+// its neighbours at distance 1 in x, y, z, and at distance 2 where the
+// 4th-order terms apply (h4: x and y, v4: z).
+struct SStencil {
+  float s0, xm1, xp1, ym1, yp1, zm1, zp1, xm2, xp2, ym2, yp2, zm2, zp2;
+};
+
+__device__ __forceinline__ void adv_s_load(const Grid &g, const Opts &o, bool h4, bool v4, const float *s,
+                                           uint32_t q, SStencil &L) {
   const uint32_t sx = 4u * (uint32_t)g.nx, sk = 4u * (uint32_t)g.nxy;
-  const float s0 = LD(s, q);
-  const float sxm1 = LD(s, q - 4), sxp1 = LD(s, q + 4);
-  const float sym1 = LD(s, q - sx), syp1 = LD(s, q + sx);
-  const float szm1 = LD(s, q - sk), szp1 = LD(s, q + sk);
+  L.s0 = LD(s, q);
+  L.xm1 = LD(s, q - 4);
+  L.xp1 = LD(s, q + 4);
+  L.ym1 = LD(s, q - sx);
+  L.yp1 = LD(s, q + sx);
+  L.zm1 = LD(s, q - sk);
+  L.zp1 = LD(s, q + sk);
+  if (o.advopt != 1 && h4) {
+    L.xm2 = LD(s, q - 8);
+    L.xp2 = LD(s, q + 8);
+    L.ym2 = LD(s, q - 2 * sx);
+    L.yp2 = LD(s, q + 2 * sx);
+    if (o.advopt == 3 && v4) {
+      L.zm2 = LD(s, q - 2 * sk);
+      L.zp2 = LD(s, q + 2 * sk);
+    }
+  }
+}
+
+// advs for one scalar at (i,j,k) from flux weights and stencil values
+// already in registers: fu[m] = rstxu(i-1+m), fv[m] = rstxv(j-1+m),
+// fw[m] = rstxwc(k-1+m), m=0..3.  Same terms and operation order as adv_s.
+__device__ __forceinline__ float adv_s_val(const Opts &o, const Consts &c, bool h4, bool v4, const float *fu,
+                                           const float *fv, const float *fw, const SStencil &L) {
+  const float s0 = L.s0;
+  const float sxm1 = L.xm1, sxp1 = L.xp1;
+  const float sym1 = L.ym1, syp1 = L.yp1;
+  const float szm1 = L.zm1, szp1 = L.zp1;
   const float t1a = fu[1] * (s0 - sxm1) * c.dxv05n, t1b = fu[2] * (sxp1 - s0) * c.dxv05n;
   const float t2a = fv[1] * (s0 - sym1) * c.dyv05n, t2b = fv[2] * (syp1 - s0) * c.dyv05n;
   const float t3a = fw[1] * (s0 - szm1) * c.dzv05n, t3b = fw[2] * (szp1 - s0) * c.dzv05n;
   if (o.advopt == 1) return (t3a + t3b) + ((t1a + t1b) + (t2a + t2b));
   float f = (t1a + t1b) + (t2a + t2b);
   if (h4) {
-    const float sxm2 = LD(s, q - 8), sxp2 = LD(s, q + 8);
-    const float sym2 = LD(s, q - 2 * sx), syp2 = LD(s, q + 2 * sx);
+    const float sxm2 = L.xm2, sxp2 = L.xp2;
+    const float sym2 = L.ym2, syp2 = L.yp2;
     const float T1a = (fu[0] + fu[1]) * (s0 - sxm2) * c.dxv24;
     const float T1b = (fu[2] + fu[3]) * (sxp2 - s0) * c.dxv24;
     const float T2a = (fv[0] + fv[1]) * (s0 - sym2) * c.dyv24;
@@ -442,7 +469,7 @@ __device__ __forceinline__ float adv_s_pre(const Grid &g, const Opts &o, const C
   if (o.advopt == 2) return f + (t3a + t3b);
   float vadv = t3a + t3b;
   if (h4 && v4) {
-    const float szm2 = LD(s, q - 2 * sk), szp2 = LD(s, q + 2 * sk);
+    const float szm2 = L.zm2, szp2 = L.zp2;
     const float T3a = (fw[0] + fw[1]) * (s0 - szm2) * c.dzv24;
     const float T3b = (fw[2] + fw[3]) * (szp2 - s0) * c.dzv24;
     vadv = c.fourd3 * vadv + (T3a + T3b);
@@ -480,8 +507,14 @@ __global__ void __launch_bounds__(64 * CRESS_FS_BY) k_force_s(Tiles T, Grid g, O
     r = 0.5f * (rbct2(in.rbct, g, i, j, k) + rbct2(in.rbct, g, i, j, k + 1));
     rst = MQ(CRESS_RST, in.rst, q, k);
   }
+  // each scalar's stencil is loaded before the previous scalar's forcing
+  // is stored, so that the loads of consecutive scalars overlap
+  SStencil cur, nxt;
+  adv_s_load(g, o, h4, v4, in.s[0], q, cur);
   for (int sc = 0; sc < nscalar; sc++) {
-    float f = adv_s_pre(g, o, c, h4, v4, fu, fv, fw, in.s[sc], q);
+    if (sc + 1 < nscalar) adv_s_load(g, o, h4, v4, in.s[sc + 1], q, nxt);
+    float f = adv_s_val(o, c, h4, v4, fu, fv, fw, cur);
+    cur = nxt;
     if (SM) {
       const float *sp = in.sp[sc];
       f = smooth4(f, c, o.smtopt, h4, k == 2, k == g.nk - 2, [&](int di, int dj, int dk) {
