@@ -96,17 +96,23 @@ __global__ void k_prep(Tiles T, Grid g, Opts o, const float *u, const float *v, 
   const int j = xt.y * blockDim.y + threadIdx.y;
   const int k = 1 + xt.z;
   if (i > g.ni + 1 || j > g.nj + 1) return;
-  if (i >= 1 && i <= g.ni - 1 && j >= 1 && j <= g.nj - 1) {
-    const float c = wc_value(g, o, w, jcb8w, i, j, k);
+  // every value is formed before the first store, so no load waits on one
+  const bool dw = i >= 1 && i <= g.ni - 1 && j >= 1 && j <= g.nj - 1;
+  const bool du = k <= g.nk - 1 && j >= 1 && j <= g.nj - 1 && i >= o.iw && i <= g.ni + 1 - o.ie;
+  const bool dv = k <= g.nk - 1 && i >= 1 && i <= g.ni - 1 && j >= o.js && j <= g.nj + 1 - o.jn;
+  float c = 0.0f, xw = 0.0f, xu = 0.0f, xv = 0.0f;
+  if (dw) {
+    c = wc_value(g, o, w, jcb8w, i, j, k);
+    xw = MT(CRESS_RST8W, rst8w, i, j, k) * c;
+  }
+  if (du) xu = MT(CRESS_RST8U, rst8u, i, j, k) * AT(u, i, j, k);
+  if (dv) xv = MT(CRESS_RST8V, rst8v, i, j, k) * AT(v, i, j, k);
+  if (dw) {
     AT(wc, i, j, k) = c;
-    AT(rstxwc, i, j, k) = MT(CRESS_RST8W, rst8w, i, j, k) * c;
+    AT(rstxwc, i, j, k) = xw;
   }
-  if (k <= g.nk - 1) {
-    if (j >= 1 && j <= g.nj - 1 && i >= o.iw && i <= g.ni + 1 - o.ie)
-      AT(rstxu, i, j, k) = MT(CRESS_RST8U, rst8u, i, j, k) * AT(u, i, j, k);
-    if (i >= 1 && i <= g.ni - 1 && j >= o.js && j <= g.nj + 1 - o.jn)
-      AT(rstxv, i, j, k) = MT(CRESS_RST8V, rst8v, i, j, k) * AT(v, i, j, k);
-  }
+  if (du) AT(rstxu, i, j, k) = xu;
+  if (dv) AT(rstxv, i, j, k) = xv;
 }
 
 // ---------------------------------------------------------------------------
@@ -233,7 +239,11 @@ __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float 
   const int k = 2 + xt.z;
   if (i < 2 || i > g.ni - 1 || j > g.nj - 1) return;
   const bool sp = k >= o.ksp0 - 1 && k <= g.nk - 2;
-  if (j <= g.nj - 2) {
+  // both forcings are formed before either is stored, so that v's loads
+  // need not wait for u's store
+  const bool du = j <= g.nj - 2, dv = i <= g.ni - 2;
+  float uout = 0.0f, vout = 0.0f;
+  if (du) {
     float f = 0.0f;
     if (SM) {
       const bool band = j >= 2 + o.js && j <= g.nj - 2 - o.jn && i >= 2 + o.iw && i <= g.ni - 1 - o.ie;
@@ -253,9 +263,9 @@ __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float 
       };
       f = f + (cr(i - 1) + cr(i));
     }
-    AT(ufrc, i, j, k) = adv_u(f, g, o, c, in.u, in.rstxu, in.rstxv, in.rstxwc, i, j, k);
+    uout = adv_u(f, g, o, c, in.u, in.rstxu, in.rstxv, in.rstxwc, i, j, k);
   }
-  if (i <= g.ni - 2) {
+  if (dv) {
     float f = 0.0f;
     if (SM) {
       const bool band = j >= 2 + o.js && j <= g.nj - 1 - o.jn && i >= 2 + o.iw && i <= g.ni - 2 - o.ie;
@@ -275,8 +285,10 @@ __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float 
       };
       f = f + (cr(j - 1) + cr(j));
     }
-    AT(vfrc, i, j, k) = adv_v(f, g, o, c, in.v, in.rstxu, in.rstxv, in.rstxwc, i, j, k);
+    vout = adv_v(f, g, o, c, in.v, in.rstxu, in.rstxv, in.rstxwc, i, j, k);
   }
+  if (du) AT(ufrc, i, j, k) = uout;
+  if (dv) AT(vfrc, i, j, k) = vout;
 }
 
 // moist buoyancy at a scalar level, buoywb.f90:204-297 (gwmopt=0)

@@ -779,16 +779,19 @@ __global__ void __launch_bounds__(256) k_asselin_sat(Tiles T, Grid g, float m1fc
   if (i < 1 || i > g.ni || j > g.nj) return;
   const uint32_t q = 4u * (uint32_t)g.at(i, j, k);
   const bool in = i <= g.ni - 1 && j <= g.nj - 1 && k <= g.nk - 1;
+  // all 36 loads before the first store, so that none waits on a store
   float x[12], xf[12];
+  bool ok[12];
   for (int f = 0; f < 12; f++) {
-    const bool ok = in || (f == 0 && i == g.ni && j <= g.nj - 1 && k <= g.nk - 1) ||
-                    (f == 1 && j == g.nj && i <= g.ni - 1 && k <= g.nk - 1) ||
-                    (f == 2 && k == g.nk && i <= g.ni - 1 && j <= g.nj - 1);
-    if (!ok) continue;
+    ok[f] = in || (f == 0 && i == g.ni && j <= g.nj - 1 && k <= g.nk - 1) ||
+            (f == 1 && j == g.nj && i <= g.ni - 1 && k <= g.nk - 1) ||
+            (f == 2 && k == g.nk && i <= g.ni - 1 && j <= g.nj - 1);
+    if (!ok[f]) continue;
     xf[f] = LD(a.xf[f], q);
     x[f] = m1fc2 * LD(a.x[f], q) + filcoe * (xf[f] + LD(a.xp[f], q));
-    if (!in || f <= 3 || f == 7 || f == 9 || f == 10) ST(a.x[f], q, x[f]);
   }
+  for (int f = 0; f < 12; f++)
+    if (ok[f] && (!in || f <= 3 || f == 7 || f == 9 || f == 10)) ST(a.x[f], q, x[f]);
   if (!in) return;
   const float pbr = MQ(CRESS_PBR, a.pbr, q, k), tb = MQ(CRESS_PTBR, a.ptbr, q, k);
   satadj_cell(thresq, pbr, x[3], tb, x[4], x[5], x[6], x[8], x[11]);
