@@ -1229,6 +1229,67 @@ int cress_advs(const cress_config *cfg, const float *rstxu, const float *rstxv, 
   return CRESS_OK;
 }
 
+int cress_advuvw(const cress_config *cfg, const float *rstxu, const float *rstxv, const float *rstxwc,
+                 const float *u, const float *v, const float *w, float *ufrc, float *vfrc, float *wfrc) {
+  if (!cfg || !rstxu || !rstxv || !rstxwc || !u || !v || !w || !ufrc || !vfrc || !wfrc) return CRESS_EINVAL;
+  if (cfg->ni < 7 || cfg->nj < 7 || cfg->nk < 7 || cfg->advopt < 1 || cfg->advopt > 3) return CRESS_EINVAL;
+  const int sides[4] = {cfg->wbc, cfg->ebc, cfg->sbc, cfg->nbc};
+  for (int s = 0; s < 4; s++)
+    if (sides[s] != -1 && sides[s] != 7) return CRESS_EUNSUPPORTED;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CRESS_ENODEV;
+  const Grid g = make_grid(cfg->ni, cfg->nj, cfg->nk);
+  const size_t n = (size_t)g.n();
+  Consts c;
+  fold_consts(cfg, &c);
+  Opts o = {};
+  o.advopt = cfg->advopt;
+  // the 4th-order terms stop one point short of a non-periodic side
+  // (sethalo.f90:536-580)
+  o.iw = cfg->wbc == -1 ? 0 : 1;
+  o.ie = cfg->ebc == -1 ? 0 : 1;
+  o.js = cfg->sbc == -1 ? 0 : 1;
+  o.jn = cfg->nbc == -1 ? 0 : 1;
+  DevBuf a, b, d, e, f, h, pu, pv, pw;
+  int rc;
+  if ((rc = to_dev(a, rstxu, n)) || (rc = to_dev(b, rstxv, n)) || (rc = to_dev(d, rstxwc, n)) ||
+      (rc = to_dev(e, u, n)) || (rc = to_dev(f, v, n)) || (rc = to_dev(h, w, n)) || (rc = to_dev(pu, ufrc, n)) ||
+      (rc = to_dev(pv, vfrc, n)) || (rc = to_dev(pw, wfrc, n)))
+    return rc;
+  hipLaunchKernelGGL(k_advuvw_only, grid2(g.ni - 2, g.nj - 2, 64, 4, g.nk - 2), dim3(64, 4), 0, 0, g, o, c,
+                     (const float *)e.p, (const float *)f.p, (const float *)h.p, (const float *)a.p,
+                     (const float *)b.p, (const float *)d.p, (float *)pu.p, (float *)pv.p, (float *)pw.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(ufrc, pu.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(vfrc, pv.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(wfrc, pw.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return CRESS_OK;
+}
+
+// The three staggered-point boundary fills share one kernel; kind 0/1/2 =
+// bc8u/bc8v/bc8w with the sides' options lo, hi.
+static int bc8(const cress_config *cfg, int kind, int lo, int hi, int kmax, float *var) {
+  if (!cfg || !var || cfg->ni < 4 || cfg->nj < 4 || cfg->nk < 4) return CRESS_EINVAL;
+  if (kind < 2 && (kmax < 1 || kmax > cfg->nk)) return CRESS_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CRESS_ENODEV;
+  const Grid g = make_grid(cfg->ni, cfg->nj, cfg->nk);
+  const size_t n = (size_t)(g.ni + 2) * (g.nj + 2) * (size_t)(kind == 2 ? g.nk : kmax);
+  DevBuf a;
+  int rc;
+  if ((rc = to_dev(a, var, n))) return rc;
+  const long pts = kind == 2 ? (long)(g.ni + 2) * (g.nj + 2) : (long)(kind == 0 ? g.nj + 2 : g.ni + 2) * kmax;
+  hipLaunchKernelGGL(k_bc8, dim3((unsigned)((pts + 63) / 64), 2), dim3(64), 0, 0, g, kind, lo, hi, kmax,
+                     (float *)a.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(var, a.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return CRESS_OK;
+}
+
+int cress_bc8u(const cress_config *cfg, int kmax, float *var8u) { return cfg ? bc8(cfg, 0, cfg->wbc, cfg->ebc, kmax, var8u) : CRESS_EINVAL; }
+int cress_bc8v(const cress_config *cfg, int kmax, float *var8v) { return cfg ? bc8(cfg, 1, cfg->sbc, cfg->nbc, kmax, var8v) : CRESS_EINVAL; }
+int cress_bc8w(const cress_config *cfg, float *var8w) { return cfg ? bc8(cfg, 2, cfg->bbc, cfg->tbc, cfg->nk, var8w) : CRESS_EINVAL; }
+
 int cress_bcycle(const cress_config *cfg, int iwsnd, int iwrcv, int iesnd, int iercv, int jssnd, int jsrcv,
                  int jnsnd, int jnrcv, int kmax, float *var) {
   if (!cfg || !var || kmax < 1 || kmax > cfg->nk) return CRESS_EINVAL;

@@ -138,4 +138,8 @@ namespace cress {
 __global__ void k_gaussel(Grid g, int istr, int iend, int jstr, int jend, int kstr, int kend,
                           const float *rr, float *ss, const float *tt, float *ff);
 __global__ void k_advs_only(Grid g, Opts o, Consts c, ScalarIn in);
+__global__ void k_advuvw_only(Grid g, Opts o, Consts c, const float *u, const float *v, const float *w,
+                              const float *rxu, const float *rxv, const float *rxw, float *ufrc, float *vfrc,
+                              float *wfrc);
+__global__ void k_bc8(Grid g, int kind, int lo, int hi, int kmax, float *var);
 }  // namespace cress

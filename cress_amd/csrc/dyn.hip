@@ -997,6 +997,50 @@ __global__ void k_advs_only(Grid g, Opts o, Consts c, ScalarIn in) {
   AT(in.frc[0], i, j, k) = adv_s(g, o, c, in.s[0], in.rstxu, in.rstxv, in.rstxwc, i, j, k);
 }
 
+// s_advuvw alone (advuvw.f90:75-912): ufrc, vfrc, wfrc plus the advection
+// of u (i=2..ni-1, j=2..nj-2, k=2..nk-2), v (i=2..ni-2, j=2..nj-1) and w
+// (i=2..ni-2, j=2..nj-2, k=2..nk-1), the terms and order of the forcing
+// kernels.
+__global__ void k_advuvw_only(Grid g, Opts o, Consts c, const float *u, const float *v, const float *w,
+                              const float *rxu, const float *rxv, const float *rxw, float *ufrc, float *vfrc,
+                              float *wfrc) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 2 + blockIdx.z;
+  if (i > g.ni - 1 || j > g.nj - 1) return;
+  if (k <= g.nk - 2 && j <= g.nj - 2) AT(ufrc, i, j, k) = adv_u(AT(ufrc, i, j, k), g, o, c, u, rxu, rxv, rxw, i, j, k);
+  if (k <= g.nk - 2 && i <= g.ni - 2) AT(vfrc, i, j, k) = adv_v(AT(vfrc, i, j, k), g, o, c, v, rxu, rxv, rxw, i, j, k);
+  if (i <= g.ni - 2 && j <= g.nj - 2) AT(wfrc, i, j, k) = adv_w(AT(wfrc, i, j, k), g, o, c, w, rxu, rxv, rxw, i, j, k);
+}
+
+// s_bc8u, s_bc8v (bc8u.f90:132-202, bc8v.f90), s_bc8w (bc8w.f90:130-184) on
+// one rank, every side an outer boundary: the staggered point on the
+// boundary takes the value two points in (rigid, option 2) or one point in
+// (options >= 3); periodic sides (-1) are left alone.  kind 0 x (u points,
+// planes i=1 and ni), 1 y (v points), 2 z (w points, levels 1 and nk).
+// One thread per point of the face pair.
+__global__ void k_bc8(Grid g, int kind, int lo, int hi, int kmax, float *var) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = blockIdx.y;
+  if (kind == 2) {
+    const int i = t % (g.ni + 2), j = t / (g.ni + 2);
+    if (j > g.nj + 1) return;
+    if (m == 0 && lo >= 2) AT(var, i, j, 1) = AT(var, i, j, lo == 2 ? 3 : 2);
+    if (m == 1 && hi >= 2) AT(var, i, j, g.nk) = AT(var, i, j, hi == 2 ? g.nk - 2 : g.nk - 1);
+    return;
+  }
+  const int n = kind == 0 ? g.nj + 2 : g.ni + 2;
+  const int a = t % n, k = 1 + t / n;
+  if (k > kmax) return;
+  if (kind == 0) {
+    if (m == 0 && lo >= 2) AT(var, 1, a, k) = AT(var, lo == 2 ? 3 : 2, a, k);
+    if (m == 1 && hi >= 2) AT(var, g.ni, a, k) = AT(var, hi == 2 ? g.ni - 2 : g.ni - 1, a, k);
+  } else {
+    if (m == 0 && lo >= 2) AT(var, a, 1, k) = AT(var, a, lo == 2 ? 3 : 2, k);
+    if (m == 1 && hi >= 2) AT(var, a, g.nj, k) = AT(var, a, hi == 2 ? g.nj - 2 : g.nj - 1, k);
+  }
+}
+
 // The forcing kernels with and without the smoothing compiled in.
 template __global__ void k_force_uv<false>(Tiles, Grid, Opts, Consts, ForceIn, float *, float *);
 template __global__ void k_force_uv<true>(Tiles, Grid, Opts, Consts, ForceIn, float *, float *);

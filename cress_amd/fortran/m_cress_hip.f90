@@ -57,8 +57,9 @@ module m_cress_hip
   public :: cress_ctx_create, cress_ctx_destroy, cress_field_size, cress_field_pitch
   public :: cress_upload, cress_download, cress_large_step, cress_sync
   public :: cress_set_timing, cress_stage_times
-  public :: cress_gaussel, cress_advs, cress_bcycle
-  public :: cress_errmsg
+  public :: cress_gaussel, cress_advs, cress_bcycle, cress_advuvw
+  public :: cress_bc8u, cress_bc8v, cress_bc8w
+  public :: cress_errmsg, base_cfg
 
   interface
 
@@ -173,6 +174,41 @@ module m_cress_hip
       integer(c_int) :: cress_advs
     end function cress_advs
 
+! tier 2: s_advuvw (advuvw.f90:75-78)
+    function cress_advuvw(cfg, rstxu, rstxv, rstxwc, u, v, w, ufrc, vfrc, &
+     &  wfrc) bind(c, name='cress_advuvw')
+      import :: c_int, c_float, cress_config
+      type(cress_config), intent(in) :: cfg
+      real(c_float), intent(in) :: rstxu(*), rstxv(*), rstxwc(*)
+      real(c_float), intent(in) :: u(*), v(*), w(*)
+      real(c_float), intent(inout) :: ufrc(*), vfrc(*), wfrc(*)
+      integer(c_int) :: cress_advuvw
+    end function cress_advuvw
+
+! tier 2: s_bc8u, s_bc8v (bc8u.f90:72, bc8v.f90:72), s_bc8w (bc8w.f90:70)
+    function cress_bc8u(cfg, kmax, var) bind(c, name='cress_bc8u')
+      import :: c_int, c_float, cress_config
+      type(cress_config), intent(in) :: cfg
+      integer(c_int), value :: kmax
+      real(c_float), intent(inout) :: var(*)
+      integer(c_int) :: cress_bc8u
+    end function cress_bc8u
+
+    function cress_bc8v(cfg, kmax, var) bind(c, name='cress_bc8v')
+      import :: c_int, c_float, cress_config
+      type(cress_config), intent(in) :: cfg
+      integer(c_int), value :: kmax
+      real(c_float), intent(inout) :: var(*)
+      integer(c_int) :: cress_bc8v
+    end function cress_bc8v
+
+    function cress_bc8w(cfg, var) bind(c, name='cress_bc8w')
+      import :: c_int, c_float, cress_config
+      type(cress_config), intent(in) :: cfg
+      real(c_float), intent(inout) :: var(*)
+      integer(c_int) :: cress_bc8w
+    end function cress_bc8w
+
 ! tier 2: s_bcycle (bcycle.f90:72-74)
     function cress_bcycle(cfg, iwsnd, iwrcv, iesnd, iercv,              &
      &  jssnd, jsrcv, jnsnd, jnrcv, kmax, var) bind(c, name='cress_bcycle')
@@ -187,6 +223,51 @@ module m_cress_hip
   end interface
 
 contains
+
+! A configuration for the per-routine shims: the given dimensions,
+! advection order and grid spacings, periodic sides, rigid top and bottom,
+! the plain-path defaults elsewhere (the shims read only what their
+! routine uses).
+  subroutine base_cfg(cfg, ni, nj, nk, advopt, dx, dy, dz)
+    type(cress_config), intent(out) :: cfg
+    integer, intent(in) :: ni, nj, nk, advopt
+    real, intent(in) :: dx, dy, dz
+    cfg%ni = ni
+    cfg%nj = nj
+    cfg%nk = nk
+    cfg%wbc = -1
+    cfg%ebc = -1
+    cfg%sbc = -1
+    cfg%nbc = -1
+    cfg%bbc = 2
+    cfg%tbc = 2
+    cfg%advopt = advopt
+    cfg%impopt = 1
+    cfg%gwmopt = 0
+    cfg%buyopt = 1
+    cfg%divopt = 1
+    cfg%cphopt = 0
+    cfg%haiopt = 0
+    cfg%vspopt = 0
+    cfg%coropt = 0
+    cfg%sthopt = 0
+    cfg%ksp0(1) = 2
+    cfg%ksp0(2) = 2
+    cfg%vspvar(1:10) = 'x'
+    cfg%nsstp = 1
+    cfg%dx = dx
+    cfg%dy = dy
+    cfg%dz = dz
+    cfg%dtbig = 1.d0
+    cfg%dtsml = 1.d0
+    cfg%weicoe = .6d0
+    cfg%filcoe = .1d0
+    cfg%thresq = 1.d-8
+    cfg%gwave = 0.d0
+    cfg%smtopt = 0
+    cfg%smhcoe = 0.d0
+    cfg%smvcoe = 0.d0
+  end subroutine base_cfg
 
 ! The message of a return code as a Fortran string.
   function cress_errmsg(code) result(msg)

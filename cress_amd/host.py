@@ -77,6 +77,10 @@ def lib():
         L.cress_gaussel.argtypes = [I] * 9 + [P] * 4
         L.cress_advs.argtypes = [C.POINTER(Config)] + [P] * 5
         L.cress_bcycle.argtypes = [C.POINTER(Config)] + [I] * 9 + [P]
+        L.cress_advuvw.argtypes = [C.POINTER(Config)] + [P] * 9
+        L.cress_bc8u.argtypes = [C.POINTER(Config), I, P]
+        L.cress_bc8v.argtypes = [C.POINTER(Config), I, P]
+        L.cress_bc8w.argtypes = [C.POINTER(Config), P]
         L.cress_ens_create.argtypes = [C.POINTER(P), I, C.POINTER(P)]
         L.cress_ens_step.argtypes = [P, I, I]
         L.cress_ens_sync.argtypes = [P]

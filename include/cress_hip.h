@@ -168,6 +168,23 @@ int cress_gaussel(int istr, int iend, int jstr, int jend, int kstr, int kend,
 int cress_advs(const cress_config *cfg, const float *rstxu, const float *rstxv,
                const float *rstxwc, const float *s, float *sfrc);
 
+/* s_advuvw(fpadvopt,fpiwest,...,fpdziv,ni,nj,nk,rstxu,rstxv,rstxwc,u,v,w,
+ *          ufrc,vfrc,wfrc,hadv,vadv,tmp1,tmp2,tmp3)  SrcOrig/advuvw.f90:75-78.
+ * Adds the advection of u, v, w to ufrc, vfrc, wfrc over the reference's
+ * ranges; the lateral sides are periodic (-1) or open (7), which decides
+ * where the 4th-order terms stop (iwest..jnorth). */
+int cress_advuvw(const cress_config *cfg, const float *rstxu, const float *rstxv, const float *rstxwc,
+                 const float *u, const float *v, const float *w, float *ufrc, float *vfrc, float *wfrc);
+
+/* s_bc8u(fpwbc,fpebc,ni,nj,kmax,var8u)  SrcOrig/bc8u.f90:72,
+ * s_bc8v(fpsbc,fpnbc,ni,nj,kmax,var8v)  SrcOrig/bc8v.f90:72,
+ * s_bc8w(fpbbc,fptbc,ni,nj,nk,var8w)    SrcOrig/bc8w.f90:70:
+ * the staggered-point fills at the outer boundaries of one rank, in place
+ * on real(0:ni+1,0:nj+1,1:kmax) (1:nk for bc8w). */
+int cress_bc8u(const cress_config *cfg, int kmax, float *var8u);
+int cress_bc8v(const cress_config *cfg, int kmax, float *var8v);
+int cress_bc8w(const cress_config *cfg, float *var8w);
+
 /* s_bcycle(fpwbc,...,iwsnd,iwrcv,iesnd,iercv,jssnd,jsrcv,jnsnd,jnrcv,ni,nj,kmax,var)
  * SrcOrig/bcycle.f90:72-74: the one-rank periodic halo fill. */
 int cress_bcycle(const cress_config *cfg, int iwsnd, int iwrcv, int iesnd, int iercv,

@@ -78,6 +78,55 @@ int or_real_size(void) { return (int)sizeof(R); }
 /* Boundary / halo fills                                                    */
 /* ------------------------------------------------------------------------ */
 
+/* s_bc8u (bc8u.f90:132-208), s_bc8v (bc8v.f90:132-208), s_bc8w
+ * (bc8w.f90:130-188) on one rank whose sides are all outer boundaries
+ * (ebw=ebe=ebs=ebn=1): kind 0 u points, x sides over j=0..nj+1,
+ * k=1..kmax, west then east; kind 1 v points, y sides over i=0..ni+1;
+ * kind 2 w points, bottom then top over i=0..ni+1, j=0..nj+1 (kmax = nk).
+ * Option 2 copies from two points in, options >= 3 from one point in. */
+void or_bc8(const or_cfg *c, int kind, int kmax, R *var)
+{
+  const int ni = c->ni, nj = c->nj, nk = c->nk;
+  const int lo = kind == 0 ? c->wbc : kind == 1 ? c->sbc : c->bbc;
+  const int hi = kind == 0 ? c->ebc : kind == 1 ? c->nbc : c->tbc;
+  const int sx = ni + 2, sxy = (ni + 2) * (nj + 2);
+#define V8(i, j, k) var[(size_t)(i) + (size_t)sx * (size_t)(j) + (size_t)sxy * (size_t)((k) - 1)]
+  if (kind == 2) {
+    for (int j = 0; j <= nj + 1; j++)
+      for (int i = 0; i <= ni + 1; i++) {
+        if (lo == 2) V8(i, j, 1) = V8(i, j, 3);
+        else if (lo >= 3) V8(i, j, 1) = V8(i, j, 2);
+      }
+    for (int j = 0; j <= nj + 1; j++)
+      for (int i = 0; i <= ni + 1; i++) {
+        if (hi == 2) V8(i, j, nk) = V8(i, j, nk - 2);
+        else if (hi >= 3) V8(i, j, nk) = V8(i, j, nk - 1);
+      }
+    return;
+  }
+  for (int k = 1; k <= kmax; k++)
+    for (int a = 0; a <= (kind == 0 ? nj : ni) + 1; a++) {
+      if (kind == 0) {
+        if (lo == 2) V8(1, a, k) = V8(3, a, k);
+        else if (lo >= 3) V8(1, a, k) = V8(2, a, k);
+      } else {
+        if (lo == 2) V8(a, 1, k) = V8(a, 3, k);
+        else if (lo >= 3) V8(a, 1, k) = V8(a, 2, k);
+      }
+    }
+  for (int k = 1; k <= kmax; k++)
+    for (int a = 0; a <= (kind == 0 ? nj : ni) + 1; a++) {
+      if (kind == 0) {
+        if (hi == 2) V8(ni, a, k) = V8(ni - 2, a, k);
+        else if (hi >= 3) V8(ni, a, k) = V8(ni - 1, a, k);
+      } else {
+        if (hi == 2) V8(a, nj, k) = V8(a, nj - 2, k);
+        else if (hi >= 3) V8(a, nj, k) = V8(a, nj - 1, k);
+      }
+    }
+#undef V8
+}
+
 /* s_bcycle, bcycle.f90:159-225.  Single rank: periodic copy in x when
  * wbc=ebc=-1 over j=0..nj+1, then in y when sbc=nbc=-1 over i=0..ni+1. */
 void or_bcycle(const or_cfg *c, int iwsnd, int iwrcv, int iesnd, int iercv,

@@ -65,6 +65,9 @@
       use m_rbcq
       use m_rbcs
       use m_bc4news
+      use m_bc8u
+      use m_bc8v
+      use m_bc8w
       use m_lbcu
       use m_lbcv
       use m_lbcw
@@ -618,6 +621,20 @@
       call s_bc4news(idwbc,idebc,idsbc,idnbc,isw,ise,jss,jsn,ni,nj,kmax, &
      &               var)
       end subroutine ref_bc4news
+
+! kind: 0 bc8u, 1 bc8v, 2 bc8w (kmax = nk)
+      subroutine ref_bc8(kind,ni,nj,kmax,var) bind(C,name='ref_bc8')
+      integer(c_int), value :: kind,ni,nj,kmax
+      real var(*)
+      select case(kind)
+      case(0)
+        call s_bc8u(idwbc,idebc,ni,nj,kmax,var)
+      case(1)
+        call s_bc8v(idsbc,idnbc,ni,nj,kmax,var)
+      case default
+        call s_bc8w(idbbc,idtbc,ni,nj,kmax,var)
+      end select
+      end subroutine ref_bc8
 
 ! kind: 1 lbcu, 2 lbcv, 3 lbcw, 4 lbcs, 5 lbcwc
       subroutine ref_lbc(kind,ni,nj,nk,var) bind(C,name='ref_lbc')

@@ -5,9 +5,11 @@ CPU: the replacement modules (cress_amd/fortran/dropin/) compile against
 the reference's own module files (the ones oracle/_ref builds from the
 reference sources), i.e. they fit the reference tree with the same module,
 generic and dummy names.
-GPU: cress_gaussel, cress_advs and cress_bcycle give bitwise the oracle's
-s_gaussel, s_advs and s_bcycle results on random arrays (the oracle being
-pinned bitwise against the reference's routines in test_oracle_ref.py).
+GPU: cress_gaussel, cress_advs, cress_advuvw, cress_bcycle and
+cress_bc8u/v/w give bitwise the oracle's s_gaussel, s_advs, s_advuvw,
+s_bcycle and s_bc8u/v/w results on random arrays (the oracle being pinned
+bitwise against the reference's routines in test_oracle_ref.py and
+test_oracle_obc.py).
 """
 import ctypes as C
 import os
@@ -28,7 +30,7 @@ NI, NJ, NK = 13, 11, 14
 
 @pytest.mark.skipif(not os.path.isdir(REFMOD) or not os.path.exists(hb.FLANG),
                     reason="reference modules or flang absent")
-@pytest.mark.parametrize("name", ["gaussel", "bcycle", "advs"])
+@pytest.mark.parametrize("name", ["gaussel", "bcycle", "advs", "advuvw", "bc8u", "bc8v", "bc8w"])
 def test_dropin_module_compiles_in_reference_tree(name, tmp_path):
     moddir = os.path.join(hb.LIBDIR, "mod")
     if not os.path.exists(os.path.join(moddir, "m_cress_hip.mod")):
@@ -91,3 +93,44 @@ def test_tier2_bcycle_matches_oracle():
         host.check(host.lib().cress_bcycle(C.byref(cfg), *args, NK, _ptr(a)), "cress_bcycle")
         ora.or_bcycle(c, *args, NK, b)
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("wbc", [-1, 7])
+@pytest.mark.parametrize("advopt", [1, 2, 3])
+def test_tier2_advuvw_matches_oracle(advopt, wbc):
+    rng = np.random.default_rng(40 + advopt + (wbc > 0))
+    rx, ry, rz, u, v, w = (_rnd(rng) for _ in range(6))
+    cfg = host.make_config(NI, NJ, NK, 700.0, 900.0, 350.0, 4.0, 1.0, advopt=advopt, cphopt=0, sthopt=0,
+                           wbc=wbc, gwave=30.0)
+    g = [_rnd(rng) for _ in range(3)]
+    o = [a.copy(order="F") for a in g]
+    host.check(host.lib().cress_advuvw(C.byref(cfg), *[_ptr(a) for a in (rx, ry, rz, u, v, w)],
+                                       *[_ptr(a) for a in g]), "cress_advuvw")
+    c = po.make_cfg(NI, NJ, NK, dx=700.0, dy=900.0, dz=350.0, advopt=advopt, wbc=wbc, gwave=30.0)
+    po.Oracle(32).or_advuvw(c, rx, ry, rz, u, v, w, *o, *[_rnd(rng) for _ in range(5)])
+    for a, b, n in zip(g, o, "uvw"):
+        np.testing.assert_array_equal(a, b, err_msg=n)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("opts", [(2, 3, 3, 2, 2, 3), (7, 7, 7, 7, 3, 2), (-1, -1, -1, -1, 2, 2)])
+def test_tier2_bc8_matches_oracle(opts):
+    rng = np.random.default_rng(50)
+    cfg = host.make_config(NI, NJ, NK, 700.0, 900.0, 350.0, 4.0, 1.0, cphopt=0)
+    c = po.make_cfg(NI, NJ, NK, dx=700.0, dy=900.0, dz=350.0)
+    for x in (cfg, c):
+        x.wbc, x.ebc, x.sbc, x.nbc, x.bbc, x.tbc = opts
+    ora = po.Oracle(32)
+    L = host.lib()
+    for kind, kmax in ((0, NK), (0, NK - 3), (1, NK), (1, 2), (2, NK)):
+        a = _rnd(rng)
+        b = a.copy(order="F")
+        if kind == 0:
+            host.check(L.cress_bc8u(C.byref(cfg), kmax, _ptr(a)), "cress_bc8u")
+        elif kind == 1:
+            host.check(L.cress_bc8v(C.byref(cfg), kmax, _ptr(a)), "cress_bc8v")
+        else:
+            host.check(L.cress_bc8w(C.byref(cfg), _ptr(a)), "cress_bc8w")
+        ora.or_bc8(c, kind, kmax, b)
+        np.testing.assert_array_equal(a, b, err_msg="kind %d kmax %d" % (kind, kmax))

@@ -154,3 +154,49 @@ def test_periodic_domain_has_no_radiation(libs):
     o.or_rbclf(c, 1, cx, cy, fld(rng), fld(rng), a)
     o.or_bc4news(c, 1, NI, 1, NJ - 1, NK, a)
     np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("opts", [(2, 3, 3, 2, 2, 3), (7, 7, 7, 7, 3, 2), (-1, -1, -1, -1, 2, 2),
+                                  (3, 2, 2, 3, 3, 3)])
+def test_bc8(libs, opts):
+    """s_bc8u/v/w (the staggered-point boundary fills): the oracle's or_bc8
+    bitwise equal to the reference's routines; and what the options fix:
+    a rigid side (2) copies the point two in, options >= 3 the point one in,
+    periodic sides (-1) are untouched."""
+    o, r = libs
+    c = po.make_cfg(NI, NJ, NK, dx=700.0, dy=900.0, dz=350.0)
+    c.wbc, c.ebc, c.sbc, c.nbc, c.bbc, c.tbc = opts
+    r.configure(c)
+    rng = np.random.default_rng(sum(abs(x) for x in opts))
+    for kind, kmax in ((0, NK), (0, NK - 3), (1, NK), (1, 2), (2, NK)):
+        a = np.asfortranarray(rng.uniform(-1.0, 1.0, (NI + 2, NJ + 2, NK)).astype(np.float32))
+        b = a.copy(order="F")
+        a0 = a.copy(order="F")
+        o.or_bc8(c, kind, kmax, a)
+        r.ref_bc8(kind, NI, NJ, kmax, b)
+        np.testing.assert_array_equal(a, b, err_msg="kind %d kmax %d" % (kind, kmax))
+        lo, hi = (opts[0], opts[1]) if kind == 0 else (opts[2], opts[3]) if kind == 1 else (opts[4], opts[5])
+        ax = kind
+        n = (NI, NJ, NK)[ax]
+        sl = lambda idx: tuple(idx if d == ax else slice(None) for d in range(3))
+        kk = slice(0, kmax) if kind < 2 else slice(None)
+        def plane(arr, idx):
+            p = arr[sl(idx)]
+            return p[..., kk] if kind < 2 else p
+        # Fortran index m on axis ax is array index m (x, y) or m-1 (z)
+        off = 0 if kind < 2 else -1
+        if lo == 2:
+            np.testing.assert_array_equal(plane(a, 1 + off), plane(a0, 3 + off))
+        elif lo >= 3:
+            np.testing.assert_array_equal(plane(a, 1 + off), plane(a0, 2 + off))
+        else:
+            np.testing.assert_array_equal(plane(a, 1 + off), plane(a0, 1 + off))
+        if hi == 2:
+            np.testing.assert_array_equal(plane(a, n + off), plane(a0, n - 2 + off))
+        elif hi >= 3:
+            np.testing.assert_array_equal(plane(a, n + off), plane(a0, n - 1 + off))
+        # nothing else changes
+        mask = np.ones(a.shape, bool)
+        mask[sl(1 + off)] = False
+        mask[sl(n + off)] = False
+        np.testing.assert_array_equal(a[mask], a0[mask])
