@@ -1266,6 +1266,43 @@ int cress_advuvw(const cress_config *cfg, const float *rstxu, const float *rstxv
   return CRESS_OK;
 }
 
+// s_advp alone is the pressure forcing kernel without sponge or smoothing
+// (its forcing is the advection term only, written over the interior).
+int cress_advp(const cress_config *cfg, const float *jcb8u, const float *jcb8v, const float *jcb8w,
+               const float *u, const float *v, const float *wc, const float *pp, float *pfrc) {
+  if (!cfg || !jcb8u || !jcb8v || !jcb8w || !u || !v || !wc || !pp || !pfrc) return CRESS_EINVAL;
+  if (cfg->ni < 7 || cfg->nj < 7 || cfg->nk < 7 || cfg->advopt < 1 || cfg->advopt > 3) return CRESS_EINVAL;
+  const int sides[4] = {cfg->wbc, cfg->ebc, cfg->sbc, cfg->nbc};
+  for (int s = 0; s < 4; s++)
+    if (sides[s] != -1 && sides[s] != 7) return CRESS_EUNSUPPORTED;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CRESS_ENODEV;
+  const Grid g = make_grid(cfg->ni, cfg->nj, cfg->nk);
+  const size_t n = (size_t)g.n();
+  Consts c;
+  fold_consts(cfg, &c);
+  Opts o = {};
+  o.advopt = cfg->advopt;
+  o.iw = cfg->wbc == -1 ? 0 : 1;
+  o.ie = cfg->ebc == -1 ? 0 : 1;
+  o.js = cfg->sbc == -1 ? 0 : 1;
+  o.jn = cfg->nbc == -1 ? 0 : 1;
+  DevBuf a, b, d, e, f, h, p, q;
+  int rc;
+  if ((rc = to_dev(a, jcb8u, n)) || (rc = to_dev(b, jcb8v, n)) || (rc = to_dev(d, jcb8w, n)) ||
+      (rc = to_dev(e, u, n)) || (rc = to_dev(f, v, n)) || (rc = to_dev(h, wc, n)) || (rc = to_dev(p, pp, n)) ||
+      (rc = to_dev(q, pfrc, n)))
+    return rc;
+  const Tiles tp = tiles(g.ni - 1, g.nj - 3, 64, CRESS_FF_BY, g.nk - 3, CRESS_FF_ORDER);
+  const float *pj = (const float *)a.p, *pp_ = (const float *)p.p;
+  hipLaunchKernelGGL(k_force_p<false>, flat(tp), dim3(64, CRESS_FF_BY), 0, 0, tp, g, o, c, pj, pj,
+                     (const float *)b.p, (const float *)d.p, (const float *)e.p, (const float *)f.p,
+                     (const float *)h.p, pp_, pp_, pj, (float *)q.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(pfrc, q.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return CRESS_OK;
+}
+
 // The three staggered-point boundary fills share one kernel; kind 0/1/2 =
 // bc8u/bc8v/bc8w with the sides' options lo, hi.
 static int bc8(const cress_config *cfg, int kind, int lo, int hi, int kmax, float *var) {

@@ -57,7 +57,7 @@ module m_cress_hip
   public :: cress_ctx_create, cress_ctx_destroy, cress_field_size, cress_field_pitch
   public :: cress_upload, cress_download, cress_large_step, cress_sync
   public :: cress_set_timing, cress_stage_times
-  public :: cress_gaussel, cress_advs, cress_bcycle, cress_advuvw
+  public :: cress_gaussel, cress_advs, cress_bcycle, cress_advuvw, cress_advp
   public :: cress_bc8u, cress_bc8v, cress_bc8w
   public :: cress_errmsg, base_cfg
 
@@ -184,6 +184,17 @@ module m_cress_hip
       real(c_float), intent(inout) :: ufrc(*), vfrc(*), wfrc(*)
       integer(c_int) :: cress_advuvw
     end function cress_advuvw
+
+! tier 2: s_advp (advp.f90:75-79), mfcopt=0, diaopt=0
+    function cress_advp(cfg, jcb8u, jcb8v, jcb8w, u, v, wc, pp, pfrc)    &
+     &  bind(c, name='cress_advp')
+      import :: c_int, c_float, cress_config
+      type(cress_config), intent(in) :: cfg
+      real(c_float), intent(in) :: jcb8u(*), jcb8v(*), jcb8w(*)
+      real(c_float), intent(in) :: u(*), v(*), wc(*), pp(*)
+      real(c_float), intent(inout) :: pfrc(*)
+      integer(c_int) :: cress_advp
+    end function cress_advp
 
 ! tier 2: s_bc8u, s_bc8v (bc8u.f90:72, bc8v.f90:72), s_bc8w (bc8w.f90:70)
     function cress_bc8u(cfg, kmax, var) bind(c, name='cress_bc8u')

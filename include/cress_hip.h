@@ -176,6 +176,14 @@ int cress_advs(const cress_config *cfg, const float *rstxu, const float *rstxv,
 int cress_advuvw(const cress_config *cfg, const float *rstxu, const float *rstxv, const float *rstxwc,
                  const float *u, const float *v, const float *w, float *ufrc, float *vfrc, float *wfrc);
 
+/* s_advp(fpadvopt,fpmpopt,fpmfcopt,fpdiaopt,fpiwest,...,fpdziv,ni,nj,nk,
+ *        mf8u,mf8v,jcb8u,jcb8v,jcb8w,u,v,wc,pp,pfrc,jcbxu,jcbxv,jcbxwc,
+ *        hadv,vadv,tmp1,tmp2,tmp3)  SrcOrig/advp.f90:75-79, mfcopt=0 and
+ * diaopt=0 (the map factors are not read): writes the advection of p'
+ * into pfrc over the interior. */
+int cress_advp(const cress_config *cfg, const float *jcb8u, const float *jcb8v, const float *jcb8w,
+               const float *u, const float *v, const float *wc, const float *pp, float *pfrc);
+
 /* s_bc8u(fpwbc,fpebc,ni,nj,kmax,var8u)  SrcOrig/bc8u.f90:72,
  * s_bc8v(fpsbc,fpnbc,ni,nj,kmax,var8v)  SrcOrig/bc8v.f90:72,
  * s_bc8w(fpbbc,fptbc,ni,nj,nk,var8w)    SrcOrig/bc8w.f90:70:

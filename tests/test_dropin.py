@@ -5,9 +5,9 @@ CPU: the replacement modules (cress_amd/fortran/dropin/) compile against
 the reference's own module files (the ones oracle/_ref builds from the
 reference sources), i.e. they fit the reference tree with the same module,
 generic and dummy names.
-GPU: cress_gaussel, cress_advs, cress_advuvw, cress_bcycle and
-cress_bc8u/v/w give bitwise the oracle's s_gaussel, s_advs, s_advuvw,
-s_bcycle and s_bc8u/v/w results on random arrays (the oracle being pinned
+GPU: cress_gaussel, cress_advs, cress_advuvw, cress_advp, cress_bcycle
+and cress_bc8u/v/w give bitwise the oracle's s_gaussel, s_advs, s_advuvw,
+s_advp, s_bcycle and s_bc8u/v/w results on random arrays (the oracle being pinned
 bitwise against the reference's routines in test_oracle_ref.py and
 test_oracle_obc.py).
 """
@@ -30,7 +30,7 @@ NI, NJ, NK = 13, 11, 14
 
 @pytest.mark.skipif(not os.path.isdir(REFMOD) or not os.path.exists(hb.FLANG),
                     reason="reference modules or flang absent")
-@pytest.mark.parametrize("name", ["gaussel", "bcycle", "advs", "advuvw", "bc8u", "bc8v", "bc8w"])
+@pytest.mark.parametrize("name", ["gaussel", "bcycle", "advs", "advuvw", "advp", "bc8u", "bc8v", "bc8w"])
 def test_dropin_module_compiles_in_reference_tree(name, tmp_path):
     moddir = os.path.join(hb.LIBDIR, "mod")
     if not os.path.exists(os.path.join(moddir, "m_cress_hip.mod")):
@@ -134,3 +134,21 @@ def test_tier2_bc8_matches_oracle(opts):
             host.check(L.cress_bc8w(C.byref(cfg), _ptr(a)), "cress_bc8w")
         ora.or_bc8(c, kind, kmax, b)
         np.testing.assert_array_equal(a, b, err_msg="kind %d kmax %d" % (kind, kmax))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("wbc", [-1, 7])
+@pytest.mark.parametrize("advopt", [1, 2, 3])
+def test_tier2_advp_matches_oracle(advopt, wbc):
+    rng = np.random.default_rng(60 + advopt + (wbc > 0))
+    j8u, j8v, j8w = (_rnd(rng, 0.5, 1.5) for _ in range(3))
+    u, v, wc, pp = (_rnd(rng) for _ in range(4))
+    cfg = host.make_config(NI, NJ, NK, 700.0, 900.0, 350.0, 4.0, 1.0, advopt=advopt, cphopt=0, sthopt=0,
+                           wbc=wbc, gwave=30.0)
+    g = _rnd(rng)
+    o = g.copy(order="F")
+    host.check(host.lib().cress_advp(C.byref(cfg), *[_ptr(a) for a in (j8u, j8v, j8w, u, v, wc, pp, g)]),
+               "cress_advp")
+    c = po.make_cfg(NI, NJ, NK, dx=700.0, dy=900.0, dz=350.0, advopt=advopt, wbc=wbc, gwave=30.0)
+    po.Oracle(32).or_advp(c, j8u, j8v, j8w, u, v, wc, pp, o, *[_rnd(rng) for _ in range(7)])
+    np.testing.assert_array_equal(g, o)
