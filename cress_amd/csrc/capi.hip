@@ -557,6 +557,109 @@ static void run_corners(cress_ctx *x, const CornerSet &cs) {
 // ---------------------------------------------------------------------------
 // One large step.
 // ---------------------------------------------------------------------------
+// One acoustic substep (s_hevi, hevi.f90:339-427, with stepuv/stepwi/
+// steppi's fills) on the future level: k_ac_ptuv, the u/v fills, the column
+// kernel, the w/p' fills and the ring columns.  oac: the open sides' phase
+// speeds (null on periodic sides).  uf and vf leave in the other buffer of
+// their pair.
+static void ac_substep(cress_ctx *x, float dts, const ObcAc *oacp) {
+  const Grid g = x->g;
+  const Opts o = x->o;
+  const Consts c = x->c;
+  const cress_config &cf = x->cfg;
+  const int ni = g.ni, nj = g.nj, nk = g.nk;
+  const int F = (x->rot + 2) % 3;
+  float **B = x->base;
+  const int open = oacp != nullptr;
+  const ObcAc oac = open ? *oacp : ObcAc{0.0f, 0.0f, 0.0f, 0.0f};
+  const float dz = (float)cf.dz;
+  const float divcv = 0.05f / dts * dz * dz;
+  FillSet fwp;
+  fwp.n = 2;
+  fill_spec(&fwp.f[0], x->w[F], 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk);
+  fill_spec(&fwp.f[1], x->pp[F], 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk);
+  AcPtUv pu;
+  pu.dts = dts;
+  pu.divcv = divcv;
+  pu.jcb = B[CRESS_JCB];
+  pu.rst8u = B[CRESS_RST8U];
+  pu.rst8v = B[CRESS_RST8V];
+  pu.rst8w = B[CRESS_RST8W];
+  pu.wc = x->wc;
+  pu.ppf = x->pp[F];
+  pu.ufrc = x->ufrc;
+  pu.vfrc = x->vfrc;
+  pu.ptil = x->ptil;
+  AcCol ac;
+  ac.dts = dts;
+  ac.ptil = x->ptil;
+  ac.pfrc = x->pfrc;
+  ac.wfrc = x->wfrc;
+  ac.jcb = B[CRESS_JCB];
+  ac.jcb8u = B[CRESS_JCB8U];
+  ac.jcb8v = B[CRESS_JCB8V];
+  ac.jcb8w = B[CRESS_JCB8W];
+  ac.rcsq = B[CRESS_RCSQ];
+  ac.rst = B[CRESS_RST];
+  ac.rbr = B[CRESS_RBR];
+  ac.rst8w = B[CRESS_RST8W];
+  ac.wf = x->w[F];
+  ac.wc = x->wc;
+  ac.ppf = x->pp[F];
+  const int nring = 2 * (ni - 1) + 2 * (nj - 3);
+  const Tiles tpu = tiles(ni, nj - 1, 64, CRESS_PTUV_BY, nk - 1, CRESS_PTUV_ORDER);
+  const Tiles tcol = tiles(ni - 1, nj - 3, 64, CRESS_COL_BY, 1, 1);
+  {
+    pu.uf = x->u[F];
+    pu.vf = x->v[F];
+    pu.ufn = x->ualt;
+    pu.vfn = x->valt;
+    LAUNCH(x, KT_AC_PTUV, k_ac_ptuv, flat(tpu), dim3(64, CRESS_PTUV_BY), 0, tpu, g, o, c, pu);
+    std::swap(x->u[F], x->ualt);
+    std::swap(x->v[F], x->valt);
+    if (open) {
+      // rbcu/rbcv from the previous substep's u, v (stepuv.f90:266-280),
+      // rbcw and rbcss on w, p' before the column kernel updates the
+      // interior next to them (stepwi.f90:286-296, steppi), then the corner
+      // means of stepuv, stepwi and steppi (bc4news)
+      ObcAcArgs oa;
+      oa.uold = x->ualt;
+      oa.vold = x->valt;
+      oa.unew = x->u[F];
+      oa.vnew = x->v[F];
+      oa.wf = x->w[F];
+      oa.ppf = x->pp[F];
+      LAUNCH(x, KT_OBC, k_obc_ac_edges, dim3((2 * (nj + 2) + 2 * (ni + 2) + 63) / 64, nk), dim3(64), 0, g,
+             oac, oa);
+      CornerSet cs;
+      cs.n = 0;
+      corner_spec(&cs, x->u[F], 1, ni, 1, nj - 1);
+      corner_spec(&cs, x->v[F], 1, ni - 1, 1, nj);
+      corner_spec(&cs, x->w[F], 1, ni - 1, 1, nj - 1);
+      corner_spec(&cs, x->pp[F], 1, ni - 1, 1, nj - 1);
+      run_corners(x, cs);
+    }
+    FillSet fuv;
+    fuv.n = open ? 0 : 2;
+    fill_spec(&fuv.f[0], x->u[F], 3, 1, ni - 2, ni, 2, 1, nj - 2, nj - 1, nk);
+    fill_spec(&fuv.f[1], x->v[F], 2, 1, ni - 2, ni - 1, 3, 1, nj - 2, nj, nk);
+    VbcSet vuv;
+    vuv.n = 2;
+    vuv.kind[0] = 1;
+    vuv.ptr[0] = x->u[F];
+    vuv.kind[1] = 2;
+    vuv.ptr[1] = x->v[F];
+    run_fill(x, fuv);
+    run_vbc(x, vuv);
+    ac.uf = x->u[F];
+    ac.vf = x->v[F];
+    LAUNCH(x, KT_AC_COL, k_ac_col, flat(tcol), dim3(64, CRESS_COL_BY), CRESS_COL_LDS, tcol, g, o, c, ac);
+    if (!open) run_fill(x, fwp);
+    LAUNCH(x, KT_AC_RING, k_ac_ring, dim3((nring + 63) / 64, nk), dim3(64), 0, g, o, B[CRESS_JCB8W], x->w[F],
+           x->wc, x->pp[F]);
+  }
+}
+
 static int large_step(cress_ctx *x, int first) {
   const Grid g = x->g;
   const Opts o = x->o;
@@ -671,92 +774,7 @@ static int large_step(cress_ctx *x, int first) {
   HIPCHK(hipMemcpyAsync(x->pp[F], x->pp[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
   LAUNCH(x, KT_PHY2CNT, k_phy2cnt, flat(tiles(ni, nj - 1, 64, 4, nk)), b2, 0, tiles(ni, nj - 1, 64, 4, nk), g,
          o, x->w[F], B[CRESS_JCB8W], x->wc);
-  const float dz = (float)cf.dz;
-  const float divcv = 0.05f / dts * dz * dz;
-  FillSet fwp;
-  fwp.n = 2;
-  fill_spec(&fwp.f[0], x->w[F], 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk);
-  fill_spec(&fwp.f[1], x->pp[F], 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk);
-  AcPtUv pu;
-  pu.dts = dts;
-  pu.divcv = divcv;
-  pu.jcb = B[CRESS_JCB];
-  pu.rst8u = B[CRESS_RST8U];
-  pu.rst8v = B[CRESS_RST8V];
-  pu.rst8w = B[CRESS_RST8W];
-  pu.wc = x->wc;
-  pu.ppf = x->pp[F];
-  pu.ufrc = x->ufrc;
-  pu.vfrc = x->vfrc;
-  pu.ptil = x->ptil;
-  AcCol ac;
-  ac.dts = dts;
-  ac.ptil = x->ptil;
-  ac.pfrc = x->pfrc;
-  ac.wfrc = x->wfrc;
-  ac.jcb = B[CRESS_JCB];
-  ac.jcb8u = B[CRESS_JCB8U];
-  ac.jcb8v = B[CRESS_JCB8V];
-  ac.jcb8w = B[CRESS_JCB8W];
-  ac.rcsq = B[CRESS_RCSQ];
-  ac.rst = B[CRESS_RST];
-  ac.rbr = B[CRESS_RBR];
-  ac.rst8w = B[CRESS_RST8W];
-  ac.wf = x->w[F];
-  ac.wc = x->wc;
-  ac.ppf = x->pp[F];
-  const int nring = 2 * (ni - 1) + 2 * (nj - 3);
-  const Tiles tpu = tiles(ni, nj - 1, 64, CRESS_PTUV_BY, nk - 1, CRESS_PTUV_ORDER);
-  const Tiles tcol = tiles(ni - 1, nj - 3, 64, CRESS_COL_BY, 1, 1);
-  for (int s = 0; s < cf.nsstp; s++) {
-    pu.uf = x->u[F];
-    pu.vf = x->v[F];
-    pu.ufn = x->ualt;
-    pu.vfn = x->valt;
-    LAUNCH(x, KT_AC_PTUV, k_ac_ptuv, flat(tpu), dim3(64, CRESS_PTUV_BY), 0, tpu, g, o, c, pu);
-    std::swap(x->u[F], x->ualt);
-    std::swap(x->v[F], x->valt);
-    if (open) {
-      // rbcu/rbcv from the previous substep's u, v (stepuv.f90:266-280),
-      // rbcw and rbcss on w, p' before the column kernel updates the
-      // interior next to them (stepwi.f90:286-296, steppi), then the corner
-      // means of stepuv, stepwi and steppi (bc4news)
-      ObcAcArgs oa;
-      oa.uold = x->ualt;
-      oa.vold = x->valt;
-      oa.unew = x->u[F];
-      oa.vnew = x->v[F];
-      oa.wf = x->w[F];
-      oa.ppf = x->pp[F];
-      LAUNCH(x, KT_OBC, k_obc_ac_edges, dim3((2 * (nj + 2) + 2 * (ni + 2) + 63) / 64, nk), dim3(64), 0, g,
-             oac, oa);
-      CornerSet cs;
-      cs.n = 0;
-      corner_spec(&cs, x->u[F], 1, ni, 1, nj - 1);
-      corner_spec(&cs, x->v[F], 1, ni - 1, 1, nj);
-      corner_spec(&cs, x->w[F], 1, ni - 1, 1, nj - 1);
-      corner_spec(&cs, x->pp[F], 1, ni - 1, 1, nj - 1);
-      run_corners(x, cs);
-    }
-    FillSet fuv;
-    fuv.n = open ? 0 : 2;
-    fill_spec(&fuv.f[0], x->u[F], 3, 1, ni - 2, ni, 2, 1, nj - 2, nj - 1, nk);
-    fill_spec(&fuv.f[1], x->v[F], 2, 1, ni - 2, ni - 1, 3, 1, nj - 2, nj, nk);
-    VbcSet vuv;
-    vuv.n = 2;
-    vuv.kind[0] = 1;
-    vuv.ptr[0] = x->u[F];
-    vuv.kind[1] = 2;
-    vuv.ptr[1] = x->v[F];
-    run_fill(x, fuv);
-    run_vbc(x, vuv);
-    ac.uf = x->u[F];
-    ac.vf = x->v[F];
-    LAUNCH(x, KT_AC_COL, k_ac_col, flat(tcol), dim3(64, CRESS_COL_BY), CRESS_COL_LDS, tcol, g, o, c, ac);
-    if (!open) run_fill(x, fwp);
-    LAUNCH(x, KT_AC_RING, k_ac_ring, dim3((nring + 63) / 64, nk), dim3(64), 0, g, o, B[CRESS_JCB8W], x->w[F],
-           x->wc, x->pp[F]);
-  }
+  for (int s = 0; s < cf.nsstp; s++) ac_substep(x, dts, open ? &oac : nullptr);
   stamp(x, 2);
 
   // ---- scalar leapfrog (steps) ----
@@ -1264,6 +1282,45 @@ int cress_advuvw(const cress_config *cfg, const float *rstxu, const float *rstxv
   HIPCHK(hipMemcpy(vfrc, pv.p, n * sizeof(float), hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(wfrc, pw.p, n * sizeof(float), hipMemcpyDeviceToHost));
   return CRESS_OK;
+}
+
+// s_hevi for one substep: a context of the caller's configuration holds
+// the arrays, the substep of the large step runs once, and the results come
+// back (periodic sides).
+int cress_hevi(const cress_config *cfg, float dts, const float *jcb, const float *jcb8u, const float *jcb8v,
+               const float *jcb8w, const float *rbr, const float *rst, const float *rst8u, const float *rst8v,
+               const float *rst8w, const float *rcsq, const float *ufrc, const float *vfrc, const float *wfrc,
+               const float *pfrc, float *uf, float *vf, float *wf, float *wc, float *ppf) {
+  const float *in[14] = {jcb, jcb8u, jcb8v, jcb8w, rbr, rst, rst8u, rst8v, rst8w, rcsq, ufrc, vfrc, wfrc, pfrc};
+  for (const float *a : in)
+    if (!a) return CRESS_EINVAL;
+  if (!cfg || !uf || !vf || !wf || !wc || !ppf || !(dts > 0.0f)) return CRESS_EINVAL;
+  if (!(cfg->wbc == -1 && cfg->ebc == -1 && cfg->sbc == -1 && cfg->nbc == -1)) return CRESS_EUNSUPPORTED;
+  cress_ctx *x = nullptr;
+  int rc = cress_ctx_create(cfg, &x);
+  if (rc) return rc;
+  const int ids[10] = {CRESS_JCB, CRESS_JCB8U, CRESS_JCB8V, CRESS_JCB8W, CRESS_RBR,
+                       CRESS_RST, CRESS_RST8U, CRESS_RST8V, CRESS_RST8W, CRESS_RCSQ};
+  for (int f = 0; f < 10 && !rc; f++) rc = cress_upload(x, ids[f], 0, in[f]);
+  if (!rc) rc = cress_upload(x, CRESS_U, 2, uf);
+  if (!rc) rc = cress_upload(x, CRESS_V, 2, vf);
+  if (!rc) rc = cress_upload(x, CRESS_W, 2, wf);
+  if (!rc) rc = cress_upload(x, CRESS_PP, 2, ppf);
+  float *scr[7] = {x->ufrc, x->vfrc, x->wfrc, x->pfrc, x->wc, x->ualt, x->valt};
+  const float *src[7] = {ufrc, vfrc, wfrc, pfrc, wc, uf, vf};
+  for (int f = 0; f < 7 && !rc; f++) rc = copy_field(x, scr[f], src[f], true);
+  if (!rc) {
+    ac_substep(x, dts, nullptr);
+    if (hipGetLastError() != hipSuccess) rc = CRESS_EHIP;
+  }
+  if (!rc) rc = cress_download(x, CRESS_U, 2, uf);
+  if (!rc) rc = cress_download(x, CRESS_V, 2, vf);
+  if (!rc) rc = cress_download(x, CRESS_W, 2, wf);
+  if (!rc) rc = cress_download(x, CRESS_PP, 2, ppf);
+  if (!rc) rc = copy_field(x, x->wc, wc, false);
+  if (!rc && hipStreamSynchronize(x->stream) != hipSuccess) rc = CRESS_EHIP;
+  cress_ctx_destroy(x);
+  return rc;
 }
 
 // s_advp alone is the pressure forcing kernel without sponge or smoothing

@@ -58,6 +58,7 @@ module m_cress_hip
   public :: cress_upload, cress_download, cress_large_step, cress_sync
   public :: cress_set_timing, cress_stage_times
   public :: cress_gaussel, cress_advs, cress_bcycle, cress_advuvw, cress_advp
+  public :: cress_hevi
   public :: cress_bc8u, cress_bc8v, cress_bc8w
   public :: cress_errmsg, base_cfg
 
@@ -184,6 +185,21 @@ module m_cress_hip
       real(c_float), intent(inout) :: ufrc(*), vfrc(*), wfrc(*)
       integer(c_int) :: cress_advuvw
     end function cress_advuvw
+
+! tier 2: s_hevi (hevi.f90:88-95), one substep, periodic sides
+    function cress_hevi(cfg, dts, jcb, jcb8u, jcb8v, jcb8w, rbr, rst,    &
+     &  rst8u, rst8v, rst8w, rcsq, ufrc, vfrc, wfrc, pfrc, uf, vf, wf, wc, &
+     &  ppf) bind(c, name='cress_hevi')
+      import :: c_int, c_float, cress_config
+      type(cress_config), intent(in) :: cfg
+      real(c_float), value :: dts
+      real(c_float), intent(in) :: jcb(*), jcb8u(*), jcb8v(*), jcb8w(*)
+      real(c_float), intent(in) :: rbr(*), rst(*), rst8u(*), rst8v(*)
+      real(c_float), intent(in) :: rst8w(*), rcsq(*)
+      real(c_float), intent(in) :: ufrc(*), vfrc(*), wfrc(*), pfrc(*)
+      real(c_float), intent(inout) :: uf(*), vf(*), wf(*), wc(*), ppf(*)
+      integer(c_int) :: cress_hevi
+    end function cress_hevi
 
 ! tier 2: s_advp (advp.f90:75-79), mfcopt=0, diaopt=0
     function cress_advp(cfg, jcb8u, jcb8v, jcb8w, u, v, wc, pp, pfrc)    &

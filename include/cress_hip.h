@@ -176,6 +176,18 @@ int cress_advs(const cress_config *cfg, const float *rstxu, const float *rstxv,
 int cress_advuvw(const cress_config *cfg, const float *rstxu, const float *rstxv, const float *rstxwc,
                  const float *u, const float *v, const float *w, float *ufrc, float *vfrc, float *wfrc);
 
+/* s_hevi(fpgwmopt,fpbuyopt,isstp,dts,gtinc,ni,nj,nk,j31,j32,jcb,jcb8u,jcb8v,
+ *        jcb8w,mf,mf8u,mf8v,rmf,rmf8u,rmf8v,ubr,vbr,ptbr,rbr,rst,rst8u,rst8v,
+ *        rst8w,rcsq,ufrc,vfrc,wfrc,pfrc,ptfrc,ucpx,...,ptptd,uf,vf,wf,wc,ppf,
+ *        ptpf,tmp1..tmp6)  SrcOrig/hevi.f90:88-95, for gwmopt=0, flat terrain
+ * (j31=j32=0), no map factors and periodic sides: one acoustic substep of
+ * uf, vf, wf, wc, ppf in place, with the halo fills of stepuv, stepwi and
+ * steppi.  buyopt and divopt come from cfg. */
+int cress_hevi(const cress_config *cfg, float dts, const float *jcb, const float *jcb8u, const float *jcb8v,
+               const float *jcb8w, const float *rbr, const float *rst, const float *rst8u, const float *rst8v,
+               const float *rst8w, const float *rcsq, const float *ufrc, const float *vfrc, const float *wfrc,
+               const float *pfrc, float *uf, float *vf, float *wf, float *wc, float *ppf);
+
 /* s_advp(fpadvopt,fpmpopt,fpmfcopt,fpdiaopt,fpiwest,...,fpdziv,ni,nj,nk,
  *        mf8u,mf8v,jcb8u,jcb8v,jcb8w,u,v,wc,pp,pfrc,jcbxu,jcbxv,jcbxwc,
  *        hadv,vadv,tmp1,tmp2,tmp3)  SrcOrig/advp.f90:75-79, mfcopt=0 and

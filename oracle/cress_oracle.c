@@ -1323,6 +1323,39 @@ static void or_forcedrv(const or_cfg *c, or_state *s, R *qall, R *ufrc, R *vfrc,
 
 /* s_smalldrv, smalldrv.f90:343-398 with s_hevi, hevi.f90:339-427: nsstp
  * acoustic substeps on the future level (uf,vf,wf,ppf). */
+/* One acoustic substep, s_hevi (hevi.f90:339-427): pgrad, stepuv,
+ * diverpih, diverpiv, advbspi 'back', pgradiv, buoywsi, stepwi, diverpiv,
+ * advbspi 'fore', steppi, with tmp[1..6] as hevi's work arrays. */
+static void or_substep(const or_cfg *c, R dts, const R *jcb, const R *jcb8u, const R *jcb8v,
+                       const R *jcb8w, const R *rbr, const R *rst, const R *rst8u, const R *rst8v,
+                       const R *rst8w, const R *rcsq, const R *ufrc, const R *vfrc, const R *wfrc,
+                       const R *pfrc, R *uf, R *vf, R *wf, R *wc, R *ppf, R *const *tmp, R *cp)
+{
+  or_pgrad(c, dts, jcb, rst8u, rst8v, rst8w, uf, vf, wc, ppf, tmp[1], tmp[2], tmp[3], tmp[4], tmp[5], tmp[6]);
+  or_stepuv(c, dts, rst8u, rst8v, ufrc, vfrc, tmp[1], tmp[2], uf, vf, cp);
+  or_diverpih(c, jcb8u, jcb8v, rcsq, uf, vf, tmp[2], tmp[4], tmp[5]);
+  or_diverpiv(c, rcsq, wf, tmp[4]);
+  or_advbspi(c, 0, rst, wf, pfrc, tmp[2], tmp[4], tmp[1]);
+  or_pgradiv(c, dts, jcb, wfrc, tmp[1], tmp[3], tmp[4]);
+  if (c->buyopt == 1) or_buoywsi(c, dts, rbr, rst, rcsq, ppf, tmp[1], tmp[3], tmp[4]);
+  or_stepwi(c, dts, jcb, jcb8w, rbr, rst, rst8w, rcsq, tmp[3], wf, wc, tmp[4], tmp[5], tmp[6], cp);
+  or_diverpiv(c, rcsq, wf, tmp[4]);
+  or_advbspi(c, 1, rst, wf, pfrc, tmp[3], tmp[4], tmp[1]);
+  or_steppi(c, dts, jcb, tmp[1], ppf, cp);
+}
+
+/* s_hevi for one substep on periodic sides, with caller-owned work arrays
+ * t1..t6 (the tier-2 entry point's counterpart). */
+void or_hevi(const or_cfg *c, R dts, const R *jcb, const R *jcb8u, const R *jcb8v, const R *jcb8w,
+             const R *rbr, const R *rst, const R *rst8u, const R *rst8v, const R *rst8w, const R *rcsq,
+             const R *ufrc, const R *vfrc, const R *wfrc, const R *pfrc, R *uf, R *vf, R *wf, R *wc,
+             R *ppf, R *t1, R *t2, R *t3, R *t4, R *t5, R *t6)
+{
+  R *const tmp[7] = {NULL, t1, t2, t3, t4, t5, t6};
+  or_substep(c, dts, jcb, jcb8u, jcb8v, jcb8w, rbr, rst, rst8u, rst8v, rst8w, rcsq, ufrc, vfrc, wfrc, pfrc, uf,
+             vf, wf, wc, ppf, tmp, NULL);
+}
+
 static void or_smalldrv(const or_cfg *c, or_state *s, int nsstp, R dts,
                         const R *ufrc, const R *vfrc, const R *wfrc, const R *pfrc, R *wc)
 {
@@ -1335,22 +1368,9 @@ static void or_smalldrv(const or_cfg *c, or_state *s, int nsstp, R dts,
   memcpy(s->w[F], s->w[P], n * sizeof(R));
   memcpy(s->pp[F], s->pp[P], n * sizeof(R));
   or_phy2cnt(c, c->sthopt, s->jcb8w, s->w[F], wc);
-  for (int isstp = 1; isstp <= nsstp; isstp++) {
-    or_pgrad(c, dts, s->jcb, s->rst8u, s->rst8v, s->rst8w, s->u[F], s->v[F], wc, s->pp[F],
-             tmp[1], tmp[2], tmp[3], tmp[4], tmp[5], tmp[6]);
-    or_stepuv(c, dts, s->rst8u, s->rst8v, ufrc, vfrc, tmp[1], tmp[2], s->u[F], s->v[F], s->cp);
-    or_diverpih(c, s->jcb8u, s->jcb8v, s->rcsq, s->u[F], s->v[F], tmp[2], tmp[4], tmp[5]);
-    or_diverpiv(c, s->rcsq, s->w[F], tmp[4]);
-    or_advbspi(c, 0, s->rst, s->w[F], pfrc, tmp[2], tmp[4], tmp[1]);
-    or_pgradiv(c, dts, s->jcb, wfrc, tmp[1], tmp[3], tmp[4]);
-    if (c->buyopt == 1)
-      or_buoywsi(c, dts, s->rbr, s->rst, s->rcsq, s->pp[F], tmp[1], tmp[3], tmp[4]);
-    or_stepwi(c, dts, s->jcb, s->jcb8w, s->rbr, s->rst, s->rst8w, s->rcsq, tmp[3], s->w[F], wc,
-              tmp[4], tmp[5], tmp[6], s->cp);
-    or_diverpiv(c, s->rcsq, s->w[F], tmp[4]);
-    or_advbspi(c, 1, s->rst, s->w[F], pfrc, tmp[3], tmp[4], tmp[1]);
-    or_steppi(c, dts, s->jcb, tmp[1], s->pp[F], s->cp);
-  }
+  for (int isstp = 1; isstp <= nsstp; isstp++)
+    or_substep(c, dts, s->jcb, s->jcb8u, s->jcb8v, s->jcb8w, s->rbr, s->rst, s->rst8u, s->rst8v, s->rst8w,
+               s->rcsq, ufrc, vfrc, wfrc, pfrc, s->u[F], s->v[F], s->w[F], wc, s->pp[F], tmp, s->cp);
 }
 
 /* s_steps, steps.f90:520-760 update, the radiative fills on open sides

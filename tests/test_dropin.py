@@ -5,9 +5,10 @@ CPU: the replacement modules (cress_amd/fortran/dropin/) compile against
 the reference's own module files (the ones oracle/_ref builds from the
 reference sources), i.e. they fit the reference tree with the same module,
 generic and dummy names.
-GPU: cress_gaussel, cress_advs, cress_advuvw, cress_advp, cress_bcycle
-and cress_bc8u/v/w give bitwise the oracle's s_gaussel, s_advs, s_advuvw,
-s_advp, s_bcycle and s_bc8u/v/w results on random arrays (the oracle being pinned
+GPU: cress_gaussel, cress_advs, cress_advuvw, cress_advp, cress_hevi,
+cress_bcycle and cress_bc8u/v/w give bitwise the oracle's s_gaussel,
+s_advs, s_advuvw, s_advp, s_hevi (one substep), s_bcycle and s_bc8u/v/w
+results on random arrays (the oracle being pinned
 bitwise against the reference's routines in test_oracle_ref.py and
 test_oracle_obc.py).
 """
@@ -30,7 +31,8 @@ NI, NJ, NK = 13, 11, 14
 
 @pytest.mark.skipif(not os.path.isdir(REFMOD) or not os.path.exists(hb.FLANG),
                     reason="reference modules or flang absent")
-@pytest.mark.parametrize("name", ["gaussel", "bcycle", "advs", "advuvw", "advp", "bc8u", "bc8v", "bc8w"])
+@pytest.mark.parametrize("name", ["gaussel", "bcycle", "advs", "advuvw", "advp", "hevi", "bc8u", "bc8v",
+                                  "bc8w"])
 def test_dropin_module_compiles_in_reference_tree(name, tmp_path):
     moddir = os.path.join(hb.LIBDIR, "mod")
     if not os.path.exists(os.path.join(moddir, "m_cress_hip.mod")):
@@ -152,3 +154,33 @@ def test_tier2_advp_matches_oracle(advopt, wbc):
     c = po.make_cfg(NI, NJ, NK, dx=700.0, dy=900.0, dz=350.0, advopt=advopt, wbc=wbc, gwave=30.0)
     po.Oracle(32).or_advp(c, j8u, j8v, j8w, u, v, wc, pp, o, *[_rnd(rng) for _ in range(7)])
     np.testing.assert_array_equal(g, o)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["bubble", "vortex"])
+def test_tier2_hevi_matches_oracle(name):
+    """One acoustic substep through cress_hevi equals the oracle's or_hevi
+    (the substep of its small-step driver) bitwise over whole arrays, on a
+    case's base state and initial fields with random forcings."""
+    from cress_amd import cases
+    case = cases.build(name, 12, 10, 16, cphopt=0)
+    cfg = case.cfg
+    rng = np.random.default_rng(70)
+    names = ["jcb", "jcb8u", "jcb8v", "jcb8w", "rbr", "rst", "rst8u", "rst8v", "rst8w", "rcsq"]
+    base = [np.asfortranarray(case.base[k].astype(np.float32)) for k in names]
+    frc = [np.asfortranarray((1e-3 * rng.uniform(-1.0, 1.0, base[0].shape)).astype(np.float32))
+           for _ in range(4)]
+    st = [np.asfortranarray(case.init[k].astype(np.float32)) for k in ("u", "v", "w")]
+    wc = np.asfortranarray((st[2] / base[3]).astype(np.float32))
+    pp = np.asfortranarray(case.init["pp"].astype(np.float32))
+    g = [a.copy(order="F") for a in (st[0], st[1], st[2], wc, pp)]
+    o = [a.copy(order="F") for a in g]
+    dts = float(np.float32(cfg.dtsml))
+    host.check(host.lib().cress_hevi(C.byref(cfg), C.c_float(dts), *[_ptr(a) for a in base + frc + g]),
+               "cress_hevi")
+    m = po.Model(case, 32)
+    tmp = [np.zeros_like(base[0], order="F") for _ in range(6)]
+    m.o.lib.or_hevi(C.byref(m.cfg), m.o._real(dts), *[a.ctypes.data_as(C.c_void_p) for a in base + frc + o + tmp])
+    for a, b, n in zip(g, o, ["uf", "vf", "wf", "wc", "ppf"]):
+        np.testing.assert_array_equal(a, b, err_msg=n)
+    assert not np.array_equal(g[0], st[0])
