@@ -1323,6 +1323,88 @@ int cress_hevi(const cress_config *cfg, float dts, const float *jcb, const float
   return rc;
 }
 
+// s_steps on host arrays: a context of the caller's configuration holds
+// them; the large step's own scalar leapfrog (k_steps), fills and vbcs run
+// once (periodic sides).
+int cress_steps(const cress_config *cfg, float dtb, const float *rst, const float *ptpp, const float *ptfrc,
+                const float *qvp, const float *qvfrc, const float *qwtrp, const float *qicep, const float *nicep,
+                float *qwtrf, float *qicef, float *nicef, float *ptpf, float *qvf, float *dtdrst) {
+  if (!cfg || !rst || !ptpp || !ptfrc || !qvp || !qvfrc || !ptpf || !qvf || !dtdrst || !(dtb > 0.0f))
+    return CRESS_EINVAL;
+  const bool withq = cfg->cphopt == 2;
+  if (withq && (!qwtrp || !qicep || !nicep || !qwtrf || !qicef || !nicef)) return CRESS_EINVAL;
+  if (!(cfg->wbc == -1 && cfg->ebc == -1 && cfg->sbc == -1 && cfg->nbc == -1)) return CRESS_EUNSUPPORTED;
+  cress_ctx *x = nullptr;
+  int rc = cress_ctx_create(cfg, &x);
+  if (rc) return rc;
+  const Grid g = x->g;
+  const int ni = g.ni, nj = g.nj, nk = g.nk;
+  const size_t h = (size_t)(ni + 2) * (nj + 2) * nk;   // one host field
+  const int P = (x->rot + 0) % 3, F = (x->rot + 2) % 3;
+  rc = cress_upload(x, CRESS_RST, 0, rst);
+  if (!rc) rc = cress_upload(x, CRESS_PTP, 0, ptpp);
+  if (!rc) rc = cress_upload(x, CRESS_QV, 0, qvp);
+  // the cells steps does not write keep what the caller's arrays hold
+  if (!rc) rc = cress_upload(x, CRESS_PTP, 2, ptpf);
+  if (!rc) rc = cress_upload(x, CRESS_QV, 2, qvf);
+  if (!rc) rc = copy_field(x, x->ptfrc, ptfrc, true);
+  if (!rc) rc = copy_field(x, x->qvfrc, qvfrc, true);
+  const int sid[5] = {CRESS_QC, CRESS_QR, CRESS_QI, CRESS_QS, CRESS_QG};
+  const float *sp[5] = {qwtrp, qwtrp + h, qicep, qicep + h, qicep + 2 * h};
+  float *sf[5] = {qwtrf, qwtrf + h, qicef, qicef + h, qicef + 2 * h};
+  for (int s = 0; s < 5 && withq && !rc; s++) {
+    rc = cress_upload(x, sid[s], 0, sp[s]);
+    if (!rc) rc = cress_upload(x, sid[s], 2, sf[s]);
+  }
+  if (withq && !rc) rc = cress_upload(x, CRESS_NCI, 0, nicep);
+  if (withq && !rc) rc = cress_upload(x, CRESS_NCI, 2, nicef);
+  float *ddev = nullptr;
+  if (!rc && hipMalloc(&ddev, (size_t)g.n() * sizeof(float)) != hipSuccess) rc = CRESS_ENOMEM;
+  if (!rc) rc = copy_field(x, ddev, dtdrst, true);
+  if (!rc) {
+    const long n = x->n;
+    StepsIn sti;
+    sti.ptpp = x->ptp[P];
+    sti.ptfrc = x->ptfrc;
+    sti.qp = x->q[P];
+    sti.qvfrc = x->qvfrc;
+    sti.ncip = x->nci[P];
+    sti.ptpf = x->ptp[F];
+    sti.qf = x->q[F];
+    sti.ncif = x->nci[F];
+    sti.qn = n;
+    const Tiles ts = tiles(ni - 1, nj - 3, 64, 4, nk - 3);
+    const float dtb2 = 2.0f * dtb;
+    float **B = x->base;
+    hipLaunchKernelGGL(k_steps, flat(ts), dim3(64, 4), 0, x->stream, ts, g, dtb2, B[CRESS_RST], sti, withq ? 1 : 0);
+    hipLaunchKernelGGL(k_dtdrst, grid2(ni - 3, nj - 3, 64, 4, nk - 3), dim3(64, 4), 0, x->stream, g, dtb2,
+                       B[CRESS_RST], ddev);
+    FillSet fs;
+    VbcSet vs;
+    float *fl[8] = {x->ptp[F], x->q[F], x->q[F] + n, x->q[F] + 2 * n, x->q[F] + 3 * n, x->q[F] + 4 * n,
+                    x->q[F] + 5 * n, x->nci[F]};
+    fs.n = withq ? 8 : 2;
+    vs.n = fs.n;
+    for (int s = 0; s < fs.n; s++) {
+      fill_spec(&fs.f[s], fl[s], 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk);
+      vs.kind[s] = 0;
+      vs.ptr[s] = fl[s];
+    }
+    run_fill(x, fs);
+    run_vbc(x, vs);
+    if (hipGetLastError() != hipSuccess) rc = CRESS_EHIP;
+  }
+  if (!rc) rc = cress_download(x, CRESS_PTP, 2, ptpf);
+  if (!rc) rc = cress_download(x, CRESS_QV, 2, qvf);
+  for (int s = 0; s < 5 && withq && !rc; s++) rc = cress_download(x, sid[s], 2, sf[s]);
+  if (withq && !rc) rc = cress_download(x, CRESS_NCI, 2, nicef);
+  if (!rc) rc = copy_field(x, ddev, dtdrst, false);
+  if (!rc && hipStreamSynchronize(x->stream) != hipSuccess) rc = CRESS_EHIP;
+  if (ddev) hipFree(ddev);
+  cress_ctx_destroy(x);
+  return rc;
+}
+
 // s_advp alone is the pressure forcing kernel without sponge or smoothing
 // (its forcing is the advection term only, written over the interior).
 int cress_advp(const cress_config *cfg, const float *jcb8u, const float *jcb8v, const float *jcb8w,

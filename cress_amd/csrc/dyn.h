@@ -142,4 +142,5 @@ __global__ void k_advuvw_only(Grid g, Opts o, Consts c, const float *u, const fl
                               const float *rxu, const float *rxv, const float *rxw, float *ufrc, float *vfrc,
                               float *wfrc);
 __global__ void k_bc8(Grid g, int kind, int lo, int hi, int kmax, float *var);
+__global__ void k_dtdrst(Grid g, float dtb2, const float *rst, float *dtdrst);
 }  // namespace cress

@@ -880,6 +880,17 @@ __global__ void k_steps(Tiles T, Grid g, float dtb2, const float *rst, StepsIn s
   s.ncif[p] = s.ncip[p] + s.ncif[p] * d;
 }
 
+// steps.f90's dtdrst = 2*dtb/rst over the interior (the array s_steps
+// returns beside the future level).
+__global__ void k_dtdrst(Grid g, float dtb2, const float *rst, float *dtdrst) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 2 + blockIdx.z;
+  if (i > g.ni - 2 || j > g.nj - 2) return;
+  const long p = g.at(i, j, k);
+  dtdrst[p] = dtb2 / MP(CRESS_RST, rst, p, k);
+}
+
 // ---------------------------------------------------------------------------
 // Halo fills.
 // s_bcycle (bcycle.f90:159-225): x pass over j=0..nj+1 then y pass over

@@ -58,7 +58,7 @@ module m_cress_hip
   public :: cress_upload, cress_download, cress_large_step, cress_sync
   public :: cress_set_timing, cress_stage_times
   public :: cress_gaussel, cress_advs, cress_bcycle, cress_advuvw, cress_advp
-  public :: cress_hevi
+  public :: cress_hevi, cress_steps
   public :: cress_bc8u, cress_bc8v, cress_bc8w
   public :: cress_errmsg, base_cfg
 
@@ -200,6 +200,20 @@ module m_cress_hip
       real(c_float), intent(inout) :: uf(*), vf(*), wf(*), wc(*), ppf(*)
       integer(c_int) :: cress_hevi
     end function cress_hevi
+
+! tier 2: s_steps (steps.f90:114-127), periodic sides
+    function cress_steps(cfg, dtb, rst, ptpp, ptfrc, qvp, qvfrc, qwtrp,  &
+     &  qicep, nicep, qwtrf, qicef, nicef, ptpf, qvf, dtdrst)              &
+     &  bind(c, name='cress_steps')
+      import :: c_int, c_float, cress_config
+      type(cress_config), intent(in) :: cfg
+      real(c_float), value :: dtb
+      real(c_float), intent(in) :: rst(*), ptpp(*), ptfrc(*), qvp(*)
+      real(c_float), intent(in) :: qvfrc(*), qwtrp(*), qicep(*), nicep(*)
+      real(c_float), intent(inout) :: qwtrf(*), qicef(*), nicef(*)
+      real(c_float), intent(inout) :: ptpf(*), qvf(*), dtdrst(*)
+      integer(c_int) :: cress_steps
+    end function cress_steps
 
 ! tier 2: s_advp (advp.f90:75-79), mfcopt=0, diaopt=0
     function cress_advp(cfg, jcb8u, jcb8v, jcb8w, u, v, wc, pp, pfrc)    &

@@ -80,6 +80,7 @@ def lib():
         L.cress_advuvw.argtypes = [C.POINTER(Config)] + [P] * 9
         L.cress_advp.argtypes = [C.POINTER(Config)] + [P] * 8
         L.cress_hevi.argtypes = [C.POINTER(Config), C.c_float] + [P] * 19
+        L.cress_steps.argtypes = [C.POINTER(Config), C.c_float] + [P] * 14
         L.cress_bc8u.argtypes = [C.POINTER(Config), I, P]
         L.cress_bc8v.argtypes = [C.POINTER(Config), I, P]
         L.cress_bc8w.argtypes = [C.POINTER(Config), P]

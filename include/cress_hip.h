@@ -188,6 +188,20 @@ int cress_hevi(const cress_config *cfg, float dts, const float *jcb, const float
                const float *rst8w, const float *rcsq, const float *ufrc, const float *vfrc, const float *wfrc,
                const float *pfrc, float *uf, float *vf, float *wf, float *wc, float *ppf);
 
+/* s_steps(fpgpvvar,...,fmois,dtb,gtinc,atinc,ni,nj,nk,nqw,nnw,nqi,nni,nqa,
+ *         qvbr,rst,ptp,ptpp,qv,qvp,qwtr,qwtrp,...,ptfrc,qvfrc,qwtrf,...,
+ *         ptpf,qvf,dtdrst)  SrcOrig/steps.f90:114-127, for gwmopt=0,
+ * advopt<=3, periodic sides, no nudging, aerosol, tracer or TKE: the
+ * leapfrog of theta', qv and, with cphopt=2, qc, qr (qwtr*(..,1:2)), qi,
+ * qs, qg (qice*(..,1:3)) and nci, then the halo fills and vbcs.  qwtrf,
+ * qicef, nicef hold the species forcings on entry and the future level on
+ * return, as in the reference; ptpf, qvf and dtdrst (= 2 dtb / rst over
+ * the interior) are written.  The species arrays are ignored for
+ * cphopt=0. */
+int cress_steps(const cress_config *cfg, float dtb, const float *rst, const float *ptpp, const float *ptfrc,
+                const float *qvp, const float *qvfrc, const float *qwtrp, const float *qicep, const float *nicep,
+                float *qwtrf, float *qicef, float *nicef, float *ptpf, float *qvf, float *dtdrst);
+
 /* s_advp(fpadvopt,fpmpopt,fpmfcopt,fpdiaopt,fpiwest,...,fpdziv,ni,nj,nk,
  *        mf8u,mf8v,jcb8u,jcb8v,jcb8w,u,v,wc,pp,pfrc,jcbxu,jcbxv,jcbxwc,
  *        hadv,vadv,tmp1,tmp2,tmp3)  SrcOrig/advp.f90:75-79, mfcopt=0 and

@@ -1424,6 +1424,24 @@ static void or_steps(const or_cfg *c, or_state *s, R dtb, const R *ptfrc, const 
   if (c->cphopt) or_vbcs(c, s->nci[F]);
 }
 
+/* s_steps on given arrays (the tier-2 entry point's counterpart): qp, qf
+ * hold qv, qc, qr, qi, qs, qg one field after another; qf's species and
+ * ncif hold the forcings on entry; periodic sides. */
+void or_steps_a(const or_cfg *c, R dtb, R *rst, R *ptpp, const R *ptfrc, R *qp, const R *qvfrc, R *ncip,
+                R *ptpf, R *qf, R *ncif, R *dtdrst)
+{
+  or_state st;
+  memset(&st, 0, sizeof st);
+  st.rst = rst;
+  st.ptp[0] = ptpp;
+  st.ptp[2] = ptpf;
+  st.q[0] = qp;
+  st.q[2] = qf;
+  st.nci[0] = ncip;
+  st.nci[2] = ncif;
+  or_steps(c, &st, dtb, ptfrc, qvfrc, dtdrst);
+}
+
 /* s_timeflt, timeflt.f90:335-560: Asselin filter on the present level. */
 static void or_asselin(const or_cfg *c, R *x, const R *xp, const R *xf, int ilo, int ihi,
                        int jhi, int khi)

@@ -6,9 +6,9 @@ the reference's own module files (the ones oracle/_ref builds from the
 reference sources), i.e. they fit the reference tree with the same module,
 generic and dummy names.
 GPU: cress_gaussel, cress_advs, cress_advuvw, cress_advp, cress_hevi,
-cress_bcycle and cress_bc8u/v/w give bitwise the oracle's s_gaussel,
-s_advs, s_advuvw, s_advp, s_hevi (one substep), s_bcycle and s_bc8u/v/w
-results on random arrays (the oracle being pinned
+cress_steps, cress_bcycle and cress_bc8u/v/w give bitwise the oracle's
+s_gaussel, s_advs, s_advuvw, s_advp, s_hevi (one substep), s_steps,
+s_bcycle and s_bc8u/v/w results on random arrays (the oracle being pinned
 bitwise against the reference's routines in test_oracle_ref.py and
 test_oracle_obc.py).
 """
@@ -31,8 +31,8 @@ NI, NJ, NK = 13, 11, 14
 
 @pytest.mark.skipif(not os.path.isdir(REFMOD) or not os.path.exists(hb.FLANG),
                     reason="reference modules or flang absent")
-@pytest.mark.parametrize("name", ["gaussel", "bcycle", "advs", "advuvw", "advp", "hevi", "bc8u", "bc8v",
-                                  "bc8w"])
+@pytest.mark.parametrize("name", ["gaussel", "bcycle", "advs", "advuvw", "advp", "hevi", "steps", "bc8u",
+                                  "bc8v", "bc8w"])
 def test_dropin_module_compiles_in_reference_tree(name, tmp_path):
     moddir = os.path.join(hb.LIBDIR, "mod")
     if not os.path.exists(os.path.join(moddir, "m_cress_hip.mod")):
@@ -184,3 +184,45 @@ def test_tier2_hevi_matches_oracle(name):
     for a, b, n in zip(g, o, ["uf", "vf", "wf", "wc", "ppf"]):
         np.testing.assert_array_equal(a, b, err_msg=n)
     assert not np.array_equal(g[0], st[0])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cphopt", [0, 2])
+def test_tier2_steps_matches_oracle(cphopt):
+    """cress_steps (the scalar leapfrog, halo fills and vbcs of s_steps)
+    equals the oracle's or_steps bitwise over whole arrays, with species
+    forcings that drive some cells below zero (the max(0, .) clamp)."""
+    from cress_amd import cases
+    case = cases.build("bubble", 12, 10, 16, cphopt=cphopt)
+    cfg = case.cfg
+    rng = np.random.default_rng(80 + cphopt)
+    shp = (NI, NJ, NK)
+    shp = case.base["rst"].shape
+    F = lambda a: np.asfortranarray(np.asarray(a, np.float32))
+    R = lambda lo, hi, *extra: F(rng.uniform(lo, hi, shp + extra))
+    rst = F(case.base["rst"])
+    ptpp, qvp = F(case.init["ptp"]), F(case.init["qv"])
+    ptfrc, qvfrc = R(-1e-3, 1e-3), R(-1e-6, 1e-6)
+    qwp, qip, nip = R(0.0, 1e-3, 2), R(0.0, 1e-3, 3), R(0.0, 1e3)
+    qwf, qif, nif = R(-1e-6, 1e-6, 2), R(-1e-6, 1e-6, 3), R(-1.0, 1.0)
+    ptpf, qvf, dtd = R(-1.0, 1.0), R(0.0, 1e-2), R(0.0, 1.0)
+    dtb = float(np.float32(cfg.dtbig))
+    g = [a.copy(order="F") for a in (qwf, qif, nif, ptpf, qvf, dtd)]
+    host.check(host.lib().cress_steps(C.byref(cfg), C.c_float(dtb), *[_ptr(a) for a in
+                                      (rst, ptpp, ptfrc, qvp, qvfrc, qwp, qip, nip)],
+                                      *[_ptr(a) for a in g]), "cress_steps")
+    m = po.Model(case, 32)
+    qp = F(np.concatenate([qvp[..., None], qwp, qip], axis=3))
+    qf = F(np.concatenate([qvf[..., None], qwf, qif], axis=3))
+    o_ptpf, o_nif, o_dtd = ptpf.copy(order="F"), nif.copy(order="F"), dtd.copy(order="F")
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    m.o.lib.or_steps_a(C.byref(m.cfg), m.o._real(dtb), vp(rst), vp(ptpp), vp(ptfrc), vp(qp), vp(qvfrc), vp(nip),
+                       vp(o_ptpf), vp(qf), vp(o_nif), vp(o_dtd))
+    np.testing.assert_array_equal(g[3], o_ptpf, err_msg="ptpf")
+    np.testing.assert_array_equal(g[4], qf[..., 0], err_msg="qvf")
+    np.testing.assert_array_equal(g[5], o_dtd, err_msg="dtdrst")
+    if cphopt == 2:
+        np.testing.assert_array_equal(g[0], qf[..., 1:3], err_msg="qwtrf")
+        np.testing.assert_array_equal(g[1], qf[..., 3:6], err_msg="qicef")
+        np.testing.assert_array_equal(g[2], o_nif, err_msg="nicef")
+        assert (g[0] == 0.0).any() and (g[0] > 0.0).any()
