@@ -660,6 +660,75 @@ static void ac_substep(cress_ctx *x, float dts, const ObcAc *oacp) {
   }
 }
 
+// s_cloudphy for cphopt=2 on the future level (cloudphy.f90:277-338 ->
+// coldblk.f90:321-481): k_micro_src (with the scalar leapfrog fused unless
+// the future level is given) and k_micro_fall.
+static void micro_stage(cress_ctx *x, float dtb, const ObcLf &olf, int open, int given) {
+  const Grid g = x->g;
+  const Consts c = x->c;
+  const cress_config &cf = x->cfg;
+  const int ni = g.ni, nj = g.nj, nk = g.nk;
+  const long n = x->n;
+  const int P = (x->rot + 0) % 3, C = (x->rot + 1) % 3, F = (x->rot + 2) % 3;
+  float **B = x->base;
+  const dim3 b2(64, 4);
+  {
+    // leapfrog: the microphysics step is 2*dtb (cloudphy.f90:281-285)
+    const float dtbs = 2.0f * dtb;
+    const MicroK mk = micro_fold(dtbs, cf.thresq, (float)cf.dz);
+    MicroArgs ma;
+    ma.jcb = B[CRESS_JCB];
+    ma.pbr = B[CRESS_PBR];
+    ma.ptbr = B[CRESS_PTBR];
+    ma.rbr = B[CRESS_RBR];
+    ma.ppp = x->pp[P];
+    ma.ptpp = x->ptp[P];
+    ma.qp = x->q[P];
+    ma.ncip = x->nci[P];
+    ma.ptpf = x->ptp[F];
+    ma.rst = B[CRESS_RST];
+    ma.ptfrc = x->ptfrc;
+    ma.qvfrc = x->qvfrc;
+    ma.sfrc = x->sfrc;
+    ma.qf = x->q[F];
+    ma.ncif = x->nci[F];
+    ma.urq = x->ms.f[0];
+    ma.usq = x->ms.f[1];
+    ma.ugq = x->ms.f[2];
+    ma.dtpmin = (unsigned *)x->ms.col;
+    ma.qn = n;
+    ma.ptpc = x->ptp[C];
+    ma.qc = x->q[C];
+    ma.ncic = x->nci[C];
+    ma.lf = olf;
+    ma.open = open;
+    ma.given = given;
+    unsigned seed;
+    memcpy(&seed, &dtbs, sizeof seed);
+    LAUNCH(x, KT_MICRO_SEED, k_micro_seed, dim3(1), dim3(64), 0, ma.dtpmin, seed);
+    LAUNCH(x, KT_MICRO_SRC, k_micro_src, flat(tiles(ni, nj - 1, 64, 4, nk - 1)), b2, 0, tiles(ni, nj - 1, 64, 4, nk - 1), g, mk, ma);
+    FallArgs fa;
+    fa.rbr = B[CRESS_RBR];
+    fa.rst = B[CRESS_RST];
+    fa.qp = x->q[P];
+    fa.urq = ma.urq;
+    fa.usq = ma.usq;
+    fa.ugq = ma.ugq;
+    fa.qf = x->q[F];
+    fa.ncif = x->nci[F];
+    fa.prwtr = x->prwtr;
+    fa.price = x->price;
+    fa.dtpmin = ma.dtpmin;
+    fa.qn = n;
+    fa.n2 = (long)(ni + 2) * (nj + 2);
+    fa.dtb = dtbs;
+    fa.dziv = c.dziv;
+    fa.thresq = c.thresq;
+    const Tiles tfall = tiles(ni - CRESS_FALL_ORG, nj - 1, 64, 1, 1, 1);
+    LAUNCH(x, KT_MICRO_FALL, k_micro_fall, flat(tfall), dim3(64, 6), 0, tfall, g, fa);
+  }
+}
+
 static int large_step(cress_ctx *x, int first) {
   const Grid g = x->g;
   const Opts o = x->o;
@@ -822,60 +891,7 @@ static int large_step(cress_ctx *x, int first) {
   stamp(x, 3);
 
   // ---- microphysics (cloudphy) on the future level ----
-  if (cf.cphopt == 2) {
-    // leapfrog: the microphysics step is 2*dtb (cloudphy.f90:281-285)
-    const float dtbs = 2.0f * dtb;
-    const MicroK mk = micro_fold(dtbs, cf.thresq, (float)cf.dz);
-    MicroArgs ma;
-    ma.jcb = B[CRESS_JCB];
-    ma.pbr = B[CRESS_PBR];
-    ma.ptbr = B[CRESS_PTBR];
-    ma.rbr = B[CRESS_RBR];
-    ma.ppp = x->pp[P];
-    ma.ptpp = x->ptp[P];
-    ma.qp = x->q[P];
-    ma.ncip = x->nci[P];
-    ma.ptpf = x->ptp[F];
-    ma.rst = B[CRESS_RST];
-    ma.ptfrc = x->ptfrc;
-    ma.qvfrc = x->qvfrc;
-    ma.sfrc = x->sfrc;
-    ma.qf = x->q[F];
-    ma.ncif = x->nci[F];
-    ma.urq = x->ms.f[0];
-    ma.usq = x->ms.f[1];
-    ma.ugq = x->ms.f[2];
-    ma.dtpmin = (unsigned *)x->ms.col;
-    ma.qn = n;
-    ma.ptpc = x->ptp[C];
-    ma.qc = x->q[C];
-    ma.ncic = x->nci[C];
-    ma.lf = olf;
-    ma.open = open;
-    unsigned seed;
-    memcpy(&seed, &dtbs, sizeof seed);
-    LAUNCH(x, KT_MICRO_SEED, k_micro_seed, dim3(1), dim3(64), 0, ma.dtpmin, seed);
-    LAUNCH(x, KT_MICRO_SRC, k_micro_src, flat(tiles(ni, nj - 1, 64, 4, nk - 1)), b2, 0, tiles(ni, nj - 1, 64, 4, nk - 1), g, mk, ma);
-    FallArgs fa;
-    fa.rbr = B[CRESS_RBR];
-    fa.rst = B[CRESS_RST];
-    fa.qp = x->q[P];
-    fa.urq = ma.urq;
-    fa.usq = ma.usq;
-    fa.ugq = ma.ugq;
-    fa.qf = x->q[F];
-    fa.ncif = x->nci[F];
-    fa.prwtr = x->prwtr;
-    fa.price = x->price;
-    fa.dtpmin = ma.dtpmin;
-    fa.qn = n;
-    fa.n2 = (long)(ni + 2) * (nj + 2);
-    fa.dtb = dtbs;
-    fa.dziv = c.dziv;
-    fa.thresq = c.thresq;
-    const Tiles tfall = tiles(ni - CRESS_FALL_ORG, nj - 1, 64, 1, 1, 1);
-    LAUNCH(x, KT_MICRO_FALL, k_micro_fall, flat(tfall), dim3(64, 6), 0, tfall, g, fa);
-  }
+  if (cf.cphopt == 2) micro_stage(x, dtb, olf, open, 0);
   stamp(x, 4);
 
   // ---- Asselin filter (timeflt), then satadjst on present and future ----
@@ -1401,6 +1417,45 @@ int cress_steps(const cress_config *cfg, float dtb, const float *rst, const floa
   if (!rc) rc = copy_field(x, ddev, dtdrst, false);
   if (!rc && hipStreamSynchronize(x->stream) != hipSuccess) rc = CRESS_EHIP;
   if (ddev) hipFree(ddev);
+  cress_ctx_destroy(x);
+  return rc;
+}
+
+// s_cloudphy on host arrays for cphopt=2: a context of the caller's
+// configuration holds them and the large step's microphysics stage runs
+// once on the given future level.
+int cress_cloudphy(const cress_config *cfg, float dtb, const float *jcb, const float *pbr, const float *ptbr,
+                   const float *rbr, const float *rst, const float *ppp, const float *ptpp, const float *qvp,
+                   const float *qwtrp, const float *qicep, const float *nicep, float *ptpf, float *qvf,
+                   float *qwtrf, float *qicef, float *nicef, float *prwtr, float *price) {
+  const float *in[12] = {jcb, pbr, ptbr, rbr, rst, ppp, ptpp, qvp, qwtrp, qicep, nicep, ptpf};
+  for (const float *a : in)
+    if (!a) return CRESS_EINVAL;
+  if (!cfg || !qvf || !qwtrf || !qicef || !nicef || !prwtr || !price || !(dtb > 0.0f)) return CRESS_EINVAL;
+  if (cfg->cphopt != 2) return CRESS_EUNSUPPORTED;
+  cress_ctx *x = nullptr;
+  int rc = cress_ctx_create(cfg, &x);
+  if (rc) return rc;
+  const size_t h = (size_t)(cfg->ni + 2) * (cfg->nj + 2) * cfg->nk;
+  const int bid[5] = {CRESS_JCB, CRESS_PBR, CRESS_PTBR, CRESS_RBR, CRESS_RST};
+  for (int f = 0; f < 5 && !rc; f++) rc = cress_upload(x, bid[f], 0, in[f]);
+  const int fid[8] = {CRESS_PTP, CRESS_QV, CRESS_QC, CRESS_QR, CRESS_QI, CRESS_QS, CRESS_QG, CRESS_NCI};
+  const float *past[8] = {ptpp, qvp, qwtrp, qwtrp + h, qicep, qicep + h, qicep + 2 * h, nicep};
+  float *fut[8] = {ptpf, qvf, qwtrf, qwtrf + h, qicef, qicef + h, qicef + 2 * h, nicef};
+  if (!rc) rc = cress_upload(x, CRESS_PP, 0, ppp);
+  for (int f = 0; f < 8 && !rc; f++) {
+    rc = cress_upload(x, fid[f], 0, past[f]);
+    if (!rc) rc = cress_upload(x, fid[f], 2, fut[f]);
+  }
+  if (!rc) rc = cress_upload(x, CRESS_PRWTR, 0, prwtr);
+  if (!rc) rc = cress_upload(x, CRESS_PRICE, 0, price);
+  if (!rc) {
+    micro_stage(x, dtb, ObcLf{0.0f, 0.0f, 0.0f, 0.0f}, 0, 1);
+    if (hipGetLastError() != hipSuccess) rc = CRESS_EHIP;
+  }
+  for (int f = 0; f < 8 && !rc; f++) rc = cress_download(x, fid[f], 2, fut[f]);
+  if (!rc) rc = cress_download(x, CRESS_PRWTR, 0, prwtr);
+  if (!rc) rc = cress_download(x, CRESS_PRICE, 0, price);
   cress_ctx_destroy(x);
   return rc;
 }

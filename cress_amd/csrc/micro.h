@@ -39,6 +39,10 @@ struct MicroArgs {
   const float *ptpc, *qc, *ncic;
   ObcLf lf;
   int open;
+  // 1: the future level already holds what steps leaves (s_cloudphy's
+  // input on its own); 0: it is formed here from the past level and the
+  // forcings
+  int given;
 };
 
 // First column of the sedimentation kernel's x tiles (measured at

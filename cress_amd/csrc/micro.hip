@@ -116,7 +116,16 @@ __global__ void __launch_bounds__(256) CRESS_MICRO_ATTR k_micro_src(Tiles T, Gri
     // (rbcpt, rbcqv, rbcq, rbcs) and bc4news's corner means.
     const int ks = k == 1 ? 2 : (k == g.nk - 1 ? g.nk - 2 : k);
     float ptpf, qvf, qcf, qrf, qif, qsf, qgf, ncif;
-    if (a.open && (i == 1 || i == g.ni - 1 || j == 1 || j == g.nj - 1)) {
+    if (a.given) {
+      ptpf = a.ptpf[p];
+      qvf = a.qf[p];
+      qcf = a.qf[p + qn];
+      qrf = a.qf[p + 2 * qn];
+      qif = a.qf[p + 3 * qn];
+      qsf = a.qf[p + 4 * qn];
+      qgf = a.qf[p + 5 * qn];
+      ncif = a.ncif[p];
+    } else if (a.open && (i == 1 || i == g.ni - 1 || j == 1 || j == g.nj - 1)) {
       ptpf = obc_lf_ring(g, a.lf, false, a.ptpc, a.ptpp, i, j, ks);
       qvf = obc_lf_ring(g, a.lf, true, a.qc, a.qp, i, j, ks);
       qcf = obc_lf_ring(g, a.lf, true, a.qc + qn, a.qp + qn, i, j, ks);

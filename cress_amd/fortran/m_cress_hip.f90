@@ -58,7 +58,7 @@ module m_cress_hip
   public :: cress_upload, cress_download, cress_large_step, cress_sync
   public :: cress_set_timing, cress_stage_times
   public :: cress_gaussel, cress_advs, cress_bcycle, cress_advuvw, cress_advp
-  public :: cress_hevi, cress_steps
+  public :: cress_hevi, cress_steps, cress_cloudphy
   public :: cress_bc8u, cress_bc8v, cress_bc8w
   public :: cress_errmsg, base_cfg
 
@@ -214,6 +214,22 @@ module m_cress_hip
       real(c_float), intent(inout) :: ptpf(*), qvf(*), dtdrst(*)
       integer(c_int) :: cress_steps
     end function cress_steps
+
+! tier 2: s_cloudphy (cloudphy.f90:84-91), cphopt=2, haiopt=0
+    function cress_cloudphy(cfg, dtb, jcb, pbr, ptbr, rbr, rst, ppp,     &
+     &  ptpp, qvp, qwtrp, qicep, nicep, ptpf, qvf, qwtrf, qicef, nicef,  &
+     &  prwtr, price) bind(c, name='cress_cloudphy')
+      import :: c_int, c_float, cress_config
+      type(cress_config), intent(in) :: cfg
+      real(c_float), value :: dtb
+      real(c_float), intent(in) :: jcb(*), pbr(*), ptbr(*), rbr(*)
+      real(c_float), intent(in) :: rst(*), ppp(*), ptpp(*), qvp(*)
+      real(c_float), intent(in) :: qwtrp(*), qicep(*), nicep(*)
+      real(c_float), intent(inout) :: ptpf(*), qvf(*), qwtrf(*)
+      real(c_float), intent(inout) :: qicef(*), nicef(*)
+      real(c_float), intent(inout) :: prwtr(*), price(*)
+      integer(c_int) :: cress_cloudphy
+    end function cress_cloudphy
 
 ! tier 2: s_advp (advp.f90:75-79), mfcopt=0, diaopt=0
     function cress_advp(cfg, jcb8u, jcb8v, jcb8w, u, v, wc, pp, pfrc)    &

@@ -202,6 +202,21 @@ int cress_steps(const cress_config *cfg, float dtb, const float *rst, const floa
                 const float *qvp, const float *qvfrc, const float *qwtrp, const float *qicep, const float *nicep,
                 float *qwtrf, float *qicef, float *nicef, float *ptpf, float *qvf, float *dtdrst);
 
+/* s_cloudphy(fpadvopt,fpcphopt,fphaiopt,nclstp,dtb,dtcl,ni,nj,nk,nqw,nnw,
+ *            nqi,nni,km,jcb,jcb8w,pbr,ptbr,rbr,rst,wf,ppp,ptpp,qvp,qwtrp,
+ *            qicep,qallp,ptpf,qvf,qwtrf,nwtrf,nwtrp,qicef,nicef,nicep,
+ *            qcwtrf,qcicef,prwtr,price,rbv,pi,p,tmp1..tmp11)
+ * SrcOrig/cloudphy.f90:84-91, for the cold-rain scheme (cphopt=2,
+ * haiopt=0, advopt<=3): getvdens, getexner, diagnw, diagnsg, adjstnci,
+ * termblk, srcblk, fallblk on the future level (ptpf, qvf, qwtrf(..,1:2),
+ * qicef(..,1:3), nicef) from the past level, over 2 dtb; prwtr
+ * (0:ni+1,0:nj+1,1:2,1:2) and price (..,1:2,1:3) get the surface rate and
+ * accumulation.  Within 1e-5 of the FP32 reference (binary64 exp/log). */
+int cress_cloudphy(const cress_config *cfg, float dtb, const float *jcb, const float *pbr, const float *ptbr,
+                   const float *rbr, const float *rst, const float *ppp, const float *ptpp, const float *qvp,
+                   const float *qwtrp, const float *qicep, const float *nicep, float *ptpf, float *qvf,
+                   float *qwtrf, float *qicef, float *nicef, float *prwtr, float *price);
+
 /* s_advp(fpadvopt,fpmpopt,fpmfcopt,fpdiaopt,fpiwest,...,fpdziv,ni,nj,nk,
  *        mf8u,mf8v,jcb8u,jcb8v,jcb8w,u,v,wc,pp,pfrc,jcbxu,jcbxv,jcbxwc,
  *        hadv,vadv,tmp1,tmp2,tmp3)  SrcOrig/advp.f90:75-79, mfcopt=0 and

@@ -1166,6 +1166,31 @@ int or_cloudphy(const or_cfg *c, or_state *s, R dtb, int *npstp_out)
   return 0;
 }
 
+/* s_cloudphy on given arrays (the tier-2 entry point's counterpart): qp, qf
+ * hold qv, qc, qr, qi, qs, qg one field after another; the future level
+ * holds steps' result on entry. */
+int or_cloudphy_a(const or_cfg *c, R dtb, R *jcb, R *pbr, R *ptbr, R *rbr, R *rst, R *ppp, R *ptpp, R *qp,
+                  R *ncip, R *ptpf, R *qf, R *ncif, R *prwtr, R *price)
+{
+  or_state st;
+  memset(&st, 0, sizeof st);
+  st.jcb = jcb;
+  st.pbr = pbr;
+  st.ptbr = ptbr;
+  st.rbr = rbr;
+  st.rst = rst;
+  st.pp[0] = ppp;
+  st.ptp[0] = ptpp;
+  st.q[0] = qp;
+  st.nci[0] = ncip;
+  st.ptp[2] = ptpf;
+  st.q[2] = qf;
+  st.nci[2] = ncif;
+  st.prwtr = prwtr;
+  st.price = price;
+  return or_cloudphy(c, &st, dtb, NULL);
+}
+
 static void micro_hook(const or_cfg *c, or_state *s, R dtb, R *qallp)
 {
   (void)qallp;

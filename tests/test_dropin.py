@@ -8,7 +8,8 @@ generic and dummy names.
 GPU: cress_gaussel, cress_advs, cress_advuvw, cress_advp, cress_hevi,
 cress_steps, cress_bcycle and cress_bc8u/v/w give bitwise the oracle's
 s_gaussel, s_advs, s_advuvw, s_advp, s_hevi (one substep), s_steps,
-s_bcycle and s_bc8u/v/w results on random arrays (the oracle being pinned
+s_bcycle and s_bc8u/v/w results, and cress_cloudphy the oracle's
+s_cloudphy within 1e-5 (the microphysics bar) on random arrays (the oracle being pinned
 bitwise against the reference's routines in test_oracle_ref.py and
 test_oracle_obc.py).
 """
@@ -32,7 +33,7 @@ NI, NJ, NK = 13, 11, 14
 @pytest.mark.skipif(not os.path.isdir(REFMOD) or not os.path.exists(hb.FLANG),
                     reason="reference modules or flang absent")
 @pytest.mark.parametrize("name", ["gaussel", "bcycle", "advs", "advuvw", "advp", "hevi", "steps", "bc8u",
-                                  "bc8v", "bc8w"])
+                                  "bc8v", "bc8w", "cloudphy"])
 def test_dropin_module_compiles_in_reference_tree(name, tmp_path):
     moddir = os.path.join(hb.LIBDIR, "mod")
     if not os.path.exists(os.path.join(moddir, "m_cress_hip.mod")):
@@ -226,3 +227,56 @@ def test_tier2_steps_matches_oracle(cphopt):
         np.testing.assert_array_equal(g[1], qf[..., 3:6], err_msg="qicef")
         np.testing.assert_array_equal(g[2], o_nif, err_msg="nicef")
         assert (g[0] == 0.0).any() and (g[0] > 0.0).any()
+
+
+@pytest.mark.gpu
+def test_tier2_cloudphy_matches_oracle():
+    """cress_cloudphy (the cold-rain microphysics on a given future level)
+    against the oracle's or_cloudphy: every field within 1e-5 of the FP32
+    oracle (max |difference| / max |value|), the bar of the full step's
+    microphysics; and the scheme does change the state."""
+    from cress_amd import cases
+    case = cases.build("bubble", 12, 10, 24, cphopt=2)
+    cfg = case.cfg
+    rng = np.random.default_rng(90)
+    F = lambda a: np.asfortranarray(np.asarray(a, np.float32))
+    b = {k: F(case.base[k]) for k in ("jcb", "pbr", "ptbr", "rbr", "rst")}
+    ini = case.init
+    zeros = np.zeros(b["rst"].shape, np.float32)
+    past = {k: F(ini[k]) if k in ini else F(zeros) for k in ("pp", "ptp", "qv", "qc", "qr", "qi", "qs", "qg", "nci")}
+    fut = {k: F(past[k] * (1.0 + 0.01 * rng.uniform(-1.0, 1.0, zeros.shape))) for k in past if k != "pp"}
+    qwp, qip = F(np.stack([past["qc"], past["qr"]], 3)), F(np.stack([past["qi"], past["qs"], past["qg"]], 3))
+    g = {"ptpf": fut["ptp"].copy(order="F"), "qvf": fut["qv"].copy(order="F"),
+         "qwtrf": F(np.stack([fut["qc"], fut["qr"]], 3)), "qicef": F(np.stack([fut["qi"], fut["qs"], fut["qg"]], 3)),
+         "nicef": fut["nci"].copy(order="F"),
+         "prwtr": np.zeros((cfg.ni + 2, cfg.nj + 2, 2, 2), np.float32, order="F"),
+         "price": np.zeros((cfg.ni + 2, cfg.nj + 2, 2, 3), np.float32, order="F")}
+    dtb = float(np.float32(cfg.dtbig))
+    host.check(host.lib().cress_cloudphy(
+        C.byref(cfg), C.c_float(dtb), *[_ptr(b[k]) for k in ("jcb", "pbr", "ptbr", "rbr", "rst")],
+        _ptr(past["pp"]), _ptr(past["ptp"]), _ptr(past["qv"]), _ptr(qwp), _ptr(qip), _ptr(past["nci"]),
+        *[_ptr(g[k]) for k in ("ptpf", "qvf", "qwtrf", "qicef", "nicef", "prwtr", "price")]), "cress_cloudphy")
+    m = po.Model(case, 32)
+    qp = F(np.stack([past[k] for k in ("qv", "qc", "qr", "qi", "qs", "qg")], 3))
+    qf = F(np.stack([fut[k] for k in ("qv", "qc", "qr", "qi", "qs", "qg")], 3))
+    optp, onci = fut["ptp"].copy(order="F"), fut["nci"].copy(order="F")
+    opw = np.zeros_like(g["prwtr"], order="F")
+    opi = np.zeros_like(g["price"], order="F")
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    rc = m.o.lib.or_cloudphy_a(C.byref(m.cfg), m.o._real(dtb), *[vp(b[k]) for k in ("jcb", "pbr", "ptbr", "rbr", "rst")],
+                               vp(past["pp"]), vp(past["ptp"]), vp(qp), vp(past["nci"]), vp(optp), vp(qf), vp(onci),
+                               vp(opw), vp(opi))
+    assert rc == 0
+
+    def rel(a, r):
+        a, r = np.asarray(a, np.float64), np.asarray(r, np.float64)
+        return np.abs(a - r).max() / max(np.abs(r).max(), 1e-30)
+
+    pairs = [("ptpf", g["ptpf"], optp), ("qvf", g["qvf"], qf[..., 0]), ("qcf", g["qwtrf"][..., 0], qf[..., 1]),
+             ("qrf", g["qwtrf"][..., 1], qf[..., 2]), ("qif", g["qicef"][..., 0], qf[..., 3]),
+             ("qsf", g["qicef"][..., 1], qf[..., 4]), ("qgf", g["qicef"][..., 2], qf[..., 5]),
+             ("ncif", g["nicef"], onci), ("prwtr", g["prwtr"], opw), ("price", g["price"], opi)]
+    for name, a, r in pairs:
+        assert rel(a, r) <= 1e-5, (name, rel(a, r))
+    assert not np.array_equal(g["qwtrf"][..., 0], fut["qc"])
+    assert np.abs(opw).max() > 0.0
