@@ -13,6 +13,7 @@
 #include "dyn.h"
 #include "micro.h"
 #include "obc.h"
+#include "turb.h"
 
 #include <cstring>
 #include <utility>
@@ -1458,6 +1459,57 @@ int cress_cloudphy(const cress_config *cfg, float dtb, const float *jcb, const f
   if (!rc) rc = cress_download(x, CRESS_PRICE, 0, price);
   cress_ctx_destroy(x);
   return rc;
+}
+
+// The first four calls of s_turbdrv (tubopt=2): every input and output
+// array goes to the device, the kernels write the ranges the reference
+// writes, and the outputs come back whole.
+int cress_turbcoef(const cress_config *cfg, float dtb, int moist, const float *jcb, const float *jcb8u,
+                   const float *jcb8v, const float *jcb8w, const float *pbr, const float *ptbr, const float *rbr,
+                   const float *ppp, const float *ptpp, const float *qvp, const float *qallp, const float *up,
+                   const float *vp, const float *wp, const float *tkep, float *t11, float *t22, float *t33,
+                   float *t12, float *t13, float *t23, float *t31, float *t32, float *ssq, float *nsq8w, float *rkh,
+                   float *rkv, float *priv) {
+  const float *in[15] = {jcb, jcb8u, jcb8v, jcb8w, pbr, ptbr, rbr, ppp, ptpp, qvp, qallp, up, vp, wp, tkep};
+  float *out[13] = {t11, t22, t33, t12, t13, t23, t31, t32, ssq, nsq8w, rkh, rkv, priv};
+  for (const float *a : in)
+    if (!a) return CRESS_EINVAL;
+  for (float *a : out)
+    if (!a) return CRESS_EINVAL;
+  if (!cfg || cfg->ni < 4 || cfg->nj < 4 || cfg->nk < 4 || !(dtb > 0.0f) || !(cfg->dx > 0.0) ||
+      !(cfg->dy > 0.0) || !(cfg->dz > 0.0) || cfg->cphopt < 0)
+    return CRESS_EINVAL;
+  const int sides[4] = {cfg->wbc, cfg->ebc, cfg->sbc, cfg->nbc};
+  for (int s = 0; s < 4; s++)
+    if (sides[s] != -1 && sides[s] != 7) return CRESS_EUNSUPPORTED;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CRESS_ENODEV;
+  const Grid g = make_grid(cfg->ni, cfg->nj, cfg->nk);
+  const size_t n = (size_t)g.n();
+  DevBuf di[15], dout[13];
+  int rc;
+  for (int f = 0; f < 15; f++)
+    if ((rc = to_dev(di[f], in[f], n))) return rc;
+  for (int f = 0; f < 13; f++)
+    if ((rc = to_dev(dout[f], out[f], n))) return rc;
+  TurbArgs a = {};
+  a.moist = moist ? 1 : 0;
+  a.cphopt = cfg->cphopt;
+  a.wbc = cfg->wbc, a.ebc = cfg->ebc, a.sbc = cfg->sbc, a.nbc = cfg->nbc, a.bbc = cfg->bbc, a.tbc = cfg->tbc;
+  // the 4th-order reduction flags: 0 on periodic sides (sethalo.f90:536-580)
+  a.iw = cfg->wbc == -1 ? 0 : 1, a.ie = cfg->ebc == -1 ? 0 : 1;
+  a.js = cfg->sbc == -1 ? 0 : 1, a.jn = cfg->nbc == -1 ? 0 : 1;
+  a.dx = (float)cfg->dx, a.dy = (float)cfg->dy, a.dz = (float)cfg->dz, a.dtb = dtb, a.thresq = (float)cfg->thresq;
+  const float **ip[15] = {&a.jcb, &a.jcb8u, &a.jcb8v, &a.jcb8w, &a.pbr, &a.ptbr, &a.rbr, &a.pp,
+                          &a.ptp, &a.qv, &a.qall, &a.u, &a.v, &a.w, &a.tke};
+  for (int f = 0; f < 15; f++) *ip[f] = (const float *)di[f].p;
+  float **op[13] = {&a.s11, &a.s22, &a.s33, &a.s12, &a.s13, &a.s23, &a.s31, &a.s32,
+                    &a.ssq, &a.nsq8w, &a.rkh, &a.rkv, &a.priv};
+  for (int f = 0; f < 13; f++) *op[f] = (float *)dout[f].p;
+  turb_coef(g, a);
+  HIPCHK(hipGetLastError());
+  for (int f = 0; f < 13; f++) HIPCHK(hipMemcpy(out[f], dout[f].p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return CRESS_OK;
 }
 
 // s_advp alone is the pressure forcing kernel without sponge or smoothing

@@ -1,0 +1,25 @@
+// The coefficients of the 1.5-order TKE closure (tubopt=2): the first four
+// calls of s_turbdrv (turbdrv.f90:245-276) -- s_bruntv, s_defomten,
+// s_defomssq, s_eddyvis -- for flat terrain without map factors
+// (trnopt=0, mfcopt=0), no surface physics (sfcopt=0) and isotropic grids
+// (isoopt=1).
+#pragma once
+#include "common.h"
+
+namespace cress {
+
+struct TurbArgs {
+  int moist, cphopt;                  // fmois 'moist' / 'dry'; cloud physics on when cphopt != 0
+  int wbc, ebc, sbc, nbc, bbc, tbc;
+  int iw, ie, js, jn;                 // 4th-order reduction flags (0 on periodic sides)
+  float dx, dy, dz, dtb, thresq;
+  const float *jcb, *jcb8u, *jcb8v, *jcb8w, *pbr, *ptbr, *rbr;
+  const float *pp, *ptp, *qv, *qall, *u, *v, *w, *tke;   // past level
+  float *s11, *s22, *s33, *s12, *s13, *s23, *s31, *s32;  // deformation tensor
+  float *ssq, *nsq8w, *rkh, *rkv, *priv;
+};
+
+// Launches the kernels on the null stream, in the reference's order.
+void turb_coef(const Grid &g, const TurbArgs &a);
+
+}  // namespace cress

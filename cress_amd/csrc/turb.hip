@@ -1,0 +1,191 @@
+// Kernels of the TKE-closure coefficients (turbdrv.f90:245-276, tubopt=2).
+// Each writes the index ranges the reference writes; the arithmetic keeps
+// Fortran's left-to-right order term by term, so with -ffp-contract=off
+// the tensor, its square and dry / cloud-free N^2 are the reference's
+// bits.  exp and log are taken in binary64 and rounded to binary32 (N^2 in
+// cloudy air, the length scale ln0 of the eddy viscosity): within 1e-5 of
+// the reference's FP32 libm.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "common.h"
+#include "dyn.h"
+#include "turb.h"
+
+namespace cress {
+
+#define AT(a, i, j, k) a[g.at(i, j, k)]
+
+__device__ __forceinline__ float expd(float x) { return (float)exp((double)x); }
+__device__ __forceinline__ float logd(float x) { return (float)log((double)x); }
+
+// comphy.f90:49-130, commath.f90:51,177
+#define TB_G 9.8f
+#define TB_CP 1004.f
+#define TB_CW 4.218e3f
+#define TB_CI 2.106e3f
+#define TB_RD 287.f
+#define TB_P0 1.e5f
+#define TB_T0 273.16f
+#define TB_TLOW 233.16f
+#define TB_LV0 2.50078e6f
+#define TB_LF0 3.34e5f
+#define TB_EPSVA .622f
+#define TB_EPSAV 1.6077f
+#define TB_EPS 1.e-20f
+
+// The moist potential temperature and the factor a of bruntv.f90:276-300
+// at one cell (cloud physics on).
+__device__ __forceinline__ void moist_pt_a(const Grid &g, const TurbArgs &a, int i, int j, int k, float &pt,
+                                           float &fa) {
+  const float rddvcp = TB_RD / TB_CP, cwmci = TB_CW - TB_CI, p0iv = 1.f / TB_P0;
+  const float qv = AT(a.qv, i, j, k);
+  pt = AT(a.ptbr, i, j, k) + AT(a.ptp, i, j, k);
+  const float t = pt * expd(rddvcp * logd(p0iv * (AT(a.pbr, i, j, k) + AT(a.pp, i, j, k))));
+  float ak = TB_LV0 * expd((.167f + 3.67e-4f * t) * logd(TB_T0 / t));
+  if (t <= TB_TLOW) ak = ak + (TB_LF0 + cwmci * (t - TB_T0));
+  const float lhcpt = ak / (TB_CP * t);
+  pt = pt * expd(lhcpt * qv);
+  ak = ak * qv / (TB_RD * t);
+  fa = (1.f + ak) / (1.f + TB_EPSVA * lhcpt * ak);
+}
+
+__device__ __forceinline__ float ptv_at(const Grid &g, const TurbArgs &a, int i, int j, int k) {
+  const float pt = AT(a.ptbr, i, j, k) + AT(a.ptp, i, j, k);
+  if (!a.moist) return pt;
+  const float qv = AT(a.qv, i, j, k);
+  return pt * (1.f + TB_EPSAV * qv) / (1.f + qv);
+}
+
+// s_bruntv: N^2 at w points over i 1..ni-1, j 1..nj-1, k 2..nk-1, and its
+// copies to k=1 and k=nk (bruntv.f90:374-384).
+__global__ void k_bruntv(Grid g, TurbArgs a) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y, k = 1 + blockIdx.z;
+  if (i > g.ni - 1) return;
+  const int kk = k == 1 ? 2 : k == g.nk ? g.nk - 1 : k;
+  const float dziv = 1.f / a.dz;
+  const float gdzv = TB_G * dziv, gdzv05 = .5f * TB_G * dziv;
+  const float den = AT(a.jcb8w, i, j, kk) * (AT(a.ptbr, i, j, kk - 1) + AT(a.ptbr, i, j, kk));
+  float n;
+  if (a.moist && a.cphopt != 0 && AT(a.qall, i, j, kk) > a.thresq) {
+    float p0, a0, p1, a1;
+    moist_pt_a(g, a, i, j, kk - 1, p0, a0);
+    moist_pt_a(g, a, i, j, kk, p1, a1);
+    n = gdzv05 *
+        ((AT(a.qall, i, j, kk - 1) - AT(a.qall, i, j, kk)) +
+         (p1 - p0) * (a0 + a1) / (AT(a.ptbr, i, j, kk - 1) + AT(a.ptbr, i, j, kk))) /
+        AT(a.jcb8w, i, j, kk);
+  } else {
+    n = gdzv * (ptv_at(g, a, i, j, kk) - ptv_at(g, a, i, j, kk - 1)) / den;
+  }
+  AT(a.nsq8w, i, j, k) = n;
+}
+
+// s_defomten, trnopt=0 and mfcopt=0 (defomten.f90:253-1090): every
+// component at one (i,j,k), over the ranges the reference writes.
+__global__ void k_defomten(Grid g, TurbArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = 1 + blockIdx.z;
+  if (i > g.ni + 1) return;
+  const int ni = g.ni, nj = g.nj, nk = g.nk;
+  const float dxiv = 1.f / a.dx, dyiv = 1.f / a.dy, dziv = 1.f / a.dz;
+#define UJ(i, j, k) (AT(a.u, i, j, k) * AT(a.jcb8u, i, j, k))
+#define VJ(i, j, k) (AT(a.v, i, j, k) * AT(a.jcb8v, i, j, k))
+#define WJ(i, j, k) (AT(a.w, i, j, k) * AT(a.jcb8w, i, j, k))
+  if (k <= nk - 1) {
+    if (j >= 1 + a.js && j <= nj - a.jn && i >= 1 + a.iw && i <= ni - a.ie) {
+      const float t3 = 4.f / ((AT(a.jcb, i - 1, j - 1, k) + AT(a.jcb, i, j, k)) +
+                              (AT(a.jcb, i - 1, j, k) + AT(a.jcb, i, j - 1, k)));
+      AT(a.s12, i, j, k) = t3 * ((UJ(i, j, k) - UJ(i, j - 1, k)) * dyiv + (VJ(i, j, k) - VJ(i - 1, j, k)) * dxiv);
+    }
+    if (j >= 1 && j <= nj - 1 && i >= 1 && i <= ni - 1) {
+      const float jcbiv2 = 2.f / AT(a.jcb, i, j, k);
+      AT(a.s11, i, j, k) = jcbiv2 * ((UJ(i + 1, j, k) - UJ(i, j, k)) * dxiv);
+      AT(a.s22, i, j, k) = jcbiv2 * ((VJ(i, j + 1, k) - VJ(i, j, k)) * dyiv);
+      AT(a.s33, i, j, k) = jcbiv2 * (AT(a.w, i, j, k + 1) - AT(a.w, i, j, k)) * dziv;
+    }
+  }
+  if (k >= 2 && k <= nk - 1) {
+    if (j >= 1 && j <= nj - 1 && i >= 1 + a.iw && i <= ni - a.ie) {
+      const float t3 = 2.f / (AT(a.jcb8w, i - 1, j, k) + AT(a.jcb8w, i, j, k));
+      const float s = ((WJ(i, j, k) - WJ(i - 1, j, k)) * dxiv + (AT(a.u, i, j, k) - AT(a.u, i, j, k - 1)) * dziv) * t3;
+      AT(a.s13, i, j, k) = s;
+      AT(a.s31, i, j, k) = s;
+    }
+    if (j >= 1 + a.js && j <= nj - a.jn && i >= 1 && i <= ni - 1) {
+      const float t3 = 2.f / (AT(a.jcb8w, i, j - 1, k) + AT(a.jcb8w, i, j, k));
+      const float s = ((WJ(i, j, k) - WJ(i, j - 1, k)) * dyiv + (AT(a.v, i, j, k) - AT(a.v, i, j, k - 1)) * dziv) * t3;
+      AT(a.s23, i, j, k) = s;
+      AT(a.s32, i, j, k) = s;
+    }
+  }
+#undef UJ
+#undef VJ
+#undef WJ
+}
+
+// s_bcten (bcten.f90) on s31 and s32: level 1 and nk from 3 and nk-2.
+__global__ void k_bcten(Grid g, int bbc, int tbc, float *s31, float *s32) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.nxy) return;
+  float *t = blockIdx.y ? s32 : s31;
+  const long lo = g.nxy * 0, l3 = g.nxy * 2, hi = g.nxy * (g.nk - 1), h2 = g.nxy * (g.nk - 3);
+  if (bbc == 2) t[lo + p] = -t[l3 + p];
+  else if (bbc >= 3) t[lo + p] = t[l3 + p];
+  if (tbc == 2) t[hi + p] = -t[h2 + p];
+  else if (tbc >= 3) t[hi + p] = t[h2 + p];
+}
+
+// s_defomssq (defomssq.f90:95-120) and s_eddyvis (eddyvis.f90:454-486,
+// tubopt=2, sfcopt=0, isoopt=1, mfcopt=0) at one interior cell.
+__global__ void k_eddyvis(Grid g, TurbArgs a) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y, k = 1 + blockIdx.z;
+  if (i > g.ni - 1) return;
+  const float a12 = (AT(a.s12, i, j, k) + AT(a.s12, i + 1, j + 1, k)) + (AT(a.s12, i + 1, j, k) + AT(a.s12, i, j + 1, k));
+  const float a31 = (AT(a.s31, i, j, k) + AT(a.s31, i + 1, j, k + 1)) + (AT(a.s31, i, j, k + 1) + AT(a.s31, i + 1, j, k));
+  const float a32 = (AT(a.s32, i, j, k) + AT(a.s32, i, j + 1, k + 1)) + (AT(a.s32, i, j + 1, k) + AT(a.s32, i, j, k + 1));
+  const float s11 = AT(a.s11, i, j, k), s22 = AT(a.s22, i, j, k), s33 = AT(a.s33, i, j, k);
+  const float ssq = .5f * (s33 * s33 + (s11 * s11 + s22 * s22)) + .0625f * (a12 * a12 + (a31 * a31 + a32 * a32));
+  AT(a.ssq, i, j, k) = ssq;
+  const float ds3 = a.dx * a.dy * a.dz;
+  const float ckm = .1f, ckmin = 1.e-6f, ckmax = .125f / a.dtb;
+  const float ln0 = expd((1.f / 3.f) * logd(ds3 * AT(a.jcb, i, j, k)));
+  const float ln02 = ln0 * ln0;
+  const float nsq = AT(a.nsq8w, i, j, k) + AT(a.nsq8w, i, j, k + 1);
+  const float tke = AT(a.tke, i, j, k);
+  float ln;
+  if (nsq < 0.f) {
+    ln = ln0;
+  } else {
+    float m = .76f * sqrtf(tke / (nsq + TB_EPS));
+    if (!(m < ln0)) m = ln0;
+    ln = .1f * ln0;
+    if (!(ln > m)) ln = m;
+  }
+  const float priv = 1.f + 2.f * ln / ln0;
+  float kv = ckm * ln * sqrtf(tke);
+  if (priv * nsq < ssq) {
+    const float lo = ckmin * ln02;
+    if (lo > kv) kv = lo;
+  }
+  const float hi = ckmax * ln02;
+  kv = AT(a.rbr, i, j, k) * (hi < kv ? hi : kv);
+  AT(a.priv, i, j, k) = priv;
+  AT(a.rkv, i, j, k) = kv;
+  AT(a.rkh, i, j, k) = kv;
+}
+
+void turb_coef(const Grid &g, const TurbArgs &a) {
+  const dim3 blk(64);
+  hipLaunchKernelGGL(k_bruntv, dim3((g.ni - 1 + 63) / 64, g.nj - 1, g.nk), blk, 0, 0, g, a);
+  hipLaunchKernelGGL(k_defomten, dim3((g.ni + 2 + 63) / 64, g.nj + 2, g.nk), blk, 0, 0, g, a);
+  // defomten.f90:1095-1102
+  const unsigned px = (unsigned)(((long)(g.nj + 2) * g.nk + 63) / 64), py = (unsigned)(((long)(g.ni + 2) * g.nk + 63) / 64);
+  hipLaunchKernelGGL(k_bc8, dim3(px, 2), blk, 0, 0, g, 0, a.wbc, a.ebc, g.nk, a.s12);
+  hipLaunchKernelGGL(k_bc8, dim3(py, 2), blk, 0, 0, g, 1, a.sbc, a.nbc, g.nk, a.s12);
+  hipLaunchKernelGGL(k_bc8, dim3(px, 2), blk, 0, 0, g, 0, a.wbc, a.ebc, g.nk, a.s31);
+  hipLaunchKernelGGL(k_bc8, dim3(py, 2), blk, 0, 0, g, 1, a.sbc, a.nbc, g.nk, a.s32);
+  hipLaunchKernelGGL(k_bcten, dim3((unsigned)((g.nxy + 63) / 64), 2), blk, 0, 0, g, a.bbc, a.tbc, a.s31, a.s32);
+  hipLaunchKernelGGL(k_eddyvis, dim3((g.ni - 1 + 63) / 64, g.nj - 1, g.nk - 1), blk, 0, 0, g, a);
+}
+
+}  // namespace cress

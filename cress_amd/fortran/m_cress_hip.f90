@@ -58,7 +58,7 @@ module m_cress_hip
   public :: cress_upload, cress_download, cress_large_step, cress_sync
   public :: cress_set_timing, cress_stage_times
   public :: cress_gaussel, cress_advs, cress_bcycle, cress_advuvw, cress_advp
-  public :: cress_hevi, cress_steps, cress_cloudphy
+  public :: cress_hevi, cress_steps, cress_cloudphy, cress_turbcoef
   public :: cress_bc8u, cress_bc8v, cress_bc8w
   public :: cress_errmsg, base_cfg
 
@@ -230,6 +230,26 @@ module m_cress_hip
       real(c_float), intent(inout) :: prwtr(*), price(*)
       integer(c_int) :: cress_cloudphy
     end function cress_cloudphy
+
+! tier 2: the first four calls of s_turbdrv (turbdrv.f90:245-276), tubopt=2
+    function cress_turbcoef(cfg, dtb, moist, jcb, jcb8u, jcb8v, jcb8w,   &
+     &  pbr, ptbr, rbr, ppp, ptpp, qvp, qallp, up, vp, wp, tkep, t11,     &
+     &  t22, t33, t12, t13, t23, t31, t32, ssq, nsq8w, rkh, rkv, priv)    &
+     &  bind(c, name='cress_turbcoef')
+      import :: c_int, c_float, cress_config
+      type(cress_config), intent(in) :: cfg
+      real(c_float), value :: dtb
+      integer(c_int), value :: moist
+      real(c_float), intent(in) :: jcb(*), jcb8u(*), jcb8v(*), jcb8w(*)
+      real(c_float), intent(in) :: pbr(*), ptbr(*), rbr(*), ppp(*), ptpp(*)
+      real(c_float), intent(in) :: qvp(*), qallp(*), up(*), vp(*), wp(*)
+      real(c_float), intent(in) :: tkep(*)
+      real(c_float), intent(inout) :: t11(*), t22(*), t33(*), t12(*)
+      real(c_float), intent(inout) :: t13(*), t23(*), t31(*), t32(*)
+      real(c_float), intent(inout) :: ssq(*), nsq8w(*), rkh(*), rkv(*)
+      real(c_float), intent(inout) :: priv(*)
+      integer(c_int) :: cress_turbcoef
+    end function cress_turbcoef
 
 ! tier 2: s_advp (advp.f90:75-79), mfcopt=0, diaopt=0
     function cress_advp(cfg, jcb8u, jcb8v, jcb8w, u, v, wc, pp, pfrc)    &

@@ -82,6 +82,7 @@ def lib():
         L.cress_hevi.argtypes = [C.POINTER(Config), C.c_float] + [P] * 19
         L.cress_steps.argtypes = [C.POINTER(Config), C.c_float] + [P] * 14
         L.cress_cloudphy.argtypes = [C.POINTER(Config), C.c_float] + [P] * 18
+        L.cress_turbcoef.argtypes = [C.POINTER(Config), C.c_float, I] + [P] * 28
         L.cress_bc8u.argtypes = [C.POINTER(Config), I, P]
         L.cress_bc8v.argtypes = [C.POINTER(Config), I, P]
         L.cress_bc8w.argtypes = [C.POINTER(Config), P]

@@ -217,6 +217,28 @@ int cress_cloudphy(const cress_config *cfg, float dtb, const float *jcb, const f
                    const float *qwtrp, const float *qicep, const float *nicep, float *ptpf, float *qvf,
                    float *qwtrf, float *qicef, float *nicef, float *prwtr, float *price);
 
+/* The first four calls of s_turbdrv (SrcOrig/turbdrv.f90:245-276) with
+ * tubopt=2 (1.5-order TKE closure), isoopt=1, trnopt=0, mfcopt=0, sfcopt=0:
+ *   s_bruntv(fpcphopt,fpdziv,fpthresq,fmois,ni,nj,nk,jcb8w,pbr,ptbr,ppp,ptpp,
+ *            qvp,qallp,nsq8w,...)                       bruntv.f90:68-69
+ *   s_defomten(...,jcb,jcb8u,jcb8v,jcb8w,...,up,vp,wp,t11,t22,t33,t12,t13,
+ *              t23,t31,t32,...)                         defomten.f90:81-86
+ *   s_defomssq(ni,nj,nk,t11,t22,t33,t12,t31,t32,ssq)     defomssq.f90:68
+ *   s_eddyvis(...,dtb,ni,nj,nk,zph,jcb,rmf,rbr,tkep,ssq,nsq8w,rkh,rkv,priv)
+ *                                                        eddyvis.f90:83-86
+ * moist selects fmois 'moist' (else 'dry'); cfg->cphopt != 0 takes the
+ * cloudy N^2 above cfg->thresq.  Arrays real(0:ni+1,0:nj+1,1:nk); the
+ * outputs are written where the reference writes them and are otherwise
+ * left as passed.  Periodic (-1) or open (7) sides.  The tensor, ssq and
+ * dry / cloud-free N^2 are the FP32 reference's bits; rkh, rkv, priv and
+ * cloudy N^2 (binary64 exp/log) are within 1e-5. */
+int cress_turbcoef(const cress_config *cfg, float dtb, int moist, const float *jcb, const float *jcb8u,
+                   const float *jcb8v, const float *jcb8w, const float *pbr, const float *ptbr, const float *rbr,
+                   const float *ppp, const float *ptpp, const float *qvp, const float *qallp, const float *up,
+                   const float *vp, const float *wp, const float *tkep, float *t11, float *t22, float *t33,
+                   float *t12, float *t13, float *t23, float *t31, float *t32, float *ssq, float *nsq8w, float *rkh,
+                   float *rkv, float *priv);
+
 /* s_advp(fpadvopt,fpmpopt,fpmfcopt,fpdiaopt,fpiwest,...,fpdziv,ni,nj,nk,
  *        mf8u,mf8v,jcb8u,jcb8v,jcb8w,u,v,wc,pp,pfrc,jcbxu,jcbxv,jcbxwc,
  *        hadv,vadv,tmp1,tmp2,tmp3)  SrcOrig/advp.f90:75-79, mfcopt=0 and

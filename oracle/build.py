@@ -24,7 +24,7 @@ def build_oracle():
     src = os.path.join(HERE, "cress_oracle.c")
     deps = [src, os.path.join(HERE, "cress_oracle.h"), os.path.join(HERE, "cress_micro.c"),
             os.path.join(HERE, "cress_obc.c"),
-            os.path.join(HERE, "cress_smooth.c")]
+            os.path.join(HERE, "cress_smooth.c"), os.path.join(HERE, "cress_turb.c")]
     libs = []
     for prec, defs in ((32, ["-DOR_FP32", "-DREAL=float"]), (64, ["-DREAL=double"])):
         lib = os.path.join(OUT, "liboracle%d.so" % prec)

@@ -1580,3 +1580,4 @@ int or_large_step(const or_cfg *c, or_state *s, int first, double dtbig, double 
   or_swp2nxt(c, s);
   return 0;
 }
+#include "cress_turb.c"

@@ -77,6 +77,10 @@
       use m_smoo4qv
       use m_smoo4p
       use m_smoo4uvw
+      use m_bruntv
+      use m_defomten
+      use m_defomssq
+      use m_eddyvis
 
       implicit none
 
@@ -684,5 +688,36 @@
      &                  m8,a,b,c,f1,f2,f3,t1,t2,t3,t4,t5)
       end select
       end subroutine ref_smoo4
+
+! The first four calls of s_turbdrv (turbdrv.f90:245-276), tubopt=2,
+! isoopt=1, with separate work arrays (w1..w4 3-D, w5 2-D).
+      subroutine ref_turbcoef(moist,dtb,ni,nj,nk,jcb,jcb8u,jcb8v,jcb8w,  &
+     &   pbr,ptbr,rbr,ppp,ptpp,qvp,qallp,up,vp,wp,tkep,t11,t22,t33,t12,   &
+     &   t13,t23,t31,t32,ssq,nsq8w,rkh,rkv,priv,w1,w2,w3,w4,w5,zph,mf)    &
+     &   bind(C,name='ref_turbcoef')
+      integer(c_int), value :: moist,ni,nj,nk
+      real, value :: dtb
+      real jcb(*),jcb8u(*),jcb8v(*),jcb8w(*),pbr(*),ptbr(*),rbr(*)
+      real ppp(*),ptpp(*),qvp(*),qallp(*),up(*),vp(*),wp(*),tkep(*)
+      real t11(*),t22(*),t33(*),t12(*),t13(*),t23(*),t31(*),t32(*)
+      real ssq(*),nsq8w(*),rkh(*),rkv(*),priv(*)
+      real w1(*),w2(*),w3(*),w4(*),w5(*),zph(*),mf(*)
+      character(len=5) fmois
+      iname(idtubopt)=2
+      iname(idisoopt)=1
+      fmois='dry  '
+      if(moist.ne.0) fmois='moist'
+      call s_bruntv(idcphopt,iddziv,idthresq,fmois,ni,nj,nk,jcb8w,       &
+     &              pbr,ptbr,ppp,ptpp,qvp,qallp,nsq8w,w1,w2,w3,w5)
+      call s_defomten(idtrnopt,idmpopt,idmfcopt,idiwest,idieast,         &
+     &                idjsouth,idjnorth,iddxiv,iddyiv,iddziv,ni,nj,nk,   &
+     &                zph,zph,jcb,jcb8u,jcb8v,jcb8w,mf,mf,mf,up,vp,wp,   &
+     &                t11,t22,t33,t12,t13,t23,t31,t32,w1,w2,w3,w4)
+      call s_defomssq(ni,nj,nk,t11,t22,t33,t12,t31,t32,ssq)
+      call s_eddyvis(idmpopt,idmfcopt,idsfcopt,idtubopt,idisoopt,        &
+     &               iddx,iddy,iddz,dtb,ni,nj,nk,zph,jcb,mf,rbr,         &
+     &               tkep,ssq,nsq8w,rkh,rkv,priv)
+      iname(idtubopt)=0
+      end subroutine ref_turbcoef
 
       end module cress_ref_harness

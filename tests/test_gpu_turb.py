@@ -1,0 +1,84 @@
+"""cress_turbcoef -- the coefficients of the 1.5-order TKE closure
+(turbdrv.f90:245-276: s_bruntv, s_defomten, s_defomssq, s_eddyvis with
+tubopt=2, isoopt=1, flat terrain without map factors, sfcopt=0) -- against
+the oracle's or_turbcoef, itself pinned bitwise to the reference's routines
+(tests/test_oracle_turb.py).  Outputs start from the same random contents,
+so what the kernels leave alone is checked too.  The deformation tensor,
+ssq and N^2 of dry or cloud-free air are bitwise; rkh, rkv, priv and N^2
+with cloud physics (exp and log in binary64 on the GPU, FP32 libm in the
+oracle) are within 1e-5 (max |difference| / max |value|) -- with cloud
+physics, the eddy viscosity stage on the GPU's own N^2 and ssq."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from cress_amd import host
+from oracle import pyoracle as po
+from turb_inputs import IN, OUT, inputs
+
+pytestmark = pytest.mark.gpu
+
+NI, NJ, NK = 70, 19, 24
+DTB = 6.0
+EXACT = ["t11", "t22", "t33", "t12", "t13", "t23", "t31", "t32", "ssq"]
+CLOSE = ["rkh", "rkv", "priv"]
+
+
+def _ptr(a):
+    assert a.flags["F_CONTIGUOUS"] and a.dtype == np.float32
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def rel(a, r):
+    a, r = np.asarray(a, np.float64), np.asarray(r, np.float64)
+    return np.abs(a - r).max() / max(np.abs(r).max(), 1e-30)
+
+
+@pytest.mark.parametrize("bbc", [2, 3])
+@pytest.mark.parametrize("wbc", [-1, 7])
+@pytest.mark.parametrize("moist,cphopt", [(0, 0), (1, 0), (1, 2)])
+def test_turbcoef_matches_oracle(moist, cphopt, wbc, bbc):
+    d, outs = inputs(NI, NJ, NK, 31 + cphopt + (wbc > 0) + 3 * bbc + 7 * moist)
+    gpu = {n: a.copy(order="F") for n, a in outs.items()}
+    ora = {n: a.copy(order="F") for n, a in outs.items()}
+    cfg = host.make_config(NI, NJ, NK, 700.0, 900.0, 350.0, DTB, 1.0, cphopt=cphopt, wbc=wbc, gwave=30.0)
+    cfg.bbc = cfg.tbc = bbc
+    cfg.thresq = 1.e-5
+    print("cress_turbcoef moist=%d cphopt=%d wbc=%d bbc=%d %dx%dx%d" % (moist, cphopt, wbc, bbc, NI, NJ, NK),
+          flush=True)
+    host.check(host.lib().cress_turbcoef(C.byref(cfg), C.c_float(DTB), moist, *[_ptr(d[n]) for n in IN],
+                                         *[_ptr(gpu[n]) for n in OUT]), "cress_turbcoef")
+    c = po.make_cfg(NI, NJ, NK, dx=700.0, dy=900.0, dz=350.0, wbc=wbc, cphopt=cphopt, bbc=bbc, tbc=bbc,
+                    thresq=1.e-5, gwave=30.0)
+    z = lambda: np.zeros((NI + 2, NJ + 2, NK), np.float32, order="F")
+    po.Oracle(32).or_turbcoef(c, DTB, moist, *[d[n] for n in IN], *[ora[n] for n in OUT], z(), z(), z(), z())
+    for n in EXACT:
+        np.testing.assert_array_equal(gpu[n], ora[n], err_msg=n)
+    if moist and cphopt:
+        assert rel(gpu["nsq8w"], ora["nsq8w"]) <= 1e-5
+        # eddyvis sums N^2 of two levels, which may nearly cancel; the
+        # mixing length then amplifies the ulps of cloudy N^2, as it does
+        # in the reference.  The stage is checked on the GPU's own N^2 and
+        # ssq.
+        for n in CLOSE:
+            ora[n] = outs[n].copy(order="F")
+        po.Oracle(32).or_eddyvis(c, DTB, d["jcb"], d["rbr"], d["tkep"], gpu["ssq"], gpu["nsq8w"], ora["rkh"],
+                                 ora["rkv"], ora["priv"])
+    else:
+        np.testing.assert_array_equal(gpu["nsq8w"], ora["nsq8w"])
+    for n in CLOSE:
+        assert rel(gpu[n], ora[n]) <= 1e-5, (n, rel(gpu[n], ora[n]))
+        untouched = np.ones(gpu[n].shape, bool)
+        untouched[1:NI, 1:NJ, 0:NK - 1] = False
+        np.testing.assert_array_equal(gpu[n][untouched], outs[n][untouched], err_msg=n)
+    assert not np.array_equal(gpu["rkv"], outs["rkv"])
+
+
+def test_turbcoef_rejects_bad_arguments():
+    cfg = host.make_config(NI, NJ, NK, 700.0, 900.0, 350.0, DTB, 1.0, cphopt=0)
+    d, outs = inputs(NI, NJ, NK, 3)
+    args = [_ptr(d[n]) for n in IN] + [_ptr(outs[n]) for n in OUT]
+    assert host.lib().cress_turbcoef(C.byref(cfg), C.c_float(0.0), 1, *args) == -1
+    cfg.wbc = 4
+    assert host.lib().cress_turbcoef(C.byref(cfg), C.c_float(DTB), 1, *args) == -6

@@ -1,0 +1,76 @@
+"""The coefficients of the 1.5-order TKE closure (tubopt=2, isoopt=1, flat
+terrain without map factors, sfcopt=0): the oracle's or_turbcoef
+(oracle/cress_turb.c) against the reference's own s_bruntv, s_defomten,
+s_defomssq and s_eddyvis called in turbdrv.f90's order, bitwise over whole
+arrays -- dry, moist without and with cloud physics, periodic and open
+sides, mirrored and copied bottom/top -- and against closed forms of the
+deformation tensor for a linear shear flow."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from turb_inputs import IN, OUT, inputs
+
+NI, NJ, NK = 13, 11, 14
+DTB = 6.0
+
+pytestmark = pytest.mark.skipif(not po.have_ref(), reason="oracle/_ref not built")
+
+
+@pytest.fixture(scope="module")
+def libs():
+    return po.Oracle(32), po.Ref()
+
+
+def cfg(wbc, cphopt, bbc):
+    return po.make_cfg(NI, NJ, NK, dx=700.0, dy=900.0, dz=350.0, wbc=wbc, cphopt=cphopt, bbc=bbc, tbc=bbc,
+                       thresq=1.e-5, gwave=30.0)
+
+
+def fld():
+    return np.zeros((NI + 2, NJ + 2, NK), np.float32, order="F")
+
+
+@pytest.mark.parametrize("bbc", [2, 3])
+@pytest.mark.parametrize("wbc", [-1, 7])
+@pytest.mark.parametrize("moist,cphopt", [(0, 0), (1, 0), (1, 2)])
+def test_turbcoef_matches_reference(libs, moist, cphopt, wbc, bbc):
+    o, r = libs
+    c = cfg(wbc, cphopt, bbc)
+    r.configure(c)
+    d, outs = inputs(NI, NJ, NK, 11 + cphopt + (wbc > 0) + 3 * bbc + 7 * moist)
+    mine = {n: a.copy(order="F") for n, a in outs.items()}
+    ref = {n: a.copy(order="F") for n, a in outs.items()}
+    o.or_turbcoef(c, DTB, moist, *[d[n] for n in IN], *[mine[n] for n in OUT], *[fld() for _ in range(4)])
+    r.ref_turbcoef(moist, DTB, NI, NJ, NK, *[d[n] for n in IN], *[ref[n] for n in OUT],
+                   *[fld() for _ in range(7)])
+    for n in OUT:
+        np.testing.assert_array_equal(mine[n], ref[n], err_msg=n)
+    I, J, Kk = slice(1, NI), slice(1, NJ), slice(0, NK - 1)
+    nsq = mine["nsq8w"][I, J, Kk] + mine["nsq8w"][I, J, 1:NK]
+    assert (nsq < 0).any() and (nsq > 0).any()
+    assert (mine["rkv"][I, J, Kk] > 0).any()
+
+
+def test_deformation_of_a_linear_shear(libs):
+    """u = a z, v = b x, w = 0, unit Jacobians: s13 = s31 = a and s12 = b
+    everywhere inside, s11 = s22 = s33 = 0, so ssq = a^2 + b^2 where the
+    four averaged neighbours are all inside."""
+    o, _ = libs
+    c = cfg(-1, 0, 3)
+    d, outs = inputs(NI, NJ, NK, 5)
+    a, b = np.float32(2.0 ** -6), np.float32(2.0 ** -7)
+    for n in ("jcb", "jcb8u", "jcb8v", "jcb8w"):
+        d[n][...] = 1.0
+    z = (np.arange(NK, dtype=np.float32) * np.float32(350.0))[None, None, :]
+    x = (np.arange(NI + 2, dtype=np.float32) * np.float32(700.0))[:, None, None]
+    d["up"][...] = a * z
+    d["vp"][...] = b * x
+    d["wp"][...] = 0.0
+    o.or_turbcoef(c, DTB, 0, *[d[n] for n in IN], *[outs[n] for n in OUT], *[fld() for _ in range(4)])
+    np.testing.assert_allclose(outs["t13"][2:NI - 1, 1:NJ, 1:NK - 1], a, rtol=1e-5)
+    np.testing.assert_allclose(outs["t31"][2:NI - 1, 1:NJ, 1:NK - 1], a, rtol=1e-5)
+    np.testing.assert_allclose(outs["t12"][1:NI + 1, 1:NJ + 1, 0:NK - 1], b, rtol=1e-5)
+    for n in ("t11", "t22", "t33"):
+        np.testing.assert_array_equal(outs[n][1:NI, 1:NJ, 0:NK - 1], 0.0)
+    np.testing.assert_allclose(outs["ssq"][2:NI - 1, 1:NJ - 1, 1:NK - 2], a * a + b * b, rtol=1e-5)
