@@ -1512,6 +1512,60 @@ int cress_turbcoef(const cress_config *cfg, float dtb, int moist, const float *j
   return CRESS_OK;
 }
 
+// s_turbdrv (tubopt=2) whole: the velocity forcings and the arrays it
+// returns (rkh, rkv, priv, ssq, nsq8w, t13, t23, t33) go to the device and
+// back; its work arrays t11, t22, t12, t31, t32 stay on the device.
+int cress_turbdrv(const cress_config *cfg, float dtb, int moist, const float *jcb, const float *jcb8u,
+                  const float *jcb8v, const float *jcb8w, const float *pbr, const float *ptbr, const float *rbr,
+                  const float *up, const float *vp, const float *wp, const float *ppp, const float *ptpp,
+                  const float *qvp, const float *tkep, const float *qallp, float *ufrc, float *vfrc, float *wfrc,
+                  float *rkh, float *rkv, float *priv, float *ssq, float *nsq8w, float *t13, float *t23, float *t33) {
+  const float *in[15] = {jcb, jcb8u, jcb8v, jcb8w, pbr, ptbr, rbr, ppp, ptpp, qvp, qallp, up, vp, wp, tkep};
+  float *out[11] = {ufrc, vfrc, wfrc, rkh, rkv, priv, ssq, nsq8w, t13, t23, t33};
+  for (const float *a : in)
+    if (!a) return CRESS_EINVAL;
+  for (float *a : out)
+    if (!a) return CRESS_EINVAL;
+  if (!cfg || cfg->ni < 4 || cfg->nj < 4 || cfg->nk < 4 || !(dtb > 0.0f) || !(cfg->dx > 0.0) ||
+      !(cfg->dy > 0.0) || !(cfg->dz > 0.0) || cfg->cphopt < 0)
+    return CRESS_EINVAL;
+  const int sides[4] = {cfg->wbc, cfg->ebc, cfg->sbc, cfg->nbc};
+  for (int s = 0; s < 4; s++)
+    if (sides[s] != -1 && sides[s] != 7) return CRESS_EUNSUPPORTED;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CRESS_ENODEV;
+  const Grid g = make_grid(cfg->ni, cfg->nj, cfg->nk);
+  const size_t n = (size_t)g.n();
+  DevBuf di[15], dout[11], dw[5];
+  int rc;
+  for (int f = 0; f < 15; f++)
+    if ((rc = to_dev(di[f], in[f], n))) return rc;
+  for (int f = 0; f < 11; f++)
+    if ((rc = to_dev(dout[f], out[f], n))) return rc;
+  for (int f = 0; f < 5; f++) {
+    if ((rc = to_dev(dw[f], nullptr, n))) return rc;
+    HIPCHK(hipMemset(dw[f].p, 0, n * sizeof(float)));
+  }
+  TurbArgs a = {};
+  a.moist = moist ? 1 : 0;
+  a.cphopt = cfg->cphopt;
+  a.wbc = cfg->wbc, a.ebc = cfg->ebc, a.sbc = cfg->sbc, a.nbc = cfg->nbc, a.bbc = cfg->bbc, a.tbc = cfg->tbc;
+  a.iw = cfg->wbc == -1 ? 0 : 1, a.ie = cfg->ebc == -1 ? 0 : 1;
+  a.js = cfg->sbc == -1 ? 0 : 1, a.jn = cfg->nbc == -1 ? 0 : 1;
+  a.dx = (float)cfg->dx, a.dy = (float)cfg->dy, a.dz = (float)cfg->dz, a.dtb = dtb, a.thresq = (float)cfg->thresq;
+  const float **ip[15] = {&a.jcb, &a.jcb8u, &a.jcb8v, &a.jcb8w, &a.pbr, &a.ptbr, &a.rbr, &a.pp,
+                          &a.ptp, &a.qv, &a.qall, &a.u, &a.v, &a.w, &a.tke};
+  for (int f = 0; f < 15; f++) *ip[f] = (const float *)di[f].p;
+  float **op[11] = {&a.ufrc, &a.vfrc, &a.wfrc, &a.rkh, &a.rkv, &a.priv, &a.ssq, &a.nsq8w, &a.s13, &a.s23, &a.s33};
+  for (int f = 0; f < 11; f++) *op[f] = (float *)dout[f].p;
+  float **wp5[5] = {&a.s11, &a.s22, &a.s12, &a.s31, &a.s32};
+  for (int f = 0; f < 5; f++) *wp5[f] = (float *)dw[f].p;
+  turb_drv(g, a);
+  HIPCHK(hipGetLastError());
+  for (int f = 0; f < 11; f++) HIPCHK(hipMemcpy(out[f], dout[f].p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return CRESS_OK;
+}
+
 // s_advp alone is the pressure forcing kernel without sponge or smoothing
 // (its forcing is the advection term only, written over the interior).
 int cress_advp(const cress_config *cfg, const float *jcb8u, const float *jcb8v, const float *jcb8w,

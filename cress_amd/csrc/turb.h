@@ -17,9 +17,15 @@ struct TurbArgs {
   const float *pp, *ptp, *qv, *qall, *u, *v, *w, *tke;   // past level
   float *s11, *s22, *s33, *s12, *s13, *s23, *s31, *s32;  // deformation tensor
   float *ssq, *nsq8w, *rkh, *rkv, *priv;
+  float *ufrc, *vfrc, *wfrc;                             // velocity forcings (turb_drv)
 };
 
 // Launches the kernels on the null stream, in the reference's order.
 void turb_coef(const Grid &g, const TurbArgs &a);
+
+// All of s_turbdrv (turbdrv.f90:245-287): turb_coef, then s_strsten
+// (sfcopt=0) and s_turbuvw (trnopt=0, mfcopt=0), which overwrite ufrc,
+// vfrc and wfrc over their interiors.
+void turb_drv(const Grid &g, const TurbArgs &a);
 
 }  // namespace cress

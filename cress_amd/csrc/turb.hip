@@ -188,4 +188,74 @@ void turb_coef(const Grid &g, const TurbArgs &a) {
   hipLaunchKernelGGL(k_eddyvis, dim3((g.ni - 1 + 63) / 64, g.nj - 1, g.nk - 1), blk, 0, 0, g, a);
 }
 
+// s_strsten (strsten.f90:150-210), sfcopt=0: the tensor times the eddy
+// viscosities, in place (each point reads only its own component).
+__global__ void k_strsten(Grid g, TurbArgs a) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y, k = 1 + blockIdx.z;
+  if (i > g.ni - 1) return;
+  const int ni = g.ni, nj = g.nj, nk = g.nk;
+  const float *h = a.rkh, *v = a.rkv;
+  AT(a.s11, i, j, k) = AT(h, i, j, k) * AT(a.s11, i, j, k);
+  AT(a.s22, i, j, k) = AT(h, i, j, k) * AT(a.s22, i, j, k);
+  AT(a.s33, i, j, k) = AT(v, i, j, k) * AT(a.s33, i, j, k);
+  if (i >= 2 && j >= 2)
+    AT(a.s12, i, j, k) = .25f * AT(a.s12, i, j, k) *
+                         ((AT(h, i - 1, j - 1, k) + AT(h, i, j, k)) + (AT(h, i - 1, j, k) + AT(h, i, j - 1, k)));
+  if (k < 2 || i < 2 || j < 2) return;
+  if (j <= nj - 2) {
+    AT(a.s13, i, j, k) = .25f * AT(a.s13, i, j, k) *
+                         ((AT(v, i - 1, j, k - 1) + AT(v, i, j, k)) + (AT(v, i - 1, j, k) + AT(v, i, j, k - 1)));
+    AT(a.s31, i, j, k) = .25f * AT(a.s31, i, j, k) *
+                         ((AT(h, i - 1, j, k - 1) + AT(h, i, j, k)) + (AT(h, i - 1, j, k) + AT(h, i, j, k - 1)));
+  }
+  if (i <= ni - 2) {
+    AT(a.s23, i, j, k) = .25f * AT(a.s23, i, j, k) *
+                         ((AT(v, i, j - 1, k - 1) + AT(v, i, j, k)) + (AT(v, i, j - 1, k) + AT(v, i, j, k - 1)));
+    AT(a.s32, i, j, k) = .25f * AT(a.s32, i, j, k) *
+                         ((AT(h, i, j - 1, k - 1) + AT(h, i, j, k)) + (AT(h, i, j - 1, k) + AT(h, i, j, k - 1)));
+  }
+  (void)nk;
+}
+
+// s_turbuvw (turbuvw.f90:288-352, 795-859, 1300-1330), trnopt=0, mfcopt=0,
+// the leapfrog branch: the stress divergence overwrites the forcings.  The
+// reference's rescaled t11 / t22 and its tmp1 products are formed where
+// used (the same single products).
+__global__ void k_turbuvw(Grid g, TurbArgs a) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x, j = 2 + blockIdx.y, k = 2 + blockIdx.z;
+  if (i > g.ni - 1) return;
+  const int ni = g.ni, nj = g.nj, nk = g.nk;
+  const float dxiv = 1.f / a.dx, dyiv = 1.f / a.dy, dziv = 1.f / a.dz;
+  const float dxiv05 = .5f * dxiv, dyiv05 = .5f * dyiv;
+#define TU(i, j, k) ((AT(a.jcb8u, i, j - 1, k) + AT(a.jcb8u, i, j, k)) * AT(a.s12, i, j, k))
+#define TV(i, j, k) ((AT(a.jcb8v, i - 1, j, k) + AT(a.jcb8v, i, j, k)) * AT(a.s12, i, j, k))
+#define TW1(i, j, k) ((AT(a.jcb8u, i, j, k - 1) + AT(a.jcb8u, i, j, k)) * AT(a.s31, i, j, k))
+#define TW2(i, j, k) ((AT(a.jcb8v, i, j, k - 1) + AT(a.jcb8v, i, j, k)) * AT(a.s32, i, j, k))
+  if (k <= nk - 2 && j <= nj - 2) {
+    const float t0 = AT(a.jcb, i, j, k) * AT(a.s11, i, j, k), tm = AT(a.jcb, i - 1, j, k) * AT(a.s11, i - 1, j, k);
+    AT(a.ufrc, i, j, k) = ((t0 - tm) * dxiv + (TU(i, j + 1, k) - TU(i, j, k)) * dyiv05) +
+                          (AT(a.s13, i, j, k + 1) - AT(a.s13, i, j, k)) * dziv;
+  }
+  if (k <= nk - 2 && i <= ni - 2) {
+    const float t0 = AT(a.jcb, i, j, k) * AT(a.s22, i, j, k), tm = AT(a.jcb, i, j - 1, k) * AT(a.s22, i, j - 1, k);
+    AT(a.vfrc, i, j, k) = ((t0 - tm) * dyiv + (TV(i + 1, j, k) - TV(i, j, k)) * dxiv05) +
+                          (AT(a.s23, i, j, k + 1) - AT(a.s23, i, j, k)) * dziv;
+  }
+  if (j <= nj - 2 && i <= ni - 2)
+    AT(a.wfrc, i, j, k) = (AT(a.s33, i, j, k) - AT(a.s33, i, j, k - 1)) * dziv +
+                          ((TW1(i + 1, j, k) - TW1(i, j, k)) * dxiv05 + (TW2(i, j + 1, k) - TW2(i, j, k)) * dyiv05);
+#undef TU
+#undef TV
+#undef TW1
+#undef TW2
+}
+
+void turb_drv(const Grid &g, const TurbArgs &a) {
+  turb_coef(g, a);
+  const dim3 blk(64);
+  hipLaunchKernelGGL(k_strsten, dim3((g.ni - 1 + 63) / 64, g.nj - 1, g.nk - 1), blk, 0, 0, g, a);
+  hipLaunchKernelGGL(k_bcten, dim3((unsigned)((g.nxy + 63) / 64), 2), blk, 0, 0, g, a.bbc, a.tbc, a.s31, a.s32);
+  hipLaunchKernelGGL(k_turbuvw, dim3((g.ni - 2 + 63) / 64, g.nj - 2, g.nk - 2), blk, 0, 0, g, a);
+}
+
 }  // namespace cress

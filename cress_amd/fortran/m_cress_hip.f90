@@ -59,6 +59,7 @@ module m_cress_hip
   public :: cress_set_timing, cress_stage_times
   public :: cress_gaussel, cress_advs, cress_bcycle, cress_advuvw, cress_advp
   public :: cress_hevi, cress_steps, cress_cloudphy, cress_turbcoef
+  public :: cress_turbdrv
   public :: cress_bc8u, cress_bc8v, cress_bc8w
   public :: cress_errmsg, base_cfg
 
@@ -250,6 +251,25 @@ module m_cress_hip
       real(c_float), intent(inout) :: priv(*)
       integer(c_int) :: cress_turbcoef
     end function cress_turbcoef
+
+! tier 2: s_turbdrv (turbdrv.f90:80-85), tubopt=2
+    function cress_turbdrv(cfg, dtb, moist, jcb, jcb8u, jcb8v, jcb8w,    &
+     &  pbr, ptbr, rbr, up, vp, wp, ppp, ptpp, qvp, tkep, qallp, ufrc,    &
+     &  vfrc, wfrc, rkh, rkv, priv, ssq, nsq8w, t13, t23, t33)            &
+     &  bind(c, name='cress_turbdrv')
+      import :: c_int, c_float, cress_config
+      type(cress_config), intent(in) :: cfg
+      real(c_float), value :: dtb
+      integer(c_int), value :: moist
+      real(c_float), intent(in) :: jcb(*), jcb8u(*), jcb8v(*), jcb8w(*)
+      real(c_float), intent(in) :: pbr(*), ptbr(*), rbr(*), up(*), vp(*)
+      real(c_float), intent(in) :: wp(*), ppp(*), ptpp(*), qvp(*), tkep(*)
+      real(c_float), intent(in) :: qallp(*)
+      real(c_float), intent(inout) :: ufrc(*), vfrc(*), wfrc(*), rkh(*)
+      real(c_float), intent(inout) :: rkv(*), priv(*), ssq(*), nsq8w(*)
+      real(c_float), intent(inout) :: t13(*), t23(*), t33(*)
+      integer(c_int) :: cress_turbdrv
+    end function cress_turbdrv
 
 ! tier 2: s_advp (advp.f90:75-79), mfcopt=0, diaopt=0
     function cress_advp(cfg, jcb8u, jcb8v, jcb8w, u, v, wc, pp, pfrc)    &

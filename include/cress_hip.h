@@ -239,6 +239,22 @@ int cress_turbcoef(const cress_config *cfg, float dtb, int moist, const float *j
                    float *t12, float *t13, float *t23, float *t31, float *t32, float *ssq, float *nsq8w, float *rkh,
                    float *rkv, float *priv);
 
+/* s_turbdrv(fmois,dtb,ni,nj,nk,zph,j31,j32,jcb,jcb8u,jcb8v,jcb8w,mf,mf8u,
+ *           mf8v,rmf,rmf8u,rmf8v,pbr,ptbr,rbr,up,vp,wp,ppp,ptpp,qvp,tkep,qallp,
+ *           ufrc,vfrc,wfrc,rkh,rkv,priv,ssq,nsq8w,t13,t23,t33,t11,t22,t12,t31,
+ *           t32,tmp1)  SrcOrig/turbdrv.f90:80-85, for the options of
+ * cress_turbcoef, then s_strsten (sfcopt=0) and s_turbuvw (trnopt=0,
+ * mfcopt=0): ufrc, vfrc, wfrc are overwritten with the stress divergence
+ * over their interiors; rkh, rkv, priv, ssq, nsq8w, t13, t23, t33 are
+ * returned as the reference returns them (t11, t22, t12, t31, t32 and tmp1
+ * are the reference's work arrays and are not taken).  The forcings are
+ * within 1e-5 of the FP32 reference (they carry the eddy viscosity). */
+int cress_turbdrv(const cress_config *cfg, float dtb, int moist, const float *jcb, const float *jcb8u,
+                  const float *jcb8v, const float *jcb8w, const float *pbr, const float *ptbr, const float *rbr,
+                  const float *up, const float *vp, const float *wp, const float *ppp, const float *ptpp,
+                  const float *qvp, const float *tkep, const float *qallp, float *ufrc, float *vfrc, float *wfrc,
+                  float *rkh, float *rkv, float *priv, float *ssq, float *nsq8w, float *t13, float *t23, float *t33);
+
 /* s_advp(fpadvopt,fpmpopt,fpmfcopt,fpdiaopt,fpiwest,...,fpdziv,ni,nj,nk,
  *        mf8u,mf8v,jcb8u,jcb8v,jcb8w,u,v,wc,pp,pfrc,jcbxu,jcbxv,jcbxwc,
  *        hadv,vadv,tmp1,tmp2,tmp3)  SrcOrig/advp.f90:75-79, mfcopt=0 and

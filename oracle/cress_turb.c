@@ -205,3 +205,106 @@ void or_turbcoef(const or_cfg *c, R dtb, int moist, const R *jcb, const R *jcb8u
   or_defomssq(c, t11, t22, t33, t12, t31, t32, ssq);
   or_eddyvis(c, dtb, jcb, rbr, tkep, ssq, nsq8w, rkh, rkv, priv);
 }
+
+/* s_strsten (strsten.f90:150-210), sfcopt=0: the tensor times the eddy
+ * viscosities, in place; then s_bcten on t31 and t32. */
+void or_strsten(const or_cfg *c, const R *rkh, const R *rkv, R *t11, R *t22, R *t33, R *t12, R *t13, R *t23,
+                R *t31, R *t32)
+{
+  DECL_DIMS(c);
+  for (int k = 1; k <= nk - 1; k++) {
+    for (int j = 1; j <= nj - 1; j++)
+      for (int i = 1; i <= ni - 1; i++) {
+        A3(t11, i, j, k) = A3(rkh, i, j, k) * A3(t11, i, j, k);
+        A3(t22, i, j, k) = A3(rkh, i, j, k) * A3(t22, i, j, k);
+        A3(t33, i, j, k) = A3(rkv, i, j, k) * A3(t33, i, j, k);
+      }
+    for (int j = 2; j <= nj - 1; j++)
+      for (int i = 2; i <= ni - 1; i++)
+        A3(t12, i, j, k) = K(.25) * A3(t12, i, j, k) *
+                           ((A3(rkh, i - 1, j - 1, k) + A3(rkh, i, j, k)) + (A3(rkh, i - 1, j, k) + A3(rkh, i, j - 1, k)));
+  }
+  for (int k = 2; k <= nk - 1; k++) {
+    for (int j = 2; j <= nj - 2; j++)
+      for (int i = 2; i <= ni - 1; i++)
+        A3(t13, i, j, k) = K(.25) * A3(t13, i, j, k) *
+                           ((A3(rkv, i - 1, j, k - 1) + A3(rkv, i, j, k)) + (A3(rkv, i - 1, j, k) + A3(rkv, i, j, k - 1)));
+    for (int j = 2; j <= nj - 1; j++)
+      for (int i = 2; i <= ni - 2; i++)
+        A3(t23, i, j, k) = K(.25) * A3(t23, i, j, k) *
+                           ((A3(rkv, i, j - 1, k - 1) + A3(rkv, i, j, k)) + (A3(rkv, i, j - 1, k) + A3(rkv, i, j, k - 1)));
+  }
+  for (int k = 2; k <= nk - 1; k++) {
+    for (int j = 2; j <= nj - 2; j++)
+      for (int i = 2; i <= ni - 1; i++)
+        A3(t31, i, j, k) = K(.25) * A3(t31, i, j, k) *
+                           ((A3(rkh, i - 1, j, k - 1) + A3(rkh, i, j, k)) + (A3(rkh, i - 1, j, k) + A3(rkh, i, j, k - 1)));
+    for (int j = 2; j <= nj - 1; j++)
+      for (int i = 2; i <= ni - 2; i++)
+        A3(t32, i, j, k) = K(.25) * A3(t32, i, j, k) *
+                           ((A3(rkh, i, j - 1, k - 1) + A3(rkh, i, j, k)) + (A3(rkh, i, j - 1, k) + A3(rkh, i, j, k - 1)));
+  }
+  or_bcten(c, t31);
+  or_bcten(c, t32);
+}
+
+/* s_turbuvw (turbuvw.f90:288-352, 795-859, 1300-1330), trnopt=0,
+ * mfcopt=0, called with idoneopt (=1, so the leapfrog branch): the
+ * divergence of the stresses overwrites ufrc, vfrc, wfrc over their
+ * interiors.  t11 and t22 are rescaled in place as in the reference; the
+ * products the reference keeps in tmp1 are formed where used. */
+void or_turbuvw(const or_cfg *c, const R *jcb, const R *jcb8u, const R *jcb8v, R *t11, R *t22, const R *t33,
+                const R *t12, const R *t13, const R *t23, const R *t31, const R *t32, R *ufrc, R *vfrc, R *wfrc)
+{
+  DECL_DIMS(c);
+  const R dxiv = K(1.0) / (R)c->dx, dyiv = K(1.0) / (R)c->dy, dziv = K(1.0) / (R)c->dz;
+  const R dxiv05 = K(.5) * dxiv, dyiv05 = K(.5) * dyiv;
+#define TU(i, j, k) ((A3(jcb8u, i, j - 1, k) + A3(jcb8u, i, j, k)) * A3(t12, i, j, k))
+#define TV(i, j, k) ((A3(jcb8v, i - 1, j, k) + A3(jcb8v, i, j, k)) * A3(t12, i, j, k))
+#define TW1(i, j, k) ((A3(jcb8u, i, j, k - 1) + A3(jcb8u, i, j, k)) * A3(t31, i, j, k))
+#define TW2(i, j, k) ((A3(jcb8v, i, j, k - 1) + A3(jcb8v, i, j, k)) * A3(t32, i, j, k))
+  for (int k = 2; k <= nk - 2; k++)
+    for (int j = 2; j <= nj - 2; j++)
+      for (int i = 1; i <= ni - 1; i++) A3(t11, i, j, k) = A3(jcb, i, j, k) * A3(t11, i, j, k);
+  for (int k = 2; k <= nk - 2; k++)
+    for (int j = 2; j <= nj - 2; j++)
+      for (int i = 2; i <= ni - 1; i++)
+        A3(ufrc, i, j, k) = ((A3(t11, i, j, k) - A3(t11, i - 1, j, k)) * dxiv + (TU(i, j + 1, k) - TU(i, j, k)) * dyiv05) +
+                            (A3(t13, i, j, k + 1) - A3(t13, i, j, k)) * dziv;
+  for (int k = 2; k <= nk - 2; k++)
+    for (int j = 1; j <= nj - 1; j++)
+      for (int i = 2; i <= ni - 2; i++) A3(t22, i, j, k) = A3(jcb, i, j, k) * A3(t22, i, j, k);
+  for (int k = 2; k <= nk - 2; k++)
+    for (int j = 2; j <= nj - 1; j++)
+      for (int i = 2; i <= ni - 2; i++)
+        A3(vfrc, i, j, k) = ((A3(t22, i, j, k) - A3(t22, i, j - 1, k)) * dyiv + (TV(i + 1, j, k) - TV(i, j, k)) * dxiv05) +
+                            (A3(t23, i, j, k + 1) - A3(t23, i, j, k)) * dziv;
+  for (int k = 2; k <= nk - 1; k++) {
+    for (int j = 2; j <= nj - 2; j++)
+      for (int i = 2; i <= ni - 1; i++) A3(t11, i, j, k) = TW1(i, j, k);
+    for (int j = 2; j <= nj - 1; j++)
+      for (int i = 2; i <= ni - 2; i++) A3(t22, i, j, k) = TW2(i, j, k);
+  }
+  for (int k = 2; k <= nk - 1; k++)
+    for (int j = 2; j <= nj - 2; j++)
+      for (int i = 2; i <= ni - 2; i++)
+        A3(wfrc, i, j, k) = (A3(t33, i, j, k) - A3(t33, i, j, k - 1)) * dziv +
+                            ((A3(t11, i + 1, j, k) - A3(t11, i, j, k)) * dxiv05 + (A3(t22, i, j + 1, k) - A3(t22, i, j, k)) * dyiv05);
+#undef TU
+#undef TV
+#undef TW1
+#undef TW2
+}
+
+/* s_turbdrv (turbdrv.f90:245-287) for the options above. */
+void or_turbdrv(const or_cfg *c, R dtb, int moist, const R *jcb, const R *jcb8u, const R *jcb8v, const R *jcb8w,
+                const R *pbr, const R *ptbr, const R *rbr, const R *ppp, const R *ptpp, const R *qvp, const R *qallp,
+                const R *up, const R *vp, const R *wp, const R *tkep, R *ufrc, R *vfrc, R *wfrc, R *rkh, R *rkv,
+                R *priv, R *ssq, R *nsq8w, R *t13, R *t23, R *t33, R *t11, R *t22, R *t12, R *t31, R *t32, R *w1,
+                R *w2, R *w3, R *w4)
+{
+  or_turbcoef(c, dtb, moist, jcb, jcb8u, jcb8v, jcb8w, pbr, ptbr, rbr, ppp, ptpp, qvp, qallp, up, vp, wp, tkep, t11,
+              t22, t33, t12, t13, t23, t31, t32, ssq, nsq8w, rkh, rkv, priv, w1, w2, w3, w4);
+  or_strsten(c, rkh, rkv, t11, t22, t33, t12, t13, t23, t31, t32);
+  or_turbuvw(c, jcb, jcb8u, jcb8v, t11, t22, t33, t12, t13, t23, t31, t32, ufrc, vfrc, wfrc);
+}

@@ -81,6 +81,7 @@
       use m_defomten
       use m_defomssq
       use m_eddyvis
+      use m_turbdrv
 
       implicit none
 
@@ -719,5 +720,29 @@
      &               tkep,ssq,nsq8w,rkh,rkv,priv)
       iname(idtubopt)=0
       end subroutine ref_turbcoef
+
+! s_turbdrv itself (turbdrv.f90:80-85), tubopt=2, isoopt=1.
+      subroutine ref_turbdrv(moist,dtb,ni,nj,nk,jcb,jcb8u,jcb8v,jcb8w,   &
+     &   pbr,ptbr,rbr,up,vp,wp,ppp,ptpp,qvp,tkep,qallp,ufrc,vfrc,wfrc,    &
+     &   rkh,rkv,priv,ssq,nsq8w,t13,t23,t33,t11,t22,t12,t31,t32,w1,zph,   &
+     &   mf) bind(C,name='ref_turbdrv')
+      integer(c_int), value :: moist,ni,nj,nk
+      real, value :: dtb
+      real jcb(*),jcb8u(*),jcb8v(*),jcb8w(*),pbr(*),ptbr(*),rbr(*)
+      real up(*),vp(*),wp(*),ppp(*),ptpp(*),qvp(*),tkep(*),qallp(*)
+      real ufrc(*),vfrc(*),wfrc(*),rkh(*),rkv(*),priv(*),ssq(*),nsq8w(*)
+      real t13(*),t23(*),t33(*),t11(*),t22(*),t12(*),t31(*),t32(*)
+      real w1(*),zph(*),mf(*)
+      character(len=5) fmois
+      iname(idtubopt)=2
+      iname(idisoopt)=1
+      fmois='dry  '
+      if(moist.ne.0) fmois='moist'
+      call s_turbdrv(fmois,dtb,ni,nj,nk,zph,zph,zph,jcb,jcb8u,jcb8v,     &
+     &               jcb8w,mf,mf,mf,mf,mf,mf,pbr,ptbr,rbr,up,vp,wp,ppp,  &
+     &               ptpp,qvp,tkep,qallp,ufrc,vfrc,wfrc,rkh,rkv,priv,    &
+     &               ssq,nsq8w,t13,t23,t33,t11,t22,t12,t31,t32,w1)
+      iname(idtubopt)=0
+      end subroutine ref_turbdrv
 
       end module cress_ref_harness

@@ -82,3 +82,53 @@ def test_turbcoef_rejects_bad_arguments():
     assert host.lib().cress_turbcoef(C.byref(cfg), C.c_float(0.0), 1, *args) == -1
     cfg.wbc = 4
     assert host.lib().cress_turbcoef(C.byref(cfg), C.c_float(DTB), 1, *args) == -6
+
+
+DRV_IN = ["jcb", "jcb8u", "jcb8v", "jcb8w", "pbr", "ptbr", "rbr", "up", "vp", "wp", "ppp", "ptpp", "qvp", "tkep",
+          "qallp"]
+DRV_OUT = ["ufrc", "vfrc", "wfrc", "rkh", "rkv", "priv", "ssq", "nsq8w", "t13", "t23", "t33"]
+
+
+@pytest.mark.parametrize("wbc", [-1, 7])
+@pytest.mark.parametrize("moist,cphopt", [(0, 0), (1, 0), (1, 2)])
+def test_turbdrv_matches_oracle(moist, cphopt, wbc):
+    """cress_turbdrv (all of s_turbdrv).  The coefficients as above; the
+    stress tensor scaled by strsten and the velocity forcings of turbuvw
+    bitwise equal to the oracle's strsten and turbuvw run on the oracle's
+    tensor with the GPU's eddy viscosities; the forcings of the oracle's
+    whole chain within 1e-5 where no cloud physics is on."""
+    d, outs = inputs(NI, NJ, NK, 61 + cphopt + (wbc > 0) + 7 * moist)
+    rng = np.random.default_rng(5)
+    for n in ("ufrc", "vfrc", "wfrc"):
+        outs[n] = np.asfortranarray(rng.uniform(-1.0, 1.0, (NI + 2, NJ + 2, NK)).astype(np.float32))
+    gpu = {n: outs[n].copy(order="F") for n in DRV_OUT}
+    cfg = host.make_config(NI, NJ, NK, 700.0, 900.0, 350.0, DTB, 1.0, cphopt=cphopt, wbc=wbc, gwave=30.0)
+    cfg.thresq = 1.e-5
+    print("cress_turbdrv moist=%d cphopt=%d wbc=%d %dx%dx%d" % (moist, cphopt, wbc, NI, NJ, NK), flush=True)
+    host.check(host.lib().cress_turbdrv(C.byref(cfg), C.c_float(DTB), moist, *[_ptr(d[n]) for n in DRV_IN],
+                                        *[_ptr(gpu[n]) for n in DRV_OUT]), "cress_turbdrv")
+    c = po.make_cfg(NI, NJ, NK, dx=700.0, dy=900.0, dz=350.0, wbc=wbc, cphopt=cphopt, thresq=1.e-5, gwave=30.0)
+    o = po.Oracle(32)
+    z = lambda: np.zeros((NI + 2, NJ + 2, NK), np.float32, order="F")
+    # stage check: the oracle's tensor, the GPU's eddy viscosities
+    t = {n: outs[n].copy(order="F") for n in OUT}
+    o.or_turbcoef(c, DTB, moist, *[d[n] for n in IN], *[t[n] for n in OUT], z(), z(), z(), z())
+    np.testing.assert_array_equal(gpu["ssq"][1:NI, 1:NJ, 0:NK - 1], t["ssq"][1:NI, 1:NJ, 0:NK - 1])
+    f = {n: outs[n].copy(order="F") for n in ("ufrc", "vfrc", "wfrc")}
+    o.or_strsten(c, gpu["rkh"], gpu["rkv"], *[t[n] for n in ("t11", "t22", "t33", "t12", "t13", "t23", "t31", "t32")])
+    o.or_turbuvw(c, d["jcb"], d["jcb8u"], d["jcb8v"],
+                 *[t[n] for n in ("t11", "t22", "t33", "t12", "t13", "t23", "t31", "t32")], f["ufrc"], f["vfrc"],
+                 f["wfrc"])
+    for n in ("ufrc", "vfrc", "wfrc"):
+        np.testing.assert_array_equal(gpu[n], f[n], err_msg=n)
+    for n in ("t13", "t23", "t33"):
+        np.testing.assert_array_equal(gpu[n], t[n], err_msg=n)
+    if not (moist and cphopt):
+        full = {n: outs[n].copy(order="F") for n in DRV_OUT}
+        w = {n: outs[n].copy(order="F") for n in ("t11", "t22", "t12", "t31", "t32")}
+        o.or_turbdrv(c, DTB, moist, *[d[n] for n in IN], *[full[n] for n in DRV_OUT],
+                     *[w[n] for n in ("t11", "t22", "t12", "t31", "t32")], z(), z(), z(), z())
+        for n in ("ufrc", "vfrc", "wfrc"):
+            assert rel(gpu[n], full[n]) <= 1e-5, (n, rel(gpu[n], full[n]))
+    for n in ("ufrc", "vfrc", "wfrc"):
+        assert not np.array_equal(gpu[n], outs[n])

@@ -74,3 +74,42 @@ def test_deformation_of_a_linear_shear(libs):
     for n in ("t11", "t22", "t33"):
         np.testing.assert_array_equal(outs[n][1:NI, 1:NJ, 0:NK - 1], 0.0)
     np.testing.assert_allclose(outs["ssq"][2:NI - 1, 1:NJ - 1, 1:NK - 2], a * a + b * b, rtol=1e-5)
+
+
+DRV_OUT = ["ufrc", "vfrc", "wfrc", "rkh", "rkv", "priv", "ssq", "nsq8w", "t13", "t23", "t33", "t11", "t22", "t12",
+           "t31", "t32"]
+INTERIOR = ["rkh", "rkv", "priv", "ssq"]   # the reference's work arrays elsewhere
+
+
+def drv_inputs(seed):
+    d, outs = inputs(NI, NJ, NK, seed)
+    rng = np.random.default_rng(seed + 1000)
+    for n in ("ufrc", "vfrc", "wfrc"):
+        outs[n] = np.asfortranarray(rng.uniform(-1.0, 1.0, (NI + 2, NJ + 2, NK)).astype(np.float32))
+    return d, outs
+
+
+@pytest.mark.parametrize("bbc", [2, 3])
+@pytest.mark.parametrize("wbc", [-1, 7])
+@pytest.mark.parametrize("moist,cphopt", [(0, 0), (1, 0), (1, 2)])
+def test_turbdrv_matches_reference(libs, moist, cphopt, wbc, bbc):
+    """The whole of s_turbdrv (coefficients, strsten, turbuvw) against the
+    reference's own s_turbdrv: the velocity forcings and every array it
+    returns, bitwise."""
+    o, r = libs
+    c = cfg(wbc, cphopt, bbc)
+    r.configure(c)
+    d, outs = drv_inputs(21 + cphopt + (wbc > 0) + 3 * bbc + 7 * moist)
+    mine = {n: a.copy(order="F") for n, a in outs.items()}
+    ref = {n: a.copy(order="F") for n, a in outs.items()}
+    o.or_turbdrv(c, DTB, moist, *[d[n] for n in IN], *[mine[n] for n in DRV_OUT], *[fld() for _ in range(4)])
+    r.ref_turbdrv(moist, DTB, NI, NJ, NK, *[d[n] for n in ("jcb", "jcb8u", "jcb8v", "jcb8w", "pbr", "ptbr", "rbr",
+                                                          "up", "vp", "wp", "ppp", "ptpp", "qvp", "tkep", "qallp")],
+                  *[ref[n] for n in DRV_OUT], *[fld() for _ in range(3)])
+    for n in DRV_OUT:
+        if n in INTERIOR:
+            np.testing.assert_array_equal(mine[n][1:NI, 1:NJ, 0:NK - 1], ref[n][1:NI, 1:NJ, 0:NK - 1], err_msg=n)
+        else:
+            np.testing.assert_array_equal(mine[n], ref[n], err_msg=n)
+    for n in ("ufrc", "vfrc", "wfrc"):
+        assert not np.array_equal(mine[n], outs[n])
