@@ -1566,6 +1566,36 @@ int cress_turbdrv(const cress_config *cfg, float dtb, int moist, const float *jc
   return CRESS_OK;
 }
 
+// s_turbflx for one scalar: the fluxes come back over the ranges the
+// reference writes, the rest of h1, h2, h3 as passed.
+int cress_turbflx(const cress_config *cfg, const float *jcb, const float *s, const float *rkh8u,
+                  const float *rkh8v, const float *rkv8w, float *h1, float *h2, float *h3) {
+  const float *in[5] = {jcb, s, rkh8u, rkh8v, rkv8w};
+  float *out[3] = {h1, h2, h3};
+  for (const float *a : in)
+    if (!a) return CRESS_EINVAL;
+  for (float *a : out)
+    if (!a) return CRESS_EINVAL;
+  if (!cfg || cfg->ni < 4 || cfg->nj < 4 || cfg->nk < 3 || !(cfg->dx > 0.0) || !(cfg->dy > 0.0) || !(cfg->dz > 0.0))
+    return CRESS_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CRESS_ENODEV;
+  const Grid g = make_grid(cfg->ni, cfg->nj, cfg->nk);
+  const size_t n = (size_t)g.n();
+  DevBuf di[5], dout[3];
+  int rc;
+  for (int f = 0; f < 5; f++)
+    if ((rc = to_dev(di[f], in[f], n))) return rc;
+  for (int f = 0; f < 3; f++)
+    if ((rc = to_dev(dout[f], out[f], n))) return rc;
+  turb_flx(g, (float)cfg->dx, (float)cfg->dy, (float)cfg->dz, (const float *)di[0].p, (const float *)di[1].p,
+           (const float *)di[2].p, (const float *)di[3].p, (const float *)di[4].p, (float *)dout[0].p,
+           (float *)dout[1].p, (float *)dout[2].p);
+  HIPCHK(hipGetLastError());
+  for (int f = 0; f < 3; f++) HIPCHK(hipMemcpy(out[f], dout[f].p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return CRESS_OK;
+}
+
 // s_advp alone is the pressure forcing kernel without sponge or smoothing
 // (its forcing is the advection term only, written over the interior).
 int cress_advp(const cress_config *cfg, const float *jcb8u, const float *jcb8v, const float *jcb8w,

@@ -28,4 +28,8 @@ void turb_coef(const Grid &g, const TurbArgs &a);
 // vfrc and wfrc over their interiors.
 void turb_drv(const Grid &g, const TurbArgs &a);
 
+// s_turbflx (turbflx.f90:72-74), trnopt=0, sfcopt=0.
+void turb_flx(const Grid &g, float dx, float dy, float dz, const float *jcb, const float *s, const float *kh8u,
+              const float *kh8v, const float *kv8w, float *h1, float *h2, float *h3);
+
 }  // namespace cress

@@ -258,4 +258,27 @@ void turb_drv(const Grid &g, const TurbArgs &a) {
   hipLaunchKernelGGL(k_turbuvw, dim3((g.ni - 2 + 63) / 64, g.nj - 2, g.nk - 2), blk, 0, 0, g, a);
 }
 
+// s_turbflx (turbflx.f90:180-262), trnopt=0, sfcopt=0: the turbulent
+// fluxes of one scalar at u, v and w points over the reference's ranges.
+__global__ void k_turbflx(Grid g, float dx, float dy, float dz, const float *jcb, const float *s,
+                          const float *kh8u, const float *kh8v, const float *kv8w, float *h1, float *h2, float *h3) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x, j = 2 + blockIdx.y, k = 1 + blockIdx.z;
+  if (i > g.ni - 1) return;
+  const int ni = g.ni, nj = g.nj, nk = g.nk;
+  const float dxiv = 1.f / dx, dyiv = 1.f / dy, dziv = 1.f / dz;
+  const float dxv05 = .5f * dxiv, dyv05 = .5f * dyiv;
+  const float js = AT(s, i, j, k) * AT(jcb, i, j, k);
+  if (j <= nj - 2) AT(h1, i, j, k) = AT(kh8u, i, j, k) * (js - AT(s, i - 1, j, k) * AT(jcb, i - 1, j, k)) * dxv05;
+  if (i <= ni - 2) AT(h2, i, j, k) = AT(kh8v, i, j, k) * (js - AT(s, i, j - 1, k) * AT(jcb, i, j - 1, k)) * dyv05;
+  if (k >= 2 && i <= ni - 2 && j <= nj - 2)
+    AT(h3, i, j, k) = AT(kv8w, i, j, k) * (AT(s, i, j, k) - AT(s, i, j, k - 1)) * dziv;
+  (void)nk;
+}
+
+void turb_flx(const Grid &g, float dx, float dy, float dz, const float *jcb, const float *s, const float *kh8u,
+              const float *kh8v, const float *kv8w, float *h1, float *h2, float *h3) {
+  hipLaunchKernelGGL(k_turbflx, dim3((g.ni - 2 + 63) / 64, g.nj - 2, g.nk - 1), dim3(64), 0, 0, g, dx, dy, dz, jcb,
+                     s, kh8u, kh8v, kv8w, h1, h2, h3);
+}
+
 }  // namespace cress

@@ -255,6 +255,14 @@ int cress_turbdrv(const cress_config *cfg, float dtb, int moist, const float *jc
                   const float *qvp, const float *tkep, const float *qallp, float *ufrc, float *vfrc, float *wfrc,
                   float *rkh, float *rkv, float *priv, float *ssq, float *nsq8w, float *t13, float *t23, float *t33);
 
+/* s_turbflx(fptrnopt,fpsfcopt,fpdxiv,fpdyiv,fpdziv,ni,nj,nk,j31,j32,jcb,s,
+ *           sfrc,rkh8u,rkh8v,rkv8w,h1,h2,h3,j31s,j32s,jcbs)
+ * SrcOrig/turbflx.f90:72-74, trnopt=0 and sfcopt=0 (j31, j32, sfrc and the
+ * work arrays are not read): the turbulent fluxes h1, h2, h3 of the scalar
+ * s at u, v, w points, written where the reference writes them; bitwise. */
+int cress_turbflx(const cress_config *cfg, const float *jcb, const float *s, const float *rkh8u,
+                  const float *rkh8v, const float *rkv8w, float *h1, float *h2, float *h3);
+
 /* s_advp(fpadvopt,fpmpopt,fpmfcopt,fpdiaopt,fpiwest,...,fpdziv,ni,nj,nk,
  *        mf8u,mf8v,jcb8u,jcb8v,jcb8w,u,v,wc,pp,pfrc,jcbxu,jcbxv,jcbxwc,
  *        hadv,vadv,tmp1,tmp2,tmp3)  SrcOrig/advp.f90:75-79, mfcopt=0 and

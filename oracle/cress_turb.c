@@ -308,3 +308,28 @@ void or_turbdrv(const or_cfg *c, R dtb, int moist, const R *jcb, const R *jcb8u,
   or_strsten(c, rkh, rkv, t11, t22, t33, t12, t13, t23, t31, t32);
   or_turbuvw(c, jcb, jcb8u, jcb8v, t11, t22, t33, t12, t13, t23, t31, t32, ufrc, vfrc, wfrc);
 }
+
+/* s_turbflx (turbflx.f90:180-262), trnopt=0, sfcopt=0: the turbulent
+ * fluxes of a scalar s at u, v and w points.  The reference's jcbs
+ * (s*jcb) is formed where used. */
+void or_turbflx(const or_cfg *c, const R *jcb, const R *s, const R *rkh8u, const R *rkh8v, const R *rkv8w, R *h1,
+                R *h2, R *h3)
+{
+  DECL_DIMS(c);
+  const R dxiv = K(1.0) / (R)c->dx, dyiv = K(1.0) / (R)c->dy, dziv = K(1.0) / (R)c->dz;
+  const R dxv05 = K(.5) * dxiv, dyv05 = K(.5) * dyiv;
+#define JS(i, j, k) (A3(s, i, j, k) * A3(jcb, i, j, k))
+  for (int k = 1; k <= nk - 1; k++) {
+    for (int j = 2; j <= nj - 2; j++)
+      for (int i = 2; i <= ni - 1; i++)
+        A3(h1, i, j, k) = A3(rkh8u, i, j, k) * (JS(i, j, k) - JS(i - 1, j, k)) * dxv05;
+    for (int j = 2; j <= nj - 1; j++)
+      for (int i = 2; i <= ni - 2; i++)
+        A3(h2, i, j, k) = A3(rkh8v, i, j, k) * (JS(i, j, k) - JS(i, j - 1, k)) * dyv05;
+  }
+#undef JS
+  for (int k = 2; k <= nk - 1; k++)
+    for (int j = 2; j <= nj - 2; j++)
+      for (int i = 2; i <= ni - 2; i++)
+        A3(h3, i, j, k) = A3(rkv8w, i, j, k) * (A3(s, i, j, k) - A3(s, i, j, k - 1)) * dziv;
+}

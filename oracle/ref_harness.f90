@@ -82,6 +82,7 @@
       use m_defomssq
       use m_eddyvis
       use m_turbdrv
+      use m_turbflx
 
       implicit none
 
@@ -744,5 +745,14 @@
      &               ssq,nsq8w,t13,t23,t33,t11,t22,t12,t31,t32,w1)
       iname(idtubopt)=0
       end subroutine ref_turbdrv
+
+      subroutine ref_turbflx(ni,nj,nk,jcb,s,rkh8u,rkh8v,rkv8w,h1,h2,h3,  &
+     &   w1,w2,w3,w4) bind(C,name='ref_turbflx')
+      integer(c_int), value :: ni,nj,nk
+      real jcb(*),s(*),rkh8u(*),rkh8v(*),rkv8w(*),h1(*),h2(*),h3(*)
+      real w1(*),w2(*),w3(*),w4(*)
+      call s_turbflx(idtrnopt,idsfcopt,iddxiv,iddyiv,iddziv,ni,nj,nk,    &
+     &               w1,w1,jcb,s,w2,rkh8u,rkh8v,rkv8w,h1,h2,h3,w3,w3,w4)
+      end subroutine ref_turbflx
 
       end module cress_ref_harness

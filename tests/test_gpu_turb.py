@@ -132,3 +132,24 @@ def test_turbdrv_matches_oracle(moist, cphopt, wbc):
             assert rel(gpu[n], full[n]) <= 1e-5, (n, rel(gpu[n], full[n]))
     for n in ("ufrc", "vfrc", "wfrc"):
         assert not np.array_equal(gpu[n], outs[n])
+
+
+@pytest.mark.parametrize("wbc", [-1, 7])
+def test_turbflx_matches_oracle(wbc):
+    """cress_turbflx against the oracle's or_turbflx (pinned to s_turbflx):
+    bitwise over whole arrays."""
+    rng = np.random.default_rng(17 + (wbc > 0))
+    f = lambda lo, hi: np.asfortranarray(rng.uniform(lo, hi, (NI + 2, NJ + 2, NK)).astype(np.float32))
+    jcb, s, kh, kv, kw = f(0.9, 1.1), f(290.0, 310.0), f(0.0, 50.0), f(0.0, 50.0), f(0.0, 20.0)
+    h = [f(-1.0, 1.0) for _ in range(3)]
+    gpu = [a.copy(order="F") for a in h]
+    ora = [a.copy(order="F") for a in h]
+    cfg = host.make_config(NI, NJ, NK, 700.0, 900.0, 350.0, DTB, 1.0, cphopt=0, wbc=wbc, gwave=30.0)
+    print("cress_turbflx wbc=%d %dx%dx%d" % (wbc, NI, NJ, NK), flush=True)
+    host.check(host.lib().cress_turbflx(C.byref(cfg), *[_ptr(a) for a in (jcb, s, kh, kv, kw)],
+                                        *[_ptr(a) for a in gpu]), "cress_turbflx")
+    c = po.make_cfg(NI, NJ, NK, dx=700.0, dy=900.0, dz=350.0, wbc=wbc, cphopt=0, gwave=30.0)
+    po.Oracle(32).or_turbflx(c, jcb, s, kh, kv, kw, *ora)
+    for a, b, n in zip(gpu, ora, ("h1", "h2", "h3")):
+        np.testing.assert_array_equal(a, b, err_msg=n)
+    assert not np.array_equal(gpu[2], h[2])

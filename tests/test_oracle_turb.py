@@ -113,3 +113,22 @@ def test_turbdrv_matches_reference(libs, moist, cphopt, wbc, bbc):
             np.testing.assert_array_equal(mine[n], ref[n], err_msg=n)
     for n in ("ufrc", "vfrc", "wfrc"):
         assert not np.array_equal(mine[n], outs[n])
+
+
+@pytest.mark.parametrize("wbc", [-1, 7])
+def test_turbflx_matches_reference(libs, wbc):
+    """s_turbflx (trnopt=0, sfcopt=0): the fluxes of a scalar, bitwise."""
+    o, r = libs
+    c = cfg(wbc, 0, 2)
+    r.configure(c)
+    rng = np.random.default_rng(3 + (wbc > 0))
+    f = lambda lo, hi: np.asfortranarray(rng.uniform(lo, hi, (NI + 2, NJ + 2, NK)).astype(np.float32))
+    jcb, s, kh, kv, kw = f(0.9, 1.1), f(290.0, 310.0), f(0.0, 50.0), f(0.0, 50.0), f(0.0, 20.0)
+    h = [f(-1.0, 1.0) for _ in range(3)]
+    mine = [a.copy(order="F") for a in h]
+    ref = [a.copy(order="F") for a in h]
+    o.or_turbflx(c, jcb, s, kh, kv, kw, *mine)
+    r.ref_turbflx(NI, NJ, NK, jcb, s, kh, kv, kw, *ref, *[fld() for _ in range(4)])
+    for a, b, n in zip(mine, ref, ("h1", "h2", "h3")):
+        np.testing.assert_array_equal(a, b, err_msg=n)
+        assert not np.array_equal(a, h[("h1", "h2", "h3").index(n)])
