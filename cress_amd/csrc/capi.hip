@@ -1596,6 +1596,31 @@ int cress_turbflx(const cress_config *cfg, const float *jcb, const float *s, con
   return CRESS_OK;
 }
 
+// s_turbs for one scalar: sfrc gains the flux divergence over its interior.
+int cress_turbs(const cress_config *cfg, const float *jcb8u, const float *jcb8v, const float *h1, const float *h2,
+                const float *h3, float *sfrc) {
+  const float *in[5] = {jcb8u, jcb8v, h1, h2, h3};
+  for (const float *a : in)
+    if (!a) return CRESS_EINVAL;
+  if (!sfrc || !cfg || cfg->ni < 4 || cfg->nj < 4 || cfg->nk < 4 || !(cfg->dx > 0.0) || !(cfg->dy > 0.0) ||
+      !(cfg->dz > 0.0))
+    return CRESS_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CRESS_ENODEV;
+  const Grid g = make_grid(cfg->ni, cfg->nj, cfg->nk);
+  const size_t n = (size_t)g.n();
+  DevBuf di[5], d;
+  int rc;
+  for (int f = 0; f < 5; f++)
+    if ((rc = to_dev(di[f], in[f], n))) return rc;
+  if ((rc = to_dev(d, sfrc, n))) return rc;
+  turb_s(g, (float)cfg->dx, (float)cfg->dy, (float)cfg->dz, (const float *)di[0].p, (const float *)di[1].p,
+         (const float *)di[2].p, (const float *)di[3].p, (const float *)di[4].p, (float *)d.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(sfrc, d.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return CRESS_OK;
+}
+
 // s_advp alone is the pressure forcing kernel without sponge or smoothing
 // (its forcing is the advection term only, written over the interior).
 int cress_advp(const cress_config *cfg, const float *jcb8u, const float *jcb8v, const float *jcb8w,

@@ -32,4 +32,8 @@ void turb_drv(const Grid &g, const TurbArgs &a);
 void turb_flx(const Grid &g, float dx, float dy, float dz, const float *jcb, const float *s, const float *kh8u,
               const float *kh8v, const float *kv8w, float *h1, float *h2, float *h3);
 
+// s_turbs (turbs.f90:72-75), trnopt=0, mfcopt=0.
+void turb_s(const Grid &g, float dx, float dy, float dz, const float *jcb8u, const float *jcb8v, const float *h1,
+            const float *h2, const float *h3, float *sfrc);
+
 }  // namespace cress

@@ -281,4 +281,23 @@ void turb_flx(const Grid &g, float dx, float dy, float dz, const float *jcb, con
                      s, kh8u, kh8v, kv8w, h1, h2, h3);
 }
 
+// s_turbs (turbs.f90:189-212), trnopt=0, mfcopt=0: the divergence of the
+// fluxes added to the scalar's forcing over i,j 2..n-2, k 2..nk-2.
+__global__ void k_turbs(Grid g, float dx, float dy, float dz, const float *jcb8u, const float *jcb8v,
+                        const float *h1, const float *h2, const float *h3, float *sfrc) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x, j = 2 + blockIdx.y, k = 2 + blockIdx.z;
+  if (i > g.ni - 2) return;
+  const float dxiv = 1.f / dx, dyiv = 1.f / dy, dziv = 1.f / dz;
+  AT(sfrc, i, j, k) = AT(sfrc, i, j, k) +
+                      ((AT(h3, i, j, k + 1) - AT(h3, i, j, k)) * dziv +
+                       ((AT(jcb8u, i + 1, j, k) * AT(h1, i + 1, j, k) - AT(jcb8u, i, j, k) * AT(h1, i, j, k)) * dxiv +
+                        (AT(jcb8v, i, j + 1, k) * AT(h2, i, j + 1, k) - AT(jcb8v, i, j, k) * AT(h2, i, j, k)) * dyiv));
+}
+
+void turb_s(const Grid &g, float dx, float dy, float dz, const float *jcb8u, const float *jcb8v, const float *h1,
+            const float *h2, const float *h3, float *sfrc) {
+  hipLaunchKernelGGL(k_turbs, dim3((g.ni - 3 + 63) / 64, g.nj - 3, g.nk - 3), dim3(64), 0, 0, g, dx, dy, dz, jcb8u,
+                     jcb8v, h1, h2, h3, sfrc);
+}
+
 }  // namespace cress

@@ -59,7 +59,7 @@ module m_cress_hip
   public :: cress_set_timing, cress_stage_times
   public :: cress_gaussel, cress_advs, cress_bcycle, cress_advuvw, cress_advp
   public :: cress_hevi, cress_steps, cress_cloudphy, cress_turbcoef
-  public :: cress_turbdrv, cress_turbflx
+  public :: cress_turbdrv, cress_turbflx, cress_turbs
   public :: cress_bc8u, cress_bc8v, cress_bc8w
   public :: cress_errmsg, base_cfg
 
@@ -280,6 +280,16 @@ module m_cress_hip
       real(c_float), intent(inout) :: h1(*), h2(*), h3(*)
       integer(c_int) :: cress_turbflx
     end function cress_turbflx
+
+! tier 2: s_turbs (turbs.f90:72-75), trnopt=0, mfcopt=0
+    function cress_turbs(cfg, jcb8u, jcb8v, h1, h2, h3, sfrc)            &
+     &  bind(c, name='cress_turbs')
+      import :: c_int, c_float, cress_config
+      type(cress_config), intent(in) :: cfg
+      real(c_float), intent(in) :: jcb8u(*), jcb8v(*), h1(*), h2(*), h3(*)
+      real(c_float), intent(inout) :: sfrc(*)
+      integer(c_int) :: cress_turbs
+    end function cress_turbs
 
 ! tier 2: s_advp (advp.f90:75-79), mfcopt=0, diaopt=0
     function cress_advp(cfg, jcb8u, jcb8v, jcb8w, u, v, wc, pp, pfrc)    &

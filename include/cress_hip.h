@@ -263,6 +263,14 @@ int cress_turbdrv(const cress_config *cfg, float dtb, int moist, const float *jc
 int cress_turbflx(const cress_config *cfg, const float *jcb, const float *s, const float *rkh8u,
                   const float *rkh8v, const float *rkv8w, float *h1, float *h2, float *h3);
 
+/* s_turbs(fptrnopt,fpmpopt,fpmfcopt,fpdxiv,fpdyiv,fpdziv,ni,nj,nk,j31,j32,
+ *         jcb8u,jcb8v,mf,rmf,rmf8u,rmf8v,h1,h2,h3,sfrc,tmp1,tmp2,tmp3)
+ * SrcOrig/turbs.f90:72-75, trnopt=0 and mfcopt=0 (j31, j32 and the map
+ * factors are not read): sfrc += the divergence of the fluxes h1, h2, h3
+ * over i 2..ni-2, j 2..nj-2, k 2..nk-2; bitwise. */
+int cress_turbs(const cress_config *cfg, const float *jcb8u, const float *jcb8v, const float *h1, const float *h2,
+                const float *h3, float *sfrc);
+
 /* s_advp(fpadvopt,fpmpopt,fpmfcopt,fpdiaopt,fpiwest,...,fpdziv,ni,nj,nk,
  *        mf8u,mf8v,jcb8u,jcb8v,jcb8w,u,v,wc,pp,pfrc,jcbxu,jcbxv,jcbxwc,
  *        hadv,vadv,tmp1,tmp2,tmp3)  SrcOrig/advp.f90:75-79, mfcopt=0 and

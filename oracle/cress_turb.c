@@ -333,3 +333,19 @@ void or_turbflx(const or_cfg *c, const R *jcb, const R *s, const R *rkh8u, const
       for (int i = 2; i <= ni - 2; i++)
         A3(h3, i, j, k) = A3(rkv8w, i, j, k) * (A3(s, i, j, k) - A3(s, i, j, k - 1)) * dziv;
 }
+
+/* s_turbs (turbs.f90:189-212), trnopt=0, mfcopt=0: the divergence of the
+ * turbulent fluxes h1, h2, h3 added to sfrc.  The reference's tmp1/tmp2
+ * (jcb8u*h1, jcb8v*h2) are formed where used. */
+void or_turbs(const or_cfg *c, const R *jcb8u, const R *jcb8v, const R *h1, const R *h2, const R *h3, R *sfrc)
+{
+  DECL_DIMS(c);
+  const R dxiv = K(1.0) / (R)c->dx, dyiv = K(1.0) / (R)c->dy, dziv = K(1.0) / (R)c->dz;
+  for (int k = 2; k <= nk - 2; k++)
+    for (int j = 2; j <= nj - 2; j++)
+      for (int i = 2; i <= ni - 2; i++)
+        A3(sfrc, i, j, k) = A3(sfrc, i, j, k) +
+                            ((A3(h3, i, j, k + 1) - A3(h3, i, j, k)) * dziv +
+                             ((A3(jcb8u, i + 1, j, k) * A3(h1, i + 1, j, k) - A3(jcb8u, i, j, k) * A3(h1, i, j, k)) * dxiv +
+                              (A3(jcb8v, i, j + 1, k) * A3(h2, i, j + 1, k) - A3(jcb8v, i, j, k) * A3(h2, i, j, k)) * dyiv));
+}

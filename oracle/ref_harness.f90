@@ -83,6 +83,7 @@
       use m_eddyvis
       use m_turbdrv
       use m_turbflx
+      use m_turbs
 
       implicit none
 
@@ -754,5 +755,15 @@
       call s_turbflx(idtrnopt,idsfcopt,iddxiv,iddyiv,iddziv,ni,nj,nk,    &
      &               w1,w1,jcb,s,w2,rkh8u,rkh8v,rkv8w,h1,h2,h3,w3,w3,w4)
       end subroutine ref_turbflx
+
+      subroutine ref_turbs(ni,nj,nk,jcb8u,jcb8v,h1,h2,h3,sfrc,w1,w2,w3, &
+     &   w4) bind(C,name='ref_turbs')
+      integer(c_int), value :: ni,nj,nk
+      real jcb8u(*),jcb8v(*),h1(*),h2(*),h3(*),sfrc(*)
+      real w1(*),w2(*),w3(*),w4(*)
+      call s_turbs(idtrnopt,idmpopt,idmfcopt,iddxiv,iddyiv,iddziv,       &
+     &             ni,nj,nk,w1,w1,jcb8u,jcb8v,w4,w4,w4,w4,h1,h2,h3,sfrc, &
+     &             w1,w2,w3)
+      end subroutine ref_turbs
 
       end module cress_ref_harness

@@ -33,7 +33,7 @@ NI, NJ, NK = 13, 11, 14
 @pytest.mark.skipif(not os.path.isdir(REFMOD) or not os.path.exists(hb.FLANG),
                     reason="reference modules or flang absent")
 @pytest.mark.parametrize("name", ["gaussel", "bcycle", "advs", "advuvw", "advp", "hevi", "steps", "bc8u",
-                                  "bc8v", "bc8w", "cloudphy", "turbdrv", "turbflx"])
+                                  "bc8v", "bc8w", "cloudphy", "turbdrv", "turbflx", "turbs"])
 def test_dropin_module_compiles_in_reference_tree(name, tmp_path):
     moddir = os.path.join(hb.LIBDIR, "mod")
     if not os.path.exists(os.path.join(moddir, "m_cress_hip.mod")):

@@ -153,3 +153,20 @@ def test_turbflx_matches_oracle(wbc):
     for a, b, n in zip(gpu, ora, ("h1", "h2", "h3")):
         np.testing.assert_array_equal(a, b, err_msg=n)
     assert not np.array_equal(gpu[2], h[2])
+
+
+@pytest.mark.parametrize("wbc", [-1, 7])
+def test_turbs_matches_oracle(wbc):
+    """cress_turbs against the oracle's or_turbs (pinned to s_turbs):
+    bitwise over the whole forcing array."""
+    rng = np.random.default_rng(23 + (wbc > 0))
+    f = lambda lo, hi: np.asfortranarray(rng.uniform(lo, hi, (NI + 2, NJ + 2, NK)).astype(np.float32))
+    ju, jv, h1, h2, h3, sf = f(0.9, 1.1), f(0.9, 1.1), f(-5.0, 5.0), f(-5.0, 5.0), f(-5.0, 5.0), f(-1.0, 1.0)
+    gpu, ora = sf.copy(order="F"), sf.copy(order="F")
+    cfg = host.make_config(NI, NJ, NK, 700.0, 900.0, 350.0, DTB, 1.0, cphopt=0, wbc=wbc, gwave=30.0)
+    print("cress_turbs wbc=%d %dx%dx%d" % (wbc, NI, NJ, NK), flush=True)
+    host.check(host.lib().cress_turbs(C.byref(cfg), *[_ptr(a) for a in (ju, jv, h1, h2, h3, gpu)]), "cress_turbs")
+    c = po.make_cfg(NI, NJ, NK, dx=700.0, dy=900.0, dz=350.0, wbc=wbc, cphopt=0, gwave=30.0)
+    po.Oracle(32).or_turbs(c, ju, jv, h1, h2, h3, ora)
+    np.testing.assert_array_equal(gpu, ora)
+    assert not np.array_equal(gpu, sf)

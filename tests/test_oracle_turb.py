@@ -132,3 +132,20 @@ def test_turbflx_matches_reference(libs, wbc):
     for a, b, n in zip(mine, ref, ("h1", "h2", "h3")):
         np.testing.assert_array_equal(a, b, err_msg=n)
         assert not np.array_equal(a, h[("h1", "h2", "h3").index(n)])
+
+
+@pytest.mark.parametrize("wbc", [-1, 7])
+def test_turbs_matches_reference(libs, wbc):
+    """s_turbs (trnopt=0, mfcopt=0): the flux divergence added to the
+    forcing, bitwise."""
+    o, r = libs
+    c = cfg(wbc, 0, 2)
+    r.configure(c)
+    rng = np.random.default_rng(8 + (wbc > 0))
+    f = lambda lo, hi: np.asfortranarray(rng.uniform(lo, hi, (NI + 2, NJ + 2, NK)).astype(np.float32))
+    ju, jv, h1, h2, h3, sf = f(0.9, 1.1), f(0.9, 1.1), f(-5.0, 5.0), f(-5.0, 5.0), f(-5.0, 5.0), f(-1.0, 1.0)
+    mine, ref = sf.copy(order="F"), sf.copy(order="F")
+    o.or_turbs(c, ju, jv, h1, h2, h3, mine)
+    r.ref_turbs(NI, NJ, NK, ju, jv, h1, h2, h3, ref, *[fld() for _ in range(4)])
+    np.testing.assert_array_equal(mine, ref)
+    assert not np.array_equal(mine, sf)
