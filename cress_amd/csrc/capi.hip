@@ -1621,6 +1621,23 @@ int cress_turbs(const cress_config *cfg, const float *jcb8u, const float *jcb8v,
   return CRESS_OK;
 }
 
+// s_kh8uv: rkh8u and rkh8v come back over the ranges the reference writes.
+int cress_kh8uv(const cress_config *cfg, const float *rkh, float *rkh8u, float *rkh8v) {
+  if (!cfg || !rkh || !rkh8u || !rkh8v || cfg->ni < 3 || cfg->nj < 3 || cfg->nk < 2) return CRESS_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CRESS_ENODEV;
+  const Grid g = make_grid(cfg->ni, cfg->nj, cfg->nk);
+  const size_t n = (size_t)g.n();
+  DevBuf a, b, d;
+  int rc;
+  if ((rc = to_dev(a, rkh, n)) || (rc = to_dev(b, rkh8u, n)) || (rc = to_dev(d, rkh8v, n))) return rc;
+  turb_kh8uv(g, (const float *)a.p, (float *)b.p, (float *)d.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(rkh8u, b.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(rkh8v, d.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return CRESS_OK;
+}
+
 // s_advp alone is the pressure forcing kernel without sponge or smoothing
 // (its forcing is the advection term only, written over the interior).
 int cress_advp(const cress_config *cfg, const float *jcb8u, const float *jcb8v, const float *jcb8w,

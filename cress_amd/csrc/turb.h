@@ -36,4 +36,7 @@ void turb_flx(const Grid &g, float dx, float dy, float dz, const float *jcb, con
 void turb_s(const Grid &g, float dx, float dy, float dz, const float *jcb8u, const float *jcb8v, const float *h1,
             const float *h2, const float *h3, float *sfrc);
 
+// s_kh8uv (kh8uv.f90:60), mfcopt=0.
+void turb_kh8uv(const Grid &g, const float *rkh, float *rkh8u, float *rkh8v);
+
 }  // namespace cress

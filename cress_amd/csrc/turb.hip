@@ -300,4 +300,16 @@ void turb_s(const Grid &g, float dx, float dy, float dz, const float *jcb8u, con
                      jcb8v, h1, h2, h3, sfrc);
 }
 
+// s_kh8uv (kh8uv.f90:214-232), mfcopt=0: rkh summed onto u and v points.
+__global__ void k_kh8uv(Grid g, const float *rkh, float *rkh8u, float *rkh8v) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y, k = 1 + blockIdx.z;
+  if (i > g.ni - 1) return;
+  if (i >= 2) AT(rkh8u, i, j, k) = AT(rkh, i - 1, j, k) + AT(rkh, i, j, k);
+  if (j >= 2) AT(rkh8v, i, j, k) = AT(rkh, i, j - 1, k) + AT(rkh, i, j, k);
+}
+
+void turb_kh8uv(const Grid &g, const float *rkh, float *rkh8u, float *rkh8v) {
+  hipLaunchKernelGGL(k_kh8uv, dim3((g.ni - 1 + 63) / 64, g.nj - 1, g.nk - 1), dim3(64), 0, 0, g, rkh, rkh8u, rkh8v);
+}
+
 }  // namespace cress

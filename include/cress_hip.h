@@ -271,6 +271,12 @@ int cress_turbflx(const cress_config *cfg, const float *jcb, const float *s, con
 int cress_turbs(const cress_config *cfg, const float *jcb8u, const float *jcb8v, const float *h1, const float *h2,
                 const float *h3, float *sfrc);
 
+/* s_kh8uv(fpmpopt,fpmfcopt,ni,nj,nk,rmf,rkh,rkh8u,rkh8v)
+ * SrcOrig/kh8uv.f90:60, mfcopt=0 (rmf is not read): rkh8u(i) =
+ * rkh(i-1)+rkh(i) over i 2..ni-1, j 1..nj-1, rkh8v likewise in j, k 1..nk-1;
+ * bitwise. */
+int cress_kh8uv(const cress_config *cfg, const float *rkh, float *rkh8u, float *rkh8v);
+
 /* s_advp(fpadvopt,fpmpopt,fpmfcopt,fpdiaopt,fpiwest,...,fpdziv,ni,nj,nk,
  *        mf8u,mf8v,jcb8u,jcb8v,jcb8w,u,v,wc,pp,pfrc,jcbxu,jcbxv,jcbxwc,
  *        hadv,vadv,tmp1,tmp2,tmp3)  SrcOrig/advp.f90:75-79, mfcopt=0 and

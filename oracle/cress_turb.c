@@ -349,3 +349,16 @@ void or_turbs(const or_cfg *c, const R *jcb8u, const R *jcb8v, const R *h1, cons
                              ((A3(jcb8u, i + 1, j, k) * A3(h1, i + 1, j, k) - A3(jcb8u, i, j, k) * A3(h1, i, j, k)) * dxiv +
                               (A3(jcb8v, i, j + 1, k) * A3(h2, i, j + 1, k) - A3(jcb8v, i, j, k) * A3(h2, i, j, k)) * dyiv));
 }
+
+/* s_kh8uv (kh8uv.f90:214-232), mfcopt=0: twice the mean eddy viscosity at
+ * u and v points (the halving is folded into the flux coefficients). */
+void or_kh8uv(const or_cfg *c, const R *rkh, R *rkh8u, R *rkh8v)
+{
+  DECL_DIMS(c);
+  for (int k = 1; k <= nk - 1; k++) {
+    for (int j = 1; j <= nj - 1; j++)
+      for (int i = 2; i <= ni - 1; i++) A3(rkh8u, i, j, k) = A3(rkh, i - 1, j, k) + A3(rkh, i, j, k);
+    for (int j = 2; j <= nj - 1; j++)
+      for (int i = 1; i <= ni - 1; i++) A3(rkh8v, i, j, k) = A3(rkh, i, j - 1, k) + A3(rkh, i, j, k);
+  }
+}

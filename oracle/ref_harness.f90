@@ -84,6 +84,7 @@
       use m_turbdrv
       use m_turbflx
       use m_turbs
+      use m_kh8uv
 
       implicit none
 
@@ -765,5 +766,12 @@
      &             ni,nj,nk,w1,w1,jcb8u,jcb8v,w4,w4,w4,w4,h1,h2,h3,sfrc, &
      &             w1,w2,w3)
       end subroutine ref_turbs
+
+      subroutine ref_kh8uv(ni,nj,nk,rkh,rkh8u,rkh8v,w1)                 &
+     &   bind(C,name='ref_kh8uv')
+      integer(c_int), value :: ni,nj,nk
+      real rkh(*),rkh8u(*),rkh8v(*),w1(*)
+      call s_kh8uv(idmpopt,idmfcopt,ni,nj,nk,w1,rkh,rkh8u,rkh8v)
+      end subroutine ref_kh8uv
 
       end module cress_ref_harness

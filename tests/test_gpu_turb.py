@@ -170,3 +170,18 @@ def test_turbs_matches_oracle(wbc):
     po.Oracle(32).or_turbs(c, ju, jv, h1, h2, h3, ora)
     np.testing.assert_array_equal(gpu, ora)
     assert not np.array_equal(gpu, sf)
+
+
+def test_kh8uv_matches_oracle():
+    """cress_kh8uv against the oracle's or_kh8uv (pinned to s_kh8uv)."""
+    rng = np.random.default_rng(29)
+    f = lambda: np.asfortranarray(rng.uniform(0.0, 50.0, (NI + 2, NJ + 2, NK)).astype(np.float32))
+    kh, u0, v0 = f(), f(), f()
+    gu, gv, ou, ov = u0.copy(order="F"), v0.copy(order="F"), u0.copy(order="F"), v0.copy(order="F")
+    cfg = host.make_config(NI, NJ, NK, 700.0, 900.0, 350.0, DTB, 1.0, cphopt=0)
+    print("cress_kh8uv %dx%dx%d" % (NI, NJ, NK), flush=True)
+    host.check(host.lib().cress_kh8uv(C.byref(cfg), _ptr(kh), _ptr(gu), _ptr(gv)), "cress_kh8uv")
+    c = po.make_cfg(NI, NJ, NK, dx=700.0, dy=900.0, dz=350.0, cphopt=0)
+    po.Oracle(32).or_kh8uv(c, kh, ou, ov)
+    np.testing.assert_array_equal(gu, ou)
+    np.testing.assert_array_equal(gv, ov)

@@ -149,3 +149,19 @@ def test_turbs_matches_reference(libs, wbc):
     r.ref_turbs(NI, NJ, NK, ju, jv, h1, h2, h3, ref, *[fld() for _ in range(4)])
     np.testing.assert_array_equal(mine, ref)
     assert not np.array_equal(mine, sf)
+
+
+def test_kh8uv_matches_reference(libs):
+    """s_kh8uv (mfcopt=0), bitwise over whole arrays."""
+    o, r = libs
+    c = cfg(-1, 0, 2)
+    r.configure(c)
+    rng = np.random.default_rng(4)
+    f = lambda: np.asfortranarray(rng.uniform(0.0, 50.0, (NI + 2, NJ + 2, NK)).astype(np.float32))
+    kh, u0, v0 = f(), f(), f()
+    mu, mv, ru, rv = u0.copy(order="F"), v0.copy(order="F"), u0.copy(order="F"), v0.copy(order="F")
+    o.or_kh8uv(c, kh, mu, mv)
+    r.ref_kh8uv(NI, NJ, NK, kh, ru, rv, fld())
+    np.testing.assert_array_equal(mu, ru)
+    np.testing.assert_array_equal(mv, rv)
+    assert not np.array_equal(mu, u0)
