@@ -791,12 +791,12 @@ static int large_step(cress_ctx *x, int first) {
   if (sm) {
     LAUNCH(x, KT_FORCE_UV, k_force_uv<true>, flat(tuv), bf, 0, tuv, g, o, c, fi, x->ufrc, x->vfrc);
     LAUNCH(x, KT_FORCE_W, k_force_w<true>, flat(tw), bf, 0, tw, g, o, c, fi, x->wfrc);
-    LAUNCH(x, KT_FORCE_P, k_force_p<true>, flat(tp), bf, 0, tp, g, o, c, B[CRESS_JCB], B[CRESS_JCB8U],
+    LAUNCH(x, KT_FORCE_P, k_force_p, flat(tp), bf, 0, tp, g, o, c, B[CRESS_JCB], B[CRESS_JCB8U],
            B[CRESS_JCB8V], B[CRESS_JCB8W], x->u[C], x->v[C], x->wc, x->pp[C], x->pp[P], x->rbct, x->pfrc);
   } else {
     LAUNCH(x, KT_FORCE_UV, k_force_uv<false>, flat(tuv), bf, 0, tuv, g, o, c, fi, x->ufrc, x->vfrc);
     LAUNCH(x, KT_FORCE_W, k_force_w<false>, flat(tw), bf, 0, tw, g, o, c, fi, x->wfrc);
-    LAUNCH(x, KT_FORCE_P, k_force_p<false>, flat(tp), bf, 0, tp, g, o, c, B[CRESS_JCB], B[CRESS_JCB8U],
+    LAUNCH(x, KT_FORCE_P, k_force_p, flat(tp), bf, 0, tp, g, o, c, B[CRESS_JCB], B[CRESS_JCB8U],
            B[CRESS_JCB8V], B[CRESS_JCB8W], x->u[C], x->v[C], x->wc, x->pp[C], x->pp[P], x->rbct, x->pfrc);
   }
   ScalarIn si;
@@ -941,7 +941,11 @@ static int large_step(cress_ctx *x, int first) {
   stamp(x, 6);
 
   // ---- second halo rows for 4th-order advection (shift2nd) ----
-  for (int lv = 0; lv < 2 && open; lv++) {
+  // s_shift2nd does nothing unless a 4th-order stencil reads the rows
+  // (shift2nd.f90:266-268: advopt>=2 or the 4th-order smoothing; tubopt
+  // and the exchange option 6 are not supported here).
+  const bool need2 = x->cfg.advopt >= 2 || x->cfg.smtopt % 10 == 2 || x->cfg.smtopt % 10 == 3;
+  for (int lv = 0; lv < 2 && open && need2; lv++) {
     // open sides: only the corners of the outer ring (shift2nd.f90:4590-4700)
     const int S = lv == 0 ? C : F;
     CornerSet cs;
@@ -953,7 +957,7 @@ static int large_step(cress_ctx *x, int first) {
     for (int f = 0; f < (withq ? 10 : 4); f++) corner_spec(&cs, sc[f], 0, ni, 0, nj);
     run_corners(x, cs);
   }
-  for (int lv = 0; lv < 2 && !open; lv++) {
+  for (int lv = 0; lv < 2 && !open && need2; lv++) {
     const int S = lv == 0 ? C : F;
     FillSet fs;
     fs.n = withq ? 12 : 6;
@@ -1084,10 +1088,18 @@ int cress_ens_create(cress_ctx *const *members, int n, cress_ens **out) {
     delete e;
     return CRESS_EHIP;
   }
-  e->join.resize(n);
+  e->join.assign(n, nullptr);
+  e->fork = nullptr;
   bool ok = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) == hipSuccess;
   for (int i = 0; i < n && ok; i++) ok = hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming) == hipSuccess;
-  if (!ok) return CRESS_EHIP;
+  if (!ok) {
+    for (hipEvent_t ev : e->join)
+      if (ev) hipEventDestroy(ev);
+    if (e->fork) hipEventDestroy(e->fork);
+    hipStreamDestroy(e->master);
+    delete e;
+    return CRESS_EHIP;
+  }
   *out = e;
   return CRESS_OK;
 }
@@ -1667,7 +1679,7 @@ int cress_advp(const cress_config *cfg, const float *jcb8u, const float *jcb8v, 
     return rc;
   const Tiles tp = tiles(g.ni - 1, g.nj - 3, 64, CRESS_FF_BY, g.nk - 3, CRESS_FF_ORDER);
   const float *pj = (const float *)a.p, *pp_ = (const float *)p.p;
-  hipLaunchKernelGGL(k_force_p<false>, flat(tp), dim3(64, CRESS_FF_BY), 0, 0, tp, g, o, c, pj, pj,
+  hipLaunchKernelGGL(k_force_p, flat(tp), dim3(64, CRESS_FF_BY), 0, 0, tp, g, o, c, pj, pj,
                      (const float *)b.p, (const float *)d.p, (const float *)e.p, (const float *)f.p,
                      (const float *)h.p, pp_, pp_, pj, (float *)q.p);
   HIPCHK(hipGetLastError());

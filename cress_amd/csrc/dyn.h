@@ -116,7 +116,6 @@ template <bool SM>
 __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *ufrc, float *vfrc);
 template <bool SM>
 __global__ void k_force_w(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *wfrc);
-template <bool SM>
 __global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, const float *jcb8u,
                           const float *jcb8v, const float *jcb8w, const float *u, const float *v,
                           const float *wc, const float *pp, const float *ppp, const float *rbct,

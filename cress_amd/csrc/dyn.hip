@@ -332,9 +332,11 @@ __global__ void k_force_w(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *
 
 // ---------------------------------------------------------------------------
 // Pressure forcing: s_forcep (forcep.f90:254-308) = advp (advp.f90:249-785,
-// diaopt=0) then vspp (vspp.f90:180-194).  pfrc over the interior.
+// diaopt=0) then vspp (vspp.f90:180-194).  pfrc over the interior.  No
+// numerical smoothing: s_forcedrv passes the constant-zero option id in
+// forcep's smtopt slot (forcedrv.f90:598, idzeropt), so the reference's
+// large step never smooths p'.
 // ---------------------------------------------------------------------------
-template <bool SM>
 __global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, const float *jcb8u,
                           const float *jcb8v, const float *jcb8w, const float *u, const float *v,
                           const float *wc, const float *pp, const float *ppp, const float *rbct,
@@ -377,11 +379,6 @@ __global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, c
       }
       f = f + vadv;
     }
-  }
-  if (SM) {
-    const bool band = j >= 2 + o.js && j <= g.nj - 2 - o.jn && i >= 2 + o.iw && i <= g.ni - 2 - o.ie;
-    f = smooth4(f, c, o.smtopt, band, k == 2, k == g.nk - 2,
-                [&](int di, int dj, int dk) { return AT(ppp, i + di, j + dj, k + dk); });
   }
   if (o.vsp_p && k >= o.ksp0 - 1 && k <= g.nk - 2)
     f = f - 0.5f * (rbct2(rbct, g, i, j, k) + rbct2(rbct, g, i, j, k + 1)) * MT(CRESS_JCB, jcb, i, j, k) *
@@ -1057,12 +1054,6 @@ template __global__ void k_force_uv<false>(Tiles, Grid, Opts, Consts, ForceIn, f
 template __global__ void k_force_uv<true>(Tiles, Grid, Opts, Consts, ForceIn, float *, float *);
 template __global__ void k_force_w<false>(Tiles, Grid, Opts, Consts, ForceIn, float *);
 template __global__ void k_force_w<true>(Tiles, Grid, Opts, Consts, ForceIn, float *);
-template __global__ void k_force_p<false>(Tiles, Grid, Opts, Consts, const float *, const float *, const float *,
-                                          const float *, const float *, const float *, const float *,
-                                          const float *, const float *, const float *, float *);
-template __global__ void k_force_p<true>(Tiles, Grid, Opts, Consts, const float *, const float *, const float *,
-                                         const float *, const float *, const float *, const float *,
-                                         const float *, const float *, const float *, float *);
 template __global__ void k_force_s<false>(Tiles, Grid, Opts, Consts, ScalarIn, int);
 template __global__ void k_force_s<true>(Tiles, Grid, Opts, Consts, ScalarIn, int);
 

@@ -2,9 +2,10 @@
 // Each writes the index ranges the reference writes; the arithmetic keeps
 // Fortran's left-to-right order term by term, so with -ffp-contract=off
 // the tensor, its square and dry / cloud-free N^2 are the reference's
-// bits.  exp and log are taken in binary64 and rounded to binary32 (N^2 in
-// cloudy air, the length scale ln0 of the eddy viscosity): within 1e-5 of
-// the reference's FP32 libm.
+// bits.  exp and log are binary32 OCML calls (N^2 in cloudy air, the
+// length scale ln0 = (ds^3 jcb)^(1/3) of the eddy viscosity, taken as
+// exp(log(x)/3) as the reference writes it): within 1e-5 of the reference's
+// FP32 libm results.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -16,8 +17,10 @@ namespace cress {
 
 #define AT(a, i, j, k) a[g.at(i, j, k)]
 
-__device__ __forceinline__ float expd(float x) { return (float)exp((double)x); }
-__device__ __forceinline__ float logd(float x) { return (float)log((double)x); }
+// exp and log in binary32 (ROCm OCML expf/logf, at most 1 ulp from the
+// correctly rounded result); the reference's FP32 build uses glibc's.
+__device__ __forceinline__ float expd(float x) { return expf(x); }
+__device__ __forceinline__ float logd(float x) { return logf(x); }
 
 // comphy.f90:49-130, commath.f90:51,177
 #define TB_G 9.8f

@@ -4,7 +4,9 @@
   oracle/_ref/liboracle32.so  -- cress_oracle.c with REAL=float
   oracle/_ref/liboracle64.so  -- cress_oracle.c with REAL=double
   oracle/_ref/libcressref32.so -- the reference's own routines (build_ref.py),
-                                  only where /root/reference is present.
+                                  only where /root/reference is present;
+  oracle/_ref/libcressrefmpi32.so -- likewise the routines that reach its
+                                  halo-exchange layer, with the image's MPICH.
 
 Plain gcc, -O2, no -march (x86-64 baseline has no FMA, and -ffp-contract=off
 keeps it that way), so the FP32 build rounds like the reference's FP32 build.
@@ -48,6 +50,7 @@ def build(with_ref=True):
         sys.path.insert(0, HERE)
         import build_ref
         build_ref.build(32)
+        build_ref.build_mpi(32)
     return libs
 
 

@@ -14,6 +14,19 @@ without running them -- they need csh and MPI):
     reached), so no external library is linked;
   * flang -O2 -fPIC, FP32 (the reference's default REAL), no OpenMP.
 Outputs go to oracle/_ref/ only (git-ignored).
+
+build_mpi() makes oracle/_ref/libcressrefmpi32.so in the same way for the
+routines whose module closure reaches the halo-exchange layer (the harness
+oracle/ref_harness_mpi.f90: s_intgdrv, s_hevi, s_stepuv, s_stepwi,
+s_steppi, s_steps, s_cloudphy, s_fallblk, s_phy2cnt, s_shift2nd).  Those
+need an MPI library: the image's MPICH (/opt/conda: mpif.h, libmpifort,
+libmpi).  Its Fortran module file is gfortran's and unreadable by flang, so
+m_defmpi is compiled with the alternative the reference itself carries:
+defmpi.f90 says `use mpi` with `include 'mpif.h'` commented out beside it
+(SrcOrig/defmpi.f90:18,22); the staged copy under oracle/_ref/ comments
+out the `use` line and restores the `include` line, placed after the
+module's `implicit none` (flang does not accept the include's declarations
+before it); nothing else changes.
 """
 import os
 import re
@@ -27,17 +40,30 @@ OUT = os.path.join(HERE, "_ref")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 
 
-def scan_sources():
+def solver_objects():
+    """Object names of the solver's build list (SrcOrig/Make_solver)."""
+    txt = open(os.path.join(REF, "SrcOrig", "Make_solver"), errors="replace").read()
+    return set(re.findall(r"^\s*(\w+)\.o\s*\\?\s*$", txt, re.M))
+
+
+def scan_sources(prefer_solver=False):
+    """Source files by name, the module each defines and the modules each
+    uses.  Some modules have two sources (inimpi_multi.f90 and
+    inimpi_single.f90, ...); with prefer_solver the one in the solver's
+    object list (the _multi, MPI variant) defines it, else the later name."""
     files = {}
     for d in (os.path.join(REF, "SrcOrig"),
               os.path.join(REF, "SrcDevelop", "0rig", "UserModAdd")):
         for f in sorted(os.listdir(d)):
             if f.endswith(".f90"):
                 files[f[:-4]] = os.path.join(d, f)      # overlay wins
+    objs = solver_objects() if prefer_solver else set()
     mod2file, uses = {}, {}
     for name, path in files.items():
         txt = open(path, errors="replace").read().lower()
         for m in re.findall(r"^\s*module\s+(m_\w+)", txt, re.M):
+            if m in mod2file and prefer_solver and mod2file[m] in objs and name not in objs:
+                continue
             mod2file[m] = name
         uses[name] = set(re.findall(r"^\s*use\s+(m_\w+)", txt, re.M))
     return files, mod2file, uses
@@ -123,5 +149,79 @@ def build(prec=32, jobs=8):
     return lib
 
 
+MPI_INC = os.environ.get("CRESS_MPI_INC", "/opt/conda/include")
+MPI_LIB = os.environ.get("CRESS_MPI_LIB", "/opt/conda/lib")
+
+
+def have_mpi():
+    return (os.path.exists(os.path.join(MPI_INC, "mpif.h"))
+            and os.path.exists(os.path.join(MPI_LIB, "libmpifort.so")))
+
+
+def stage_defmpi(files, dst):
+    """The reference's defmpi.f90 with its `use mpi` line commented out and
+    its commented `include 'mpif.h'` line restored."""
+    txt = open(files["defmpi"]).read()
+    use = re.compile(r"^([ \t]*)use mpi[ \t]*$", re.M | re.I)
+    inc = re.compile(r"^!OLD([ \t]*)include 'mpif.h'[ \t]*$", re.M | re.I)
+    if len(use.findall(txt)) != 1 or len(inc.findall(txt)) != 1:
+        raise SystemExit("defmpi.f90 does not have the expected use/include lines")
+    txt = use.sub(r"!\1use mpi", txt)
+    # The include's specification statements may not precede IMPLICIT NONE
+    # (flang rejects that order), so the restored line moves below it.
+    txt = inc.sub("!OLD  (moved below implicit none)", txt)
+    imp = re.compile(r"^([ \t]*)implicit none[ \t]*$", re.M | re.I)
+    if len(imp.findall(txt)) != 1:
+        raise SystemExit("defmpi.f90 does not have one implicit none line")
+    txt = imp.sub(r"\1implicit none\n\1include 'mpif.h'", txt)
+    os.makedirs(os.path.dirname(dst), exist_ok=True)
+    if not os.path.exists(dst) or open(dst).read() != txt:
+        with open(dst, "w") as f:
+            f.write(txt)
+    return dst
+
+
+def build_mpi(prec=32, jobs=8):
+    if not os.path.isdir(os.path.join(REF, "SrcOrig")):
+        print("reference sources absent; oracle/_ref (MPI part) not rebuilt")
+        return None
+    if not have_mpi():
+        print("MPICH not found; oracle/_ref (MPI part) not built")
+        return None
+    files, mod2file, uses = scan_sources(prefer_solver=True)
+    obj = os.path.join(OUT, "objmpi%d" % prec)
+    os.makedirs(obj, exist_ok=True)
+    files["defmpi"] = stage_defmpi(files, os.path.join(obj, "src", "defmpi.f90"))
+    h1 = os.path.join(HERE, "ref_harness.f90")
+    h2 = os.path.join(HERE, "ref_harness_mpi.f90")
+    names = closure(harness_modules(h1) + harness_modules(h2), mod2file, uses)
+    flags = ["-O2", "-fPIC", "-ffp-contract=off", "-module-dir", obj, "-I", obj, "-I", MPI_INC]
+    if prec == 64:
+        flags.append("-fdefault-real-8")
+    objs = []
+    for group in levels(names, mod2file, uses):
+        cmds = []
+        for n in group:
+            o = os.path.join(obj, n + ".o")
+            objs.append(o)
+            src = files[n]
+            if (os.path.exists(o) and os.path.getmtime(o) > os.path.getmtime(src)):
+                continue
+            cmds.append([FLANG, "-c"] + flags + [src, "-o", o])
+        with ThreadPoolExecutor(jobs) as ex:
+            list(ex.map(run, cmds))
+    hos = []
+    for h in (h1, h2):
+        ho = os.path.join(obj, os.path.basename(h).replace(".f90", ".o"))
+        run([FLANG, "-c"] + flags + [h, "-o", ho])
+        hos.append(ho)
+    lib = os.path.join(OUT, "libcressrefmpi%d.so" % prec)
+    run([FLANG, "-shared", "-fPIC", "-o", lib] + hos + objs +
+        ["-L" + MPI_LIB, "-Wl,-rpath," + MPI_LIB, "-lmpifort", "-lmpi"])
+    print("built", lib, "from", len(objs), "reference files")
+    return lib
+
+
 if __name__ == "__main__":
     build(int(sys.argv[1]) if len(sys.argv) > 1 else 32)
+    build_mpi(int(sys.argv[1]) if len(sys.argv) > 1 else 32)

@@ -1296,7 +1296,8 @@ static void or_forcedrv(const or_cfg *c, or_state *s, R *qall, R *ufrc, R *vfrc,
   /* forcep */
   or_advp(c, s->jcb8u, s->jcb8v, s->jcb8w, s->u[C], s->v[C], wc, s->pp[C], pfrc,
           t[0], t[1], t[2], t[3], t[4], t[5], t[6]);
-  if (sm) or_smoo4s(c, 2, NULL, NULL, s->pp[P], pfrc, t[0], t[1], t[2], t[3]);
+  /* no smoothing of p': s_forcedrv passes the constant-zero option id as
+   * forcep's smtopt (forcedrv.f90:598, idzeropt) */
   if (c->vspopt >= 1 && c->vspvar[3] == 'o') or_vsps(c, s->jcb, s->pp[P], NULL, s->rbct, pfrc);
   /* forcept */
   or_advs(c, rstxu, rstxv, rstxwc, s->ptp[C], ptfrc, t[0], t[1], t[2], t[3]);
@@ -1478,6 +1479,9 @@ void or_shift2nd(const or_cfg *c, or_state *s, int lv)
 {
   const int ni = c->ni, nj = c->nj, nk = c->nk;
   const size_t n = (size_t)nfield(c);
+  /* shift2nd.f90:266-268: nothing to do unless a 4th-order stencil reads
+   * the second halo (advopt>=2 or smtopt 2/3; tubopt=0, no option-6 side) */
+  if (!(c->advopt >= 2 || c->smtopt % 10 == 2 || c->smtopt % 10 == 3)) return;
   or_bcycle(c, 4, 0, ni - 3, ni + 1, 3, 0, nj - 3, nj, nk, s->u[lv]);
   or_bcycle(c, 3, 0, ni - 3, ni, 4, 0, nj - 3, nj + 1, nk, s->v[lv]);
   or_bcycle(c, 3, 0, ni - 3, ni, 3, 0, nj - 3, nj, nk, s->w[lv]);
