@@ -37,12 +37,12 @@ enum { NSTAGE = 8 };
 enum {
   KT_PREP, KT_FORCE_UV, KT_FORCE_W, KT_FORCE_P, KT_FORCE_S, KT_PHY2CNT, KT_AC_PTUV,
   KT_AC_COL, KT_AC_RING, KT_FILL, KT_VBC, KT_STEPS, KT_TIMEFLT, KT_COPY, KT_MICRO_SEED, KT_MICRO_SRC,
-  KT_MICRO_FALL, KT_SATADJ, KT_OBC, KT_N
+  KT_MICRO_FALL, KT_SATADJ, KT_OBC, KT_MLP_SELECT, KT_MLP_INFER, KT_N
 };
 static const char *kKernelNames[KT_N] = {
     "k_prep", "k_force_uv", "k_force_w", "k_force_p", "k_force_s", "k_phy2cnt", "k_ac_ptuv",
     "k_ac_col", "k_ac_ring", "k_bcycle", "k_vbc", "k_steps", "k_timeflt", "memcpy_d2d",
-    "k_micro_seed", "k_micro_src", "k_micro_fall", "k_satadjst", "k_obc"};
+    "k_micro_seed", "k_micro_src", "k_micro_fall", "k_satadjst", "k_obc", "k_mlp_select", "k_mlp_infer"};
 enum { MAXEV = 1024 };
 
 struct cress_ctx {
@@ -70,6 +70,11 @@ struct cress_ctx {
   float obc_dtb, obc_dts;
   int obc_have;
   std::vector<void *> allocs;
+  // learned microphysics: 0 srcblk, 1 srcblk recording pairs into mlp,
+  // 2 the network (cress_ctx_set_mlp); ml holds the per-step buffers
+  struct cress_mlp *mlp;
+  int mlp_mode;
+  MlpStep ml;
   int timing;
   int tkern;  // timing of one kernel family only (-1: every family and the stages)
   hipEvent_t ev[NSTAGE + 1];
@@ -91,6 +96,7 @@ const char *cress_error_string(int code) {
     case CRESS_ENOMEM: return "device out of memory";
     case CRESS_ERANGE: return "field id or level out of range";
     case CRESS_EUNSUPPORTED: return "option outside the implemented plain path";
+    case CRESS_ETOOBIG: return "grid larger than the fixed scratch buffer";
   }
   return "unknown error";
 }
@@ -685,6 +691,8 @@ static void micro_stage(cress_ctx *x, float dtb, const ObcLf &olf, int open, int
     ma.ppp = x->pp[P];
     ma.ptpp = x->ptp[P];
     ma.qp = x->q[P];
+    ma.qpw = x->q[P] + n;
+    ma.qpi = x->q[P] + 3 * n;
     ma.ncip = x->nci[P];
     ma.ptpf = x->ptp[F];
     ma.rst = B[CRESS_RST];
@@ -692,6 +700,8 @@ static void micro_stage(cress_ctx *x, float dtb, const ObcLf &olf, int open, int
     ma.qvfrc = x->qvfrc;
     ma.sfrc = x->sfrc;
     ma.qf = x->q[F];
+    ma.qfw = x->q[F] + n;
+    ma.qfi = x->q[F] + 3 * n;
     ma.ncif = x->nci[F];
     ma.urq = x->ms.f[0];
     ma.usq = x->ms.f[1];
@@ -704,6 +714,36 @@ static void micro_stage(cress_ctx *x, float dtb, const ObcLf &olf, int open, int
     ma.lf = olf;
     ma.open = open;
     ma.given = given;
+    ma.mode = x->mlp ? x->mlp_mode : 0;
+    ma.cap = MlpCapture{nullptr, nullptr, nullptr, 0};
+    ma.mflag = nullptr;
+    ma.mout = nullptr;
+    if (ma.mode == 1) {
+      const cress_mlp_impl *m = (const cress_mlp_impl *)x->mlp;
+      ma.cap = MlpCapture{m->din, m->dtgt, m->dcount, m->cap};
+    } else if (ma.mode == 2) {
+      const cress_mlp_impl *m = (const cress_mlp_impl *)x->mlp;
+      MlpSelectArgs sa;
+      sa.pbr = B[CRESS_PBR];
+      sa.ptbr = B[CRESS_PTBR];
+      sa.rbr = B[CRESS_RBR];
+      sa.ppp = x->pp[P];
+      sa.ptpp = x->ptp[P];
+      sa.qp = ma.qp;
+      sa.qpw = ma.qpw;
+      sa.qpi = ma.qpi;
+      sa.qn = n;
+      sa.thresq = (float)cf.thresq;
+      sa.s = x->ml;
+      sa.norm = m->norm;
+      LAUNCH(x, KT_MLP_SELECT, k_u32_set, dim3(1), dim3(64), 0, x->ml.count, 0u);
+      const Tiles tsel = tiles(ni, nj - 1, 64, 4, nk - 1);
+      LAUNCH(x, KT_MLP_SELECT, k_mlp_select, flat(tsel), b2, 0, tsel, g, sa);
+      const MlpNet net{m->w, m->p, m->norm};
+      LAUNCH(x, KT_MLP_INFER, k_mlp_infer, dim3(1024), dim3(256), 0, net, x->ml, n);
+      ma.mflag = x->ml.flag;
+      ma.mout = x->ml.out;
+    }
     unsigned seed;
     memcpy(&seed, &dtbs, sizeof seed);
     LAUNCH(x, KT_MICRO_SEED, k_micro_seed, dim3(1), dim3(64), 0, ma.dtpmin, seed);
@@ -711,11 +751,13 @@ static void micro_stage(cress_ctx *x, float dtb, const ObcLf &olf, int open, int
     FallArgs fa;
     fa.rbr = B[CRESS_RBR];
     fa.rst = B[CRESS_RST];
-    fa.qp = x->q[P];
+    fa.qpw = ma.qpw;
+    fa.qpi = ma.qpi;
     fa.urq = ma.urq;
     fa.usq = ma.usq;
     fa.ugq = ma.ugq;
-    fa.qf = x->q[F];
+    fa.qfw = ma.qfw;
+    fa.qfi = ma.qfi;
     fa.ncif = x->nci[F];
     fa.prwtr = x->prwtr;
     fa.price = x->price;
@@ -1733,5 +1775,35 @@ int cress_bcycle(const cress_config *cfg, int iwsnd, int iwrcv, int iesnd, int i
   hipLaunchKernelGGL(k_bcycle_y, dim3((g.ni + 2 + 63) / 64, kmax, 1), dim3(64), 0, 0, g, fs);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpy(var, a.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return CRESS_OK;
+}
+
+
+// Attach a network to a context (mode 1: record pairs from the plain
+// scheme into its capture buffers; 2: use it for the source terms of
+// every active cell inside its envelope) or detach it (mlp NULL or mode 0).
+int cress_ctx_set_mlp(cress_ctx *x, cress_mlp *mlp, int mode) {
+  if (!x || mode < 0 || mode > 2 || (mode && !mlp)) return CRESS_EINVAL;
+  if (mode && x->cfg.cphopt != 2) return CRESS_EUNSUPPORTED;
+  if (mode == 2 && !x->ml.flag) {
+    const long n = x->n;
+    float *f = nullptr;
+    int rc = dalloc(x, &f, (size_t)n);
+    if (!rc) x->ml.flag = (int *)f;
+    if (!rc) rc = dalloc(x, &f, (size_t)n);
+    if (!rc) x->ml.list = (int *)f;
+    if (!rc) rc = dalloc(x, &f, (size_t)n * MLP_INP / 2);
+    if (!rc) x->ml.x = (unsigned short *)f;
+    if (!rc) rc = dalloc(x, &f, 1);
+    if (!rc) x->ml.count = (unsigned *)f;
+    if (!rc) rc = dalloc(x, &f, (size_t)n * MLP_NOUT);
+    if (!rc) x->ml.out = f;
+    if (rc) {
+      x->ml = MlpStep{nullptr, nullptr, nullptr, nullptr, nullptr};
+      return rc;
+    }
+  }
+  x->mlp = mode ? mlp : nullptr;
+  x->mlp_mode = mode;
   return CRESS_OK;
 }

@@ -3,6 +3,7 @@
 // (satadjst.f90:148-189).  The context's step driver launches them.
 #pragma once
 #include "common.h"
+#include "mlp.h"
 #include "obc.h"
 
 namespace cress {
@@ -29,8 +30,12 @@ struct MicroK {
 struct MicroArgs {
   const float *jcb, *pbr, *ptbr, *rbr, *rst;
   const float *ptfrc, *qvfrc, *sfrc;    // steps forcings (sfrc: qc, qr, qi, qs, qg, nci)
-  const float *ppp, *ptpp, *qp, *ncip;  // past level (qp: six species stacked)
-  float *ptpf, *qf, *ncif;              // future level
+  // past level: qp holds qv, qpw the liquid species (qc, qr) and qpi the
+  // ice species (qi, qs, qg), each block of qn floats per species as the
+  // reference's qv, qwtr and qice arrays; the context's stacked six-species
+  // block sets qpw = qp + qn, qpi = qp + 3 qn
+  const float *ppp, *ptpp, *qp, *qpw, *qpi, *ncip;
+  float *ptpf, *qf, *qfw, *qfi, *ncif;  // future level, the same layout
   float *urq, *usq, *ugq;               // fall speeds handed to k_micro_fall
   unsigned *dtpmin;                     // binary32 bits of the CFL minimum
   long qn;                              // floats per species block
@@ -43,7 +48,27 @@ struct MicroArgs {
   // input on its own); 0: it is formed here from the past level and the
   // forcings
   int given;
+  // sources: 0 srcblk; 1 srcblk and record (state, tendency) pairs into
+  // cap; 2 per cell as k_mlp_select chose (mflag), the network's
+  // tendencies from mout (7 blocks of qn floats)
+  int mode;
+  MlpCapture cap;
+  const int *mflag;
+  const float *mout;
 };
+
+// The network's choice per cell and the list of its cells (MlpStep), from
+// the past level: no active process (no hydrometeor above thresq and no
+// ice nucleation from vapour) -> 2; active inside the training envelope ->
+// 1, with the normalised features appended to the list; else 0 (srcblk).
+struct MlpSelectArgs {
+  const float *pbr, *ptbr, *rbr, *ppp, *ptpp, *qp, *qpw, *qpi;
+  long qn;
+  float thresq;
+  MlpStep s;
+  const float *norm;
+};
+__global__ void k_mlp_select(Tiles T, Grid g, MlpSelectArgs a);
 
 // First column of the sedimentation kernel's x tiles (measured at
 // 1024x1024x80: from i=1 2.85 ms, from i=0 3.24 ms per launch).
@@ -52,8 +77,8 @@ struct MicroArgs {
 #endif
 
 struct FallArgs {
-  const float *rbr, *rst, *qp, *urq, *usq, *ugq;
-  float *qf, *ncif, *prwtr, *price;
+  const float *rbr, *rst, *qpw, *qpi, *urq, *usq, *ugq;  // species blocks as in MicroArgs
+  float *qfw, *qfi, *ncif, *prwtr, *price;
   const unsigned *dtpmin;
   long qn;   // floats per species block
   long n2;   // floats per 2-D level of the precipitation arrays

@@ -103,8 +103,8 @@ __global__ void __launch_bounds__(256) CRESS_MICRO_ATTR k_micro_src(Tiles T, Gri
     const float pr = MP(CRESS_PBR, a.pbr, p, k) + a.ppp[p];
     const float pi = ex((rd / cp) * lg((1.0f / p0) * pr));
     // past level
-    const float qvp = a.qp[p], qcp = a.qp[p + qn], qrp = a.qp[p + 2 * qn], qip = a.qp[p + 3 * qn];
-    const float qsp = a.qp[p + 4 * qn], qgp = a.qp[p + 5 * qn], ncip = a.ncip[p];
+    const float qvp = a.qp[p], qcp = a.qpw[p], qrp = a.qpw[p + qn], qip = a.qpi[p];
+    const float qsp = a.qpi[p + qn], qgp = a.qpi[p + 2 * qn], ncip = a.ncip[p];
     // diagnw / diagnsg
     const float nccp = nclcst * (1.0f / rbr);
     const float ncrp = fminr(fmaxr(sqrtf(sqrtf((nr0 * nr0 * nr0 / (cc * rhow)) * rbr * qrp)) * (1.0f / rbr),
@@ -124,11 +124,11 @@ __global__ void __launch_bounds__(256) CRESS_MICRO_ATTR k_micro_src(Tiles T, Gri
     if (a.given) {
       ptpf = a.ptpf[p];
       qvf = a.qf[p];
-      qcf = a.qf[p + qn];
-      qrf = a.qf[p + 2 * qn];
-      qif = a.qf[p + 3 * qn];
-      qsf = a.qf[p + 4 * qn];
-      qgf = a.qf[p + 5 * qn];
+      qcf = a.qfw[p];
+      qrf = a.qfw[p + qn];
+      qif = a.qfi[p];
+      qsf = a.qfi[p + qn];
+      qgf = a.qfi[p + 2 * qn];
       ncif = a.ncif[p];
     } else if (a.open && (i == 1 || i == g.ni - 1 || j == 1 || j == g.nj - 1)) {
       ptpf = obc_lf_ring(g, a.lf, false, a.ptpc, a.ptpp, i, j, ks);
@@ -146,11 +146,11 @@ __global__ void __launch_bounds__(256) CRESS_MICRO_ATTR k_micro_src(Tiles T, Gri
       const float d = dtb / MP(CRESS_RST, a.rst, ps, ks);   // dtb is the leapfrog's 2*dtb here
       ptpf = a.ptpp[ps] + a.ptfrc[ps] * d;
       qvf = fmaxr(a.qp[ps] + a.qvfrc[ps] * d, 0.0f);
-      qcf = fmaxr(a.qp[ps + qn] + a.sfrc[ps] * d, 0.0f);
-      qrf = fmaxr(a.qp[ps + 2 * qn] + a.sfrc[ps + qn] * d, 0.0f);
-      qif = fmaxr(a.qp[ps + 3 * qn] + a.sfrc[ps + 2 * qn] * d, 0.0f);
-      qsf = fmaxr(a.qp[ps + 4 * qn] + a.sfrc[ps + 3 * qn] * d, 0.0f);
-      qgf = fmaxr(a.qp[ps + 5 * qn] + a.sfrc[ps + 4 * qn] * d, 0.0f);
+      qcf = fmaxr(a.qpw[ps] + a.sfrc[ps] * d, 0.0f);
+      qrf = fmaxr(a.qpw[ps + qn] + a.sfrc[ps + qn] * d, 0.0f);
+      qif = fmaxr(a.qpi[ps] + a.sfrc[ps + 2 * qn] * d, 0.0f);
+      qsf = fmaxr(a.qpi[ps + qn] + a.sfrc[ps + 3 * qn] * d, 0.0f);
+      qgf = fmaxr(a.qpi[ps + 2 * qn] + a.sfrc[ps + 4 * qn] * d, 0.0f);
       ncif = a.ncip[ps] + a.sfrc[ps + 5 * qn] * d;
     }
     // adjstnci before the sources
@@ -165,347 +165,407 @@ __global__ void __launch_bounds__(256) CRESS_MICRO_ATTR k_micro_src(Tiles T, Gri
         ? K.cusq * ex(gus * lnr0r) * ex((bus * (1.0f / 3.0f)) * lg((1.0f / (cc * rhos)) * qsp / ncsp)) : 0.0f;
     const float ugq = qgp > thresq
         ? K.cugq * ex(gug * lnr0r) * ex((bug * (1.0f / 3.0f)) * lg((1.0f / (cc * rhog)) * qgp / ncgp)) : 0.0f;
-    // ---- setblk ----
-    const float oned3 = 1.0f / 3.0f;
-    const float t = (MP(CRESS_PTBR, a.ptbr, p, k) + a.ptpp[p]) * pi;
-    const float tcel = t - t0;
-    const float esw = es0 * ex(17.269f * tcel / (t - 35.86f));
-    const float esi = es0 * ex(21.875f * tcel / (t - 7.66f));
-    const float qvsst0 = qvp - (epsva * es0) / (pr - es0);
-    const float qvsw = epsva * esw / (pr - esw);
-    const float qvsi = epsva * esi / (pr - esi);
-    const float lv = lv0 * ex((0.167f + 3.67e-4f * t) * lg(t0 / t));
-    const float lf = lf0 + (cw - ci) * tcel;
-    const float ls = lv + lf;
-    const float p20dvp = p20 / pr;
-    const float nu = p20dvp * ex(1.754f * lg(K.cnu * t));
-    const float kp = (kp0 / (120.0f + t)) * ex(1.5f * lg((1.0f / t23) * t));
-    const float mu = rbr * nu;
-    const float dv = p20dvp * ex(1.81f * lg(K.cdv * t));
-    const float cvnt = sqrtf(sqrtf(r0 * rbv) / nu);
-    const float diaqc = qcp > thresq ? ex(oned3 * lg((6.0f / (cc * rhow)) * qcp / nccp)) : 0.0f;
-    float diaqr = 0.0f, vntr = 0.0f, mi = 0.0f, diaqi = 0.0f, diaqs = 0.0f, vnts = 0.0f, diaqg = 0.0f, vntg = 0.0f;
-    if (qrp > thresq) {
-      diaqr = ex(oned3 * lg((1.0f / (cc * rhow)) * qrp / ncrp));
-      vntr = rbr * ncrp * (0.78f * diaqr + cvnt * K.cvntr * ex(K.pdiaqr * lg(diaqr)));
-    }
-    if (qip > thresq) {
-      mi = qip / ncip;
-      diaqi = ex(oned3 * lg((6.0f / (cc * rhoi)) * mi));
-    }
-    if (qsp > thresq) {
-      diaqs = ex(oned3 * lg((1.0f / (cc * rhos)) * qsp / ncsp));
-      vnts = rbr * ncsp * (0.78f * diaqs + cvnt * K.cvnts * ex(K.pdiaqs * lg(diaqs)));
-    }
-    if (qgp > thresq) {
-      diaqg = ex(oned3 * lg((1.0f / (cc * rhog)) * qgp / ncgp));
-      vntg = rbr * ncgp * (0.78f * diaqg + cvnt * K.cvntg * ex(K.pdiaqg * lg(diaqg)));
-    }
-    // ---- nuc1stv ----
-    float nuvi = 0.0f;
-    if (qvp > thresq && qvp > qvsi && t > tlow && t < t0) {
-      const float qvssi = qvp - qvsi;
-      const float aa = 15.25f * qvp / qvsi - 10.08f;
-      nuvi = aa < 40.0f ? fmaxr(0.0f, fminr(mi0 * ex(aa) * rbv - qip, qvssi)) : qvssi;
-    }
-    // ---- nuc1stc ----
-    float nuci = 0.0f;
-    if (qcp > thresq) {
-      const float nufci = (t > tlow && t < t0)
-          ? fminr(K.rwdt2 * (ex(-0.66f * tcel) - 1.0f) * qcp * qcp / nccp, qcp) : 0.0f;
-      float nucci = 0.0f;
-      if (t > tlow && t < 270.16f) {
-        const float piv = 1.0f / pr;
-        const float knd = K.cknd * piv * t;
-        const float dar = K.cdar * t * (1.0f + knd) / mu;
-        const float f1 = K.cc45 * ex(1.3f * lg(270.16f - t)) * diaqc;
-        // the reference's contact branch compares tc with t where tc=t, so
-        // its thermophoretic term is always zero (nuc1stc.f90:334-360)
-        nucci = (0.0f + f1 * dar) * qcp * dtb;
-      }
-      const float nuhci = t <= tlow ? qcp : 0.0f;
-      nuci = nufci + nucci + nuhci;
-    }
-    // ---- collect ----
-    const float r0rsq = sqrtf(r0 * rbr);
-    float diaqr2 = 0.0f, diaqr3 = 0.0f, qr2b = 0.0f, diaqs2 = 0.0f, diaqs3 = 0.0f, qs2b = 0.0f;
-    float diaqg2 = 0.0f, qg2b = 0.0f;
-    if (qrp > thresq) { diaqr2 = diaqr * diaqr; diaqr3 = diaqr * diaqr2; qr2b = r0rsq * ex((2.0f + bur) * lg(diaqr)); }
-    if (qsp > thresq) { diaqs2 = diaqs * diaqs; diaqs3 = diaqs * diaqs2; qs2b = r0rsq * ex((2.0f + bus) * lg(diaqs)); }
-    if (qgp > thresq) { diaqg2 = diaqg * diaqg; qg2b = r0rsq * ex((2.0f + bug) * lg(diaqg)); }
-    float clcr = 0.0f, clcs = 0.0f, clcg = 0.0f, clri = 0.0f, clrs = 0.0f, clrg = 0.0f, clir = 0.0f, clsr = 0.0f;
-    float clis = 0.0f, clig = 0.0f, clsg = 0.0f, aa, bb, sink;
-    if (t > tlow) {
-      if (qcp > thresq) {
-        const float cstk = ((1.0f / 9.0f) * rhow) * diaqc * diaqc;
-        if (qrp > thresq) {
-          aa = cstk * urq / (diaqr * mu);
-          bb = aa + 0.5f;
-          aa = aa * aa / (bb * bb);
-          clcr = K.cclcr * aa * qr2b * ncrp * qcp;
-        }
-        if (qsp > thresq) {
-          aa = cstk * usq / (diaqs * mu);
-          bb = aa + 0.5f;
-          const float ecs = aa * aa / (bb * bb);
-          clcs = K.cclcs * qs2b * ecs * ncsp * qcp;
-        }
-        if (qgp > thresq) {
-          aa = cstk * ugq / (diaqg * mu);
-          bb = aa + 0.5f;
-          aa = aa * aa / (bb * bb);
-          clcg = K.cclcg * aa * qg2b * ncgp * qcp;
-        }
-        sink = clcr + clcs + clcg;
-        if (qcp < sink) { aa = qcp / sink; clcr = clcr * aa; clcs = clcs * aa; clcg = clcg * aa; }
-      }
+    // ---- sources: srcblk (plain), the network's tendencies, or nothing ----
+    // mode 2 reads the cell's choice from k_mlp_select; every other mode
+    // runs srcblk, and mode 1 also records the cell's (state, tendency)
+    // pair when a process is active.
+    const int how = a.mode == 2 ? a.mflag[p] : 0;
+    if (how == 0) {
+      const float ptpf0 = ptpf, qvf0 = qvf, qcf0 = qcf, qrf0 = qrf, qif0 = qif, qsf0 = qsf, qgf0 = qgf;
+      // ---- setblk ----
+      const float oned3 = 1.0f / 3.0f;
+      const float t = (MP(CRESS_PTBR, a.ptbr, p, k) + a.ptpp[p]) * pi;
+      const float tcel = t - t0;
+      const float esw = es0 * ex(17.269f * tcel / (t - 35.86f));
+      const float esi = es0 * ex(21.875f * tcel / (t - 7.66f));
+      const float qvsst0 = qvp - (epsva * es0) / (pr - es0);
+      const float qvsw = epsva * esw / (pr - esw);
+      const float qvsi = epsva * esi / (pr - esi);
+      const float lv = lv0 * ex((0.167f + 3.67e-4f * t) * lg(t0 / t));
+      const float lf = lf0 + (cw - ci) * tcel;
+      const float ls = lv + lf;
+      const float p20dvp = p20 / pr;
+      const float nu = p20dvp * ex(1.754f * lg(K.cnu * t));
+      const float kp = (kp0 / (120.0f + t)) * ex(1.5f * lg((1.0f / t23) * t));
+      const float mu = rbr * nu;
+      const float dv = p20dvp * ex(1.81f * lg(K.cdv * t));
+      const float cvnt = sqrtf(sqrtf(r0 * rbv) / nu);
+      const float diaqc = qcp > thresq ? ex(oned3 * lg((6.0f / (cc * rhow)) * qcp / nccp)) : 0.0f;
+      float diaqr = 0.0f, vntr = 0.0f, mi = 0.0f, diaqi = 0.0f, diaqs = 0.0f, vnts = 0.0f, diaqg = 0.0f, vntg = 0.0f;
       if (qrp > thresq) {
-        if (qip > thresq && t < t0) { clri = K.cclri * qr2b * ncrp * qip; clir = clri; }
-        if (qsp > thresq) {
-          aa = 2.0f * diaqr * diaqs;
-          bb = urq - usq;
-          bb = ncrp * ncsp * rbr * sqrtf(bb * bb + 0.04f * urq * usq);
-          clrs = bb * K.cclrs * (5.0f * diaqr2 + aa + 0.5f * diaqs2) * diaqr3;
-          clsr = bb * K.cclsr * (5.0f * diaqs2 + aa + 0.5f * diaqr2) * diaqs3;
-        }
-        if (qgp > thresq) {
-          aa = 5.0f * diaqr2 + 2.0f * diaqr * diaqg + 0.5f * diaqg2;
-          bb = urq - ugq;
-          clrg = aa * K.cclrg * diaqr3 * ncrp * ncgp * rbr * sqrtf(bb * bb + 0.04f * urq * ugq);
-        }
-        sink = clri + clrs + clrg;
-        if (qrp < sink) { aa = qrp / sink; clri = clri * aa; clrs = clrs * aa; clrg = clrg * aa; }
+        diaqr = ex(oned3 * lg((1.0f / (cc * rhow)) * qrp / ncrp));
+        vntr = rbr * ncrp * (0.78f * diaqr + cvnt * K.cvntr * ex(K.pdiaqr * lg(diaqr)));
       }
-    }
-    if (qip > thresq && t < t0) {
-      if (qsp > thresq) clis = K.cclis * qs2b * ncsp * qip;
-      if (qgp > thresq) clig = K.cclig * qg2b * ncgp * qip;
-      sink = clir + clis + clig;
-      if (qip < sink) { aa = qip / sink; clir = clir * aa; clis = clis * aa; clig = clig * aa; }
-    }
-    if (qsp > thresq) {
+      if (qip > thresq) {
+        mi = qip / ncip;
+        diaqi = ex(oned3 * lg((6.0f / (cc * rhoi)) * mi));
+      }
+      if (qsp > thresq) {
+        diaqs = ex(oned3 * lg((1.0f / (cc * rhos)) * qsp / ncsp));
+        vnts = rbr * ncsp * (0.78f * diaqs + cvnt * K.cvnts * ex(K.pdiaqs * lg(diaqs)));
+      }
       if (qgp > thresq) {
-        aa = 5.0f * diaqs2 + 2.0f * diaqs * diaqg + 0.5f * diaqg2;
-        bb = usq - ugq;
-        clsg = aa * (t < t0 ? K.cclsg : K.cclsg3) * diaqs3 * ncsp * ncgp * rbr * sqrtf(bb * bb + 0.04f * usq * ugq);
+        diaqg = ex(oned3 * lg((1.0f / (cc * rhog)) * qgp / ncgp));
+        vntg = rbr * ncgp * (0.78f * diaqg + cvnt * K.cvntg * ex(K.pdiaqg * lg(diaqg)));
       }
-      sink = clsr + clsg;
-      if (qsp < sink) { aa = qsp / sink; clsr = clsr * aa; clsg = clsg * aa; }
-    }
-    // ---- prodctwg ----
-    float pgwet = -1.0f;
-    if (qgp > thresq && tcel < t0cel) {
-      const float pgdry = clcg + clrg + clig + clsg;
-      const float lfice = lf + cw * tcel;
-      const float cligw = (1.0f / eig) * clig, clsgw = (1.0f / esg) * clsg;
-      const float pg = K.cc2dtn * vntg * (lv * dv * rbr * qvsst0 + kp * tcel) / (lfice * rbr) +
-                       (cligw + clsgw) * (1.0f - ci * tcel / lfice);
-      if (pg > 0.0f && pg < pgdry) {
-        pgwet = pg;
-        clig = cligw;
-        clsg = clsgw;
+      // ---- nuc1stv ----
+      float nuvi = 0.0f;
+      if (qvp > thresq && qvp > qvsi && t > tlow && t < t0) {
+        const float qvssi = qvp - qvsi;
+        const float aa = 15.25f * qvp / qvsi - 10.08f;
+        nuvi = aa < 40.0f ? fmaxr(0.0f, fminr(mi0 * ex(aa) * rbv - qip, qvssi)) : qvssi;
+      }
+      // ---- nuc1stc ----
+      float nuci = 0.0f;
+      if (qcp > thresq) {
+        const float nufci = (t > tlow && t < t0)
+            ? fminr(K.rwdt2 * (ex(-0.66f * tcel) - 1.0f) * qcp * qcp / nccp, qcp) : 0.0f;
+        float nucci = 0.0f;
+        if (t > tlow && t < 270.16f) {
+          const float piv = 1.0f / pr;
+          const float knd = K.cknd * piv * t;
+          const float dar = K.cdar * t * (1.0f + knd) / mu;
+          const float f1 = K.cc45 * ex(1.3f * lg(270.16f - t)) * diaqc;
+          // the reference's contact branch compares tc with t where tc=t, so
+          // its thermophoretic term is always zero (nuc1stc.f90:334-360)
+          nucci = (0.0f + f1 * dar) * qcp * dtb;
+        }
+        const float nuhci = t <= tlow ? qcp : 0.0f;
+        nuci = nufci + nucci + nuhci;
+      }
+      // ---- collect ----
+      const float r0rsq = sqrtf(r0 * rbr);
+      float diaqr2 = 0.0f, diaqr3 = 0.0f, qr2b = 0.0f, diaqs2 = 0.0f, diaqs3 = 0.0f, qs2b = 0.0f;
+      float diaqg2 = 0.0f, qg2b = 0.0f;
+      if (qrp > thresq) { diaqr2 = diaqr * diaqr; diaqr3 = diaqr * diaqr2; qr2b = r0rsq * ex((2.0f + bur) * lg(diaqr)); }
+      if (qsp > thresq) { diaqs2 = diaqs * diaqs; diaqs3 = diaqs * diaqs2; qs2b = r0rsq * ex((2.0f + bus) * lg(diaqs)); }
+      if (qgp > thresq) { diaqg2 = diaqg * diaqg; qg2b = r0rsq * ex((2.0f + bug) * lg(diaqg)); }
+      float clcr = 0.0f, clcs = 0.0f, clcg = 0.0f, clri = 0.0f, clrs = 0.0f, clrg = 0.0f, clir = 0.0f, clsr = 0.0f;
+      float clis = 0.0f, clig = 0.0f, clsg = 0.0f, aa, bb, sink;
+      if (t > tlow) {
+        if (qcp > thresq) {
+          const float cstk = ((1.0f / 9.0f) * rhow) * diaqc * diaqc;
+          if (qrp > thresq) {
+            aa = cstk * urq / (diaqr * mu);
+            bb = aa + 0.5f;
+            aa = aa * aa / (bb * bb);
+            clcr = K.cclcr * aa * qr2b * ncrp * qcp;
+          }
+          if (qsp > thresq) {
+            aa = cstk * usq / (diaqs * mu);
+            bb = aa + 0.5f;
+            const float ecs = aa * aa / (bb * bb);
+            clcs = K.cclcs * qs2b * ecs * ncsp * qcp;
+          }
+          if (qgp > thresq) {
+            aa = cstk * ugq / (diaqg * mu);
+            bb = aa + 0.5f;
+            aa = aa * aa / (bb * bb);
+            clcg = K.cclcg * aa * qg2b * ncgp * qcp;
+          }
+          sink = clcr + clcs + clcg;
+          if (qcp < sink) { aa = qcp / sink; clcr = clcr * aa; clcs = clcs * aa; clcg = clcg * aa; }
+        }
+        if (qrp > thresq) {
+          if (qip > thresq && t < t0) { clri = K.cclri * qr2b * ncrp * qip; clir = clri; }
+          if (qsp > thresq) {
+            aa = 2.0f * diaqr * diaqs;
+            bb = urq - usq;
+            bb = ncrp * ncsp * rbr * sqrtf(bb * bb + 0.04f * urq * usq);
+            clrs = bb * K.cclrs * (5.0f * diaqr2 + aa + 0.5f * diaqs2) * diaqr3;
+            clsr = bb * K.cclsr * (5.0f * diaqs2 + aa + 0.5f * diaqr2) * diaqs3;
+          }
+          if (qgp > thresq) {
+            aa = 5.0f * diaqr2 + 2.0f * diaqr * diaqg + 0.5f * diaqg2;
+            bb = urq - ugq;
+            clrg = aa * K.cclrg * diaqr3 * ncrp * ncgp * rbr * sqrtf(bb * bb + 0.04f * urq * ugq);
+          }
+          sink = clri + clrs + clrg;
+          if (qrp < sink) { aa = qrp / sink; clri = clri * aa; clrs = clrs * aa; clrg = clrg * aa; }
+        }
+      }
+      if (qip > thresq && t < t0) {
+        if (qsp > thresq) clis = K.cclis * qs2b * ncsp * qip;
+        if (qgp > thresq) clig = K.cclig * qg2b * ncgp * qip;
         sink = clir + clis + clig;
         if (qip < sink) { aa = qip / sink; clir = clir * aa; clis = clis * aa; clig = clig * aa; }
+      }
+      if (qsp > thresq) {
+        if (qgp > thresq) {
+          aa = 5.0f * diaqs2 + 2.0f * diaqs * diaqg + 0.5f * diaqg2;
+          bb = usq - ugq;
+          clsg = aa * (t < t0 ? K.cclsg : K.cclsg3) * diaqs3 * ncsp * ncgp * rbr * sqrtf(bb * bb + 0.04f * usq * ugq);
+        }
         sink = clsr + clsg;
         if (qsp < sink) { aa = qsp / sink; clsr = clsr * aa; clsg = clsg * aa; }
       }
-    }
-    // ---- distrpg ----
-    float clrsg = 0.0f;
-    if (qsp > thresq) {
-      if (t < t0) {
-        aa = diaqs * diaqs;
-        bb = diaqr * diaqr;
-        aa = (rhos * rhos) * aa * aa * aa;
-        bb = (rhow * rhow) * bb * bb * bb;
-        const float alpha = aa / (aa + bb), alpha1 = 1.0f - alpha;
-        clrsg = alpha1 * clrs;
-        clrs = alpha * clrs;
-        clsr = alpha1 * clsr;
-      } else {
-        clrsg = clrs;
-        clrs = 0.0f;
+      // ---- prodctwg ----
+      float pgwet = -1.0f;
+      if (qgp > thresq && tcel < t0cel) {
+        const float pgdry = clcg + clrg + clig + clsg;
+        const float lfice = lf + cw * tcel;
+        const float cligw = (1.0f / eig) * clig, clsgw = (1.0f / esg) * clsg;
+        const float pg = K.cc2dtn * vntg * (lv * dv * rbr * qvsst0 + kp * tcel) / (lfice * rbr) +
+                         (cligw + clsgw) * (1.0f - ci * tcel / lfice);
+        if (pg > 0.0f && pg < pgdry) {
+          pgwet = pg;
+          clig = cligw;
+          clsg = clsgw;
+          sink = clir + clis + clig;
+          if (qip < sink) { aa = qip / sink; clir = clir * aa; clis = clis * aa; clig = clig * aa; }
+          sink = clsr + clsg;
+          if (qsp < sink) { aa = qsp / sink; clsr = clsr * aa; clsg = clsg * aa; }
+        }
       }
-    }
-    // ---- aggregat ----
-    const float agin = qip > thresq ? rbr * ncip * qip * ex(oned3 * lg(K.cagin_ag * rbv)) : 0.0f;
-    // ---- melting ----
-    const float cmlxr1 = K.cc2dt * rbv * (kp * tcel + lv * dv * rbr * qvsst0);
-    const float cmlxr2 = cw * tcel;
-    float mlic = (qip > thresq && tcel >= t0cel) ? qip : 0.0f;
-    float mlsr = (qsp > thresq && tcel >= t0cel) ? fmaxr((cmlxr1 * vnts + cmlxr2 * (clcs + clrs)) / lf, 0.0f) : 0.0f;
-    float mlgr = (qgp > thresq && tcel >= t0cel) ? fmaxr((cmlxr1 * vntg + cmlxr2 * (clcg + clrg)) / lf, 0.0f) : 0.0f;
-    // ---- depsit ----
-    float vdvr = 0.0f, vdvi = 0.0f, vdvs = 0.0f, vdvg = 0.0f;
-    if (t > tlow) {
-      const float qvssw = qvp - qvsw, qvssi = qvp - qvsi;
-      const float ad = 1.0f / (rv * kp * t * t);
-      const float bd = dv * rbr;
-      const float cd = K.ccdtb4 * rbv;
-      const float gi = 1.0f / (ls * ls * ad + 1.0f / (bd * qvsi));
-      const float cvdvx1 = cd * (qvp / qvsw - 1.0f) / (lv * lv * ad + 1.0f / (bd * qvsw));
-      const float cvdvx2 = K.ccdtb4 * dv * qvsst0;
-      const float cvdvx3 = cd * (qvp / qvsi - 1.0f);
-      const float cvdvx4 = ad * ls * lf;
-      if (qrp > thresq && qvp < qvsw) vdvr = fmaxr(cvdvx1 * vntr, qvssw);
-      if (qip > thresq && t < t0 - 0.01f) {
-        const int itsc = (int)(-tcel);
-        vdvi = c_ckoe[itsc] * (qvp - qvsi) / (qvsw - qvsi) * ncip * ex(c_pkoe[itsc] * lg(mi)) * dtb;
+      // ---- distrpg ----
+      float clrsg = 0.0f;
+      if (qsp > thresq) {
+        if (t < t0) {
+          aa = diaqs * diaqs;
+          bb = diaqr * diaqr;
+          aa = (rhos * rhos) * aa * aa * aa;
+          bb = (rhow * rhow) * bb * bb * bb;
+          const float alpha = aa / (aa + bb), alpha1 = 1.0f - alpha;
+          clrsg = alpha1 * clrs;
+          clrs = alpha * clrs;
+          clsr = alpha1 * clsr;
+        } else {
+          clrsg = clrs;
+          clrs = 0.0f;
+        }
+      }
+      // ---- aggregat ----
+      const float agin = qip > thresq ? rbr * ncip * qip * ex(oned3 * lg(K.cagin_ag * rbv)) : 0.0f;
+      // ---- melting ----
+      const float cmlxr1 = K.cc2dt * rbv * (kp * tcel + lv * dv * rbr * qvsst0);
+      const float cmlxr2 = cw * tcel;
+      float mlic = (qip > thresq && tcel >= t0cel) ? qip : 0.0f;
+      float mlsr = (qsp > thresq && tcel >= t0cel) ? fmaxr((cmlxr1 * vnts + cmlxr2 * (clcs + clrs)) / lf, 0.0f) : 0.0f;
+      float mlgr = (qgp > thresq && tcel >= t0cel) ? fmaxr((cmlxr1 * vntg + cmlxr2 * (clcg + clrg)) / lf, 0.0f) : 0.0f;
+      // ---- depsit ----
+      float vdvr = 0.0f, vdvi = 0.0f, vdvs = 0.0f, vdvg = 0.0f;
+      if (t > tlow) {
+        const float qvssw = qvp - qvsw, qvssi = qvp - qvsi;
+        const float ad = 1.0f / (rv * kp * t * t);
+        const float bd = dv * rbr;
+        const float cd = K.ccdtb4 * rbv;
+        const float gi = 1.0f / (ls * ls * ad + 1.0f / (bd * qvsi));
+        const float cvdvx1 = cd * (qvp / qvsw - 1.0f) / (lv * lv * ad + 1.0f / (bd * qvsw));
+        const float cvdvx2 = K.ccdtb4 * dv * qvsst0;
+        const float cvdvx3 = cd * (qvp / qvsi - 1.0f);
+        const float cvdvx4 = ad * ls * lf;
+        if (qrp > thresq && qvp < qvsw) vdvr = fmaxr(cvdvx1 * vntr, qvssw);
+        if (qip > thresq && t < t0 - 0.01f) {
+          const int itsc = (int)(-tcel);
+          vdvi = c_ckoe[itsc] * (qvp - qvsi) / (qvsw - qvsi) * ncip * ex(c_pkoe[itsc] * lg(mi)) * dtb;
+        }
+        if (qsp > thresq) {
+          if (t > t0) vdvs = mlsr > 0.0f ? cvdvx2 * vnts : cvdvx1 * vnts;
+          else vdvs = gi * (cvdvx3 * vnts - cvdvx4 * clcs);
+        }
+        if (qgp > thresq) {
+          if (t > t0) vdvg = mlgr > 0.0f ? cvdvx2 * vntg : cvdvx1 * vntg;
+          else vdvg = gi * (cvdvx3 * vntg - cvdvx4 * clcg);
+        }
+        sink = vdvi + vdvs + vdvg;
+        if ((qvssi < sink && qvssi > 0.0f) || (qvssi > sink && qvssi < 0.0f)) {
+          const float h = qvssi / sink;
+          vdvi = vdvi * h; vdvs = vdvs * h; vdvg = vdvg * h;
+        }
+      }
+      // ---- convers ----
+      float cncr = 0.0f, cnis = 0.0f, cnsg = 0.0f;
+      if (qcp > thresq && t > tlow && qcp >= K.qccm * nccp)
+        cncr = K.ccncr * rbr * ex(-oned3 * lg(nccp)) * ex((7.0f / 3.0f) * lg(qcp)) / mu;
+      if (t < t0) {
+        if (qip > thresq) {
+          float cn = rbr * ex(oned3 * lg(K.cagin_cv * rbv)) * qip * qip / lg(K.diaqs0 / diaqi);
+          if (vdvi >= 0.0f) {
+            const float ms05 = 0.5f * ms0;
+            if (mi < ms05) cn = cn + mi / (ms0 - mi) * vdvi;
+            else cn = cn + (vdvi + (1.0f - ms05 / mi) * qip);
+          }
+          cnis = fminr(cn, qip);
+        }
+        if (qsp > thresq && vdvs > 0.0f && vdvs < clcs) cnsg = fminr(K.ccnsg * clcs, qsp);
+      }
+      // ---- nuc2nd ----
+      float spsi = 0.0f, spgi = 0.0f;
+      if (t <= 270.16f && t > 265.16f) {
+        const float fr = t > 268.16f ? (270.16f - t) * K.mi0352 * rbv : (t - 265.16f) * K.mi0353 * rbv;
+        spsi = fr * clcs;
+        spgi = pgwet > 0.0f ? 0.0f : fr * clcg;
+      }
+      // ---- freezing ----
+      float frrg = 0.0f;
+      if (qrp > thresq && tcel < t0cel) {
+        if (tcel <= tlow - t0) {
+          frrg = qrp;
+        } else {
+          const float d3 = diaqr * diaqr * diaqr;
+          frrg = fminr(qrp, K.cfrrg * d3 * d3 * (ex(-0.66f * tcel) - 1.0f) * ncrp);
+        }
+      }
+      // ---- shedding ----
+      float shsr = (qsp > thresq && t >= t0) ? clcs + clrs : 0.0f;
+      float shgr = 0.0f;
+      if (qgp > thresq) {
+        if (t >= t0) shgr = clcg + clrg;
+        else if (pgwet > 0.0f) shgr = (clcg + clrg + clig + clsg) - pgwet;
+      }
+      // ---- more0q ----
+      float h;
+      if (qcp > thresq) {
+        sink = nuci + clcr + clcs + clcg + cncr - mlic;
+        if (qcf < sink) {
+          h = qcf / sink;
+          nuci *= h; clcr *= h; clcs *= h; clcg *= h; cncr *= h; mlic *= h;
+        }
+      }
+      if (qip > thresq) {
+        sink = clir + clis + clig + cnis + mlic - vdvi - nuvi - nuci - spsi - spgi;
+        if (qif < sink) {
+          h = qif / sink;
+          clir *= h; clis *= h; clig *= h; cnis *= h; mlic *= h; vdvi *= h; nuvi *= h; nuci *= h; spsi *= h; spgi *= h;
+        }
+      }
+      if (qrp > thresq) {
+        sink = clri + clrs + clrg + clrsg + frrg - vdvr - clcr - cncr - mlsr - mlgr - shsr - shgr;
+        if (qrf < sink) {
+          h = qrf / sink;
+          clri *= h; clrs *= h; clrg *= h; clrsg *= h; frrg *= h; vdvr *= h; clcr *= h; cncr *= h;
+          mlsr *= h; mlgr *= h; shsr *= h; shgr *= h;
+        }
       }
       if (qsp > thresq) {
-        if (t > t0) vdvs = mlsr > 0.0f ? cvdvx2 * vnts : cvdvx1 * vnts;
-        else vdvs = gi * (cvdvx3 * vnts - cvdvx4 * clcs);
+        sink = clsr + clsg + cnsg + spsi + mlsr + shsr - vdvs - clcs - clrs - clis - cnis;
+        if (qsf < sink) {
+          h = qsf / sink;
+          clsr *= h; clsg *= h; cnsg *= h; spsi *= h; mlsr *= h; shsr *= h; vdvs *= h; clcs *= h; clrs *= h;
+          clis *= h; cnis *= h;
+        }
       }
       if (qgp > thresq) {
-        if (t > t0) vdvg = mlgr > 0.0f ? cvdvx2 * vntg : cvdvx1 * vntg;
-        else vdvg = gi * (cvdvx3 * vntg - cvdvx4 * clcg);
-      }
-      sink = vdvi + vdvs + vdvg;
-      if ((qvssi < sink && qvssi > 0.0f) || (qvssi > sink && qvssi < 0.0f)) {
-        const float h = qvssi / sink;
-        vdvi = vdvi * h; vdvs = vdvs * h; vdvg = vdvg * h;
-      }
-    }
-    // ---- convers ----
-    float cncr = 0.0f, cnis = 0.0f, cnsg = 0.0f;
-    if (qcp > thresq && t > tlow && qcp >= K.qccm * nccp)
-      cncr = K.ccncr * rbr * ex(-oned3 * lg(nccp)) * ex((7.0f / 3.0f) * lg(qcp)) / mu;
-    if (t < t0) {
-      if (qip > thresq) {
-        float cn = rbr * ex(oned3 * lg(K.cagin_cv * rbv)) * qip * qip / lg(K.diaqs0 / diaqi);
-        if (vdvi >= 0.0f) {
-          const float ms05 = 0.5f * ms0;
-          if (mi < ms05) cn = cn + mi / (ms0 - mi) * vdvi;
-          else cn = cn + (vdvi + (1.0f - ms05 / mi) * qip);
+        sink = spgi + mlgr + shgr - vdvg - clri - clir - clsr - clcg - clrg - clig - clsg - clrsg - cnsg - frrg;
+        if (qgf < sink) {
+          h = qgf / sink;
+          spgi *= h; mlgr *= h; shgr *= h; vdvg *= h; clri *= h; clir *= h; clsr *= h; clcg *= h; clrg *= h;
+          clig *= h; clsg *= h; clrsg *= h; cnsg *= h; frrg *= h;
         }
-        cnis = fminr(cn, qip);
       }
-      if (qsp > thresq && vdvs > 0.0f && vdvs < clcs) cnsg = fminr(K.ccnsg * clcs, qsp);
-    }
-    // ---- nuc2nd ----
-    float spsi = 0.0f, spgi = 0.0f;
-    if (t <= 270.16f && t > 265.16f) {
-      const float fr = t > 268.16f ? (270.16f - t) * K.mi0352 * rbv : (t - 265.16f) * K.mi0353 * rbv;
-      spsi = fr * clcs;
-      spgi = pgwet > 0.0f ? 0.0f : fr * clcg;
-    }
-    // ---- freezing ----
-    float frrg = 0.0f;
-    if (qrp > thresq && tcel < t0cel) {
-      if (tcel <= tlow - t0) {
-        frrg = qrp;
+      // ---- newblk ----
+      const float cppiv = 1.0f / (cp * pi);
+      const float clix = clir + clis + clig;
+      const float spxi = spsi + spgi;
+      const float cxcr = clcr + cncr;
+      const float cxsg = clsr + clsg + cnsg;
+      const float nmci = nuci - mlic;
+      const float mssr = mlsr + shsr;
+      const float msgr = mlgr + shgr;
+      const float nuvdvx = vdvi + vdvs + vdvg + nuvi;
+      const float clnmci = clcs + clcg + nmci;
+      const float cfmsxr = clri + clrs + clrg + clrsg + frrg - mssr - msgr;
+      ptpf = ptpf + (vdvr * lv + (clnmci + cfmsxr) * lf + nuvdvx * ls) * cppiv;
+      float qvsink = nuvdvx + vdvr;
+      float qxsink = cfmsxr - cxcr - vdvr;
+      if (qrf < qxsink) {
+        qxsink = cxcr + clnmci + (qxsink - qrf);
+        qrf = 0.0f;
       } else {
-        const float d3 = diaqr * diaqr * diaqr;
-        frrg = fminr(qrp, K.cfrrg * d3 * d3 * (ex(-0.66f * tcel) - 1.0f) * ncrp);
+        qrf = qrf - qxsink;
+        qxsink = cxcr + clnmci;
       }
-    }
-    // ---- shedding ----
-    float shsr = (qsp > thresq && t >= t0) ? clcs + clrs : 0.0f;
-    float shgr = 0.0f;
-    if (qgp > thresq) {
-      if (t >= t0) shgr = clcg + clrg;
-      else if (pgwet > 0.0f) shgr = (clcg + clrg + clig + clsg) - pgwet;
-    }
-    // ---- more0q ----
-    float h;
-    if (qcp > thresq) {
-      sink = nuci + clcr + clcs + clcg + cncr - mlic;
-      if (qcf < sink) {
-        h = qcf / sink;
-        nuci *= h; clcr *= h; clcs *= h; clcg *= h; cncr *= h; mlic *= h;
+      if (qcf < qxsink) {
+        const float dqv = qxsink - qcf;
+        ptpf = ptpf + cppiv * dqv * lv;
+        qvsink = qvsink + dqv;
+        qcf = 0.0f;
+      } else {
+        qcf = qcf - qxsink;
       }
-    }
-    if (qip > thresq) {
-      sink = clir + clis + clig + cnis + mlic - vdvi - nuvi - nuci - spsi - spgi;
-      if (qif < sink) {
-        h = qif / sink;
-        clir *= h; clis *= h; clig *= h; cnis *= h; mlic *= h; vdvi *= h; nuvi *= h; nuci *= h; spsi *= h; spgi *= h;
+      qxsink = clix - spxi - nmci + cnis - vdvi - nuvi;
+      if (qif < qxsink) {
+        const float dqv = qxsink - qif;
+        ptpf = ptpf + cppiv * dqv * ls;
+        qvsink = qvsink + dqv;
+        qif = 0.0f;
+      } else {
+        qif = qif - qxsink;
       }
-    }
-    if (qrp > thresq) {
-      sink = clri + clrs + clrg + clrsg + frrg - vdvr - clcr - cncr - mlsr - mlgr - shsr - shgr;
-      if (qrf < sink) {
-        h = qrf / sink;
-        clri *= h; clrs *= h; clrg *= h; clrsg *= h; frrg *= h; vdvr *= h; clcr *= h; cncr *= h;
-        mlsr *= h; mlgr *= h; shsr *= h; shgr *= h;
+      qvf = fmaxr(qvf - qvsink, 0.0f);
+      qsf = fmaxr(qsf - (cxsg + mssr + spsi - vdvs - clcs - clrs - clis - cnis), 0.0f);
+      qgf = fmaxr(qgf + (cxsg - msgr - spgi + vdvg + clri + clir + clcg + clrg + clig + clrsg + frrg), 0.0f);
+      if (qcp > thresq) ncif = ncif + nuci * nccp / qcp;
+      if (qip > thresq) {
+        if (vdvi < 0.0f)
+          ncif = ncif - (agin + (clix + mlic - vdvi) / mi - (spxi + nuvi) * (1.0f / mi0) + cnis * (1.0f / ms0));
+        else
+          ncif = ncif - (agin + (clix + mlic) / mi - (spxi + nuvi) * (1.0f / mi0) + cnis * (1.0f / ms0));
+      } else {
+        ncif = ncif + (spxi + nuvi) * (1.0f / mi0);
       }
-    }
-    if (qsp > thresq) {
-      sink = clsr + clsg + cnsg + spsi + mlsr + shsr - vdvs - clcs - clrs - clis - cnis;
-      if (qsf < sink) {
-        h = qsf / sink;
-        clsr *= h; clsg *= h; cnsg *= h; spsi *= h; mlsr *= h; shsr *= h; vdvs *= h; clcs *= h; clrs *= h;
-        clis *= h; cnis *= h;
+      if (a.mode == 1) {
+        const bool act = qcp > thresq || qrp > thresq || qip > thresq || qsp > thresq || qgp > thresq || nuvi > 0.0f;
+        const unsigned long long ball = __ballot(act);
+        if (act) {
+          const int lane = (int)(threadIdx.x + threadIdx.y * blockDim.x) & 63;
+          const int lead = __ffsll((long long)ball) - 1;
+          unsigned long long base = 0;
+          if (lane == lead) base = atomicAdd(a.cap.count, (unsigned long long)__popcll(ball));
+          base = __shfl(base, lead);
+          const unsigned long long idx = base + __popcll(ball & ((1ull << lane) - 1ull));
+          if (idx < (unsigned long long)a.cap.cap) {
+            float *fi = a.cap.in + idx * MLP_NIN, *ft = a.cap.tgt + idx * MLP_NOUT;
+            fi[0] = t; fi[1] = pr; fi[2] = rbr;
+            fi[3] = qvp; fi[4] = qcp; fi[5] = qrp; fi[6] = qip; fi[7] = qsp; fi[8] = qgp;
+            const float rdt = 1.0f / dtb;
+            ft[0] = (ptpf - ptpf0) * pi * rdt;
+            ft[1] = (qvf - qvf0) * rdt; ft[2] = (qcf - qcf0) * rdt; ft[3] = (qrf - qrf0) * rdt;
+            ft[4] = (qif - qif0) * rdt; ft[5] = (qsf - qsf0) * rdt; ft[6] = (qgf - qgf0) * rdt;
+          }
+        }
       }
-    }
-    if (qgp > thresq) {
-      sink = spgi + mlgr + shgr - vdvg - clri - clir - clsr - clcg - clrg - clig - clsg - clrsg - cnsg - frrg;
-      if (qgf < sink) {
-        h = qgf / sink;
-        spgi *= h; mlgr *= h; shgr *= h; vdvg *= h; clri *= h; clir *= h; clsr *= h; clcg *= h; clrg *= h;
-        clig *= h; clsg *= h; clrsg *= h; cnsg *= h; frrg *= h;
+    } else if (how == 1) {
+      // the network's tendencies over the step 2 dtb, then the hard
+      // constraints of newblk.f90:416-489: no species below zero, total
+      // water conserved (vapour takes the balance of the hydrometeor
+      // changes, which are scaled down when the vapour cannot), and the
+      // latent heating that the phase changes imply,
+      //   cp pi dtheta = Lv (-dqv) + Lf (dqi + dqs + dqg),
+      // with Lv, Lf at the cell's temperature as setblk forms them.
+      const float t = (MP(CRESS_PTBR, a.ptbr, p, k) + a.ptpp[p]) * pi;
+      const float lv = lv0 * ex((0.167f + 3.67e-4f * t) * lg(t0 / t));
+      const float lf = lf0 + (cw - ci) * (t - t0);
+      const long qn7 = a.qn;
+      float dqc = fmaxr(a.mout[2 * qn7 + p] * dtb, -qcf);
+      float dqr = fmaxr(a.mout[3 * qn7 + p] * dtb, -qrf);
+      float dqi = fmaxr(a.mout[4 * qn7 + p] * dtb, -qif);
+      float dqs = fmaxr(a.mout[5 * qn7 + p] * dtb, -qsf);
+      float dqg = fmaxr(a.mout[6 * qn7 + p] * dtb, -qgf);
+      float dqh = dqc + dqr + dqi + dqs + dqg;
+      if (dqh > qvf) {
+        const float sc = qvf / dqh;
+        dqc *= sc; dqr *= sc; dqi *= sc; dqs *= sc; dqg *= sc;
+        dqh = dqc + dqr + dqi + dqs + dqg;
       }
-    }
-    // ---- newblk ----
-    const float cppiv = 1.0f / (cp * pi);
-    const float clix = clir + clis + clig;
-    const float spxi = spsi + spgi;
-    const float cxcr = clcr + cncr;
-    const float cxsg = clsr + clsg + cnsg;
-    const float nmci = nuci - mlic;
-    const float mssr = mlsr + shsr;
-    const float msgr = mlgr + shgr;
-    const float nuvdvx = vdvi + vdvs + vdvg + nuvi;
-    const float clnmci = clcs + clcg + nmci;
-    const float cfmsxr = clri + clrs + clrg + clrsg + frrg - mssr - msgr;
-    ptpf = ptpf + (vdvr * lv + (clnmci + cfmsxr) * lf + nuvdvx * ls) * cppiv;
-    float qvsink = nuvdvx + vdvr;
-    float qxsink = cfmsxr - cxcr - vdvr;
-    if (qrf < qxsink) {
-      qxsink = cxcr + clnmci + (qxsink - qrf);
-      qrf = 0.0f;
-    } else {
-      qrf = qrf - qxsink;
-      qxsink = cxcr + clnmci;
-    }
-    if (qcf < qxsink) {
-      const float dqv = qxsink - qcf;
-      ptpf = ptpf + cppiv * dqv * lv;
-      qvsink = qvsink + dqv;
-      qcf = 0.0f;
-    } else {
-      qcf = qcf - qxsink;
-    }
-    qxsink = clix - spxi - nmci + cnis - vdvi - nuvi;
-    if (qif < qxsink) {
-      const float dqv = qxsink - qif;
-      ptpf = ptpf + cppiv * dqv * ls;
-      qvsink = qvsink + dqv;
-      qif = 0.0f;
-    } else {
-      qif = qif - qxsink;
-    }
-    qvf = fmaxr(qvf - qvsink, 0.0f);
-    qsf = fmaxr(qsf - (cxsg + mssr + spsi - vdvs - clcs - clrs - clis - cnis), 0.0f);
-    qgf = fmaxr(qgf + (cxsg - msgr - spgi + vdvg + clri + clir + clcg + clrg + clig + clrsg + frrg), 0.0f);
-    if (qcp > thresq) ncif = ncif + nuci * nccp / qcp;
-    if (qip > thresq) {
-      if (vdvi < 0.0f)
-        ncif = ncif - (agin + (clix + mlic - vdvi) / mi - (spxi + nuvi) * (1.0f / mi0) + cnis * (1.0f / ms0));
-      else
-        ncif = ncif - (agin + (clix + mlic) / mi - (spxi + nuvi) * (1.0f / mi0) + cnis * (1.0f / ms0));
-    } else {
-      ncif = ncif + (spxi + nuvi) * (1.0f / mi0);
+      const float dqv = -dqh;
+      ptpf = ptpf + (lv * dqh + lf * (dqi + dqs + dqg)) * (1.0f / (cp * pi));
+      qvf = fmaxr(qvf + dqv, 0.0f);
+      qcf = fmaxr(qcf + dqc, 0.0f);
+      qrf = fmaxr(qrf + dqr, 0.0f);
+      qif = fmaxr(qif + dqi, 0.0f);
+      qsf = fmaxr(qsf + dqs, 0.0f);
+      qgf = fmaxr(qgf + dqg, 0.0f);
     }
     // adjstnci after the sources
     ncif = fminr(fmaxr(ncif, (0.5f / mimax) * qif), (1.e2f / mi0) * qif);
     a.ptpf[p] = ptpf;
     a.qf[p] = qvf;
-    a.qf[p + qn] = qcf;
-    a.qf[p + 2 * qn] = qrf;
-    a.qf[p + 3 * qn] = qif;
-    a.qf[p + 4 * qn] = qsf;
-    a.qf[p + 5 * qn] = qgf;
+    a.qfw[p] = qcf;
+    a.qfw[p + qn] = qrf;
+    a.qfi[p] = qif;
+    a.qfi[p + qn] = qsf;
+    a.qfi[p + 2 * qn] = qgf;
     a.ncif[p] = ncif;
     a.urq[p] = urq;
     a.usq[p] = usq;
@@ -530,6 +590,55 @@ __global__ void __launch_bounds__(256) CRESS_MICRO_ATTR k_micro_src(Tiles T, Gri
   if (tid == 0) atomicMin(a.dtpmin, __float_as_uint(red[0]));
 }
 #pragma clang fp contract(off)
+
+// ---------------------------------------------------------------------------
+// The network's cells for this step (MlpSelectArgs in micro.h).  Same cells
+// as k_micro_src.  The activity test is srcblk's own gating: every process
+// rate is zero unless a hydrometeor exceeds thresq or nuc1stv fires
+// (qv > thresq, qv > qvsi, tlow < T < t0; nuc1stv.f90, setblk.f90:333-339).
+__global__ void __launch_bounds__(256) k_mlp_select(Tiles T, Grid g, MlpSelectArgs a) {
+  const uint3 xt = xcd_tile(T);
+  const int i = xt.x * blockDim.x + threadIdx.x;
+  const int j = 1 + xt.y * blockDim.y + threadIdx.y;
+  const int k = 1 + xt.z;
+  bool net = false;
+  float f[MLP_NIN];
+  long p = 0;
+  if (i >= 1 && i <= g.ni - 1 && j <= g.nj - 1) {
+    using namespace mc;
+    p = g.at(i, j, k);
+    const long qn = a.qn;
+    const float thresq = a.thresq;
+    const float rbr = MP(CRESS_RBR, a.rbr, p, k);
+    const float pr = MP(CRESS_PBR, a.pbr, p, k) + a.ppp[p];
+    const float pi = ex((rd / cp) * lg((1.0f / p0) * pr));
+    const float t = (MP(CRESS_PTBR, a.ptbr, p, k) + a.ptpp[p]) * pi;
+    const float qvp = a.qp[p], qcp = a.qpw[p], qrp = a.qpw[p + qn], qip = a.qpi[p];
+    const float qsp = a.qpi[p + qn], qgp = a.qpi[p + 2 * qn];
+    const float esi = es0 * ex(21.875f * (t - t0) / (t - 7.66f));
+    const float qvsi = epsva * esi / (pr - esi);
+    const bool act = qcp > thresq || qrp > thresq || qip > thresq || qsp > thresq || qgp > thresq ||
+                     (qvp > thresq && qvp > qvsi && t > tlow && t < t0);
+    f[0] = t; f[1] = pr; f[2] = rbr; f[3] = qvp; f[4] = qcp; f[5] = qrp; f[6] = qip; f[7] = qsp; f[8] = qgp;
+    bool inside = true;
+    for (int c = 0; c < MLP_NIN; c++) inside = inside && f[c] >= a.norm[64 + c] && f[c] <= a.norm[80 + c];
+    net = act && inside;
+    a.s.flag[p] = !act ? 2 : (inside ? 1 : 0);
+  }
+  const unsigned long long ball = __ballot(net);
+  if (net) {
+    const int lane = (int)(threadIdx.x + threadIdx.y * blockDim.x) & 63;
+    const int lead = __ffsll((long long)ball) - 1;
+    unsigned base = 0;
+    if (lane == lead) base = atomicAdd(a.s.count, (unsigned)__popcll(ball));
+    base = __shfl(base, lead);
+    const unsigned idx = base + (unsigned)__popcll(ball & ((1ull << lane) - 1ull));
+    a.s.list[idx] = (int)p;
+    unsigned short *x = a.s.x + (long)idx * MLP_INP;
+    for (int c = 0; c < MLP_INP; c++)
+      x[c] = c < MLP_NIN ? f2bf((f[c] - a.norm[c]) * a.norm[16 + c]) : (unsigned short)0;
+  }
+}
 
 // fallblk.f90:439-541 on the columns (i,j), i in 1..ni-1, j = 1+tile y,
 // one wavefront per species (threadIdx.y: qc, qi, nci, qr, qs, qg), each
@@ -565,12 +674,12 @@ __global__ void __launch_bounds__(384) k_micro_fall(Tiles T, Grid g, FallArgs a)
     const float *ufield = nullptr, *qpres = nullptr;
     float ucst = 0.0f, *prec = nullptr;
     switch (sp) {
-      case 0: q = a.qf + qn; qpres = a.qp + qn; ucst = mc::ucqcst; prec = a.prwtr; break;
-      case 1: q = a.qf + 3 * qn; qpres = a.qp + 3 * qn; ucst = mc::uiqcst; prec = a.price; break;
-      case 2: q = a.ncif; qpres = a.qp + 3 * qn; ucst = mc::uincst; break;
-      case 3: q = a.qf + 2 * qn; ufield = a.urq; prec = a.prwtr + 2 * a.n2; break;
-      case 4: q = a.qf + 4 * qn; ufield = a.usq; prec = a.price + 2 * a.n2; break;
-      default: q = a.qf + 5 * qn; ufield = a.ugq; prec = a.price + 4 * a.n2; break;
+      case 0: q = a.qfw; qpres = a.qpw; ucst = mc::ucqcst; prec = a.prwtr; break;
+      case 1: q = a.qfi; qpres = a.qpi; ucst = mc::uiqcst; prec = a.price; break;
+      case 2: q = a.ncif; qpres = a.qpi; ucst = mc::uincst; break;
+      case 3: q = a.qfw + qn; ufield = a.urq; prec = a.prwtr + 2 * a.n2; break;
+      case 4: q = a.qfi + qn; ufield = a.usq; prec = a.price + 2 * a.n2; break;
+      default: q = a.qfi + 2 * qn; ufield = a.ugq; prec = a.price + 4 * a.n2; break;
     }
     const float thresq = a.thresq;
     auto speed = [&](uint32_t o) {
@@ -691,7 +800,7 @@ __global__ void __launch_bounds__(384) k_micro_fall(Tiles T, Grid g, FallArgs a)
   // adjstnci is pointwise: the six wavefronts share the column's levels
   __syncthreads();
   if (in) {
-    const float *qi = a.qf + 3 * qn;
+    const float *qi = a.qfi;
     for (int k = 1 + sp; k <= g.nk - 1; k += (int)blockDim.y) {
       const uint32_t o = q1 + (uint32_t)(k - 1) * sk;
       const float qv = LD(qi, o);

@@ -30,7 +30,7 @@ FID.update({"fc": 64, "rbct": 65, "prwtr": 66, "price": 67})
 
 ERR = {0: "ok", -1: "invalid argument", -2: "no usable GPU", -3: "HIP runtime error",
        -4: "device out of memory", -5: "field id or level out of range",
-       -6: "option outside the implemented plain path"}
+       -6: "option outside the implemented plain path", -7: "grid larger than a fixed-size buffer"}
 
 
 class CressError(RuntimeError):
@@ -94,6 +94,20 @@ def lib():
         L.cress_ens_step.argtypes = [P, I, I]
         L.cress_ens_sync.argtypes = [P]
         L.cress_ens_destroy.argtypes = [P]
+        L.cress_mlp_create.argtypes = [C.c_long, I, C.POINTER(P)]
+        L.cress_mlp_destroy.argtypes = [P]
+        L.cress_mlp_init.argtypes = [P, C.c_ulonglong]
+        L.cress_mlp_set.argtypes = [P, P, P]
+        L.cress_mlp_get.argtypes = [P, P, P]
+        L.cress_mlp_samples.argtypes = [P, C.POINTER(C.c_long)]
+        L.cress_mlp_clear_samples.argtypes = [P]
+        L.cress_mlp_put_samples.argtypes = [P, P, P, C.c_long]
+        L.cress_mlp_get_samples.argtypes = [P, C.c_long, C.c_long, P, P]
+        L.cress_mlp_fit_norm.argtypes = [P, C.c_float]
+        L.cress_mlp_forward.argtypes = [P, P, C.c_long, P]
+        L.cress_mlp_grad.argtypes = [P, P, P, I, P, C.POINTER(D)]
+        L.cress_mlp_train.argtypes = [P, I, I, C.c_float, C.c_uint, P]
+        L.cress_ctx_set_mlp.argtypes = [P, P, I]
         _lib = L
     return _lib
 
@@ -190,6 +204,11 @@ class Solver:
     def set_timing(self, on=True):
         check(self.L.cress_set_timing(self.ctx, 1 if on else 0), "cress_set_timing")
 
+    def set_mlp(self, mlp, mode):
+        """mode 0: srcblk; 1: srcblk, recording training pairs into mlp;
+        2: the network for the active cells inside its envelope."""
+        check(self.L.cress_ctx_set_mlp(self.ctx, mlp.m if mlp is not None else None, mode), "cress_ctx_set_mlp")
+
     def stage_times(self):
         buf = (D * 8)()
         check(self.L.cress_stage_times(self.ctx, buf, 8), "cress_stage_times")
@@ -224,6 +243,90 @@ class Ensemble:
             self.close()
         except Exception:
             pass
+
+
+class Mlp:
+    """The learned microphysics network (cress_mlp_*): parameters, training
+    pairs and the training kernels on the GPU."""
+
+    NIN, NOUT = 9, 7
+
+    def __init__(self, capacity=1 << 20, max_batch=1 << 15):
+        self.L = lib()
+        self.m = C.c_void_p()
+        check(self.L.cress_mlp_create(capacity, max_batch, C.byref(self.m)), "cress_mlp_create")
+        self.nparam = self.L.cress_mlp_nparam()
+        self.nnorm = self.L.cress_mlp_nnorm()
+
+    def close(self):
+        if self.m:
+            self.L.cress_mlp_destroy(self.m)
+            self.m = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self, seed):
+        check(self.L.cress_mlp_init(self.m, seed), "cress_mlp_init")
+
+    def set(self, params=None, norm=None):
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        n = None if norm is None else np.ascontiguousarray(norm, np.float32)
+        check(self.L.cress_mlp_set(self.m, None if p is None else p.ctypes.data,
+                                   None if n is None else n.ctypes.data), "cress_mlp_set")
+
+    def get(self):
+        p = np.empty(self.nparam, np.float32)
+        n = np.empty(self.nnorm, np.float32)
+        check(self.L.cress_mlp_get(self.m, p.ctypes.data, n.ctypes.data), "cress_mlp_get")
+        return p, n
+
+    def samples(self):
+        c = C.c_long()
+        check(self.L.cress_mlp_samples(self.m, C.byref(c)), "cress_mlp_samples")
+        return c.value
+
+    def clear_samples(self):
+        check(self.L.cress_mlp_clear_samples(self.m), "cress_mlp_clear_samples")
+
+    def put_samples(self, inp, tgt):
+        a = np.ascontiguousarray(inp, np.float32)
+        b = np.ascontiguousarray(tgt, np.float32)
+        check(self.L.cress_mlp_put_samples(self.m, a.ctypes.data, b.ctypes.data, a.shape[0]),
+              "cress_mlp_put_samples")
+
+    def get_samples(self, first, n):
+        a = np.empty((n, self.NIN), np.float32)
+        b = np.empty((n, self.NOUT), np.float32)
+        check(self.L.cress_mlp_get_samples(self.m, first, n, a.ctypes.data, b.ctypes.data),
+              "cress_mlp_get_samples")
+        return a, b
+
+    def fit_norm(self, margin=0.05):
+        check(self.L.cress_mlp_fit_norm(self.m, margin), "cress_mlp_fit_norm")
+
+    def forward(self, inp):
+        a = np.ascontiguousarray(inp, np.float32)
+        out = np.empty((a.shape[0], self.NOUT), np.float32)
+        check(self.L.cress_mlp_forward(self.m, a.ctypes.data, a.shape[0], out.ctypes.data), "cress_mlp_forward")
+        return out
+
+    def grad(self, inp, tgt):
+        a = np.ascontiguousarray(inp, np.float32)
+        b = np.ascontiguousarray(tgt, np.float32)
+        g = np.empty(self.nparam, np.float32)
+        loss = D()
+        check(self.L.cress_mlp_grad(self.m, a.ctypes.data, b.ctypes.data, a.shape[0], g.ctypes.data,
+                                    C.byref(loss)), "cress_mlp_grad")
+        return g, loss.value
+
+    def train(self, steps, batch, lr, seed=1):
+        losses = np.empty(steps, np.float64)
+        check(self.L.cress_mlp_train(self.m, steps, batch, lr, seed, losses.ctypes.data), "cress_mlp_train")
+        return losses
 
 
 def device_count():

@@ -39,7 +39,8 @@ enum {
   CRESS_EHIP = -3,        /* HIP runtime error */
   CRESS_ENOMEM = -4,      /* device allocation failed */
   CRESS_ERANGE = -5,      /* field id / level out of range */
-  CRESS_EUNSUPPORTED = -6 /* option outside the implemented plain path */
+  CRESS_EUNSUPPORTED = -6, /* option outside the implemented plain path */
+  CRESS_ETOOBIG = -7       /* grid larger than a fixed-size buffer allows */
 };
 
 /* Resolved run options.  Names and meanings follow the reference namelist
@@ -211,11 +212,52 @@ int cress_steps(const cress_config *cfg, float dtb, const float *rst, const floa
  * termblk, srcblk, fallblk on the future level (ptpf, qvf, qwtrf(..,1:2),
  * qicef(..,1:3), nicef) from the past level, over 2 dtb; prwtr
  * (0:ni+1,0:nj+1,1:2,1:2) and price (..,1:2,1:3) get the surface rate and
- * accumulation.  Within 1e-5 of the FP32 reference (binary64 exp/log). */
+ * accumulation.  Within 1e-5 of the FP32 reference (binary32 exp/log). */
 int cress_cloudphy(const cress_config *cfg, float dtb, const float *jcb, const float *pbr, const float *ptbr,
                    const float *rbr, const float *rst, const float *ppp, const float *ptpp, const float *qvp,
                    const float *qwtrp, const float *qicep, const float *nicep, float *ptpf, float *qvf,
                    float *qwtrf, float *qicef, float *nicef, float *prwtr, float *price);
+
+/* ---- Learned microphysics (no counterpart in the reference) ----
+ * A 9 -> 128 -> 128 -> 128 -> 7 GELU network standing in for the local
+ * source terms of s_srcblk (srcblk.f90:488-663, called from
+ * coldblk.f90:354-358; sedimentation and saturation adjustment stay on the
+ * plain path).  Features (raw units): T, p, rho, qv, qc, qr, qi, qs, qg of
+ * the past level; targets: the change srcblk makes to the future level per
+ * second of the microphysics step, as dT/dt (= pi dtheta'/dt) and dqv/dt ..
+ * dqg/dt.  BF16 weights and activations into the matrix cores, FP32
+ * accumulation and master weights, Adam.  Parameters are the FP32 block
+ * W1[16][128] b1 W2[128][128] b2 W3[128][128] b3 W4[128][16] b4[16]
+ * (cress_mlp_nparam() floats; rows/columns beyond 9 inputs / 7 outputs are
+ * zero); the normalisation block (cress_mlp_nnorm() floats) is in_mu[16]
+ * in_rsd[16] out_mu[16] out_sd[16] env_lo[16] env_hi[16]. */
+typedef struct cress_mlp cress_mlp;
+int cress_mlp_nparam(void);
+int cress_mlp_nnorm(void);
+/* capacity: training pairs held on the device; max_batch: largest batch. */
+int cress_mlp_create(long capacity, int max_batch, cress_mlp **out);
+int cress_mlp_destroy(cress_mlp *m);
+int cress_mlp_init(cress_mlp *m, unsigned long long seed);       /* Glorot-uniform weights */
+int cress_mlp_set(cress_mlp *m, const float *params, const float *norm);   /* either may be NULL */
+int cress_mlp_get(cress_mlp *m, float *params, float *norm);
+int cress_mlp_samples(cress_mlp *m, long *count);
+int cress_mlp_clear_samples(cress_mlp *m);
+int cress_mlp_put_samples(cress_mlp *m, const float *in, const float *tgt, long n);   /* rows of 9 / 7 */
+int cress_mlp_get_samples(cress_mlp *m, long first, long n, float *in, float *tgt);
+/* normalisation and the envelope [min - margin*range, max + margin*range]
+ * from the held pairs */
+int cress_mlp_fit_norm(cress_mlp *m, float margin);
+/* the step's fused inference kernel on n host rows of raw features */
+int cress_mlp_forward(cress_mlp *m, const float *in, long n, float *out);
+/* the training kernels' gradient of the mean squared error (normalised
+ * targets) on n host rows, no update */
+int cress_mlp_grad(cress_mlp *m, const float *in, const float *tgt, int n, float *grad, double *loss);
+/* steps Adam steps on random batches of the held pairs; losses[steps] may be NULL */
+int cress_mlp_train(cress_mlp *m, int steps, int batch, float lr, unsigned seed, double *losses);
+/* mode 0: srcblk; 1: srcblk, recording (features, targets) of every active
+ * cell into m; 2: the network for active cells inside its envelope, srcblk
+ * for the other active cells, nothing where no process is active. */
+int cress_ctx_set_mlp(cress_ctx *x, cress_mlp *m, int mode);
 
 /* The first four calls of s_turbdrv (SrcOrig/turbdrv.f90:245-276) with
  * tubopt=2 (1.5-order TKE closure), isoopt=1, trnopt=0, mfcopt=0, sfcopt=0:
