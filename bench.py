@@ -113,7 +113,103 @@ def parse():
     ap.add_argument("--ensemble", type=int, default=12,
                     help="members of the batched-ensemble measurement (0: none)")
     ap.add_argument("--ens-steps", type=int, default=6)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shapes instead of autotuning them on the workload")
+    ap.add_argument("--mlp", type=int, default=1, help="measure the learned microphysics (0: no)")
+    ap.add_argument("--mlp-nxy", type=int, default=256)
+    ap.add_argument("--mlp-steps", type=int, default=60,
+                    help="large steps of the plain and the network run, from the cold start")
+    ap.add_argument("--mlp-train", type=int, default=3000, help="Adam steps")
     return ap.parse_args()
+
+
+MLP_FLOP = 2 * (9 * 128 + 128 * 128 + 128 * 128 + 128 * 7)   # forward flop per cell
+PEAK_BF16 = 2.5e15                                           # dense BF16 matrix-core peak
+
+
+def mlp_bench(args):
+    """The learned microphysics (a 9-128-128-128-7 GELU network in place of
+    srcblk's local sources) on the supercell (open sides, cold rain, dtbig
+    3 s, 6 substeps) at mlp_nxy^2 x 64: training pairs recorded from the
+    plain scheme's own steps, the network trained on the GPU (BF16 matrix
+    cores, FP32 accumulation, Adam), then the same mlp_steps large steps
+    from the cold start with srcblk and with the network, timed, and scored
+    on the accumulated surface precipitation and the mixing ratios."""
+    from cress_amd import cases, host, learned
+    n = args.mlp_nxy
+    case = cases.build("supercell", n, n, 64)
+    cfg = case.cfg
+    cells = (cfg.ni - 3) * (cfg.nj - 3) * (cfg.nk - 3)
+    k = args.mlp_steps
+    m = host.Mlp(capacity=1 << 25, max_batch=16384)
+    t0 = time.perf_counter()
+    npairs = learned.record(m, case, k)
+    t_rec = time.perf_counter() - t0
+    m.fit_norm(0.05)
+    m.init(1)
+    t0 = time.perf_counter()
+    losses = learned.train(m, args.mlp_train, batch=16384, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    print("bench: mlp %d pairs from %d plain steps (%.1f s), %d Adam steps in %.1f s, loss %.3g -> %.3g"
+          % (npairs, k, t_rec, args.mlp_train, t_train, losses[:20].mean(), losses[-20:].mean()),
+          file=sys.stderr, flush=True)
+    L = host.lib()
+    nk = L.cress_kernel_count()
+    L.cress_kernel_name.restype = __import__("ctypes").c_char_p
+    names = [L.cress_kernel_name(i).decode() for i in range(nk)]
+
+    def run(mode):
+        import ctypes as C
+        s = host.Solver(cfg)
+        case.load(s)
+        if mode:
+            s.set_mlp(m, 2)
+        s.step(1, first=True)
+        s.sync()
+        t = time.perf_counter()
+        s.step(k - 1)
+        s.sync()
+        el = time.perf_counter() - t
+        # one more step with every kernel family timed
+        L.cress_reset_timing(s.ctx)
+        s.set_timing(True)
+        s.step(1)
+        s.sync()
+        s.set_timing(False)
+        tot = (C.c_double * nk)()
+        cnt = (C.c_long * nk)()
+        L.cress_kernel_times(s.ctx, tot, cnt, nk)
+        kt = {names[i]: tot[i] / max(cnt[i], 1) for i in range(nk) if cnt[i]}
+        net_cells = s.mlp_cells() if mode else 0
+        return s, el / (k - 1), kt, net_cells
+
+    s_plain, t_plain, kt_plain, _ = run(0)
+    s_net, t_net, kt_net, ncells = run(1)
+    sc = learned.score(s_net, s_plain)
+    t_inf = kt_net.get("k_mlp_infer", float("nan")) * 1e-3
+    achieved = ncells * MLP_FLOP / t_inf if t_inf > 0 else float("nan")
+    for s in (s_plain, s_net):
+        s.close()
+    m.close()
+    micro = lambda kt: sum(kt.get(x, 0.0) for x in ("k_micro_src", "k_mlp_select", "k_mlp_infer"))
+    out = {"workload": "supercell %dx%dx64 (open sides, cold rain, nsstp %d)" % (n, n, cfg.nsstp),
+           "steps": k, "pairs": npairs, "train_steps": args.mlp_train, "train_batch": 16384,
+           "train_s": t_train, "loss_first": float(losses[:20].mean()), "loss_last": float(losses[-20:].mean()),
+           "plain_ms_per_step": 1e3 * t_plain, "net_ms_per_step": 1e3 * t_net, "speedup": t_plain / t_net,
+           "micro_sources_ms": {"plain": micro(kt_plain), "net": micro(kt_net)},
+           "kernels_ms": {"plain": {k: kt_plain.get(k, 0.0) for k in ("k_micro_src", "k_micro_fall")},
+                          "net": {k: kt_net.get(k, 0.0) for k in ("k_mlp_select", "k_mlp_infer", "k_micro_src",
+                                                                  "k_micro_fall")}},
+           "net_cells_per_step": ncells, "net_cell_frac": ncells / cells,
+           "precip_plain": sc["precip_plain"], "precip_net": sc["precip"],
+           "precip_rel_err": sc["precip_rel_err"],
+           "q_rmse_rel": {q: sc["rmse_" + q] / max(sc["scale_" + q], 1e-30) for q in learned.QS},
+           "mfma": {"kernel": "k_mlp_infer", "achieved": achieved / 1e12, "peak": PEAK_BF16 / 1e12,
+                    "unit": "TFLOP/s", "frac": achieved / PEAK_BF16}}
+    print("bench: mlp plain %.3f ms/step, network %.3f ms/step (%.2fx), precip err %.3g, infer %.1f TFLOP/s"
+          % (1e3 * t_plain, 1e3 * t_net, t_plain / t_net, sc["precip_rel_err"], achieved / 1e12),
+          file=sys.stderr, flush=True)
+    return out
 
 
 def ensemble_bench(args):
@@ -135,6 +231,8 @@ def ensemble_bench(args):
         s = host.Solver(c.cfg)
         c.load(s)
         del c
+        if not args.no_tune:
+            s.autotune()          # cached per grid size: members of one size are tuned once
         solvers.append(s)
         cells += n * n * nz
         print("bench: ensemble member %d %dx%dx%d" % (m, n, n, nz), file=sys.stderr, flush=True)
@@ -207,6 +305,9 @@ def main():
     case.load(s)
     del case
     s.sync()
+    shapes = s.get_launch() if args.no_tune else s.autotune()
+    print("bench: launch shapes %s (%s)" % (shapes, "default" if args.no_tune else "autotuned"),
+          file=sys.stderr, flush=True)
     s.step(1, first=True)
     import ctypes as C
     L = host.lib()
@@ -258,6 +359,7 @@ def main():
                         sorted(kern.items(), key=lambda kv: -kv[1][0]))), file=sys.stderr, flush=True)
     s.close()
     ensemble = ensemble_bench(args) if args.ensemble > 0 else None
+    mlp = mlp_bench(args) if args.mlp else None
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args, nz)
@@ -277,13 +379,15 @@ def main():
         "config": {"workload": "%s %dx%dx%d" % (args.workload, nx, ny, nz), "nsstp": cfg.nsstp,
                    "dtbig": cfg.dtbig, "cphopt": cfg.cphopt, "advopt": cfg.advopt,
                    "lateral": ("open (option 7, gwave %g m/s)" % cfg.gwave) if cfg.wbc == 7 else "periodic",
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU",
+                   "launch_shapes": dict(shapes, how="default" if args.no_tune else "autotuned")},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
                      "step_frac": step_b * cells / (ms_per_step * 1e-3) / PEAK_HBM,
                      "step_bytes_per_cell": step_b},
         "stages_ms": {k: v / nwarm for k, v in stages.items()},
         "ensemble": ensemble,
+        "mlp": mlp,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -10,12 +10,16 @@
 #include <vector>
 
 #include "common.h"
+#include "diag.h"
 #include "dyn.h"
 #include "micro.h"
 #include "obc.h"
 #include "turb.h"
 
 #include <cstring>
+#include <initializer_list>
+#include <map>
+#include <mutex>
 #include <utility>
 
 using namespace cress;
@@ -39,6 +43,31 @@ enum {
   KT_AC_COL, KT_AC_RING, KT_FILL, KT_VBC, KT_STEPS, KT_TIMEFLT, KT_COPY, KT_MICRO_SEED, KT_MICRO_SRC,
   KT_MICRO_FALL, KT_SATADJ, KT_OBC, KT_MLP_SELECT, KT_MLP_INFER, KT_N
 };
+constexpr int kDiagBlocks = 1024;   // blocks of the FP64 diagnostics' first pass
+
+// Default launch shapes: the fastest measured at 1024x1024x80 (dyn.h).
+static LaunchTune tune_default() {
+  LaunchTune t;
+  t.ptuv_by = 16;
+  t.ptuv_slots = 0;
+  t.ptuv_order = 1;
+  t.col_by = 2;
+  t.col_lds = 4096;
+  t.fs_by = 4;
+  t.fs_order = 2;
+  t.ff_by = 4;
+  t.ff_order = 2;
+  return t;
+}
+
+// The fused ptil/stepuv kernel for a grid (per-level tables or not) and a
+// halo layout.
+static auto ptuv_kernel(const Grid &g, int slots) {
+  const unsigned upu = (1u << CRESS_RST8U) | (1u << CRESS_RST8V) | (1u << CRESS_RST8W) | (1u << CRESS_JCB);
+  const bool u = (g.umask & upu) == upu;
+  return slots ? (u ? k_ac_ptuv<true, 1> : k_ac_ptuv<false, 1>) : (u ? k_ac_ptuv<true, 0> : k_ac_ptuv<false, 0>);
+}
+
 static const char *kKernelNames[KT_N] = {
     "k_prep", "k_force_uv", "k_force_w", "k_force_p", "k_force_s", "k_phy2cnt", "k_ac_ptuv",
     "k_ac_col", "k_ac_ring", "k_bcycle", "k_vbc", "k_steps", "k_timeflt", "memcpy_d2d",
@@ -56,6 +85,17 @@ struct cress_ctx {
   float *base[CRESS_NBASE];
   float *fc, *rbct;
   float *tab;  // per-level copies of the horizontally uniform base fields (g.tab)
+  std::vector<float> tabh;  // host copy of tab
+  // Thomas coefficients of the column kernel (AcCol::coef) for the dts of
+  // slot s (accoef_dts[s]): slots 0 and 1 hold the cold-start and the full
+  // substep, slot 2 any other dts a caller asks for; accoef_ok[s] = 0 when
+  // the metrics are not uniform or the slot is stale
+  float *accoef;
+  float accoef_dts[3];
+  int accoef_ok[3];
+  double *diag;  // cress_diag: kDiagBlocks x DIAG_N partials, DIAG_N results, nk+1 heights
+  LaunchTune tune;  // launch shapes (tune_default, cress_autotune)
+  int tune_ver;     // bumped whenever tune changes (captured ensemble graphs bake it in)
   // prognostics by physical slot; logical level l lives in slot (rot+l)%3
   float *u[3], *v[3], *w[3], *pp[3], *ptp[3], *q[3], *nci[3];
   int rot;
@@ -204,6 +244,7 @@ int cress_ctx_create(const cress_config *cfg, cress_ctx **out) {
   o.vsp_q = vs && cfg->vspvar[6] == 'o';
   o.ksp0 = cfg->ksp0[1];
   o.smtopt = cfg->smtopt;
+  x->tune = tune_default();
   if (hipStreamCreate(&x->stream) != hipSuccess) {
     delete x;
     return CRESS_EHIP;
@@ -213,6 +254,14 @@ int cress_ctx_create(const cress_config *cfg, cress_ctx **out) {
 #define AL(p, cnt) if ((rc = dalloc(x, &(p), (size_t)(cnt))) != 0) goto fail
   for (int f = 0; f < CRESS_NBASE; f++) AL(x->base[f], n);
   AL(x->tab, (long)CRESS_NBASE * cfg->nk);
+  x->tabh.assign((size_t)CRESS_NBASE * cfg->nk, 0.0f);
+  AL(x->accoef, 3L * 3 * (cfg->nk + 1));
+  {
+    float *dp = nullptr;
+    AL(dp, 2L * ((size_t)kDiagBlocks * DIAG_N + DIAG_N + cfg->nk + 1));
+    x->diag = (double *)dp;
+  }
+  for (int s = 0; s < 3; s++) x->accoef_ok[s] = 0;
   x->g.tab = x->tab;
   x->g.umask = 0u;
   AL(x->fc, 2 * n2);
@@ -320,6 +369,100 @@ int cress_field_size(const cress_ctx *x, int f, size_t *nfloat) {
   return CRESS_OK;
 }
 
+// The column kernel's Thomas coefficients for one dts when rst8w, rcsq, rst,
+// rbr and jcb (and, for the kernel's other reads, jcb8u, jcb8v, jcb8w) are
+// horizontally uniform (AcCol::coef), formed on the host with
+// the kernel's own binary32 operations in its order (stepwi.f90:391-416,
+// gaussel.f90:189-236); the build keeps FMA contraction off on the host too,
+// so these are the bits each column would compute.
+static bool ac_uniform(const cress_ctx *x) {
+  const unsigned need = (1u << CRESS_RST8W) | (1u << CRESS_RCSQ) | (1u << CRESS_RST) | (1u << CRESS_RBR) |
+                        (1u << CRESS_JCB) | (1u << CRESS_JCB8U) | (1u << CRESS_JCB8V) | (1u << CRESS_JCB8W);
+  return (x->g.umask & need) == need;
+}
+
+static void ac_coef_host(const cress_ctx *x, float dts, float *coef) {
+  const Consts &c = x->c;
+  const int nk = x->g.nk;
+  auto tab = [&](int id, int k) { return x->tabh[(size_t)id * nk + k - 1]; };
+  const float sbsqzi = dts * dts * c.weicoe * c.weicoe * c.dziv;
+  const float sbsqg5 = c.g05 * dts * dts * c.weicoe * c.weicoe;
+  float fwc_km1 = 0.0f, wcc_km1 = 0.0f, rcsq_km1 = 0.0f, rst_km1 = 0.0f, ssn_km1 = 0.0f;
+  for (int k = 0; k < 3 * (nk + 1); k++) coef[k] = 0.0f;
+  for (int k = 2; k <= nk - 2; k++) {
+    const float rcsq = tab(CRESS_RCSQ, k), rst = tab(CRESS_RST, k);
+    const float jcb = tab(CRESS_JCB, k), rbr = tab(CRESS_RBR, k);
+    const float t1 = c.g05 * rbr / rcsq;
+    const float t2 = c.dziv / jcb;
+    const float fwc = t1 + t2, wcc = t1 - t2;
+    if (k >= 3) {
+      const float rst8w = tab(CRESS_RST8W, k);
+      float rr, ss, tt;
+      if (x->o.buyopt == 0) {
+        const float aa = sbsqzi / rst8w;
+        const float mm = aa * rcsq_km1;
+        const float nn = aa * rcsq;
+        rr = -mm * fwc_km1;
+        ss = 1.0f + (nn * fwc - mm * wcc_km1);
+        tt = nn * wcc;
+      } else {
+        const float rstiv = 1.0f / rst8w;
+        const float aa = rstiv * sbsqzi;
+        const float bb = rstiv * sbsqg5;
+        const float mm = bb * rst_km1 - aa * rcsq_km1;
+        const float nn = bb * rst + aa * rcsq;
+        rr = mm * fwc_km1;
+        ss = 1.0f + (mm * wcc_km1 + nn * fwc);
+        tt = nn * wcc;
+      }
+      const float r = k == 3 ? 1.0f / ss : 1.0f / (ss - rr * ssn_km1);
+      const float ssn = tt * r;
+      coef[3 * k] = rr;
+      coef[3 * k + 1] = r;
+      coef[3 * k + 2] = ssn;
+      ssn_km1 = ssn;
+    }
+    fwc_km1 = fwc;
+    wcc_km1 = wcc;
+    rcsq_km1 = rcsq;
+    rst_km1 = rst;
+  }
+}
+
+static int upload_accoef(cress_ctx *x, int slot, float dts) {
+  const int nk = x->g.nk;
+  std::vector<float> h(3 * (size_t)(nk + 1));
+  ac_coef_host(x, dts, h.data());
+  HIPCHK(hipMemcpyAsync(x->accoef + (size_t)slot * 3 * (nk + 1), h.data(), h.size() * sizeof(float),
+                        hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  x->accoef_dts[slot] = dts;
+  x->accoef_ok[slot] = 1;
+  return CRESS_OK;
+}
+
+// After any change of the base fields: the tables of the large step's two
+// substep lengths (the cold start's dtsml/2 and dtsml), or none.
+static int refresh_accoef(cress_ctx *x) {
+  for (int s = 0; s < 3; s++) x->accoef_ok[s] = 0;
+  if (!ac_uniform(x)) return CRESS_OK;
+  int rc = upload_accoef(x, 0, 0.5f * (float)x->cfg.dtsml);
+  if (!rc) rc = upload_accoef(x, 1, (float)x->cfg.dtsml);
+  return rc;
+}
+
+// The table for dts, or null (non-uniform metrics: the kernel forms the
+// coefficients per column).  A dts other than the large step's two (the
+// per-substep entry point) is formed and uploaded on demand, synchronously.
+static const float *accoef_for(cress_ctx *x, float dts) {
+  if (!ac_uniform(x)) return nullptr;
+  const size_t stride = 3 * (size_t)(x->g.nk + 1);
+  for (int s = 0; s < 3; s++)
+    if (x->accoef_ok[s] && x->accoef_dts[s] == dts) return x->accoef + s * stride;
+  if (upload_accoef(x, 2, dts) != CRESS_OK) return nullptr;
+  return x->accoef + 2 * stride;
+}
+
 // A base field whose columns all hold the same bits, level by level: its
 // per-level values go to the table and the kernels read them from there.
 static int note_uniform(cress_ctx *x, int f, const float *host) {
@@ -340,11 +483,12 @@ static int note_uniform(cress_ctx *x, int f, const float *host) {
     HIPCHK(hipMemcpyAsync(x->tab + (long)f * nk, col.data(), nk * sizeof(float), hipMemcpyHostToDevice,
                           x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
+    for (int k = 0; k < nk; k++) x->tabh[(size_t)f * nk + k] = col[k];
     x->g.umask |= 1u << f;
   } else {
     x->g.umask &= ~(1u << f);
   }
-  return CRESS_OK;
+  return refresh_accoef(x);
 }
 
 // Host rows of ni+2 floats <-> device rows of g.nx floats.
@@ -397,7 +541,11 @@ int cress_field_device_ptr(cress_ctx *x, int f, int level, void **dptr) {
   float *d = field_ptr(x, f, level, &c);
   if (!d) return CRESS_ERANGE;
   // the caller may write the field: read it as a 3-D field from now on
-  if (f >= 0 && f < CRESS_NBASE) x->g.umask &= ~(1u << f);
+  if (f >= 0 && f < CRESS_NBASE) {
+    x->g.umask &= ~(1u << f);
+    int rc = refresh_accoef(x);
+    if (rc) return rc;
+  }
   *dptr = d;
   return CRESS_OK;
 }
@@ -569,25 +717,17 @@ static void run_corners(cress_ctx *x, const CornerSet &cs) {
 // kernel, the w/p' fills and the ring columns.  oac: the open sides' phase
 // speeds (null on periodic sides).  uf and vf leave in the other buffer of
 // their pair.
-static void ac_substep(cress_ctx *x, float dts, const ObcAc *oacp) {
-  const Grid g = x->g;
-  const Opts o = x->o;
-  const Consts c = x->c;
-  const cress_config &cf = x->cfg;
-  const int ni = g.ni, nj = g.nj, nk = g.nk;
+// Arguments of the fused ptil/stepuv kernel and of the column kernel for
+// one substep of length dts on the context's current future level (uf, vf
+// read from u[F], v[F]; the ptil/stepuv kernel writes the other buffer of
+// each pair).
+static AcPtUv ptuv_args(cress_ctx *x, float dts) {
   const int F = (x->rot + 2) % 3;
   float **B = x->base;
-  const int open = oacp != nullptr;
-  const ObcAc oac = open ? *oacp : ObcAc{0.0f, 0.0f, 0.0f, 0.0f};
-  const float dz = (float)cf.dz;
-  const float divcv = 0.05f / dts * dz * dz;
-  FillSet fwp;
-  fwp.n = 2;
-  fill_spec(&fwp.f[0], x->w[F], 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk);
-  fill_spec(&fwp.f[1], x->pp[F], 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk);
+  const float dz = (float)x->cfg.dz;
   AcPtUv pu;
   pu.dts = dts;
-  pu.divcv = divcv;
+  pu.divcv = 0.05f / dts * dz * dz;
   pu.jcb = B[CRESS_JCB];
   pu.rst8u = B[CRESS_RST8U];
   pu.rst8v = B[CRESS_RST8V];
@@ -597,6 +737,16 @@ static void ac_substep(cress_ctx *x, float dts, const ObcAc *oacp) {
   pu.ufrc = x->ufrc;
   pu.vfrc = x->vfrc;
   pu.ptil = x->ptil;
+  pu.uf = x->u[F];
+  pu.vf = x->v[F];
+  pu.ufn = x->ualt;
+  pu.vfn = x->valt;
+  return pu;
+}
+
+static AcCol col_args(cress_ctx *x, float dts) {
+  const int F = (x->rot + 2) % 3;
+  float **B = x->base;
   AcCol ac;
   ac.dts = dts;
   ac.ptil = x->ptil;
@@ -613,15 +763,41 @@ static void ac_substep(cress_ctx *x, float dts, const ObcAc *oacp) {
   ac.wf = x->w[F];
   ac.wc = x->wc;
   ac.ppf = x->pp[F];
+  ac.uf = x->u[F];
+  ac.vf = x->v[F];
+  ac.coef = accoef_for(x, dts);
+  return ac;
+}
+
+static void launch_ptuv(cress_ctx *x, const LaunchTune &tn, const AcPtUv &pu) {
+  const Grid g = x->g;
+  const Tiles tpu = tiles(g.ni, g.nj - 1, 64, tn.ptuv_by, g.nk - 1, tn.ptuv_order);
+  const auto kpu = ptuv_kernel(g, tn.ptuv_slots);
+  LAUNCH(x, KT_AC_PTUV, kpu, flat(tpu), dim3(64, tn.ptuv_by), 0, tpu, g, x->o, x->c, pu);
+}
+
+static void launch_col(cress_ctx *x, const LaunchTune &tn, const AcCol &ac) {
+  const Grid g = x->g;
+  const Tiles tcol = tiles(g.ni - 1, g.nj - 3, 64, tn.col_by, 1, 1);
+  const auto kcol = ac.coef ? k_ac_col<true> : k_ac_col<false>;
+  LAUNCH(x, KT_AC_COL, kcol, flat(tcol), dim3(64, tn.col_by), tn.col_lds, tcol, g, x->o, x->c, ac);
+}
+
+static void ac_substep(cress_ctx *x, float dts, const ObcAc *oacp) {
+  const Grid g = x->g;
+  const Opts o = x->o;
+  const int ni = g.ni, nj = g.nj, nk = g.nk;
+  const int F = (x->rot + 2) % 3;
+  float **B = x->base;
+  const int open = oacp != nullptr;
+  const ObcAc oac = open ? *oacp : ObcAc{0.0f, 0.0f, 0.0f, 0.0f};
+  FillSet fwp;
+  fwp.n = 2;
+  fill_spec(&fwp.f[0], x->w[F], 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk);
+  fill_spec(&fwp.f[1], x->pp[F], 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk);
   const int nring = 2 * (ni - 1) + 2 * (nj - 3);
-  const Tiles tpu = tiles(ni, nj - 1, 64, CRESS_PTUV_BY, nk - 1, CRESS_PTUV_ORDER);
-  const Tiles tcol = tiles(ni - 1, nj - 3, 64, CRESS_COL_BY, 1, 1);
   {
-    pu.uf = x->u[F];
-    pu.vf = x->v[F];
-    pu.ufn = x->ualt;
-    pu.vfn = x->valt;
-    LAUNCH(x, KT_AC_PTUV, k_ac_ptuv, flat(tpu), dim3(64, CRESS_PTUV_BY), 0, tpu, g, o, c, pu);
+    launch_ptuv(x, x->tune, ptuv_args(x, dts));
     std::swap(x->u[F], x->ualt);
     std::swap(x->v[F], x->valt);
     if (open) {
@@ -658,9 +834,7 @@ static void ac_substep(cress_ctx *x, float dts, const ObcAc *oacp) {
     vuv.ptr[1] = x->v[F];
     run_fill(x, fuv);
     run_vbc(x, vuv);
-    ac.uf = x->u[F];
-    ac.vf = x->v[F];
-    LAUNCH(x, KT_AC_COL, k_ac_col, flat(tcol), dim3(64, CRESS_COL_BY), CRESS_COL_LDS, tcol, g, o, c, ac);
+    launch_col(x, x->tune, col_args(x, dts));
     if (!open) run_fill(x, fwp);
     LAUNCH(x, KT_AC_RING, k_ac_ring, dim3((nring + 63) / 64, nk), dim3(64), 0, g, o, B[CRESS_JCB8W], x->w[F],
            x->wc, x->pp[F]);
@@ -740,7 +914,7 @@ static void micro_stage(cress_ctx *x, float dtb, const ObcLf &olf, int open, int
       const Tiles tsel = tiles(ni, nj - 1, 64, 4, nk - 1);
       LAUNCH(x, KT_MLP_SELECT, k_mlp_select, flat(tsel), b2, 0, tsel, g, sa);
       const MlpNet net{m->w, m->p, m->norm};
-      LAUNCH(x, KT_MLP_INFER, k_mlp_infer, dim3(1024), dim3(256), 0, net, x->ml, n);
+      LAUNCH(x, KT_MLP_INFER, k_mlp_infer, dim3(256), dim3(MLP_INFER_THREADS), 0, net, x->ml, n);
       ma.mflag = x->ml.flag;
       ma.mout = x->ml.out;
     }
@@ -772,27 +946,26 @@ static void micro_stage(cress_ctx *x, float dtb, const ObcLf &olf, int open, int
   }
 }
 
-static int large_step(cress_ctx *x, int first) {
+// The forcing kernels (forcedrv) on the present level with launch shapes
+// tn; parts selects kernels (all of them in the large step, one family at a
+// time in cress_autotune).
+enum { FORCE_PREP = 1, FORCE_UV = 2, FORCE_W = 4, FORCE_P = 8, FORCE_S = 16, FORCE_ALL = 31 };
+static void forcing_stage(cress_ctx *x, const LaunchTune &tn, unsigned parts) {
   const Grid g = x->g;
   const Opts o = x->o;
   const Consts c = x->c;
   const cress_config &cf = x->cfg;
-  hipStream_t st = x->stream;
   const int ni = g.ni, nj = g.nj, nk = g.nk;
   const long n = x->n;
-  const int P = (x->rot + 0) % 3, C = (x->rot + 1) % 3, F = (x->rot + 2) % 3;
-  const float dtb = first ? 0.5f * (float)cf.dtbig : (float)cf.dtbig;
-  const float dts = first ? 0.5f * (float)cf.dtsml : (float)cf.dtsml;
+  const int P = (x->rot + 0) % 3, C = (x->rot + 1) % 3;
   float **B = x->base;
   const dim3 b2(64, 4);
-  const int open = cf.wbc == 7;
-  ObcAc oac;
-  ObcLf olf;
-  obc_speeds(x, first, &oac, &olf);
-
-  stamp(x, 0);
   // ---- forcing (forcedrv) on the present level ----
-  LAUNCH(x, KT_PREP, k_prep, flat(tiles(ni + 2, nj + 2, 64, 4, nk)), b2, 0, tiles(ni + 2, nj + 2, 64, 4, nk), g, o, x->u[C], x->v[C],
+  // every base/metric field a per-level table: the kernels' table builds
+  const bool U = (g.umask & ((1u << CRESS_NBASE) - 1u)) == ((1u << CRESS_NBASE) - 1u);
+  const auto kprep = U ? k_prep<true> : k_prep<false>;
+  if (parts & FORCE_PREP)
+    LAUNCH(x, KT_PREP, kprep, flat(tiles(ni + 2, nj + 2, 64, 4, nk)), b2, 0, tiles(ni + 2, nj + 2, 64, 4, nk), g, o, x->u[C], x->v[C],
                      x->w[C], B[CRESS_RST8U], B[CRESS_RST8V], B[CRESS_RST8W], B[CRESS_JCB8W], x->wc,
                      x->rstxu, x->rstxv, x->rstxwc);
   ForceIn fi;
@@ -825,22 +998,21 @@ static int large_step(cress_ctx *x, int first) {
   // (smtopt 2/3) and without, so that the plain path carries none of its
   // registers
   const bool sm = o.smtopt != 0;
-  const dim3 bf(64, CRESS_FF_BY), bs(64, CRESS_FS_BY);
-  const Tiles tuv = tiles(ni, nj - 2, 64, CRESS_FF_BY, nk - 3, CRESS_FF_ORDER);
-  const Tiles tw = tiles(ni - 1, nj - 3, 64, CRESS_FF_BY, nk - 2, CRESS_FF_ORDER);
-  const Tiles tp = tiles(ni - 1, nj - 3, 64, CRESS_FF_BY, nk - 3, CRESS_FF_ORDER);
-  const Tiles ts = tiles(ni - 1, nj - 3, 64, CRESS_FS_BY, nk - 3, CRESS_FS_ORDER);
-  if (sm) {
-    LAUNCH(x, KT_FORCE_UV, k_force_uv<true>, flat(tuv), bf, 0, tuv, g, o, c, fi, x->ufrc, x->vfrc);
-    LAUNCH(x, KT_FORCE_W, k_force_w<true>, flat(tw), bf, 0, tw, g, o, c, fi, x->wfrc);
-    LAUNCH(x, KT_FORCE_P, k_force_p, flat(tp), bf, 0, tp, g, o, c, B[CRESS_JCB], B[CRESS_JCB8U],
-           B[CRESS_JCB8V], B[CRESS_JCB8W], x->u[C], x->v[C], x->wc, x->pp[C], x->pp[P], x->rbct, x->pfrc);
-  } else {
-    LAUNCH(x, KT_FORCE_UV, k_force_uv<false>, flat(tuv), bf, 0, tuv, g, o, c, fi, x->ufrc, x->vfrc);
-    LAUNCH(x, KT_FORCE_W, k_force_w<false>, flat(tw), bf, 0, tw, g, o, c, fi, x->wfrc);
-    LAUNCH(x, KT_FORCE_P, k_force_p, flat(tp), bf, 0, tp, g, o, c, B[CRESS_JCB], B[CRESS_JCB8U],
-           B[CRESS_JCB8V], B[CRESS_JCB8W], x->u[C], x->v[C], x->wc, x->pp[C], x->pp[P], x->rbct, x->pfrc);
-  }
+  const dim3 bf(64, tn.ff_by), bs(64, tn.fs_by);
+  const Tiles tuv = tiles(ni, nj - 2, 64, tn.ff_by, nk - 3, tn.ff_order);
+  const Tiles tw = tiles(ni - 1, nj - 3, 64, tn.ff_by, nk - 2, tn.ff_order);
+  const Tiles tp = tiles(ni - 1, nj - 3, 64, tn.ff_by, nk - 3, tn.ff_order);
+  const Tiles ts = tiles(ni - 1, nj - 3, 64, tn.fs_by, nk - 3, tn.fs_order);
+  const auto kuv = sm ? (U ? k_force_uv<true, true> : k_force_uv<true, false>)
+                      : (U ? k_force_uv<false, true> : k_force_uv<false, false>);
+  const auto kw = sm ? (U ? k_force_w<true, true> : k_force_w<true, false>)
+                     : (U ? k_force_w<false, true> : k_force_w<false, false>);
+  const auto kp = U ? k_force_p<true> : k_force_p<false>;
+  if (parts & FORCE_UV) LAUNCH(x, KT_FORCE_UV, kuv, flat(tuv), bf, 0, tuv, g, o, c, fi, x->ufrc, x->vfrc);
+  if (parts & FORCE_W) LAUNCH(x, KT_FORCE_W, kw, flat(tw), bf, 0, tw, g, o, c, fi, x->wfrc);
+  if (parts & FORCE_P)
+    LAUNCH(x, KT_FORCE_P, kp, flat(tp), bf, 0, tp, g, o, c, B[CRESS_JCB], B[CRESS_JCB8U], B[CRESS_JCB8V],
+         B[CRESS_JCB8W], x->u[C], x->v[C], x->wc, x->pp[C], x->pp[P], x->rbct, x->pfrc);
   ScalarIn si;
   const int nsc = cf.cphopt == 2 ? 8 : 2;
   {
@@ -868,10 +1040,31 @@ static int large_step(cress_ctx *x, int first) {
   si.ptbr = B[CRESS_PTBR];
   si.rbr = B[CRESS_RBR];
   si.w = x->w[C];
-  if (sm)
-    LAUNCH(x, KT_FORCE_S, k_force_s<true>, flat(ts), bs, 0, ts, g, o, c, si, nsc);
-  else
-    LAUNCH(x, KT_FORCE_S, k_force_s<false>, flat(ts), bs, 0, ts, g, o, c, si, nsc);
+  const auto ks = sm ? (U ? k_force_s<true, true> : k_force_s<true, false>)
+                     : (U ? k_force_s<false, true> : k_force_s<false, false>);
+  if (parts & FORCE_S) LAUNCH(x, KT_FORCE_S, ks, flat(ts), bs, 0, ts, g, o, c, si, nsc);
+}
+
+static int large_step(cress_ctx *x, int first) {
+  const Grid g = x->g;
+  const Opts o = x->o;
+  const Consts c = x->c;
+  const cress_config &cf = x->cfg;
+  hipStream_t st = x->stream;
+  const int ni = g.ni, nj = g.nj, nk = g.nk;
+  const long n = x->n;
+  const int P = (x->rot + 0) % 3, C = (x->rot + 1) % 3, F = (x->rot + 2) % 3;
+  const float dtb = first ? 0.5f * (float)cf.dtbig : (float)cf.dtbig;
+  const float dts = first ? 0.5f * (float)cf.dtsml : (float)cf.dtsml;
+  float **B = x->base;
+  const dim3 b2(64, 4);
+  const int open = cf.wbc == 7;
+  ObcAc oac;
+  ObcLf olf;
+  obc_speeds(x, first, &oac, &olf);
+
+  stamp(x, 0);
+  forcing_stage(x, x->tune, FORCE_ALL);
   stamp(x, 1);
 
   // ---- acoustic substeps (smalldrv / hevi) on the future level ----
@@ -884,7 +1077,9 @@ static int large_step(cress_ctx *x, int first) {
   HIPCHK(hipMemcpyAsync(x->valt, x->v[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(x->w[F], x->w[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(x->pp[F], x->pp[P], n * sizeof(float), hipMemcpyDeviceToDevice, st));
-  LAUNCH(x, KT_PHY2CNT, k_phy2cnt, flat(tiles(ni, nj - 1, 64, 4, nk)), b2, 0, tiles(ni, nj - 1, 64, 4, nk), g,
+  const bool U = (g.umask & ((1u << CRESS_NBASE) - 1u)) == ((1u << CRESS_NBASE) - 1u);
+  const auto kph = U ? k_phy2cnt<true> : k_phy2cnt<false>;
+  LAUNCH(x, KT_PHY2CNT, kph, flat(tiles(ni, nj - 1, 64, 4, nk)), b2, 0, tiles(ni, nj - 1, 64, 4, nk), g,
          o, x->w[F], B[CRESS_JCB8W], x->wc);
   for (int s = 0; s < cf.nsstp; s++) ac_substep(x, dts, open ? &oac : nullptr);
   stamp(x, 2);
@@ -1063,8 +1258,10 @@ struct EnsState {
   float *u[3], *v[3], *ualt, *valt;
   float obc_dtb, obc_dts;
   int obc_have;
+  int tune_ver;
   bool operator==(const EnsState &o) const {
     if (rot != o.rot || ualt != o.ualt || valt != o.valt || obc_have != o.obc_have) return false;
+    if (tune_ver != o.tune_ver) return false;
     if (obc_have && (obc_dtb != o.obc_dtb || obc_dts != o.obc_dts)) return false;
     for (int l = 0; l < 3; l++)
       if (u[l] != o.u[l] || v[l] != o.v[l]) return false;
@@ -1084,6 +1281,7 @@ static EnsState ens_get(const cress_ctx *x) {
   t.obc_dtb = x->obc_dtb;
   t.obc_dts = x->obc_dts;
   t.obc_have = x->obc_have;
+  t.tune_ver = x->tune_ver;
   return t;
 }
 
@@ -1250,6 +1448,237 @@ int cress_ens_sync(cress_ens *e) {
 // ---------------------------------------------------------------------------
 // Tier 2 shims
 // ---------------------------------------------------------------------------
+// Mass and energy budgets of one level in binary64 (diag.hip); out gets
+// CRESS_NDIAG values in the order of include/cress_hip.h.
+int cress_diag(cress_ctx *x, int level, double *out) {
+  if (!x || !out || level < 0 || level > 2) return CRESS_EINVAL;
+  const Grid g = x->g;
+  const int nk = g.nk, s = (x->rot + level) % 3;
+  const long n = x->n;
+  double *part = x->diag, *res = x->diag + (size_t)kDiagBlocks * DIAG_N, *zs = res + DIAG_N;
+  // heights of the scalar levels over flat terrain from the per-level
+  // Jacobian (z at w-level 2 is the surface): z_w(k+1) = z_w(k) + dz jcb(k),
+  // z_s(k) = z_w(k) + dz jcb(k)/2; NaN when jcb varies horizontally
+  std::vector<double> z((size_t)nk + 1, 0.0);
+  const bool jcbu = (g.umask >> CRESS_JCB) & 1u;
+  double zw = 0.0;
+  for (int k = 2; k <= nk - 1; k++) {
+    const double jk = jcbu ? (double)x->tabh[(size_t)CRESS_JCB * nk + k - 1] : NAN;
+    z[k] = zw + 0.5 * x->cfg.dz * jk;
+    zw += x->cfg.dz * jk;
+  }
+  HIPCHK(hipMemcpyAsync(zs, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice, x->stream));
+  DiagArgs a;
+  a.u = x->u[s];
+  a.v = x->v[s];
+  a.w = x->w[s];
+  a.pp = x->pp[s];
+  a.ptp = x->ptp[s];
+  a.q = x->q[s];
+  a.qn = n;
+  a.rst = x->base[CRESS_RST];
+  a.jcb = x->base[CRESS_JCB];
+  a.pbr = x->base[CRESS_PBR];
+  a.ptbr = x->base[CRESS_PTBR];
+  a.prwtr = x->prwtr;
+  a.price = x->price;
+  a.zs = zs;
+  a.dx = x->cfg.dx;
+  a.dy = x->cfg.dy;
+  a.dz = x->cfg.dz;
+  hipLaunchKernelGGL(k_diag_cells, dim3(kDiagBlocks), dim3(256), 0, x->stream, g, a, part);
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, x->stream, (const double *)part, kDiagBlocks, res);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, res, DIAG_N * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return CRESS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Launch-shape autotuner.  Each tunable kernel family is timed on this
+// context's own grid for every candidate shape (rows per workgroup, tile
+// order, halo layout, LDS occupancy cap), and the fastest is kept.  The
+// candidates write only scratch arrays and the future level, which the next
+// large step forms again from the past level, so tuning may run between any
+// two steps; the results do not depend on the shapes (tests show them
+// bitwise equal).  Results are cached per process by grid size and options.
+namespace {
+constexpr int kTuneInts = 9;
+
+void tune_to_ints(const LaunchTune &t, int *v) {
+  const int a[kTuneInts] = {t.ptuv_by, t.ptuv_slots, t.ptuv_order, t.col_by, t.col_lds,
+                            t.fs_by,   t.fs_order,   t.ff_by,      t.ff_order};
+  for (int i = 0; i < kTuneInts; i++) v[i] = a[i];
+}
+
+bool tune_valid(const LaunchTune &t) {
+  auto in = [](int v, std::initializer_list<int> set) {
+    for (int s : set)
+      if (v == s) return true;
+    return false;
+  };
+  return in(t.ptuv_by, {1, 2, 4, 8, 16}) && in(t.ptuv_slots, {0, 1}) && in(t.ptuv_order, {0, 1, 2}) &&
+         in(t.col_by, {1, 2, 4}) && t.col_lds >= 0 && t.col_lds <= 65536 && in(t.fs_by, {1, 2, 4, 8}) &&
+         in(t.fs_order, {0, 1, 2}) && in(t.ff_by, {1, 2, 4, 8, 16}) && in(t.ff_order, {0, 1, 2});
+}
+
+std::mutex tune_mu;
+std::map<std::vector<long>, LaunchTune> tune_cache;
+
+std::vector<long> tune_key(const cress_ctx *x) {
+  const cress_config &c = x->cfg;
+  return {c.ni, c.nj, c.nk, c.cphopt, c.advopt, c.smtopt, c.wbc, c.divopt, c.buyopt, (long)x->g.umask};
+}
+}  // namespace
+
+int cress_set_launch(cress_ctx *x, const int *shape, int n) {
+  if (!x || !shape || n != kTuneInts) return CRESS_EINVAL;
+  LaunchTune t;
+  t.ptuv_by = shape[0];
+  t.ptuv_slots = shape[1];
+  t.ptuv_order = shape[2];
+  t.col_by = shape[3];
+  t.col_lds = shape[4];
+  t.fs_by = shape[5];
+  t.fs_order = shape[6];
+  t.ff_by = shape[7];
+  t.ff_order = shape[8];
+  if (!tune_valid(t)) return CRESS_EINVAL;
+  x->tune = t;
+  x->tune_ver++;
+  return CRESS_OK;
+}
+
+int cress_get_launch(const cress_ctx *x, int *shape, int n) {
+  if (!x || !shape || n < kTuneInts) return CRESS_EINVAL;
+  tune_to_ints(x->tune, shape);
+  return CRESS_OK;
+}
+
+int cress_autotune(cress_ctx *x, int *shape, int n) {
+  if (!x || (shape && n < kTuneInts)) return CRESS_EINVAL;
+  if (x->timing) return CRESS_EINVAL;
+  const std::vector<long> key = tune_key(x);
+  {
+    std::lock_guard<std::mutex> lk(tune_mu);
+    auto it = tune_cache.find(key);
+    if (it != tune_cache.end()) {
+      x->tune = it->second;
+      x->tune_ver++;
+      if (shape) tune_to_ints(x->tune, shape);
+      return CRESS_OK;
+    }
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) hipEventDestroy(e0);
+    return CRESS_EHIP;
+  }
+  int rc = CRESS_OK;
+  // mean device time of one launch sequence: one untimed run, one timed
+  // run to size the repetitions (about 2 ms in all, 3 to 40 runs), then
+  // the timed repetitions
+  auto time_ms = [&](auto launch) -> float {
+    launch();
+    float ms = 0.0f;
+    if (hipEventRecord(e0, x->stream) != hipSuccess) rc = CRESS_EHIP;
+    launch();
+    if (hipEventRecord(e1, x->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      rc = CRESS_EHIP;
+    int reps = ms > 0.0f ? (int)(2.0f / ms) : 40;
+    reps = reps < 3 ? 3 : (reps > 40 ? 40 : reps);
+    if (hipEventRecord(e0, x->stream) != hipSuccess) rc = CRESS_EHIP;
+    for (int r = 0; r < reps; r++) launch();
+    if (hipEventRecord(e1, x->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      rc = CRESS_EHIP;
+    return ms / (float)reps;
+  };
+  LaunchTune best = x->tune;
+  const float dts = (float)x->cfg.dtsml;
+  // fused ptil/stepuv
+  {
+    const AcPtUv pu = ptuv_args(x, dts);
+    float tmin = 1e30f;
+    for (int by : {4, 8, 16})
+      for (int slots : {0, 1})
+        for (int order : {1, 2}) {
+          LaunchTune t = best;
+          t.ptuv_by = by;
+          t.ptuv_slots = slots;
+          t.ptuv_order = order;
+          const float ms = time_ms([&] { launch_ptuv(x, t, pu); });
+          if (ms < tmin) {
+            tmin = ms;
+            best.ptuv_by = by;
+            best.ptuv_slots = slots;
+            best.ptuv_order = order;
+          }
+        }
+  }
+  // column kernel
+  {
+    const AcCol ac = col_args(x, dts);
+    float tmin = 1e30f;
+    for (int by : {1, 2, 4})
+      for (int lds : {0, 4096, 8192, 16384}) {
+        LaunchTune t = best;
+        t.col_by = by;
+        t.col_lds = lds;
+        const float ms = time_ms([&] { launch_col(x, t, ac); });
+        if (ms < tmin) {
+          tmin = ms;
+          best.col_by = by;
+          best.col_lds = lds;
+        }
+      }
+  }
+  // scalar forcing, then the u/v, w and p' forcing together
+  {
+    float tmin = 1e30f;
+    for (int by : {2, 4, 8})
+      for (int order : {1, 2}) {
+        LaunchTune t = best;
+        t.fs_by = by;
+        t.fs_order = order;
+        const float ms = time_ms([&] { forcing_stage(x, t, FORCE_S); });
+        if (ms < tmin) {
+          tmin = ms;
+          best.fs_by = by;
+          best.fs_order = order;
+        }
+      }
+  }
+  {
+    float tmin = 1e30f;
+    for (int by : {2, 4, 8})
+      for (int order : {1, 2}) {
+        LaunchTune t = best;
+        t.ff_by = by;
+        t.ff_order = order;
+        const float ms = time_ms([&] { forcing_stage(x, t, FORCE_UV | FORCE_W | FORCE_P); });
+        if (ms < tmin) {
+          tmin = ms;
+          best.ff_by = by;
+          best.ff_order = order;
+        }
+      }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (hipGetLastError() != hipSuccess) rc = CRESS_EHIP;
+  if (rc) return rc;
+  x->tune = best;
+  x->tune_ver++;
+  {
+    std::lock_guard<std::mutex> lk(tune_mu);
+    tune_cache[key] = best;
+  }
+  if (shape) tune_to_ints(best, shape);
+  return CRESS_OK;
+}
+
 struct DevBuf {
   void *p = nullptr;
   ~DevBuf() {
@@ -1719,9 +2148,10 @@ int cress_advp(const cress_config *cfg, const float *jcb8u, const float *jcb8v, 
       (rc = to_dev(e, u, n)) || (rc = to_dev(f, v, n)) || (rc = to_dev(h, wc, n)) || (rc = to_dev(p, pp, n)) ||
       (rc = to_dev(q, pfrc, n)))
     return rc;
-  const Tiles tp = tiles(g.ni - 1, g.nj - 3, 64, CRESS_FF_BY, g.nk - 3, CRESS_FF_ORDER);
+  const LaunchTune tn = tune_default();
+  const Tiles tp = tiles(g.ni - 1, g.nj - 3, 64, tn.ff_by, g.nk - 3, tn.ff_order);
   const float *pj = (const float *)a.p, *pp_ = (const float *)p.p;
-  hipLaunchKernelGGL(k_force_p, flat(tp), dim3(64, CRESS_FF_BY), 0, 0, tp, g, o, c, pj, pj,
+  hipLaunchKernelGGL(k_force_p<false>, flat(tp), dim3(64, tn.ff_by), 0, 0, tp, g, o, c, pj, pj,
                      (const float *)b.p, (const float *)d.p, (const float *)e.p, (const float *)f.p,
                      (const float *)h.p, pp_, pp_, pj, (float *)q.p);
   HIPCHK(hipGetLastError());
@@ -1782,6 +2212,17 @@ int cress_bcycle(const cress_config *cfg, int iwsnd, int iwrcv, int iesnd, int i
 // Attach a network to a context (mode 1: record pairs from the plain
 // scheme into its capture buffers; 2: use it for the source terms of
 // every active cell inside its envelope) or detach it (mlp NULL or mode 0).
+int cress_ctx_mlp_cells(cress_ctx *x, long *cells) {
+  if (!x || !cells) return CRESS_EINVAL;
+  *cells = 0;
+  if (x->mlp_mode != 2 || !x->ml.count) return CRESS_OK;
+  unsigned c = 0;
+  HIPCHK(hipStreamSynchronize(x->stream));
+  HIPCHK(hipMemcpy(&c, x->ml.count, sizeof c, hipMemcpyDeviceToHost));
+  *cells = (long)c;
+  return CRESS_OK;
+}
+
 int cress_ctx_set_mlp(cress_ctx *x, cress_mlp *mlp, int mode) {
   if (!x || mode < 0 || mode > 2 || (mode && !mlp)) return CRESS_EINVAL;
   if (mode && x->cfg.cphopt != 2) return CRESS_EUNSUPPORTED;

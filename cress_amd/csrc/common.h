@@ -28,7 +28,19 @@ struct Grid {
     return (long)i + nx * (long)j + nxy * (long)(k - 1);
   }
   __host__ __device__ long n() const { return nxy * (long)nk; }
+  // byte offset of (i, j, k) in 32 bits (fields are below 4 GiB): the
+  // scalar-base + 32-bit vector-offset addressing form, one VGPR per
+  // address and 32-bit integer arithmetic for the neighbours' offsets
+  __host__ __device__ uint32_t off(int i, int j, int k) const {
+    return 4u * ((uint32_t)i + (uint32_t)nx * (uint32_t)j + (uint32_t)nxy * (uint32_t)(k - 1));
+  }
 };
+
+// Element at a 32-bit byte offset from a field's base.
+__host__ __device__ __forceinline__ const float &at32(const float *a, uint32_t o) {
+  return *(const float *)((const char *)a + o);
+}
+__host__ __device__ __forceinline__ float &at32(float *a, uint32_t o) { return *(float *)((char *)a + o); }
 
 inline Grid make_grid(int ni, int nj, int nk) {
   Grid g;
@@ -69,8 +81,14 @@ inline Grid make_grid_padded(int ni, int nj, int nk) {
 #define MUNI(id) ((g.umask >> (id)) & 1u)
 #define MTAB(id, k) g.tab[(id) * g.nk + (k) - 1]
 #define MP(id, arr, p, k) (MUNI(id) ? MTAB(id, k) : (arr)[p])
-#define MT(id, arr, i, j, k) (MUNI(id) ? MTAB(id, k) : (arr)[g.at(i, j, k)])
+#define MT(id, arr, i, j, k) (MUNI(id) ? MTAB(id, k) : at32(arr, g.off(i, j, k)))
 #define MQ(id, arr, q, k) (MUNI(id) ? MTAB(id, k) : *(const float *)((const char *)(arr) + (q)))
+// In a kernel templated on U (true: every base/metric field it reads is a
+// per-level table), the table read needs no run-time test, so the compiler
+// keeps it a scalar load instead of selecting between two addresses.
+#define MQU(id, arr, q, k) (U ? MTAB(id, k) : MQ(id, arr, q, k))
+#define MTU(id, arr, i, j, k) (U ? MTAB(id, k) : MT(id, arr, i, j, k))
+#define MPU(id, arr, p, k) (U ? MTAB(id, k) : MP(id, arr, p, k))
 
 // Flattened launches with an XCD-aware tile order.  The workgroups of a
 // launch go to the 8 XCDs round-robin (workgroup b to XCD b % 8), and each

@@ -18,51 +18,32 @@ struct ScalarIn {
   const float *rstxu, *rstxv, *rstxwc, *rst, *qvbr, *rbct, *ptbr, *rbr, *w;
 };
 
-// Launch shape of the column kernel: CRESS_COL_BY rows of 64 columns per
-// workgroup; CRESS_COL_LDS bytes of LDS per workgroup hold the per-level
-// Thomas table when the metrics are uniform and cap how many column
-// wavefronts share a CU.  Measured at 1024x1024x80, ms per launch: with
-// the metrics streamed as 3-D fields, 8 waves/SIMD 2.84, 5 (16 KiB) 2.27,
-// 3 (24 KiB) 2.33 -- more sweeps evicted each other's neighbour rows from
-// L2; with the metrics from per-level tables, 16 KiB 1.41, 12 KiB 1.27,
-// 8 KiB 1.26, 4 KiB (occupancy set by registers) 1.26.
-#ifndef CRESS_COL_BY
-#define CRESS_COL_BY 2
-#endif
-#ifndef CRESS_COL_LDS
-#define CRESS_COL_LDS 4096
-#endif
+// Launch shapes.  Each stencil and column kernel takes its rows per
+// workgroup from blockDim.y (at most the _BYMAX below) and its tile order
+// from Tiles, so one build holds every shape; the context's defaults (the
+// fastest measured at 1024x1024x80) can be replaced per grid by the
+// autotuner (cress_autotune).  Measurements at 1024x1024x80, ms per launch:
+//   column kernel, metrics streamed as 3-D fields: 8 waves/SIMD 2.84, 5
+//   (16 KiB LDS cap) 2.27, 3 (24 KiB) 2.33; metrics from per-level tables:
+//   16 KiB 1.41, 12 KiB 1.27, 8 KiB 1.26, 4 KiB (registers decide) 1.26;
+//   fused ptil/stepuv: x-fastest 64x8 1.405, 64x4 1.412; z-fastest 64x4
+//   1.571, 64x8 1.580; from per-level tables 64x4 1.11, 64x8 1.14, 64x16
+//   1.08, z-fastest 64x8 1.16; 64x4 threads each marching 16 rows in y
+//   1.17-1.21; own-load-first 64x16 0.99, with the slot layout 64x8 0.97;
+//   scalar forcing 64x4 3.59, 64x8 3.79, 64x16 3.66, 64x16 x-fastest 4.97;
+//   u/v forcing 64x4 1.91, 64x8 2.01, 64x16 2.42.
+constexpr int CRESS_PTUV_BYMAX = 16;
+constexpr int CRESS_COL_BYMAX = 4;
+constexpr int CRESS_FS_BYMAX = 8;
 
-// Rows per workgroup of the fused ptil/stepuv kernel and its tile order
-// (measured at 1024x1024x80, ms per launch, metrics streamed as 3-D fields:
-// x-fastest 64x8 1.405, 64x4 1.412; z-fastest 64x4 1.571, 64x8 1.580; from
-// per-level tables: 64x4 1.11, 64x8 1.14, 64x16 1.08, z-fastest 64x8 1.16.
-// A variant with 64x4 threads each marching 16 rows in y, ptil(i,j-1)
-// carried in a register and ptil(i-1,j) from a wave shuffle, ran 1.17-1.21.)
-#ifndef CRESS_PTUV_BY
-#define CRESS_PTUV_BY 16
-#endif
-// Launch shape of the forcing kernels: 64 x CRESS_FS_BY threads per
-// workgroup for the scalar forcing, 64 x CRESS_FF_BY for u, v, w and p', one
-// level per workgroup, tiles in the given order.  Measured at 1024x1024x80,
-// ms per launch: scalar forcing 64x4 3.59, 64x8 3.79, 64x16 3.66, 64x16
-// x-fastest 4.97; u/v forcing 64x4 1.91, 64x8 2.01, 64x16 2.42.
-#ifndef CRESS_FS_BY
-#define CRESS_FS_BY 4
-#endif
-#ifndef CRESS_FS_ORDER
-#define CRESS_FS_ORDER 2
-#endif
-#ifndef CRESS_FF_BY
-#define CRESS_FF_BY 4
-#endif
-#ifndef CRESS_FF_ORDER
-#define CRESS_FF_ORDER 2
-#endif
-
-#ifndef CRESS_PTUV_ORDER
-#define CRESS_PTUV_ORDER 1
-#endif
+// The tunable launch parameters of one context (defaults and ranges in
+// capi.hip: tune_default, cress_autotune).
+struct LaunchTune {
+  int ptuv_by, ptuv_slots, ptuv_order;   // fused ptil/stepuv
+  int col_by, col_lds;                   // column kernel (LDS bytes cap occupancy)
+  int fs_by, fs_order;                   // scalar forcing
+  int ff_by, ff_order;                   // u/v, w and p' forcing
+};
 
 struct AcPtUv {
   float dts, divcv;
@@ -76,6 +57,12 @@ struct AcCol {
   float *ptil;  // read; its consumed slots hold the Thomas super-diagonal
   const float *jcb, *jcb8u, *jcb8v, *jcb8w, *rcsq, *rst, *rbr, *rst8w;
   float *wf, *wc, *ppf;
+  // With rst8w, rcsq, rst, rbr, jcb, jcb8u, jcb8v, jcb8w horizontally uniform
+  // (per-level tables, read without a run-time test), the Thomas
+  // coefficients of stepwi/gaussel depend on the level and dts only:
+  // coef[3k] = rr(k), coef[3k+1] = 1/(ss(k) - rr(k) ss'(k-1)) (1/ss(3) at
+  // k=3), coef[3k+2] = ss'(k), k = 3..nk-2 (k_ac_col<true>); else null.
+  const float *coef;
 };
 
 struct StepsIn {
@@ -107,22 +94,32 @@ struct AsselinSet {
   const float *xf[12];
 };
 
+// U: every base/metric field the kernel reads is a per-level table (Grid::
+// umask has all of CRESS_JCB..CRESS_RCSQ), read without a run-time test.
+template <bool U>
 __global__ void k_phy2cnt(Tiles T, Grid g, Opts o, const float *w, const float *jcb8w, float *wc);
+template <bool U>
 __global__ void k_prep(Tiles T, Grid g, Opts o, const float *u, const float *v, const float *w,
                        const float *rst8u, const float *rst8v, const float *rst8w,
                        const float *jcb8w, float *wc, float *rstxu, float *rstxv, float *rstxwc);
 // SM: with the 4th-order smoothing (smtopt 2/3) compiled in
-template <bool SM>
+template <bool SM, bool U>
 __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *ufrc, float *vfrc);
-template <bool SM>
+template <bool SM, bool U>
 __global__ void k_force_w(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *wfrc);
+template <bool U>
 __global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, const float *jcb8u,
                           const float *jcb8v, const float *jcb8w, const float *u, const float *v,
                           const float *wc, const float *pp, const float *ppp, const float *rbct,
                           float *pfrc);
-template <bool SM>
+template <bool SM, bool U>
 __global__ void k_force_s(Tiles T, Grid g, Opts o, Consts c, ScalarIn in, int nscalar);
+// U: rst8u, rst8v, rst8w and jcb are per-level tables; SLOTS: the ptil
+// tile and its halo in slots of consecutive points (else one point per
+// thread, the halo row and column by the first row and lane)
+template <bool U, int SLOTS>
 __global__ void k_ac_ptuv(Tiles T, Grid g, Opts o, Consts c, AcPtUv a);
+template <bool UNI>
 __global__ void k_ac_col(Tiles T, Grid g, Opts o, Consts c, AcCol a);
 __global__ void k_ac_ring(Grid g, Opts o, const float *jcb8w, float *wf, float *wc, float *ppf);
 __global__ void k_steps(Tiles T, Grid g, float dtb2, const float *rst, StepsIn s, int withq);

@@ -11,7 +11,7 @@
 
 namespace cress {
 
-#define AT(a, i, j, k) a[g.at(i, j, k)]
+#define AT(a, i, j, k) at32(a, g.off(i, j, k))
 // Load / store at a 32-bit byte offset from a kernel-argument base: the
 // scalar-base + vector-offset addressing form, one VGPR per address.
 #define LD(arr, off) (*(const float *)((const char *)(arr) + (off)))
@@ -19,7 +19,8 @@ namespace cress {
 
 // rbct(1:ni,1:nj,1:nk,1:2), component 2 (vspdmp.f90:269-285)
 __device__ __forceinline__ float rbct2(const float *rbct, const Grid &g, int i, int j, int k) {
-  return rbct[(long)(i - 1) + (long)g.ni * ((long)(j - 1) + (long)g.nj * ((long)(k - 1) + g.nk))];
+  return at32(rbct, 4u * ((uint32_t)(i - 1) + (uint32_t)g.ni * ((uint32_t)(j - 1) + (uint32_t)g.nj *
+                                                                 ((uint32_t)(k - 1) + (uint32_t)g.nk))));
 }
 
 // vspuvw.f90:222-238: sponge coefficient averaged to the u/v level
@@ -57,37 +58,41 @@ __device__ __forceinline__ float smooth4(float f, const Consts &c, int smtopt, b
 // Contravariant w (phy2cnt.f90:193-227 then vbcwc.f90:130-186) and the
 // mass-flux velocities rstx* (rstuvwc.f90:172-210).
 // ---------------------------------------------------------------------------
+template <bool U>
 __device__ __forceinline__ float wc_raw(const Grid &g, int sthopt, const float *w,
                                         const float *jcb8w, int i, int j, int k) {
-  return sthopt == 0 ? AT(w, i, j, k) : AT(w, i, j, k) / MT(CRESS_JCB8W, jcb8w, i, j, k);
+  return sthopt == 0 ? AT(w, i, j, k) : AT(w, i, j, k) / MTU(CRESS_JCB8W, jcb8w, i, j, k);
 }
 
+template <bool U>
 __device__ __forceinline__ float wc_value(const Grid &g, const Opts &o, const float *w,
                                           const float *jcb8w, int i, int j, int k) {
   const int nk = g.nk;
   if (k == 1) {
-    if (o.bbc == 2) return -wc_raw(g, o.sthopt, w, jcb8w, i, j, 3);
-    return wc_raw(g, o.sthopt, w, jcb8w, i, j, 2);
+    if (o.bbc == 2) return -wc_raw<U>(g, o.sthopt, w, jcb8w, i, j, 3);
+    return wc_raw<U>(g, o.sthopt, w, jcb8w, i, j, 2);
   }
   if (k == 2 && o.bbc == 2) return 0.0f;
   if (k == nk) {
-    if (o.tbc == 2) return -wc_raw(g, o.sthopt, w, jcb8w, i, j, nk - 2);
-    return wc_raw(g, o.sthopt, w, jcb8w, i, j, nk - 1);
+    if (o.tbc == 2) return -wc_raw<U>(g, o.sthopt, w, jcb8w, i, j, nk - 2);
+    return wc_raw<U>(g, o.sthopt, w, jcb8w, i, j, nk - 1);
   }
   if (k == nk - 1 && o.tbc == 2) return 0.0f;
-  return wc_raw(g, o.sthopt, w, jcb8w, i, j, k);
+  return wc_raw<U>(g, o.sthopt, w, jcb8w, i, j, k);
 }
 
 // wc over i=1..ni-1, j=1..nj-1, k=1..nk (the phy2cnt+vbcwc write range)
+template <bool U>
 __global__ void k_phy2cnt(Tiles T, Grid g, Opts o, const float *w, const float *jcb8w, float *wc) {
   const uint3 xt = xcd_tile(T);
   const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 1 + xt.y * blockDim.y + threadIdx.y;
   const int k = 1 + xt.z;
   if (i < 1 || i > g.ni - 1 || j > g.nj - 1) return;
-  AT(wc, i, j, k) = wc_value(g, o, w, jcb8w, i, j, k);
+  AT(wc, i, j, k) = wc_value<U>(g, o, w, jcb8w, i, j, k);
 }
 
+template <bool U>
 __global__ void k_prep(Tiles T, Grid g, Opts o, const float *u, const float *v, const float *w,
                        const float *rst8u, const float *rst8v, const float *rst8w,
                        const float *jcb8w, float *wc, float *rstxu, float *rstxv, float *rstxwc) {
@@ -102,11 +107,11 @@ __global__ void k_prep(Tiles T, Grid g, Opts o, const float *u, const float *v, 
   const bool dv = k <= g.nk - 1 && i >= 1 && i <= g.ni - 1 && j >= o.js && j <= g.nj + 1 - o.jn;
   float c = 0.0f, xw = 0.0f, xu = 0.0f, xv = 0.0f;
   if (dw) {
-    c = wc_value(g, o, w, jcb8w, i, j, k);
-    xw = MT(CRESS_RST8W, rst8w, i, j, k) * c;
+    c = wc_value<U>(g, o, w, jcb8w, i, j, k);
+    xw = MTU(CRESS_RST8W, rst8w, i, j, k) * c;
   }
-  if (du) xu = MT(CRESS_RST8U, rst8u, i, j, k) * AT(u, i, j, k);
-  if (dv) xv = MT(CRESS_RST8V, rst8v, i, j, k) * AT(v, i, j, k);
+  if (du) xu = MTU(CRESS_RST8U, rst8u, i, j, k) * AT(u, i, j, k);
+  if (dv) xv = MTU(CRESS_RST8V, rst8v, i, j, k) * AT(v, i, j, k);
   if (dw) {
     AT(wc, i, j, k) = c;
     AT(rstxwc, i, j, k) = xw;
@@ -231,7 +236,7 @@ __device__ float adv_w(float f, const Grid &g, const Opts &o, const Consts &c, c
 // ufrc over i=2..ni-1, j=2..nj-2 and vfrc over i=2..ni-2, j=2..nj-1; k=2..nk-2.
 // Accumulation order of forceuvw.f90:434-626: 0, smoothing, sponge,
 // Coriolis, advection.
-template <bool SM>
+template <bool SM, bool U>
 __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *ufrc, float *vfrc) {
   const uint3 xt = xcd_tile(T);
   const int i = xt.x * blockDim.x + threadIdx.x;
@@ -250,16 +255,17 @@ __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float 
       f = smooth4(f, c, o.smtopt, band, k == 2, k == g.nk - 2, [&](int di, int dj, int dk) {
         const long p = g.at(i + di, j + dj, k + dk);
         const int kk = k + dk;
-        return MP(CRESS_RST8U, in.rst8u, p, kk) * (in.up[p] - MP(CRESS_UBR, in.ubr, p, kk)) /
-               MP(CRESS_JCB8U, in.jcb8u, p, kk);
+        return MPU(CRESS_RST8U, in.rst8u, p, kk) * (in.up[p] - MPU(CRESS_UBR, in.ubr, p, kk)) /
+               MPU(CRESS_JCB8U, in.jcb8u, p, kk);
       });
     }
     if (o.vsp_u && sp)
-      f = f - MT(CRESS_RST8U, in.rst8u, i, j, k) * (rbct8s(in.rbct, g, i - 1, j, k) + rbct8s(in.rbct, g, i, j, k)) *
-                  (AT(in.up, i, j, k) - MT(CRESS_UBR, in.ubr, i, j, k));
+      f = f - MTU(CRESS_RST8U, in.rst8u, i, j, k) * (rbct8s(in.rbct, g, i - 1, j, k) + rbct8s(in.rbct, g, i, j, k)) *
+                  (AT(in.up, i, j, k) - MTU(CRESS_UBR, in.ubr, i, j, k));
     if (o.coropt) {
       auto cr = [&](int ii) {
-        return in.fc[(long)ii + (long)(g.ni + 2) * j] * MT(CRESS_RST, in.rst, ii, j, k) * (AT(in.v, ii, j, k) + AT(in.v, ii, j + 1, k));
+        return at32(in.fc, 4u * ((uint32_t)ii + (uint32_t)(g.ni + 2) * (uint32_t)j)) * MTU(CRESS_RST, in.rst, ii, j, k) *
+               (AT(in.v, ii, j, k) + AT(in.v, ii, j + 1, k));
       };
       f = f + (cr(i - 1) + cr(i));
     }
@@ -272,16 +278,17 @@ __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float 
       f = smooth4(f, c, o.smtopt, band, k == 2, k == g.nk - 2, [&](int di, int dj, int dk) {
         const long p = g.at(i + di, j + dj, k + dk);
         const int kk = k + dk;
-        return MP(CRESS_RST8V, in.rst8v, p, kk) * (in.vp[p] - MP(CRESS_VBR, in.vbr, p, kk)) /
-               MP(CRESS_JCB8V, in.jcb8v, p, kk);
+        return MPU(CRESS_RST8V, in.rst8v, p, kk) * (in.vp[p] - MPU(CRESS_VBR, in.vbr, p, kk)) /
+               MPU(CRESS_JCB8V, in.jcb8v, p, kk);
       });
     }
     if (o.vsp_v && sp)
-      f = f - MT(CRESS_RST8V, in.rst8v, i, j, k) * (rbct8s(in.rbct, g, i, j - 1, k) + rbct8s(in.rbct, g, i, j, k)) *
-                  (AT(in.vp, i, j, k) - MT(CRESS_VBR, in.vbr, i, j, k));
+      f = f - MTU(CRESS_RST8V, in.rst8v, i, j, k) * (rbct8s(in.rbct, g, i, j - 1, k) + rbct8s(in.rbct, g, i, j, k)) *
+                  (AT(in.vp, i, j, k) - MTU(CRESS_VBR, in.vbr, i, j, k));
     if (o.coropt) {
       auto cr = [&](int jj) {
-        return -in.fc[(long)i + (long)(g.ni + 2) * jj] * MT(CRESS_RST, in.rst, i, jj, k) * (AT(in.u, i, jj, k) + AT(in.u, i + 1, jj, k));
+        return -at32(in.fc, 4u * ((uint32_t)i + (uint32_t)(g.ni + 2) * (uint32_t)jj)) * MTU(CRESS_RST, in.rst, i, jj, k) *
+               (AT(in.u, i, jj, k) + AT(in.u, i + 1, jj, k));
       };
       f = f + (cr(j - 1) + cr(j));
     }
@@ -292,12 +299,13 @@ __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float 
 }
 
 // moist buoyancy at a scalar level, buoywb.f90:204-297 (gwmopt=0)
+template <bool U>
 __device__ __forceinline__ float wb8s_at(const Grid &g, const Opts &o, const Consts &c, const ForceIn &in,
                                          int i, int j, int k) {
   const long p = g.at(i, j, k);
-  const float qvbr = MP(CRESS_QVBR, in.qvbr, p, k);
+  const float qvbr = MPU(CRESS_QVBR, in.qvbr, p, k);
   const float qvd = in.q[p] - qvbr;
-  const float rst = MP(CRESS_RST, in.rst, p, k), ptbr = MP(CRESS_PTBR, in.ptbr, p, k);
+  const float rst = MPU(CRESS_RST, in.rst, p, k), ptbr = MPU(CRESS_PTBR, in.ptbr, p, k);
   if (o.cphopt == 0)
     return c.g05 * rst * (qvd / (0.622f + qvbr) - qvd / (1.0f + qvbr) + in.ptp[p] / ptbr);
   // qall: totalqwi.f90:163-164, qc+qr+qi+qs+qg left to right
@@ -307,7 +315,7 @@ __device__ __forceinline__ float wb8s_at(const Grid &g, const Opts &o, const Con
 }
 
 // wfrc over i=2..ni-2, j=2..nj-2, k=2..nk-1
-template <bool SM>
+template <bool SM, bool U>
 __global__ void k_force_w(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *wfrc) {
   const uint3 xt = xcd_tile(T);
   const int i = xt.x * blockDim.x + threadIdx.x;
@@ -320,13 +328,13 @@ __global__ void k_force_w(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *
     f = smooth4(f, c, o.smtopt, band, k == 2, k == g.nk - 1, [&](int di, int dj, int dk) {
       const long p = g.at(i + di, j + dj, k + dk);
       const int kk = k + dk;
-      return MP(CRESS_RST8W, in.rst8w, p, kk) * in.wp[p] / MP(CRESS_JCB8W, in.jcb8w, p, kk);
+      return MPU(CRESS_RST8W, in.rst8w, p, kk) * in.wp[p] / MPU(CRESS_JCB8W, in.jcb8w, p, kk);
     });
   }
   if (o.vsp_w && k >= o.ksp0 && k <= g.nk - 1)
-    f = f - rbct2(in.rbct, g, i, j, k) * MT(CRESS_RST8W, in.rst8w, i, j, k) * AT(in.wp, i, j, k);
+    f = f - rbct2(in.rbct, g, i, j, k) * MTU(CRESS_RST8W, in.rst8w, i, j, k) * AT(in.wp, i, j, k);
   if (o.buyopt == 1 && k >= 3 && k <= g.nk - 2)
-    f = f + (wb8s_at(g, o, c, in, i, j, k - 1) + wb8s_at(g, o, c, in, i, j, k));
+    f = f + (wb8s_at<U>(g, o, c, in, i, j, k - 1) + wb8s_at<U>(g, o, c, in, i, j, k));
   AT(wfrc, i, j, k) = adv_w(f, g, o, c, in.w, in.rstxu, in.rstxv, in.rstxwc, i, j, k);
 }
 
@@ -337,6 +345,7 @@ __global__ void k_force_w(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *
 // forcep's smtopt slot (forcedrv.f90:598, idzeropt), so the reference's
 // large step never smooths p'.
 // ---------------------------------------------------------------------------
+template <bool U>
 __global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, const float *jcb8u,
                           const float *jcb8v, const float *jcb8w, const float *u, const float *v,
                           const float *wc, const float *pp, const float *ppp, const float *rbct,
@@ -346,9 +355,9 @@ __global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, c
   const int j = 2 + xt.y * blockDim.y + threadIdx.y;
   const int k = 2 + xt.z;
   if (i < 2 || i > g.ni - 2 || j > g.nj - 2) return;
-  auto jxu = [&](int ii, int jj) { return MT(CRESS_JCB8U, jcb8u, ii, jj, k) * AT(u, ii, jj, k); };
-  auto jxv = [&](int ii, int jj) { return MT(CRESS_JCB8V, jcb8v, ii, jj, k) * AT(v, ii, jj, k); };
-  auto jxw = [&](int kk) { return MT(CRESS_JCB8W, jcb8w, i, j, kk) * AT(wc, i, j, kk); };
+  auto jxu = [&](int ii, int jj) { return MTU(CRESS_JCB8U, jcb8u, ii, jj, k) * AT(u, ii, jj, k); };
+  auto jxv = [&](int ii, int jj) { return MTU(CRESS_JCB8V, jcb8v, ii, jj, k) * AT(v, ii, jj, k); };
+  auto jxw = [&](int kk) { return MTU(CRESS_JCB8W, jcb8w, i, j, kk) * AT(wc, i, j, kk); };
   auto t1 = [&](int ii) { return jxu(ii, j) * (AT(pp, ii, j, k) - AT(pp, ii - 1, j, k)) * c.dxv05n; };
   auto t2 = [&](int jj) { return jxv(i, jj) * (AT(pp, i, jj, k) - AT(pp, i, jj - 1, k)) * c.dyv05n; };
   auto t3 = [&](int kk) { return jxw(kk) * (AT(pp, i, j, kk) - AT(pp, i, j, kk - 1)) * c.dzv05n; };
@@ -381,7 +390,7 @@ __global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, c
     }
   }
   if (o.vsp_p && k >= o.ksp0 - 1 && k <= g.nk - 2)
-    f = f - 0.5f * (rbct2(rbct, g, i, j, k) + rbct2(rbct, g, i, j, k + 1)) * MT(CRESS_JCB, jcb, i, j, k) *
+    f = f - 0.5f * (rbct2(rbct, g, i, j, k) + rbct2(rbct, g, i, j, k + 1)) * MTU(CRESS_JCB, jcb, i, j, k) *
                 AT(ppp, i, j, k);
   AT(pfrc, i, j, k) = f;
 }
@@ -490,8 +499,8 @@ __device__ __forceinline__ float adv_s_val(const Opts &o, const Consts &c, bool 
 // (advs.f90:205-420), the vspopt=2 sponge (vsps.f90:160-198, vspqv) and, for
 // theta', advbspt (advbspt.f90:143-190).  The flux weights are loaded once
 // and shared by the scalars.
-template <bool SM>
-__global__ void __launch_bounds__(64 * CRESS_FS_BY) k_force_s(Tiles T, Grid g, Opts o, Consts c, ScalarIn in, int nscalar) {
+template <bool SM, bool U>
+__global__ void __launch_bounds__(64 * CRESS_FS_BYMAX) k_force_s(Tiles T, Grid g, Opts o, Consts c, ScalarIn in, int nscalar) {
   const uint3 xt = xcd_tile(T);
   const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 2 + xt.y * blockDim.y + threadIdx.y;
@@ -514,7 +523,7 @@ __global__ void __launch_bounds__(64 * CRESS_FS_BY) k_force_s(Tiles T, Grid g, O
   float r = 0.0f, rst = 0.0f;
   if (sponge) {
     r = 0.5f * (rbct2(in.rbct, g, i, j, k) + rbct2(in.rbct, g, i, j, k + 1));
-    rst = MQ(CRESS_RST, in.rst, q, k);
+    rst = MQU(CRESS_RST, in.rst, q, k);
   }
   // each scalar's stencil is loaded before the previous scalar's forcing
   // is stored, so that the loads of consecutive scalars overlap
@@ -529,21 +538,21 @@ __global__ void __launch_bounds__(64 * CRESS_FS_BY) k_force_s(Tiles T, Grid g, O
       f = smooth4(f, c, o.smtopt, h4, k == 2, k == g.nk - 2, [&](int di, int dj, int dk) {
         const long p = g.at(i + di, j + dj, k + dk);
         const int kk = k + dk;
-        return sc == 1 ? MP(CRESS_RBR, in.rbr, p, kk) * (sp[p] - MP(CRESS_QVBR, in.qvbr, p, kk))
-                       : MP(CRESS_RBR, in.rbr, p, kk) * sp[p];
+        return sc == 1 ? MPU(CRESS_RBR, in.rbr, p, kk) * (sp[p] - MPU(CRESS_QVBR, in.qvbr, p, kk))
+                       : MPU(CRESS_RBR, in.rbr, p, kk) * sp[p];
       });
     }
     const int vsp = sc == 0 ? o.vsp_pt : (sc == 1 ? o.vsp_qv : o.vsp_q);
     if (vsp && sponge) {
-      const float d = sc == 1 ? LD(in.sp[sc], q) - MQ(CRESS_QVBR, in.qvbr, q, k) : LD(in.sp[sc], q);
+      const float d = sc == 1 ? LD(in.sp[sc], q) - MQU(CRESS_QVBR, in.qvbr, q, k) : LD(in.sp[sc], q);
       f = f - r * rst * d;
     }
     if (sc == 0) {
       const float dziv25 = 0.25f * c.dziv;
       // level kk of offset qq
       auto a8 = [&](uint32_t qq, int kk) {
-        return (MQ(CRESS_RBR, in.rbr, qq - sk, kk - 1) + MQ(CRESS_RBR, in.rbr, qq, kk)) * LD(in.w, qq) *
-               (MQ(CRESS_PTBR, in.ptbr, qq - sk, kk - 1) - MQ(CRESS_PTBR, in.ptbr, qq, kk)) * dziv25;
+        return (MQU(CRESS_RBR, in.rbr, qq - sk, kk - 1) + MQU(CRESS_RBR, in.rbr, qq, kk)) * LD(in.w, qq) *
+               (MQU(CRESS_PTBR, in.ptbr, qq - sk, kk - 1) - MQU(CRESS_PTBR, in.ptbr, qq, kk)) * dziv25;
       };
       f = f + (a8(q, k) + a8(q + sk, k + 1));
     }
@@ -559,48 +568,83 @@ __global__ void __launch_bounds__(64 * CRESS_FS_BY) k_force_s(Tiles T, Grid g, O
 // ptil = pp + divcv*jcb*div3d over i=1..ni-1, j=1..nj-1, k=1..nk-1) fused
 // with stepuv.f90:288-310's interior update uf += dts*(ufrc+upg)/rst8u,
 // upg = ((jcb*ptil)(i-1) - (jcb*ptil)(i))/dx (pgrad.f90:357-395), and the
-// same for vf.  A 64 x CRESS_PTUV_BY tile computes ptil on itself and on its i-1 column
+// same for vf.  A 64 x blockDim.y tile computes ptil on itself and on its i-1 column
 // and j-1 row into LDS, then updates uf and vf from it.  uf and vf are read
 // from one buffer and written to another, because neighbouring tiles read
 // the old values for their halo ptil; ptil goes to memory for the column
 // kernel's vertical pressure gradient.
+template <bool U>
 __device__ __forceinline__ float ptil_at(const Grid &g, const Opts &o, const Consts &c, const AcPtUv &a, int i,
                                          int j, int k) {
-  const long p = g.at(i, j, k);
-  if (o.divopt == 0) return a.ppf[p];
-  const float s1a = MP(CRESS_RST8U, a.rst8u, p, k) * a.uf[p], s1b = MP(CRESS_RST8U, a.rst8u, p + 1, k) * a.uf[p + 1];
-  const float s2a = MP(CRESS_RST8V, a.rst8v, p, k) * a.vf[p];
-  const float s2b = MP(CRESS_RST8V, a.rst8v, p + g.nx, k) * a.vf[p + g.nx];
-  const float s3a = MP(CRESS_RST8W, a.rst8w, p, k) * a.wc[p];
-  const float s3b = MP(CRESS_RST8W, a.rst8w, p + g.nxy, k + 1) * a.wc[p + g.nxy];
+  const uint32_t q = g.off(i, j, k), sx = 4u * (uint32_t)g.nx, sk = 4u * (uint32_t)g.nxy;
+  if (o.divopt == 0) return LD(a.ppf, q);
+  const float s1a = MQU(CRESS_RST8U, a.rst8u, q, k) * LD(a.uf, q);
+  const float s1b = MQU(CRESS_RST8U, a.rst8u, q + 4, k) * LD(a.uf, q + 4);
+  const float s2a = MQU(CRESS_RST8V, a.rst8v, q, k) * LD(a.vf, q);
+  const float s2b = MQU(CRESS_RST8V, a.rst8v, q + sx, k) * LD(a.vf, q + sx);
+  const float s3a = MQU(CRESS_RST8W, a.rst8w, q, k) * LD(a.wc, q);
+  const float s3b = MQU(CRESS_RST8W, a.rst8w, q + sk, k + 1) * LD(a.wc, q + sk);
   const float div = (s3a - s3b) * c.dziv + ((s1a - s1b) * c.dxiv + (s2a - s2b) * c.dyiv);
-  return a.ppf[p] + a.divcv * MP(CRESS_JCB, a.jcb, p, k) * div;
+  return LD(a.ppf, q) + a.divcv * MQU(CRESS_JCB, a.jcb, q, k) * div;
 }
 
-__global__ void __launch_bounds__(64 * CRESS_PTUV_BY) k_ac_ptuv(Tiles T, Grid g, Opts o, Consts c, AcPtUv a) {
+template <bool U, int SLOTS>
+__global__ void __launch_bounds__(64 * CRESS_PTUV_BYMAX) k_ac_ptuv(Tiles T, Grid g, Opts o, Consts c, AcPtUv a) {
   const uint3 xt = xcd_tile(T);
-  __shared__ float P[CRESS_PTUV_BY + 1][66];   // P[y][x]: j = j0-1+y, i = i0-1+x
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i0 = xt.x * 64, j0 = 1 + xt.y * CRESS_PTUV_BY, k = 1 + xt.z;
+  __shared__ float P[CRESS_PTUV_BYMAX + 1][66];   // P[y][x]: j = j0-1+y, i = i0-1+x
+  const int tx = threadIdx.x, ty = threadIdx.y, by = blockDim.y;
+  const int i0 = xt.x * 64, j0 = 1 + xt.y * by, k = 1 + xt.z;
   const int i = i0 + tx, j = j0 + ty;
   const bool own = i >= 1 && i <= g.ni - 1 && j <= g.nj - 1;
-  const float pown = own ? ptil_at(g, o, c, a, i, j, k) : 0.0f;
-  P[ty + 1][tx + 1] = pown;
-  if (ty == 0) P[0][tx + 1] = (j0 - 1 >= 1 && i >= 1 && i <= g.ni - 1) ? ptil_at(g, o, c, a, i, j0 - 1, k) : 0.0f;
-  if (tx == 0) P[ty + 1][0] = (i0 - 1 >= 1 && j <= g.nj - 1) ? ptil_at(g, o, c, a, i0 - 1, j, k) : 0.0f;
-  __syncthreads();
-  if (!own) return;
-  const long p = g.at(i, j, k);
-  a.ptil[p] = pown;
-  if (k < 2 || k > g.nk - 2 || i < 2 || j < 2) return;
-  const float pc = MP(CRESS_JCB, a.jcb, p, k) * pown;
-  if (j <= g.nj - 2) {
-    const float upg = (MP(CRESS_JCB, a.jcb, p - 1, k) * P[ty + 1][tx] - pc) * c.dxiv;
-    a.ufn[p] = a.uf[p] + a.dts * (a.ufrc[p] + upg) / MP(CRESS_RST8U, a.rst8u, p, k);
+  const uint32_t q = g.off(i, j, k), sx = 4u * (uint32_t)g.nx;
+  // the update's own-cell operands are loaded first: they do not depend on
+  // the shared ptil values
+  const bool upd = own && k >= 2 && k <= g.nk - 2 && i >= 2 && j >= 2;
+  const bool du = upd && j <= g.nj - 2, dv = upd && i <= g.ni - 2;
+  float uf0 = 0.0f, ufr = 0.0f, vf0 = 0.0f, vfr = 0.0f;
+  if (du) {
+    uf0 = LD(a.uf, q);
+    ufr = LD(a.ufrc, q);
   }
-  if (i <= g.ni - 2) {
-    const float vpg = (MP(CRESS_JCB, a.jcb, p - g.nx, k) * P[ty][tx + 1] - pc) * c.dyiv;
-    a.vfn[p] = a.vf[p] + a.dts * (a.vfrc[p] + vpg) / MP(CRESS_RST8V, a.rst8v, p, k);
+  if (dv) {
+    vf0 = LD(a.vf, q);
+    vfr = LD(a.vfrc, q);
+  }
+  float pown;
+  if (SLOTS) {
+    // ptil on the (by+1) x 65 points of the tile with its i-1 column and j-1
+    // row, in slots of consecutive points per thread, all loads issued
+    // without branching on the thread's position
+    const int np = (by + 1) * 65, tid = ty * 64 + tx;
+    for (int e = tid; e < np; e += 64 * by) {
+      const int yy = e / 65, xx = e - 65 * (e / 65);
+      const int ii = i0 - 1 + xx, jj = j0 - 1 + yy;
+      const bool ok = ii >= 1 && ii <= g.ni - 1 && jj >= 1 && jj <= g.nj - 1;
+      P[yy][xx] = ok ? ptil_at<U>(g, o, c, a, ii, jj, k) : 0.0f;
+    }
+    __syncthreads();
+    if (!own) return;
+    pown = P[ty + 1][tx + 1];
+  } else {
+    // each thread its own point; the first row and the first lane also the
+    // halo row and column
+    pown = own ? ptil_at<U>(g, o, c, a, i, j, k) : 0.0f;
+    P[ty + 1][tx + 1] = pown;
+    if (ty == 0) P[0][tx + 1] = (j0 - 1 >= 1 && i >= 1 && i <= g.ni - 1) ? ptil_at<U>(g, o, c, a, i, j0 - 1, k) : 0.0f;
+    if (tx == 0) P[ty + 1][0] = (i0 - 1 >= 1 && j <= g.nj - 1) ? ptil_at<U>(g, o, c, a, i0 - 1, j, k) : 0.0f;
+    __syncthreads();
+    if (!own) return;
+  }
+  ST(a.ptil, q, pown);
+  if (!upd) return;
+  const float pc = MQU(CRESS_JCB, a.jcb, q, k) * pown;
+  if (du) {
+    const float upg = (MQU(CRESS_JCB, a.jcb, q - 4, k) * P[ty + 1][tx] - pc) * c.dxiv;
+    ST(a.ufn, q, uf0 + a.dts * (ufr + upg) / MQU(CRESS_RST8U, a.rst8u, q, k));
+  }
+  if (dv) {
+    const float vpg = (MQU(CRESS_JCB, a.jcb, q - sx, k) * P[ty][tx + 1] - pc) * c.dyiv;
+    ST(a.vfn, q, vf0 + a.dts * (vfr + vpg) / MQU(CRESS_RST8V, a.rst8v, q, k));
   }
 }
 
@@ -619,16 +663,17 @@ __global__ void __launch_bounds__(64 * CRESS_PTUV_BY) k_ac_ptuv(Tiles T, Grid g,
 // next use).  ss' depends on the metrics and dts alone, so when those are
 // horizontally uniform it is one number per level: the workgroup's first
 // lane keeps it in an LDS table instead, and the ptil round trip goes away.
-__global__ void __launch_bounds__(128) k_ac_col(Tiles T, Grid g, Opts o, Consts c, AcCol a) {
-  extern __shared__ float sst[];
+template <bool UNI>
+__global__ void __launch_bounds__(64 * CRESS_COL_BYMAX) k_ac_col(Tiles T, Grid g, Opts o, Consts c, AcCol a) {
   const uint3 xt = xcd_tile(T);
   const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 2 + xt.y * blockDim.y + threadIdx.y;
   const bool act = i >= 2 && i <= g.ni - 2 && j <= g.nj - 2;
-  // the first column of the workgroup's first row, always active
-  const bool lead = threadIdx.y == 0 && i == (xt.x == 0 ? 2 : (int)(xt.x * blockDim.x));
-  const bool uni = MUNI(CRESS_RST8W) && MUNI(CRESS_RCSQ) && MUNI(CRESS_RST) && MUNI(CRESS_RBR) &&
-                   MUNI(CRESS_JCB) && g.nk * 4 <= CRESS_COL_LDS;
+  // UNI: the metrics are horizontally uniform, and the Thomas coefficients
+  // are read from a per-level table the host formed with this kernel's
+  // operations (AcCol::coef)
+  constexpr bool uni = UNI, U = UNI;
+  const float *coef = a.coef;
   const int nk = g.nk;
   // byte offsets (a field is below 4 GiB; checked at context creation)
   const uint32_t sk = 4u * (uint32_t)g.nxy, sx = 4u * (uint32_t)g.nx;
@@ -669,26 +714,41 @@ __global__ void __launch_bounds__(128) k_ac_col(Tiles T, Grid g, Opts o, Consts 
     const float wf_kp1 = n_wf1, u0 = n_u0, u1 = n_u1, v0 = n_v0, v1 = n_v1;
     const float pfrc = n_pf, pt_k = n_pt, pp_k = n_pp, wfrc = n_fw;
     if (k + 1 <= nk - 2) pre1(q + sk);
-    const float rcsq = MQ(CRESS_RCSQ, a.rcsq, q, k), rst = MQ(CRESS_RST, a.rst, q, k);
-    const float jcb = MQ(CRESS_JCB, a.jcb, q, k), rbr = MQ(CRESS_RBR, a.rbr, q, k);
-    const float ju0 = MQ(CRESS_JCB8U, a.jcb8u, q, k) * u0;
-    const float ju1 = MQ(CRESS_JCB8U, a.jcb8u, q + 4, k) * u1;
-    const float jv0 = MQ(CRESS_JCB8V, a.jcb8v, q, k) * v0;
-    const float jv1 = MQ(CRESS_JCB8V, a.jcb8v, q + sx, k) * v1;
+    const float rcsq = MQU(CRESS_RCSQ, a.rcsq, q, k), rst = MQU(CRESS_RST, a.rst, q, k);
+    const float jcb = MQU(CRESS_JCB, a.jcb, q, k), rbr = MQU(CRESS_RBR, a.rbr, q, k);
+    const float ju0 = MQU(CRESS_JCB8U, a.jcb8u, q, k) * u0;
+    const float ju1 = MQU(CRESS_JCB8U, a.jcb8u, q + 4, k) * u1;
+    const float jv0 = MQU(CRESS_JCB8V, a.jcb8v, q, k) * v0;
+    const float jv1 = MQU(CRESS_JCB8V, a.jcb8v, q + sx, k) * v1;
     const float hdiv = rcsq * ((ju0 - ju1) * c.dxiv + (jv0 - jv1) * c.dyiv);
     const float vdiv = rcsq * (wf_k - wf_kp1) * c.dziv;
     const float fp = pfrc + hdiv + c.weic1m * (c.g05 * (wf_k + wf_kp1) * rst + vdiv);
     ST(wc, q, fp);
     const float fpdvj = fp / jcb;
     const float wb = (pp_k * rst + fp * rbr * dtw) / rcsq * c.g05n;
-    const float t1 = c.g05 * rbr / rcsq;
-    const float t2 = c.dziv / jcb;
-    const float fwc = t1 + t2, wcc = t1 - t2;
-    if (k >= 3) {
+    float fwc = 0.0f, wcc = 0.0f;
+    if constexpr (!UNI) {
+      const float t1 = c.g05 * rbr / rcsq;
+      const float t2 = c.dziv / jcb;
+      fwc = t1 + t2;
+      wcc = t1 - t2;
+    }
+    if (k >= 3 && uni) {
       const float wpg = (pt_km1 - pt_k) * c.dziv;
       float fw = wfrc + wpg + (fpdvj_km1 - fpdvj) * dtdzw;
       if (o.buyopt == 1) fw = fw + (wb_km1 + wb);
-      const float rst8w = MQ(CRESS_RST8W, a.rst8w, q, k);
+      const float rst8w = MTAB(CRESS_RST8W, k);
+      float wfn = wf_k + dts * fw / rst8w;
+      const float rr = coef[3 * k], r = coef[3 * k + 1];
+      if (k == 3) wfn = wfn + 0.25f * rr * (0.0f + 0.0f);   // bbcw on flat terrain
+      const float ffn = k == 3 ? wfn * r : (wfn - rr * ff_km1) * r;
+      ST(wf, q, ffn);
+      ff_km1 = ffn;
+    } else if (k >= 3) {
+      const float wpg = (pt_km1 - pt_k) * c.dziv;
+      float fw = wfrc + wpg + (fpdvj_km1 - fpdvj) * dtdzw;
+      if (o.buyopt == 1) fw = fw + (wb_km1 + wb);
+      const float rst8w = MQU(CRESS_RST8W, a.rst8w, q, k);
       float wfn = wf_k + dts * fw / rst8w;
       float rr, ss, tt;
       if (o.buyopt == 0) {
@@ -720,8 +780,7 @@ __global__ void __launch_bounds__(128) k_ac_col(Tiles T, Grid g, Opts o, Consts 
         ssn = tt * r;
         ffn = (wfn - rr * ff_km1) * r;
       }
-      if (!uni) ST(ptil, q - sk, ssn);
-      else if (lead) sst[k] = ssn;
+      ST(ptil, q - sk, ssn);
       ST(wf, q, ffn);
       ssn_km1 = ssn;
       ff_km1 = ffn;
@@ -735,14 +794,13 @@ __global__ void __launch_bounds__(128) k_ac_col(Tiles T, Grid g, Opts o, Consts 
     pt_km1 = pt_k;
     wf_k = wf_kp1;
   }
-  if (uni) __syncthreads();   // the ss' table is complete
   if (!act) return;
   // pass 2 (downward, k=nk-2..2): back substitution (gaussel.f90:248-260)
   // fused with diverpiv + advbspi 'fore' and steppi, whose per-level update
   // of ppf needs only w(k), w(k+1) and fp(k), in any order of k.  The
   // levels 2 and nk-1 take w after vbcw; wc gets its final value.
-  const float wc2 = wc_value(g, o, wf, a.jcb8w, i, j, 2);
-  const float wcm = wc_value(g, o, wf, a.jcb8w, i, j, nk - 1);
+  const float wc2 = wc_value<U>(g, o, wf, a.jcb8w, i, j, 2);
+  const float wcm = wc_value<U>(g, o, wf, a.jcb8w, i, j, nk - 1);
   const float w2 = o.bbc == 2 ? -0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f)) : AT(wf, i, j, 2);
   const float wm = o.tbc == 2 ? 0.0f : AT(wf, i, j, nk - 1);
   float wkp1 = wm;
@@ -754,7 +812,7 @@ __global__ void __launch_bounds__(128) k_ac_col(Tiles T, Grid g, Opts o, Consts 
   auto pre2 = [&](int kk, uint32_t qq) {
     if (kk <= nk - 3 && kk >= 3) {
       m_wf = LD(wf, qq);
-      m_ss = uni ? sst[kk] : LD(ptil, qq - sk);
+      m_ss = uni ? coef[3 * kk + 2] : LD(ptil, qq - sk);
     }
     m_fp = LD(wc, qq);
     m_pp = LD(ppf, qq);
@@ -769,26 +827,26 @@ __global__ void __launch_bounds__(128) k_ac_col(Tiles T, Grid g, Opts o, Consts 
     } else if (k == 2) {
       wk = w2;
     }
-    const float vdiv = MQ(CRESS_RCSQ, a.rcsq, q, k) * (wk - wkp1) * c.dziv;
-    const float fp = fp_k + c.weicoe * (c.g05 * (wk + wkp1) * MQ(CRESS_RST, a.rst, q, k) + vdiv);
-    ST(ppf, q, pp_k + dts * fp / MQ(CRESS_JCB, a.jcb, q, k));
-    ST(wc, q, k == 2 ? wc2 : (o.sthopt == 0 ? wk : wk / MQ(CRESS_JCB8W, a.jcb8w, q, k)));
+    const float vdiv = MQU(CRESS_RCSQ, a.rcsq, q, k) * (wk - wkp1) * c.dziv;
+    const float fp = fp_k + c.weicoe * (c.g05 * (wk + wkp1) * MQU(CRESS_RST, a.rst, q, k) + vdiv);
+    ST(ppf, q, pp_k + dts * fp / MQU(CRESS_JCB, a.jcb, q, k));
+    ST(wc, q, k == 2 ? wc2 : (o.sthopt == 0 ? wk : wk / MQU(CRESS_JCB8W, a.jcb8w, q, k)));
     wkp1 = wk;
   }
   // phy2cnt+vbcwc at the outer rows and vbcw, from the solved column
   AT(wc, i, j, nk - 1) = wcm;
-  AT(wc, i, j, 1) = wc_value(g, o, wf, a.jcb8w, i, j, 1);
-  AT(wc, i, j, nk) = wc_value(g, o, wf, a.jcb8w, i, j, nk);
+  AT(wc, i, j, 1) = wc_value<U>(g, o, wf, a.jcb8w, i, j, 1);
+  AT(wc, i, j, nk) = wc_value<U>(g, o, wf, a.jcb8w, i, j, nk);
   if (o.bbc == 2) {
     const float z = 0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
-    AT(wf, i, j, 1) = -MT(CRESS_JCB8W, a.jcb8w, i, j, 3) * wc_raw(g, o.sthopt, wf, a.jcb8w, i, j, 3) - z;
+    AT(wf, i, j, 1) = -MT(CRESS_JCB8W, a.jcb8w, i, j, 3) * wc_raw<U>(g, o.sthopt, wf, a.jcb8w, i, j, 3) - z;
     AT(wf, i, j, 2) = w2;
   } else {
     AT(wf, i, j, 1) = AT(wf, i, j, 2);
   }
   if (o.tbc == 2) {
     const float z = 0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
-    AT(wf, i, j, nk) = -MT(CRESS_JCB8W, a.jcb8w, i, j, nk - 2) * wc_raw(g, o.sthopt, wf, a.jcb8w, i, j, nk - 2) - z;
+    AT(wf, i, j, nk) = -MT(CRESS_JCB8W, a.jcb8w, i, j, nk - 2) * wc_raw<U>(g, o.sthopt, wf, a.jcb8w, i, j, nk - 2) - z;
     AT(wf, i, j, nk - 1) = 0.0f;
   } else {
     AT(wf, i, j, nk) = AT(wf, i, j, nk - 1);
@@ -817,7 +875,7 @@ __global__ void k_ac_ring(Grid g, Opts o, const float *jcb8w, float *wf, float *
   else if (t < 2 * nxr + (nyr - 2)) { i = 1; j = 2 + (t - 2 * nxr); }
   else { i = g.ni - 1; j = 2 + (t - 2 * nxr - (nyr - 2)); }
   const int nk = g.nk;
-  auto raw = [&](int kk) { return wc_raw(g, o.sthopt, wf, jcb8w, i, j, kk); };
+  auto raw = [&](int kk) { return wc_raw<false>(g, o.sthopt, wf, jcb8w, i, j, kk); };
   const float z = 0.25f * ((0.0f + 0.0f) + (0.0f + 0.0f));
   if (k == 1) {
     if (o.bbc == 2) {
@@ -1050,11 +1108,27 @@ __global__ void k_bc8(Grid g, int kind, int lo, int hi, int kmax, float *var) {
 }
 
 // The forcing kernels with and without the smoothing compiled in.
-template __global__ void k_force_uv<false>(Tiles, Grid, Opts, Consts, ForceIn, float *, float *);
-template __global__ void k_force_uv<true>(Tiles, Grid, Opts, Consts, ForceIn, float *, float *);
-template __global__ void k_force_w<false>(Tiles, Grid, Opts, Consts, ForceIn, float *);
-template __global__ void k_force_w<true>(Tiles, Grid, Opts, Consts, ForceIn, float *);
-template __global__ void k_force_s<false>(Tiles, Grid, Opts, Consts, ScalarIn, int);
-template __global__ void k_force_s<true>(Tiles, Grid, Opts, Consts, ScalarIn, int);
+template __global__ void k_ac_ptuv<false, 0>(Tiles, Grid, Opts, Consts, AcPtUv);
+template __global__ void k_ac_ptuv<true, 0>(Tiles, Grid, Opts, Consts, AcPtUv);
+template __global__ void k_ac_ptuv<false, 1>(Tiles, Grid, Opts, Consts, AcPtUv);
+template __global__ void k_ac_ptuv<true, 1>(Tiles, Grid, Opts, Consts, AcPtUv);
+template __global__ void k_ac_col<false>(Tiles, Grid, Opts, Consts, AcCol);
+template __global__ void k_ac_col<true>(Tiles, Grid, Opts, Consts, AcCol);
+#define CRESS_INST_U(U)                                                                                      \
+  template __global__ void k_phy2cnt<U>(Tiles, Grid, Opts, const float *, const float *, float *);           \
+  template __global__ void k_prep<U>(Tiles, Grid, Opts, const float *, const float *, const float *,         \
+                                     const float *, const float *, const float *, const float *, float *,   \
+                                     float *, float *, float *);                                             \
+  template __global__ void k_force_p<U>(Tiles, Grid, Opts, Consts, const float *, const float *,             \
+                                        const float *, const float *, const float *, const float *,          \
+                                        const float *, const float *, const float *, const float *, float *);\
+  template __global__ void k_force_uv<false, U>(Tiles, Grid, Opts, Consts, ForceIn, float *, float *);       \
+  template __global__ void k_force_uv<true, U>(Tiles, Grid, Opts, Consts, ForceIn, float *, float *);        \
+  template __global__ void k_force_w<false, U>(Tiles, Grid, Opts, Consts, ForceIn, float *);                 \
+  template __global__ void k_force_w<true, U>(Tiles, Grid, Opts, Consts, ForceIn, float *);                  \
+  template __global__ void k_force_s<false, U>(Tiles, Grid, Opts, Consts, ScalarIn, int);                    \
+  template __global__ void k_force_s<true, U>(Tiles, Grid, Opts, Consts, ScalarIn, int);
+CRESS_INST_U(false)
+CRESS_INST_U(true)
 
 }  // namespace cress

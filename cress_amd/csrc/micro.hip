@@ -15,6 +15,7 @@
 // Rates follow the reference's formulas and operation order; options fixed:
 // flqcqi_opt=1, evapor_opt=1, nuc2nd_opt=1, advopt<=3.
 #include "micro.h"
+#include "xmath.h"
 
 namespace cress {
 
@@ -46,13 +47,12 @@ constexpr float ucqcst = 0.1f, uiqcst = 0.0001f, uincst = 0.0001f;
 __constant__ float c_ckoe[41];
 __constant__ float c_pkoe[41];
 
-// exp and log in binary32: the device library's expf/logf (ROCm OCML,
-// at most 1 ulp from the correctly rounded result away from denormals).  The
-// reference's FP32 build calls glibc's expf/logf (correctly rounded in
-// practice), so single results may differ by 1 ulp; the fields stay within
-// 1e-5 of the FP32 oracle after a step.
-__device__ __forceinline__ float ex(float x) { return expf(x); }
-__device__ __forceinline__ float lg(float x) { return logf(x); }
+// exp and log in binary32 (xmath.h: float-float internals, the correctly
+// rounded result but for arguments within ~2^-40 of a rounding boundary).
+// The reference's FP32 build calls glibc's expf/logf, which round within
+// about 0.5 ulp; the two agree on all but ~1e-4 of arguments.
+__device__ __forceinline__ float ex(float x) { return xexpf(x); }
+__device__ __forceinline__ float lg(float x) { return xlogf(x); }
 
 #define LD(arr, off) (*(const float *)((const char *)(arr) + (off)))
 #define ST(arr, off, v) (*(float *)((char *)(arr) + (off)) = (v))
@@ -82,10 +82,6 @@ constexpr int kFallPipe = 8;
 #endif
 
 // ---------------------------------------------------------------------------
-// The source kernel's results are compared with the reference to a relative
-// bar, not bit for bit (binary32 exp/log above), so a*b+c may fuse into one
-// fma here; every other kernel of this file keeps -ffp-contract=off.
-#pragma clang fp contract(fast)
 __global__ void __launch_bounds__(256) CRESS_MICRO_ATTR k_micro_src(Tiles T, Grid g, MicroK K, MicroArgs a) {
   const uint3 xt = xcd_tile(T);
   const int i = xt.x * blockDim.x + threadIdx.x;
@@ -589,7 +585,6 @@ __global__ void __launch_bounds__(256) CRESS_MICRO_ATTR k_micro_src(Tiles T, Gri
   }
   if (tid == 0) atomicMin(a.dtpmin, __float_as_uint(red[0]));
 }
-#pragma clang fp contract(off)
 
 // ---------------------------------------------------------------------------
 // The network's cells for this step (MlpSelectArgs in micro.h).  Same cells

@@ -65,6 +65,15 @@ __device__ __forceinline__ float mlp_gelu(float z) {
   return 0.5f * z * (1.0f + t);
 }
 
+// The same GELU with the hardware exp2 and reciprocal (about 1 ulp each) in
+// place of the library exp and the IEEE division: for the inference
+// kernel, whose activations are rounded to BF16 anyway.
+// 0.5 z (1 + tanh u) = z / (1 + e^{-2u}), e^{-2u} = 2^{-2u log2(e)}.
+__device__ __forceinline__ float mlp_gelu_fast(float z) {
+  const float u = z * __builtin_fmaf(0.7978845608f * 0.044715f, z * z, 0.7978845608f);
+  return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-2.8853900818f * u));
+}
+
 __device__ __forceinline__ float mlp_dgelu(float z) {
   const float u = 0.7978845608f * (z + 0.044715f * z * z * z);
   const float t = 1.0f - 2.0f / (1.0f + __expf(2.0f * u));
@@ -106,6 +115,9 @@ struct GemmArgs {
 
 __global__ void k_gemm_bf16(GemmArgs g);
 __global__ void k_mlp_infer(MlpNet net, MlpStep s, long n);
+// workgroups (of MLP_INFER_THREADS threads) of k_mlp_infer for up to n cells
+unsigned mlp_infer_blocks(long n);
+constexpr int MLP_INFER_THREADS = 512;
 __global__ void k_mlp_gather(const float *in, const float *tgt, long nsamp, const float *norm, unsigned seed,
                              int identity, int batch, unsigned short *x, float *t);
 __global__ void k_mlp_loss(const float *y, const float *t, int batch, float scale, float *dy, unsigned short *dyb,

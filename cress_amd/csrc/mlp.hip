@@ -23,6 +23,23 @@ __device__ __forceinline__ f32x4_t mfma16(bf16x4_t a, bf16x4_t b, f32x4_t c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// Four floats to a BF16 fragment, round to nearest even (v_cvt_pk_bf16_f32).
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ bf16x4_t pack4(float a, float b, float c, float d) {
+  bf16x2_t lo, hi;
+  lo[0] = (__bf16)a;
+  lo[1] = (__bf16)b;
+  hi[0] = (__bf16)c;
+  hi[1] = (__bf16)d;
+  const unsigned ul = __builtin_bit_cast(unsigned, lo), uh = __builtin_bit_cast(unsigned, hi);
+  bf16x4_t o;
+  o[0] = (short)(ul & 0xffffu);
+  o[1] = (short)(ul >> 16);
+  o[2] = (short)(uh & 0xffffu);
+  o[3] = (short)(uh >> 16);
+  return o;
+}
+
 __device__ __forceinline__ bf16x4_t ld4(const unsigned short *p) {
   // four consecutive BF16 values, 8-byte aligned
   const uint2 u = *(const uint2 *)p;
@@ -106,34 +123,38 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
 }
 
 // ---------------------------------------------------------------------------
-// Fused inference over the step's list of network cells.  Each wavefront
-// takes 16 cells at a time (grid-stride); the weights sit in LDS as BF16,
-// transposed to [out][in] with padded rows so the A fragments are 8-byte
-// reads without bank conflicts.
+// Fused inference over the step's list of network cells.  A workgroup of
+// 16 wavefronts shares one copy of the weights in LDS (BF16, transposed to
+// [out][in] with padded rows so the A fragments are 8-byte reads without
+// bank conflicts); each wavefront takes MLP_G groups of 16 cells at a time
+// (grid-stride), so every weight fragment read from LDS feeds MLP_G matrix
+// operations, and the activations never leave registers.
 constexpr int IW1 = MLP_INP + 8, IWH = MLP_H + 8;
+constexpr int MLP_G = 2;            // 16-cell groups per wavefront pass
+constexpr int MLP_WAVES = MLP_INFER_THREADS / 64;   // wavefronts per workgroup
 
-__global__ void __launch_bounds__(256) k_mlp_infer(MlpNet net, MlpStep s, long n) {
+__global__ void __launch_bounds__(64 * MLP_WAVES) k_mlp_infer(MlpNet net, MlpStep s, long n) {
   __shared__ __align__(16) unsigned short w1[MLP_H * IW1];
   __shared__ __align__(16) unsigned short w2[MLP_H * IWH];
   __shared__ __align__(16) unsigned short w3[MLP_H * IWH];
   __shared__ __align__(16) unsigned short w4[MLP_OUTP * IWH];
   __shared__ float bias[3 * MLP_H + MLP_OUTP];
   __shared__ float omu[MLP_OUTP], osd[MLP_OUTP];
-  const int tid = threadIdx.x;
-  for (int e = tid; e < MLP_INP * MLP_H; e += 256) {      // W1[in][out] -> w1[out][in]
+  const int tid = threadIdx.x, nth = 64 * MLP_WAVES;
+  for (int e = tid; e < MLP_INP * MLP_H; e += nth) {      // W1[in][out] -> w1[out][in]
     const int i = e / MLP_H, o = e % MLP_H;
     w1[o * IW1 + i] = net.w[MLP_OW1 + e];
   }
-  for (int e = tid; e < MLP_H * MLP_H; e += 256) {
+  for (int e = tid; e < MLP_H * MLP_H; e += nth) {
     const int i = e / MLP_H, o = e % MLP_H;
     w2[o * IWH + i] = net.w[MLP_OW2 + e];
     w3[o * IWH + i] = net.w[MLP_OW3 + e];
   }
-  for (int e = tid; e < MLP_H * MLP_OUTP; e += 256) {
+  for (int e = tid; e < MLP_H * MLP_OUTP; e += nth) {
     const int i = e / MLP_OUTP, o = e % MLP_OUTP;
     w4[o * IWH + i] = net.w[MLP_OW4 + e];
   }
-  for (int e = tid; e < MLP_H; e += 256) {
+  for (int e = tid; e < MLP_H; e += nth) {
     bias[e] = net.p[MLP_OB1 + e];
     bias[MLP_H + e] = net.p[MLP_OB2 + e];
     bias[2 * MLP_H + e] = net.p[MLP_OB3 + e];
@@ -145,41 +166,60 @@ __global__ void __launch_bounds__(256) k_mlp_infer(MlpNet net, MlpStep s, long n
   }
   __syncthreads();
   const unsigned cnt = *s.count;
-  const unsigned ngroups = (cnt + 15u) / 16u;
+  const unsigned npass = (cnt + 16u * MLP_G - 1u) / (16u * MLP_G);
   const int wave = tid >> 6, lane = tid & 63, r = lane & 15, q = lane >> 4;
-  const unsigned nwaves = gridDim.x * 4u;
-  for (unsigned grp = blockIdx.x * 4u + (unsigned)wave; grp < ngroups; grp += nwaves) {
-    const unsigned smp = grp * 16u + (unsigned)r;
-    const bool valid = smp < cnt;
-    bf16x4_t xb = bf16x4_t{0, 0, 0, 0};
-    if (valid) xb = ld4(s.x + (long)smp * MLP_INP + 4 * q);
-    bf16x4_t h[8], h2[8];
-    f32x4_t acc[8];
-    // layer 1: 128 x 16 by 16 x 16 samples
-    for (int mb = 0; mb < 8; mb++) {
-      acc[mb] = mfma16(ld4(&w1[(mb * 16 + r) * IW1 + 4 * q]), xb, f32x4_t{0.f, 0.f, 0.f, 0.f});
+  const unsigned nwaves = gridDim.x * (unsigned)MLP_WAVES;
+  for (unsigned pass = blockIdx.x * (unsigned)MLP_WAVES + (unsigned)wave; pass < npass; pass += nwaves) {
+    bf16x4_t h[MLP_G][8];
+    f32x4_t acc[MLP_G][8];
+    // layer 1: 128 x 16 weights by 16 x 16 cells per group
+    for (int gi = 0; gi < MLP_G; gi++) {
+      const unsigned smp = (pass * MLP_G + gi) * 16u + (unsigned)r;
+      bf16x4_t xb = bf16x4_t{0, 0, 0, 0};
+      if (smp < cnt) xb = ld4(s.x + (long)smp * MLP_INP + 4 * q);
+      for (int mb = 0; mb < 8; mb++)
+        acc[gi][mb] = mfma16(ld4(&w1[(mb * 16 + r) * IW1 + 4 * q]), xb, f32x4_t{0.f, 0.f, 0.f, 0.f});
     }
-    for (int mb = 0; mb < 8; mb++)
-      for (int v = 0; v < 4; v++) h[mb][v] = (short)f2bf(mlp_gelu(acc[mb][v] + bias[mb * 16 + 4 * q + v]));
-    // layers 2 and 3: 128 x 128 by 128 x 16
+    for (int gi = 0; gi < MLP_G; gi++)
+      for (int mb = 0; mb < 8; mb++) {
+        const float *bq = bias + mb * 16 + 4 * q;
+        h[gi][mb] = pack4(mlp_gelu_fast(acc[gi][mb][0] + bq[0]), mlp_gelu_fast(acc[gi][mb][1] + bq[1]),
+                          mlp_gelu_fast(acc[gi][mb][2] + bq[2]), mlp_gelu_fast(acc[gi][mb][3] + bq[3]));
+      }
+    // layers 2 and 3: 128 x 128 weights by 128 x 16 cells per group; each
+    // weight fragment serves every group
     for (int layer = 0; layer < 2; layer++) {
       const unsigned short *w = layer == 0 ? w2 : w3;
       const float *bb = bias + (layer + 1) * MLP_H;
-      for (int mb = 0; mb < 8; mb++) acc[mb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+      for (int gi = 0; gi < MLP_G; gi++)
+        for (int mb = 0; mb < 8; mb++) acc[gi][mb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
       for (int kb = 0; kb < 8; kb++)
-        for (int mb = 0; mb < 8; mb++) acc[mb] = mfma16(ld4(&w[(mb * 16 + r) * IWH + kb * 16 + 4 * q]), h[kb], acc[mb]);
-      for (int mb = 0; mb < 8; mb++)
-        for (int v = 0; v < 4; v++) h2[mb][v] = (short)f2bf(mlp_gelu(acc[mb][v] + bb[mb * 16 + 4 * q + v]));
-      for (int mb = 0; mb < 8; mb++) h[mb] = h2[mb];
+        for (int mb = 0; mb < 8; mb++) {
+          const bf16x4_t a = ld4(&w[(mb * 16 + r) * IWH + kb * 16 + 4 * q]);
+          for (int gi = 0; gi < MLP_G; gi++) acc[gi][mb] = mfma16(a, h[gi][kb], acc[gi][mb]);
+        }
+      for (int gi = 0; gi < MLP_G; gi++)
+        for (int mb = 0; mb < 8; mb++) {
+          const float *bq = bb + mb * 16 + 4 * q;
+          h[gi][mb] = pack4(mlp_gelu_fast(acc[gi][mb][0] + bq[0]), mlp_gelu_fast(acc[gi][mb][1] + bq[1]),
+                            mlp_gelu_fast(acc[gi][mb][2] + bq[2]), mlp_gelu_fast(acc[gi][mb][3] + bq[3]));
+        }
     }
-    // layer 4: 16 x 128 by 128 x 16; rows 0..6 are the outputs
-    f32x4_t y = f32x4_t{0.f, 0.f, 0.f, 0.f};
-    for (int kb = 0; kb < 8; kb++) y = mfma16(ld4(&w4[r * IWH + kb * 16 + 4 * q]), h[kb], y);
-    if (valid && q < 2) {
-      const long cell = s.list[smp];
-      for (int v = 0; v < 4; v++) {
-        const int o = 4 * q + v;
-        if (o < MLP_NOUT) s.out[(long)o * n + cell] = (y[v] + bias[3 * MLP_H + o]) * osd[o] + omu[o];
+    // layer 4: 16 x 128 weights by 128 x 16 cells; rows 0..6 are the outputs
+    f32x4_t y[MLP_G];
+    for (int gi = 0; gi < MLP_G; gi++) y[gi] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    for (int kb = 0; kb < 8; kb++) {
+      const bf16x4_t a = ld4(&w4[r * IWH + kb * 16 + 4 * q]);
+      for (int gi = 0; gi < MLP_G; gi++) y[gi] = mfma16(a, h[gi][kb], y[gi]);
+    }
+    for (int gi = 0; gi < MLP_G; gi++) {
+      const unsigned smp = (pass * MLP_G + gi) * 16u + (unsigned)r;
+      if (smp < cnt && q < 2) {
+        const long cell = s.list[smp];
+        for (int v = 0; v < 4; v++) {
+          const int o = 4 * q + v;
+          if (o < MLP_NOUT) s.out[(long)o * n + cell] = (y[gi][v] + bias[3 * MLP_H + o]) * osd[o] + omu[o];
+        }
       }
     }
   }
@@ -303,6 +343,14 @@ __global__ void k_mlp_stats(const float *a, long rows, int cols, double *sum, do
 
 __global__ void k_u32_set(unsigned *p, unsigned v) {
   if (threadIdx.x == 0) *p = v;
+}
+
+// Workgroups of k_mlp_infer for up to n cells: one per CU (its LDS holds one
+// copy of the weights) up to the chip, fewer when n is small.
+unsigned mlp_infer_blocks(long n) {
+  const long passes = (n + 16L * MLP_G - 1) / (16L * MLP_G);
+  const long b = (passes + MLP_WAVES - 1) / MLP_WAVES;
+  return (unsigned)(b < 1 ? 1 : (b > 256 ? 256 : b));
 }
 
 }  // namespace cress
@@ -672,9 +720,7 @@ int cress_mlp_forward(cress_mlp *m, const float *in, long n, float *out) {
   if (!rc) {
     MlpNet net{m->w, m->p, m->norm};
     MlpStep s{nullptr, list, m->X, cnt, o};
-    const unsigned groups = (unsigned)((n + 15) / 16);
-    const unsigned blocks = groups / 4 + 1 < 1024u ? groups / 4 + 1 : 1024u;
-    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, m->st, net, s, n);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(mlp_infer_blocks(n)), dim3(64 * MLP_WAVES), 0, m->st, net, s, n);
     std::vector<float> t((size_t)n * MLP_NOUT);
     if (hipStreamSynchronize(m->st) != hipSuccess ||
         hipMemcpy(t.data(), o, (size_t)n * MLP_NOUT * 4, hipMemcpyDeviceToHost) != hipSuccess)

@@ -2,25 +2,27 @@
 // Each writes the index ranges the reference writes; the arithmetic keeps
 // Fortran's left-to-right order term by term, so with -ffp-contract=off
 // the tensor, its square and dry / cloud-free N^2 are the reference's
-// bits.  exp and log are binary32 OCML calls (N^2 in cloudy air, the
-// length scale ln0 = (ds^3 jcb)^(1/3) of the eddy viscosity, taken as
-// exp(log(x)/3) as the reference writes it): within 1e-5 of the reference's
-// FP32 libm results.
+// bits.  exp and log are binary32 with float-float internals (xmath.h; N^2
+// in cloudy air, the length scale ln0 = (ds^3 jcb)^(1/3) of the eddy
+// viscosity, taken as exp(log(x)/3) as the reference writes it): within
+// 1e-5 of the reference's FP32 libm results.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
 #include "common.h"
 #include "dyn.h"
 #include "turb.h"
+#include "xmath.h"
 
 namespace cress {
 
 #define AT(a, i, j, k) a[g.at(i, j, k)]
 
-// exp and log in binary32 (ROCm OCML expf/logf, at most 1 ulp from the
-// correctly rounded result); the reference's FP32 build uses glibc's.
-__device__ __forceinline__ float expd(float x) { return expf(x); }
-__device__ __forceinline__ float logd(float x) { return logf(x); }
+// exp and log in binary32 (xmath.h: the correctly rounded result but for
+// arguments within ~2^-40 of a rounding boundary); the reference's FP32
+// build uses glibc's, which agrees on all but ~1e-4 of arguments.
+__device__ __forceinline__ float expd(float x) { return xexpf(x); }
+__device__ __forceinline__ float logd(float x) { return xlogf(x); }
 
 // comphy.f90:49-130, commath.f90:51,177
 #define TB_G 9.8f

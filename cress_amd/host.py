@@ -27,6 +27,14 @@ FID = {name: i for i, name in enumerate(BASE_FIELDS)}
 PROG_FIELDS = ["u", "v", "w", "pp", "ptp", "qv", "qc", "qr", "qi", "qs", "qg", "nci"]
 FID.update({name: 32 + i for i, name in enumerate(PROG_FIELDS)})
 FID.update({"fc": 64, "rbct": 65, "prwtr": 66, "price": 67})
+# cress_diag's result order (include/cress_hip.h)
+DIAG_NAMES = ["vapour", "liquid", "ice", "precip", "dry", "ke", "ie", "pe", "theta",
+              "wmax", "wmin", "ptpmax", "ptpmin"]
+NDIAG = len(DIAG_NAMES)
+# launch shapes (cress_autotune / cress_set_launch order)
+LAUNCH_NAMES = ["ptuv_by", "ptuv_slots", "ptuv_order", "col_by", "col_lds", "fs_by", "fs_order", "ff_by",
+                "ff_order"]
+NLAUNCH = len(LAUNCH_NAMES)
 
 ERR = {0: "ok", -1: "invalid argument", -2: "no usable GPU", -3: "HIP runtime error",
        -4: "device out of memory", -5: "field id or level out of range",
@@ -208,6 +216,37 @@ class Solver:
         """mode 0: srcblk; 1: srcblk, recording training pairs into mlp;
         2: the network for the active cells inside its envelope."""
         check(self.L.cress_ctx_set_mlp(self.ctx, mlp.m if mlp is not None else None, mode), "cress_ctx_set_mlp")
+
+    def diag(self, level=1):
+        """FP64 mass and energy budgets of a level (cress_diag): a dict of
+        vapour, liquid, ice, precip (kg), dry (kg), ke, ie, pe (J), theta
+        (sum rst theta' dV), wmax, wmin, ptpmax, ptpmin."""
+        buf = (D * NDIAG)()
+        check(self.L.cress_diag(self.ctx, level, buf), "cress_diag")
+        return dict(zip(DIAG_NAMES, list(buf)))
+
+    def autotune(self):
+        """Time every candidate launch shape on this grid and keep the
+        fastest (cress_autotune); returns the chosen shapes."""
+        buf = (C.c_int * NLAUNCH)()
+        check(self.L.cress_autotune(self.ctx, buf, NLAUNCH), "cress_autotune")
+        return dict(zip(LAUNCH_NAMES, list(buf)))
+
+    def set_launch(self, shapes):
+        """Set the launch shapes (a dict with LAUNCH_NAMES keys)."""
+        buf = (C.c_int * NLAUNCH)(*[int(shapes[k]) for k in LAUNCH_NAMES])
+        check(self.L.cress_set_launch(self.ctx, buf, NLAUNCH), "cress_set_launch")
+
+    def get_launch(self):
+        buf = (C.c_int * NLAUNCH)()
+        check(self.L.cress_get_launch(self.ctx, buf, NLAUNCH), "cress_get_launch")
+        return dict(zip(LAUNCH_NAMES, list(buf)))
+
+    def mlp_cells(self):
+        """Cells the network handled in the last large step (mode 2)."""
+        c = C.c_long()
+        check(self.L.cress_ctx_mlp_cells(self.ctx, C.byref(c)), "cress_ctx_mlp_cells")
+        return c.value
 
     def stage_times(self):
         buf = (D * 8)()

@@ -139,6 +139,40 @@ const char *cress_kernel_name(int id);
 int cress_kernel_times(cress_ctx *ctx, double *total_ms, long *launches, int n);
 int cress_reset_timing(cress_ctx *ctx);
 
+/* Launch shapes of the tunable kernel families, as 9 ints: fused
+ * ptil/stepuv rows per workgroup (4, 8, 16), halo layout (0 one point per
+ * thread, 1 slots), tile order (1 x-fastest, 2 z-fastest); column kernel
+ * rows (1, 2, 4) and LDS bytes per workgroup (caps occupancy); scalar
+ * forcing rows and order; u/v, w, p' forcing rows and order.  No
+ * counterpart in the reference (tile shapes are a GPU matter).  Results do
+ * not depend on the shapes.  cress_autotune times every candidate on the
+ * context's own grid and keeps the fastest (cached per process by grid and
+ * options); it writes only scratch arrays and the future level, so it may
+ * run between any two large steps, with timing off.  shape may be NULL. */
+#define CRESS_NLAUNCH 9
+int cress_autotune(cress_ctx *ctx, int *shape, int n);
+int cress_set_launch(cress_ctx *ctx, const int *shape, int n);
+int cress_get_launch(const cress_ctx *ctx, int *shape, int n);
+
+/* Mass and energy budgets of one level (0 past, 1 present, 2 future),
+ * reduced over the interior (i,j,k = 2..n-2) in binary64 on the device, in a
+ * fixed order (repeatable bit for bit).  The reference monitors extremes
+ * (getmxn.f90:173-232) and accumulates the surface precipitation
+ * (upwqp.f90:199-202); out gets CRESS_NDIAG values:
+ *   0 vapour mass, sum rst qv dV (kg; rst = rho_bar J, dV = dx dy dz)
+ *   1 liquid water mass (qc + qr)          2 ice mass (qi + qs + qg)
+ *   3 accumulated surface precipitation over the interior columns, as
+ *     water mass (kg): sum of the accumulations (m) x rhow x dx dy
+ *   4 dry-air mass, sum J rho_d dV with rho_d = p eps/((eps + qv) Rd T)
+ *   5 kinetic energy, sum J rho_d (u^2+v^2+w^2)/2 dV (velocities averaged to
+ *     the scalar point)            6 internal energy, sum J rho_d cv T dV
+ *   7 potential energy, sum J rho_d g z dV (z of the scalar levels over flat
+ *     terrain; NaN when jcb varies horizontally)
+ *   8 sum rst theta' dV            9, 10 max and min w
+ *   11, 12 max and min theta' */
+#define CRESS_NDIAG 13
+int cress_diag(cress_ctx *ctx, int level, double *out);
+
 /* ---- Ensemble: contexts of any sizes stepped together ------------------
  * The reference runs one model per job; these have no counterpart there.
  * They batch independent contexts (members) on one GPU: a cycle of large
@@ -163,6 +197,7 @@ int cress_ens_destroy(cress_ens *e);
  * SrcOrig/gaussel.f90:72-73, impopt=1 path.  ss and ff are overwritten. */
 int cress_gaussel(int istr, int iend, int jstr, int jend, int kstr, int kend,
                   int ni, int nj, int nk, float *rr, float *ss, float *tt, float *ff);
+
 
 /* s_advs(fpadvopt,...,ni,nj,nk,rstxu,rstxv,rstxwc,s,sfrc,vadv,tmp1,tmp2,tmp3)
  * SrcOrig/advs.f90:75-77.  Writes sfrc over the interior. */
@@ -258,6 +293,8 @@ int cress_mlp_train(cress_mlp *m, int steps, int batch, float lr, unsigned seed,
  * cell into m; 2: the network for active cells inside its envelope, srcblk
  * for the other active cells, nothing where no process is active. */
 int cress_ctx_set_mlp(cress_ctx *x, cress_mlp *m, int mode);
+/* Cells the network handled in the last large step (mode 2), else 0. */
+int cress_ctx_mlp_cells(cress_ctx *x, long *cells);
 
 /* The first four calls of s_turbdrv (SrcOrig/turbdrv.f90:245-276) with
  * tubopt=2 (1.5-order TKE closure), isoopt=1, trnopt=0, mfcopt=0, sfcopt=0:

@@ -6,7 +6,10 @@
   oracle/_ref/libcressref32.so -- the reference's own routines (build_ref.py),
                                   only where /root/reference is present;
   oracle/_ref/libcressrefmpi32.so -- likewise the routines that reach its
-                                  halo-exchange layer, with the image's MPICH.
+                                  halo-exchange layer, with the image's MPICH;
+  oracle/_ref/libcressrefdrop32.so -- the same with the Fortran drop-in
+                                  modules in place of four reference modules,
+                                  their C calls answered by the oracle.
 
 Plain gcc, -O2, no -march (x86-64 baseline has no FMA, and -ffp-contract=off
 keeps it that way), so the FP32 build rounds like the reference's FP32 build.
@@ -51,6 +54,7 @@ def build(with_ref=True):
         import build_ref
         build_ref.build(32)
         build_ref.build_mpi(32)
+        build_ref.build_dropin()
     return libs
 
 

@@ -222,6 +222,53 @@ def build_mpi(prec=32, jobs=8):
     return lib
 
 
+# Drop-in modules run inside the reference's own solver on the CPU: the
+# modules of cress_amd/fortran/dropin/ replace the reference's sources of
+# the same modules, and oracle/dropin_cpu.c answers their tier-2 C calls
+# with the oracle (liboracle32.so).  Built from the same objects as
+# build_mpi(); the replaced reference objects are left out of the link.
+DROPINS = ("gaussel", "advs", "advuvw", "advp")
+
+
+def build_dropin(names=DROPINS, jobs=8):
+    if not os.path.isdir(os.path.join(REF, "SrcOrig")) or not have_mpi():
+        print("reference sources or MPICH absent; oracle/_ref (drop-in part) not built")
+        return None
+    base = build_mpi(32, jobs)
+    if base is None:
+        return None
+    files, mod2file, uses = scan_sources(prefer_solver=True)
+    objm = os.path.join(OUT, "objmpi32")
+    objd = os.path.join(OUT, "objdrop32")
+    os.makedirs(objd, exist_ok=True)
+    repo = os.path.dirname(HERE)
+    fdir = os.path.join(repo, "cress_amd", "fortran")
+    flags = ["-O2", "-fPIC", "-ffp-contract=off", "-I", objm, "-I", objd, "-I", MPI_INC, "-module-dir", objd]
+    run([FLANG, "-c"] + flags + [os.path.join(fdir, "m_cress_hip.f90"), "-o", os.path.join(objd, "m_cress_hip.o")])
+    dobjs = [os.path.join(objd, "m_cress_hip.o")]
+    for n in names:
+        o = os.path.join(objd, n + ".o")
+        run([FLANG, "-c"] + flags + [os.path.join(fdir, "dropin", n + ".f90"), "-o", o])
+        dobjs.append(o)
+    shim = os.path.join(objd, "dropin_cpu.o")
+    run(["gcc", "-c", "-O2", "-fPIC", "-ffp-contract=off", "-I", os.path.join(repo, "include"), "-I", HERE,
+         os.path.join(HERE, "dropin_cpu.c"), "-o", shim])
+    h1 = os.path.join(HERE, "ref_harness.f90")
+    h2 = os.path.join(HERE, "ref_harness_mpi.f90")
+    closure_names = closure(harness_modules(h1) + harness_modules(h2), mod2file, uses)
+    replaced = {mod2file["m_" + n] for n in names}
+    robjs = [os.path.join(objm, n + ".o") for g in levels(closure_names, mod2file, uses) for n in g
+             if n not in replaced]
+    hobjs = [os.path.join(objm, "ref_harness.o"), os.path.join(objm, "ref_harness_mpi.o")]
+    lib = os.path.join(OUT, "libcressrefdrop32.so")
+    run([FLANG, "-shared", "-fPIC", "-o", lib] + hobjs + robjs + dobjs + [shim] +
+        ["-L" + OUT, "-Wl,-rpath," + OUT, "-loracle32", "-lm",
+         "-L" + MPI_LIB, "-Wl,-rpath," + MPI_LIB, "-lmpifort", "-lmpi"])
+    print("built", lib, "with the drop-ins", ", ".join(names))
+    return lib
+
+
 if __name__ == "__main__":
     build(int(sys.argv[1]) if len(sys.argv) > 1 else 32)
     build_mpi(int(sys.argv[1]) if len(sys.argv) > 1 else 32)
+    build_dropin()

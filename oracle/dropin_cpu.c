@@ -1,0 +1,127 @@
+/* The tier-2 entry points of include/cress_hip.h that the Fortran drop-in
+ * modules call (cress_gaussel, cress_advs, cress_advuvw, cress_advp),
+ * implemented by the oracle on the CPU -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Linked, with the drop-in modules of cress_amd/fortran/dropin/ in place of
+ * the reference's m_gaussel, m_advs, m_advuvw and m_advp, into the
+ * reference's own solver routines (oracle/build_ref.py: build_dropin), it
+ * runs the drop-ins inside s_intgdrv on this machine: the whole step must
+ * stay bitwise the unmodified reference's, which checks everything the
+ * drop-ins pass across the boundary (argument order, the option ids they
+ * resolve with getiname/getrname, the spacings, the array extents).  The
+ * oracle's routines are pinned bitwise against the reference's own
+ * (tests/test_oracle_ref.py), so any difference is the boundary's. */
+#include <stdlib.h>
+#include <string.h>
+
+#include "cress_hip.h"
+#include "cress_oracle.h"
+
+typedef float R;
+
+void or_gaussel(const or_cfg *c, int istr, int iend, int jstr, int jend, int kstr, int kend, R *rr, R *ss,
+                R *tt, R *ff);
+void or_advs(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc, const R *s, R *sfrc, R *vadv,
+             R *tmp1, R *tmp2, R *tmp3);
+void or_advuvw(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc, const R *u, const R *v,
+               const R *w, R *ufrc, R *vfrc, R *wfrc, R *hadv, R *vadv, R *tmp1, R *tmp2, R *tmp3);
+void or_advp(const or_cfg *c, const R *jcb8u, const R *jcb8v, const R *jcb8w, const R *u, const R *v,
+             const R *wc, const R *pp, R *pfrc, R *jcbxu, R *jcbxv, R *jcbxwc, R *vadv, R *tmp1, R *tmp2,
+             R *tmp3);
+
+/* the oracle's configuration for a tier-2 call's cress_config */
+static void to_or(const cress_config *g, or_cfg *c) {
+  memset(c, 0, sizeof *c);
+  c->ni = g->ni;
+  c->nj = g->nj;
+  c->nk = g->nk;
+  c->wbc = g->wbc;
+  c->ebc = g->ebc;
+  c->sbc = g->sbc;
+  c->nbc = g->nbc;
+  c->bbc = g->bbc;
+  c->tbc = g->tbc;
+  c->advopt = g->advopt;
+  c->impopt = g->impopt;
+  c->gwmopt = g->gwmopt;
+  c->buyopt = g->buyopt;
+  c->divopt = g->divopt;
+  c->cphopt = g->cphopt;
+  c->haiopt = g->haiopt;
+  c->vspopt = g->vspopt;
+  c->coropt = g->coropt;
+  c->sthopt = g->sthopt;
+  /* the 4th-order reduction next to non-periodic sides (sethalo.f90:536-580) */
+  c->iwest = g->wbc == -1 ? 0 : 1;
+  c->ieast = g->ebc == -1 ? 0 : 1;
+  c->jsouth = g->sbc == -1 ? 0 : 1;
+  c->jnorth = g->nbc == -1 ? 0 : 1;
+  c->ksp0[0] = g->ksp0[0];
+  c->ksp0[1] = g->ksp0[1];
+  memcpy(c->vspvar, g->vspvar, sizeof c->vspvar);
+  c->dx = g->dx;
+  c->dy = g->dy;
+  c->dz = g->dz;
+  c->weicoe = g->weicoe;
+  c->filcoe = g->filcoe;
+  c->thresq = g->thresq;
+  c->gwave = g->gwave;
+  c->smtopt = g->smtopt;
+  c->smhcoe = g->smhcoe;
+  c->smvcoe = g->smvcoe;
+}
+
+static R *scratch(const cress_config *g, int n) {
+  return (R *)calloc((size_t)n * (size_t)(g->ni + 2) * (size_t)(g->nj + 2) * (size_t)g->nk, sizeof(R));
+}
+
+const char *cress_error_string(int code) { return code == CRESS_OK ? "ok" : "error (CPU drop-in shim)"; }
+
+int cress_gaussel(int istr, int iend, int jstr, int jend, int kstr, int kend, int ni, int nj, int nk, float *rr,
+                  float *ss, float *tt, float *ff) {
+  or_cfg c;
+  memset(&c, 0, sizeof c);
+  c.ni = ni;
+  c.nj = nj;
+  c.nk = nk;
+  c.impopt = 1;
+  or_gaussel(&c, istr, iend, jstr, jend, kstr, kend, rr, ss, tt, ff);
+  return CRESS_OK;
+}
+
+int cress_advs(const cress_config *cfg, const float *rstxu, const float *rstxv, const float *rstxwc,
+               const float *s, float *sfrc) {
+  or_cfg c;
+  to_or(cfg, &c);
+  R *t = scratch(cfg, 4);
+  if (!t) return CRESS_ENOMEM;
+  const size_t n = (size_t)(cfg->ni + 2) * (size_t)(cfg->nj + 2) * (size_t)cfg->nk;
+  or_advs(&c, rstxu, rstxv, rstxwc, s, sfrc, t, t + n, t + 2 * n, t + 3 * n);
+  free(t);
+  return CRESS_OK;
+}
+
+int cress_advuvw(const cress_config *cfg, const float *rstxu, const float *rstxv, const float *rstxwc,
+                 const float *u, const float *v, const float *w, float *ufrc, float *vfrc, float *wfrc) {
+  or_cfg c;
+  to_or(cfg, &c);
+  R *t = scratch(cfg, 5);
+  if (!t) return CRESS_ENOMEM;
+  const size_t n = (size_t)(cfg->ni + 2) * (size_t)(cfg->nj + 2) * (size_t)cfg->nk;
+  or_advuvw(&c, rstxu, rstxv, rstxwc, u, v, w, ufrc, vfrc, wfrc, t, t + n, t + 2 * n, t + 3 * n, t + 4 * n);
+  free(t);
+  return CRESS_OK;
+}
+
+int cress_advp(const cress_config *cfg, const float *jcb8u, const float *jcb8v, const float *jcb8w,
+               const float *u, const float *v, const float *wc, const float *pp, float *pfrc) {
+  or_cfg c;
+  to_or(cfg, &c);
+  R *t = scratch(cfg, 7);
+  if (!t) return CRESS_ENOMEM;
+  const size_t n = (size_t)(cfg->ni + 2) * (size_t)(cfg->nj + 2) * (size_t)cfg->nk;
+  or_advp(&c, jcb8u, jcb8v, jcb8w, u, v, wc, pp, pfrc, t, t + n, t + 2 * n, t + 3 * n, t + 4 * n, t + 5 * n,
+          t + 6 * n);
+  free(t);
+  return CRESS_OK;
+}

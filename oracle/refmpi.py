@@ -19,6 +19,10 @@ import numpy as np
 from oracle import pyoracle as po
 
 LIB = os.path.join(po.REFDIR, "libcressrefmpi32.so")
+# the same routines with the drop-in modules of cress_amd/fortran/dropin/ in
+# place of the reference's m_gaussel, m_advs, m_advuvw, m_advp, answered by
+# the oracle on the CPU (build_ref.py: build_dropin)
+LIB_DROPIN = os.path.join(po.REFDIR, "libcressrefdrop32.so")
 
 # base block of the harness (ref_harness_mpi.f90)
 BASE = ["jcb", "jcb8u", "jcb8v", "jcb8w", "ubr", "vbr", "pbr", "ptbr", "qvbr", "rbr", "rst",
@@ -34,8 +38,8 @@ def have():
 class Lib:
     """Handle on the library inside the child: configured for one case."""
 
-    def __init__(self, cfg):
-        self.lib = C.CDLL(LIB)
+    def __init__(self, cfg, path=LIB):
+        self.lib = C.CDLL(path)
         self.cfg = cfg
         self.lib.ref_mpi_init()
         f = C.c_float
@@ -65,16 +69,17 @@ class Lib:
         return call
 
 
-def _child(job, cfgbytes, args):
+def _child(job, cfgbytes, args, path):
     cfg = po.Cfg.from_buffer_copy(cfgbytes)
-    return job(Lib(cfg), *args)
+    return job(Lib(cfg, path), *args)
 
 
-def run(job, cfg, *args):
-    """job(lib, *args) in a fresh spawned process; cfg is an oracle Cfg."""
+def run(job, cfg, *args, lib=LIB):
+    """job(lib, *args) in a fresh spawned process; cfg is an oracle Cfg;
+    lib the library's path."""
     ctx = mp.get_context("spawn")
     with ctx.Pool(1) as pool:
-        return pool.apply(_child, (job, bytes(cfg), args))
+        return pool.apply(_child, (job, bytes(cfg), args, lib))
 
 
 # ---------------------------------------------------------------------------
@@ -115,7 +120,7 @@ def _job_intgdrv(lib, sc_vals, blocks, nsteps):
     return out
 
 
-def intgdrv(case, nsteps):
+def intgdrv(case, nsteps, lib=LIB):
     """nsteps large steps of the reference's s_intgdrv from the case's
     initial state, cold start (the first step forward with half intervals).
     Returns per step (present level (ni+2,nj+2,nk,16) in LEV order, prwtr,
@@ -125,7 +130,7 @@ def intgdrv(case, nsteps):
     moist = 1 if sc.cphopt != 0 or np.any(case.init.get("qv", 0.0)) else 0
     vals = (sc.ni, sc.nj, sc.nk, (sc.ksp0[0], sc.ksp0[1]), sc.nsstp, float(sc.dtbig),
             float(sc.dtsml), moist)
-    return run(_job_intgdrv, cfg, vals, state_blocks(case), nsteps)
+    return run(_job_intgdrv, cfg, vals, state_blocks(case), nsteps, lib=lib)
 
 
 # ---------------------------------------------------------------------------

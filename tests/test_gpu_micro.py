@@ -2,9 +2,11 @@
 against the oracle, through the C ABI, on states seeded with every
 hydrometeor species and a supersaturated core.
 
-The GPU evaluates exp/log in binary64 and rounds once; the reference's FP32
-build calls glibc's expf/logf, which differ from that in ~0.06% of
-arguments, so the microphysics cannot be bitwise and the bars are relative:
+The GPU evaluates exp/log in binary32 with float-float internals (the
+correctly rounded result but for arguments within ~2^-40 of a rounding
+boundary); the reference's FP32 build calls glibc's expf/logf, which differ
+from that in ~1e-4 of arguments, so the microphysics cannot be bitwise and
+the bars are relative:
 err(x) = max|x - x_ref| / max|x_ref| over the interior of each field.
   * one large step: err vs the FP32 oracle <= 1e-5 for every field;
   * ten large steps vs the FP64 oracle: err <= max(1e-5, 2 * err of the FP32
