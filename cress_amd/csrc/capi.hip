@@ -1714,6 +1714,65 @@ int cress_gaussel(int istr, int iend, int jstr, int jend, int kstr, int kend, in
   return CRESS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// cress_gaussel without allocation: the four column arrays are staged in a
+// file-scope device array instead of hipMalloc'ed buffers.  Each GPU holds
+// its own copy of `scratch` (CRESS_GAUSSEL_SCRATCH_FLOATS floats, 32 MiB)
+// from the library's first use on it until the process ends.  In host code
+// the array's name is not a device address; its address on each device is
+// looked up once (hipGetSymbolAddress, with that device current) and kept in
+// scratch_dev[].  Calls overlapping on one GPU share the array and race.
+static __device__ __align__(256) float scratch[CRESS_GAUSSEL_SCRATCH_FLOATS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+float *scratch_dev[kMaxDevices];
+
+int scratch_ptr(float **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return CRESS_EHIP;
+  if (dev < 0 || dev >= kMaxDevices) return CRESS_ENODEV;
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return CRESS_EHIP;
+    scratch_dev[dev] = (float *)p;
+  }
+  *out = scratch_dev[dev];
+  return CRESS_OK;
+}
+}  // namespace
+
+int cress_gaussel_noalloc(int istr, int iend, int jstr, int jend, int kstr, int kend, int ni, int nj, int nk,
+                          float *rr, float *ss, float *tt, float *ff) {
+  if (!rr || !ss || !tt || !ff || ni < 1 || nj < 1 || nk < 1) return CRESS_EINVAL;
+  if (istr < 0 || iend > ni + 1 || jstr < 0 || jend > nj + 1 || kstr < 1 || kend > nk || kstr > kend)
+    return CRESS_EINVAL;
+  const Grid g = make_grid(ni, nj, nk);
+  const size_t n = (size_t)g.n();
+  if (n > (size_t)CRESS_GAUSSEL_SCRATCH_FLOATS / 4) return CRESS_ETOOBIG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CRESS_ENODEV;
+  float *base = nullptr;
+  int rc = scratch_ptr(&base);
+  if (rc) return rc;
+  // the four arrays back to back; every element is overwritten by its
+  // upload, so nothing of the previous call's contents is read
+  float *a = base, *b = base + n, *c = base + 2 * n, *d = base + 3 * n;
+  HIPCHK(hipMemcpy(a, rr, n * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b, ss, n * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c, tt, n * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d, ff, n * sizeof(float), hipMemcpyHostToDevice));
+  const int ncol = (iend - istr + 1) * (jend - jstr + 1);
+  hipLaunchKernelGGL(k_gaussel, dim3((ncol + 63) / 64), dim3(64), 0, 0, g, istr, iend, jstr, jend, kstr, kend,
+                     (const float *)a, b, (const float *)c, d);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(ss, b, n * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(ff, d, n * sizeof(float), hipMemcpyDeviceToHost));
+  return CRESS_OK;
+}
+
 int cress_advs(const cress_config *cfg, const float *rstxu, const float *rstxv, const float *rstxwc,
                const float *s, float *sfrc) {
   if (!cfg || !rstxu || !rstxv || !rstxwc || !s || !sfrc) return CRESS_EINVAL;

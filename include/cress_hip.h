@@ -198,6 +198,17 @@ int cress_ens_destroy(cress_ens *e);
 int cress_gaussel(int istr, int iend, int jstr, int jend, int kstr, int kend,
                   int ni, int nj, int nk, float *rr, float *ss, float *tt, float *ff);
 
+/* cress_gaussel without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * loops over many small column boxes, callers that forbid hipMalloc).  The
+ * four arrays are staged in a fixed device array of
+ * CRESS_GAUSSEL_SCRATCH_FLOATS floats per GPU (held until the process
+ * ends), so (ni+2)(nj+2)nk may be at most a quarter of that, else
+ * CRESS_ETOOBIG.  Calls that overlap on one GPU share the array and race.
+ * Same arguments, results and errors as cress_gaussel otherwise. */
+#define CRESS_GAUSSEL_SCRATCH_FLOATS (8 * 1024 * 1024)
+int cress_gaussel_noalloc(int istr, int iend, int jstr, int jend, int kstr, int kend,
+                          int ni, int nj, int nk, float *rr, float *ss, float *tt, float *ff);
 
 /* s_advs(fpadvopt,...,ni,nj,nk,rstxu,rstxv,rstxwc,s,sfrc,vadv,tmp1,tmp2,tmp3)
  * SrcOrig/advs.f90:75-77.  Writes sfrc over the interior. */
