@@ -115,6 +115,8 @@ def parse():
     ap.add_argument("--ens-steps", type=int, default=6)
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shapes instead of autotuning them on the workload")
+    ap.add_argument("--launch", default=None,
+                    help="explicit launch shapes, e.g. ptuv_by=16,col_by=2 (the rest default; no tuning)")
     ap.add_argument("--mlp", type=int, default=1, help="measure the learned microphysics (0: no)")
     ap.add_argument("--mlp-nxy", type=int, default=256)
     ap.add_argument("--mlp-steps", type=int, default=60,
@@ -305,9 +307,19 @@ def main():
     case.load(s)
     del case
     s.sync()
-    shapes = s.get_launch() if args.no_tune else s.autotune()
-    print("bench: launch shapes %s (%s)" % (shapes, "default" if args.no_tune else "autotuned"),
-          file=sys.stderr, flush=True)
+    how = "default" if args.no_tune else "autotuned"
+    if args.launch:
+        shapes = s.get_launch()
+        for kv in args.launch.split(","):
+            k, v = kv.split("=")
+            if k not in shapes:
+                raise SystemExit("bench: unknown launch shape %r (known: %s)" % (k, ", ".join(shapes)))
+            shapes[k] = int(v)
+        s.set_launch(shapes)
+        shapes, how = s.get_launch(), "explicit"
+    else:
+        shapes = s.get_launch() if args.no_tune else s.autotune()
+    print("bench: launch shapes %s (%s)" % (shapes, how), file=sys.stderr, flush=True)
     s.step(1, first=True)
     import ctypes as C
     L = host.lib()
@@ -380,7 +392,7 @@ def main():
                    "dtbig": cfg.dtbig, "cphopt": cfg.cphopt, "advopt": cfg.advopt,
                    "lateral": ("open (option 7, gwave %g m/s)" % cfg.gwave) if cfg.wbc == 7 else "periodic",
                    "parallelism": "single GPU",
-                   "launch_shapes": dict(shapes, how="default" if args.no_tune else "autotuned")},
+                   "launch_shapes": dict(shapes, how=how)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
                      "step_frac": step_b * cells / (ms_per_step * 1e-3) / PEAK_HBM,

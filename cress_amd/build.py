@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libcresship.so")
-SOURCES = ["dyn.hip", "micro.hip", "obc.hip", "turb.hip", "mlp.hip", "diag.hip", "capi.hip"]
+SOURCES = ["dyn.hip", "micro.hip", "obc.hip", "turb.hip", "mlp.hip", "diag.hip", "setup.hip", "capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 FSRC = os.path.join(HERE, "fortran")

@@ -152,9 +152,25 @@ def splitmix_uniform(seed, idx):
 # assembly
 # --------------------------------------------------------------------------
 class Case:
-    def __init__(self, cfg, base, fc, rbct, init, zph, zs, name):
+    def __init__(self, cfg, base, fc, rbct, init, zph, zs, name, snd=None):
         self.cfg, self.base, self.fc, self.rbct, self.init = cfg, base, fc, rbct, init
         self.zph, self.zs, self.name = zph, zs, name
+        self.snd = snd
+
+    def sounding(self, dzf=25.0):
+        """The case's sounding as a fine 1-D profile (levels every dzf m from
+        the ground to 1 km above the model top: z, u, v, p, pt, qv in FP64,
+        hydrostatic), with the vertical-grid, sponge and Coriolis settings:
+        the input of cress_set_base (Solver.set_base)."""
+        sd = self.snd
+        ztop = float(self.zph[-1]) + 1000.0
+        z = np.arange(0.0, ztop + dzf, dzf)
+        th, rh = sd["fn"](z)
+        p, qv, _ = hydrostatic(z, th, rh)
+        u, v = hodograph(z, sd["shear"])
+        return {"z": z, "u": u, "v": v, "p": p, "pt": th, "qv": qv, "zsfc": 0.0,
+                "dzmin": sd["dzmin"] or 0.0, "layer1": 0.0, "layer2": sd["layer2"],
+                "botbar": sd["botbar"], "vspbar": 0.003333, "fc": sd["fcor"]}
 
     def write_state(self, path):
         """The base state, fc, rbct and the past and present levels in the
@@ -260,6 +276,27 @@ def _hydrometeors(X, Y, xc, yc, zs, tk, p, qv, ip, jp, seed, ring, cellnoise=Tru
         qvn[:, :, k] = np.where(b > 0.5, 1.01 * qvs, qvn[:, :, k])
     out["qv"] = qvn
     return out
+
+
+def hodograph(z, shear):
+    """Quarter-circle hodograph of radius 8 m/s below 2 km, then straight to
+    U_s = shear at 6 km (Weisman-Klemp style), shifted to a storm-relative
+    frame; zero wind when shear is None."""
+    u = np.zeros(len(z))
+    v = np.zeros(len(z))
+    if shear is None:
+        return u, v
+    r, us = 8.0, shear
+    for k, zk in enumerate(z):
+        if zk <= 2000.0:
+            a = 0.5 * math.pi * max(zk, 0.0) / 2000.0
+            u[k], v[k] = r - r * math.cos(a), r * math.sin(a)
+        elif zk <= 6000.0:
+            u[k] = r + (us - r) * (zk - 2000.0) / 4000.0
+            v[k] = r
+        else:
+            u[k], v[k] = us, r
+    return u - 0.5 * us, v - 0.5 * r
 
 
 def build(name="bubble", nx=None, ny=None, nz=None, seed=None, cphopt=2, dtbig=None,
@@ -469,4 +506,6 @@ def build(name="bubble", nx=None, ny=None, nz=None, seed=None, cphopt=2, dtbig=N
                            gwave=(gwave_default if gwave is None else gwave),
                            smtopt=smtopt, smhcoe=smhcoe if smtopt else 0.0,
                            smvcoe=smvcoe if smtopt else 0.0)
-    return Case(cfg, base, fc, rbct, init, zw, zs, name)
+    snd = {"fn": sounding, "shear": shear, "dzmin": dzmin if sthopt == 1 else None,
+           "layer2": (nk - 3) * dz, "botbar": botbar, "fcor": fcor}
+    return Case(cfg, base, fc, rbct, init, zw, zs, name, snd)

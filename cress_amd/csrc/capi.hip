@@ -12,6 +12,7 @@
 #include "common.h"
 #include "diag.h"
 #include "dyn.h"
+#include "setup.h"
 #include "micro.h"
 #include "obc.h"
 #include "turb.h"
@@ -95,6 +96,7 @@ struct cress_ctx {
   int accoef_ok[3];
   double *diag;  // cress_diag: kDiagBlocks x DIAG_N partials, DIAG_N results, nk+1 heights
   LaunchTune tune;  // launch shapes (tune_default, cress_autotune)
+  double *setupcol;   // cress_set_base's binary64 columns (SC_N x (nk+1)), or null
   int tune_ver;     // bumped whenever tune changes (captured ensemble graphs bake it in)
   // prognostics by physical slot; logical level l lives in slot (rot+l)%3
   float *u[3], *v[3], *w[3], *pp[3], *ptp[3], *q[3], *nci[3];
@@ -245,6 +247,7 @@ int cress_ctx_create(const cress_config *cfg, cress_ctx **out) {
   o.ksp0 = cfg->ksp0[1];
   o.smtopt = cfg->smtopt;
   x->tune = tune_default();
+  x->setupcol = nullptr;
   if (hipStreamCreate(&x->stream) != hipSuccess) {
     delete x;
     return CRESS_EHIP;
@@ -1676,6 +1679,96 @@ int cress_autotune(cress_ctx *x, int *shape, int n) {
     tune_cache[key] = best;
   }
   if (shape) tune_to_ints(best, shape);
+  return CRESS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Base state from a 1-D sounding in binary64 on the device (setup.hip).
+int cress_set_base(cress_ctx *x, const cress_sounding *s) {
+  if (!x || !s || s->nlev < 1 || !s->z || !s->u || !s->v || !s->p || !s->pt || !s->qv) return CRESS_EINVAL;
+  const cress_config &cf = x->cfg;
+  if (cf.sthopt != 0 && cf.sthopt != 1) return CRESS_EUNSUPPORTED;
+  if (cf.sthopt == 1 && !(s->dzmin > 0.0)) return CRESS_EINVAL;
+  for (int k = 0; k < s->nlev; k++)
+    if (!(s->z[k + 1] > s->z[k])) return CRESS_EINVAL;
+  const Grid g = x->g;
+  const int nk = g.nk, ld = nk + 1;
+  const size_t nl = (size_t)s->nlev + 1;
+  if (!x->setupcol) {
+    float *t = nullptr;
+    int rc = dalloc(x, &t, 2 * (size_t)SC_N * ld);
+    if (rc) return rc;
+    x->setupcol = (double *)t;
+  }
+  double *snd = nullptr;
+  if (hipMalloc(&snd, 6 * nl * sizeof(double)) != hipSuccess) return CRESS_ENOMEM;
+  const double *src[6] = {s->z, s->u, s->v, s->p, s->pt, s->qv};
+  int rc = CRESS_OK;
+  for (int c = 0; c < 6 && !rc; c++)
+    if (hipMemcpyAsync(snd + c * nl, src[c], nl * sizeof(double), hipMemcpyHostToDevice, x->stream) != hipSuccess)
+      rc = CRESS_EHIP;
+  SetupArgs a;
+  a.nk = nk;
+  a.sthopt = cf.sthopt;
+  a.bbc = cf.bbc;
+  a.tbc = cf.tbc;
+  a.dz = cf.dz;
+  a.zsfc = s->zsfc;
+  a.dzmin = s->dzmin;
+  a.layer1 = s->layer1;
+  a.layer2 = s->layer2;
+  a.botbar = s->botbar;
+  a.vspbar = s->vspbar;
+  a.nlev = s->nlev;
+  a.z = snd;
+  a.u = snd + nl;
+  a.v = snd + 2 * nl;
+  a.p = snd + 3 * nl;
+  a.pt = snd + 4 * nl;
+  a.qv = snd + 5 * nl;
+  SetupFill f;
+  f.col = x->setupcol;
+  for (int id = 0; id < CRESS_NBASE; id++) f.base[id] = x->base[id];
+  f.tab = x->tab;
+  f.rbct = x->rbct;
+  f.fc = x->fc;
+  f.fcor = s->fc;
+  if (!rc) {
+    hipLaunchKernelGGL(k_setup_col, dim3(1), dim3(64), 0, x->stream, a, x->setupcol);
+    const long cells = g.nxy * (long)nk;
+    hipLaunchKernelGGL(k_setup_fill, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, x->stream, g, f);
+    if (hipGetLastError() != hipSuccess) rc = CRESS_EHIP;
+  }
+  std::vector<double> zph(ld);
+  if (!rc && (hipMemcpyAsync(zph.data(), x->setupcol + SC_ZPH * ld, ld * sizeof(double), hipMemcpyDeviceToHost,
+                             x->stream) != hipSuccess ||
+              hipMemcpyAsync(x->tabh.data(), x->tab, x->tabh.size() * sizeof(float), hipMemcpyDeviceToHost,
+                             x->stream) != hipSuccess ||
+              hipStreamSynchronize(x->stream) != hipSuccess))
+    rc = CRESS_EHIP;
+  hipFree(snd);
+  if (rc) return rc;
+  // every base field is now one value per level; vspdmp's lowest sponge
+  // level: one below the first zeta level (from k=3) above botbar
+  x->g.umask = (1u << CRESS_NBASE) - 1u;
+  int ksp0 = 2;
+  for (int k = 3; k <= nk; k++)
+    if (zph[k] > s->botbar) {
+      ksp0 = k - 1;
+      break;
+    }
+  x->cfg.ksp0[0] = 2;
+  x->cfg.ksp0[1] = ksp0;
+  x->o.ksp0 = ksp0;
+  return refresh_accoef(x);
+}
+
+int cress_base_columns(const cress_ctx *x, double *out, int ncol) {
+  if (!x || !out || ncol < 1 || ncol > SC_N) return CRESS_EINVAL;
+  if (!x->setupcol) return CRESS_ERANGE;
+  const size_t ld = (size_t)x->g.nk + 1;
+  if (hipMemcpy(out, x->setupcol, (size_t)ncol * ld * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    return CRESS_EHIP;
   return CRESS_OK;
 }
 

@@ -31,6 +31,18 @@ FID.update({"fc": 64, "rbct": 65, "prwtr": 66, "price": 67})
 DIAG_NAMES = ["vapour", "liquid", "ice", "precip", "dry", "ke", "ie", "pe", "theta",
               "wmax", "wmin", "ptpmax", "ptpmin"]
 NDIAG = len(DIAG_NAMES)
+# cress_base_columns' order
+SETUP_COLS = ["zph", "zph8s", "ubr", "vbr", "pbr", "ptbr", "qvbr", "rbr", "jcb", "jcb8w", "rst", "rst8w", "rcsq",
+              "rbct", "ptvbr", "pibr"]
+NSETUPCOL = len(SETUP_COLS)
+
+
+class Sounding(C.Structure):
+    """cress_sounding (include/cress_hip.h)."""
+    _fields_ = [("nlev", C.c_int)] + [(k, C.POINTER(C.c_double)) for k in ("z", "u", "v", "p", "pt", "qv")] + \
+               [(k, C.c_double) for k in ("zsfc", "dzmin", "layer1", "layer2", "botbar", "vspbar", "fc")]
+
+
 # launch shapes (cress_autotune / cress_set_launch order)
 LAUNCH_NAMES = ["ptuv_by", "ptuv_slots", "ptuv_order", "col_by", "col_lds", "fs_by", "fs_order", "ff_by",
                 "ff_order"]
@@ -116,6 +128,16 @@ def lib():
         L.cress_mlp_grad.argtypes = [P, P, P, I, P, C.POINTER(D)]
         L.cress_mlp_train.argtypes = [P, I, I, C.c_float, C.c_uint, P]
         L.cress_ctx_set_mlp.argtypes = [P, P, I]
+        L.cress_ctx_mlp_cells.argtypes = [P, C.POINTER(C.c_long)]
+        L.cress_mlp_nparam.argtypes = []
+        L.cress_mlp_nnorm.argtypes = []
+        L.cress_diag.argtypes = [P, I, C.POINTER(D)]
+        L.cress_autotune.argtypes = [P, C.POINTER(I), I]
+        L.cress_set_launch.argtypes = [P, C.POINTER(I), I]
+        L.cress_get_launch.argtypes = [P, C.POINTER(I), I]
+        L.cress_set_base.argtypes = [P, C.POINTER(Sounding)]
+        L.cress_base_columns.argtypes = [P, C.POINTER(D), I]
+        L.cress_gaussel_noalloc.argtypes = [I] * 9 + [P] * 4
         _lib = L
     return _lib
 
@@ -224,6 +246,29 @@ class Solver:
         buf = (D * NDIAG)()
         check(self.L.cress_diag(self.ctx, level, buf), "cress_diag")
         return dict(zip(DIAG_NAMES, list(buf)))
+
+    def set_base(self, snd):
+        """Base state and metrics from a 1-D sounding, in binary64 on the
+        device (cress_set_base).  snd: dict with arrays z, u, v, p, pt, qv
+        (levels 0..nlev) and scalars zsfc, dzmin, layer1, layer2, botbar,
+        vspbar, fc."""
+        arrs = {k: np.ascontiguousarray(snd[k], np.float64) for k in ("z", "u", "v", "p", "pt", "qv")}
+        s = Sounding()
+        s.nlev = len(arrs["z"]) - 1
+        for k, a in arrs.items():
+            setattr(s, k, a.ctypes.data_as(C.POINTER(D)))
+        for k in ("zsfc", "dzmin", "layer1", "layer2", "botbar", "vspbar", "fc"):
+            setattr(s, k, float(snd.get(k, 0.0)))
+        check(self.L.cress_set_base(self.ctx, C.byref(s)), "cress_set_base")
+        return s
+
+    def base_columns(self):
+        """cress_set_base's binary64 per-level columns: dict name -> (nk+1,)
+        array (slot 0 unused)."""
+        nk = self.cfg.nk
+        out = np.empty((NSETUPCOL, nk + 1), np.float64)
+        check(self.L.cress_base_columns(self.ctx, out.ctypes.data_as(C.POINTER(D)), NSETUPCOL), "cress_base_columns")
+        return dict(zip(SETUP_COLS, out))
 
     def autotune(self):
         """Time every candidate launch shape on this grid and keep the

@@ -139,6 +139,37 @@ const char *cress_kernel_name(int id);
 int cress_kernel_times(cress_ctx *ctx, double *total_ms, long *launches, int n);
 int cress_reset_timing(cress_ctx *ctx);
 
+/* Base state and metrics from a 1-D sounding, computed in binary64 on the
+ * device and rounded once into the binary32 fields (flat terrain, every
+ * column the same): the reference's setup chain s_stretch
+ * (stretch.f90:174-359, sthopt 0/1 from the context's options), s_jacobian
+ * (jacobian.f90:243-251), s_intrpsnd (intrpsnd.f90:99-301: vint13 linear
+ * interpolation of the sounding to the scalar heights), s_setbase and
+ * s_bcbase (setbase.f90:171-185, bcbase.f90:120-190), s_baserho
+ * (baserho.f90:153-170), s_var8uvw with bc8w (var8uvw.f90:166-294),
+ * s_sndwave (sndwave.f90:119) and s_vspdmp (vspdmp.f90:160-285, vspopt=2,
+ * which also sets the sponge's lowest level ksp0).  Replaces every base
+ * field (CRESS_JCB..CRESS_RCSQ), rbct and fc.  The top slot nk of the
+ * scalar fields and of jcb, which the reference leaves unset, repeats nk-1. */
+typedef struct cress_sounding {
+  int nlev;                 /* the sounding has levels 0..nlev (nlev >= 1) */
+  const double *z;          /* heights, m, increasing */
+  const double *u, *v;      /* wind, m/s */
+  const double *p;          /* pressure, Pa */
+  const double *pt;         /* potential temperature, K */
+  const double *qv;         /* water vapour mixing ratio, kg/kg */
+  double zsfc, dzmin, layer1, layer2;  /* &gridsth: surface height and the
+                                          stretching's layers (sthopt=1) */
+  double botbar, vspbar;    /* sponge above botbar (m), coefficient (1/s) */
+  double fc;                /* Coriolis parameter (constant f-plane), 1/s */
+} cress_sounding;
+int cress_set_base(cress_ctx *ctx, const cress_sounding *snd);
+/* The setup's binary64 per-level columns (k = 0..nk, slot 0 unused), ncol
+ * of them in this order: zph (zeta levels), zph8s, ubr, vbr, pbr, ptbr, qvbr,
+ * rbr, jcb, jcb8w, rst, rst8w, rcsq, rbct (vspopt=2), ptvbr, pibr. */
+#define CRESS_NSETUPCOL 16
+int cress_base_columns(const cress_ctx *ctx, double *out, int ncol);
+
 /* Launch shapes of the tunable kernel families, as 9 ints: fused
  * ptil/stepuv rows per workgroup (4, 8, 16), halo layout (0 one point per
  * thread, 1 slots), tile order (1 x-fastest, 2 z-fastest); column kernel

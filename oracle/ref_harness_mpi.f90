@@ -43,6 +43,13 @@
       use m_fallblk
       use m_phy2cnt
       use m_shift2nd
+      use m_stretch
+      use m_intrpsnd
+      use m_setbase
+      use m_baserho
+      use m_var8uvw
+      use m_sndwave
+      use m_vspdmp
 
       implicit none
 
@@ -414,5 +421,81 @@
      &  lev(0,0,1,14,lv),zero3(0,0,1,29),zero3(0,0,1,32),               &
      &  zero3(0,0,1,35),zero3(0,0,1,36),zero3(0,0,1,37))
       end subroutine ref_shift2nd
+
+! The reference's base-state setup over flat terrain, in the order the
+! solver runs it: s_stretch with ztop1=z(nk-2), ztop2=z(nk-1)
+! (phycood.f90:178-179; z(k)=zsfc+(k-2)dz, getz.f90:113; sthopt=0 takes z
+! itself), zph = zsth, jcb as s_jacobian forms it over zeta
+! (jacobian.f90:243-251) and its averages by s_var8uvw
+! (jacobian.f90:339), then s_intrpsnd, s_setbase, s_baserho
+! (inivar.f90:667-684), s_sndwave (inivar.f90:850) and s_vspdmp.
+! base gets jcb,jcb8u,jcb8v,jcb8w,ubr,vbr,pbr,ptbr,qvbr,rbr,rst,rst8u,
+! rst8v,rst8w,rcsq (the harness's base block order).
+      subroutine ref_setup_base(ni,nj,nk,sthopt,dz,zsfc,dzmin,layer1,     &
+     &   layer2,nlev,z1d,u1d,v1d,p1d,pt1d,qv1d,botbar,vspbar,zsth,base,  &
+     &   rbct,ksp0) bind(C,name='ref_setup_base')
+      integer(c_int), value :: ni,nj,nk,sthopt,nlev
+      real, value :: dz,zsfc,dzmin,layer1,layer2,botbar,vspbar
+      real, intent(in) :: z1d(0:nlev),u1d(0:nlev),v1d(0:nlev)
+      real, intent(in) :: p1d(0:nlev),pt1d(0:nlev),qv1d(0:nlev)
+      real, intent(out) :: zsth(1:nk)
+      real, intent(inout) :: base(0:ni+1,0:nj+1,1:nk,1:15)
+      real, intent(inout) :: rbct(1:ni,1:nj,1:nk,1:2)
+      integer(c_int), intent(out) :: ksp0(2)
+      real z(1:nk),dzsth(1:nk),z1dmax(1:nk)
+      real, allocatable :: zph(:,:,:),t1(:,:,:),t2(:,:,:),t3(:,:,:)
+      integer i,j,k,kk(2)
+      allocate(zph(0:ni+1,0:nj+1,1:nk))
+      allocate(t1(0:ni+1,0:nj+1,1:nk),t2(0:ni+1,0:nj+1,1:nk))
+      allocate(t3(0:ni+1,0:nj+1,1:nk))
+      iname(idsthopt)=sthopt
+      rname(idzsfc)=zsfc
+      rname(iddzmin)=dzmin
+      rname(idlayer1)=layer1
+      rname(idlayer2)=layer2
+      cname(idsndtyp)='zpm'
+      iname(idvspopt)=2
+      rname(idvspgpv)=0.e0
+      rname(idbotgpv)=0.e0
+      rname(idvspbar)=vspbar
+      rname(idbotbar)=botbar
+      do k=1,nk
+        z(k)=zsfc+real(k-2)*dz
+      end do
+      if(sthopt.eq.0) then
+        zsth(1:nk)=z(1:nk)
+      else
+        call s_stretch(idsthopt,idzsfc,iddzmin,idlayer1,idlayer2,       &
+     &    z(nk-2),z(nk-1),nk,zsth,dzsth)
+      end if
+      do k=1,nk
+        zph(:,:,k)=zsth(k)
+      end do
+      base(:,:,:,1)=0.e0
+      do k=1,nk-1
+        do j=0,nj+1
+        do i=0,ni+1
+          base(i,j,k,1)=(zph(i,j,k+1)-zph(i,j,k))/(z(k+1)-z(k))
+        end do
+        end do
+      end do
+      call s_var8uvw(idwbc,idebc,idexbopt,ni,nj,nk,base(0,0,1,1),         &
+     &  base(0,0,1,2),base(0,0,1,3),base(0,0,1,4))
+      call s_intrpsnd(idsndtyp,ni,nj,nk,zph,base(0,0,1,5),base(0,0,1,6), &
+     &  base(0,0,1,7),base(0,0,1,8),base(0,0,1,9),t1,t2,t3,nlev,z1d,u1d, &
+     &  v1d,p1d,pt1d,qv1d)
+      call s_setbase(ni,nj,nk,zph,base(0,0,1,5),base(0,0,1,6),           &
+     &  base(0,0,1,7),base(0,0,1,8),base(0,0,1,9),base(0,0,1,10),t1,t2,  &
+     &  t3)
+      call s_baserho(idadvopt,idsmtopt,ni,nj,nk,base(0,0,1,1),           &
+     &  base(0,0,1,10),base(0,0,1,11),base(0,0,1,12),base(0,0,1,13),     &
+     &  base(0,0,1,14))
+      call s_sndwave(ni,nj,nk,base(0,0,1,7),base(0,0,1,15))
+      call s_vspdmp(idvspopt,idvspgpv,idvspbar,idbotgpv,idbotbar,kk,ni,  &
+     &  nj,nk,zph,rbct,z1dmax)
+      ksp0(1)=kk(1)
+      ksp0(2)=kk(2)
+      deallocate(zph,t1,t2,t3)
+      end subroutine ref_setup_base
 
       end module cress_ref_mpi

@@ -79,7 +79,9 @@ def run(job, cfg, *args, lib=LIB):
     lib the library's path."""
     ctx = mp.get_context("spawn")
     with ctx.Pool(1) as pool:
-        return pool.apply(_child, (job, bytes(cfg), args, lib))
+        # a reference routine that stops the process (its error exit) would
+        # leave apply() waiting for ever: bound the wait
+        return pool.apply_async(_child, (job, bytes(cfg), args, lib)).get(timeout=600)
 
 
 # ---------------------------------------------------------------------------
@@ -149,3 +151,27 @@ def calls(cfg, seq):
     another in one child process configured for cfg (an oracle Cfg);
     returns each call's argument list after the call."""
     return run(_job_calls, cfg, seq)
+
+
+# ---------------------------------------------------------------------------
+# Base-state setup (s_stretch, s_jacobian, s_intrpsnd, s_setbase, s_baserho,
+# s_var8uvw, s_sndwave, s_vspdmp)
+
+def setup_base(sc, snd):
+    """The reference's FP32 base-state chain over flat terrain for a solver
+    config sc and a sounding dict (cases.Case.sounding: arrays z, u, v, p,
+    pt, qv and scalars zsfc, dzmin, layer1, layer2, botbar, vspbar).
+    Returns (zsth (nk,), base block (ni+2,nj+2,nk,15) in BASE order,
+    rbct (ni,nj,nk,2), ksp0 (2,))."""
+    cfg = po.cfg_from_solver_config(sc)
+    ni, nj, nk = sc.ni, sc.nj, sc.nk
+    arrs = [np.ascontiguousarray(snd[k], np.float32) for k in ("z", "u", "v", "p", "pt", "qv")]
+    zsth = np.zeros(nk, np.float32)
+    base = np.zeros((ni + 2, nj + 2, nk, len(BASE)), np.float32, order="F")
+    rbct = np.zeros((ni, nj, nk, 2), np.float32, order="F")
+    ksp0 = np.zeros(2, np.int32)
+    f = lambda k: float(snd.get(k, 0.0))
+    args = [ni, nj, nk, sc.sthopt, float(sc.dz), f("zsfc"), f("dzmin"), f("layer1"), f("layer2"),
+            len(arrs[0]) - 1] + arrs + [f("botbar"), f("vspbar"), zsth, base, rbct, ksp0]
+    out = calls(cfg, [("ref_setup_base", args)])[0]
+    return out[-4], out[-3], out[-2], out[-1]
