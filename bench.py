@@ -287,7 +287,21 @@ def cpu_baseline(args, nz):
             break
     return {"value": nxy * nxy * nz * n / el, "unit": "cell-steps/s", "cores": threads, "kind": "port",
             "sample": "FP32 oracle (C restatement, %d OpenMP threads), %s %dx%dx%d, %d large steps after 1 warm-up"
-                      % (threads, args.workload, nxy, nxy, nz, n)}
+                      % (threads, args.workload, nxy, nxy, nz, n),
+            "cpu_model": host_cpu_model(), "cpus_allowed": len(os.sched_getaffinity(0)),
+            "omp": {k: os.environ.get(k, "unset") for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}}
+
+
+def host_cpu_model():
+    """The host CPU's model name (first processor of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main():

@@ -352,7 +352,7 @@ int cress_ctx_mlp_cells(cress_ctx *x, long *cells);
  * outputs are written where the reference writes them and are otherwise
  * left as passed.  Periodic (-1) or open (7) sides.  The tensor, ssq and
  * dry / cloud-free N^2 are the FP32 reference's bits; rkh, rkv, priv and
- * cloudy N^2 (binary64 exp/log) are within 1e-5. */
+ * cloudy N^2 (binary32 exp/log of their own, not libm's) are within 1e-5. */
 int cress_turbcoef(const cress_config *cfg, float dtb, int moist, const float *jcb, const float *jcb8u,
                    const float *jcb8v, const float *jcb8w, const float *pbr, const float *ptbr, const float *rbr,
                    const float *ppp, const float *ptpp, const float *qvp, const float *qallp, const float *up,

@@ -1,9 +1,11 @@
 /* The tier-2 entry points of include/cress_hip.h that the Fortran drop-in
- * modules call (cress_gaussel, cress_advs, cress_advuvw, cress_advp),
+ * modules call (cress_gaussel, cress_advs, cress_advuvw, cress_advp,
+ * cress_turbdrv, cress_turbflx, cress_turbs),
  * implemented by the oracle on the CPU -- TEST INFRASTRUCTURE ONLY.
  *
  * Linked, with the drop-in modules of cress_amd/fortran/dropin/ in place of
- * the reference's m_gaussel, m_advs, m_advuvw and m_advp, into the
+ * the reference's m_gaussel, m_advs, m_advuvw, m_advp, m_turbdrv, m_turbflx
+ * and m_turbs, into the
  * reference's own solver routines (oracle/build_ref.py: build_dropin), it
  * runs the drop-ins inside s_intgdrv on this machine: the whole step must
  * stay bitwise the unmodified reference's, which checks everything the
@@ -28,6 +30,14 @@ void or_advuvw(const or_cfg *c, const R *rstxu, const R *rstxv, const R *rstxwc,
 void or_advp(const or_cfg *c, const R *jcb8u, const R *jcb8v, const R *jcb8w, const R *u, const R *v,
              const R *wc, const R *pp, R *pfrc, R *jcbxu, R *jcbxv, R *jcbxwc, R *vadv, R *tmp1, R *tmp2,
              R *tmp3);
+void or_turbdrv(const or_cfg *c, R dtb, int moist, const R *jcb, const R *jcb8u, const R *jcb8v, const R *jcb8w,
+                const R *pbr, const R *ptbr, const R *rbr, const R *ppp, const R *ptpp, const R *qvp, const R *qallp,
+                const R *up, const R *vp, const R *wp, const R *tkep, R *ufrc, R *vfrc, R *wfrc, R *rkh, R *rkv,
+                R *priv, R *ssq, R *nsq8w, R *t13, R *t23, R *t33, R *t11, R *t22, R *t12, R *t31, R *t32, R *w1,
+                R *w2, R *w3, R *w4);
+void or_turbflx(const or_cfg *c, const R *jcb, const R *s, const R *rkh8u, const R *rkh8v, const R *rkv8w, R *h1,
+                R *h2, R *h3);
+void or_turbs(const or_cfg *c, const R *jcb8u, const R *jcb8v, const R *h1, const R *h2, const R *h3, R *sfrc);
 
 /* the oracle's configuration for a tier-2 call's cress_config */
 static void to_or(const cress_config *g, or_cfg *c) {
@@ -123,5 +133,40 @@ int cress_advp(const cress_config *cfg, const float *jcb8u, const float *jcb8v, 
   or_advp(&c, jcb8u, jcb8v, jcb8w, u, v, wc, pp, pfrc, t, t + n, t + 2 * n, t + 3 * n, t + 4 * n, t + 5 * n,
           t + 6 * n);
   free(t);
+  return CRESS_OK;
+}
+
+int cress_turbdrv(const cress_config *cfg, float dtb, int moist, const float *jcb, const float *jcb8u,
+                  const float *jcb8v, const float *jcb8w, const float *pbr, const float *ptbr, const float *rbr,
+                  const float *up, const float *vp, const float *wp, const float *ppp, const float *ptpp,
+                  const float *qvp, const float *tkep, const float *qallp, float *ufrc, float *vfrc, float *wfrc,
+                  float *rkh, float *rkv, float *priv, float *ssq, float *nsq8w, float *t13, float *t23, float *t33) {
+  or_cfg c;
+  to_or(cfg, &c);
+  /* the reference's work arrays (t11, t22, t12, t31, t32 and four more),
+   * which the entry point does not take */
+  R *t = scratch(cfg, 9);
+  if (!t) return CRESS_ENOMEM;
+  const size_t n = (size_t)(cfg->ni + 2) * (size_t)(cfg->nj + 2) * (size_t)cfg->nk;
+  or_turbdrv(&c, dtb, moist, jcb, jcb8u, jcb8v, jcb8w, pbr, ptbr, rbr, ppp, ptpp, qvp, qallp, up, vp, wp, tkep, ufrc,
+             vfrc, wfrc, rkh, rkv, priv, ssq, nsq8w, t13, t23, t33, t, t + n, t + 2 * n, t + 3 * n, t + 4 * n,
+             t + 5 * n, t + 6 * n, t + 7 * n, t + 8 * n);
+  free(t);
+  return CRESS_OK;
+}
+
+int cress_turbflx(const cress_config *cfg, const float *jcb, const float *s, const float *rkh8u,
+                  const float *rkh8v, const float *rkv8w, float *h1, float *h2, float *h3) {
+  or_cfg c;
+  to_or(cfg, &c);
+  or_turbflx(&c, jcb, s, rkh8u, rkh8v, rkv8w, h1, h2, h3);
+  return CRESS_OK;
+}
+
+int cress_turbs(const cress_config *cfg, const float *jcb8u, const float *jcb8v, const float *h1, const float *h2,
+                const float *h3, float *sfrc) {
+  or_cfg c;
+  to_or(cfg, &c);
+  or_turbs(&c, jcb8u, jcb8v, h1, h2, h3, sfrc);
   return CRESS_OK;
 }

@@ -146,11 +146,11 @@ def _job_calls(lib, calls):
     return [args for _, args in calls]
 
 
-def calls(cfg, seq):
+def calls(cfg, seq, lib=LIB):
     """Run the harness entry points in seq = [(name, [args]), ...] one after
     another in one child process configured for cfg (an oracle Cfg);
     returns each call's argument list after the call."""
-    return run(_job_calls, cfg, seq)
+    return run(_job_calls, cfg, seq, lib=lib)
 
 
 # ---------------------------------------------------------------------------
