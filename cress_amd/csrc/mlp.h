@@ -122,6 +122,7 @@ __global__ void k_mlp_gather(const float *in, const float *tgt, long nsamp, cons
                              int identity, int batch, unsigned short *x, float *t);
 __global__ void k_mlp_loss(const float *y, const float *t, int batch, float scale, float *dy, unsigned short *dyb,
                            double *loss);
+constexpr int MLP_CS_LANES = 4, MLP_CS_ROWS = 128;   // k_mlp_colsum's row lanes and rows per workgroup
 __global__ void k_mlp_colsum(const float *d, long rows, int cols, long stride, float *out);
 __global__ void k_mlp_adam(float *p, const float *gr, float *m, float *v, unsigned short *w, int n, float lr,
                            float b1, float b2, float eps, float c1, float c2, const unsigned char *trainable);

@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
 
 // ---------------------------------------------------------------------------
 // Fused inference over the step's list of network cells.  A workgroup of
-// 16 wavefronts shares one copy of the weights in LDS (BF16, transposed to
+// MLP_WAVES wavefronts shares one copy of the weights in LDS (BF16, transposed to
 // [out][in] with padded rows so the A fragments are 8-byte reads without
 // bank conflicts); each wavefront takes MLP_G groups of 16 cells at a time
 // (grid-stride), so every weight fragment read from LDS feeds MLP_G matrix
@@ -271,16 +271,27 @@ __global__ void k_mlp_loss(const float *y, const float *t, int batch, float scal
   if ((threadIdx.x & 63) == 0) atomicAdd(loss, se);
 }
 
-// Column sums of a rows x cols matrix (row stride `stride`) added into out:
-// one thread per column, rows split over blockIdx.y.
-__global__ void k_mlp_colsum(const float *d, long rows, int cols, long stride, float *out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
-  const long per = (rows + gridDim.y - 1) / gridDim.y;
-  const long r0 = (long)blockIdx.y * per, r1 = min(rows, r0 + per);
+// Column sums of a rows x cols matrix (row stride `stride`) added into out.
+// A workgroup of 64 x MLP_CS_LANES threads takes 64 columns (x, coalesced
+// along a row) and MLP_CS_ROWS rows, each of the MLP_CS_LANES row lanes
+// summing every MLP_CS_LANES-th row; the lanes are combined in LDS and one
+// atomic per column adds the workgroup's sum.
+__global__ void __launch_bounds__(64 * MLP_CS_LANES) k_mlp_colsum(const float *d, long rows, int cols, long stride,
+                                                                   float *out) {
+  __shared__ float part[MLP_CS_LANES][64];
+  const int c = blockIdx.x * 64 + threadIdx.x;
+  const long r0 = (long)blockIdx.y * MLP_CS_ROWS;
+  const long r1 = min(rows, r0 + MLP_CS_ROWS);
   float s = 0.0f;
-  for (long r = r0; r < r1; r++) s += d[r * stride + c];
-  atomicAdd(&out[c], s);
+  if (c < cols)
+    for (long r = r0 + threadIdx.y; r < r1; r += MLP_CS_LANES) s += d[r * stride + c];
+  part[threadIdx.y][threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.y == 0 && c < cols) {
+    float t = part[0][threadIdx.x];
+    for (int y = 1; y < MLP_CS_LANES; y++) t += part[y][threadIdx.x];
+    atomicAdd(&out[c], t);
+  }
 }
 
 // Adam (Kingma & Ba 2015) on the FP32 master block; c1 = 1/(1-b1^t),
@@ -413,13 +424,13 @@ void fwd_bwd(cress_mlp *m, int B, double *loss) {
                      loss);
   fill(m, m->g, 0.0f, MLP_NPARAM);
   const int kc = 2048;                                   // split of the batch in the weight gradients
-  const unsigned rows_per = 4096;                        // rows per colsum block
-  const dim3 cs_grid(1, (unsigned)((B + rows_per - 1) / rows_per));
+  const dim3 cs_block(64, MLP_CS_LANES);
+  const unsigned cs_rows = (unsigned)((B + MLP_CS_ROWS - 1) / MLP_CS_ROWS);
   // output layer
   gemm(m, m->H3, 1, H, m->dYb, MLP_OUTP, 1, H, MLP_OUTP, B, EPI_ATOMIC, m->g + MLP_OW4, MLP_OUTP, nullptr, nullptr, 0,
        nullptr, kc);
-  hipLaunchKernelGGL(k_mlp_colsum, cs_grid, dim3(128), 0, m->st, m->dY, (long)B, MLP_OUTP, (long)MLP_OUTP,
-                     m->g + MLP_OB4);
+  hipLaunchKernelGGL(k_mlp_colsum, dim3((MLP_OUTP + 63) / 64, cs_rows), cs_block, 0, m->st, m->dY, (long)B,
+                     MLP_OUTP, (long)MLP_OUTP, m->g + MLP_OB4);
   // hidden layers: dH = dZ_next W_next^T, dZ = dH * gelu'(Z), dW = H_prev^T dZ
   struct L { int ow, ob; const float *Z; const unsigned short *Hprev; int kin; int on; };
   const L layers[3] = {{MLP_OW3, MLP_OB3, m->Z3, m->H2, H, MLP_OW4},
@@ -433,7 +444,8 @@ void fwd_bwd(cress_mlp *m, int B, double *loss) {
     // dH = dnext [B x nnext] * W_next^T [nnext x H]  (W_next stored [H][nnext])
     gemm(m, dnext, nnext, 1, w + c.on, 1, nnext, B, H, nnext, EPI_DGELU, m->D, H, nullptr, dz, H, c.Z);
     gemm(m, c.Hprev, 1, c.kin, dz, H, 1, c.kin, H, B, EPI_ATOMIC, m->g + c.ow, H, nullptr, nullptr, 0, nullptr, kc);
-    hipLaunchKernelGGL(k_mlp_colsum, cs_grid, dim3(128), 0, m->st, m->D, (long)B, H, (long)H, m->g + c.ob);
+    hipLaunchKernelGGL(k_mlp_colsum, dim3((H + 63) / 64, cs_rows), cs_block, 0, m->st, m->D, (long)B, H, (long)H,
+                       m->g + c.ob);
     dnext = dz;
     nnext = H;
   }

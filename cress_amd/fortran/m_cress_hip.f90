@@ -22,6 +22,13 @@ module m_cress_hip
   integer(c_int), parameter, public :: cress_enomem = -4
   integer(c_int), parameter, public :: cress_erange = -5
   integer(c_int), parameter, public :: cress_eunsupported = -6
+  integer(c_int), parameter, public :: cress_etoobig = -7
+
+! sizes of the result arrays of cress_diag, cress_autotune /
+! cress_set_launch / cress_get_launch and cress_base_columns
+  integer(c_int), parameter, public :: cress_ndiag = 13
+  integer(c_int), parameter, public :: cress_nlaunch = 9
+  integer(c_int), parameter, public :: cress_nsetupcol = 16
 
 ! field ids (base state: level 0 only; prognostics: 0 past, 1 present,
 ! 2 future)
@@ -61,7 +68,22 @@ module m_cress_hip
   public :: cress_hevi, cress_steps, cress_cloudphy, cress_turbcoef
   public :: cress_turbdrv, cress_turbflx, cress_turbs
   public :: cress_bc8u, cress_bc8v, cress_bc8w
+  public :: cress_kh8uv, cress_gaussel_noalloc
+  public :: cress_set_base, cress_base_columns, cress_diag
+  public :: cress_autotune, cress_set_launch, cress_get_launch
+  public :: cress_ens_create, cress_ens_step, cress_ens_sync, cress_ens_destroy
+  public :: cress_mlp_create, cress_mlp_destroy, cress_mlp_init
+  public :: cress_mlp_fit_norm, cress_mlp_train, cress_ctx_set_mlp
+  public :: cress_ctx_mlp_cells
   public :: cress_errmsg, base_cfg
+
+! a 1-D sounding for cress_set_base: z, u, v, p, pt, qv point to
+! real(c_double) arrays of nlev+1 levels (heights increasing)
+  type, bind(c), public :: cress_sounding
+    integer(c_int) :: nlev
+    type(c_ptr) :: z, u, v, p, pt, qv
+    real(c_double) :: zsfc, dzmin, layer1, layer2, botbar, vspbar, fc
+  end type cress_sounding
 
   interface
 
@@ -336,6 +358,165 @@ module m_cress_hip
       real(c_float), intent(inout) :: var(*)
       integer(c_int) :: cress_bcycle
     end function cress_bcycle
+
+! tier 2: s_kh8uv (kh8uv.f90:60), mfcopt=0
+    function cress_kh8uv(cfg, rkh, rkh8u, rkh8v) bind(c, name='cress_kh8uv')
+      import :: c_int, c_float, cress_config
+      type(cress_config), intent(in) :: cfg
+      real(c_float), intent(in) :: rkh(*)
+      real(c_float), intent(inout) :: rkh8u(*), rkh8v(*)
+      integer(c_int) :: cress_kh8uv
+    end function cress_kh8uv
+
+! cress_gaussel without device allocation (a fixed device array stages the
+! four arrays; larger boxes return cress_etoobig)
+    function cress_gaussel_noalloc(istr, iend, jstr, jend, kstr, kend,  &
+     &  ni, nj, nk, rr, ss, tt, ff) bind(c, name='cress_gaussel_noalloc')
+      import :: c_int, c_float
+      integer(c_int), value :: istr, iend, jstr, jend, kstr, kend
+      integer(c_int), value :: ni, nj, nk
+      real(c_float), intent(in) :: rr(*), tt(*)
+      real(c_float), intent(inout) :: ss(*), ff(*)
+      integer(c_int) :: cress_gaussel_noalloc
+    end function cress_gaussel_noalloc
+
+! base state from a sounding in binary64 on the GPU (stretch, Jacobian,
+! intrpsnd, setbase, baserho, var8uvw, sndwave, vspdmp)
+    function cress_set_base(ctx, snd) bind(c, name='cress_set_base')
+      import :: c_int, c_ptr, cress_sounding
+      type(c_ptr), value :: ctx
+      type(cress_sounding), intent(in) :: snd
+      integer(c_int) :: cress_set_base
+    end function cress_set_base
+
+    function cress_base_columns(ctx, out, ncol)                        &
+     &  bind(c, name='cress_base_columns')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out(*)
+      integer(c_int), value :: ncol
+      integer(c_int) :: cress_base_columns
+    end function cress_base_columns
+
+! binary64 budgets of a level (cress_ndiag values)
+    function cress_diag(ctx, level, out) bind(c, name='cress_diag')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: level
+      real(c_double), intent(out) :: out(*)
+      integer(c_int) :: cress_diag
+    end function cress_diag
+
+! launch shapes (cress_nlaunch integers)
+    function cress_autotune(ctx, shape, n) bind(c, name='cress_autotune')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: shape(*)
+      integer(c_int), value :: n
+      integer(c_int) :: cress_autotune
+    end function cress_autotune
+
+    function cress_set_launch(ctx, shape, n)                           &
+     &  bind(c, name='cress_set_launch')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(in) :: shape(*)
+      integer(c_int), value :: n
+      integer(c_int) :: cress_set_launch
+    end function cress_set_launch
+
+    function cress_get_launch(ctx, shape, n)                           &
+     &  bind(c, name='cress_get_launch')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: shape(*)
+      integer(c_int), value :: n
+      integer(c_int) :: cress_get_launch
+    end function cress_get_launch
+
+! ensembles: members(1:n) are contexts
+    function cress_ens_create(members, n, e)                           &
+     &  bind(c, name='cress_ens_create')
+      import :: c_int, c_ptr
+      type(c_ptr), intent(in) :: members(*)
+      integer(c_int), value :: n
+      type(c_ptr), intent(out) :: e
+      integer(c_int) :: cress_ens_create
+    end function cress_ens_create
+
+    function cress_ens_step(e, nsteps, first) bind(c, name='cress_ens_step')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: nsteps, first
+      integer(c_int) :: cress_ens_step
+    end function cress_ens_step
+
+    function cress_ens_sync(e) bind(c, name='cress_ens_sync')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int) :: cress_ens_sync
+    end function cress_ens_sync
+
+    function cress_ens_destroy(e) bind(c, name='cress_ens_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int) :: cress_ens_destroy
+    end function cress_ens_destroy
+
+! the learned microphysics
+    function cress_mlp_create(capacity, max_batch, m)                  &
+     &  bind(c, name='cress_mlp_create')
+      import :: c_int, c_long, c_ptr
+      integer(c_long), value :: capacity
+      integer(c_int), value :: max_batch
+      type(c_ptr), intent(out) :: m
+      integer(c_int) :: cress_mlp_create
+    end function cress_mlp_create
+
+    function cress_mlp_destroy(m) bind(c, name='cress_mlp_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int) :: cress_mlp_destroy
+    end function cress_mlp_destroy
+
+    function cress_mlp_init(m, seed) bind(c, name='cress_mlp_init')
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: m
+      integer(c_long_long), value :: seed
+      integer(c_int) :: cress_mlp_init
+    end function cress_mlp_init
+
+    function cress_mlp_fit_norm(m, margin) bind(c, name='cress_mlp_fit_norm')
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      real(c_float), value :: margin
+      integer(c_int) :: cress_mlp_fit_norm
+    end function cress_mlp_fit_norm
+
+    function cress_mlp_train(m, steps, batch, lr, seed, losses)        &
+     &  bind(c, name='cress_mlp_train')
+      import :: c_int, c_ptr, c_float, c_double
+      type(c_ptr), value :: m
+      integer(c_int), value :: steps, batch
+      real(c_float), value :: lr
+      integer(c_int), value :: seed
+      real(c_double), intent(out) :: losses(*)
+      integer(c_int) :: cress_mlp_train
+    end function cress_mlp_train
+
+    function cress_ctx_set_mlp(ctx, m, mode) bind(c, name='cress_ctx_set_mlp')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, m
+      integer(c_int), value :: mode
+      integer(c_int) :: cress_ctx_set_mlp
+    end function cress_ctx_set_mlp
+
+    function cress_ctx_mlp_cells(ctx, cells) bind(c, name='cress_ctx_mlp_cells')
+      import :: c_int, c_ptr, c_long
+      type(c_ptr), value :: ctx
+      integer(c_long), intent(out) :: cells
+      integer(c_int) :: cress_ctx_mlp_cells
+    end function cress_ctx_mlp_cells
 
   end interface
 

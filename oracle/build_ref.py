@@ -268,6 +268,38 @@ def build_dropin(names=DROPINS, jobs=8):
     return lib
 
 
+# The reference's own solver program (solver.f90 and its module closure,
+# the solver's build list) with OpenMP, FP32, one executable: the CPU
+# baseline of the reference itself, timed in this container only (it never
+# travels to a GPU machine).  Same staged defmpi.f90 as build_mpi().
+def build_solver(jobs=8, opt="-O3"):
+    if not os.path.isdir(os.path.join(REF, "SrcOrig")) or not have_mpi():
+        print("reference sources or MPICH absent; oracle/_ref/solver32.exe not built")
+        return None
+    files, mod2file, uses = scan_sources(prefer_solver=True)
+    obj = os.path.join(OUT, "objexe32")
+    os.makedirs(obj, exist_ok=True)
+    files["defmpi"] = stage_defmpi(files, os.path.join(obj, "src", "defmpi.f90"))
+    names = closure([m for m in uses["solver"]], mod2file, uses) | {"solver"}
+    flags = [opt, "-fopenmp", "-module-dir", obj, "-I", obj, "-I", MPI_INC]
+    objs = []
+    for group in levels(names, mod2file, uses):
+        cmds = []
+        for n in group:
+            o = os.path.join(obj, n + ".o")
+            objs.append(o)
+            if os.path.exists(o) and os.path.getmtime(o) > os.path.getmtime(files[n]):
+                continue
+            cmds.append([FLANG, "-c"] + flags + [files[n], "-o", o])
+        with ThreadPoolExecutor(jobs) as ex:
+            list(ex.map(run, cmds))
+    exe = os.path.join(OUT, "solver32.exe")
+    run([FLANG, "-fopenmp", "-o", exe] + objs +
+        ["-L" + MPI_LIB, "-Wl,-rpath," + MPI_LIB, "-lmpifort", "-lmpi"])
+    print("built", exe, "from", len(objs), "reference files")
+    return exe
+
+
 if __name__ == "__main__":
     build(int(sys.argv[1]) if len(sys.argv) > 1 else 32)
     build_mpi(int(sys.argv[1]) if len(sys.argv) > 1 else 32)

@@ -2,11 +2,13 @@
 and open, with even and odd substep counts, stepped through captured
 hipGraph cycles give bitwise the results of stepping each member alone with
 cress_large_step -- checked after every call, with graph replays, eager
-steps that do not fill a cycle, downloads and recaptures in between."""
+steps that do not fill a cycle, downloads and recaptures in between -- and
+eight uneven members stepped together match the oracle run on each alone."""
 import numpy as np
 import pytest
 
 from cress_amd import cases, host
+from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 
@@ -57,3 +59,57 @@ def test_ensemble_rejects_a_member_twice():
     with pytest.raises(host.CressError):
         host.Ensemble([s, s])
     s.close()
+
+
+def uneven_members():
+    """Eight members of different sizes, sides, substep counts and physics."""
+    return [
+        cases.build("bubble", 12, 10, 16, cphopt=0),
+        cases.build("bubble", 14, 12, 20, cphopt=2),                        # nsstp 5
+        cases.build("supercell", 16, 14, 16, cphopt=0, wbc=7, gwave=30.0),
+        cases.build("supercell", 12, 16, 18, cphopt=2, wbc=7, gwave=30.0),
+        cases.build("vortex", 20, 18, 16, cphopt=0),
+        cases.build("vortex", 13, 11, 14, cphopt=2),
+        cases.build("bubble", 18, 14, 12, cphopt=0, advopt=2),
+        cases.build("supercell", 14, 14, 22, cphopt=0),
+    ]
+
+
+def test_uneven_members_match_the_oracle():
+    """Eight uneven members stepped together (a cold-start step, then one
+    graph cycle of six steps) against the oracle run on each member alone:
+    dry members bitwise equal to the FP32 oracle over the interior; members
+    with microphysics within the large step's microphysics bar (err vs the
+    FP64 oracle <= max(1e-5, 2 x the FP32 oracle's own err))."""
+    cs = uneven_members()
+    sols = []
+    for c in cs:
+        s = host.Solver(c.cfg)
+        c.load(s)
+        sols.append(s)
+    ens = host.Ensemble(sols)
+    ens.step(1, first=True)
+    ens.step(6)
+    ens.sync()
+    nsteps = 7
+    for c, s in zip(cs, sols):
+        cfg = c.cfg
+        cut = lambda a: a[2:cfg.ni - 1, 2:cfg.nj - 1, 1:cfg.nk - 2]
+        m32 = po.Model(c, 32)
+        m32.step(nsteps, first=True)
+        if cfg.cphopt == 0:
+            for f in ("u", "v", "w", "pp", "ptp", "qv"):
+                np.testing.assert_array_equal(cut(s.download(f, 1)), cut(m32.get(f, 1)),
+                                              err_msg="%s %dx%dx%d %s" % (c.name, cfg.ni, cfg.nj, cfg.nk, f))
+        else:
+            m64 = po.Model(c, 64)
+            m64.step(nsteps, first=True)
+            for f in FIELDS:
+                x64 = cut(m64.get(f, 1)).astype(np.float64)
+                sc = max(np.abs(x64).max(), 1e-30)
+                eg = np.abs(cut(s.download(f, 1)) - x64).max() / sc
+                eo = np.abs(cut(m32.get(f, 1)) - x64).max() / sc
+                assert eg <= max(1e-5, 2.0 * eo), (c.name, f, eg, eo)
+    ens.close()
+    for s in sols:
+        s.close()
