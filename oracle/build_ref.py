@@ -194,7 +194,8 @@ def build_mpi(prec=32, jobs=8):
     files["defmpi"] = stage_defmpi(files, os.path.join(obj, "src", "defmpi.f90"))
     h1 = os.path.join(HERE, "ref_harness.f90")
     h2 = os.path.join(HERE, "ref_harness_mpi.f90")
-    names = closure(harness_modules(h1) + harness_modules(h2), mod2file, uses)
+    h3 = os.path.join(HERE, "ref_harness_files.f90")   # linked by build_dropin
+    names = closure(harness_modules(h1) + harness_modules(h2) + harness_modules(h3), mod2file, uses)
     flags = ["-O2", "-fPIC", "-ffp-contract=off", "-module-dir", obj, "-I", obj, "-I", MPI_INC]
     if prec == 64:
         flags.append("-fdefault-real-8")
@@ -245,7 +246,8 @@ def build_dropin(names=DROPINS, jobs=8):
     fdir = os.path.join(repo, "cress_amd", "fortran")
     flags = ["-O2", "-fPIC", "-ffp-contract=off", "-I", objm, "-I", objd, "-I", MPI_INC, "-module-dir", objd]
     run([FLANG, "-c"] + flags + [os.path.join(fdir, "m_cress_hip.f90"), "-o", os.path.join(objd, "m_cress_hip.o")])
-    dobjs = [os.path.join(objd, "m_cress_hip.o")]
+    run([FLANG, "-c"] + flags + [os.path.join(fdir, "m_cress_files.f90"), "-o", os.path.join(objd, "m_cress_files.o")])
+    dobjs = [os.path.join(objd, "m_cress_hip.o"), os.path.join(objd, "m_cress_files.o")]
     for n in names:
         o = os.path.join(objd, n + ".o")
         run([FLANG, "-c"] + flags + [os.path.join(fdir, "dropin", n + ".f90"), "-o", o])
@@ -253,15 +255,21 @@ def build_dropin(names=DROPINS, jobs=8):
     shim = os.path.join(objd, "dropin_cpu.o")
     run(["gcc", "-c", "-O2", "-fPIC", "-ffp-contract=off", "-I", os.path.join(repo, "include"), "-I", HERE,
          os.path.join(HERE, "dropin_cpu.c"), "-o", shim])
+    mock = os.path.join(objd, "ctx_cpu.o")
+    run(["gcc", "-c", "-O2", "-fPIC", "-I", os.path.join(repo, "include"), os.path.join(HERE, "ctx_cpu.c"),
+         "-o", mock])
     h1 = os.path.join(HERE, "ref_harness.f90")
     h2 = os.path.join(HERE, "ref_harness_mpi.f90")
-    closure_names = closure(harness_modules(h1) + harness_modules(h2), mod2file, uses)
+    h3 = os.path.join(HERE, "ref_harness_files.f90")
+    h3o = os.path.join(objd, "ref_harness_files.o")
+    run([FLANG, "-c"] + flags + [h3, "-o", h3o])
+    closure_names = closure(harness_modules(h1) + harness_modules(h2) + harness_modules(h3), mod2file, uses)
     replaced = {mod2file["m_" + n] for n in names}
     robjs = [os.path.join(objm, n + ".o") for g in levels(closure_names, mod2file, uses) for n in g
              if n not in replaced]
     hobjs = [os.path.join(objm, "ref_harness.o"), os.path.join(objm, "ref_harness_mpi.o")]
     lib = os.path.join(OUT, "libcressrefdrop32.so")
-    run([FLANG, "-shared", "-fPIC", "-o", lib] + hobjs + robjs + dobjs + [shim] +
+    run([FLANG, "-shared", "-fPIC", "-o", lib] + hobjs + [h3o] + robjs + dobjs + [shim, mock] +
         ["-L" + OUT, "-Wl,-rpath," + OUT, "-loracle32", "-lm",
          "-L" + MPI_LIB, "-Wl,-rpath," + MPI_LIB, "-lmpifort", "-lmpi"])
     print("built", lib, "with the drop-ins", ", ".join(names))

@@ -1,0 +1,437 @@
+! Test-infrastructure harness, part 3: the restart file of the Fortran host
+! (cress_amd/fortran/m_cress_files.f90: write_restart, read_restart) next to
+! the reference's own writer and reader, s_outres (outres.f90:102) and
+! s_rdres (rdres.f90:100, processing type 'skip ', as inivar.f90:725 calls
+! it for iniopt=12).  The host's routines move fields through cress_upload
+! and cress_download, answered here by a host-memory stand-in
+! (oracle/ctx_cpu.c), so everything runs on the CPU.  Linked only into
+! oracle/_ref/libcressrefdrop32.so.
+!
+! Arrays: base(0:ni+1,0:nj+1,1:nk,1:5) = ubr, vbr, pbr, ptbr, qvbr;
+! lev(0:ni+1,0:nj+1,1:nk,1:12,1:2) = u, v, w, pp, ptp, qv, qc, qr, qi, qs,
+! qg, nci of the past (1) and present (2) level; prwtr(0:ni+1,0:nj+1,1:2,
+! 1:2), price(0:ni+1,0:nj+1,1:2,1:3).  Options come from ref_configure
+! (sides, advopt, cphopt); the rest of what s_outres reads is set here:
+! sfcopt = qcgopt = aslopt = trkopt = tubopt = diaopt = 0.  The open-side
+! phase speeds are the ones the host's phase_pair forms (the same arrays go
+! to both writers); their values are checked elsewhere, against s_phvuvw
+! and s_phvs.
+
+      module cress_ref_files
+
+      use iso_c_binding
+      use m_comindx
+      use m_comname
+      use m_comionum
+      use m_commpi, only: ngrp
+      use m_outres
+      use m_rdres
+      use m_cress_files, only: write_restart, read_restart, phase_pair
+      use m_cress_hip, only: cress_upload, cress_download, fid_ubr,     &
+     &  fid_u, fid_prwtr, fid_price
+
+      implicit none
+
+      integer, parameter, private :: i8 = selected_int_kind(18)
+
+      interface
+        function or_mock_create(ni, nj, nk) bind(C, name='or_mock_create')
+          import :: c_int, c_ptr
+          integer(c_int), value :: ni, nj, nk
+          type(c_ptr) :: or_mock_create
+        end function or_mock_create
+        subroutine or_mock_destroy(p) bind(C, name='or_mock_destroy')
+          import :: c_ptr
+          type(c_ptr), value :: p
+        end subroutine or_mock_destroy
+      end interface
+
+      contains
+
+      function fstr(s, n) result(r)
+      character(kind=c_char), intent(in) :: s(*)
+      integer(c_int), intent(in) :: n
+      character(len=n) :: r
+      integer :: i
+      do i = 1, n
+        r(i:i) = s(i)
+      end do
+      end function fstr
+
+! The host's writer on a stand-in context holding the given fields.
+      subroutine pin_write_ours(ni,nj,nk,moist,cphopt,opn,dx,dy,gwave,   &
+     &   dtb,dts,fn,nfn,base,lev,prwtr,price)                            &
+     &   bind(C,name='pin_write_ours')
+      integer(c_int), value :: ni,nj,nk,moist,cphopt,opn,nfn
+      real(c_float), value :: dx,dy,gwave,dtb,dts
+      character(kind=c_char), intent(in) :: fn(*)
+      real(c_float), intent(in) :: base(0:ni+1,0:nj+1,1:nk,1:5)
+      real(c_float), intent(in) :: lev(0:ni+1,0:nj+1,1:nk,1:12,1:2)
+      real(c_float), intent(in) :: prwtr(0:ni+1,0:nj+1,1:2,1:2)
+      real(c_float), intent(in) :: price(0:ni+1,0:nj+1,1:2,1:3)
+      type(c_ptr) :: ctx
+      integer :: f, l
+      integer(c_int) :: rc
+      ctx = or_mock_create(ni, nj, nk)
+      rc = 0
+      do f = 1, 5
+        rc = rc + cress_upload(ctx, fid_ubr + f - 1, 0, base(0,0,1,f))
+      end do
+      do l = 1, 2
+        do f = 1, 12
+          rc = rc + cress_upload(ctx, fid_u + f - 1, l - 1,             &
+     &                           lev(0,0,1,f,l))
+        end do
+      end do
+      rc = rc + cress_upload(ctx, fid_prwtr, 0, prwtr)
+      rc = rc + cress_upload(ctx, fid_price, 0, price)
+      if(rc /= 0) stop 'pin_write_ours: upload'
+      call write_restart(ctx, fstr(fn, nfn), ni, nj, nk, moist /= 0,     &
+     &  cphopt, opn /= 0, dx, dy, gwave, dtb, dts)
+      call or_mock_destroy(ctx)
+      end subroutine pin_write_ours
+
+! The host's reader into a stand-in context; the fields it set come back
+! in base, lev, prwtr, price (moist: 1 for a moist file).
+      subroutine pin_read_ours(ni,nj,nk,cphopt,opn,dx,dy,gwave,dtb,dts,  &
+     &   fn,nfn,moist,base,lev,prwtr,price) bind(C,name='pin_read_ours')
+      integer(c_int), value :: ni,nj,nk,cphopt,opn,nfn
+      real(c_float), value :: dx,dy,gwave,dtb,dts
+      character(kind=c_char), intent(in) :: fn(*)
+      integer(c_int), intent(out) :: moist
+      real(c_float), intent(inout) :: base(0:ni+1,0:nj+1,1:nk,1:5)
+      real(c_float), intent(inout) :: lev(0:ni+1,0:nj+1,1:nk,1:12,1:2)
+      real(c_float), intent(inout) :: prwtr(0:ni+1,0:nj+1,1:2,1:2)
+      real(c_float), intent(inout) :: price(0:ni+1,0:nj+1,1:2,1:3)
+      type(c_ptr) :: ctx
+      logical :: wet
+      integer :: f, l
+      integer(c_int) :: rc
+      ctx = or_mock_create(ni, nj, nk)
+      call read_restart(ctx, fstr(fn, nfn), ni, nj, nk, cphopt,         &
+     &  opn /= 0, dx, dy, gwave, dtb, dts, wet)
+      moist = 0
+      if(wet) moist = 1
+      do f = 1, 5
+        rc = cress_download(ctx, fid_ubr + f - 1, 0, base(0,0,1,f))
+      end do
+      do l = 1, 2
+        do f = 1, 12
+          rc = cress_download(ctx, fid_u + f - 1, l - 1, lev(0,0,1,f,l))
+        end do
+      end do
+      rc = cress_download(ctx, fid_prwtr, 0, prwtr)
+      rc = cress_download(ctx, fid_price, 0, price)
+      call or_mock_destroy(ctx)
+      end subroutine pin_read_ours
+
+! Name tables s_outres and s_rdres read for a restart at ctime (s) under
+! <dir>pin.
+      subroutine set_names(dir, ndir, ctime_s)
+      character(len=*), intent(in) :: dir
+      integer, intent(in) :: ndir
+      real, intent(in) :: ctime_s
+      integer :: i
+! the table of free unit numbers getunit/putunit draw from, as s_allociot
+! leaves it at one rank (allociot.f90: nio = nsub+1 units from 11)
+! one group domain (xgroup = ygroup = 1): the file name
+! <exprim>res<ctime>.pe<sub>.bin (outres.f90:777-782)
+      ngrp = 1
+      if(.not. allocated(iolst)) then
+        nio = 2
+        allocate(iolst(1:nio))
+        do i = 1, nio
+          iolst(i) = 10 + i
+        end do
+      end if
+      cname(idexprim) = ' '
+      cname(idexprim)(1:3) = 'pin'
+      iname(idncexp) = 3
+      cname(idcrsdir) = ' '
+      cname(idcrsdir)(1:ndir) = dir
+      iname(idnccrs) = ndir
+      rname(idstime) = ctime_s
+      rname(idetime) = ctime_s
+      rname(idresitv) = ctime_s
+      iname(idsfcopt) = 0
+      iname(idqcgopt) = 0
+      iname(idaslopt) = 0
+      iname(idtrkopt) = 0
+      iname(idtubopt) = 0
+      iname(iddiaopt) = 0
+! no maximum-velocity record (dmpvar(12:12) other than '+' or '-')
+      cname(iddmpvar) = ' '
+      cname(iddmpvar)(1:16) = 'xxxxxxxxxxxxxxxx'
+      end subroutine set_names
+
+      subroutine species(cphopt, nqw, nnw, nqi, nni)
+      integer, intent(in) :: cphopt
+      integer, intent(out) :: nqw, nnw, nqi, nni
+! setdim.f90:205-229, haiopt=0
+      nqw = 1
+      if(cphopt /= 0) nqw = 2
+      nnw = 1
+      if(cphopt > 1) nnw = nqw
+      nqi = 1
+      nni = 1
+      if(cphopt > 1) then
+        nqi = 3
+        nni = 3
+      end if
+      end subroutine species
+
+! The reference's s_outres on the same fields: writes
+! <dir>pinres<ctime, i8.8>.pe0000.bin (and the checking file).
+      subroutine ref_outres_pin(ni,nj,nk,moist,cphopt,dx,dy,gwave,dtb,   &
+     &   dts,dir,ndir,ctime_s,base,lev,prwtr,price)                      &
+     &   bind(C,name='ref_outres_pin')
+      integer(c_int), value :: ni,nj,nk,moist,cphopt,ndir
+      real(c_float), value :: dx,dy,gwave,dtb,dts,ctime_s
+      character(kind=c_char), intent(in) :: dir(*)
+      real(c_float), intent(in) :: base(0:ni+1,0:nj+1,1:nk,1:5)
+      real(c_float), intent(in) :: lev(0:ni+1,0:nj+1,1:nk,1:12,1:2)
+      real(c_float), intent(in) :: prwtr(0:ni+1,0:nj+1,1:2,1:2)
+      real(c_float), intent(in) :: price(0:ni+1,0:nj+1,1:2,1:3)
+      integer :: nqw, nnw, nqi, nni, nqa(0:4), nund, n
+      real, allocatable :: qwtr(:,:,:,:), qwtrp(:,:,:,:), nwtr(:,:,:,:)
+      real, allocatable :: qice(:,:,:,:), qicep(:,:,:,:), nice(:,:,:,:)
+      real, allocatable :: nicep(:,:,:,:), zw(:,:,:,:), zi(:,:,:,:)
+      real, allocatable :: za(:,:,:,:), z3(:,:,:), z2(:,:), zu(:,:,:)
+      real, allocatable :: cx(:,:,:,:), cy(:,:,:,:)
+      real, allocatable :: pw(:,:,:,:), pr(:,:,:,:)
+      character(len=5) :: fmois
+      integer(i8) :: ctime
+
+      call set_names(fstr(dir, ndir), ndir, ctime_s)
+      call species(cphopt, nqw, nnw, nqi, nni)
+      nqa(0:4) = 0
+      nund = 1
+      allocate(qwtr(0:ni+1,0:nj+1,nk,nqw), qwtrp(0:ni+1,0:nj+1,nk,nqw))
+      allocate(nwtr(0:ni+1,0:nj+1,nk,nnw))
+      allocate(qice(0:ni+1,0:nj+1,nk,nqi), qicep(0:ni+1,0:nj+1,nk,nqi))
+      allocate(nice(0:ni+1,0:nj+1,nk,nni), nicep(0:ni+1,0:nj+1,nk,nni))
+      allocate(zw(0:ni+1,0:nj+1,nk,max(nqw,nnw)))
+      allocate(zi(0:ni+1,0:nj+1,nk,max(nqi,nni)))
+      allocate(za(0:ni+1,0:nj+1,nk,1), z3(0:ni+1,0:nj+1,nk))
+      allocate(z2(0:ni+1,0:nj+1), zu(0:ni+1,0:nj+1,nund))
+      allocate(pw(0:ni+1,0:nj+1,2,nqw), pr(0:ni+1,0:nj+1,2,nqi))
+! per field: the phase-speed pair the host writes for it (1 u, 2 v, 3 w,
+! 4 p, 5 the large-step scalars)
+      allocate(cx(nj,nk,2,5), cy(ni,nk,2,5))
+      do n = 1, 5
+        call phase_pair(n, ni, nj, nk, dx, dy, gwave, dtb, dts,          &
+     &                  cx(:,:,:,n), cy(:,:,:,n))
+      end do
+      qwtr = 0.e0
+      qwtrp = 0.e0
+      nwtr = 0.e0
+      qice = 0.e0
+      qicep = 0.e0
+      nice = 0.e0
+      nicep = 0.e0
+      zw = 0.e0
+      zi = 0.e0
+      za = 0.e0
+      z3 = 0.e0
+      z2 = 0.e0
+      zu = 0.e0
+      pw = 0.e0
+      pr = 0.e0
+      if(cphopt /= 0) then
+        qwtr(:,:,:,1) = lev(:,:,:,7,2)
+        qwtr(:,:,:,2) = lev(:,:,:,8,2)
+        qwtrp(:,:,:,1) = lev(:,:,:,7,1)
+        qwtrp(:,:,:,2) = lev(:,:,:,8,1)
+        pw(:,:,:,1:2) = prwtr(:,:,:,1:2)
+      end if
+      if(cphopt > 1) then
+        do n = 1, 3
+          qice(:,:,:,n) = lev(:,:,:,8+n,2)
+          qicep(:,:,:,n) = lev(:,:,:,8+n,1)
+        end do
+        nice(:,:,:,1) = lev(:,:,:,12,2)
+        nicep(:,:,:,1) = lev(:,:,:,12,1)
+        pr(:,:,:,1:3) = price(:,:,:,1:3)
+      end if
+      fmois = 'dry  '
+      if(moist /= 0) fmois = 'moist'
+      ctime = int(ctime_s + .1e0, i8) * 1000_i8
+      call s_outres(idexprim,idcrsdir,iddmpvar,idncexp,idnccrs,         &
+     &  idwbc,idebc,idsbc,idnbc,idsfcopt,idadvopt,idcphopt,idqcgopt,     &
+     &  idaslopt,idtrkopt,idtubopt,iddiaopt,idetime,idresitv,fmois,      &
+     &  ctime,ni,nj,nk,nqw,nnw,nqi,nni,nqa,nund,                         &
+     &  base(:,:,:,1),base(:,:,:,2),base(:,:,:,3),base(:,:,:,4),         &
+     &  base(:,:,:,5),                                                   &
+     &  lev(:,:,:,1,2),lev(:,:,:,1,1),lev(:,:,:,2,2),lev(:,:,:,2,1),     &
+     &  lev(:,:,:,3,2),lev(:,:,:,3,1),lev(:,:,:,4,2),lev(:,:,:,4,1),     &
+     &  lev(:,:,:,5,2),lev(:,:,:,5,1),lev(:,:,:,6,2),lev(:,:,:,6,1),     &
+     &  qwtr,qwtrp,nwtr,nwtr,qice,qicep,nice,nicep,                      &
+     &  zw,zw,zi,zi,za,za,z3,z3,z3,z3,                                   &
+     &  cx(:,:,:,1),cy(:,:,:,1),cx(:,:,:,2),cy(:,:,:,2),                 &
+     &  cx(:,:,:,3),cy(:,:,:,3),cx(:,:,:,4),cy(:,:,:,4),                 &
+     &  cx(:,:,:,5),cy(:,:,:,5),cx(:,:,:,5),cy(:,:,:,5),                 &
+     &  spread(cx(:,:,:,5),4,nqw),spread(cy(:,:,:,5),4,nqw),             &
+     &  spread(cx(:,:,:,5),4,nnw),spread(cy(:,:,:,5),4,nnw),             &
+     &  spread(cx(:,:,:,5),4,nqi),spread(cy(:,:,:,5),4,nqi),             &
+     &  spread(cx(:,:,:,5),4,nni),spread(cy(:,:,:,5),4,nni),             &
+     &  spread(cx(:,:,:,5),4,nqw),spread(cy(:,:,:,5),4,nqw),             &
+     &  spread(cx(:,:,:,5),4,nqi),spread(cy(:,:,:,5),4,nqi),             &
+     &  spread(cx(:,:,:,5),4,1),spread(cy(:,:,:,5),4,1),                 &
+     &  cx(:,:,:,5),cy(:,:,:,5),cx(:,:,:,5),cy(:,:,:,5),                 &
+     &  z3,pw,pr,z3,z2,z2,zu,zu)
+      end subroutine ref_outres_pin
+
+! The reference's s_rdres reading <dir>pinres<ctime>.pe0000.bin into base,
+! lev, prwtr, price; with open sides (opn), same_speeds = 1 when every
+! phase-speed array it read equals the pair the host forms for that field
+! (moist: 1 for 'moist').
+      subroutine ref_rdres_pin(ni,nj,nk,cphopt,opn,dx,dy,gwave,dtb,dts,  &
+     &   dir,ndir,ctime_s,moist,same_speeds,base,lev,prwtr,price)        &
+     &   bind(C,name='ref_rdres_pin')
+      integer(c_int), value :: ni,nj,nk,cphopt,opn,ndir
+      real(c_float), value :: dx,dy,gwave,dtb,dts,ctime_s
+      character(kind=c_char), intent(in) :: dir(*)
+      integer(c_int), intent(out) :: moist, same_speeds
+      real(c_float), intent(inout) :: base(0:ni+1,0:nj+1,1:nk,1:5)
+      real(c_float), intent(inout) :: lev(0:ni+1,0:nj+1,1:nk,1:12,1:2)
+      real(c_float), intent(inout) :: prwtr(0:ni+1,0:nj+1,1:2,1:2)
+      real(c_float), intent(inout) :: price(0:ni+1,0:nj+1,1:2,1:3)
+      integer :: nqw, nnw, nqi, nni, nqa(0:4), nund, n, f, l
+      real, allocatable :: a3(:,:,:,:,:), qw(:,:,:,:,:), nw(:,:,:,:,:)
+      real, allocatable :: qi(:,:,:,:,:), ni3(:,:,:,:,:)
+      real, allocatable :: zw(:,:,:,:), zi(:,:,:,:), za(:,:,:,:)
+      real, allocatable :: z3(:,:,:), z2(:,:), zu(:,:,:)
+      real, allocatable :: s1(:,:,:,:), s2(:,:,:,:)
+      real, allocatable :: w1x(:,:,:,:), w1y(:,:,:,:), w2x(:,:,:,:)
+      real, allocatable :: w2y(:,:,:,:), i1x(:,:,:,:), i1y(:,:,:,:)
+      real, allocatable :: i2x(:,:,:,:), i2y(:,:,:,:)
+      real, allocatable :: cx(:,:,:,:), cy(:,:,:,:)
+      real, allocatable :: pw(:,:,:,:), pr(:,:,:,:), bb(:,:,:,:)
+      real, allocatable :: zx(:,:,:,:), zy(:,:,:,:), ax(:,:,:,:)
+      real, allocatable :: ay(:,:,:,:)
+      character(len=5) :: fmois
+      logical :: same
+
+      call set_names(fstr(dir, ndir), ndir, ctime_s)
+      call species(cphopt, nqw, nnw, nqi, nni)
+      nqa(0:4) = 0
+      nund = 1
+      allocate(a3(0:ni+1,0:nj+1,nk,6,2), bb(0:ni+1,0:nj+1,nk,5))
+      allocate(qw(0:ni+1,0:nj+1,nk,nqw,2), nw(0:ni+1,0:nj+1,nk,nnw,2))
+      allocate(qi(0:ni+1,0:nj+1,nk,nqi,2), ni3(0:ni+1,0:nj+1,nk,nni,2))
+      allocate(zw(0:ni+1,0:nj+1,nk,max(nqw,nnw)))
+      allocate(zi(0:ni+1,0:nj+1,nk,max(nqi,nni)))
+      allocate(za(0:ni+1,0:nj+1,nk,1), z3(0:ni+1,0:nj+1,nk))
+      allocate(z2(0:ni+1,0:nj+1), zu(0:ni+1,0:nj+1,nund))
+      allocate(s1(nj,nk,2,7), s2(ni,nk,2,7))
+      allocate(w1x(nj,nk,2,nqw), w1y(ni,nk,2,nqw), w2x(nj,nk,2,nnw))
+      allocate(w2y(ni,nk,2,nnw), i1x(nj,nk,2,nqi), i1y(ni,nk,2,nqi))
+      allocate(i2x(nj,nk,2,nni), i2y(ni,nk,2,nni))
+      allocate(zx(nj,nk,2,max(nqw,nqi)), zy(ni,nk,2,max(nqw,nqi)))
+      allocate(ax(nj,nk,2,1), ay(ni,nk,2,1))
+      allocate(pw(0:ni+1,0:nj+1,2,nqw), pr(0:ni+1,0:nj+1,2,nqi))
+      allocate(cx(nj,nk,2,5), cy(ni,nk,2,5))
+      a3 = 0.e0
+      bb = 0.e0
+      qw = 0.e0
+      nw = 0.e0
+      qi = 0.e0
+      ni3 = 0.e0
+      zw = 0.e0
+      zi = 0.e0
+      za = 0.e0
+      z3 = 0.e0
+      z2 = 0.e0
+      zu = 0.e0
+      s1 = 0.e0
+      s2 = 0.e0
+      w1x = 0.e0
+      w1y = 0.e0
+      w2x = 0.e0
+      w2y = 0.e0
+      i1x = 0.e0
+      i1y = 0.e0
+      i2x = 0.e0
+      i2y = 0.e0
+      zx = 0.e0
+      zy = 0.e0
+      ax = 0.e0
+      ay = 0.e0
+      pw = 0.e0
+      pr = 0.e0
+      call s_rdres(idexprim,idcrsdir,iddmpvar,idncexp,idnccrs,          &
+     &  idwbc,idebc,idsbc,idnbc,idsfcopt,idadvopt,idcphopt,idqcgopt,     &
+     &  idaslopt,idtrkopt,idtubopt,iddiaopt,idstime,'skip ',fmois,       &
+     &  ni,nj,nk,nqw,nnw,nqi,nni,nqa,nund,                               &
+     &  bb(:,:,:,1),bb(:,:,:,2),bb(:,:,:,3),bb(:,:,:,4),bb(:,:,:,5),     &
+     &  a3(:,:,:,1,2),a3(:,:,:,1,1),a3(:,:,:,2,2),a3(:,:,:,2,1),         &
+     &  a3(:,:,:,3,2),a3(:,:,:,3,1),a3(:,:,:,4,2),a3(:,:,:,4,1),         &
+     &  a3(:,:,:,5,2),a3(:,:,:,5,1),a3(:,:,:,6,2),a3(:,:,:,6,1),         &
+     &  qw(:,:,:,:,2),qw(:,:,:,:,1),nw(:,:,:,:,2),nw(:,:,:,:,1),         &
+     &  qi(:,:,:,:,2),qi(:,:,:,:,1),ni3(:,:,:,:,2),ni3(:,:,:,:,1),       &
+     &  zw,zw,zi,zi,za,za,z3,z3,z3,z3,                                   &
+     &  s1(:,:,:,1),s2(:,:,:,1),s1(:,:,:,2),s2(:,:,:,2),                 &
+     &  s1(:,:,:,3),s2(:,:,:,3),s1(:,:,:,4),s2(:,:,:,4),                 &
+     &  s1(:,:,:,5),s2(:,:,:,5),s1(:,:,:,6),s2(:,:,:,6),                 &
+     &  w1x,w1y,w2x,w2y,i1x,i1y,i2x,i2y,zx,zy,zx,zy,ax,ay,               &
+     &  s1(:,:,:,7),s2(:,:,:,7),s1(:,:,:,7),s2(:,:,:,7),                 &
+     &  z3,pw,pr,z3,z2,z2,zu,zu)
+      moist = 0
+      if(fmois == 'moist') moist = 1
+      base(:,:,:,1:5) = bb(:,:,:,1:5)
+      do l = 1, 2
+        do f = 1, 6
+          lev(:,:,:,f,l) = a3(:,:,:,f,l)
+        end do
+        if(cphopt /= 0) then
+          lev(:,:,:,7,l) = qw(:,:,:,1,l)
+          lev(:,:,:,8,l) = qw(:,:,:,2,l)
+        end if
+        if(cphopt > 1) then
+          do n = 1, 3
+            lev(:,:,:,8+n,l) = qi(:,:,:,n,l)
+          end do
+          lev(:,:,:,12,l) = ni3(:,:,:,1,l)
+        end if
+      end do
+      if(cphopt /= 0) prwtr(:,:,:,1:2) = pw(:,:,:,1:2)
+      if(cphopt > 1) price(:,:,:,1:3) = pr(:,:,:,1:3)
+! the speeds read against the host's pairs (u, v, w, p, pt, then qv and
+! every species with the large-step scalars' pair)
+      do n = 1, 5
+        call phase_pair(n, ni, nj, nk, dx, dy, gwave, dtb, dts,          &
+     &                  cx(:,:,:,n), cy(:,:,:,n))
+      end do
+      same = all(s1(:,:,:,1) == cx(:,:,:,1)) .and.                      &
+     &       all(s2(:,:,:,1) == cy(:,:,:,1)) .and.                      &
+     &       all(s1(:,:,:,2) == cx(:,:,:,2)) .and.                      &
+     &       all(s2(:,:,:,2) == cy(:,:,:,2)) .and.                      &
+     &       all(s1(:,:,:,3) == cx(:,:,:,3)) .and.                      &
+     &       all(s2(:,:,:,3) == cy(:,:,:,3)) .and.                      &
+     &       all(s1(:,:,:,4) == cx(:,:,:,4)) .and.                      &
+     &       all(s2(:,:,:,4) == cy(:,:,:,4)) .and.                      &
+     &       all(s1(:,:,:,5) == cx(:,:,:,5)) .and.                      &
+     &       all(s2(:,:,:,5) == cy(:,:,:,5))
+      if(moist /= 0) then
+        same = same .and. all(s1(:,:,:,6) == cx(:,:,:,5)) .and.         &
+     &                    all(s2(:,:,:,6) == cy(:,:,:,5))
+      end if
+      if(moist /= 0 .and. cphopt /= 0) then
+        do n = 1, nqw
+          same = same .and. all(w1x(:,:,:,n) == cx(:,:,:,5)) .and.      &
+     &                      all(w1y(:,:,:,n) == cy(:,:,:,5))
+        end do
+      end if
+      if(moist /= 0 .and. cphopt > 1) then
+        do n = 1, nqi
+          same = same .and. all(i1x(:,:,:,n) == cx(:,:,:,5)) .and.      &
+     &                      all(i1y(:,:,:,n) == cy(:,:,:,5))
+        end do
+        same = same .and. all(i2x(:,:,:,1) == cx(:,:,:,5)) .and.        &
+     &                    all(i2y(:,:,:,1) == cy(:,:,:,5))
+      end if
+      same_speeds = 0
+      if(same .or. opn == 0) same_speeds = 1
+      end subroutine ref_rdres_pin
+
+      end module cress_ref_files
