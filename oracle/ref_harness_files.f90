@@ -1,8 +1,10 @@
-! Test-infrastructure harness, part 3: the restart file of the Fortran host
-! (cress_amd/fortran/m_cress_files.f90: write_restart, read_restart) next to
-! the reference's own writer and reader, s_outres (outres.f90:102) and
-! s_rdres (rdres.f90:100, processing type 'skip ', as inivar.f90:725 calls
-! it for iniopt=12).  The host's routines move fields through cress_upload
+! Test-infrastructure harness, part 3: the restart and dump files of the
+! Fortran host (cress_amd/fortran/m_cress_files.f90: write_restart,
+! read_restart, write_dump) next to the reference's own writers and reader:
+! s_outres (outres.f90:102), s_rdres (rdres.f90:100, processing type
+! 'skip ', as inivar.f90:725 calls it for iniopt=12), and s_opendmp
+! (opendmp.f90:95), s_outdmp (outdmp.f90:102), s_closedmp (closedmp.f90:82)
+! for dumps.  The host's routines move fields through cress_upload
 ! and cress_download, answered here by a host-memory stand-in
 ! (oracle/ctx_cpu.c), so everything runs on the CPU.  Linked only into
 ! oracle/_ref/libcressrefdrop32.so.
@@ -26,7 +28,11 @@
       use m_commpi, only: ngrp
       use m_outres
       use m_rdres
-      use m_cress_files, only: write_restart, read_restart, phase_pair
+      use m_opendmp
+      use m_outdmp
+      use m_closedmp
+      use m_cress_files, only: write_restart, read_restart, phase_pair,  &
+     &  write_dump
       use m_cress_hip, only: cress_upload, cress_download, fid_ubr,     &
      &  fid_u, fid_prwtr, fid_price
 
@@ -433,5 +439,122 @@
       same_speeds = 0
       if(same .or. opn == 0) same_speeds = 1
       end subroutine ref_rdres_pin
+
+! The host's dump writer on a stand-in context holding the base state and
+! one level (uploaded as the present level) of the prognostics.
+      subroutine pin_dump_ours(ni,nj,nk,moist,cphopt,dmpvar,fn,nfn,base, &
+     &   lev,prwtr,price) bind(C,name='pin_dump_ours')
+      integer(c_int), value :: ni,nj,nk,moist,cphopt,nfn
+      character(kind=c_char), intent(in) :: dmpvar(16), fn(*)
+      real(c_float), intent(in) :: base(0:ni+1,0:nj+1,1:nk,1:5)
+      real(c_float), intent(in) :: lev(0:ni+1,0:nj+1,1:nk,1:12)
+      real(c_float), intent(in) :: prwtr(0:ni+1,0:nj+1,1:2,1:2)
+      real(c_float), intent(in) :: price(0:ni+1,0:nj+1,1:2,1:3)
+      type(c_ptr) :: ctx
+      integer :: f
+      integer(c_int) :: rc
+      ctx = or_mock_create(ni, nj, nk)
+      rc = 0
+      do f = 1, 5
+        rc = rc + cress_upload(ctx, fid_ubr + f - 1, 0, base(0,0,1,f))
+      end do
+      do f = 1, 12
+        rc = rc + cress_upload(ctx, fid_u + f - 1, 1, lev(0,0,1,f))
+      end do
+      rc = rc + cress_upload(ctx, fid_prwtr, 0, prwtr)
+      rc = rc + cress_upload(ctx, fid_price, 0, price)
+      if(rc /= 0) stop 'pin_dump_ours: upload'
+      call write_dump(ctx, fstr(fn, nfn), ni, nj, nk, moist /= 0,        &
+     &  cphopt, fstr(dmpvar, 16), 1)
+      call or_mock_destroy(ctx)
+      end subroutine pin_dump_ours
+
+! The reference's s_opendmp, s_outdmp and s_closedmp for the same fields
+! (dmpfmt=2 direct access, dmplev=1, dmpmon=0, a word of 4 bytes): writes
+! <dir>pindmp<ctime, i8.8>.pe0000.bin (and the checking file).  Flat
+! terrain: zph(k) = (k-2) dz.
+      subroutine ref_dump_pin(ni,nj,nk,moist,cphopt,dz,dmpvar,dir,ndir,   &
+     &   ctime_s,base,lev,prwtr,price) bind(C,name='ref_dump_pin')
+      integer(c_int), value :: ni,nj,nk,moist,cphopt,ndir
+      real(c_float), value :: dz,ctime_s
+      character(kind=c_char), intent(in) :: dmpvar(16), dir(*)
+      real(c_float), intent(in) :: base(0:ni+1,0:nj+1,1:nk,1:5)
+      real(c_float), intent(in) :: lev(0:ni+1,0:nj+1,1:nk,1:12)
+      real(c_float), intent(in) :: prwtr(0:ni+1,0:nj+1,1:2,1:2)
+      real(c_float), intent(in) :: price(0:ni+1,0:nj+1,1:2,1:3)
+      integer :: nqw, nnw, nqi, nni, nqa(0:4), n, k
+      real, allocatable :: qwtr(:,:,:,:), nwtr(:,:,:,:), qice(:,:,:,:)
+      real, allocatable :: nice(:,:,:,:), zw(:,:,:,:), zi(:,:,:,:)
+      real, allocatable :: za(:,:,:,:), z3(:,:,:), z2(:,:)
+      real, allocatable :: zph(:,:,:), zsth(:), z1d(:), wk(:,:,:,:)
+      real, allocatable :: pw(:,:,:,:), pr(:,:,:,:), mx(:,:,:)
+      character(len=5) :: fmois
+      integer(i8) :: ctime
+
+      call set_names(fstr(dir, ndir), ndir, ctime_s)
+      cname(iddmpvar) = ' '
+      cname(iddmpvar)(1:16) = fstr(dmpvar, 16)
+      iname(idwlngth) = 4
+      iname(iddmpfmt) = 2
+      iname(iddmplev) = 1
+      iname(iddmpmon) = 0
+      iname(idhaiopt) = 0
+      rname(iddmpitv) = ctime_s
+      rname(idmonitv) = ctime_s
+      rname(iddz) = dz
+      call species(cphopt, nqw, nnw, nqi, nni)
+      nqa(0:4) = 0
+      allocate(qwtr(0:ni+1,0:nj+1,nk,nqw), nwtr(0:ni+1,0:nj+1,nk,nnw))
+      allocate(qice(0:ni+1,0:nj+1,nk,nqi), nice(0:ni+1,0:nj+1,nk,nni))
+      allocate(zw(0:ni+1,0:nj+1,nk,nqw), zi(0:ni+1,0:nj+1,nk,nqi))
+      allocate(za(0:ni+1,0:nj+1,nk,1), z3(0:ni+1,0:nj+1,nk))
+      allocate(z2(0:ni+1,0:nj+1), zph(0:ni+1,0:nj+1,nk), zsth(nk))
+      allocate(z1d(nk), wk(0:ni+1,0:nj+1,nk,6), mx(0:ni+1,0:nj+1,nk))
+      allocate(pw(0:ni+1,0:nj+1,2,nqw), pr(0:ni+1,0:nj+1,2,nqi))
+      qwtr = 0.e0
+      nwtr = 0.e0
+      qice = 0.e0
+      nice = 0.e0
+      zw = 0.e0
+      zi = 0.e0
+      za = 0.e0
+      z3 = 0.e0
+      z2 = 0.e0
+      z1d = 0.e0
+      wk = 0.e0
+      mx = 0.e0
+      pw = 0.e0
+      pr = 0.e0
+      do k = 1, nk
+        zsth(k) = real(k - 2) * dz
+        zph(:,:,k) = zsth(k)
+      end do
+      if(cphopt /= 0) then
+        qwtr(:,:,:,1) = lev(:,:,:,7)
+        qwtr(:,:,:,2) = lev(:,:,:,8)
+        pw(:,:,:,1:2) = prwtr(:,:,:,1:2)
+      end if
+      if(cphopt > 1) then
+        do n = 1, 3
+          qice(:,:,:,n) = lev(:,:,:,8+n)
+        end do
+        nice(:,:,:,1) = lev(:,:,:,12)
+        pr(:,:,:,1:3) = price(:,:,:,1:3)
+      end if
+      fmois = 'dry  '
+      if(moist /= 0) fmois = 'moist'
+      ctime = int(ctime_s + .1e0, i8) * 1000_i8
+      call s_opendmp(idexprim,idcrsdir,idncexp,idnccrs,idwlngth,         &
+     &  iddmpfmt,iddmplev,iddmpmon,idetime,iddmpitv,idmonitv,iddz,ctime,  &
+     &  ni,nj,nk,zsth,z1d)
+      call s_outdmp(iddmpvar,idcphopt,idhaiopt,idqcgopt,idaslopt,        &
+     &  idtrkopt,idtubopt,iddmplev,iddz,fmois,ni,nj,nk,nqw,nnw,nqi,nni,   &
+     &  nqa,zph,zsth,z2,base(:,:,:,1),base(:,:,:,2),base(:,:,:,3),        &
+     &  base(:,:,:,4),base(:,:,:,5),lev(:,:,:,1),lev(:,:,:,2),            &
+     &  lev(:,:,:,3),lev(:,:,:,4),lev(:,:,:,5),lev(:,:,:,6),qwtr,nwtr,    &
+     &  qice,nice,zw,zi,za,z3,z3,mx,pw,pr,z1d,wk(:,:,:,1),wk(:,:,:,2),    &
+     &  wk(:,:,:,3),wk(:,:,:,4),wk(:,:,:,5),wk(:,:,:,6))
+      call s_closedmp(iddmpmon,ctime)
+      end subroutine ref_dump_pin
 
       end module cress_ref_files

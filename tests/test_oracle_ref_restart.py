@@ -114,3 +114,42 @@ def test_restart_matches_reference(tmp_path, name, kw, moist):
         np.testing.assert_array_equal(ro[-2], pw)
     if moist and sc.cphopt == 2:
         np.testing.assert_array_equal(ro[-1], pi)
+
+
+@pytest.mark.parametrize("name,kw,moist,dmpvar", [
+    ("bubble", dict(cphopt=2), 1, "oooooooxxxxxxoxx"),
+    ("supercell", dict(cphopt=2, wbc=7, gwave=30.0), 1, "---------xxxxoxx"),
+    ("vortex", dict(cphopt=0), 1, "oo-o-oxxxxxxxxxx"),
+    ("bubble", dict(cphopt=0), 0, "ooooooxxxxxxxxxx"),
+])
+def test_dump_matches_reference(tmp_path, name, kw, moist, dmpvar):
+    """The host's dump (write_dump: dmpfmt=2, dmplev=1, dmpmon=0) of one
+    time is the same bytes as the reference's s_opendmp + s_outdmp +
+    s_closedmp on the same fields: base and perturbation ('o') or their sum
+    ('-'), u, v, w averaged to the scalar points, the species, and the
+    surface precipitation (dmpvar(14:14))."""
+    case = cases.build(name, 12, 10, 14, **kw)
+    sc = case.cfg
+    cfg = po.cfg_from_solver_config(sc)
+    base, lev, pw, pi = _state(case, 57 + len(name) + sc.cphopt)
+    lv = np.asfortranarray(lev[..., 1])
+    dA, dB = str(tmp_path / "ours") + "/", str(tmp_path / "ref") + "/"
+    os.makedirs(dA)
+    os.makedirs(dB)
+    fname = "pindmp%08d.pe0000.bin" % int(CTIME)
+    fA, fB = dA + fname, dB + fname
+    dv = dmpvar.encode()
+    seq = [
+        ("pin_dump_ours", [sc.ni, sc.nj, sc.nk, moist, sc.cphopt, dv, fA.encode(), len(fA), base, lv, pw, pi]),
+        ("ref_dump_pin", [sc.ni, sc.nj, sc.nk, moist, sc.cphopt, float(sc.dz), dv, dB.encode(), len(dB),
+                          float(CTIME), base, lv, pw, pi]),
+    ]
+    refmpi.calls(cfg, seq, lib=refmpi.LIB_DROPIN)
+    a, b = open(fA, "rb").read(), open(fB, "rb").read()
+    rec = (sc.ni - 3) * (sc.nj - 3) * 4
+    assert len(b) > 0 and len(a) == len(b), (len(a), len(b), rec)
+    if a != b:
+        ra = np.frombuffer(a, np.float32).reshape(-1, rec // 4)
+        rb = np.frombuffer(b, np.float32).reshape(-1, rec // 4)
+        bad = [r for r in range(ra.shape[0]) if not np.array_equal(ra[r], rb[r])]
+        raise AssertionError("records differ: %s of %d" % (bad[:10], ra.shape[0]))
