@@ -58,7 +58,6 @@ static LaunchTune tune_default() {
   t.fs_order = 2;
   t.ff_by = 4;
   t.ff_order = 2;
-  t.fly = 0;
   return t;
 }
 
@@ -968,10 +967,7 @@ static void forcing_stage(cress_ctx *x, const LaunchTune &tn, unsigned parts) {
   // every base/metric field a per-level table: the kernels' table builds
   const bool U = (g.umask & ((1u << CRESS_NBASE) - 1u)) == ((1u << CRESS_NBASE) - 1u);
   const auto kprep = U ? k_prep<true> : k_prep<false>;
-  // with tn.fly the forcing kernels form the mass fluxes (and force_p wc)
-  // where they use them, and k_prep is not launched
-  const bool fly = tn.fly != 0;
-  if ((parts & FORCE_PREP) && !fly)
+  if (parts & FORCE_PREP)
     LAUNCH(x, KT_PREP, kprep, flat(tiles(ni + 2, nj + 2, 64, 4, nk)), b2, 0, tiles(ni + 2, nj + 2, 64, 4, nk), g, o, x->u[C], x->v[C],
                      x->w[C], B[CRESS_RST8U], B[CRESS_RST8V], B[CRESS_RST8W], B[CRESS_JCB8W], x->wc,
                      x->rstxu, x->rstxv, x->rstxwc);
@@ -1010,21 +1006,16 @@ static void forcing_stage(cress_ctx *x, const LaunchTune &tn, unsigned parts) {
   const Tiles tw = tiles(ni - 1, nj - 3, 64, tn.ff_by, nk - 2, tn.ff_order);
   const Tiles tp = tiles(ni - 1, nj - 3, 64, tn.ff_by, nk - 3, tn.ff_order);
   const Tiles ts = tiles(ni - 1, nj - 3, 64, tn.fs_by, nk - 3, tn.fs_order);
-  const auto kuv = fly ? (sm ? (U ? k_force_uv<true, true, true> : k_force_uv<true, false, true>)
-                             : (U ? k_force_uv<false, true, true> : k_force_uv<false, false, true>))
-                      : (sm ? (U ? k_force_uv<true, true, false> : k_force_uv<true, false, false>)
-                            : (U ? k_force_uv<false, true, false> : k_force_uv<false, false, false>));
-  const auto kw = fly ? (sm ? (U ? k_force_w<true, true, true> : k_force_w<true, false, true>)
-                            : (U ? k_force_w<false, true, true> : k_force_w<false, false, true>))
-                      : (sm ? (U ? k_force_w<true, true, false> : k_force_w<true, false, false>)
-                            : (U ? k_force_w<false, true, false> : k_force_w<false, false, false>));
-  const auto kp = fly ? (U ? k_force_p<true, true> : k_force_p<false, true>)
-                      : (U ? k_force_p<true, false> : k_force_p<false, false>);
+  const auto kuv = sm ? (U ? k_force_uv<true, true> : k_force_uv<true, false>)
+                      : (U ? k_force_uv<false, true> : k_force_uv<false, false>);
+  const auto kw = sm ? (U ? k_force_w<true, true> : k_force_w<true, false>)
+                     : (U ? k_force_w<false, true> : k_force_w<false, false>);
+  const auto kp = U ? k_force_p<true> : k_force_p<false>;
   if (parts & FORCE_UV) LAUNCH(x, KT_FORCE_UV, kuv, flat(tuv), bf, 0, tuv, g, o, c, fi, x->ufrc, x->vfrc);
   if (parts & FORCE_W) LAUNCH(x, KT_FORCE_W, kw, flat(tw), bf, 0, tw, g, o, c, fi, x->wfrc);
   if (parts & FORCE_P)
     LAUNCH(x, KT_FORCE_P, kp, flat(tp), bf, 0, tp, g, o, c, B[CRESS_JCB], B[CRESS_JCB8U], B[CRESS_JCB8V],
-         B[CRESS_JCB8W], x->u[C], x->v[C], fly ? x->w[C] : x->wc, x->pp[C], x->pp[P], x->rbct, x->pfrc);
+         B[CRESS_JCB8W], x->u[C], x->v[C], x->wc, x->pp[C], x->pp[P], x->rbct, x->pfrc);
   ScalarIn si;
   const int nsc = cf.cphopt == 2 ? 8 : 2;
   {
@@ -1052,16 +1043,8 @@ static void forcing_stage(cress_ctx *x, const LaunchTune &tn, unsigned parts) {
   si.ptbr = B[CRESS_PTBR];
   si.rbr = B[CRESS_RBR];
   si.w = x->w[C];
-  si.u = x->u[C];
-  si.v = x->v[C];
-  si.rst8u = B[CRESS_RST8U];
-  si.rst8v = B[CRESS_RST8V];
-  si.rst8w = B[CRESS_RST8W];
-  si.jcb8w = B[CRESS_JCB8W];
-  const auto ks = fly ? (sm ? (U ? k_force_s<true, true, true> : k_force_s<true, false, true>)
-                            : (U ? k_force_s<false, true, true> : k_force_s<false, false, true>))
-                      : (sm ? (U ? k_force_s<true, true, false> : k_force_s<true, false, false>)
-                            : (U ? k_force_s<false, true, false> : k_force_s<false, false, false>));
+  const auto ks = sm ? (U ? k_force_s<true, true> : k_force_s<true, false>)
+                     : (U ? k_force_s<false, true> : k_force_s<false, false>);
   if (parts & FORCE_S) LAUNCH(x, KT_FORCE_S, ks, flat(ts), bs, 0, ts, g, o, c, si, nsc);
 }
 
@@ -1529,11 +1512,11 @@ int cress_diag(cress_ctx *x, int level, double *out) {
 // two steps; the results do not depend on the shapes (tests show them
 // bitwise equal).  Results are cached per process by grid size and options.
 namespace {
-constexpr int kTuneInts = 10;
+constexpr int kTuneInts = 9;
 
 void tune_to_ints(const LaunchTune &t, int *v) {
-  const int a[kTuneInts] = {t.ptuv_by, t.ptuv_slots, t.ptuv_order, t.col_by,   t.col_lds,
-                            t.fs_by,   t.fs_order,   t.ff_by,      t.ff_order, t.fly};
+  const int a[kTuneInts] = {t.ptuv_by, t.ptuv_slots, t.ptuv_order, t.col_by, t.col_lds,
+                            t.fs_by,   t.fs_order,   t.ff_by,      t.ff_order};
   for (int i = 0; i < kTuneInts; i++) v[i] = a[i];
 }
 
@@ -1545,8 +1528,7 @@ bool tune_valid(const LaunchTune &t) {
   };
   return in(t.ptuv_by, {1, 2, 4, 8, 16}) && in(t.ptuv_slots, {0, 1}) && in(t.ptuv_order, {0, 1, 2}) &&
          in(t.col_by, {1, 2, 4}) && t.col_lds >= 0 && t.col_lds <= 65536 && in(t.fs_by, {1, 2, 4, 8}) &&
-         in(t.fs_order, {0, 1, 2}) && in(t.ff_by, {1, 2, 4, 8, 16}) && in(t.ff_order, {0, 1, 2}) &&
-         in(t.fly, {0, 1});
+         in(t.fs_order, {0, 1, 2}) && in(t.ff_by, {1, 2, 4, 8, 16}) && in(t.ff_order, {0, 1, 2});
 }
 
 std::mutex tune_mu;
@@ -1570,7 +1552,6 @@ int cress_set_launch(cress_ctx *x, const int *shape, int n) {
   t.fs_order = shape[6];
   t.ff_by = shape[7];
   t.ff_order = shape[8];
-  t.fly = shape[9];
   if (!tune_valid(t)) return CRESS_EINVAL;
   x->tune = t;
   x->tune_ver++;
@@ -1692,19 +1673,6 @@ int cress_autotune(cress_ctx *x, int *shape, int n) {
           best.ff_order = order;
         }
       }
-  }
-  // the whole forcing stage with and without k_prep
-  {
-    float tmin = 1e30f;
-    for (int fly : {0, 1}) {
-      LaunchTune t = best;
-      t.fly = fly;
-      const float ms = time_ms([&] { forcing_stage(x, t, FORCE_ALL); });
-      if (ms < tmin) {
-        tmin = ms;
-        best.fly = fly;
-      }
-    }
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
@@ -2341,8 +2309,7 @@ int cress_advp(const cress_config *cfg, const float *jcb8u, const float *jcb8v, 
   const LaunchTune tn = tune_default();
   const Tiles tp = tiles(g.ni - 1, g.nj - 3, 64, tn.ff_by, g.nk - 3, tn.ff_order);
   const float *pj = (const float *)a.p, *pp_ = (const float *)p.p;
-  const auto kfp = k_force_p<false, false>;
-  hipLaunchKernelGGL(kfp, flat(tp), dim3(64, tn.ff_by), 0, 0, tp, g, o, c, pj, pj,
+  hipLaunchKernelGGL(k_force_p<false>, flat(tp), dim3(64, tn.ff_by), 0, 0, tp, g, o, c, pj, pj,
                      (const float *)b.p, (const float *)d.p, (const float *)e.p, (const float *)f.p,
                      (const float *)h.p, pp_, pp_, pj, (float *)q.p);
   HIPCHK(hipGetLastError());

@@ -16,7 +16,6 @@ struct ScalarIn {
   const float *sp[8];   // past level
   float *frc[8];        // forcing outputs
   const float *rstxu, *rstxv, *rstxwc, *rst, *qvbr, *rbct, *ptbr, *rbr, *w;
-  const float *u, *v, *rst8u, *rst8v, *rst8w, *jcb8w;   // the fluxes formed where used
 };
 
 // Launch shapes.  Each stencil and column kernel takes its rows per
@@ -44,7 +43,6 @@ struct LaunchTune {
   int col_by, col_lds;                   // column kernel (LDS bytes cap occupancy)
   int fs_by, fs_order;                   // scalar forcing
   int ff_by, ff_order;                   // u/v, w and p' forcing
-  int fly;                               // 1: forcing without k_prep (fluxes formed where used)
 };
 
 struct AcPtUv {
@@ -104,19 +102,17 @@ template <bool U>
 __global__ void k_prep(Tiles T, Grid g, Opts o, const float *u, const float *v, const float *w,
                        const float *rst8u, const float *rst8v, const float *rst8w,
                        const float *jcb8w, float *wc, float *rstxu, float *rstxv, float *rstxwc);
-// SM: with the 4th-order smoothing (smtopt 2/3) compiled in; FLY: the mass
-// fluxes rstxu, rstxv, rstxwc (and wc for k_force_p) formed where used from
-// u, v, w instead of read from k_prep's arrays
-template <bool SM, bool U, bool FLY>
+// SM: with the 4th-order smoothing (smtopt 2/3) compiled in
+template <bool SM, bool U>
 __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *ufrc, float *vfrc);
-template <bool SM, bool U, bool FLY>
+template <bool SM, bool U>
 __global__ void k_force_w(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *wfrc);
-template <bool U, bool FLY>
+template <bool U>
 __global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, const float *jcb8u,
                           const float *jcb8v, const float *jcb8w, const float *u, const float *v,
                           const float *wc, const float *pp, const float *ppp, const float *rbct,
                           float *pfrc);
-template <bool SM, bool U, bool FLY>
+template <bool SM, bool U>
 __global__ void k_force_s(Tiles T, Grid g, Opts o, Consts c, ScalarIn in, int nscalar);
 // U: rst8u, rst8v, rst8w and jcb are per-level tables; SLOTS: the ptil
 // tile and its halo in slots of consecutive points (else one point per

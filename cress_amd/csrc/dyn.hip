@@ -81,45 +81,6 @@ __device__ __forceinline__ float wc_value(const Grid &g, const Opts &o, const fl
   return wc_raw<U>(g, o.sthopt, w, jcb8w, i, j, k);
 }
 
-// The mass fluxes of s_rstuvwc (rstuvwc.f90): rstxu = rst8u u, rstxv =
-// rst8v v, rstxwc = rst8w wc, over the ranges k_prep writes and zero outside
-// them.  RxArr reads them from arrays (k_prep's, or a caller's); RxFly forms
-// each where it is used, from u, v, w and the metrics, by the same single
-// product, so both give the same bits and the large step can drop k_prep.
-struct RxArr {
-  const float *xu, *xv, *xw;
-  __device__ __forceinline__ float u(const Grid &g, const Opts &, int i, int j, int k) const { return AT(xu, i, j, k); }
-  __device__ __forceinline__ float v(const Grid &g, const Opts &, int i, int j, int k) const { return AT(xv, i, j, k); }
-  __device__ __forceinline__ float w(const Grid &g, const Opts &, int i, int j, int k) const { return AT(xw, i, j, k); }
-};
-
-template <bool U>
-struct RxFly {
-  const float *uu, *vv, *ww, *rst8u, *rst8v, *rst8w, *jcb8w;
-  __device__ __forceinline__ float u(const Grid &g, const Opts &o, int i, int j, int k) const {
-    const bool m = k <= g.nk - 1 && j >= 1 && j <= g.nj - 1 && i >= o.iw && i <= g.ni + 1 - o.ie;
-    return m ? MTU(CRESS_RST8U, rst8u, i, j, k) * AT(uu, i, j, k) : 0.0f;
-  }
-  __device__ __forceinline__ float v(const Grid &g, const Opts &o, int i, int j, int k) const {
-    const bool m = k <= g.nk - 1 && i >= 1 && i <= g.ni - 1 && j >= o.js && j <= g.nj + 1 - o.jn;
-    return m ? MTU(CRESS_RST8V, rst8v, i, j, k) * AT(vv, i, j, k) : 0.0f;
-  }
-  __device__ __forceinline__ float w(const Grid &g, const Opts &o, int i, int j, int k) const {
-    const bool m = i >= 1 && i <= g.ni - 1 && j >= 1 && j <= g.nj - 1;
-    return m ? MTU(CRESS_RST8W, rst8w, i, j, k) * wc_value<U>(g, o, ww, jcb8w, i, j, k) : 0.0f;
-  }
-};
-
-// The accessor of a forcing kernel: formed where used (FLY) or read from
-// the arrays of the argument block.
-template <bool FLY, bool U, class IN>
-__device__ __forceinline__ auto rx_of(const IN &in) {
-  if constexpr (FLY)
-    return RxFly<U>{in.u, in.v, in.w, in.rst8u, in.rst8v, in.rst8w, in.jcb8w};
-  else
-    return RxArr{in.rstxu, in.rstxv, in.rstxwc};
-}
-
 // wc over i=1..ni-1, j=1..nj-1, k=1..nk (the phy2cnt+vbcwc write range)
 template <bool U>
 __global__ void k_phy2cnt(Tiles T, Grid g, Opts o, const float *w, const float *jcb8w, float *wc) {
@@ -164,27 +125,26 @@ __global__ void k_prep(Tiles T, Grid g, Opts o, const float *u, const float *v, 
 // the reference's accumulation order: advopt=1 f+(...), advopt=2
 // (f+hadv)+(t3+t3), advopt=3 (f+hadv)+vadv.
 // ---------------------------------------------------------------------------
-template <class RX>
 __device__ float adv_u(float f, const Grid &g, const Opts &o, const Consts &c, const float *u,
-                       const RX &rx, int i, int j, int k) {
+                       const float *rxu, const float *rxv, const float *rxw, int i, int j, int k) {
   auto t1 = [&](int ii) {
-    return (rx.u(g, o, ii, j, k) + rx.u(g, o, ii + 1, j, k)) * (AT(u, ii + 1, j, k) - AT(u, ii, j, k)) * c.dxv25n;
+    return (AT(rxu, ii, j, k) + AT(rxu, ii + 1, j, k)) * (AT(u, ii + 1, j, k) - AT(u, ii, j, k)) * c.dxv25n;
   };
   auto t2 = [&](int jj) {
-    return (rx.v(g, o, i - 1, jj, k) + rx.v(g, o, i, jj, k)) * (AT(u, i, jj, k) - AT(u, i, jj - 1, k)) * c.dyv25n;
+    return (AT(rxv, i - 1, jj, k) + AT(rxv, i, jj, k)) * (AT(u, i, jj, k) - AT(u, i, jj - 1, k)) * c.dyv25n;
   };
   auto t3 = [&](int kk) {
-    return (rx.w(g, o, i - 1, j, kk) + rx.w(g, o, i, j, kk)) * (AT(u, i, j, kk) - AT(u, i, j, kk - 1)) * c.dzv25n;
+    return (AT(rxw, i - 1, j, kk) + AT(rxw, i, j, kk)) * (AT(u, i, j, kk) - AT(u, i, j, kk - 1)) * c.dzv25n;
   };
   if (o.advopt == 1) return f + ((t3(k) + t3(k + 1)) + ((t1(i - 1) + t1(i)) + (t2(j) + t2(j + 1))));
   float hadv = (t1(i - 1) + t1(i)) + (t2(j) + t2(j + 1));
   const bool h4 = j >= 2 + o.js && j <= g.nj - 2 - o.jn && i >= 2 + o.iw && i <= g.ni - 1 - o.ie;
   if (h4) {
     auto T1 = [&](int ii) {
-      return (rx.u(g, o, ii - 1, j, k) + rx.u(g, o, ii + 1, j, k)) * (AT(u, ii + 1, j, k) - AT(u, ii - 1, j, k)) * c.dxv24;
+      return (AT(rxu, ii - 1, j, k) + AT(rxu, ii + 1, j, k)) * (AT(u, ii + 1, j, k) - AT(u, ii - 1, j, k)) * c.dxv24;
     };
     auto T2 = [&](int jj) {
-      return ((rx.v(g, o, i - 1, jj, k) + rx.v(g, o, i, jj, k)) + (rx.v(g, o, i - 1, jj + 1, k) + rx.v(g, o, i, jj + 1, k))) *
+      return ((AT(rxv, i - 1, jj, k) + AT(rxv, i, jj, k)) + (AT(rxv, i - 1, jj + 1, k) + AT(rxv, i, jj + 1, k))) *
              (AT(u, i, jj + 1, k) - AT(u, i, jj - 1, k)) * c.dyv48;
     };
     hadv = c.fourd3 * hadv + ((T1(i - 1) + T1(i + 1)) + (T2(j - 1) + T2(j + 1)));
@@ -193,7 +153,7 @@ __device__ float adv_u(float f, const Grid &g, const Opts &o, const Consts &c, c
   float vadv = t3(k) + t3(k + 1);
   if (h4 && k >= 3 && k <= g.nk - 3) {
     auto T3 = [&](int kk) {
-      return ((rx.w(g, o, i - 1, j, kk) + rx.w(g, o, i, j, kk)) + (rx.w(g, o, i - 1, j, kk + 1) + rx.w(g, o, i, j, kk + 1))) *
+      return ((AT(rxw, i - 1, j, kk) + AT(rxw, i, j, kk)) + (AT(rxw, i - 1, j, kk + 1) + AT(rxw, i, j, kk + 1))) *
              (AT(u, i, j, kk + 1) - AT(u, i, j, kk - 1)) * c.dzv48;
     };
     vadv = c.fourd3 * vadv + (T3(k - 1) + T3(k + 1));
@@ -201,28 +161,27 @@ __device__ float adv_u(float f, const Grid &g, const Opts &o, const Consts &c, c
   return f + hadv + vadv;
 }
 
-template <class RX>
 __device__ float adv_v(float f, const Grid &g, const Opts &o, const Consts &c, const float *v,
-                       const RX &rx, int i, int j, int k) {
+                       const float *rxu, const float *rxv, const float *rxw, int i, int j, int k) {
   auto t1 = [&](int ii) {
-    return (rx.u(g, o, ii, j - 1, k) + rx.u(g, o, ii, j, k)) * (AT(v, ii, j, k) - AT(v, ii - 1, j, k)) * c.dxv25n;
+    return (AT(rxu, ii, j - 1, k) + AT(rxu, ii, j, k)) * (AT(v, ii, j, k) - AT(v, ii - 1, j, k)) * c.dxv25n;
   };
   auto t2 = [&](int jj) {
-    return (rx.v(g, o, i, jj, k) + rx.v(g, o, i, jj + 1, k)) * (AT(v, i, jj + 1, k) - AT(v, i, jj, k)) * c.dyv25n;
+    return (AT(rxv, i, jj, k) + AT(rxv, i, jj + 1, k)) * (AT(v, i, jj + 1, k) - AT(v, i, jj, k)) * c.dyv25n;
   };
   auto t3 = [&](int kk) {
-    return (rx.w(g, o, i, j - 1, kk) + rx.w(g, o, i, j, kk)) * (AT(v, i, j, kk) - AT(v, i, j, kk - 1)) * c.dzv25n;
+    return (AT(rxw, i, j - 1, kk) + AT(rxw, i, j, kk)) * (AT(v, i, j, kk) - AT(v, i, j, kk - 1)) * c.dzv25n;
   };
   if (o.advopt == 1) return f + ((t3(k) + t3(k + 1)) + ((t1(i) + t1(i + 1)) + (t2(j - 1) + t2(j))));
   float hadv = (t1(i) + t1(i + 1)) + (t2(j - 1) + t2(j));
   const bool h4 = j >= 2 + o.js && j <= g.nj - 1 - o.jn && i >= 2 + o.iw && i <= g.ni - 2 - o.ie;
   if (h4) {
     auto T1 = [&](int ii) {
-      return ((rx.u(g, o, ii, j - 1, k) + rx.u(g, o, ii, j, k)) + (rx.u(g, o, ii + 1, j - 1, k) + rx.u(g, o, ii + 1, j, k))) *
+      return ((AT(rxu, ii, j - 1, k) + AT(rxu, ii, j, k)) + (AT(rxu, ii + 1, j - 1, k) + AT(rxu, ii + 1, j, k))) *
              (AT(v, ii + 1, j, k) - AT(v, ii - 1, j, k)) * c.dxv48;
     };
     auto T2 = [&](int jj) {
-      return (rx.v(g, o, i, jj - 1, k) + rx.v(g, o, i, jj + 1, k)) * (AT(v, i, jj + 1, k) - AT(v, i, jj - 1, k)) * c.dyv24;
+      return (AT(rxv, i, jj - 1, k) + AT(rxv, i, jj + 1, k)) * (AT(v, i, jj + 1, k) - AT(v, i, jj - 1, k)) * c.dyv24;
     };
     hadv = c.fourd3 * hadv + ((T1(i - 1) + T1(i + 1)) + (T2(j - 1) + T2(j + 1)));
   }
@@ -230,7 +189,7 @@ __device__ float adv_v(float f, const Grid &g, const Opts &o, const Consts &c, c
   float vadv = t3(k) + t3(k + 1);
   if (h4 && k >= 3 && k <= g.nk - 3) {
     auto T3 = [&](int kk) {
-      return ((rx.w(g, o, i, j - 1, kk) + rx.w(g, o, i, j, kk)) + (rx.w(g, o, i, j - 1, kk + 1) + rx.w(g, o, i, j, kk + 1))) *
+      return ((AT(rxw, i, j - 1, kk) + AT(rxw, i, j, kk)) + (AT(rxw, i, j - 1, kk + 1) + AT(rxw, i, j, kk + 1))) *
              (AT(v, i, j, kk + 1) - AT(v, i, j, kk - 1)) * c.dzv48;
     };
     vadv = c.fourd3 * vadv + (T3(k - 1) + T3(k + 1));
@@ -238,28 +197,27 @@ __device__ float adv_v(float f, const Grid &g, const Opts &o, const Consts &c, c
   return f + hadv + vadv;
 }
 
-template <class RX>
 __device__ float adv_w(float f, const Grid &g, const Opts &o, const Consts &c, const float *w,
-                       const RX &rx, int i, int j, int k) {
+                       const float *rxu, const float *rxv, const float *rxw, int i, int j, int k) {
   auto t1 = [&](int ii) {
-    return (rx.u(g, o, ii, j, k - 1) + rx.u(g, o, ii, j, k)) * (AT(w, ii, j, k) - AT(w, ii - 1, j, k)) * c.dxv25n;
+    return (AT(rxu, ii, j, k - 1) + AT(rxu, ii, j, k)) * (AT(w, ii, j, k) - AT(w, ii - 1, j, k)) * c.dxv25n;
   };
   auto t2 = [&](int jj) {
-    return (rx.v(g, o, i, jj, k - 1) + rx.v(g, o, i, jj, k)) * (AT(w, i, jj, k) - AT(w, i, jj - 1, k)) * c.dyv25n;
+    return (AT(rxv, i, jj, k - 1) + AT(rxv, i, jj, k)) * (AT(w, i, jj, k) - AT(w, i, jj - 1, k)) * c.dyv25n;
   };
   auto t3 = [&](int kk) {
-    return (rx.w(g, o, i, j, kk) + rx.w(g, o, i, j, kk + 1)) * (AT(w, i, j, kk + 1) - AT(w, i, j, kk)) * c.dzv25n;
+    return (AT(rxw, i, j, kk) + AT(rxw, i, j, kk + 1)) * (AT(w, i, j, kk + 1) - AT(w, i, j, kk)) * c.dzv25n;
   };
   if (o.advopt == 1) return f + ((t3(k - 1) + t3(k)) + ((t1(i) + t1(i + 1)) + (t2(j) + t2(j + 1))));
   float hadv = (t1(i) + t1(i + 1)) + (t2(j) + t2(j + 1));
   const bool h4 = j >= 2 + o.js && j <= g.nj - 2 - o.jn && i >= 2 + o.iw && i <= g.ni - 2 - o.ie;
   if (h4) {
     auto T1 = [&](int ii) {
-      return ((rx.u(g, o, ii, j, k - 1) + rx.u(g, o, ii, j, k)) + (rx.u(g, o, ii + 1, j, k - 1) + rx.u(g, o, ii + 1, j, k))) *
+      return ((AT(rxu, ii, j, k - 1) + AT(rxu, ii, j, k)) + (AT(rxu, ii + 1, j, k - 1) + AT(rxu, ii + 1, j, k))) *
              (AT(w, ii + 1, j, k) - AT(w, ii - 1, j, k)) * c.dxv48;
     };
     auto T2 = [&](int jj) {
-      return ((rx.v(g, o, i, jj, k - 1) + rx.v(g, o, i, jj, k)) + (rx.v(g, o, i, jj + 1, k - 1) + rx.v(g, o, i, jj + 1, k))) *
+      return ((AT(rxv, i, jj, k - 1) + AT(rxv, i, jj, k)) + (AT(rxv, i, jj + 1, k - 1) + AT(rxv, i, jj + 1, k))) *
              (AT(w, i, jj + 1, k) - AT(w, i, jj - 1, k)) * c.dyv48;
     };
     hadv = c.fourd3 * hadv + ((T1(i - 1) + T1(i + 1)) + (T2(j - 1) + T2(j + 1)));
@@ -268,7 +226,7 @@ __device__ float adv_w(float f, const Grid &g, const Opts &o, const Consts &c, c
   float vadv = t3(k - 1) + t3(k);
   if (h4 && k >= 3 && k <= g.nk - 2) {
     auto T3 = [&](int kk) {
-      return (rx.w(g, o, i, j, kk - 1) + rx.w(g, o, i, j, kk + 1)) * (AT(w, i, j, kk + 1) - AT(w, i, j, kk - 1)) * c.dzv24;
+      return (AT(rxw, i, j, kk - 1) + AT(rxw, i, j, kk + 1)) * (AT(w, i, j, kk + 1) - AT(w, i, j, kk - 1)) * c.dzv24;
     };
     vadv = c.fourd3 * vadv + (T3(k - 1) + T3(k + 1));
   }
@@ -278,9 +236,8 @@ __device__ float adv_w(float f, const Grid &g, const Opts &o, const Consts &c, c
 // ufrc over i=2..ni-1, j=2..nj-2 and vfrc over i=2..ni-2, j=2..nj-1; k=2..nk-2.
 // Accumulation order of forceuvw.f90:434-626: 0, smoothing, sponge,
 // Coriolis, advection.
-template <bool SM, bool U, bool FLY>
+template <bool SM, bool U>
 __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *ufrc, float *vfrc) {
-  const auto rx = rx_of<FLY, U>(in);
   const uint3 xt = xcd_tile(T);
   const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 2 + xt.y * blockDim.y + threadIdx.y;
@@ -312,7 +269,7 @@ __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float 
       };
       f = f + (cr(i - 1) + cr(i));
     }
-    uout = adv_u(f, g, o, c, in.u, rx, i, j, k);
+    uout = adv_u(f, g, o, c, in.u, in.rstxu, in.rstxv, in.rstxwc, i, j, k);
   }
   if (dv) {
     float f = 0.0f;
@@ -335,7 +292,7 @@ __global__ void k_force_uv(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float 
       };
       f = f + (cr(j - 1) + cr(j));
     }
-    vout = adv_v(f, g, o, c, in.v, rx, i, j, k);
+    vout = adv_v(f, g, o, c, in.v, in.rstxu, in.rstxv, in.rstxwc, i, j, k);
   }
   if (du) AT(ufrc, i, j, k) = uout;
   if (dv) AT(vfrc, i, j, k) = vout;
@@ -358,9 +315,8 @@ __device__ __forceinline__ float wb8s_at(const Grid &g, const Opts &o, const Con
 }
 
 // wfrc over i=2..ni-2, j=2..nj-2, k=2..nk-1
-template <bool SM, bool U, bool FLY>
+template <bool SM, bool U>
 __global__ void k_force_w(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *wfrc) {
-  const auto rx = rx_of<FLY, U>(in);
   const uint3 xt = xcd_tile(T);
   const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 2 + xt.y * blockDim.y + threadIdx.y;
@@ -379,7 +335,7 @@ __global__ void k_force_w(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *
     f = f - rbct2(in.rbct, g, i, j, k) * MTU(CRESS_RST8W, in.rst8w, i, j, k) * AT(in.wp, i, j, k);
   if (o.buyopt == 1 && k >= 3 && k <= g.nk - 2)
     f = f + (wb8s_at<U>(g, o, c, in, i, j, k - 1) + wb8s_at<U>(g, o, c, in, i, j, k));
-  AT(wfrc, i, j, k) = adv_w(f, g, o, c, in.w, rx, i, j, k);
+  AT(wfrc, i, j, k) = adv_w(f, g, o, c, in.w, in.rstxu, in.rstxv, in.rstxwc, i, j, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -389,9 +345,7 @@ __global__ void k_force_w(Tiles T, Grid g, Opts o, Consts c, ForceIn in, float *
 // forcep's smtopt slot (forcedrv.f90:598, idzeropt), so the reference's
 // large step never smooths p'.
 // ---------------------------------------------------------------------------
-// FLY: the wc argument holds w, and wc (phy2cnt + vbcwc, as k_prep writes
-// it) is formed where used.
-template <bool U, bool FLY>
+template <bool U>
 __global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, const float *jcb8u,
                           const float *jcb8v, const float *jcb8w, const float *u, const float *v,
                           const float *wc, const float *pp, const float *ppp, const float *rbct,
@@ -403,13 +357,7 @@ __global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, c
   if (i < 2 || i > g.ni - 2 || j > g.nj - 2) return;
   auto jxu = [&](int ii, int jj) { return MTU(CRESS_JCB8U, jcb8u, ii, jj, k) * AT(u, ii, jj, k); };
   auto jxv = [&](int ii, int jj) { return MTU(CRESS_JCB8V, jcb8v, ii, jj, k) * AT(v, ii, jj, k); };
-  auto wcv = [&](int kk) {
-    if constexpr (FLY)
-      return wc_value<U>(g, o, wc, jcb8w, i, j, kk);
-    else
-      return AT(wc, i, j, kk);
-  };
-  auto jxw = [&](int kk) { return MTU(CRESS_JCB8W, jcb8w, i, j, kk) * wcv(kk); };
+  auto jxw = [&](int kk) { return MTU(CRESS_JCB8W, jcb8w, i, j, kk) * AT(wc, i, j, kk); };
   auto t1 = [&](int ii) { return jxu(ii, j) * (AT(pp, ii, j, k) - AT(pp, ii - 1, j, k)) * c.dxv05n; };
   auto t2 = [&](int jj) { return jxv(i, jj) * (AT(pp, i, jj, k) - AT(pp, i, jj - 1, k)) * c.dyv05n; };
   auto t3 = [&](int kk) { return jxw(kk) * (AT(pp, i, j, kk) - AT(pp, i, j, kk - 1)) * c.dzv05n; };
@@ -456,20 +404,20 @@ __global__ void k_force_p(Tiles T, Grid g, Opts o, Consts c, const float *jcb, c
 // ---------------------------------------------------------------------------
 
 
-template <class RX>
-__device__ float adv_s(const Grid &g, const Opts &o, const Consts &c, const float *s, const RX &rx, int i, int j, int k) {
-  auto t1 = [&](int ii) { return rx.u(g, o, ii, j, k) * (AT(s, ii, j, k) - AT(s, ii - 1, j, k)) * c.dxv05n; };
-  auto t2 = [&](int jj) { return rx.v(g, o, i, jj, k) * (AT(s, i, jj, k) - AT(s, i, jj - 1, k)) * c.dyv05n; };
-  auto t3 = [&](int kk) { return rx.w(g, o, i, j, kk) * (AT(s, i, j, kk) - AT(s, i, j, kk - 1)) * c.dzv05n; };
+__device__ float adv_s(const Grid &g, const Opts &o, const Consts &c, const float *s, const float *rxu,
+                       const float *rxv, const float *rxw, int i, int j, int k) {
+  auto t1 = [&](int ii) { return AT(rxu, ii, j, k) * (AT(s, ii, j, k) - AT(s, ii - 1, j, k)) * c.dxv05n; };
+  auto t2 = [&](int jj) { return AT(rxv, i, jj, k) * (AT(s, i, jj, k) - AT(s, i, jj - 1, k)) * c.dyv05n; };
+  auto t3 = [&](int kk) { return AT(rxw, i, j, kk) * (AT(s, i, j, kk) - AT(s, i, j, kk - 1)) * c.dzv05n; };
   if (o.advopt == 1) return (t3(k) + t3(k + 1)) + ((t1(i) + t1(i + 1)) + (t2(j) + t2(j + 1)));
   float f = (t1(i) + t1(i + 1)) + (t2(j) + t2(j + 1));
   const bool h4 = j >= 2 + o.js && j <= g.nj - 2 - o.jn && i >= 2 + o.iw && i <= g.ni - 2 - o.ie;
   if (h4) {
     auto T1 = [&](int ii) {
-      return (rx.u(g, o, ii, j, k) + rx.u(g, o, ii + 1, j, k)) * (AT(s, ii + 1, j, k) - AT(s, ii - 1, j, k)) * c.dxv24;
+      return (AT(rxu, ii, j, k) + AT(rxu, ii + 1, j, k)) * (AT(s, ii + 1, j, k) - AT(s, ii - 1, j, k)) * c.dxv24;
     };
     auto T2 = [&](int jj) {
-      return (rx.v(g, o, i, jj, k) + rx.v(g, o, i, jj + 1, k)) * (AT(s, i, jj + 1, k) - AT(s, i, jj - 1, k)) * c.dyv24;
+      return (AT(rxv, i, jj, k) + AT(rxv, i, jj + 1, k)) * (AT(s, i, jj + 1, k) - AT(s, i, jj - 1, k)) * c.dyv24;
     };
     f = c.fourd3 * f + ((T1(i - 1) + T1(i + 1)) + (T2(j - 1) + T2(j + 1)));
   }
@@ -477,7 +425,7 @@ __device__ float adv_s(const Grid &g, const Opts &o, const Consts &c, const floa
   float vadv = t3(k) + t3(k + 1);
   if (h4 && k >= 3 && k <= g.nk - 3) {
     auto T3 = [&](int kk) {
-      return (rx.w(g, o, i, j, kk) + rx.w(g, o, i, j, kk + 1)) * (AT(s, i, j, kk + 1) - AT(s, i, j, kk - 1)) * c.dzv24;
+      return (AT(rxw, i, j, kk) + AT(rxw, i, j, kk + 1)) * (AT(s, i, j, kk + 1) - AT(s, i, j, kk - 1)) * c.dzv24;
     };
     vadv = c.fourd3 * vadv + (T3(k - 1) + T3(k + 1));
   }
@@ -551,9 +499,8 @@ __device__ __forceinline__ float adv_s_val(const Opts &o, const Consts &c, bool 
 // (advs.f90:205-420), the vspopt=2 sponge (vsps.f90:160-198, vspqv) and, for
 // theta', advbspt (advbspt.f90:143-190).  The flux weights are loaded once
 // and shared by the scalars.
-template <bool SM, bool U, bool FLY>
+template <bool SM, bool U>
 __global__ void __launch_bounds__(64 * CRESS_FS_BYMAX) k_force_s(Tiles T, Grid g, Opts o, Consts c, ScalarIn in, int nscalar) {
-  const auto rx = rx_of<FLY, U>(in);
   const uint3 xt = xcd_tile(T);
   const int i = xt.x * blockDim.x + threadIdx.x;
   const int j = 2 + xt.y * blockDim.y + threadIdx.y;
@@ -563,10 +510,13 @@ __global__ void __launch_bounds__(64 * CRESS_FS_BYMAX) k_force_s(Tiles T, Grid g
   const uint32_t q = 4u * (uint32_t)g.at(i, j, k);
   float fu[4], fv[4], fw[4];
   for (int m = 0; m < 4; m++) {
-    fu[m] = rx.u(g, o, i - 1 + m, j, k);
-    fv[m] = rx.v(g, o, i, j - 1 + m, k);
-    fw[m] = rx.w(g, o, i, j, k - 1 + m);
+    fu[m] = LD(in.rstxu, q + 4u * m - 4u);
+    fv[m] = LD(in.rstxv, q + sx * m - sx);
   }
+  fw[0] = LD(in.rstxwc, q - sk);
+  fw[1] = LD(in.rstxwc, q);
+  fw[2] = LD(in.rstxwc, q + sk);
+  fw[3] = LD(in.rstxwc, q + 2 * sk);
   const bool h4 = j >= 2 + o.js && j <= g.nj - 2 - o.jn && i >= 2 + o.iw && i <= g.ni - 2 - o.ie;
   const bool v4 = k >= 3 && k <= g.nk - 3;
   const bool sponge = k >= o.ksp0 - 1 && k <= g.nk - 2;
@@ -1110,7 +1060,7 @@ __global__ void k_advs_only(Grid g, Opts o, Consts c, ScalarIn in) {
   const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
   const int k = 2 + blockIdx.z;
   if (i > g.ni - 2 || j > g.nj - 2) return;
-  AT(in.frc[0], i, j, k) = adv_s(g, o, c, in.s[0], RxArr{in.rstxu, in.rstxv, in.rstxwc}, i, j, k);
+  AT(in.frc[0], i, j, k) = adv_s(g, o, c, in.s[0], in.rstxu, in.rstxv, in.rstxwc, i, j, k);
 }
 
 // s_advuvw alone (advuvw.f90:75-912): ufrc, vfrc, wfrc plus the advection
@@ -1124,10 +1074,9 @@ __global__ void k_advuvw_only(Grid g, Opts o, Consts c, const float *u, const fl
   const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
   const int k = 2 + blockIdx.z;
   if (i > g.ni - 1 || j > g.nj - 1) return;
-  const RxArr rx{rxu, rxv, rxw};
-  if (k <= g.nk - 2 && j <= g.nj - 2) AT(ufrc, i, j, k) = adv_u(AT(ufrc, i, j, k), g, o, c, u, rx, i, j, k);
-  if (k <= g.nk - 2 && i <= g.ni - 2) AT(vfrc, i, j, k) = adv_v(AT(vfrc, i, j, k), g, o, c, v, rx, i, j, k);
-  if (i <= g.ni - 2 && j <= g.nj - 2) AT(wfrc, i, j, k) = adv_w(AT(wfrc, i, j, k), g, o, c, w, rx, i, j, k);
+  if (k <= g.nk - 2 && j <= g.nj - 2) AT(ufrc, i, j, k) = adv_u(AT(ufrc, i, j, k), g, o, c, u, rxu, rxv, rxw, i, j, k);
+  if (k <= g.nk - 2 && i <= g.ni - 2) AT(vfrc, i, j, k) = adv_v(AT(vfrc, i, j, k), g, o, c, v, rxu, rxv, rxw, i, j, k);
+  if (i <= g.ni - 2 && j <= g.nj - 2) AT(wfrc, i, j, k) = adv_w(AT(wfrc, i, j, k), g, o, c, w, rxu, rxv, rxw, i, j, k);
 }
 
 // s_bc8u, s_bc8v (bc8u.f90:132-202, bc8v.f90), s_bc8w (bc8w.f90:130-184) on
@@ -1165,24 +1114,20 @@ template __global__ void k_ac_ptuv<false, 1>(Tiles, Grid, Opts, Consts, AcPtUv);
 template __global__ void k_ac_ptuv<true, 1>(Tiles, Grid, Opts, Consts, AcPtUv);
 template __global__ void k_ac_col<false>(Tiles, Grid, Opts, Consts, AcCol);
 template __global__ void k_ac_col<true>(Tiles, Grid, Opts, Consts, AcCol);
-#define CRESS_INST_FLY(U, F)                                                                                 \
-  template __global__ void k_force_p<U, F>(Tiles, Grid, Opts, Consts, const float *, const float *,          \
-                                           const float *, const float *, const float *, const float *,       \
-                                           const float *, const float *, const float *, const float *,       \
-                                           float *);                                                         \
-  template __global__ void k_force_uv<false, U, F>(Tiles, Grid, Opts, Consts, ForceIn, float *, float *);    \
-  template __global__ void k_force_uv<true, U, F>(Tiles, Grid, Opts, Consts, ForceIn, float *, float *);     \
-  template __global__ void k_force_w<false, U, F>(Tiles, Grid, Opts, Consts, ForceIn, float *);              \
-  template __global__ void k_force_w<true, U, F>(Tiles, Grid, Opts, Consts, ForceIn, float *);               \
-  template __global__ void k_force_s<false, U, F>(Tiles, Grid, Opts, Consts, ScalarIn, int);                 \
-  template __global__ void k_force_s<true, U, F>(Tiles, Grid, Opts, Consts, ScalarIn, int);
 #define CRESS_INST_U(U)                                                                                      \
   template __global__ void k_phy2cnt<U>(Tiles, Grid, Opts, const float *, const float *, float *);           \
   template __global__ void k_prep<U>(Tiles, Grid, Opts, const float *, const float *, const float *,         \
                                      const float *, const float *, const float *, const float *, float *,   \
                                      float *, float *, float *);                                             \
-  CRESS_INST_FLY(U, false)                                                                                   \
-  CRESS_INST_FLY(U, true)
+  template __global__ void k_force_p<U>(Tiles, Grid, Opts, Consts, const float *, const float *,             \
+                                        const float *, const float *, const float *, const float *,          \
+                                        const float *, const float *, const float *, const float *, float *);\
+  template __global__ void k_force_uv<false, U>(Tiles, Grid, Opts, Consts, ForceIn, float *, float *);       \
+  template __global__ void k_force_uv<true, U>(Tiles, Grid, Opts, Consts, ForceIn, float *, float *);        \
+  template __global__ void k_force_w<false, U>(Tiles, Grid, Opts, Consts, ForceIn, float *);                 \
+  template __global__ void k_force_w<true, U>(Tiles, Grid, Opts, Consts, ForceIn, float *);                  \
+  template __global__ void k_force_s<false, U>(Tiles, Grid, Opts, Consts, ScalarIn, int);                    \
+  template __global__ void k_force_s<true, U>(Tiles, Grid, Opts, Consts, ScalarIn, int);
 CRESS_INST_U(false)
 CRESS_INST_U(true)
 

@@ -27,7 +27,7 @@ module m_cress_hip
 ! sizes of the result arrays of cress_diag, cress_autotune /
 ! cress_set_launch / cress_get_launch and cress_base_columns
   integer(c_int), parameter, public :: cress_ndiag = 13
-  integer(c_int), parameter, public :: cress_nlaunch = 10
+  integer(c_int), parameter, public :: cress_nlaunch = 9
   integer(c_int), parameter, public :: cress_nsetupcol = 16
 
 ! field ids (base state: level 0 only; prognostics: 0 past, 1 present,

@@ -45,7 +45,7 @@ class Sounding(C.Structure):
 
 # launch shapes (cress_autotune / cress_set_launch order)
 LAUNCH_NAMES = ["ptuv_by", "ptuv_slots", "ptuv_order", "col_by", "col_lds", "fs_by", "fs_order", "ff_by",
-                "ff_order", "fly"]
+                "ff_order"]
 NLAUNCH = len(LAUNCH_NAMES)
 
 ERR = {0: "ok", -1: "invalid argument", -2: "no usable GPU", -3: "HIP runtime error",

@@ -170,19 +170,17 @@ int cress_set_base(cress_ctx *ctx, const cress_sounding *snd);
 #define CRESS_NSETUPCOL 16
 int cress_base_columns(const cress_ctx *ctx, double *out, int ncol);
 
-/* Launch shapes of the tunable kernel families, as 10 ints: fused
+/* Launch shapes of the tunable kernel families, as 9 ints: fused
  * ptil/stepuv rows per workgroup (4, 8, 16), halo layout (0 one point per
  * thread, 1 slots), tile order (1 x-fastest, 2 z-fastest); column kernel
  * rows (1, 2, 4) and LDS bytes per workgroup (caps occupancy); scalar
- * forcing rows and order; u/v, w, p' forcing rows and order; 1 to form the
- * forcing's mass fluxes where they are used instead of in a kernel of
- * their own (s_rstuvwc's arrays).  No
+ * forcing rows and order; u/v, w, p' forcing rows and order.  No
  * counterpart in the reference (tile shapes are a GPU matter).  Results do
  * not depend on the shapes.  cress_autotune times every candidate on the
  * context's own grid and keeps the fastest (cached per process by grid and
  * options); it writes only scratch arrays and the future level, so it may
  * run between any two large steps, with timing off.  shape may be NULL. */
-#define CRESS_NLAUNCH 10
+#define CRESS_NLAUNCH 9
 int cress_autotune(cress_ctx *ctx, int *shape, int n);
 int cress_set_launch(cress_ctx *ctx, const int *shape, int n);
 int cress_get_launch(const cress_ctx *ctx, int *shape, int n);

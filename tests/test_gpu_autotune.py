@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 FIELDS = ["u", "v", "w", "pp", "ptp", "qv", "qc", "qr", "qi", "qs", "qg", "nci"]
 OTHER = {"ptuv_by": 4, "ptuv_slots": 1, "ptuv_order": 2, "col_by": 1, "col_lds": 0, "fs_by": 8, "fs_order": 1,
-         "ff_by": 2, "ff_order": 1, "fly": 1}
+         "ff_by": 2, "ff_order": 1}
 
 
 def _run(case, nsteps, shapes=None, tune=False, tune_between=False):
