@@ -2406,3 +2406,65 @@ int cress_ctx_set_mlp(cress_ctx *x, cress_mlp *mlp, int mode) {
   x->mlp_mode = mode;
   return CRESS_OK;
 }
+
+// s_forcetke: the advection of the present TKE by the kernel s_advs uses in
+// the step, then the closure's terms (turb_tke).  rkh, rkv, tkefrc and rkv8s
+// go to the device and back whole.
+int cress_forcetke(const cress_config *cfg, const float *jcb, const float *jcb8u, const float *jcb8v,
+                   const float *jcb8w, const float *rst, const float *rstxu, const float *rstxv, const float *rstxwc,
+                   const float *tke, const float *tkep, const float *rbct, const float *priv, const float *nsq8w,
+                   const float *ssq, float *rkh, float *rkv, float *tkefrc, float *rkv8s) {
+  const float *in[14] = {jcb, jcb8u, jcb8v, jcb8w, rst, rstxu, rstxv, rstxwc, tke, tkep, priv, nsq8w, ssq, rbct};
+  float *out[4] = {rkh, rkv, tkefrc, rkv8s};
+  if (!cfg) return CRESS_EINVAL;
+  const int vsp = cfg->vspopt >= 1 && cfg->vspvar[9] == 'o';
+  for (int f = 0; f < 14; f++)
+    if (!in[f] && (f < 13 || vsp)) return CRESS_EINVAL;
+  for (float *a : out)
+    if (!a) return CRESS_EINVAL;
+  if (cfg->ni < 7 || cfg->nj < 7 || cfg->nk < 7 || cfg->advopt < 1 || cfg->advopt > 3 || !(cfg->dx > 0.0) ||
+      !(cfg->dy > 0.0) || !(cfg->dz > 0.0) || (vsp && (cfg->ksp0[1] < 2 || cfg->ksp0[1] > cfg->nk)))
+    return CRESS_EINVAL;
+  if (cfg->smtopt % 10 != 0) return CRESS_EUNSUPPORTED;
+  const int sides[4] = {cfg->wbc, cfg->ebc, cfg->sbc, cfg->nbc};
+  for (int s = 0; s < 4; s++)
+    if (sides[s] != -1 && sides[s] != 7) return CRESS_EUNSUPPORTED;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CRESS_ENODEV;
+  const Grid g = make_grid(cfg->ni, cfg->nj, cfg->nk);
+  const size_t n = (size_t)g.n();
+  DevBuf di[14], dout[4];
+  int rc;
+  for (int f = 0; f < 13; f++)
+    if ((rc = to_dev(di[f], in[f], n))) return rc;
+  if (vsp && (rc = to_dev(di[13], rbct, 2 * (size_t)cfg->ni * cfg->nj * cfg->nk))) return rc;
+  for (int f = 0; f < 4; f++)
+    if ((rc = to_dev(dout[f], out[f], n))) return rc;
+  Consts c;
+  fold_consts(cfg, &c);
+  Opts o = {};
+  o.advopt = cfg->advopt;
+  o.iw = cfg->wbc == -1 ? 0 : 1;
+  o.ie = cfg->ebc == -1 ? 0 : 1;
+  o.js = cfg->sbc == -1 ? 0 : 1;
+  o.jn = cfg->nbc == -1 ? 0 : 1;
+  ScalarIn si = {};
+  si.s[0] = (const float *)di[8].p;
+  si.frc[0] = (float *)dout[2].p;
+  si.rstxu = (const float *)di[5].p;
+  si.rstxv = (const float *)di[6].p;
+  si.rstxwc = (const float *)di[7].p;
+  hipLaunchKernelGGL(k_advs_only, grid2(g.ni - 3, g.nj - 3, 64, 4, g.nk - 3), dim3(64, 4), 0, 0, g, o, c, si);
+  TkeArgs a = {};
+  a.dx = (float)cfg->dx, a.dy = (float)cfg->dy, a.dz = (float)cfg->dz;
+  a.vsp = vsp, a.ksp0 = cfg->ksp0[1];
+  a.jcb = (const float *)di[0].p, a.jcb8u = (const float *)di[1].p, a.jcb8v = (const float *)di[2].p;
+  a.jcb8w = (const float *)di[3].p, a.rst = (const float *)di[4].p, a.tkep = (const float *)di[9].p;
+  a.priv = (const float *)di[10].p, a.nsq8w = (const float *)di[11].p, a.ssq = (const float *)di[12].p;
+  a.rbct = vsp ? (const float *)di[13].p : nullptr;
+  a.rkh = (float *)dout[0].p, a.rkv = (float *)dout[1].p, a.tkefrc = (float *)dout[2].p, a.rkv8s = (float *)dout[3].p;
+  turb_tke(g, a);
+  HIPCHK(hipGetLastError());
+  for (int f = 0; f < 4; f++) HIPCHK(hipMemcpy(out[f], dout[f].p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return CRESS_OK;
+}

@@ -39,4 +39,19 @@ void turb_s(const Grid &g, float dx, float dy, float dz, const float *jcb8u, con
 // s_kh8uv (kh8uv.f90:60), mfcopt=0.
 void turb_kh8uv(const Grid &g, const float *rkh, float *rkh8u, float *rkh8v);
 
+// The arrays and options of s_forcetke past its advection call.
+struct TkeArgs {
+  float dx, dy, dz;
+  int vsp, ksp0;                       // vertical sponge on TKE; ksp0(2)
+  const float *jcb, *jcb8u, *jcb8v, *jcb8w, *rst, *tkep, *priv, *nsq8w, *ssq;
+  const float *rbct;                   // (1:ni,1:nj,1:nk,1:2), read when vsp
+  float *rkh, *rkv, *tkefrc, *rkv8s;
+};
+
+// s_forcetke (forcetke.f90:225-351) after s_advs has written tkefrc, for
+// tubopt=2, isoopt=1, trnopt=0, mfcopt=0, no smoothing or lateral sponge:
+// shear production, dissipation, the flux divergence, the vertical sponge,
+// the eddy diffusivities and the buoyancy term, on the null stream.
+void turb_tke(const Grid &g, const TkeArgs &a);
+
 }  // namespace cress

@@ -317,4 +317,94 @@ void turb_kh8uv(const Grid &g, const float *rkh, float *rkh8u, float *rkh8v) {
   hipLaunchKernelGGL(k_kh8uv, dim3((g.ni - 1 + 63) / 64, g.nj - 1, g.nk - 1), dim3(64), 0, 0, g, rkh, rkh8u, rkh8v);
 }
 
+// s_sheartke (sheartke.f90:93-101) and s_disptke (disptke.f90:180-195,
+// isoopt=1, mfcopt=0) over i,j 2..n-2, k 2..nk-2, then s_eddyvisj
+// (eddyvisj.f90:103-112, mfcopt=0) over i,j 1..n-1, k 1..nk-1; each cell
+// reads and writes only itself.  The length scale is (dx dy dz jcb / 8)^(1/3)
+// taken as exp(log(x)/3), as the reference writes it.
+__global__ void k_tke_src(Grid g, TkeArgs a) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y, k = 1 + blockIdx.z;
+  if (i > g.ni - 1) return;
+  const float jcb = AT(a.jcb, i, j, k), rkv = AT(a.rkv, i, j, k);
+  if (i >= 2 && i <= g.ni - 2 && j >= 2 && j <= g.nj - 2 && k >= 2 && k <= g.nk - 2) {
+    float f = AT(a.tkefrc, i, j, k) + jcb * rkv * AT(a.ssq, i, j, k);
+    const float ds308 = .125f * a.dx * a.dy * a.dz;
+    const float pv = AT(a.priv, i, j, k), e = AT(a.tkep, i, j, k);
+    const float ln = (pv - 1.f) * expd((1.f / 3.f) * logd(ds308 * jcb)) + TB_EPS;
+    AT(a.tkefrc, i, j, k) = f - (.37f * pv - .18f) * AT(a.rst, i, j, k) * e * sqrtf(e) / ln;
+  }
+  const float jcbiv = 1.f / jcb;
+  AT(a.rkh, i, j, k) = jcbiv * AT(a.rkh, i, j, k);
+  AT(a.rkv, i, j, k) = jcbiv * rkv;
+}
+
+// s_tkeflx (tkeflx.f90:181-330, trnopt=0, mfcopt=0) and s_turbtke
+// (turbtke.f90:191-213) fused: each cell forms the six fluxes on its faces
+// from rkh, rkv (already divided by jcb) and the past TKE with the
+// reference's operations, so the divergence is the reference's bits; then
+// s_vsps0 (vsps0.f90:114-127) from level ksp0(2)-1.  k runs from 1 so that
+// the sponge reaches level 1 when ksp0(2) = 2, as the reference's does.
+__global__ void k_tke_flx(Grid g, TkeArgs a) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x, j = 2 + blockIdx.y, k = 1 + blockIdx.z;
+  if (i > g.ni - 2) return;
+  const bool sponge = a.vsp && k >= a.ksp0 - 1;
+  if (k == 1 && !sponge) return;
+  float f = AT(a.tkefrc, i, j, k);
+  if (k >= 2) {
+    const float dxiv = 1.f / a.dx, dyiv = 1.f / a.dy, dziv = 1.f / a.dz;
+    const float dxv5 = .5f * dxiv, dyv5 = .5f * dyiv, dzv5 = .5f * dziv;
+    const float dxiv2 = 2.f * dxiv, dyiv2 = 2.f * dyiv, dziv2 = 2.f * dziv;
+    auto jt = [&](int ii, int jj) { return AT(a.tkep, ii, jj, k) * AT(a.jcb, ii, jj, k); };
+    auto h1 = [&](int ii) { return (AT(a.rkh, ii - 1, j, k) + AT(a.rkh, ii, j, k)) * (jt(ii, j) - jt(ii - 1, j)) * dxv5; };
+    auto h2 = [&](int jj) { return (AT(a.rkh, i, jj - 1, k) + AT(a.rkh, i, jj, k)) * (jt(i, jj) - jt(i, jj - 1)) * dyv5; };
+    auto h3 = [&](int kk) {
+      return (AT(a.rkv, i, j, kk - 1) + AT(a.rkv, i, j, kk)) * (AT(a.tkep, i, j, kk) - AT(a.tkep, i, j, kk - 1)) * dzv5;
+    };
+    f = f + ((h3(k + 1) - h3(k)) * dziv2 +
+             ((AT(a.jcb8u, i + 1, j, k) * h1(i + 1) - AT(a.jcb8u, i, j, k) * h1(i)) * dxiv2 +
+              (AT(a.jcb8v, i, j + 1, k) * h2(j + 1) - AT(a.jcb8v, i, j, k) * h2(j)) * dyiv2));
+  }
+  if (sponge) {
+    // rbct(i,j,k,2) and rbct(i,j,k+1,2)
+    const size_t nij = (size_t)g.ni * g.nj;
+    const size_t o = (size_t)(i - 1) + (size_t)g.ni * (j - 1) + nij * ((size_t)(k - 1) + (size_t)g.nk);
+    f = f - .5f * (a.rbct[o] + a.rbct[o + nij]) * AT(a.rst, i, j, k) * AT(a.tkep, i, j, k);
+  }
+  AT(a.tkefrc, i, j, k) = f;
+}
+
+// s_eddydif (eddydif.f90:266-330, tubopt>=2, isoopt=1) and s_buoytke
+// (buoytke.f90:96-115) down one column per thread, i,j 1..n-1: rkv8s =
+// priv rkv, rkh = rkv8s, rkv = the mean of rkv8s over the two levels
+// (k >= 2), rkv8s halved and times jcb; the buoyancy flux at w levels is
+// carried from one level to the next, and the TKE forcing of level k-1
+// takes the two around it.  Each column reads and writes only itself.
+__global__ void k_tke_dif(Grid g, TkeArgs a) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y;
+  if (i > g.ni - 1) return;
+  const bool inner = i >= 2 && i <= g.ni - 2 && j >= 2 && j <= g.nj - 2;
+  float sp = 0.f, r8p = 0.f, bp = 0.f;
+  for (int k = 1; k <= g.nk - 1; k++) {
+    const float s = AT(a.priv, i, j, k) * AT(a.rkv, i, j, k);
+    AT(a.rkh, i, j, k) = s;
+    if (k >= 2) AT(a.rkv, i, j, k) = .5f * (sp + s);
+    const float r8 = .5f * AT(a.jcb, i, j, k) * s;
+    AT(a.rkv8s, i, j, k) = r8;
+    if (inner && k >= 2) {
+      const float b = AT(a.jcb8w, i, j, k) * AT(a.nsq8w, i, j, k) * (r8p + r8);
+      if (k >= 3) AT(a.tkefrc, i, j, k - 1) = AT(a.tkefrc, i, j, k - 1) - (bp + b);
+      bp = b;
+    }
+    sp = s;
+    r8p = r8;
+  }
+}
+
+void turb_tke(const Grid &g, const TkeArgs &a) {
+  const dim3 blk(64);
+  hipLaunchKernelGGL(k_tke_src, dim3((g.ni - 1 + 63) / 64, g.nj - 1, g.nk - 1), blk, 0, 0, g, a);
+  hipLaunchKernelGGL(k_tke_flx, dim3((g.ni - 3 + 63) / 64, g.nj - 3, g.nk - 2), blk, 0, 0, g, a);
+  hipLaunchKernelGGL(k_tke_dif, dim3((g.ni - 1 + 63) / 64, g.nj - 1), blk, 0, 0, g, a);
+}
+
 }  // namespace cress

@@ -68,7 +68,7 @@ module m_cress_hip
   public :: cress_hevi, cress_steps, cress_cloudphy, cress_turbcoef
   public :: cress_turbdrv, cress_turbflx, cress_turbs
   public :: cress_bc8u, cress_bc8v, cress_bc8w
-  public :: cress_kh8uv, cress_gaussel_noalloc
+  public :: cress_kh8uv, cress_gaussel_noalloc, cress_forcetke
   public :: cress_set_base, cress_base_columns, cress_diag
   public :: cress_autotune, cress_set_launch, cress_get_launch
   public :: cress_ens_create, cress_ens_step, cress_ens_sync, cress_ens_destroy
@@ -367,6 +367,21 @@ module m_cress_hip
       real(c_float), intent(inout) :: rkh8u(*), rkh8v(*)
       integer(c_int) :: cress_kh8uv
     end function cress_kh8uv
+
+! tier 2: s_forcetke (forcetke.f90:88-94), tubopt=2, isoopt=1, trnopt=0,
+! mfcopt=0, smtopt=0, no lateral sponge
+    function cress_forcetke(cfg, jcb, jcb8u, jcb8v, jcb8w, rst, rstxu,  &
+     &  rstxv, rstxwc, tke, tkep, rbct, priv, nsq8w, ssq, rkh, rkv,      &
+     &  tkefrc, rkv8s) bind(c, name='cress_forcetke')
+      import :: c_int, c_float, cress_config
+      type(cress_config), intent(in) :: cfg
+      real(c_float), intent(in) :: jcb(*), jcb8u(*), jcb8v(*), jcb8w(*)
+      real(c_float), intent(in) :: rst(*), rstxu(*), rstxv(*), rstxwc(*)
+      real(c_float), intent(in) :: tke(*), tkep(*), rbct(*), priv(*)
+      real(c_float), intent(in) :: nsq8w(*), ssq(*)
+      real(c_float), intent(inout) :: rkh(*), rkv(*), tkefrc(*), rkv8s(*)
+      integer(c_int) :: cress_forcetke
+    end function cress_forcetke
 
 ! cress_gaussel without device allocation (a fixed device array stages the
 ! four arrays; larger boxes return cress_etoobig)

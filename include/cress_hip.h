@@ -398,6 +398,28 @@ int cress_turbs(const cress_config *cfg, const float *jcb8u, const float *jcb8v,
  * bitwise. */
 int cress_kh8uv(const cress_config *cfg, const float *rkh, float *rkh8u, float *rkh8v);
 
+/* s_forcetke(fplspvar,fpvspvar,fplspopt,fpvspopt,fpsmtopt,ksp0,ni,nj,nk,
+ *            j31,j32,jcb,jcb8u,jcb8v,jcb8w,mf,rmf,rmf8u,rmf8v,rbr,rst,rstxu,
+ *            rstxv,rstxwc,tke,tkep,rbcxy,rbct,priv,nsq8w,ssq,rkh,rkv,tkefrc,
+ *            h3,rkv8s,tmp1,tmp2,tmp3)  SrcOrig/forcetke.f90:88-94, for the
+ * options of cress_turbdrv (tubopt=2, isoopt=1, trnopt=0, mfcopt=0), the
+ * advection of cfg->advopt, no smoothing (cfg->smtopt % 10 == 0, else
+ * CRESS_EUNSUPPORTED) and no lateral sponge.  The vertical sponge s_vsps0
+ * acts when cfg->vspopt >= 1 and cfg->vspvar[9] == 'o', from level
+ * cfg->ksp0[1]-1, with rbct(1:ni,1:nj,1:nk,1:2) (may be NULL otherwise).
+ * tkefrc gets the TKE forcing (advection of tke, shear, dissipation,
+ * diffusion of tkep, sponge, buoyancy); rkh, rkv come in as s_turbdrv
+ * leaves them and go out as the scalar diffusivities the reference leaves
+ * in them (rkv at w points), rkv8s with them.  ssq is read only (the
+ * reference also uses it as a work array); j31, j32, the map factors, rbr,
+ * rbcxy, h3 and tmp1..tmp3 are not taken.  rkh, rkv, rkv8s and all but the
+ * dissipation term are the FP32 reference's bits; the dissipation's length
+ * scale (a binary32 exp/log of its own) puts tkefrc within 1e-5. */
+int cress_forcetke(const cress_config *cfg, const float *jcb, const float *jcb8u, const float *jcb8v,
+                   const float *jcb8w, const float *rst, const float *rstxu, const float *rstxv, const float *rstxwc,
+                   const float *tke, const float *tkep, const float *rbct, const float *priv, const float *nsq8w,
+                   const float *ssq, float *rkh, float *rkv, float *tkefrc, float *rkv8s);
+
 /* s_advp(fpadvopt,fpmpopt,fpmfcopt,fpdiaopt,fpiwest,...,fpdziv,ni,nj,nk,
  *        mf8u,mf8v,jcb8u,jcb8v,jcb8w,u,v,wc,pp,pfrc,jcbxu,jcbxv,jcbxwc,
  *        hadv,vadv,tmp1,tmp2,tmp3)  SrcOrig/advp.f90:75-79, mfcopt=0 and

@@ -228,7 +228,7 @@ def build_mpi(prec=32, jobs=8):
 # the same modules, and oracle/dropin_cpu.c answers their tier-2 C calls
 # with the oracle (liboracle32.so).  Built from the same objects as
 # build_mpi(); the replaced reference objects are left out of the link.
-DROPINS = ("gaussel", "advs", "advuvw", "advp", "turbdrv", "turbflx", "turbs")
+DROPINS = ("gaussel", "advs", "advuvw", "advp", "turbdrv", "turbflx", "turbs", "forcetke")
 
 
 def build_dropin(names=DROPINS, jobs=8):

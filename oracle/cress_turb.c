@@ -362,3 +362,97 @@ void or_kh8uv(const or_cfg *c, const R *rkh, R *rkh8u, R *rkh8v)
       for (int i = 1; i <= ni - 1; i++) A3(rkh8v, i, j, k) = A3(rkh, i, j - 1, k) + A3(rkh, i, j, k);
   }
 }
+
+/* s_forcetke (forcetke.f90:225-351): the forcing of the TKE equation for
+ * tubopt=2, isoopt=1, trnopt=0, mfcopt=0, no smoothing (smtopt=0) and no
+ * lateral sponge, with the vertical sponge s_vsps0 when the tenth letter
+ * of vspvar is 'o' and vspopt >= 1.  The calls in the reference's order:
+ *   s_advs     advection of the present TKE      (advs.f90:205-445)
+ *   s_sheartke +jcb*rkv*ssq                       (sheartke.f90:93-101)
+ *   s_disptke  -(.37 priv-.18) rst e^1.5 / l      (disptke.f90:180-195)
+ *   s_eddyvisj rkh, rkv divided by jcb            (eddyvisj.f90:103-112)
+ *   s_tkeflx   the fluxes of jcb e and e          (tkeflx.f90:181-330)
+ *   s_turbtke  their divergence                   (turbtke.f90:191-213)
+ *   s_vsps0    -.5 (rbct_k + rbct_k+1) rst e      (vsps0.f90:114-127)
+ *   s_eddydif  rkh, rkv8w, rkv8s for the scalars  (eddydif.f90:266-330)
+ *   s_buoytke  -(jcb8w N^2 (rkv8s_k-1 + rkv8s_k))  (buoytke.f90:96-115)
+ * rkh and rkv are updated in place as the reference updates them; rkv8s
+ * is an output; w1..w4 are work arrays (the reference's h3, tmp1..tmp3,
+ * and ssq, which it also uses as work and which is read here only). */
+void or_forcetke(const or_cfg *c, const R *jcb, const R *jcb8u, const R *jcb8v, const R *jcb8w, const R *rst,
+                 const R *rstxu, const R *rstxv, const R *rstxwc, const R *tke, const R *tkep, const R *rbct,
+                 const R *priv, const R *nsq8w, const R *ssq, R *rkh, R *rkv, R *tkefrc, R *rkv8s, R *w1, R *w2,
+                 R *w3, R *w4)
+{
+  DECL_DIMS(c);
+  or_advs(c, rstxu, rstxv, rstxwc, tke, tkefrc, w1, w2, w3, w4);
+  for (int k = 2; k <= nk - 2; k++)
+    for (int j = 2; j <= nj - 2; j++)
+      for (int i = 2; i <= ni - 2; i++)
+        A3(tkefrc, i, j, k) = A3(tkefrc, i, j, k) + A3(jcb, i, j, k) * A3(rkv, i, j, k) * A3(ssq, i, j, k);
+  const R ds308 = K(.125) * (R)c->dx * (R)c->dy * (R)c->dz;
+  for (int k = 2; k <= nk - 2; k++)
+    for (int j = 2; j <= nj - 2; j++)
+      for (int i = 2; i <= ni - 2; i++) {
+        const R pv = A3(priv, i, j, k), e = A3(tkep, i, j, k);
+        const R ln = (pv - K(1.0)) * EXP(ONED3 * LOG(ds308 * A3(jcb, i, j, k))) + C_EPS;
+        A3(tkefrc, i, j, k) = A3(tkefrc, i, j, k) - (K(.37) * pv - K(.18)) * A3(rst, i, j, k) * e * SQRT(e) / ln;
+      }
+  for (int k = 1; k <= nk - 1; k++)
+    for (int j = 1; j <= nj - 1; j++)
+      for (int i = 1; i <= ni - 1; i++) {
+        const R jcbiv = K(1.0) / A3(jcb, i, j, k);
+        A3(rkh, i, j, k) = jcbiv * A3(rkh, i, j, k);
+        A3(rkv, i, j, k) = jcbiv * A3(rkv, i, j, k);
+      }
+  /* s_tkeflx: h1, h2, h3 in w1, w2, w3; jcb*e formed where used */
+  const R dxiv = K(1.0) / (R)c->dx, dyiv = K(1.0) / (R)c->dy, dziv = K(1.0) / (R)c->dz;
+  const R dxv5 = K(.5) * dxiv, dyv5 = K(.5) * dyiv, dzv5 = K(.5) * dziv;
+#define JT(i, j, k) (A3(tkep, i, j, k) * A3(jcb, i, j, k))
+  for (int k = 1; k <= nk - 1; k++) {
+    for (int j = 2; j <= nj - 2; j++)
+      for (int i = 2; i <= ni - 1; i++)
+        A3(w1, i, j, k) = (A3(rkh, i - 1, j, k) + A3(rkh, i, j, k)) * (JT(i, j, k) - JT(i - 1, j, k)) * dxv5;
+    for (int j = 2; j <= nj - 1; j++)
+      for (int i = 2; i <= ni - 2; i++)
+        A3(w2, i, j, k) = (A3(rkh, i, j - 1, k) + A3(rkh, i, j, k)) * (JT(i, j, k) - JT(i, j - 1, k)) * dyv5;
+  }
+#undef JT
+  for (int k = 2; k <= nk - 1; k++)
+    for (int j = 2; j <= nj - 2; j++)
+      for (int i = 2; i <= ni - 2; i++)
+        A3(w3, i, j, k) = (A3(rkv, i, j, k - 1) + A3(rkv, i, j, k)) * (A3(tkep, i, j, k) - A3(tkep, i, j, k - 1)) * dzv5;
+  /* s_turbtke: jcb8u*h1 and jcb8v*h2 formed where used */
+  const R dxiv2 = K(2.0) * dxiv, dyiv2 = K(2.0) * dyiv, dziv2 = K(2.0) * dziv;
+  for (int k = 2; k <= nk - 2; k++)
+    for (int j = 2; j <= nj - 2; j++)
+      for (int i = 2; i <= ni - 2; i++)
+        A3(tkefrc, i, j, k) =
+            A3(tkefrc, i, j, k) +
+            ((A3(w3, i, j, k + 1) - A3(w3, i, j, k)) * dziv2 +
+             ((A3(jcb8u, i + 1, j, k) * A3(w1, i + 1, j, k) - A3(jcb8u, i, j, k) * A3(w1, i, j, k)) * dxiv2 +
+              (A3(jcb8v, i, j + 1, k) * A3(w2, i, j + 1, k) - A3(jcb8v, i, j, k) * A3(w2, i, j, k)) * dyiv2));
+  if (c->vspopt >= 1 && c->vspvar[9] == 'o') or_vsps(c, rst, tkep, NULL, rbct, tkefrc);
+  /* s_eddydif, tubopt>=2, isoopt=1 */
+  for (int k = 1; k <= nk - 1; k++)
+    for (int j = 1; j <= nj - 1; j++)
+      for (int i = 1; i <= ni - 1; i++) {
+        A3(rkv8s, i, j, k) = A3(priv, i, j, k) * A3(rkv, i, j, k);
+        A3(rkh, i, j, k) = A3(rkv8s, i, j, k);
+      }
+  for (int k = 2; k <= nk - 1; k++)
+    for (int j = 1; j <= nj - 1; j++)
+      for (int i = 1; i <= ni - 1; i++) A3(rkv, i, j, k) = K(.5) * (A3(rkv8s, i, j, k - 1) + A3(rkv8s, i, j, k));
+  for (int k = 1; k <= nk - 1; k++)
+    for (int j = 1; j <= nj - 1; j++)
+      for (int i = 1; i <= ni - 1; i++) A3(rkv8s, i, j, k) = K(.5) * A3(jcb, i, j, k) * A3(rkv8s, i, j, k);
+  /* s_buoytke: its tmp1 in w4 */
+  for (int k = 2; k <= nk - 1; k++)
+    for (int j = 2; j <= nj - 2; j++)
+      for (int i = 2; i <= ni - 2; i++)
+        A3(w4, i, j, k) = A3(jcb8w, i, j, k) * A3(nsq8w, i, j, k) * (A3(rkv8s, i, j, k - 1) + A3(rkv8s, i, j, k));
+  for (int k = 2; k <= nk - 2; k++)
+    for (int j = 2; j <= nj - 2; j++)
+      for (int i = 2; i <= ni - 2; i++)
+        A3(tkefrc, i, j, k) = A3(tkefrc, i, j, k) - (A3(w4, i, j, k) + A3(w4, i, j, k + 1));
+}

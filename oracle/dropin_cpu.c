@@ -1,11 +1,11 @@
 /* The tier-2 entry points of include/cress_hip.h that the Fortran drop-in
  * modules call (cress_gaussel, cress_advs, cress_advuvw, cress_advp,
- * cress_turbdrv, cress_turbflx, cress_turbs),
+ * cress_turbdrv, cress_turbflx, cress_turbs, cress_forcetke),
  * implemented by the oracle on the CPU -- TEST INFRASTRUCTURE ONLY.
  *
  * Linked, with the drop-in modules of cress_amd/fortran/dropin/ in place of
- * the reference's m_gaussel, m_advs, m_advuvw, m_advp, m_turbdrv, m_turbflx
- * and m_turbs, into the
+ * the reference's m_gaussel, m_advs, m_advuvw, m_advp, m_turbdrv, m_turbflx,
+ * m_turbs and m_forcetke, into the
  * reference's own solver routines (oracle/build_ref.py: build_dropin), it
  * runs the drop-ins inside s_intgdrv on this machine: the whole step must
  * stay bitwise the unmodified reference's, which checks everything the
@@ -38,6 +38,10 @@ void or_turbdrv(const or_cfg *c, R dtb, int moist, const R *jcb, const R *jcb8u,
 void or_turbflx(const or_cfg *c, const R *jcb, const R *s, const R *rkh8u, const R *rkh8v, const R *rkv8w, R *h1,
                 R *h2, R *h3);
 void or_turbs(const or_cfg *c, const R *jcb8u, const R *jcb8v, const R *h1, const R *h2, const R *h3, R *sfrc);
+void or_forcetke(const or_cfg *c, const R *jcb, const R *jcb8u, const R *jcb8v, const R *jcb8w, const R *rst,
+                 const R *rstxu, const R *rstxv, const R *rstxwc, const R *tke, const R *tkep, const R *rbct,
+                 const R *priv, const R *nsq8w, const R *ssq, R *rkh, R *rkv, R *tkefrc, R *rkv8s, R *w1, R *w2,
+                 R *w3, R *w4);
 
 /* the oracle's configuration for a tier-2 call's cress_config */
 static void to_or(const cress_config *g, or_cfg *c) {
@@ -168,5 +172,20 @@ int cress_turbs(const cress_config *cfg, const float *jcb8u, const float *jcb8v,
   or_cfg c;
   to_or(cfg, &c);
   or_turbs(&c, jcb8u, jcb8v, h1, h2, h3, sfrc);
+  return CRESS_OK;
+}
+
+int cress_forcetke(const cress_config *cfg, const float *jcb, const float *jcb8u, const float *jcb8v,
+                   const float *jcb8w, const float *rst, const float *rstxu, const float *rstxv, const float *rstxwc,
+                   const float *tke, const float *tkep, const float *rbct, const float *priv, const float *nsq8w,
+                   const float *ssq, float *rkh, float *rkv, float *tkefrc, float *rkv8s) {
+  or_cfg c;
+  to_or(cfg, &c);
+  R *t = scratch(cfg, 4);
+  if (!t) return CRESS_ENOMEM;
+  const size_t n = (size_t)(cfg->ni + 2) * (size_t)(cfg->nj + 2) * (size_t)cfg->nk;
+  or_forcetke(&c, jcb, jcb8u, jcb8v, jcb8w, rst, rstxu, rstxv, rstxwc, tke, tkep, rbct, priv, nsq8w, ssq, rkh, rkv,
+              tkefrc, rkv8s, t, t + n, t + 2 * n, t + 3 * n);
+  free(t);
   return CRESS_OK;
 }

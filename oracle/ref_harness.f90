@@ -85,6 +85,7 @@
       use m_turbflx
       use m_turbs
       use m_kh8uv
+      use m_forcetke
 
       implicit none
 
@@ -773,5 +774,29 @@
       real rkh(*),rkh8u(*),rkh8v(*),w1(*)
       call s_kh8uv(idmpopt,idmfcopt,ni,nj,nk,w1,rkh,rkh8u,rkh8v)
       end subroutine ref_kh8uv
+
+! s_forcetke (forcetke.f90:88-94), tubopt=2, isoopt=1, with the lateral
+! sponge and smoothing off (ref_configure); j31, j32 and the map factors
+! are not read for trnopt=0, mfcopt=0 and get the work array w1.
+      subroutine ref_forcetke(ksp02,ni,nj,nk,jcb,jcb8u,jcb8v,jcb8w,rbr,  &
+     &   rst,rstxu,rstxv,rstxwc,tke,tkep,rbct,priv,nsq8w,ssq,rkh,rkv,    &
+     &   tkefrc,h3,rkv8s,w1,w2,w3,rbcxy) bind(C,name='ref_forcetke')
+      integer(c_int), value :: ksp02,ni,nj,nk
+      real jcb(*),jcb8u(*),jcb8v(*),jcb8w(*),rbr(*),rst(*),rstxu(*)
+      real rstxv(*),rstxwc(*),tke(*),tkep(*),rbct(*),priv(*),nsq8w(*)
+      real ssq(*),rkh(*),rkv(*),tkefrc(*),h3(*),rkv8s(*),w1(*),w2(*)
+      real w3(*),rbcxy(*)
+      integer ksp0(2)
+      ksp0(1)=2
+      ksp0(2)=ksp02
+      iname(idtubopt)=2
+      iname(idisoopt)=1
+      call s_forcetke(idlspvar,idvspvar,idlspopt,idvspopt,idsmtopt,      &
+     &                ksp0,ni,nj,nk,w1,w1,jcb,jcb8u,jcb8v,jcb8w,w1,w1,   &
+     &                w1,w1,rbr,rst,rstxu,rstxv,rstxwc,tke,tkep,rbcxy,   &
+     &                rbct,priv,nsq8w,ssq,rkh,rkv,tkefrc,h3,rkv8s,w1,w2, &
+     &                w3)
+      iname(idtubopt)=0
+      end subroutine ref_forcetke
 
       end module cress_ref_harness

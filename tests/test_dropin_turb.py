@@ -1,5 +1,5 @@
 """The turbulence drop-in modules (cress_amd/fortran/dropin/turbdrv.f90,
-turbflx.f90, turbs.f90) called as the reference calls them.
+turbflx.f90, turbs.f90, forcetke.f90) called as the reference calls them.
 
 The same harness entry points (oracle/ref_harness.f90: ref_turbdrv,
 ref_turbflx, ref_turbs, which set the name tables and call s_turbdrv,
@@ -22,6 +22,7 @@ import pytest
 from oracle import pyoracle as po
 from oracle import refmpi
 from test_oracle_turb import DRV_OUT, DTB, INTERIOR, NI, NJ, NK, cfg, drv_inputs
+from turb_inputs import tke_inputs
 
 pytestmark = pytest.mark.skipif(not (refmpi.have() and os.path.exists(refmpi.LIB_DROPIN)),
                                 reason="oracle/_ref drop-in part not built")
@@ -75,3 +76,20 @@ def test_turbflx_turbs_dropins(wbc):
     a, b = both(c, "ref_turbs", [NI, NJ, NK, ju, jv] + [x * 5.0 for x in h] + [sf] + [fld() for _ in range(4)])
     np.testing.assert_array_equal(b[8], a[8], err_msg="sfrc")
     assert not np.array_equal(b[8], sf)
+
+
+@pytest.mark.parametrize("wbc,vsp", [(-1, 0), (7, 1)])
+def test_forcetke_dropin(wbc, vsp):
+    """s_forcetke through the drop-in: tkefrc, rkh, rkv and rkv8s bitwise
+    the reference's (the CPU answer carries the reference's libm exp/log)."""
+    c = po.make_cfg(NI, NJ, NK, dx=700.0, dy=900.0, dz=350.0, wbc=wbc, cphopt=0, vspopt=2 * vsp, ksp0=(2, 8),
+                    vspvar=b"ooooooooo" + (b"o" if vsp else b"x"), gwave=30.0)
+    d, outs = tke_inputs(NI, NJ, NK, 71 + (wbc > 0) + 2 * vsp)
+    args = ([8, NI, NJ, NK] + [d[n] for n in ("jcb", "jcb8u", "jcb8v", "jcb8w")] + [fld()] +
+            [d[n] for n in ("rst", "rstxu", "rstxv", "rstxwc", "tke", "tkep", "rbct", "priv", "nsq8w", "ssq")] +
+            [outs["rkh"], outs["rkv"], outs["tkefrc"], fld(), outs["rkv8s"]] + [fld() for _ in range(3)] +
+            [np.zeros((NI, NJ), np.float32, order="F")])
+    a, b = both(c, "ref_forcetke", args)
+    for n, q in (("rkh", 19), ("rkv", 20), ("tkefrc", 21), ("rkv8s", 23)):
+        np.testing.assert_array_equal(b[q], a[q], err_msg=n)
+        assert not np.array_equal(b[q], outs[n]), n

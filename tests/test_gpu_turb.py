@@ -7,7 +7,9 @@ so what the kernels leave alone is checked too.  The deformation tensor,
 ssq and N^2 of dry or cloud-free air are bitwise; rkh, rkv, priv and N^2
 with cloud physics (exp and log in binary64 on the GPU, FP32 libm in the
 oracle) are within 1e-5 (max |difference| / max |value|) -- with cloud
-physics, the eddy viscosity stage on the GPU's own N^2 and ssq."""
+physics, the eddy viscosity stage on the GPU's own N^2 and ssq.  Also the
+rest of s_turbdrv, s_turbflx, s_turbs, s_kh8uv and s_forcetke through
+their C-ABI entry points."""
 import ctypes as C
 
 import numpy as np
@@ -15,7 +17,7 @@ import pytest
 
 from cress_amd import host
 from oracle import pyoracle as po
-from turb_inputs import IN, OUT, inputs
+from turb_inputs import IN, OUT, TKE_IN, TKE_OUT, inputs, tke_inputs
 
 pytestmark = pytest.mark.gpu
 
@@ -195,3 +197,37 @@ def test_kh8uv_matches_oracle():
     po.Oracle(32).or_kh8uv(c, kh, ou, ov)
     np.testing.assert_array_equal(gu, ou)
     np.testing.assert_array_equal(gv, ov)
+
+
+@pytest.mark.parametrize("advopt,vsp,ksp", [(3, 0, 8), (3, 1, 8), (2, 1, 2)])
+@pytest.mark.parametrize("wbc", [-1, 7])
+def test_forcetke_matches_oracle(wbc, advopt, vsp, ksp):
+    """cress_forcetke against the oracle's or_forcetke (pinned bitwise to
+    s_forcetke): rkh, rkv and rkv8s bitwise; tkefrc bitwise where the
+    past TKE is zero (no dissipation), elsewhere within 1e-5 of each cell's
+    value (the dissipation's length scale takes exp and log)."""
+    d, outs = tke_inputs(NI, NJ, NK, 61 + (wbc > 0) + 2 * vsp + 4 * advopt + ksp)
+    gpu = {n: a.copy(order="F") for n, a in outs.items()}
+    ora = {n: a.copy(order="F") for n, a in outs.items()}
+    vspvar = b"ooooooooo" + (b"o" if vsp else b"x")
+    cfg = host.make_config(NI, NJ, NK, 700.0, 900.0, 350.0, DTB, 1.0, advopt=advopt, cphopt=0, wbc=wbc,
+                           gwave=30.0, vspopt=2, ksp0=(2, ksp), vspvar=vspvar)
+    print("cress_forcetke wbc=%d advopt=%d vsp=%d ksp0=%d %dx%dx%d" % (wbc, advopt, vsp, ksp, NI, NJ, NK), flush=True)
+    host.check(host.lib().cress_forcetke(C.byref(cfg), *[_ptr(d[n]) for n in TKE_IN],
+                                         *[_ptr(gpu[n]) for n in TKE_OUT]), "cress_forcetke")
+    c = po.make_cfg(NI, NJ, NK, dx=700.0, dy=900.0, dz=350.0, wbc=wbc, advopt=advopt, cphopt=0, vspopt=2,
+                    ksp0=(2, ksp), vspvar=vspvar, gwave=30.0)
+    sh = (NI + 2, NJ + 2, NK)
+    po.Oracle(32).or_forcetke(c, *[d[n] for n in TKE_IN], *[ora[n] for n in TKE_OUT],
+                              *[np.zeros(sh, np.float32, order="F") for _ in range(4)])
+    for n in ("rkh", "rkv", "rkv8s"):
+        np.testing.assert_array_equal(gpu[n], ora[n], err_msg=n)
+    g, o = gpu["tkefrc"], ora["tkefrc"]
+    calm = np.zeros(sh, bool)
+    calm[2:NI - 1, 2:NJ - 1, 1:NK - 2] = d["tkep"][2:NI - 1, 2:NJ - 1, 1:NK - 2] == 0
+    assert calm.sum() > 100
+    np.testing.assert_array_equal(g[calm], o[calm])
+    e = worst_elementwise(g, o)
+    print("  tkefrc: max rel %.2e, worst cell %.2e, %d cells differ" % (rel(g, o), e, (g != o).sum()))
+    assert e <= 1e-5, e
+    assert not np.array_equal(g, outs["tkefrc"])

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from turb_inputs import IN, OUT, inputs
+from turb_inputs import IN, OUT, TKE_IN, TKE_OUT, inputs, tke_inputs
 
 NI, NJ, NK = 13, 11, 14
 DTB = 6.0
@@ -165,3 +165,68 @@ def test_kh8uv_matches_reference(libs):
     np.testing.assert_array_equal(mu, ru)
     np.testing.assert_array_equal(mv, rv)
     assert not np.array_equal(mu, u0)
+
+
+@pytest.mark.parametrize("advopt", [2, 3])
+@pytest.mark.parametrize("vsp", [0, 1])
+@pytest.mark.parametrize("wbc", [-1, 7])
+def test_forcetke_matches_reference(libs, wbc, vsp, advopt):
+    """s_forcetke (tubopt=2, isoopt=1, trnopt=0, mfcopt=0, no smoothing or
+    lateral sponge) against the reference's own: the TKE forcing and the
+    diffusivities it leaves in rkh, rkv and rkv8s, bitwise over whole
+    arrays, with the vertical sponge on TKE on and off."""
+    o, r = libs
+    c = po.make_cfg(NI, NJ, NK, dx=700.0, dy=900.0, dz=350.0, wbc=wbc, advopt=advopt, cphopt=0, vspopt=2 * vsp,
+                    ksp0=(2, 8), vspvar=b"ooooooooo" + (b"o" if vsp else b"x"), gwave=30.0)
+    r.configure(c)
+    d, outs = tke_inputs(NI, NJ, NK, 31 + (wbc > 0) + 2 * vsp + 4 * advopt)
+    mine = {n: a.copy(order="F") for n, a in outs.items()}
+    ref = {n: a.copy(order="F") for n, a in outs.items()}
+    ssq_ref = d["ssq"].copy(order="F")
+    o.or_forcetke(c, *[d[n] for n in TKE_IN], *[mine[n] for n in TKE_OUT], *[fld() for _ in range(4)])
+    r.ref_forcetke(8, NI, NJ, NK, *[d[n] for n in ("jcb", "jcb8u", "jcb8v", "jcb8w")], fld(),
+                   *[d[n] for n in ("rst", "rstxu", "rstxv", "rstxwc", "tke", "tkep", "rbct", "priv", "nsq8w")],
+                   ssq_ref, ref["rkh"], ref["rkv"], ref["tkefrc"], fld(), ref["rkv8s"], *[fld() for _ in range(3)],
+                   np.zeros((NI, NJ), np.float32, order="F"))
+    for n in TKE_OUT:
+        np.testing.assert_array_equal(mine[n], ref[n], err_msg=n)
+        assert not np.array_equal(mine[n], outs[n]), n
+    # the sponge changes the forcing only from level ksp0(2)-1 up
+    I, J = slice(2, NI - 1), slice(2, NJ - 1)
+    if vsp:
+        off = {n: a.copy(order="F") for n, a in outs.items()}
+        c0 = po.make_cfg(NI, NJ, NK, dx=700.0, dy=900.0, dz=350.0, wbc=wbc, advopt=advopt, cphopt=0, vspopt=0,
+                         ksp0=(2, 8), gwave=30.0)
+        o.or_forcetke(c0, *[d[n] for n in TKE_IN], *[off[n] for n in TKE_OUT], *[fld() for _ in range(4)])
+        diff = mine["tkefrc"][I, J, 1:NK - 2] != off["tkefrc"][I, J, 1:NK - 2]
+        # Fortran level 7 = ksp0(2)-1 is index 5 of the slice from level 2
+        assert not diff[:, :, :5].any() and diff[:, :, 5:].mean() > 0.6   # a fifth of the cells hold no TKE
+
+
+def test_forcetke_dissipation_closed_form(libs):
+    """With no wind, uniform TKE, S^2 = N^2 = 0 and unit metrics, only the
+    dissipation is left: -(.37 priv - .18) e^1.5 / ((priv-1) (dx dy dz/8)^(1/3))
+    (Deardorff's, as disptke.f90 writes it), and the diffusivities become
+    rkh = priv rkv, rkv8w the mean of the two levels, rkv8s half of it."""
+    o, _ = libs
+    c = po.make_cfg(NI, NJ, NK, dx=800.0, dy=800.0, dz=400.0, wbc=-1, cphopt=0, vspopt=0)
+    d, outs = tke_inputs(NI, NJ, NK, 5)
+    one = np.ones_like(d["jcb"], order="F")
+    for n in ("jcb", "jcb8u", "jcb8v", "jcb8w", "rst"):
+        d[n] = one.copy(order="F")
+    for n in ("rstxu", "rstxv", "rstxwc", "nsq8w", "ssq"):
+        d[n] = np.zeros_like(one, order="F")
+    d["tke"] = np.full_like(one, 1.5, order="F")
+    d["tkep"] = d["tke"].copy(order="F")
+    d["priv"] = np.full_like(one, 2.5, order="F")
+    kv = 10.0 * np.arange(1, NK + 1, dtype=np.float32)[None, None, :] * one
+    outs["rkv"] = np.asfortranarray(kv)
+    outs["rkh"] = outs["rkv"].copy(order="F")
+    o.or_forcetke(c, *[d[n] for n in TKE_IN], *[outs[n] for n in TKE_OUT], *[fld() for _ in range(4)])
+    ln = 1.5 * (800.0 * 800.0 * 400.0 / 8.0) ** (1.0 / 3.0)
+    want = -(0.37 * 2.5 - 0.18) * 1.5 ** 1.5 / ln
+    np.testing.assert_allclose(outs["tkefrc"][2:NI - 1, 2:NJ - 1, 1:NK - 2], want, rtol=2e-6)
+    k = np.arange(1, NK, dtype=np.float64)
+    np.testing.assert_allclose(outs["rkh"][3, 4, 0:NK - 1], 2.5 * 10.0 * k, rtol=1e-6)
+    np.testing.assert_allclose(outs["rkv"][3, 4, 1:NK - 1], 2.5 * 10.0 * (k[1:] - 0.5), rtol=1e-6)
+    np.testing.assert_allclose(outs["rkv8s"][3, 4, 0:NK - 1], 0.5 * 2.5 * 10.0 * k, rtol=1e-6)

@@ -33,3 +33,34 @@ def inputs(ni, nj, nk, seed):
     d["tkep"] = np.asfortranarray(np.maximum(t, 0.0).astype(np.float32))
     outs = {n: f(-1.0, 1.0) for n in OUT}
     return d, outs
+
+
+TKE_IN = ["jcb", "jcb8u", "jcb8v", "jcb8w", "rst", "rstxu", "rstxv", "rstxwc", "tke", "tkep", "rbct", "priv",
+          "nsq8w", "ssq"]
+TKE_OUT = ["rkh", "rkv", "tkefrc", "rkv8s"]
+
+
+def tke_inputs(ni, nj, nk, seed):
+    """Inputs of s_forcetke at the magnitudes the closure gives them: the
+    Prandtl-number factor priv in its range 1..3, eddy viscosities of tens
+    of m^2/s, S^2 and N^2 of a sheared, partly unstable flow, TKE from zero
+    up, and mass fluxes of 10 m/s winds.  The outputs start as noise so
+    that the cells a routine leaves alone are checked too."""
+    rng = np.random.default_rng(seed)
+    sh = (ni + 2, nj + 2, nk)
+
+    def f(lo, hi, shape=sh):
+        return np.asfortranarray(rng.uniform(lo, hi, shape).astype(np.float32))
+
+    d = {n: f(0.9, 1.1) for n in ("jcb", "jcb8u", "jcb8v", "jcb8w")}
+    d["rst"] = f(0.8, 1.2)
+    for n in ("rstxu", "rstxv", "rstxwc"):
+        d[n] = f(-10.0, 10.0)
+    for n in ("tke", "tkep"):
+        d[n] = np.asfortranarray(np.maximum(f(-0.5, 2.0), 0.0).astype(np.float32))
+    d["rbct"] = f(0.0, 0.01, (ni, nj, nk, 2))
+    d["priv"] = f(1.2, 3.0)
+    d["nsq8w"] = f(-1.e-4, 3.e-4)
+    d["ssq"] = f(0.0, 2.e-4)
+    outs = {"rkh": f(0.0, 40.0), "rkv": f(0.0, 40.0), "tkefrc": f(-1.0, 1.0), "rkv8s": f(-1.0, 1.0)}
+    return d, outs
