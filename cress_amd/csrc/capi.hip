@@ -2063,6 +2063,42 @@ int cress_steps(const cress_config *cfg, float dtb, const float *rst, const floa
   return rc;
 }
 
+// The TKE lines of s_steps: a dry context of the caller's configuration
+// holds rst, and its theta' slots hold tkep (past) and tkef (future); the
+// future level then takes the same halo fills and vbcs as theta'.
+int cress_stepstke(const cress_config *cfg, float dtb, const float *rst, const float *tkep, float *tkef) {
+  if (!cfg || !rst || !tkep || !tkef || !(dtb > 0.0f)) return CRESS_EINVAL;
+  if (!(cfg->wbc == -1 && cfg->ebc == -1 && cfg->sbc == -1 && cfg->nbc == -1)) return CRESS_EUNSUPPORTED;
+  cress_config dry = *cfg;
+  dry.cphopt = 0;
+  cress_ctx *x = nullptr;
+  int rc = cress_ctx_create(&dry, &x);
+  if (rc) return rc;
+  const Grid g = x->g;
+  const int ni = g.ni, nj = g.nj, nk = g.nk;
+  const int P = (x->rot + 0) % 3, F = (x->rot + 2) % 3;
+  rc = cress_upload(x, CRESS_RST, 0, rst);
+  if (!rc) rc = cress_upload(x, CRESS_PTP, 0, tkep);
+  if (!rc) rc = cress_upload(x, CRESS_PTP, 2, tkef);
+  if (!rc) {
+    hipLaunchKernelGGL(k_steps_tke, grid2(ni - 3, nj - 3, 64, 4, nk - 3), dim3(64, 4), 0, x->stream, g, 2.0f * dtb,
+                       x->base[CRESS_RST], x->ptp[P], x->ptp[F]);
+    FillSet fs;
+    VbcSet vs;
+    fs.n = vs.n = 1;
+    fill_spec(&fs.f[0], x->ptp[F], 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk);
+    vs.kind[0] = 0;
+    vs.ptr[0] = x->ptp[F];
+    run_fill(x, fs);
+    run_vbc(x, vs);
+    if (hipGetLastError() != hipSuccess) rc = CRESS_EHIP;
+  }
+  if (!rc) rc = cress_download(x, CRESS_PTP, 2, tkef);
+  if (!rc && hipStreamSynchronize(x->stream) != hipSuccess) rc = CRESS_EHIP;
+  cress_ctx_destroy(x);
+  return rc;
+}
+
 // s_cloudphy on host arrays for cphopt=2: a context of the caller's
 // configuration holds them and the large step's microphysics stage runs
 // once on the given future level.

@@ -139,4 +139,5 @@ __global__ void k_advuvw_only(Grid g, Opts o, Consts c, const float *u, const fl
                               float *wfrc);
 __global__ void k_bc8(Grid g, int kind, int lo, int hi, int kmax, float *var);
 __global__ void k_dtdrst(Grid g, float dtb2, const float *rst, float *dtdrst);
+__global__ void k_steps_tke(Grid g, float dtb2, const float *rst, const float *tkep, float *tkef);
 }  // namespace cress

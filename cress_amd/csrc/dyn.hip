@@ -946,6 +946,20 @@ __global__ void k_dtdrst(Grid g, float dtb2, const float *rst, float *dtdrst) {
   dtdrst[p] = dtb2 / MP(CRESS_RST, rst, p, k);
 }
 
+// The TKE line of s_steps for tubopt>=2 (steps.f90:997-1012): tkef holds
+// the forcing on entry and the future TKE, clipped at zero, on return;
+// 2 dtb / rst is formed as k_dtdrst forms it.
+__global__ void k_steps_tke(Grid g, float dtb2, const float *rst, const float *tkep, float *tkef) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 2 + blockIdx.z;
+  if (i > g.ni - 2 || j > g.nj - 2) return;
+  const long p = g.at(i, j, k);
+  const float d = dtb2 / MP(CRESS_RST, rst, p, k);
+  const float a = tkep[p] + tkef[p] * d;
+  tkef[p] = a > 0.0f ? a : 0.0f;
+}
+
 // ---------------------------------------------------------------------------
 // Halo fills.
 // s_bcycle (bcycle.f90:159-225): x pass over j=0..nj+1 then y pass over

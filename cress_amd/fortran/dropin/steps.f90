@@ -2,9 +2,10 @@
 ! Drop-in replacement for the reference's module m_steps
 ! (SrcOrig/steps.f90:114-127): the leapfrog of theta', qv and, with the
 ! cold-rain scheme (cphopt=2), qc, qr, qi, qs, qg and nci, with the halo
-! fills and vbcs, runs on the GPU through cress_steps.  Same module and
+! fills and vbcs, runs on the GPU through cress_steps, and with the TKE
+! closure (tubopt=2) the TKE's through cress_stepstke.  Same module and
 ! dummy list.  Taken: gwmopt=0, exbopt=0, advopt<=3, a moist state,
-! cphopt 0 or 2 with haiopt=0, no aerosol, tracer or TKE, no GPV nudging,
+! cphopt 0 or 2 with haiopt=0, no aerosol or tracer, no GPV nudging,
 ! periodic sides (the phase speeds and nudging arrays are not read).  The
 ! options come from the reference's name tables (m_getiname, ids of
 ! m_comindx).
@@ -158,7 +159,7 @@
       call getiname(idtbc,tbc)
 
       if(gwmopt.ne.0.or.exbopt.ne.0.or.advopt.gt.3.or.haiopt.ne.0.or.    &
-     &   aslopt.ne.0.or.trkopt.ne.0.or.tubopt.ne.0.or.                  &
+     &   aslopt.ne.0.or.trkopt.ne.0.or.(tubopt.ne.0.and.tubopt.ne.2).or.&
      &   (cphopt.ne.0.and.cphopt.ne.2).or.fmois(1:5).ne.'moist') then
         write(6,'(a)') 's_steps: outside the options the GPU path takes'
         stop
@@ -174,6 +175,10 @@
 
       rc=cress_steps(cfg,dtb,rst,ptpp,ptfrc,qvp,qvfrc,qwtrp,qicep,nicep, &
      &               qwtrf,qicef,nicef,ptpf,qvf,dtdrst)
+
+      if(rc.eq.cress_ok.and.tubopt.ge.2) then
+        rc=cress_stepstke(cfg,dtb,rst,tkep,tkef)
+      end if
 
       if(rc.ne.cress_ok) then
         write(6,'(2a)') 's_steps: ',trim(cress_errmsg(rc))

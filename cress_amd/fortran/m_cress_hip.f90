@@ -69,6 +69,7 @@ module m_cress_hip
   public :: cress_turbdrv, cress_turbflx, cress_turbs
   public :: cress_bc8u, cress_bc8v, cress_bc8w
   public :: cress_kh8uv, cress_gaussel_noalloc, cress_forcetke
+  public :: cress_stepstke
   public :: cress_set_base, cress_base_columns, cress_diag
   public :: cress_autotune, cress_set_launch, cress_get_launch
   public :: cress_ens_create, cress_ens_step, cress_ens_sync, cress_ens_destroy
@@ -382,6 +383,17 @@ module m_cress_hip
       real(c_float), intent(inout) :: rkh(*), rkv(*), tkefrc(*), rkv8s(*)
       integer(c_int) :: cress_forcetke
     end function cress_forcetke
+
+! tier 2: the TKE lines of s_steps (steps.f90:997-1012), periodic sides
+    function cress_stepstke(cfg, dtb, rst, tkep, tkef)                  &
+     &  bind(c, name='cress_stepstke')
+      import :: c_int, c_float, cress_config
+      type(cress_config), intent(in) :: cfg
+      real(c_float), value :: dtb
+      real(c_float), intent(in) :: rst(*), tkep(*)
+      real(c_float), intent(inout) :: tkef(*)
+      integer(c_int) :: cress_stepstke
+    end function cress_stepstke
 
 ! cress_gaussel without device allocation (a fixed device array stages the
 ! four arrays; larger boxes return cress_etoobig)

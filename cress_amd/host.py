@@ -108,6 +108,7 @@ def lib():
         L.cress_turbs.argtypes = [C.POINTER(Config)] + [P] * 6
         L.cress_kh8uv.argtypes = [C.POINTER(Config)] + [P] * 3
         L.cress_forcetke.argtypes = [C.POINTER(Config)] + [P] * 18
+        L.cress_stepstke.argtypes = [C.POINTER(Config), C.c_float] + [P] * 3
         L.cress_bc8u.argtypes = [C.POINTER(Config), I, P]
         L.cress_bc8v.argtypes = [C.POINTER(Config), I, P]
         L.cress_bc8w.argtypes = [C.POINTER(Config), P]

@@ -280,6 +280,14 @@ int cress_steps(const cress_config *cfg, float dtb, const float *rst, const floa
                 const float *qvp, const float *qvfrc, const float *qwtrp, const float *qicep, const float *nicep,
                 float *qwtrf, float *qicef, float *nicef, float *ptpf, float *qvf, float *dtdrst);
 
+/* The TKE lines of s_steps (SrcOrig/steps.f90:997-1012, 4170-4172,
+ * 4350-4352, 4493-4495) for tubopt>=2, periodic sides: tkef holds the TKE
+ * forcing on entry and the future TKE on return, max(0, tkep + tkef *
+ * 2 dtb / rst) over the interior, then the halo fills and vbcs as for the
+ * other scalars; bitwise.  With cress_steps it covers s_steps with TKE; it
+ * has no separate counterpart in the reference. */
+int cress_stepstke(const cress_config *cfg, float dtb, const float *rst, const float *tkep, float *tkef);
+
 /* s_cloudphy(fpadvopt,fpcphopt,fphaiopt,nclstp,dtb,dtcl,ni,nj,nk,nqw,nnw,
  *            nqi,nni,km,jcb,jcb8w,pbr,ptbr,rbr,rst,wf,ppp,ptpp,qvp,qwtrp,
  *            qicep,qallp,ptpf,qvf,qwtrf,nwtrf,nwtrp,qicef,nicef,nicep,

@@ -1443,6 +1443,26 @@ void or_steps_a(const or_cfg *c, R dtb, R *rst, R *ptpp, const R *ptfrc, R *qp, 
   or_steps(c, &st, dtb, ptfrc, qvfrc, dtdrst);
 }
 
+/* The TKE lines of s_steps for tubopt>=2, one rank with periodic sides:
+ * the leapfrog clipped at zero (steps.f90:997-1012; dtdrst = 2 dtb / rst
+ * as above), then bcycle, bc4news and vbcs (steps.f90:4170-4172,
+ * 4350-4352, 4493-4495).  tkef holds the forcing on entry. */
+void or_stepstke(const or_cfg *c, R dtb, const R *rst, const R *tkep, R *tkef)
+{
+  DECL_DIMS(c);
+  const R dtb2 = K(2.0) * dtb;
+  for (int k = 2; k <= nk - 2; k++)
+    for (int j = 2; j <= nj - 2; j++)
+      for (int i = 2; i <= ni - 2; i++) {
+        const R d = dtb2 / A3(rst, i, j, k);
+        const R a = A3(tkep, i, j, k) + A3(tkef, i, j, k) * d;
+        A3(tkef, i, j, k) = a > K(0.0) ? a : K(0.0);
+      }
+  or_bcycle(c, 2, 1, ni - 2, ni - 1, 2, 1, nj - 2, nj - 1, nk, tkef);
+  or_bc4news(c, 1, ni - 1, 1, nj - 1, nk, tkef);
+  or_vbcs(c, tkef);
+}
+
 /* s_timeflt, timeflt.f90:335-560: Asselin filter on the present level. */
 static void or_asselin(const or_cfg *c, R *x, const R *xp, const R *xf, int ilo, int ihi,
                        int jhi, int khi)

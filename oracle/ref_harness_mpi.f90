@@ -348,6 +348,42 @@
      &  lev(0,0,1,5,3),lev(0,0,1,6,3),dtdrst)
       end subroutine ref_steps
 
+! s_steps with tubopt=2: as ref_steps, with the TKE levels tke, tkep and
+! the TKE forcing / future level tkef in place of the zero arrays.
+      subroutine ref_steps_tke(ni,nj,nk,dtb,qvbr,rst,lev,ptfrc,qvfrc,     &
+     &   dtdrst,tke,tkep,tkef) bind(C,name='ref_steps_tke')
+      integer(c_int), value :: ni,nj,nk
+      real, value :: dtb
+      real qvbr(*),rst(*),ptfrc(*),qvfrc(*),dtdrst(*),tke(*),tkep(*)
+      real tkef(*)
+      real lev(0:ni+1,0:nj+1,1:nk,1:16,1:3)
+      if(ni.ne.mi.or.nj.ne.mj.or.nk.ne.mk) stop 'ref_steps_tke: setup'
+      iname(idtubopt)=2
+      call s_steps(idgpvvar,idexbvar,idexbopt,idgwmopt,idadvopt,        &
+     &  idcphopt,idhaiopt,idqcgopt,idaslopt,idtrkopt,idtubopt,'moist',  &
+     &  dtb,0.e0,0.e0,ni,nj,nk,nqw,nnw,nqi,nni,nqa,qvbr,rst,            &
+     &  lev(0,0,1,5,2),lev(0,0,1,5,1),lev(0,0,1,6,2),lev(0,0,1,6,1),    &
+     &  lev(0,0,1,7,2),lev(0,0,1,7,1),lev(0,0,1,12,2),lev(0,0,1,12,1),  &
+     &  lev(0,0,1,9,2),lev(0,0,1,9,1),lev(0,0,1,14,2),lev(0,0,1,14,1),  &
+     &  zero3(0,0,1,11),zero3(0,0,1,14),zero3(0,0,1,17),zero3(0,0,1,20),&
+     &  zero3(0,0,1,23),zero3(0,0,1,24),zero3(0,0,1,25),zero3(0,0,1,26),&
+     &  tke,tkep,                                                       &
+     &  cpx(1,1,1,5),cpy(1,1,1,5),cpx(1,1,1,6),cpy(1,1,1,6),            &
+     &  cpx(1,1,1,7),cpy(1,1,1,7),cpx(1,1,1,9),cpy(1,1,1,9),            &
+     &  cpx(1,1,1,11),cpy(1,1,1,11),cpx(1,1,1,14),cpy(1,1,1,14),        &
+     &  cpx(1,1,1,17),cpy(1,1,1,17),cpx(1,1,1,19),cpy(1,1,1,19),        &
+     &  cpx(1,1,1,22),cpy(1,1,1,22),cpx(1,1,1,23),cpy(1,1,1,23),        &
+     &  cpx(1,1,1,24),cpy(1,1,1,24),                                    &
+     &  zero3(0,0,1,1),zero3(0,0,1,1),zero3(0,0,1,1),zero3(0,0,1,1),    &
+     &  zero3(0,0,1,1),zero3(0,0,1,1),zero3(0,0,1,1),zero3(0,0,1,1),    &
+     &  zero3(0,0,1,1),zero3(0,0,1,1),                                  &
+     &  ptfrc,qvfrc,lev(0,0,1,7,3),lev(0,0,1,12,3),lev(0,0,1,9,3),      &
+     &  lev(0,0,1,14,3),zero3(0,0,1,29),zero3(0,0,1,32),                &
+     &  zero3(0,0,1,35),zero3(0,0,1,36),tkef,                           &
+     &  lev(0,0,1,5,3),lev(0,0,1,6,3),dtdrst)
+      iname(idtubopt)=0
+      end subroutine ref_steps_tke
+
 ! s_cloudphy (cloudphy.f90:84-474) with cphopt=2: reads the past level of
 ! lev, updates its future level and the precipitation in place.
       subroutine ref_cloudphy(ni,nj,nk,dtb,base,wf,lev,qallp,prwtr,price)&

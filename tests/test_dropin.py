@@ -230,6 +230,33 @@ def test_tier2_steps_matches_oracle(cphopt):
 
 
 @pytest.mark.gpu
+def test_tier2_stepstke_matches_oracle():
+    """cress_stepstke (the TKE lines of s_steps) equals the oracle's
+    or_stepstke, pinned bitwise to the reference's s_steps with TKE,
+    over the whole array, the clamp at zero acting on part of it."""
+    from cress_amd import cases
+    case = cases.build("bubble", 12, 10, 16, cphopt=0)
+    cfg = case.cfg
+    rng = np.random.default_rng(91)
+    shp = case.base["rst"].shape
+    F = lambda a: np.asfortranarray(np.asarray(a, np.float32))
+    rst = F(case.base["rst"])
+    tkep = F(np.maximum(rng.uniform(-0.5, 2.0, shp), 0.0))
+    tkef = F(rng.uniform(-0.05, 0.05, shp))
+    dtb = float(np.float32(cfg.dtbig))
+    g = tkef.copy(order="F")
+    print("cress_stepstke bubble %s" % (shp,), flush=True)
+    host.check(host.lib().cress_stepstke(C.byref(cfg), C.c_float(dtb), _ptr(rst), _ptr(tkep), _ptr(g)),
+               "cress_stepstke")
+    m = po.Model(case, 32)
+    o = tkef.copy(order="F")
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    m.o.lib.or_stepstke(C.byref(m.cfg), m.o._real(dtb), vp(rst), vp(tkep), vp(o))
+    np.testing.assert_array_equal(g, o)
+    assert (g == 0.0).any() and (g > 0.0).any()
+
+
+@pytest.mark.gpu
 def test_tier2_cloudphy_matches_oracle():
     """cress_cloudphy (the cold-rain microphysics on a given future level)
     against the oracle's or_cloudphy: every field within 1e-5 of the FP32

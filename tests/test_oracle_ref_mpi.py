@@ -229,6 +229,33 @@ def test_steps(bubble):
     assert (qf[..., 1] == 0.0).any()    # the max(0, .) clamp acted
 
 
+def test_steps_tke(bubble):
+    """The TKE lines of s_steps (tubopt=2: the leapfrog clipped at zero and
+    the halo and vertical fills of tkef): the oracle's or_stepstke against
+    the reference's s_steps run with TKE, bitwise over the whole array."""
+    case, cfg, o = bubble
+    rng = np.random.default_rng(15)
+    b = _base(case)
+    shp = b["rst"].shape
+    lev = _levels(case, rng)
+    ptfrc, qvfrc = _rnd(rng, shp, -1e-3, 1e-3), _rnd(rng, shp, -1e-6, 1e-6)
+    tke = np.asfortranarray(np.maximum(_rnd(rng, shp, -0.5, 2.0), 0.0).astype(F32))
+    tkep = np.asfortranarray(np.maximum(_rnd(rng, shp, -0.5, 2.0), 0.0).astype(F32))
+    tkef = _rnd(rng, shp, -0.05, 0.05)
+    dtb = float(F32(case.cfg.dtbig))
+    out = refmpi.calls(cfg, [("ref_steps_tke", [cfg.ni, cfg.nj, cfg.nk, dtb, b["qvbr"], b["rst"],
+                                                lev.copy(order="F"), ptfrc, qvfrc, np.zeros(shp, F32, order="F"),
+                                                tke, tkep, tkef.copy(order="F")])])
+    r = out[0][12]
+    mine = tkef.copy(order="F")
+    o.or_stepstke(cfg, dtb, b["rst"], tkep, mine)
+    np.testing.assert_array_equal(mine, r)
+    inner = mine[2:cfg.ni - 1, 2:cfg.nj - 1, 1:cfg.nk - 2]
+    assert (inner == 0.0).any() and (inner > 0.0).any()   # the clamp acted, and not everywhere
+    # vbcs: level 1 copies level 2 over i, j 1..n-1
+    np.testing.assert_array_equal(mine[1:cfg.ni, 1:cfg.nj, 0], mine[1:cfg.ni, 1:cfg.nj, 1])
+
+
 def test_cloudphy(bubble):
     case, cfg, o = bubble
     rng = np.random.default_rng(14)
